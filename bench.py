@@ -1,0 +1,252 @@
+#!/usr/bin/env python3
+"""Benchmark of the clustered-forward + IBL compute path on one MI355X.
+
+One step = one frame of the 3840x2160 workload: light culling of 65,536
+lights into the 32x18x48 cluster grid (cap 128) followed by clustered
+Cook-Torrance + split-sum IBL shading of 8,294,400 struct-of-arrays fragments
+(the per-frame stages; cluster AABBs are rebuilt only when the camera's
+projection changes, as in the reference, and are timed separately). Inputs
+are synthetic and seeded, resident in HBM before timing starts. The IBL maps
+the shading samples are produced by this library's own stages from the
+2048x1024 procedural equirect (cube 6x1024^2, irradiance 6x32^2, 10-mip GGX
+prefilter from 512^2, BRDF LUT 512^2 x 1024 samples); those stages are timed
+once each and reported beside the frame metric.
+
+Prints one JSON line. `value` is fragments shaded per second (Mfrag/s) over
+whole frames (cull + shade).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cluster-light tests/s + Mfrag/s shaded; IBL filter ms vs plain FP32 @<=2e-3 err"
+PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak (datasheet)
+PEAK_HBM_GBS = 8000.0      # MI355X HBM3E peak (datasheet)
+# algorithmic work per fragment of the shading kernel: 160 FP32 ops of per-
+# fragment set-up + ambient, 140 per listed light; 45 bytes of fragment data
+# in and out (depth 4, normal 12, albedo 12, roughness 4, metallic 1, RGB 12)
+OPS_FRAG, OPS_PAIR, BYTES_FRAG = 160, 140, 45
+OPS_TEST = 16              # FP32 ops per cluster-light test
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-ibl", action="store_true", help="skip the one-off IBL stage timings")
+    p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample duration")
+    return p.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this path runs on a single MI355X. "
+              "Re-run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from hellovulkan_amd import abi
+    from tests.helpers import CFG1, camera
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    ctx = abi.Context(0)
+    stream = torch.cuda.current_stream(dev)
+
+    W, H, grid, NL, CAP = CFG1["width"], CFG1["height"], CFG1["grid"], CFG1["n_lights"], CFG1["cap"]
+    F = W * H
+    Cn = grid[0] * grid[1] * grid[2]
+    desc, cam, ubo, ivp = camera(W, H, grid)
+
+    # ---------------- IBL maps (cfg2 sizes) built by the library's own stages
+    t0 = time.time()
+    EQW, EQH, CUBE, DIFF, SPEC, SPEC_MIPS, LUT, LUT_N, SPEC_N = 2048, 1024, 1024, 32, 512, 10, 512, 1024, 1024
+    eq = abi.scene_equirect(77, EQW, EQH, threads=16)
+    d_eq = torch.from_numpy(eq).to(dev)
+    d_cube = torch.empty(abi.cube_chain_floats(CUBE, abi.mip_count(CUBE)), dtype=torch.float32, device=dev)
+    d_diff = torch.empty(6 * DIFF * DIFF * 4, dtype=torch.float32, device=dev)
+    d_spec = torch.empty(abi.cube_chain_floats(SPEC, SPEC_MIPS), dtype=torch.float32, device=dev)
+    d_lut = torch.empty(LUT * LUT * 2, dtype=torch.float32, device=dev)
+
+    def timed(fn, reps=1):
+        fn()  # warm (also allocates scratch)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    ibl = {}
+    ibl["equirect_to_cube_ms"] = timed(lambda: ctx.equirect_to_cube(d_eq, EQW, EQH, d_cube, CUBE))
+    ibl["cube_mips_ms"] = timed(lambda: ctx.cube_mips(d_cube, CUBE, abi.mip_count(CUBE)))
+    if args.no_ibl:
+        ctx.filter(abi.FILTER_DIFFUSE, d_cube, CUBE, d_diff, DIFF, 1, 0)
+        ctx.filter(abi.FILTER_SPECULAR, d_cube, CUBE, d_spec, SPEC, SPEC_MIPS, SPEC_N)
+        ctx.brdf_lut(LUT, LUT, LUT_N, d_lut)
+        torch.cuda.synchronize()
+    else:
+        ibl["diffuse_plain_fp32_ms"] = timed(lambda: ctx.filter(abi.FILTER_DIFFUSE, d_cube, CUBE, d_diff, DIFF, 1, 0))
+        ibl["specular_plain_fp32_ms"] = timed(
+            lambda: ctx.filter(abi.FILTER_SPECULAR, d_cube, CUBE, d_spec, SPEC, SPEC_MIPS, SPEC_N))
+        ibl["brdf_lut_ms"] = timed(lambda: ctx.brdf_lut(LUT, LUT, LUT_N, d_lut))
+    log(f"IBL maps ready in {time.time() - t0:.1f}s: {ibl}")
+
+    # ---------------- frame inputs
+    lights = abi.scene_lights(1001, NL)
+    fr = abi.scene_fragments(1002, desc, ivp, threads=16)
+    d_lights = torch.from_numpy(lights.view(np.uint8)).to(dev)
+    d_fr = {k: torch.from_numpy(v).to(dev) for k, v in fr.items()}
+    frs = abi.frag_soa(d_fr, W, H, ivp)
+    maps = abi.ibl_maps(d_spec, SPEC, SPEC_MIPS, d_diff, DIFF, d_lut, LUT, LUT)
+    pc = abi.pbr_defaults()
+    d_aabb = torch.empty(Cn * 8, dtype=torch.float32, device=dev)
+    d_cells = torch.empty(Cn * 2, dtype=torch.int32, device=dev)
+    d_idx = torch.empty(Cn * CAP, dtype=torch.int32, device=dev)
+    d_tot = torch.zeros(1, dtype=torch.int32, device=dev)
+    d_rgb = torch.empty(3 * F, dtype=torch.float32, device=dev)
+    aabb_ms = timed(lambda: ctx.build_aabbs(ubo, d_aabb), reps=10)
+
+    def cull():
+        ctx.cull_lights(ubo, d_aabb, d_lights, NL, CAP, d_cells, d_idx, Cn * CAP, d_tot)
+
+    def shade():
+        ctx.shade(frs, cam, ubo, pc, d_lights, NL, d_cells, d_idx, maps, d_rgb)
+
+    # ---------------- warmup + timed frames, per-stage HIP events on the launch stream
+    for _ in range(args.warmup):
+        cull()
+        shade()
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        ev[k][0].record(stream)
+        cull()
+        ev[k][1].record(stream)
+        shade()
+        ev[k][2].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    cull_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+    shade_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
+    ms_per_step = wall * 1e3 / args.steps
+
+    # ---------------- algorithmic work of the dominant (shading) kernel
+    cells = d_cells.cpu().numpy().view(np.uint32).reshape(-1, 2)
+    counts = cells[:, 1].astype(np.int64)
+    z = fr["depth"].astype(np.float64)
+    lin = (0.1 * 100.0) / (100.0 + z * (0.1 - 100.0))
+    zi = np.clip(np.floor(np.maximum(np.log2(lin) * ubo.slice_scaling + ubo.slice_bias, 0)), 0, grid[2] - 1)
+    px = np.arange(F) % W
+    py = np.arange(F) // W
+    cx = ((px + 0.5) / (W / grid[0])).astype(np.int64)
+    cy = ((py + 0.5) / (H / grid[1])).astype(np.int64)
+    lf = counts[cx + cy * grid[0] + zi.astype(np.int64) * grid[0] * grid[1]]
+    pairs = int(lf.sum())
+    ops = OPS_FRAG * F + OPS_PAIR * pairs
+    achieved_tflops = ops / (shade_ms * 1e-3) / 1e12
+    bytes_alg = BYTES_FRAG * F
+    roofline = {
+        "kernel": "shade_kernel",
+        "bound": "valu",
+        "achieved": round(achieved_tflops, 3),
+        "peak": PEAK_FP32_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 4),
+        "traffic": None,
+        "ops_per_launch": ops,
+        "fragment_light_pairs": pairs,
+        "hbm_term_GBs": round(bytes_alg / (shade_ms * 1e-3) / 1e9, 1),
+        "hbm_term_frac": round(bytes_alg / (shade_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+    }
+    tests = Cn * NL
+    result = {
+        "metric": METRIC,
+        "value": round(F / (ms_per_step * 1e-3) / 1e6, 2),
+        "unit": "Mfrag/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "clustered-forward frame 3840x2160, 32x18x48 clusters, 65536 lights, cap 128, "
+                               "8294400 SoA fragments; IBL maps from a 2048x1024 procedural equirect",
+                   "width": W, "height": H, "clusters": list(grid), "lights": NL, "cap": CAP, "fragments": F},
+        "cull_ms": round(cull_ms, 4),
+        "shade_ms": round(shade_ms, 4),
+        "aabb_ms": round(aabb_ms, 4),
+        "cluster_light_tests_per_s": round(tests / (cull_ms * 1e-3), 1),
+        "cull_valu_frac": round(tests * OPS_TEST / (cull_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+        "list_entries": int(d_tot.item()),
+        "ibl_stage_ms": {k: round(v, 4) for k, v in ibl.items()},
+        "roofline": roofline,
+    }
+
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(args, ubo, cam, desc, ivp, lights, fr, d_spec, d_diff, d_lut, F, pc)
+    print(json.dumps(result), flush=True)
+    ctx.close()
+
+
+def cpu_baseline(args, ubo, cam, desc, ivp, lights, fr, d_spec, d_diff, d_lut, F, pc):
+    """The FP32 oracle on the host cores: full light culling of the same frame,
+    then shading of a fixed deterministic fragment sample; the frame rate is
+    F / (cull time + per-fragment shading time * F)."""
+    from oracle import hvx_oracle as O
+    threads = max(1, min(16, os.cpu_count() or 1))
+    t = time.perf_counter()
+    aabbs = O.build_aabbs(ubo)
+    cells, idx, _, _ = O.cull_lights(ubo, aabbs, lights, 128, threads=threads)
+    t_cull = time.perf_counter() - t
+    maps = dict(spec=d_spec.cpu().numpy(), spec_size=512, spec_mips=10, diff=d_diff.cpu().numpy(), diff_size=32,
+                lut=d_lut.cpu().numpy(), lut_w=512, lut_h=512)
+    rng = np.random.default_rng(5)
+    n = 20000
+    ids = np.sort(rng.choice(F, n, replace=False)).astype(np.uint32)
+    t = time.perf_counter()
+    O.shade(fr, desc.width, desc.height, ivp, cam, ubo, pc, lights, cells, idx, maps, ids[:2000], threads=threads)
+    per = (time.perf_counter() - t) / 2000
+    n = int(min(F, max(2000, (args.cpu_seconds - t_cull) / max(per, 1e-9))))
+    ids = np.sort(rng.choice(F, n, replace=False)).astype(np.uint32)
+    t = time.perf_counter()
+    O.shade(fr, desc.width, desc.height, ivp, cam, ubo, pc, lights, cells, idx, maps, ids, threads=threads)
+    t_shade = time.perf_counter() - t
+    frame_s = t_cull + t_shade / n * F
+    return {
+        "value": round(F / frame_s / 1e6, 4),
+        "unit": "Mfrag/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"FP32 oracle (-O2 -ffp-contract=off), {threads} threads: full culling of the frame "
+                  f"({t_cull:.2f} s) + shading of {n} seeded random fragments ({t_shade:.2f} s), frame time = "
+                  f"cull + shading time scaled to {F} fragments",
+        "cull_s": round(t_cull, 3),
+        "cluster_light_tests_per_s": round(27648 * 65536 / t_cull, 1),
+    }
+
+
+if __name__ == "__main__":
+    main()

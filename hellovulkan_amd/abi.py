@@ -1,0 +1,330 @@
+"""ctypes binding of the C ABI in include/hvx.h and include/hvx_scene.h.
+
+The library is loaded from this package directory (built in-tree by
+hellovulkan_amd/build.py). torch is imported first when available so that the
+HIP runtime torch ships is the one libhvx.so binds to (both carry the soname
+libamdhip64.so.7): device pointers and streams from torch are then valid in
+libhvx calls. A missing library raises; there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+try:  # load torch's HIP runtime before libhvx.so resolves libamdhip64.so.7
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover - torch is present in this image
+    torch = None
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libhvx.so")
+
+HVX_OK = 0
+HVX_ERR_NULL = 1
+HVX_ERR_INVALID_ARG = 2
+HVX_ERR_CAPACITY = 3
+HVX_ERR_ALIGN = 4
+HVX_ERR_HIP = 5
+HVX_ERR_UNSUPPORTED = 6
+HVX_ERR_NO_DEVICE = 7
+
+FILTER_DIFFUSE = 0
+FILTER_SPECULAR = 1
+
+f4 = C.c_float * 4
+f16 = C.c_float * 16
+
+
+class Light(C.Structure):
+    _fields_ = [("position", f4), ("color", f4), ("radius", C.c_float), ("_pad", C.c_float * 3)]
+
+
+class AABB(C.Structure):
+    _fields_ = [("min_point", f4), ("max_point", f4)]
+
+
+class LightCell(C.Structure):
+    _fields_ = [("offset", C.c_uint32), ("count", C.c_uint32)]
+
+
+class CFUBO(C.Structure):
+    _fields_ = [
+        ("camera_inverse_projection", f16), ("camera_view", f16), ("screen_size", C.c_float * 2),
+        ("slice_scaling", C.c_float), ("slice_bias", C.c_float), ("camera_near", C.c_float),
+        ("camera_far", C.c_float), ("slice_count_x", C.c_uint32), ("slice_count_y", C.c_uint32),
+        ("slice_count_z", C.c_uint32), ("_pad", C.c_uint32 * 3),
+    ]
+
+
+class CameraUBO(C.Structure):
+    _fields_ = [("projection", f16), ("view", f16), ("position", f4), ("camera_near", C.c_float),
+                ("camera_far", C.c_float), ("_pad", C.c_float * 2)]
+
+
+class PBRPC(C.Structure):
+    _fields_ = [("light_intensity", C.c_float), ("base_reflectivity", C.c_float),
+                ("max_reflection_lod", C.c_float), ("light_falloff", C.c_float),
+                ("albedo_multiplier", C.c_float)]
+
+
+class FragSoA(C.Structure):
+    _fields_ = [("depth", C.c_void_p), ("normal", C.c_void_p * 3), ("albedo", C.c_void_p * 3),
+                ("roughness", C.c_void_p), ("metallic", C.c_void_p), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("inv_view_proj", f16)]
+
+
+class IBLMaps(C.Structure):
+    _fields_ = [("specular", C.c_void_p), ("specular_size", C.c_uint32), ("specular_mips", C.c_uint32),
+                ("diffuse", C.c_void_p), ("diffuse_size", C.c_uint32), ("_pad0", C.c_uint32),
+                ("brdf_lut", C.c_void_p), ("lut_width", C.c_uint32), ("lut_height", C.c_uint32)]
+
+
+class FilterOpts(C.Structure):
+    _fields_ = [("emulate_static_viewport", C.c_uint32), ("_reserved", C.c_uint32 * 3)]
+
+
+class CameraDesc(C.Structure):
+    _fields_ = [("eye", C.c_float * 3), ("target", C.c_float * 3), ("fovy", C.c_float),
+                ("znear", C.c_float), ("zfar", C.c_float), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("slice_count_x", C.c_uint32), ("slice_count_y", C.c_uint32), ("slice_count_z", C.c_uint32)]
+
+
+# numpy views of the array-of-struct layouts
+LIGHT_DTYPE = np.dtype([("position", "<f4", 4), ("color", "<f4", 4), ("radius", "<f4"), ("_pad", "<f4", 3)])
+AABB_DTYPE = np.dtype([("min_point", "<f4", 4), ("max_point", "<f4", 4)])
+CELL_DTYPE = np.dtype([("offset", "<u4"), ("count", "<u4")])
+assert LIGHT_DTYPE.itemsize == C.sizeof(Light) == 48
+assert C.sizeof(AABB) == 32 and C.sizeof(LightCell) == 8
+assert C.sizeof(CFUBO) == 176 and C.sizeof(CameraUBO) == 160 and C.sizeof(PBRPC) == 20
+
+vp = C.c_void_p
+u32 = C.c_uint32
+P = C.POINTER
+
+# (name, restype, argtypes) of every exported entry point of include/hvx.h
+# and include/hvx_scene.h
+SIGNATURES = [
+    ("hvx_abi_version", C.c_int, []),
+    ("hvx_status_str", C.c_char_p, [C.c_int]),
+    ("hvx_ctx_create", C.c_int, [C.c_int, P(vp)]),
+    ("hvx_ctx_destroy", C.c_int, [vp]),
+    ("hvx_ctx_last_error", C.c_char_p, [vp]),
+    ("hvx_cf_build_aabbs", C.c_int, [vp, P(CFUBO), vp, vp]),
+    ("hvx_cf_cull_lights", C.c_int, [vp, P(CFUBO), vp, vp, u32, u32, vp, vp, C.c_size_t, vp, vp, vp]),
+    ("hvx_cf_shade", C.c_int, [vp, P(FragSoA), P(CameraUBO), P(CFUBO), P(PBRPC), vp, u32, vp, vp,
+                               P(IBLMaps), vp, vp]),
+    ("hvx_ibl_brdf_lut", C.c_int, [vp, u32, u32, u32, vp, vp]),
+    ("hvx_ibl_equirect_to_cube", C.c_int, [vp, vp, u32, u32, vp, u32, vp]),
+    ("hvx_ibl_cube_mips", C.c_int, [vp, vp, u32, u32, vp]),
+    ("hvx_ibl_filter", C.c_int, [vp, C.c_int, vp, u32, vp, u32, u32, u32, P(FilterOpts), vp]),
+    ("hvx_cube_chain_floats", C.c_size_t, [u32, u32]),
+    ("hvx_cube_level_offset_floats", C.c_size_t, [u32, u32]),
+    ("hvx_mip_count", u32, [u32]),
+    ("hvx_camera_build", C.c_int, [P(CameraDesc), P(CameraUBO), P(CFUBO), P(C.c_float)]),
+    ("hvx_scene_lights", C.c_int, [C.c_uint64, u32, vp]),
+    ("hvx_scene_fragments", C.c_int, [C.c_uint64, P(CameraDesc), P(C.c_float), vp, vp, vp, vp, vp, vp, vp, vp,
+                                      vp, C.c_int]),
+    ("hvx_scene_equirect", C.c_int, [C.c_uint64, u32, u32, vp, C.c_int]),
+]
+
+_lib = None
+
+
+def load_library() -> C.CDLL:
+    """Loads libhvx.so from the package directory; raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run hellovulkan_amd/build.py "
+                               "(or __graft_entry__.build()) first")
+        lib = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+class HvxError(RuntimeError):
+    def __init__(self, status: int, where: str, detail: str = ""):
+        self.status = status
+        lib = load_library()
+        msg = f"{where}: {lib.hvx_status_str(status).decode()} ({status})"
+        if detail:
+            msg += f": {detail}"
+        super().__init__(msg)
+
+
+def camera_build(eye, target, fovy, znear, zfar, width, height, grid):
+    """Returns (CameraDesc, CameraUBO, CFUBO, inv_view_proj[16] as np.float32)."""
+    lib = load_library()
+    d = CameraDesc()
+    d.eye[:] = [float(v) for v in eye]
+    d.target[:] = [float(v) for v in target]
+    d.fovy, d.znear, d.zfar = float(fovy), float(znear), float(zfar)
+    d.width, d.height = int(width), int(height)
+    d.slice_count_x, d.slice_count_y, d.slice_count_z = (int(g) for g in grid)
+    cam, ubo = CameraUBO(), CFUBO()
+    ivp = np.zeros(16, np.float32)
+    st = lib.hvx_camera_build(C.byref(d), C.byref(cam), C.byref(ubo), ivp.ctypes.data_as(P(C.c_float)))
+    if st:
+        raise HvxError(st, "hvx_camera_build")
+    return d, cam, ubo, ivp
+
+
+def scene_lights(seed: int, n: int) -> np.ndarray:
+    out = np.zeros(n, LIGHT_DTYPE)
+    st = load_library().hvx_scene_lights(C.c_uint64(seed), n, out.ctypes.data)
+    if st:
+        raise HvxError(st, "hvx_scene_lights")
+    return out
+
+
+def scene_fragments(seed: int, desc: CameraDesc, ivp: np.ndarray, threads: int = 8) -> dict:
+    W, H = desc.width, desc.height
+    planes = {k: np.empty(W * H, np.float32) for k in ("depth", "nx", "ny", "nz", "ar", "ag", "ab", "rough")}
+    metal = np.empty(W * H, np.uint8)
+    st = load_library().hvx_scene_fragments(
+        C.c_uint64(seed), C.byref(desc), np.ascontiguousarray(ivp, np.float32).ctypes.data_as(P(C.c_float)),
+        *(planes[k].ctypes.data for k in ("depth", "nx", "ny", "nz", "ar", "ag", "ab", "rough")),
+        metal.ctypes.data, threads)
+    if st:
+        raise HvxError(st, "hvx_scene_fragments")
+    planes["metal"] = metal
+    return planes
+
+
+def scene_equirect(seed: int, w: int, h: int, threads: int = 8) -> np.ndarray:
+    out = np.empty((h, w, 4), np.float32)
+    st = load_library().hvx_scene_equirect(C.c_uint64(seed), w, h, out.ctypes.data, threads)
+    if st:
+        raise HvxError(st, "hvx_scene_equirect")
+    return out
+
+
+def cube_chain_floats(size: int, mips: int) -> int:
+    return int(load_library().hvx_cube_chain_floats(size, mips))
+
+
+def cube_level_offset_floats(size: int, level: int) -> int:
+    return int(load_library().hvx_cube_level_offset_floats(size, level))
+
+
+def mip_count(size: int) -> int:
+    return int(load_library().hvx_mip_count(size))
+
+
+def _ptr(t) -> int:
+    """Device (or host) address of a torch tensor / numpy array; 0 for None."""
+    if t is None:
+        return 0
+    if hasattr(t, "data_ptr"):
+        return int(t.data_ptr())
+    return int(t.ctypes.data)
+
+
+class Context:
+    """Owns an hvx_ctx; each method is one C-ABI call on the given HIP stream
+    (a torch.cuda.Stream, a raw handle, or None for torch's current stream)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        st = self.lib.hvx_ctx_create(device, C.byref(h))
+        if st:
+            raise HvxError(st, "hvx_ctx_create")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            self.lib.hvx_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _stream(self, stream):
+        if stream is None and torch is not None:
+            return torch.cuda.current_stream(self.device).cuda_stream
+        if stream is not None and hasattr(stream, "cuda_stream"):
+            return stream.cuda_stream
+        return stream or 0
+
+    def _check(self, st: int, where: str):
+        if st:
+            raise HvxError(st, where, self.lib.hvx_ctx_last_error(self.h).decode())
+
+    def build_aabbs(self, ubo: CFUBO, clusters, stream=None):
+        self._check(self.lib.hvx_cf_build_aabbs(self.h, C.byref(ubo), _ptr(clusters), self._stream(stream)),
+                    "hvx_cf_build_aabbs")
+
+    def cull_lights(self, ubo: CFUBO, clusters, lights, n_lights: int, cap: int, cells, indices,
+                    indices_capacity: int, total=None, overflow=None, stream=None):
+        self._check(self.lib.hvx_cf_cull_lights(self.h, C.byref(ubo), _ptr(clusters), _ptr(lights), n_lights, cap,
+                                                _ptr(cells), _ptr(indices), indices_capacity, _ptr(total),
+                                                _ptr(overflow), self._stream(stream)), "hvx_cf_cull_lights")
+
+    def shade(self, frags: FragSoA, cam: CameraUBO, ubo: CFUBO, pc: PBRPC, lights, n_lights: int, cells, indices,
+              maps: IBLMaps, rgb, stream=None):
+        self._check(self.lib.hvx_cf_shade(self.h, C.byref(frags), C.byref(cam), C.byref(ubo), C.byref(pc),
+                                          _ptr(lights), n_lights, _ptr(cells), _ptr(indices), C.byref(maps),
+                                          _ptr(rgb), self._stream(stream)), "hvx_cf_shade")
+
+    def brdf_lut(self, w: int, h: int, samples: int, out, stream=None):
+        self._check(self.lib.hvx_ibl_brdf_lut(self.h, w, h, samples, _ptr(out), self._stream(stream)),
+                    "hvx_ibl_brdf_lut")
+
+    def equirect_to_cube(self, eq, w: int, h: int, cube, face: int, stream=None):
+        self._check(self.lib.hvx_ibl_equirect_to_cube(self.h, _ptr(eq), w, h, _ptr(cube), face,
+                                                      self._stream(stream)), "hvx_ibl_equirect_to_cube")
+
+    def cube_mips(self, chain, face: int, levels: int, stream=None):
+        self._check(self.lib.hvx_ibl_cube_mips(self.h, _ptr(chain), face, levels, self._stream(stream)),
+                    "hvx_ibl_cube_mips")
+
+    def filter(self, ftype: int, cube, face: int, out, out_face: int, out_mips: int, samples: int,
+               emulate_static_viewport: bool = False, stream=None):
+        opts = FilterOpts()
+        opts.emulate_static_viewport = 1 if emulate_static_viewport else 0
+        self._check(self.lib.hvx_ibl_filter(self.h, ftype, _ptr(cube), face, _ptr(out), out_face, out_mips, samples,
+                                            C.byref(opts), self._stream(stream)), "hvx_ibl_filter")
+
+
+def frag_soa(planes: dict, width: int, height: int, ivp) -> FragSoA:
+    """FragSoA over a dict of per-plane tensors/arrays (keys as scene_fragments)."""
+    fr = FragSoA()
+    fr.depth = _ptr(planes["depth"])
+    fr.normal[0], fr.normal[1], fr.normal[2] = (_ptr(planes[k]) for k in ("nx", "ny", "nz"))
+    fr.albedo[0], fr.albedo[1], fr.albedo[2] = (_ptr(planes[k]) for k in ("ar", "ag", "ab"))
+    fr.roughness = _ptr(planes["rough"])
+    fr.metallic = _ptr(planes["metal"])
+    fr.width, fr.height = width, height
+    fr.inv_view_proj[:] = [float(v) for v in np.asarray(ivp, np.float32)]
+    return fr
+
+
+def ibl_maps(spec, spec_size, spec_mips, diff, diff_size, lut, lut_w, lut_h) -> IBLMaps:
+    m = IBLMaps()
+    m.specular, m.specular_size, m.specular_mips = _ptr(spec), spec_size, spec_mips
+    m.diffuse, m.diffuse_size = _ptr(diff), diff_size
+    m.brdf_lut, m.lut_width, m.lut_height = _ptr(lut), lut_w, lut_h
+    return m
+
+
+def pbr_defaults() -> PBRPC:
+    """Effective PushConstPBR of the clustered-forward app: UIData.h:22-29
+    defaults with albedoMultipler = 0.01 (AppPBRClusterForward.cpp:21)."""
+    pc = PBRPC()
+    pc.light_intensity = 1.75
+    pc.base_reflectivity = 0.01
+    pc.max_reflection_lod = 1.0
+    pc.light_falloff = 0.1
+    pc.albedo_multiplier = 0.01
+    return pc

@@ -1,0 +1,495 @@
+// cf_kernels.hip — clustered forward on gfx950: cluster AABBs, light culling
+// and clustered Cook-Torrance + IBL shading.
+//
+// Reference kernels (HelloVulkan/Shaders/ClusteredForward):
+//   AABBGenerator.comp:39-89   -> aabb_kernel (one lane per cluster)
+//   LightCulling.comp:59-87    -> light_prep_kernel + cull_kernel + scan/scatter
+//   Scene.frag:60-149,
+//   Radiance.glsl:2-56,
+//   PBR/Ambient.glsl:1-29      -> shade_kernel (one lane per SoA fragment)
+//
+// Culling. The reference runs one invocation per cluster that walks every
+// light and keeps the first `cap` hits. Here a workgroup owns a 4x4x1 block
+// of clusters. Lights stream through in tiles of 256 (one per lane); each lane
+// tests its light against the union AABB of the block (a conservative filter:
+// the per-axis distance terms of the union box are never larger than a member
+// box's, term by term and under rounding, so a light that fails the union
+// test fails every member test). Survivors are compacted into an LDS
+// candidate list in ascending light order with a wavefront ballot and a
+// workgroup prefix sum. Once enough candidates are staged, every wave tests its
+// clusters against them, 64 candidates per step, and appends hits with a
+// second ballot + mbcnt prefix, keeping the first `cap` in ascending order —
+// exactly the reference's list. The per-test arithmetic is the reference's,
+// in the same order, with contraction off, so lists match it bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "hvx_device.h"
+#include "hvx_internal.h"
+
+namespace hvx {
+namespace {
+
+// ------------------------------------------------------------------ AABBs
+__global__ __launch_bounds__(256) void aabb_kernel(hvx_cf_ubo u, hvx_aabb* __restrict__ out) {
+    const uint32_t X = u.slice_count_x, Y = u.slice_count_y, Z = u.slice_count_z;
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= X * Y * Z) return;
+    const uint32_t x = id % X, y = (id / X) % Y, z = id / (X * Y);
+    const float npx = 1.0f / (float)X, npy = 1.0f / (float)Y;
+    float vmax[4], vmin[4];
+    matvec4(u.camera_inverse_projection, (((float)x + 1.0f) * npx) * 2.0f - 1.0f,
+            (((float)y + 1.0f) * npy) * 2.0f - 1.0f, 0.0f, 1.0f, vmax);
+    matvec4(u.camera_inverse_projection, ((float)x * npx) * 2.0f - 1.0f, ((float)y * npy) * 2.0f - 1.0f, 0.0f,
+            1.0f, vmin);
+    const f3 maxVS = {vmax[0] / vmax[3], vmax[1] / vmax[3], vmax[2] / vmax[3]};
+    const f3 minVS = {vmin[0] / vmin[3], vmin[1] / vmin[3], vmin[2] / vmin[3]};
+    const float nearv = u.camera_near, farv = u.camera_far;
+    const float ratio = farv / nearv;
+    const float tileNear = (-nearv) * powf_(ratio, (float)z / (float)Z);
+    const float tileFar = (-nearv) * powf_(ratio, (float)(z + 1) / (float)Z);
+    const float* mn = &minVS.x;
+    const float* mx = &maxVS.x;
+    hvx_aabb box;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float p0 = (mn[c] * tileNear) / minVS.z;
+        const float p1 = (mn[c] * tileFar) / minVS.z;
+        const float p2 = (mx[c] * tileNear) / maxVS.z;
+        const float p3 = (mx[c] * tileFar) / maxVS.z;
+        box.min_point[c] = gmin(gmin(p0, p1), gmin(p2, p3));
+        box.max_point[c] = gmax(gmax(p0, p1), gmax(p2, p3));
+    }
+    box.min_point[3] = 0.0f;
+    box.max_point[3] = 0.0f;
+    out[id] = box;
+}
+
+// -------------------------------------------------- view-space light prep
+// posView = cameraView * posWorld (LightCulling.comp:53), r^2 as the test
+// computes it (:56). Evaluated once per light instead of once per test.
+__global__ __launch_bounds__(256) void light_prep_kernel(hvx_cf_ubo u, const hvx_light* __restrict__ lights,
+                                                         uint32_t n, float4* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const hvx_light& l = lights[i];
+    float o[4];
+    matvec4(u.camera_view, l.position[0], l.position[1], l.position[2], l.position[3], o);
+    out[i] = make_float4(o[0], o[1], o[2], l.radius * l.radius);
+}
+
+struct Box {
+    float mn[3], mx[3];
+};
+
+// SquaredDistancePointAABB + TestSphereAABB (LightCulling.comp:26-57): terms
+// are added in axis order; an absent term adds nothing.
+__device__ __forceinline__ bool sphere_box(float4 p, const Box& b) {
+    const float v[3] = {p.x, p.y, p.z};
+    float t[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float d = 0.0f;
+        if (v[a] < b.mn[a]) d = b.mn[a] - v[a];
+        else if (v[a] > b.mx[a]) d = b.mx[a] - v[a];
+        t[a] = d * d;
+    }
+    const float sq = (t[0] + t[1]) + t[2];
+    return sq < p.w;
+}
+
+constexpr int kGX = 4, kGY = 4, kGZ = 1;
+constexpr int kG = kGX * kGY * kGZ;       // clusters per workgroup
+constexpr int kWaves = 4;                 // 256 lanes
+constexpr int kPerWave = kG / kWaves;     // clusters per wave
+constexpr int kTile = 256;                // lights per level-1 step
+constexpr int kFlush = 256;               // staged candidates that trigger level 2
+constexpr int kCandMax = kFlush + kTile;
+
+__device__ __forceinline__ uint32_t lane_prefix(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__global__ __launch_bounds__(256) void cull_kernel(hvx_cf_ubo u, const hvx_aabb* __restrict__ clusters,
+                                                   const float4* __restrict__ lvs, uint32_t n, uint32_t cap,
+                                                   int want_overflow, uint32_t* __restrict__ lists,
+                                                   uint32_t* __restrict__ counts, uint32_t* __restrict__ over) {
+    __shared__ float4 s_pos[kCandMax];
+    __shared__ uint32_t s_idx[kCandMax];
+    __shared__ uint32_t s_wcnt[kWaves];
+    __shared__ uint32_t s_wdone[kWaves];
+
+    const uint32_t X = u.slice_count_x, Y = u.slice_count_y, Z = u.slice_count_z;
+    const uint32_t ngx = (X + kGX - 1) / kGX, ngy = (Y + kGY - 1) / kGY;
+    const uint32_t g = blockIdx.x;
+    const uint32_t gx = g % ngx, gy = (g / ngx) % ngy, gz = g / (ngx * ngy);
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+    // union box of the valid clusters of this block (uniform)
+    Box U;
+    for (int a = 0; a < 3; ++a) { U.mn[a] = hvx::kInfF; U.mx[a] = -hvx::kInfF; }
+    for (int k = 0; k < kG; ++k) {
+        const uint32_t cx = gx * kGX + (k % kGX), cy = gy * kGY + (k / kGX) % kGY, cz = gz * kGZ + k / (kGX * kGY);
+        if (cx >= X || cy >= Y || cz >= Z) continue;
+        const hvx_aabb& b = clusters[cx + cy * X + cz * X * Y];
+        for (int a = 0; a < 3; ++a) {
+            U.mn[a] = gmin(U.mn[a], b.min_point[a]);
+            U.mx[a] = gmax(U.mx[a], b.max_point[a]);
+        }
+    }
+
+    // this wave's clusters
+    Box cb[kPerWave];
+    uint32_t cid[kPerWave], cnt[kPerWave], ovf[kPerWave];
+    bool done[kPerWave];
+#pragma unroll
+    for (int j = 0; j < kPerWave; ++j) {
+        const int k = (int)wave * kPerWave + j;
+        const uint32_t cx = gx * kGX + (k % kGX), cy = gy * kGY + (k / kGX) % kGY, cz = gz * kGZ + k / (kGX * kGY);
+        const bool valid = cx < X && cy < Y && cz < Z;
+        cid[j] = valid ? cx + cy * X + cz * X * Y : 0u;
+        cnt[j] = 0;
+        ovf[j] = 0;
+        done[j] = !valid;
+        if (valid) {
+            const hvx_aabb& b = clusters[cid[j]];
+            for (int a = 0; a < 3; ++a) { cb[j].mn[a] = b.min_point[a]; cb[j].mx[a] = b.max_point[a]; }
+        } else {
+            for (int a = 0; a < 3; ++a) { cb[j].mn[a] = 0.0f; cb[j].mx[a] = 0.0f; }
+        }
+    }
+
+    uint32_t base = 0, ncand = 0;
+    for (;;) {
+        // ---- level 1: one light per lane against the union box
+        const uint32_t li = base + tid;
+        bool hit = false;
+        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (li < n) {
+            p = lvs[li];
+            hit = sphere_box(p, U);
+        }
+        const uint64_t m = __ballot(hit);
+        if (lane == 0) s_wcnt[wave] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t woff = 0, tot = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t c = s_wcnt[w];
+            woff += (w < (int)wave) ? c : 0u;
+            tot += c;
+        }
+        if (hit) {
+            const uint32_t pos = ncand + woff + lane_prefix(m);
+            s_pos[pos] = p;
+            s_idx[pos] = li;
+        }
+        __syncthreads();
+        ncand += tot;
+        base += kTile;
+        if (ncand < (uint32_t)kFlush && base < n) continue;
+
+        // ---- level 2: each wave tests its clusters against the staged candidates
+        bool wave_done = true;
+#pragma unroll
+        for (int j = 0; j < kPerWave; ++j) {
+            if (!done[j]) {
+                for (uint32_t k0 = 0; k0 < ncand; k0 += 64) {
+                    const uint32_t k = k0 + lane;
+                    bool h = false;
+                    if (k < ncand) h = sphere_box(s_pos[k], cb[j]);
+                    const uint64_t mm = __ballot(h);
+                    if (cnt[j] < cap) {
+                        const uint32_t pos = cnt[j] + lane_prefix(mm);
+                        if (h && pos < cap) lists[(size_t)cid[j] * cap + pos] = s_idx[k];
+                        const uint32_t nh = (uint32_t)__popcll(mm);
+                        if (cnt[j] + nh > cap) ovf[j] = 1;
+                        cnt[j] = min(cnt[j] + nh, cap);
+                    } else if (mm != 0) {
+                        ovf[j] = 1;
+                    }
+                    if (cnt[j] >= cap && (ovf[j] || !want_overflow)) {
+                        done[j] = true;
+                        break;
+                    }
+                }
+            }
+            wave_done = wave_done && done[j];
+        }
+        if (lane == 0) s_wdone[wave] = wave_done ? 1u : 0u;
+        __syncthreads();
+        ncand = 0;
+        bool all_done = true;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) all_done = all_done && (s_wdone[w] != 0);
+        if (all_done || base >= n) break;
+    }
+
+    if (lane == 0) {
+#pragma unroll
+        for (int j = 0; j < kPerWave; ++j) {
+            const int k = (int)wave * kPerWave + j;
+            const uint32_t cx = gx * kGX + (k % kGX), cy = gy * kGY + (k / kGX) % kGY, cz = gz * kGZ + k / (kGX * kGY);
+            if (cx < X && cy < Y && cz < Z) {
+                counts[cid[j]] = cnt[j];
+                over[cid[j]] = ovf[j];
+            }
+        }
+    }
+}
+
+// Exclusive scan of the counts in cluster order -> LightCell offsets (the
+// deterministic replacement of atomicAdd(globalIndex, count)).
+__global__ __launch_bounds__(1024) void scan_kernel(const uint32_t* __restrict__ counts,
+                                                    const uint32_t* __restrict__ over, uint32_t C,
+                                                    hvx_light_cell* __restrict__ cells, uint32_t* __restrict__ total,
+                                                    uint32_t* __restrict__ novf) {
+    __shared__ uint32_t s_sum[1024];
+    __shared__ uint32_t s_ov[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (C + 1023) / 1024;
+    const uint32_t b = min(t * per, C), e = min(b + per, C);
+    uint32_t local = 0, lov = 0;
+    for (uint32_t c = b; c < e; ++c) {
+        local += counts[c];
+        lov += over[c];
+    }
+    s_sum[t] = local;
+    s_ov[t] = lov;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over 1024 partial sums
+    for (uint32_t d = 1; d < 1024; d <<= 1) {
+        const uint32_t v = t >= d ? s_sum[t - d] : 0u;
+        const uint32_t w = t >= d ? s_ov[t - d] : 0u;
+        __syncthreads();
+        s_sum[t] += v;
+        s_ov[t] += w;
+        __syncthreads();
+    }
+    uint32_t run = s_sum[t] - local;
+    for (uint32_t c = b; c < e; ++c) {
+        const uint32_t k = counts[c];
+        hvx_light_cell cell;
+        cell.offset = run;
+        cell.count = k;
+        cells[c] = cell;
+        run += k;
+    }
+    if (t == 1023) {
+        if (total) *total = s_sum[1023];
+        if (novf) *novf = s_ov[1023];
+    }
+}
+
+__global__ __launch_bounds__(256) void scatter_kernel(const uint32_t* __restrict__ lists, uint32_t cap, uint32_t C,
+                                                      const hvx_light_cell* __restrict__ cells,
+                                                      uint32_t* __restrict__ indices) {
+    const uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (c >= C) return;
+    const hvx_light_cell cell = cells[c];
+    for (uint32_t j = lane; j < cell.count; j += 64) indices[cell.offset + j] = lists[(size_t)c * cap + j];
+}
+
+// ---------------------------------------------------------------- shading
+struct ShadeArgs {
+    const float* depth;
+    const float* nrm[3];
+    const float* alb[3];
+    const float* rough;
+    const uint8_t* metal;
+    uint32_t W, H;
+    float ivp[16];
+    float cam[3];
+    float nearv, farv, scale, bias, scr_x, scr_y;
+    uint32_t X, Y, Z;
+    hvx_pbr_pc pc;
+    const hvx_light* lights;
+    const hvx_light_cell* cells;
+    const uint32_t* indices;
+    const float4* spec;
+    uint32_t spec_size, spec_mips;
+    const float4* diff;
+    uint32_t diff_size;
+    const float2* lut;
+    uint32_t lut_w, lut_h;
+    float* out;
+};
+
+__device__ __forceinline__ f3 sample_spec_lod(const ShadeArgs& a, f3 dir, float lod) {
+    const float q = (float)(a.spec_mips - 1);
+    const float lam = gclamp(lod, 0.0f, q);
+    const float fl = floorf_(lam);
+    const uint32_t l0 = (uint32_t)fl;
+    const uint32_t l1 = min(l0 + 1, a.spec_mips - 1);
+    const float d = lam - fl;
+    const int s0 = (int)max(a.spec_size >> l0, 1u), s1 = (int)max(a.spec_size >> l1, 1u);
+    const f3 c0 = sample_cube(a.spec + cube_level_offset_floats(a.spec_size, l0) / 4, s0, dir);
+    const f3 c1 = sample_cube(a.spec + cube_level_offset_floats(a.spec_size, l1) / 4, s1, dir);
+    return {(1.0f - d) * c0.x + d * c1.x, (1.0f - d) * c0.y + d * c1.y, (1.0f - d) * c0.z + d * c1.z};
+}
+
+__global__ __launch_bounds__(256) void shade_kernel(ShadeArgs a) {
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nfrag = a.W * a.H;
+    if (idx >= nfrag) return;
+    const float PI = kPiF;
+    const uint32_t px = idx % a.W, py = idx / a.W;
+    const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
+    const float z = a.depth[idx];
+    // LinearDepth + cluster lookup (Scene.frag:55-78)
+    const float lin = (a.nearv * a.farv) / (a.farv + z * (a.nearv - a.farv));
+    const float zf = gmax(log2f_(lin) * a.scale + a.bias, 0.0f);
+    uint32_t zi = (uint32_t)zf;
+    if (zi >= a.Z) zi = a.Z - 1;
+    const float tsx = a.scr_x / (float)a.X, tsy = a.scr_y / (float)a.Y;
+    const uint32_t cx = min((uint32_t)(fx / tsx), a.X - 1), cy = min((uint32_t)(fy / tsy), a.Y - 1);
+    const hvx_light_cell cell = a.cells[cx + cy * a.X + zi * a.X * a.Y];
+
+    float wh[4];
+    matvec4(a.ivp, (fx / (float)a.W) * 2.0f - 1.0f, (fy / (float)a.H) * 2.0f - 1.0f, z, 1.0f, wh);
+    const f3 wp = {wh[0] / wh[3], wh[1] / wh[3], wh[2] / wh[3]};
+
+    const float alb[3] = {a.alb[0][idx], a.alb[1][idx], a.alb[2][idx]};
+    const float metallic = (float)a.metal[idx] / 255.0f;
+    const float rough = a.rough[idx];
+    const float rr = rough + 1.0f;
+    const float alphaRough = (rr * rr) / 8.0f;
+    const f3 N = {a.nrm[0][idx], a.nrm[1][idx], a.nrm[2][idx]};
+    const f3 V = normalize3(sub3(make3(a.cam[0], a.cam[1], a.cam[2]), wp));
+    const float NoV = gmax(dot3(N, V), 0.0f);
+    const float base = a.pc.base_reflectivity;
+    float F0[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) F0[c] = base * (1.0f - metallic) + alb[c] * metallic;
+    float Lo[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Lo[c] = alb[c] * a.pc.albedo_multiplier;
+
+    const float alpha = rough * rough;
+    const float alpha2 = alpha * alpha;
+    const float oneMinusMetal = 1.0f - metallic;
+    const float GVden = NoV * (1.0f - alphaRough) + alphaRough;
+    const float GV = NoV / GVden;
+    for (uint32_t k = 0; k < cell.count; ++k) {
+        const hvx_light& l = a.lights[a.indices[cell.offset + k]];
+        const float4 lpos = *reinterpret_cast<const float4*>(l.position);
+        const float4 lcol = *reinterpret_cast<const float4*>(l.color);
+        const float lrad = l.radius;
+        const f3 d = sub3(make3(lpos.x, lpos.y, lpos.z), wp);
+        const f3 L = normalize3(d);
+        const f3 Hh = normalize3(add3(V, L));
+        const float NoH = gmax(dot3(N, Hh), 0.0f);
+        const float NoL = gmax(dot3(N, L), 0.0f);
+        const float HoV = gmax(dot3(Hh, V), 0.0f);
+        const float dist = length3(d);
+        const float att = gmax(1.0f - dist / lrad, 0.0f) / powf_(dist, a.pc.light_falloff);
+        const float rad[3] = {(lcol.x * att) * a.pc.light_intensity, (lcol.y * att) * a.pc.light_intensity,
+                              (lcol.z * att) * a.pc.light_intensity};
+        const float NoH2 = NoH * NoH;
+        float den = NoH2 * (alpha2 - 1.0f) + 1.0f;
+        den = (PI * den) * den;
+        const float D = alpha2 / den;
+        const float GL = NoL / (NoL * (1.0f - alphaRough) + alphaRough);
+        const float G = GL * GV;
+        const float p5 = pow5f(gclamp(1.0f - HoV, 0.0f, 1.0f));
+        const float denom = ((4.0f * NoV) * NoL) + 0.0001f;
+        const float DG = D * G;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float F = F0[c] + (1.0f - F0[c]) * p5;
+            const float spec = (DG * F) / denom;
+            const float kD = (1.0f - F) * oneMinusMetal;
+            const float diffuse = (kD * alb[c]) / PI;
+            Lo[c] = Lo[c] + ((diffuse + spec) * rad[c]) * NoL;
+        }
+    }
+
+    // Ambient (PBR/Ambient.glsl:1-29)
+    const float p5v = pow5f(gclamp(1.0f - NoV, 0.0f, 1.0f));
+    const f3 irr = sample_cube(a.diff, (int)a.diff_size, N);
+    const f3 I = {-V.x, -V.y, -V.z};
+    const float dNI = dot3(N, I);
+    const f3 R = sub3(I, mul3(N, 2.0f * dNI));
+    const f3 pre = sample_spec_lod(a, R, rough * a.pc.max_reflection_lod);
+    const float2 brdf = sample_2d_rg_repeat(a.lut, (int)a.lut_w, (int)a.lut_h, NoV, rough);
+    const float irr3[3] = {irr.x, irr.y, irr.z}, pre3[3] = {pre.x, pre.y, pre.z};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float F = F0[c] + (gmax(1.0f - rough, F0[c]) - F0[c]) * p5v;
+        const float kD = (1.0f - F) * oneMinusMetal;
+        const float diffuse = irr3[c] * alb[c];
+        const float specular = pre3[c] * (F * brdf.x + brdf.y);
+        const float ambient = (kD * diffuse + specular) * 1.0f;
+        a.out[(size_t)c * nfrag + idx] = (ambient + 0.0f) + Lo[c];
+    }
+}
+
+}  // namespace
+
+// ================================================================ launchers
+hipError_t launch_build_aabbs(const hvx_cf_ubo& ubo, hvx_aabb* d_clusters, hipStream_t s) {
+    const uint32_t C = ubo.slice_count_x * ubo.slice_count_y * ubo.slice_count_z;
+    hipLaunchKernelGGL(aabb_kernel, dim3((C + 255) / 256), dim3(256), 0, s, ubo, d_clusters);
+    return hipGetLastError();
+}
+
+hipError_t launch_cull(const hvx_cf_ubo& ubo, const hvx_aabb* d_clusters, const hvx_light* d_lights,
+                       uint32_t n_lights, uint32_t cap, hvx_light_cell* d_cells, uint32_t* d_indices,
+                       uint32_t* d_total, uint32_t* d_overflow, float4* light_vs, uint32_t* lists, uint32_t* counts,
+                       uint32_t* over, hipStream_t s) {
+    const uint32_t X = ubo.slice_count_x, Y = ubo.slice_count_y, Z = ubo.slice_count_z;
+    const uint32_t C = X * Y * Z;
+    if (n_lights > 0) {
+        hipLaunchKernelGGL(light_prep_kernel, dim3((n_lights + 255) / 256), dim3(256), 0, s, ubo, d_lights, n_lights,
+                           light_vs);
+    }
+    const uint32_t groups = ((X + kGX - 1) / kGX) * ((Y + kGY - 1) / kGY) * ((Z + kGZ - 1) / kGZ);
+    hipLaunchKernelGGL(cull_kernel, dim3(groups), dim3(256), 0, s, ubo, d_clusters, light_vs, n_lights, cap,
+                       d_overflow != nullptr ? 1 : 0, lists, counts, over);
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, counts, over, C, d_cells, d_total, d_overflow);
+    hipLaunchKernelGGL(scatter_kernel, dim3((C + 3) / 4), dim3(256), 0, s, lists, cap, C, d_cells, d_indices);
+    return hipGetLastError();
+}
+
+hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const hvx_cf_ubo& ubo,
+                        const hvx_pbr_pc& pc, const hvx_light* d_lights, const hvx_light_cell* d_cells,
+                        const uint32_t* d_indices, const hvx_ibl_maps& ibl, float* d_rgb, hipStream_t s) {
+    ShadeArgs a;
+    a.depth = fr.depth;
+    for (int c = 0; c < 3; ++c) {
+        a.nrm[c] = fr.normal[c];
+        a.alb[c] = fr.albedo[c];
+    }
+    a.rough = fr.roughness;
+    a.metal = fr.metallic;
+    a.W = fr.width;
+    a.H = fr.height;
+    for (int k = 0; k < 16; ++k) a.ivp[k] = fr.inv_view_proj[k];
+    for (int k = 0; k < 3; ++k) a.cam[k] = cam.position[k];
+    a.nearv = ubo.camera_near;
+    a.farv = ubo.camera_far;
+    a.scale = ubo.slice_scaling;
+    a.bias = ubo.slice_bias;
+    a.scr_x = ubo.screen_size[0];
+    a.scr_y = ubo.screen_size[1];
+    a.X = ubo.slice_count_x;
+    a.Y = ubo.slice_count_y;
+    a.Z = ubo.slice_count_z;
+    a.pc = pc;
+    a.lights = d_lights;
+    a.cells = d_cells;
+    a.indices = d_indices;
+    a.spec = reinterpret_cast<const float4*>(ibl.specular);
+    a.spec_size = ibl.specular_size;
+    a.spec_mips = ibl.specular_mips;
+    a.diff = reinterpret_cast<const float4*>(ibl.diffuse);
+    a.diff_size = ibl.diffuse_size;
+    a.lut = reinterpret_cast<const float2*>(ibl.brdf_lut);
+    a.lut_w = ibl.lut_width;
+    a.lut_h = ibl.lut_height;
+    a.out = d_rgb;
+    const uint32_t F = fr.width * fr.height;
+    hipLaunchKernelGGL(shade_kernel, dim3((F + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace hvx

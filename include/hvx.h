@@ -1,0 +1,279 @@
+/*
+ * hvx.h — C ABI of the MI355X-native clustered-forward + image-based-lighting
+ * compute path.
+ *
+ * Each entry point replaces one pipeline stage of HelloVulkan's
+ * clustered-forward / IBL path (the stage "plugins" derived from PipelineBase,
+ * HelloVulkan/Header/Pipelines/PipelineBase.h:24-95). The caller owns all
+ * device memory; every call is asynchronous on the HIP stream passed as
+ * `void* stream` (NULL = the default stream), returns an hvx_status code and
+ * never throws. A context must not be used by two host threads at once.
+ *
+ * Struct layouts mirror the reference's GPU-visible layouts byte for byte
+ * (std430 / std140 as declared in the GLSL and the C++ UBO structs) so that a
+ * reference-side caller can hand its buffers over unchanged.
+ */
+#ifndef HVX_H_
+#define HVX_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HVX_ABI_VERSION 1
+
+/* ------------------------------------------------------------------ status */
+typedef enum hvx_status {
+    HVX_OK = 0,
+    HVX_ERR_NULL = 1,          /* a required pointer is NULL                   */
+    HVX_ERR_INVALID_ARG = 2,   /* a size / count / enum is out of range        */
+    HVX_ERR_CAPACITY = 3,      /* an output buffer is too small                */
+    HVX_ERR_ALIGN = 4,         /* a device pointer is misaligned               */
+    HVX_ERR_HIP = 5,           /* the HIP runtime reported an error            */
+    HVX_ERR_UNSUPPORTED = 6,   /* option combination not implemented           */
+    HVX_ERR_NO_DEVICE = 7      /* no usable gfx950 device                      */
+} hvx_status;
+
+/* ------------------------------------------------------------------ types  */
+
+/* LightData (HelloVulkan/Header/Resources/ResourcesLight.h:13-21,
+ * Shaders/LightData.glsl:1-6): std430 stride 48 B. */
+typedef struct hvx_light {
+    float position[4]; /* world space, w = 1 */
+    float color[4];    /* linear RGB, w = 1  */
+    float radius;
+    float _pad[3];
+} hvx_light;
+
+/* AABB (ResourcesLight.h:24-30, Shaders/AABB/AABB.glsl:1-5): view space. */
+typedef struct hvx_aabb {
+    float min_point[4]; /* w = 0 */
+    float max_point[4]; /* w = 0 */
+} hvx_aabb;
+
+/* LightCell (ResourcesLight.h:33-39, ClusteredForward/Header.glsl:1-5). */
+typedef struct hvx_light_cell {
+    uint32_t offset;
+    uint32_t count;
+} hvx_light_cell;
+
+/* ClusterForwardUBO (HelloVulkan/Header/UBOs.h:67-89,
+ * ClusteredForward/Header.glsl:7-19). Matrices are column-major (glm). 176 B. */
+typedef struct hvx_cf_ubo {
+    float camera_inverse_projection[16];
+    float camera_view[16];
+    float screen_size[2];
+    float slice_scaling;
+    float slice_bias;
+    float camera_near;
+    float camera_far;
+    uint32_t slice_count_x;
+    uint32_t slice_count_y;
+    uint32_t slice_count_z;
+    uint32_t _pad[3];
+} hvx_cf_ubo;
+
+/* CameraUBO (UBOs.h:7-19, Shaders/CameraUBO.glsl:1-8). 160 B. */
+typedef struct hvx_camera_ubo {
+    float projection[16];
+    float view[16];
+    float position[4];
+    float camera_near;
+    float camera_far;
+    float _pad[2];
+} hvx_camera_ubo;
+
+/* PushConstPBR (HelloVulkan/Header/PushConstants.h:22-29). 20 B. */
+typedef struct hvx_pbr_pc {
+    float light_intensity;
+    float base_reflectivity;
+    float max_reflection_lod;
+    float light_falloff;
+    float albedo_multiplier;
+} hvx_pbr_pc;
+
+/* Struct-of-arrays fragments: the rasterised G-buffer inputs of
+ * ClusteredForward/Scene.frag:60-149, one fragment per pixel, row-major
+ * [height][width] (pixel (x, y) has gl_FragCoord.xy = (x + .5, y + .5)).
+ * World position is rebuilt from (pixel centre, depth) with inv_view_proj. */
+typedef struct hvx_frag_soa {
+    const float* depth;       /* gl_FragCoord.z, NDC depth in [0,1)          */
+    const float* normal[3];   /* world-space unit normal, planes x, y, z     */
+    const float* albedo[3];   /* linear albedo, planes r, g, b               */
+    const float* roughness;   /* [0,1]                                       */
+    const uint8_t* metallic;  /* UNORM8 (value / 255)                        */
+    uint32_t width;
+    uint32_t height;
+    float inv_view_proj[16];  /* column-major inverse(P * V)                 */
+} hvx_frag_soa;
+
+/* The three IBL maps Scene.frag samples (bindings 7, 8, 9). Cube maps are
+ * RGBA32F, mip-major, then face (+X,-X,+Y,-Y,+Z,-Z), then row, then column;
+ * level m has side max(size >> m, 1). The LUT is RG32F [height][width]. */
+typedef struct hvx_ibl_maps {
+    const float* specular;
+    uint32_t specular_size;
+    uint32_t specular_mips;
+    const float* diffuse;
+    uint32_t diffuse_size;
+    uint32_t _pad0;
+    const float* brdf_lut;
+    uint32_t lut_width;
+    uint32_t lut_height;
+} hvx_ibl_maps;
+
+/* Cube-filter type (PipelineCubeFilter.h CubeFilterType). */
+typedef enum hvx_filter_type {
+    HVX_FILTER_DIFFUSE = 0,
+    HVX_FILTER_SPECULAR = 1
+} hvx_filter_type;
+
+/* Cube-filter options. All-zero = the reference's behaviour as it renders
+ * its intended mapping (every output mip covers its whole face). */
+typedef struct hvx_filter_opts {
+    /* 1: reproduce the reference's static base-size viewport for specular
+     * mips >= 1 (PipelineCubeFilter.cpp:189-192, :303): mip i then only
+     * covers uv in [-1, -1 + 2^(1-i)] of each face. */
+    uint32_t emulate_static_viewport;
+    uint32_t _reserved[3];
+} hvx_filter_opts;
+
+typedef struct hvx_ctx hvx_ctx;
+
+/* ------------------------------------------------------------------ context */
+
+/* Creates a context bound to HIP device `device` (must be gfx950). Replaces
+ * the VulkanContext the reference's stages are constructed with
+ * (PipelineBase.h:27, VulkanContext.cpp:11-44). */
+int hvx_ctx_create(int device, hvx_ctx** out);
+int hvx_ctx_destroy(hvx_ctx* ctx);
+const char* hvx_status_str(int status);
+/* Text of the last error raised on this context ("" if none). */
+const char* hvx_ctx_last_error(const hvx_ctx* ctx);
+int hvx_abi_version(void);
+
+/* ------------------------------------------------------- clustered forward */
+
+/* Cluster AABBs for every froxel of the X*Y*Z grid of `ubo`.
+ * Replaces PipelineAABBGenerator::FillCommandBuffer
+ * (HelloVulkan/Source/Pipelines/PipelineAABBGenerator.cpp:36-83) running
+ * Shaders/ClusteredForward/AABBGenerator.comp:39-89.
+ * d_clusters: X*Y*Z entries, index x + y*X + z*X*Y. */
+int hvx_cf_build_aabbs(hvx_ctx* ctx, const hvx_cf_ubo* ubo,
+                       hvx_aabb* d_clusters, void* stream);
+
+/* Per-cluster light lists: for each cluster, the first `cap` lights in
+ * ascending index whose sphere intersects the cluster AABB (strict
+ * d^2 < r^2). Replaces PipelineLightCulling::ResetGlobalIndex +
+ * FillCommandBuffer (PipelineLightCulling.cpp:38-119) running
+ * Shaders/ClusteredForward/LightCulling.comp:59-87 (cap there is 150).
+ * The reference reserves list ranges with a global atomicAdd; here offsets
+ * are the exclusive prefix sum of counts in cluster order (deterministic).
+ *   d_indices:           indices_capacity entries; must be >= X*Y*Z*cap
+ *                        (the size ResourcesClusterForward.cpp:10-13
+ *                        allocates), else HVX_ERR_CAPACITY before launch
+ *   d_total:             optional (NULL ok): receives sum(counts), the
+ *                        value the reference leaves in globalIndex
+ *   d_overflow_count:    optional (NULL ok): receives the number of
+ *                        clusters whose list was truncated at `cap`.
+ * 1 <= cap <= 1024. */
+int hvx_cf_cull_lights(hvx_ctx* ctx, const hvx_cf_ubo* ubo,
+                       const hvx_aabb* d_clusters,
+                       const hvx_light* d_lights, uint32_t n_lights,
+                       uint32_t cap,
+                       hvx_light_cell* d_cells,
+                       uint32_t* d_indices, size_t indices_capacity,
+                       uint32_t* d_total, uint32_t* d_overflow_count,
+                       void* stream);
+
+/* Clustered Cook-Torrance + split-sum IBL shading of SoA fragments.
+ * Replaces PipelinePBRClusterForward::FillCommandBuffer
+ * (PipelinePBRClusterForward.cpp:77-113) running
+ * Shaders/ClusteredForward/Scene.frag:60-149 with Radiance.glsl:2-56 and
+ * PBR/Ambient.glsl:1-29. Output is FP32 RGB, planar [3][height*width]
+ * (the reference writes vec4 fragColor to an 8-bit attachment).
+ * Fragments whose depth maps to slice Z (depth == far) are clamped to Z-1. */
+int hvx_cf_shade(hvx_ctx* ctx, const hvx_frag_soa* frags,
+                 const hvx_camera_ubo* camera, const hvx_cf_ubo* ubo,
+                 const hvx_pbr_pc* pc,
+                 const hvx_light* d_lights, uint32_t n_lights,
+                 const hvx_light_cell* d_cells, const uint32_t* d_indices,
+                 const hvx_ibl_maps* ibl, float* d_rgb, void* stream);
+
+/* --------------------------------------------------------------------- IBL */
+
+/* Split-sum BRDF LUT, RG32F [height][width]. Replaces
+ * PipelineBRDFLUT::CreateLUT (PipelineBRDFLUT.cpp:32-108) running
+ * Shaders/IBL/BRDFLUT.comp:26-71. */
+int hvx_ibl_brdf_lut(hvx_ctx* ctx, uint32_t width, uint32_t height,
+                     uint32_t sample_count, float* d_lut_rg, void* stream);
+
+/* Equirectangular RGBA32F [h][w] (row 0 = texture t = 0) to cube level 0
+ * (6 faces of face_size^2, RGBA32F). Replaces
+ * PipelineEquirect2Cube::OffscreenRender (PipelineEquirect2Cube.cpp:232-285)
+ * running Shaders/IBL/Equirect2Cube.frag:96-107. */
+int hvx_ibl_equirect_to_cube(hvx_ctx* ctx, const float* d_equirect_rgba,
+                             uint32_t w, uint32_t h, float* d_cube_rgba,
+                             uint32_t face_size, void* stream);
+
+/* Fills levels 1..levels-1 of a cube chain from level 0 with 2x2 box means
+ * (the 2:1 LINEAR blit chain of VulkanImage::GenerateMipmap,
+ * VulkanImage.cpp:667-786). face_size must be a power of two. */
+int hvx_ibl_cube_mips(hvx_ctx* ctx, float* d_cube_chain, uint32_t face_size,
+                      uint32_t levels, void* stream);
+
+/* Diffuse irradiance (type 0: out_mips must be 1, sample_count ignored —
+ * the reference walks a fixed 0.025 rad grid) or GGX specular prefilter
+ * (type 1: mip i uses roughness i/(out_mips-1), sample_count Hammersley
+ * samples). Reads level 0 of d_cube only, bilinear with seamless edges, as
+ * the reference's maxLod = 0 sampler does. Replaces
+ * PipelineCubeFilter::OffscreenRender (PipelineCubeFilter.cpp:246-358)
+ * running Shaders/IBL/CubeFilterDiffuse.frag:35-136 and
+ * CubeFilterSpecular.frag:34-146. opts may be NULL. */
+int hvx_ibl_filter(hvx_ctx* ctx, int type, const float* d_cube,
+                   uint32_t face_size, float* d_out_chain,
+                   uint32_t out_face_size, uint32_t out_mips,
+                   uint32_t sample_count, const hvx_filter_opts* opts,
+                   void* stream);
+
+/* ------------------------------------------------------------------ camera */
+
+/* Camera description: position/target, vertical fov (radians), clip planes,
+ * viewport and cluster grid. Mirrors Camera::SetPositionAndTarget +
+ * UpdateInternal (HelloVulkan/Source/Camera.cpp:78-139) with worldUp =
+ * (0,1,0), GLM_FORCE_DEPTH_ZERO_TO_ONE and the Vulkan Y flip (P[1][1] *= -1),
+ * plus GetCameraUBO / GetClusterForwardUBO (:160-197). */
+typedef struct hvx_camera_desc {
+    float eye[3];
+    float target[3];
+    float fovy;
+    float znear;
+    float zfar;
+    uint32_t width;
+    uint32_t height;
+    uint32_t slice_count_x;
+    uint32_t slice_count_y;
+    uint32_t slice_count_z;
+} hvx_camera_desc;
+
+/* Fills the CameraUBO, the ClusterForwardUBO and inverse(P * V). Matrices are
+ * formed in double precision and rounded once to float (the reference forms
+ * them in float with glm; they are inputs of the path, shared by every
+ * stage). Any output pointer may be NULL. */
+int hvx_camera_build(const hvx_camera_desc* desc, hvx_camera_ubo* camera,
+                     hvx_cf_ubo* cf, float inv_view_proj[16]);
+
+/* Size helpers (float counts) for cube chains. */
+size_t hvx_cube_chain_floats(uint32_t face_size, uint32_t mips);
+size_t hvx_cube_level_offset_floats(uint32_t face_size, uint32_t level);
+/* Utility::MipMapCount (HelloVulkan/Header/Utility.h:42-50). */
+uint32_t hvx_mip_count(uint32_t size);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HVX_H_ */

@@ -1,0 +1,829 @@
+// hvx_oracle.cpp — CPU oracle of the clustered-forward + IBL compute path.
+//
+// TEST INFRASTRUCTURE ONLY. Nothing in the product links, loads or calls this
+// file; it is used by tests/, by __graft_entry__.smoke() and by bench.py's
+// CPU-baseline leg, and only as the checker.
+//
+// A scalar, single-invocation-at-a-time restatement of the reference's GLSL
+// (HelloVulkan/Shaders/...), templated on the arithmetic type:
+//   T = float   the parity oracle (built -O2 -ffp-contract=off, no fast-math;
+//               the GLSL transcendentals use hvx_math.h so the GPU kernels can
+//               reproduce them to the bit)
+//   T = double  the precision judge (same algorithm, libm transcendentals)
+// Vector expressions are evaluated in one documented order, left to right as
+// GLSL writes them: dot(a,b) = (ax*bx + ay*by) + az*bz; mat*vec row i =
+// ((m0i*x + m1i*y) + m2i*z) + m3i*w; normalize(v) = v * (1/sqrt(dot(v,v))).
+// Texture fetches follow the Vulkan specification's texel filtering rules
+// (bilinear LINEAR filter, REPEAT wrap, cube face selection, seamless cube
+// edges; a cube corner texel is the mean of its three neighbours).
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "../include/hvx.h"
+#include "../hellovulkan_amd/csrc/hvx_math.h"
+
+namespace {
+
+// ---------------------------------------------------------------- math policy
+template <class T> struct Mth;
+template <> struct Mth<float> {
+    static float sqrt(float x) { return hvx::sqrtf_(x); }
+    static float sin(float x) { return hvx::sinf_(x); }
+    static float cos(float x) { return hvx::cosf_(x); }
+    static float atan2(float y, float x) { return hvx::atan2f_(y, x); }
+    static float acos(float x) { return hvx::acosf_(x); }
+    static float log2(float x) { return hvx::log2f_(x); }
+    static float pow(float x, float y) { return hvx::powf_(x, y); }
+    static float pow5(float x) { return hvx::pow5f(x); }
+    static float floor(float x) { return hvx::floorf_(x); }
+    static constexpr float pi() { return hvx::kPiF; }
+};
+template <> struct Mth<double> {
+    static double sqrt(double x) { return std::sqrt(x); }
+    static double sin(double x) { return std::sin(x); }
+    static double cos(double x) { return std::cos(x); }
+    static double atan2(double y, double x) { return std::atan2(y, x); }
+    static double acos(double x) { return std::acos(x); }
+    static double log2(double x) { return std::log2(x); }
+    static double pow(double x, double y) { return std::pow(x, y); }
+    static double pow5(double x) { return std::pow(x, 5.0); }
+    static double floor(double x) { return std::floor(x); }
+    static constexpr double pi() { return 3.141592653589793; }
+};
+
+template <class T> T gmax(T a, T b) { return a < b ? b : a; }  // GLSL max
+template <class T> T gmin(T a, T b) { return b < a ? b : a; }  // GLSL min
+template <class T> T gclamp(T x, T lo, T hi) { return gmin(gmax(x, lo), hi); }
+
+template <class T> struct V3 {
+    T x, y, z;
+};
+template <class T> V3<T> add(V3<T> a, V3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T> V3<T> sub(V3<T> a, V3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T> V3<T> mul(V3<T> a, T s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class T> V3<T> neg(V3<T> a) { return {-a.x, -a.y, -a.z}; }
+template <class T> T dot(V3<T> a, V3<T> b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+template <class T> V3<T> cross(V3<T> a, V3<T> b) {
+    return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+template <class T> T length(V3<T> a) { return Mth<T>::sqrt(dot(a, a)); }
+template <class T> V3<T> normalize(V3<T> a) {
+    const T inv = T(1) / Mth<T>::sqrt(dot(a, a));
+    return mul(a, inv);
+}
+
+// column-major 4x4 (glm): element (col c, row r) at m[c*4 + r]
+template <class T> void matvec(const float* m, const T v[4], T out[4]) {
+    for (int r = 0; r < 4; ++r) {
+        out[r] = ((T(m[0 * 4 + r]) * v[0] + T(m[1 * 4 + r]) * v[1]) + T(m[2 * 4 + r]) * v[2]) +
+                 T(m[3 * 4 + r]) * v[3];
+    }
+}
+
+template <class F> void parallel_for(uint32_t n, int threads, F&& fn) {
+    if (threads <= 1 || n < 64) {
+        for (uint32_t i = 0; i < n; ++i) fn(i);
+        return;
+    }
+    std::atomic<uint32_t> next{0};
+    const uint32_t chunk = std::max<uint32_t>(1, n / (uint32_t(threads) * 16));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t) {
+        pool.emplace_back([&]() {
+            for (;;) {
+                const uint32_t b = next.fetch_add(chunk);
+                if (b >= n) break;
+                const uint32_t e = std::min(n, b + chunk);
+                for (uint32_t i = b; i < e; ++i) fn(i);
+            }
+        });
+    }
+    for (auto& th : pool) th.join();
+}
+
+// ================================================================ sampling
+// 2D bilinear, REPEAT wrap (VK_SAMPLER_ADDRESS_MODE_REPEAT), RG or RGBA
+// texels. u = s*W - 1/2, i0 = floor(u), alpha = u - i0 (Vulkan texel
+// filtering); weights w00=(1-a)(1-b), w10=a(1-b), w01=(1-a)b, w11=ab,
+// summed as ((w00*t00 + w10*t10) + w01*t01) + w11*t11.
+template <class T> int wrap_repeat(int i, int n) {
+    int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+template <class T>
+void bilinear2d_repeat(const float* tex, int W, int H, int C, T s, T t, int nc, T* out) {
+    const T u = s * T(W) - T(0.5);
+    const T v = t * T(H) - T(0.5);
+    const T fu = Mth<T>::floor(u), fv = Mth<T>::floor(v);
+    const T a = u - fu, b = v - fv;
+    const int i0 = wrap_repeat<T>((int)fu, W), i1 = wrap_repeat<T>((int)fu + 1, W);
+    const int j0 = wrap_repeat<T>((int)fv, H), j1 = wrap_repeat<T>((int)fv + 1, H);
+    const T w00 = (T(1) - a) * (T(1) - b), w10 = a * (T(1) - b);
+    const T w01 = (T(1) - a) * b, w11 = a * b;
+    const float* t00 = tex + ((size_t)j0 * W + i0) * C;
+    const float* t10 = tex + ((size_t)j0 * W + i1) * C;
+    const float* t01 = tex + ((size_t)j1 * W + i0) * C;
+    const float* t11 = tex + ((size_t)j1 * W + i1) * C;
+    for (int c = 0; c < nc; ++c) {
+        out[c] = ((w00 * T(t00[c]) + w10 * T(t10[c])) + w01 * T(t01[c])) + w11 * T(t11[c]);
+    }
+}
+
+// Cube face selection (Vulkan "Cube Map Face Selection" table). Ties go to
+// X over Y over Z (>= comparisons). Returns face and (s, t) in [0,1].
+template <class T> int cube_face_st(V3<T> r, T* s, T* t) {
+    const T ax = r.x < T(0) ? -r.x : r.x;
+    const T ay = r.y < T(0) ? -r.y : r.y;
+    const T az = r.z < T(0) ? -r.z : r.z;
+    int face;
+    T sc, tc, ma;
+    if (ax >= ay && ax >= az) {
+        ma = ax;
+        if (r.x >= T(0)) { face = 0; sc = -r.z; tc = -r.y; }
+        else { face = 1; sc = r.z; tc = -r.y; }
+    } else if (ay >= az) {
+        ma = ay;
+        if (r.y >= T(0)) { face = 2; sc = r.x; tc = r.z; }
+        else { face = 3; sc = r.x; tc = -r.z; }
+    } else {
+        ma = az;
+        if (r.z >= T(0)) { face = 4; sc = r.x; tc = -r.y; }
+        else { face = 5; sc = -r.x; tc = -r.y; }
+    }
+    *s = T(0.5) * (sc / ma + T(1));
+    *t = T(0.5) * (tc / ma + T(1));
+    return face;
+}
+
+// Seamless edge: a texel one step outside face f is the texel across the
+// edge. Computed geometrically in doubled integer units: the texel centre
+// (a, b) = (2i+1-S, 2j+1-S) lies on the face plane at distance S; folding the
+// plane over the cube edge maps the out-of-range coordinate c (|c| = S+1) to
+// the neighbouring face's plane (c -> sign(c) S) and the former major
+// coordinate to sign * (2S - |c|) = sign * (S - 1).
+struct Texel {
+    int face, i, j;
+};
+
+inline void face_point(int f, int a, int b, int S, int P[3]) {
+    switch (f) {
+        case 0: P[0] = S; P[1] = -b; P[2] = -a; break;
+        case 1: P[0] = -S; P[1] = -b; P[2] = a; break;
+        case 2: P[0] = a; P[1] = S; P[2] = b; break;
+        case 3: P[0] = a; P[1] = -S; P[2] = -b; break;
+        case 4: P[0] = a; P[1] = -b; P[2] = S; break;
+        default: P[0] = -a; P[1] = -b; P[2] = -S; break;
+    }
+}
+
+inline Texel point_texel(const int P[3], int S) {
+    int ax = std::abs(P[0]), ay = std::abs(P[1]), az = std::abs(P[2]);
+    int f, a, b;
+    if (ax >= ay && ax >= az) {
+        if (P[0] > 0) { f = 0; a = -P[2]; b = -P[1]; }
+        else { f = 1; a = P[2]; b = -P[1]; }
+    } else if (ay >= az) {
+        if (P[1] > 0) { f = 2; a = P[0]; b = P[2]; }
+        else { f = 3; a = P[0]; b = -P[2]; }
+    } else {
+        if (P[2] > 0) { f = 4; a = P[0]; b = -P[1]; }
+        else { f = 5; a = -P[0]; b = -P[1]; }
+    }
+    return {f, (a + S - 1) / 2, (b + S - 1) / 2};
+}
+
+inline Texel cube_edge_texel(int f, int i, int j, int S) {
+    const int a = 2 * i + 1 - S, b = 2 * j + 1 - S;
+    int P[3];
+    face_point(f, a, b, S, P);
+    // the major coordinate is the one equal to +-S at index m
+    int m = (f < 2) ? 0 : (f < 4 ? 1 : 2);
+    for (int k = 0; k < 3; ++k) {
+        if (k == m) continue;
+        if (std::abs(P[k]) > S) {
+            const int sk = P[k] > 0 ? 1 : -1, sm = P[m] > 0 ? 1 : -1;
+            const int over = std::abs(P[k]);
+            P[k] = sk * S;
+            P[m] = sm * (2 * S - over);
+            break;
+        }
+    }
+    return point_texel(P, S);
+}
+
+template <class T>
+void fetch_cube_texel(const float* level, int S, int f, int i, int j, T out[3]) {
+    const bool iin = i >= 0 && i < S, jin = j >= 0 && j < S;
+    auto at = [&](int ff, int ii, int jj) { return level + (((size_t)ff * S + jj) * S + ii) * 4; };
+    if (iin && jin) {
+        const float* p = at(f, i, j);
+        for (int c = 0; c < 3; ++c) out[c] = T(p[c]);
+        return;
+    }
+    if (iin || jin) {
+        const Texel t = cube_edge_texel(f, i, j, S);
+        const float* p = at(t.face, t.i, t.j);
+        for (int c = 0; c < 3; ++c) out[c] = T(p[c]);
+        return;
+    }
+    // corner: mean of the in-face corner texel and its two across-edge neighbours
+    const int ci = i < 0 ? 0 : S - 1, cj = j < 0 ? 0 : S - 1;
+    const Texel ta = cube_edge_texel(f, i, cj, S);
+    const Texel tb = cube_edge_texel(f, ci, j, S);
+    const float* p0 = at(f, ci, cj);
+    const float* pa = at(ta.face, ta.i, ta.j);
+    const float* pb = at(tb.face, tb.i, tb.j);
+    for (int c = 0; c < 3; ++c) out[c] = ((T(p0[c]) + T(pa[c])) + T(pb[c])) / T(3);
+}
+
+// Bilinear seamless sample of one cube level (RGB).
+template <class T> void sample_cube(const float* level, int S, V3<T> dir, T out[3]) {
+    T s, t;
+    const int f = cube_face_st(dir, &s, &t);
+    const T u = s * T(S) - T(0.5);
+    const T v = t * T(S) - T(0.5);
+    const T fu = Mth<T>::floor(u), fv = Mth<T>::floor(v);
+    const T a = u - fu, b = v - fv;
+    const int i0 = (int)fu, j0 = (int)fv;
+    T t00[3], t10[3], t01[3], t11[3];
+    fetch_cube_texel<T>(level, S, f, i0, j0, t00);
+    fetch_cube_texel<T>(level, S, f, i0 + 1, j0, t10);
+    fetch_cube_texel<T>(level, S, f, i0, j0 + 1, t01);
+    fetch_cube_texel<T>(level, S, f, i0 + 1, j0 + 1, t11);
+    const T w00 = (T(1) - a) * (T(1) - b), w10 = a * (T(1) - b);
+    const T w01 = (T(1) - a) * b, w11 = a * b;
+    for (int c = 0; c < 3; ++c) {
+        out[c] = ((w00 * t00[c] + w10 * t10[c]) + w01 * t01[c]) + w11 * t11[c];
+    }
+}
+
+size_t chain_offset(uint32_t S, uint32_t level) {
+    size_t off = 0;
+    for (uint32_t m = 0; m < level; ++m) {
+        const size_t s = std::max<uint32_t>(S >> m, 1u);
+        off += 6 * s * s * 4;
+    }
+    return off;
+}
+
+// textureLod on a cube chain with LINEAR mipmap mode: lambda clamped to
+// [0, q], q = mips-1; levels floor(lambda) and min(floor+1, q) blended by
+// frac(lambda) as (1-d)*lo + d*hi.
+template <class T>
+void sample_cube_lod(const float* chain, uint32_t S, uint32_t mips, V3<T> dir, T lod, T out[3]) {
+    const T q = T(mips - 1);
+    const T lam = gclamp(lod, T(0), q);
+    const T fl = Mth<T>::floor(lam);
+    const uint32_t l0 = (uint32_t)fl;
+    const uint32_t l1 = std::min(l0 + 1, mips - 1);
+    const T d = lam - fl;
+    T a[3], b[3];
+    const int s0 = (int)std::max<uint32_t>(S >> l0, 1u), s1 = (int)std::max<uint32_t>(S >> l1, 1u);
+    sample_cube<T>(chain + chain_offset(S, l0), s0, dir, a);
+    sample_cube<T>(chain + chain_offset(S, l1), s1, dir, b);
+    for (int c = 0; c < 3; ++c) out[c] = (T(1) - d) * a[c] + d * b[c];
+}
+
+// IBL/*.frag UVToXYZ (e.g. Equirect2Cube.frag:58-84)
+template <class T> V3<T> uv_to_xyz(int face, T u, T v) {
+    switch (face) {
+        case 0: return {T(1), v, -u};
+        case 1: return {T(-1), v, u};
+        case 2: return {u, T(-1), v};
+        case 3: return {u, T(1), -v};
+        case 4: return {u, v, T(1)};
+        default: return {-u, v, T(-1)};
+    }
+}
+
+// Output-texel direction of the IBL fragment shaders: texCoord = (x+.5)/D
+// (FullscreenTriangle.vert:10-17 interpolated at the pixel centre, D = the
+// viewport side), texCoordNew = texCoord*2 - 1, normalize(UVToXYZ), y = -y.
+template <class T> V3<T> texel_dir(int face, uint32_t x, uint32_t y, uint32_t D) {
+    const T u = ((T(x) + T(0.5)) / T(D)) * T(2) - T(1);
+    const T v = ((T(y) + T(0.5)) / T(D)) * T(2) - T(1);
+    V3<T> d = normalize(uv_to_xyz<T>(face, u, v));
+    d.y = -d.y;
+    return d;
+}
+
+// =================================================== clustered forward
+// AABBGenerator.comp:18-89
+template <class T> void screen_to_view(const hvx_cf_ubo& u, T sx, T sy, T out[3]) {
+    const T clip[4] = {sx * T(2) - T(1), sy * T(2) - T(1), T(0), T(1)};
+    T view[4];
+    matvec<T>(u.camera_inverse_projection, clip, view);
+    out[0] = view[0] / view[3];
+    out[1] = view[1] / view[3];
+    out[2] = view[2] / view[3];
+}
+
+template <class T> void build_aabbs(const hvx_cf_ubo& u, T* out) {
+    const uint32_t X = u.slice_count_x, Y = u.slice_count_y, Z = u.slice_count_z;
+    const T npx = T(1) / T(X), npy = T(1) / T(Y);
+    const T nearv = T(u.camera_near), farv = T(u.camera_far);
+    for (uint32_t z = 0; z < Z; ++z)
+        for (uint32_t y = 0; y < Y; ++y)
+            for (uint32_t x = 0; x < X; ++x) {
+                T maxVS[3], minVS[3];
+                screen_to_view<T>(u, (T(x) + T(1)) * npx, (T(y) + T(1)) * npy, maxVS);
+                screen_to_view<T>(u, T(x) * npx, T(y) * npy, minVS);
+                const T ratio = farv / nearv;
+                const T tileNear = (-nearv) * Mth<T>::pow(ratio, T(z) / T(Z));
+                const T tileFar = (-nearv) * Mth<T>::pow(ratio, T(z + 1) / T(Z));
+                T pts[4][3];
+                for (int c = 0; c < 3; ++c) {
+                    pts[0][c] = (minVS[c] * tileNear) / minVS[2];
+                    pts[1][c] = (minVS[c] * tileFar) / minVS[2];
+                    pts[2][c] = (maxVS[c] * tileNear) / maxVS[2];
+                    pts[3][c] = (maxVS[c] * tileFar) / maxVS[2];
+                }
+                T* o = out + (size_t)(x + y * X + z * X * Y) * 8;
+                for (int c = 0; c < 3; ++c) {
+                    o[c] = gmin(gmin(pts[0][c], pts[1][c]), gmin(pts[2][c], pts[3][c]));
+                    o[4 + c] = gmax(gmax(pts[0][c], pts[1][c]), gmax(pts[2][c], pts[3][c]));
+                }
+                o[3] = T(0);
+                o[7] = T(0);
+            }
+}
+
+// LightCulling.comp:26-57 (FP32 only: the lists are defined by it)
+inline bool test_sphere_aabb(const float pv[3], float r, const float* box) {
+    float sq = 0.0f;
+    for (int a = 0; a < 3; ++a) {
+        const float v = pv[a];
+        if (v < box[a]) {
+            const float d = box[a] - v;
+            sq += d * d;
+        } else if (v > box[4 + a]) {
+            const float d = box[4 + a] - v;
+            sq += d * d;
+        }
+    }
+    return sq < r * r;
+}
+
+// =================================================== shading (Scene.frag)
+struct MapsHost {
+    const float* spec;
+    uint32_t spec_size, spec_mips;
+    const float* diff;
+    uint32_t diff_size;
+    const float* lut;
+    uint32_t lut_w, lut_h;
+};
+
+template <class T>
+void shade_fragment(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const hvx_cf_ubo& u,
+                    const hvx_pbr_pc& pc, const hvx_light* lights, const hvx_light_cell* cells,
+                    const uint32_t* indices, const MapsHost& maps, uint32_t idx, T out[3]) {
+    using M = Mth<T>;
+    const T PI = M::pi();
+    const uint32_t W = fr.width, H = fr.height;
+    const uint32_t px = idx % W, py = idx / W;
+    const T fx = T(px) + T(0.5), fy = T(py) + T(0.5);
+    const T z = T(fr.depth[idx]);
+    const T nearv = T(u.camera_near), farv = T(u.camera_far);
+    // LinearDepth (Scene.frag:55-58) and the cluster lookup (:64-78)
+    const T lin = (nearv * farv) / (farv + z * (nearv - farv));
+    const T zf = gmax(M::log2(lin) * T(u.slice_scaling) + T(u.slice_bias), T(0));
+    uint32_t zi = (uint32_t)zf;
+    if (zi >= u.slice_count_z) zi = u.slice_count_z - 1;  // depth == far clamps to Z-1
+    const T tsx = T(u.screen_size[0]) / T(u.slice_count_x);
+    const T tsy = T(u.screen_size[1]) / T(u.slice_count_y);
+    const uint32_t cx = std::min((uint32_t)(fx / tsx), u.slice_count_x - 1);
+    const uint32_t cy = std::min((uint32_t)(fy / tsy), u.slice_count_y - 1);
+    const uint32_t cidx = cx + cy * u.slice_count_x + zi * u.slice_count_x * u.slice_count_y;
+    const uint32_t count = cells[cidx].count, offset = cells[cidx].offset;
+
+    // world position from (pixel centre, depth)
+    const T clip[4] = {(fx / T(W)) * T(2) - T(1), (fy / T(H)) * T(2) - T(1), z, T(1)};
+    T wh[4];
+    matvec<T>(fr.inv_view_proj, clip, wh);
+    const V3<T> wp = {wh[0] / wh[3], wh[1] / wh[3], wh[2] / wh[3]};
+
+    const V3<T> albedo = {T(fr.albedo[0][idx]), T(fr.albedo[1][idx]), T(fr.albedo[2][idx])};
+    const T metallic = T(fr.metallic[idx]) / T(255);
+    const T rough = T(fr.roughness[idx]);
+    const T rr = rough + T(1);
+    const T alphaRough = (rr * rr) / T(8);  // AlphaDirectLighting
+    const V3<T> N = {T(fr.normal[0][idx]), T(fr.normal[1][idx]), T(fr.normal[2][idx])};
+    const V3<T> camp = {T(cam.position[0]), T(cam.position[1]), T(cam.position[2])};
+    const V3<T> V = normalize(sub(camp, wp));
+    const T NoV = gmax(dot(N, V), T(0));
+    // F0 = mix(vec3(base), albedo, metallic) = x*(1-a) + y*a
+    const T base = T(pc.base_reflectivity);
+    const T F0[3] = {base * (T(1) - metallic) + albedo.x * metallic,
+                     base * (T(1) - metallic) + albedo.y * metallic,
+                     base * (T(1) - metallic) + albedo.z * metallic};
+    const T alb[3] = {albedo.x, albedo.y, albedo.z};
+    T Lo[3] = {alb[0] * T(pc.albedo_multiplier), alb[1] * T(pc.albedo_multiplier),
+               alb[2] * T(pc.albedo_multiplier)};
+
+    // Radiance (ClusteredForward/Radiance.glsl:2-56) per listed light
+    const T alpha = rough * rough;  // DistributionGGX
+    const T alpha2 = alpha * alpha;
+    for (uint32_t k = 0; k < count; ++k) {
+        const hvx_light& l = lights[indices[offset + k]];
+        const V3<T> lp = {T(l.position[0]), T(l.position[1]), T(l.position[2])};
+        const V3<T> d = sub(lp, wp);
+        const V3<T> L = normalize(d);
+        const V3<T> Hh = normalize(add(V, L));
+        const T NoH = gmax(dot(N, Hh), T(0));
+        const T NoL = gmax(dot(N, L), T(0));
+        const T HoV = gmax(dot(Hh, V), T(0));
+        const T dist = length(d);
+        const T att = gmax(T(1) - dist / T(l.radius), T(0)) / M::pow(dist, T(pc.light_falloff));
+        const T rad[3] = {(T(l.color[0]) * att) * T(pc.light_intensity),
+                          (T(l.color[1]) * att) * T(pc.light_intensity),
+                          (T(l.color[2]) * att) * T(pc.light_intensity)};
+        const T NoH2 = NoH * NoH;
+        T den = NoH2 * (alpha2 - T(1)) + T(1);
+        den = (PI * den) * den;
+        const T D = alpha2 / den;
+        const T GL = NoL / (NoL * (T(1) - alphaRough) + alphaRough);
+        const T GV = NoV / (NoV * (T(1) - alphaRough) + alphaRough);
+        const T G = GL * GV;
+        const T p5 = M::pow5(gclamp(T(1) - HoV, T(0), T(1)));
+        const T denom = ((T(4) * NoV) * NoL) + T(0.0001);
+        const T DG = D * G;
+        for (int c = 0; c < 3; ++c) {
+            const T F = F0[c] + (T(1) - F0[c]) * p5;
+            const T spec = (DG * F) / denom;
+            const T kD = (T(1) - F) * (T(1) - metallic);
+            const T diffuse = (kD * alb[c]) / PI;
+            Lo[c] = Lo[c] + ((diffuse + spec) * rad[c]) * NoL;
+        }
+    }
+
+    // Ambient (PBR/Ambient.glsl:1-29)
+    const T p5v = M::pow5(gclamp(T(1) - NoV, T(0), T(1)));
+    T irr[3], pre[3], brdf[2];
+    sample_cube<T>(maps.diff, (int)maps.diff_size, N, irr);
+    const V3<T> I = neg(V);
+    const T dNI = dot(N, I);
+    const V3<T> R = sub(I, mul(N, T(2) * dNI));  // reflect(I, N) = I - 2*dot(N,I)*N
+    sample_cube_lod<T>(maps.spec, maps.spec_size, maps.spec_mips, R, rough * T(pc.max_reflection_lod), pre);
+    bilinear2d_repeat<T>(maps.lut, (int)maps.lut_w, (int)maps.lut_h, 2, NoV, rough, 2, brdf);
+    for (int c = 0; c < 3; ++c) {
+        const T F = F0[c] + (gmax(T(1) - rough, F0[c]) - F0[c]) * p5v;
+        const T kD = (T(1) - F) * (T(1) - metallic);
+        const T diffuse = irr[c] * alb[c];
+        const T specular = pre[c] * (F * brdf[0] + brdf[1]);
+        const T ambient = (kD * diffuse + specular) * T(1);  // ao = 1
+        out[c] = (ambient + T(0)) + Lo[c];                   // emissive = 0
+    }
+}
+
+// =================================================== IBL
+// IBL/Header.glsl:14-37
+template <class T> V3<T> importance_sample_ggx(T xix, T xiy, V3<T> N, T rough) {
+    using M = Mth<T>;
+    const T a = rough * rough;
+    const T phi = (T(2) * M::pi()) * xix;
+    const T cosTheta = M::sqrt((T(1) - xiy) / (T(1) + (a * a - T(1)) * xiy));
+    const T sinTheta = M::sqrt(T(1) - cosTheta * cosTheta);
+    const T hx = M::cos(phi) * sinTheta, hy = M::sin(phi) * sinTheta, hz = cosTheta;
+    const T az = N.z < T(0) ? -N.z : N.z;
+    const V3<T> up = az < T(0.999) ? V3<T>{T(0), T(0), T(1)} : V3<T>{T(1), T(0), T(0)};
+    const V3<T> tangent = normalize(cross(up, N));
+    const V3<T> bitangent = cross(N, tangent);
+    const V3<T> sv = add(add(mul(tangent, hx), mul(bitangent, hy)), mul(N, hz));
+    return normalize(sv);
+}
+
+template <class T> void hammersley(uint32_t i, uint32_t n, T* x, T* y) {
+    // PBR/Hammersley.glsl: float(i)/float(N) and the radical inverse
+    if constexpr (sizeof(T) == 4) {
+        *x = float(i) / float(n);
+        *y = hvx::radical_inverse_vdc(i);
+    } else {
+        *x = double(i) / double(n);
+        *y = double(hvx::bitrev32(i)) * 2.3283064365386963e-10;
+    }
+}
+
+// BRDFLUT.comp:26-71
+template <class T> void brdf_texel(uint32_t W, uint32_t H, uint32_t n, uint32_t x, uint32_t y, T out[2]) {
+    const T NoV = (T(x) + T(0.5)) / T(W);
+    const T rough = T(1) - (T(y) + T(0.5)) / T(H);
+    const V3<T> N = {T(0), T(0), T(1)};
+    const V3<T> V = {Mth<T>::sqrt(T(1) - NoV * NoV), T(0), NoV};
+    T lx = T(0), ly = T(0);
+    for (uint32_t i = 0; i < n; ++i) {
+        T xi0, xi1;
+        hammersley<T>(i, n, &xi0, &xi1);
+        const V3<T> Hh = importance_sample_ggx<T>(xi0, xi1, N, rough);
+        const V3<T> L = normalize(sub(mul(Hh, T(2) * dot(V, Hh)), V));
+        const T dotNL = gmax(dot(N, L), T(0));
+        if (dotNL > T(0)) {
+            const T dotNV = gmax(dot(N, V), T(0));
+            const T dotVH = gmax(dot(V, Hh), T(0));
+            const T dotNH = gmax(dot(Hh, N), T(0));
+            const T alpha = (rough * rough) / T(2);  // AlphaIBL
+            const T GL = dotNL / (dotNL * (T(1) - alpha) + alpha);
+            const T GV = dotNV / (dotNV * (T(1) - alpha) + alpha);
+            const T G = GL * GV;
+            const T G_Vis = (G * dotVH) / (dotNH * dotNV);
+            const T Fc = Mth<T>::pow5(T(1) - dotVH);
+            lx = lx + (T(1) - Fc) * G_Vis;
+            ly = ly + Fc * G_Vis;
+        }
+    }
+    out[0] = lx / T(n);
+    out[1] = ly / T(n);
+}
+
+// Equirect2Cube.frag:87-107
+template <class T>
+void equirect_texel(const float* eq, uint32_t w, uint32_t h, uint32_t S, uint32_t id, T out[4]) {
+    const uint32_t x = id % S, y = (id / S) % S, face = id / (S * S);
+    const T u = ((T(x) + T(0.5)) / T(S)) * T(2) - T(1);
+    const T v = ((T(y) + T(0.5)) / T(S)) * T(2) - T(1);
+    V3<T> d = normalize(uv_to_xyz<T>((int)face, u, v));
+    d.y = -d.y;
+    const T PI = Mth<T>::pi();
+    const T s = T(0.5) + (T(0.5) * Mth<T>::atan2(d.z, d.x)) / PI;
+    const T t = T(1) - Mth<T>::acos(d.y) / PI;
+    T rgb[3];
+    bilinear2d_repeat<T>(eq, (int)w, (int)h, 4, s, t, 3, rgb);
+    out[0] = rgb[0];
+    out[1] = rgb[1];
+    out[2] = rgb[2];
+    out[3] = T(1);
+}
+
+// CubeFilterDiffuse.frag:35-65, 126-136
+template <class T> void diffuse_texel(const float* cube, uint32_t S, uint32_t Sd, uint32_t id, T out[4]) {
+    using M = Mth<T>;
+    const uint32_t x = id % Sd, y = (id / Sd) % Sd, face = id / (Sd * Sd);
+    const V3<T> N = texel_dir<T>((int)face, x, y, Sd);
+    const T PI = M::pi();
+    const T TWO_PI = PI * T(2), HALF_PI = PI * T(0.5);
+    V3<T> up = {T(0), T(1), T(0)};
+    const V3<T> right = normalize(cross(up, N));
+    up = normalize(cross(N, right));
+    const T delta = T(0.025);
+    T acc[3] = {T(0), T(0), T(0)};
+    uint32_t count = 0;
+    for (T phi = T(0); phi < TWO_PI; phi = phi + delta) {
+        const T sp = M::sin(phi), cp = M::cos(phi);
+        for (T theta = T(0); theta < HALF_PI; theta = theta + delta) {
+            const T st = M::sin(theta), ct = M::cos(theta);
+            const T tx = st * cp, ty = st * sp, tz = ct;
+            const V3<T> sv = add(add(mul(right, tx), mul(up, ty)), mul(N, tz));
+            T c[3];
+            sample_cube<T>(cube, (int)S, sv, c);
+            for (int k = 0; k < 3; ++k) acc[k] = acc[k] + (c[k] * ct) * st;
+            ++count;
+        }
+    }
+    const T inv = T(1) / T(count);
+    for (int k = 0; k < 3; ++k) out[k] = (PI * acc[k]) * inv;
+    out[3] = T(1);
+}
+
+// CubeFilterSpecular.frag:34-75, 136-146
+template <class T>
+void specular_texel(const float* cube, uint32_t S, uint32_t Sm, uint32_t D, T rough, uint32_t n, uint32_t id,
+                    T out[4]) {
+    const uint32_t x = id % Sm, y = (id / Sm) % Sm, face = id / (Sm * Sm);
+    const V3<T> N = texel_dir<T>((int)face, x, y, D);
+    const V3<T> V = N;
+    T acc[3] = {T(0), T(0), T(0)};
+    T wsum = T(0);
+    for (uint32_t i = 0; i < n; ++i) {
+        T xi0, xi1;
+        hammersley<T>(i, n, &xi0, &xi1);
+        const V3<T> Hh = importance_sample_ggx<T>(xi0, xi1, N, rough);
+        const V3<T> L = normalize(sub(mul(Hh, T(2) * dot(V, Hh)), V));
+        const T NoL = gmax(dot(N, L), T(0));
+        if (NoL > T(0)) {
+            T c[3];
+            sample_cube<T>(cube, (int)S, L, c);
+            for (int k = 0; k < 3; ++k) acc[k] = acc[k] + c[k] * NoL;
+            wsum = wsum + NoL;
+        }
+    }
+    for (int k = 0; k < 3; ++k) out[k] = acc[k] / wsum;
+    out[3] = T(1);
+}
+
+template <class T> T specular_roughness(uint32_t mip, uint32_t mips) {
+    // PipelineCubeFilter.cpp:305-311 (float(i) / float(mips - 1))
+    if (mips <= 1) return T(0);
+    return T(float(mip) / float(mips - 1));
+}
+
+MapsHost maps_of(const hvx_ibl_maps* m) {
+    return {m->specular, m->specular_size, m->specular_mips, m->diffuse, m->diffuse_size,
+            m->brdf_lut, m->lut_width, m->lut_height};
+}
+
+}  // namespace
+
+// ======================================================================= C API
+extern "C" {
+
+// ---- math primitives (vectorised over n) for the accuracy tests
+void hvxo_math_f32(int fn, const float* a, const float* b, float* out, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) {
+        switch (fn) {
+            case 0: out[i] = hvx::log2f_(a[i]); break;
+            case 1: out[i] = hvx::exp2f_(a[i]); break;
+            case 2: out[i] = hvx::powf_(a[i], b[i]); break;
+            case 3: out[i] = hvx::sinf_(a[i]); break;
+            case 4: out[i] = hvx::cosf_(a[i]); break;
+            case 5: out[i] = hvx::atan2f_(a[i], b[i]); break;
+            case 6: out[i] = hvx::acosf_(a[i]); break;
+            case 7: out[i] = hvx::pow5f(a[i]); break;
+            case 8: out[i] = hvx::sqrtf_(a[i]); break;
+            default: out[i] = 0.0f;
+        }
+    }
+}
+void hvxo_math_f64(int fn, const double* a, const double* b, double* out, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) {
+        switch (fn) {
+            case 3: out[i] = hvx::sin_d(a[i]); break;
+            case 4: out[i] = hvx::cos_d(a[i]); break;
+            case 5: out[i] = hvx::atan2_d(a[i], b[i]); break;
+            case 6: out[i] = hvx::acos_d(a[i]); break;
+            case 9: out[i] = hvx::atan_d(a[i]); break;
+            default: out[i] = 0.0;
+        }
+    }
+}
+
+void hvxo_hammersley_f32(uint32_t n, float* out /* 2n */) {
+    for (uint32_t i = 0; i < n; ++i) hammersley<float>(i, n, &out[2 * i], &out[2 * i + 1]);
+}
+
+void hvxo_ggx_sample_f32(float xix, float xiy, const float* N, float rough, float* out) {
+    const V3<float> h = importance_sample_ggx<float>(xix, xiy, V3<float>{N[0], N[1], N[2]}, rough);
+    out[0] = h.x; out[1] = h.y; out[2] = h.z;
+}
+
+int hvxo_cube_face_st_f32(const float* dir, float* st) {
+    return cube_face_st<float>(V3<float>{dir[0], dir[1], dir[2]}, &st[0], &st[1]);
+}
+
+void hvxo_cube_edge_texel(int f, int i, int j, int S, int* out /* face,i,j */) {
+    const Texel t = cube_edge_texel(f, i, j, S);
+    out[0] = t.face; out[1] = t.i; out[2] = t.j;
+}
+
+void hvxo_sample_cube_f32(const float* level, uint32_t S, const float* dirs, uint32_t n, float* out) {
+    for (uint32_t k = 0; k < n; ++k) {
+        sample_cube<float>(level, (int)S, V3<float>{dirs[3 * k], dirs[3 * k + 1], dirs[3 * k + 2]}, out + 3 * k);
+    }
+}
+
+void hvxo_texel_dir_f32(int face, uint32_t x, uint32_t y, uint32_t D, float* out) {
+    const V3<float> d = texel_dir<float>(face, x, y, D);
+    out[0] = d.x; out[1] = d.y; out[2] = d.z;
+}
+
+// ---- a2: cluster AABBs (out: X*Y*Z*8)
+void hvxo_cf_build_aabbs_f32(const hvx_cf_ubo* u, float* out) { build_aabbs<float>(*u, out); }
+void hvxo_cf_build_aabbs_f64(const hvx_cf_ubo* u, double* out) { build_aabbs<double>(*u, out); }
+
+// ---- a3: light culling (LightCulling.comp:59-87), offsets = exclusive scan
+void hvxo_cf_cull_lights(const hvx_cf_ubo* u, const float* aabbs, const hvx_light* lights, uint32_t n,
+                         uint32_t cap, hvx_light_cell* cells, uint32_t* indices, uint32_t* total,
+                         uint32_t* overflow, int threads) {
+    const uint32_t C = u->slice_count_x * u->slice_count_y * u->slice_count_z;
+    // view-space light centres: posView = cameraView * posWorld
+    std::vector<float> pv((size_t)n * 3);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float w[4] = {lights[i].position[0], lights[i].position[1], lights[i].position[2],
+                            lights[i].position[3]};
+        float o[4];
+        matvec<float>(u->camera_view, w, o);
+        pv[3 * i] = o[0]; pv[3 * i + 1] = o[1]; pv[3 * i + 2] = o[2];
+    }
+    std::vector<uint32_t> lists((size_t)C * cap);
+    std::vector<uint32_t> counts(C), over(C);
+    parallel_for(C, threads, [&](uint32_t c) {
+        const float* box = aabbs + (size_t)c * 8;
+        uint32_t cnt = 0, i = 0;
+        for (; i < n && cnt < cap; ++i) {
+            if (test_sphere_aabb(&pv[3 * i], lights[i].radius, box)) lists[(size_t)c * cap + cnt++] = i;
+        }
+        uint32_t ov = 0;
+        if (cnt == cap) {
+            for (; i < n; ++i) {
+                if (test_sphere_aabb(&pv[3 * i], lights[i].radius, box)) { ov = 1; break; }
+            }
+        }
+        counts[c] = cnt;
+        over[c] = ov;
+    });
+    uint32_t off = 0, novf = 0;
+    for (uint32_t c = 0; c < C; ++c) {
+        cells[c].offset = off;
+        cells[c].count = counts[c];
+        std::memcpy(indices + off, &lists[(size_t)c * cap], counts[c] * sizeof(uint32_t));
+        off += counts[c];
+        novf += over[c];
+    }
+    if (total) *total = off;
+    if (overflow) *overflow = novf;
+}
+
+// ---- a5-a8: shading of selected fragments (out: 3 per id)
+void hvxo_cf_shade_f32(const hvx_frag_soa* fr, const hvx_camera_ubo* cam, const hvx_cf_ubo* u,
+                       const hvx_pbr_pc* pc, const hvx_light* lights, const hvx_light_cell* cells,
+                       const uint32_t* indices, const hvx_ibl_maps* maps, const uint32_t* ids, uint32_t n_ids,
+                       float* out, int threads) {
+    const MapsHost m = maps_of(maps);
+    parallel_for(n_ids, threads, [&](uint32_t k) {
+        shade_fragment<float>(*fr, *cam, *u, *pc, lights, cells, indices, m, ids[k], out + 3 * k);
+    });
+}
+void hvxo_cf_shade_f64(const hvx_frag_soa* fr, const hvx_camera_ubo* cam, const hvx_cf_ubo* u,
+                       const hvx_pbr_pc* pc, const hvx_light* lights, const hvx_light_cell* cells,
+                       const uint32_t* indices, const hvx_ibl_maps* maps, const uint32_t* ids, uint32_t n_ids,
+                       double* out, int threads) {
+    const MapsHost m = maps_of(maps);
+    parallel_for(n_ids, threads, [&](uint32_t k) {
+        shade_fragment<double>(*fr, *cam, *u, *pc, lights, cells, indices, m, ids[k], out + 3 * k);
+    });
+}
+
+// ---- a11: BRDF LUT texels (id = y*W + x; out: 2 per id)
+void hvxo_ibl_brdf_lut_f32(uint32_t W, uint32_t H, uint32_t n, const uint32_t* ids, uint32_t n_ids, float* out,
+                           int threads) {
+    parallel_for(n_ids, threads, [&](uint32_t k) { brdf_texel<float>(W, H, n, ids[k] % W, ids[k] / W, out + 2 * k); });
+}
+void hvxo_ibl_brdf_lut_f64(uint32_t W, uint32_t H, uint32_t n, const uint32_t* ids, uint32_t n_ids, double* out,
+                           int threads) {
+    parallel_for(n_ids, threads, [&](uint32_t k) { brdf_texel<double>(W, H, n, ids[k] % W, ids[k] / W, out + 2 * k); });
+}
+
+// ---- a12: equirect -> cube texels (id = (face*S + y)*S + x; out: 4 per id)
+void hvxo_ibl_equirect_to_cube_f32(const float* eq, uint32_t w, uint32_t h, uint32_t S, const uint32_t* ids,
+                                   uint32_t n_ids, float* out, int threads) {
+    parallel_for(n_ids, threads, [&](uint32_t k) { equirect_texel<float>(eq, w, h, S, ids[k], out + 4 * k); });
+}
+void hvxo_ibl_equirect_to_cube_f64(const float* eq, uint32_t w, uint32_t h, uint32_t S, const uint32_t* ids,
+                                   uint32_t n_ids, double* out, int threads) {
+    parallel_for(n_ids, threads, [&](uint32_t k) { equirect_texel<double>(eq, w, h, S, ids[k], out + 4 * k); });
+}
+
+// ---- a13: 2x2 box mip chain in place (levels 1..levels-1)
+void hvxo_ibl_cube_mips_f32(float* chain, uint32_t S, uint32_t levels) {
+    for (uint32_t m = 1; m < levels; ++m) {
+        const uint32_t sp = std::max<uint32_t>(S >> (m - 1), 1u), sn = std::max<uint32_t>(S >> m, 1u);
+        const float* src = chain + chain_offset(S, m - 1);
+        float* dst = chain + chain_offset(S, m);
+        for (uint32_t f = 0; f < 6; ++f)
+            for (uint32_t y = 0; y < sn; ++y)
+                for (uint32_t x = 0; x < sn; ++x)
+                    for (uint32_t c = 0; c < 4; ++c) {
+                        auto at = [&](uint32_t xx, uint32_t yy) { return src[(((size_t)f * sp + yy) * sp + xx) * 4 + c]; };
+                        const float a = at(2 * x, 2 * y), b = at(2 * x + 1, 2 * y);
+                        const float cc = at(2 * x, 2 * y + 1), d = at(2 * x + 1, 2 * y + 1);
+                        dst[(((size_t)f * sn + y) * sn + x) * 4 + c] = ((a + b) + (cc + d)) * 0.25f;
+                    }
+    }
+}
+
+// ---- a14: diffuse irradiance texels (id = (face*Sd + y)*Sd + x; out: 4 per id)
+void hvxo_ibl_diffuse_f32(const float* cube, uint32_t S, uint32_t Sd, const uint32_t* ids, uint32_t n_ids, float* out,
+                          int threads) {
+    parallel_for(n_ids, threads, [&](uint32_t k) { diffuse_texel<float>(cube, S, Sd, ids[k], out + 4 * k); });
+}
+void hvxo_ibl_diffuse_f64(const float* cube, uint32_t S, uint32_t Sd, const uint32_t* ids, uint32_t n_ids,
+                          double* out, int threads) {
+    parallel_for(n_ids, threads, [&](uint32_t k) { diffuse_texel<double>(cube, S, Sd, ids[k], out + 4 * k); });
+}
+
+// ---- a15: specular prefilter texels of one mip (out: 4 per id)
+void hvxo_ibl_specular_f32(const float* cube, uint32_t S, uint32_t So, uint32_t mips, uint32_t mip, uint32_t n,
+                           int emulate_static_viewport, const uint32_t* ids, uint32_t n_ids, float* out,
+                           int threads) {
+    const uint32_t Sm = std::max<uint32_t>(So >> mip, 1u);
+    const uint32_t D = emulate_static_viewport ? So : Sm;
+    const float rough = specular_roughness<float>(mip, mips);
+    parallel_for(n_ids, threads,
+                 [&](uint32_t k) { specular_texel<float>(cube, S, Sm, D, rough, n, ids[k], out + 4 * k); });
+}
+void hvxo_ibl_specular_f64(const float* cube, uint32_t S, uint32_t So, uint32_t mips, uint32_t mip, uint32_t n,
+                           int emulate_static_viewport, const uint32_t* ids, uint32_t n_ids, double* out,
+                           int threads) {
+    const uint32_t Sm = std::max<uint32_t>(So >> mip, 1u);
+    const uint32_t D = emulate_static_viewport ? So : Sm;
+    const double rough = specular_roughness<double>(mip, mips);
+    parallel_for(n_ids, threads,
+                 [&](uint32_t k) { specular_texel<double>(cube, S, Sm, D, rough, n, ids[k], out + 4 * k); });
+}
+
+}  // extern "C"

@@ -1,0 +1,45 @@
+"""Shared scene set-up for the tests: the benchmark workloads at full or
+reduced size, built with the seeded generators of include/hvx_scene.h."""
+from __future__ import annotations
+
+import numpy as np
+
+from hellovulkan_amd import abi
+
+EYE = (0.0, 1.0, 6.0)        # AppPBRClusterForward.cpp:22
+TARGET = (0.0, 2.5, 0.0)
+FOVY = 0.785398              # CameraConfig::Zoom (Configs.h:38)
+NEAR, FAR = 0.1, 100.0       # Configs.h:42-43
+
+# workload configurations: (width, height, grid, n_lights, cap)
+CFG0 = dict(width=2560, height=1600, grid=(16, 9, 24), n_lights=1024, cap=128)
+CFG1 = dict(width=3840, height=2160, grid=(32, 18, 48), n_lights=65536, cap=128)
+
+
+def camera(width, height, grid):
+    return abi.camera_build(EYE, TARGET, FOVY, NEAR, FAR, width, height, grid)
+
+
+def to_dev(a, device="cuda"):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device)
+
+
+def random_cube_chain(rng, size, mips, lo=0.0, hi=4.0):
+    n = abi.cube_chain_floats(size, mips)
+    c = rng.uniform(lo, hi, n).astype(np.float32)
+    c[3::4] = 1.0
+    return c
+
+
+def random_ibl_maps(rng, spec_size=32, spec_mips=6, diff_size=8, lut_w=64, lut_h=64):
+    spec = random_cube_chain(rng, spec_size, spec_mips)
+    diff = random_cube_chain(rng, diff_size, 1)
+    lut = rng.uniform(0, 1, lut_w * lut_h * 2).astype(np.float32)
+    return dict(spec=spec, spec_size=spec_size, spec_mips=spec_mips, diff=diff, diff_size=diff_size, lut=lut,
+                lut_w=lut_w, lut_h=lut_h)
+
+
+def cells_to_lists(cells, indices):
+    """Per-cluster lists (tuples) from LightCell records + packed indices."""
+    return [tuple(indices[o:o + c]) for o, c in zip(cells["offset"].tolist(), cells["count"].tolist())]
