@@ -30,6 +30,8 @@ HVX_ERR_HIP = 5
 HVX_ERR_UNSUPPORTED = 6
 HVX_ERR_NO_DEVICE = 7
 
+OPT_SHADE_EXACT = 1
+
 FILTER_DIFFUSE = 0
 FILTER_SPECULAR = 1
 
@@ -111,6 +113,8 @@ SIGNATURES = [
     ("hvx_ctx_create", C.c_int, [C.c_int, P(vp)]),
     ("hvx_ctx_destroy", C.c_int, [vp]),
     ("hvx_ctx_last_error", C.c_char_p, [vp]),
+    ("hvx_ctx_set_option", C.c_int, [vp, C.c_int, C.c_int64]),
+    ("hvx_ctx_get_option", C.c_int, [vp, C.c_int, P(C.c_int64)]),
     ("hvx_cf_build_aabbs", C.c_int, [vp, P(CFUBO), vp, vp]),
     ("hvx_cf_cull_lights", C.c_int, [vp, P(CFUBO), vp, vp, u32, u32, vp, vp, C.c_size_t, vp, vp, vp]),
     ("hvx_cf_shade", C.c_int, [vp, P(FragSoA), P(CameraUBO), P(CFUBO), P(PBRPC), vp, u32, vp, vp,
@@ -260,6 +264,14 @@ class Context:
     def _check(self, st: int, where: str):
         if st:
             raise HvxError(st, where, self.lib.hvx_ctx_last_error(self.h).decode())
+
+    def set_option(self, option: int, value: int):
+        self._check(self.lib.hvx_ctx_set_option(self.h, option, value), "hvx_ctx_set_option")
+
+    def get_option(self, option: int) -> int:
+        v = C.c_int64()
+        self._check(self.lib.hvx_ctx_get_option(self.h, option, C.byref(v)), "hvx_ctx_get_option")
+        return int(v.value)
 
     def build_aabbs(self, ubo: CFUBO, clusters, stream=None):
         self._check(self.lib.hvx_cf_build_aabbs(self.h, C.byref(ubo), _ptr(clusters), self._stream(stream)),

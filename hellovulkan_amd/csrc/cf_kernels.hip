@@ -4,9 +4,7 @@
 // Reference kernels (HelloVulkan/Shaders/ClusteredForward):
 //   AABBGenerator.comp:39-89   -> aabb_kernel (one lane per cluster)
 //   LightCulling.comp:59-87    -> light_prep_kernel + cull_kernel + scan/scatter
-//   Scene.frag:60-149,
-//   Radiance.glsl:2-56,
-//   PBR/Ambient.glsl:1-29      -> shade_kernel (one lane per SoA fragment)
+// (shading lives in shade_kernels.hip)
 //
 // Culling. The reference runs one invocation per cluster that walks every
 // light and keeps the first `cap` hits. Here a workgroup owns a 4x4x1 block
@@ -289,140 +287,6 @@ __global__ __launch_bounds__(256) void scatter_kernel(const uint32_t* __restrict
     for (uint32_t j = lane; j < cell.count; j += 64) indices[cell.offset + j] = lists[(size_t)c * cap + j];
 }
 
-// ---------------------------------------------------------------- shading
-struct ShadeArgs {
-    const float* depth;
-    const float* nrm[3];
-    const float* alb[3];
-    const float* rough;
-    const uint8_t* metal;
-    uint32_t W, H;
-    float ivp[16];
-    float cam[3];
-    float nearv, farv, scale, bias, scr_x, scr_y;
-    uint32_t X, Y, Z;
-    hvx_pbr_pc pc;
-    const hvx_light* lights;
-    const hvx_light_cell* cells;
-    const uint32_t* indices;
-    const float4* spec;
-    uint32_t spec_size, spec_mips;
-    const float4* diff;
-    uint32_t diff_size;
-    const float2* lut;
-    uint32_t lut_w, lut_h;
-    float* out;
-};
-
-__device__ __forceinline__ f3 sample_spec_lod(const ShadeArgs& a, f3 dir, float lod) {
-    const float q = (float)(a.spec_mips - 1);
-    const float lam = gclamp(lod, 0.0f, q);
-    const float fl = floorf_(lam);
-    const uint32_t l0 = (uint32_t)fl;
-    const uint32_t l1 = min(l0 + 1, a.spec_mips - 1);
-    const float d = lam - fl;
-    const int s0 = (int)max(a.spec_size >> l0, 1u), s1 = (int)max(a.spec_size >> l1, 1u);
-    const f3 c0 = sample_cube(a.spec + cube_level_offset_floats(a.spec_size, l0) / 4, s0, dir);
-    const f3 c1 = sample_cube(a.spec + cube_level_offset_floats(a.spec_size, l1) / 4, s1, dir);
-    return {(1.0f - d) * c0.x + d * c1.x, (1.0f - d) * c0.y + d * c1.y, (1.0f - d) * c0.z + d * c1.z};
-}
-
-__global__ __launch_bounds__(256) void shade_kernel(ShadeArgs a) {
-    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nfrag = a.W * a.H;
-    if (idx >= nfrag) return;
-    const float PI = kPiF;
-    const uint32_t px = idx % a.W, py = idx / a.W;
-    const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
-    const float z = a.depth[idx];
-    // LinearDepth + cluster lookup (Scene.frag:55-78)
-    const float lin = (a.nearv * a.farv) / (a.farv + z * (a.nearv - a.farv));
-    const float zf = gmax(log2f_(lin) * a.scale + a.bias, 0.0f);
-    uint32_t zi = (uint32_t)zf;
-    if (zi >= a.Z) zi = a.Z - 1;
-    const float tsx = a.scr_x / (float)a.X, tsy = a.scr_y / (float)a.Y;
-    const uint32_t cx = min((uint32_t)(fx / tsx), a.X - 1), cy = min((uint32_t)(fy / tsy), a.Y - 1);
-    const hvx_light_cell cell = a.cells[cx + cy * a.X + zi * a.X * a.Y];
-
-    float wh[4];
-    matvec4(a.ivp, (fx / (float)a.W) * 2.0f - 1.0f, (fy / (float)a.H) * 2.0f - 1.0f, z, 1.0f, wh);
-    const f3 wp = {wh[0] / wh[3], wh[1] / wh[3], wh[2] / wh[3]};
-
-    const float alb[3] = {a.alb[0][idx], a.alb[1][idx], a.alb[2][idx]};
-    const float metallic = (float)a.metal[idx] / 255.0f;
-    const float rough = a.rough[idx];
-    const float rr = rough + 1.0f;
-    const float alphaRough = (rr * rr) / 8.0f;
-    const f3 N = {a.nrm[0][idx], a.nrm[1][idx], a.nrm[2][idx]};
-    const f3 V = normalize3(sub3(make3(a.cam[0], a.cam[1], a.cam[2]), wp));
-    const float NoV = gmax(dot3(N, V), 0.0f);
-    const float base = a.pc.base_reflectivity;
-    float F0[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) F0[c] = base * (1.0f - metallic) + alb[c] * metallic;
-    float Lo[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) Lo[c] = alb[c] * a.pc.albedo_multiplier;
-
-    const float alpha = rough * rough;
-    const float alpha2 = alpha * alpha;
-    const float oneMinusMetal = 1.0f - metallic;
-    const float GVden = NoV * (1.0f - alphaRough) + alphaRough;
-    const float GV = NoV / GVden;
-    for (uint32_t k = 0; k < cell.count; ++k) {
-        const hvx_light& l = a.lights[a.indices[cell.offset + k]];
-        const float4 lpos = *reinterpret_cast<const float4*>(l.position);
-        const float4 lcol = *reinterpret_cast<const float4*>(l.color);
-        const float lrad = l.radius;
-        const f3 d = sub3(make3(lpos.x, lpos.y, lpos.z), wp);
-        const f3 L = normalize3(d);
-        const f3 Hh = normalize3(add3(V, L));
-        const float NoH = gmax(dot3(N, Hh), 0.0f);
-        const float NoL = gmax(dot3(N, L), 0.0f);
-        const float HoV = gmax(dot3(Hh, V), 0.0f);
-        const float dist = length3(d);
-        const float att = gmax(1.0f - dist / lrad, 0.0f) / powf_(dist, a.pc.light_falloff);
-        const float rad[3] = {(lcol.x * att) * a.pc.light_intensity, (lcol.y * att) * a.pc.light_intensity,
-                              (lcol.z * att) * a.pc.light_intensity};
-        const float NoH2 = NoH * NoH;
-        float den = NoH2 * (alpha2 - 1.0f) + 1.0f;
-        den = (PI * den) * den;
-        const float D = alpha2 / den;
-        const float GL = NoL / (NoL * (1.0f - alphaRough) + alphaRough);
-        const float G = GL * GV;
-        const float p5 = pow5f(gclamp(1.0f - HoV, 0.0f, 1.0f));
-        const float denom = ((4.0f * NoV) * NoL) + 0.0001f;
-        const float DG = D * G;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const float F = F0[c] + (1.0f - F0[c]) * p5;
-            const float spec = (DG * F) / denom;
-            const float kD = (1.0f - F) * oneMinusMetal;
-            const float diffuse = (kD * alb[c]) / PI;
-            Lo[c] = Lo[c] + ((diffuse + spec) * rad[c]) * NoL;
-        }
-    }
-
-    // Ambient (PBR/Ambient.glsl:1-29)
-    const float p5v = pow5f(gclamp(1.0f - NoV, 0.0f, 1.0f));
-    const f3 irr = sample_cube(a.diff, (int)a.diff_size, N);
-    const f3 I = {-V.x, -V.y, -V.z};
-    const float dNI = dot3(N, I);
-    const f3 R = sub3(I, mul3(N, 2.0f * dNI));
-    const f3 pre = sample_spec_lod(a, R, rough * a.pc.max_reflection_lod);
-    const float2 brdf = sample_2d_rg_repeat(a.lut, (int)a.lut_w, (int)a.lut_h, NoV, rough);
-    const float irr3[3] = {irr.x, irr.y, irr.z}, pre3[3] = {pre.x, pre.y, pre.z};
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const float F = F0[c] + (gmax(1.0f - rough, F0[c]) - F0[c]) * p5v;
-        const float kD = (1.0f - F) * oneMinusMetal;
-        const float diffuse = irr3[c] * alb[c];
-        const float specular = pre3[c] * (F * brdf.x + brdf.y);
-        const float ambient = (kD * diffuse + specular) * 1.0f;
-        a.out[(size_t)c * nfrag + idx] = (ambient + 0.0f) + Lo[c];
-    }
-}
-
 }  // namespace
 
 // ================================================================ launchers
@@ -447,48 +311,6 @@ hipError_t launch_cull(const hvx_cf_ubo& ubo, const hvx_aabb* d_clusters, const 
                        d_overflow != nullptr ? 1 : 0, lists, counts, over);
     hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, counts, over, C, d_cells, d_total, d_overflow);
     hipLaunchKernelGGL(scatter_kernel, dim3((C + 3) / 4), dim3(256), 0, s, lists, cap, C, d_cells, d_indices);
-    return hipGetLastError();
-}
-
-hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const hvx_cf_ubo& ubo,
-                        const hvx_pbr_pc& pc, const hvx_light* d_lights, const hvx_light_cell* d_cells,
-                        const uint32_t* d_indices, const hvx_ibl_maps& ibl, float* d_rgb, hipStream_t s) {
-    ShadeArgs a;
-    a.depth = fr.depth;
-    for (int c = 0; c < 3; ++c) {
-        a.nrm[c] = fr.normal[c];
-        a.alb[c] = fr.albedo[c];
-    }
-    a.rough = fr.roughness;
-    a.metal = fr.metallic;
-    a.W = fr.width;
-    a.H = fr.height;
-    for (int k = 0; k < 16; ++k) a.ivp[k] = fr.inv_view_proj[k];
-    for (int k = 0; k < 3; ++k) a.cam[k] = cam.position[k];
-    a.nearv = ubo.camera_near;
-    a.farv = ubo.camera_far;
-    a.scale = ubo.slice_scaling;
-    a.bias = ubo.slice_bias;
-    a.scr_x = ubo.screen_size[0];
-    a.scr_y = ubo.screen_size[1];
-    a.X = ubo.slice_count_x;
-    a.Y = ubo.slice_count_y;
-    a.Z = ubo.slice_count_z;
-    a.pc = pc;
-    a.lights = d_lights;
-    a.cells = d_cells;
-    a.indices = d_indices;
-    a.spec = reinterpret_cast<const float4*>(ibl.specular);
-    a.spec_size = ibl.specular_size;
-    a.spec_mips = ibl.specular_mips;
-    a.diff = reinterpret_cast<const float4*>(ibl.diffuse);
-    a.diff_size = ibl.diffuse_size;
-    a.lut = reinterpret_cast<const float2*>(ibl.brdf_lut);
-    a.lut_w = ibl.lut_width;
-    a.lut_h = ibl.lut_height;
-    a.out = d_rgb;
-    const uint32_t F = fr.width * fr.height;
-    hipLaunchKernelGGL(shade_kernel, dim3((F + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

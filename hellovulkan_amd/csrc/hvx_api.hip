@@ -108,6 +108,25 @@ const char* hvx_status_str(int s) {
 
 const char* hvx_ctx_last_error(const hvx_ctx* ctx) { return ctx ? ctx->last_error.c_str() : ""; }
 
+int hvx_ctx_set_option(hvx_ctx* ctx, int option, int64_t value) {
+    if (!ctx) return HVX_ERR_NULL;
+    switch (option) {
+        case HVX_OPT_SHADE_EXACT:
+            if (value != 0 && value != 1) return fail(ctx, HVX_ERR_INVALID_ARG, "HVX_OPT_SHADE_EXACT is 0 or 1");
+            ctx->shade_exact = (int)value;
+            return HVX_OK;
+        default: return fail(ctx, HVX_ERR_INVALID_ARG, "unknown option %d", option);
+    }
+}
+
+int hvx_ctx_get_option(const hvx_ctx* ctx, int option, int64_t* value) {
+    if (!ctx || !value) return HVX_ERR_NULL;
+    switch (option) {
+        case HVX_OPT_SHADE_EXACT: *value = ctx->shade_exact; return HVX_OK;
+        default: return HVX_ERR_INVALID_ARG;
+    }
+}
+
 int hvx_ctx_create(int device, hvx_ctx** out) {
     if (!out) return HVX_ERR_NULL;
     *out = nullptr;
@@ -128,7 +147,7 @@ int hvx_ctx_destroy(hvx_ctx* ctx) {
     if (!ctx) return HVX_ERR_NULL;
     if (bind_device(ctx) == HVX_OK) {
         (void)hipDeviceSynchronize();
-        void* bufs[] = {ctx->cull_lists, ctx->cull_meta, ctx->light_vs, ctx->ggx_table};
+        void* bufs[] = {ctx->cull_lists, ctx->cull_meta, ctx->light_vs, ctx->ggx_table, ctx->shade_lights};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
     }
@@ -217,8 +236,13 @@ int hvx_cf_shade(hvx_ctx* ctx, const hvx_frag_soa* fr, const hvx_camera_ubo* cam
     if (misaligned(ibl->specular, 16) || misaligned(ibl->diffuse, 16) || misaligned(ibl->brdf_lut, 8))
         return fail(ctx, HVX_ERR_ALIGN, "IBL maps must be 16-byte (cubes) / 8-byte (LUT) aligned");
     if ((st = bind_device(ctx))) return st;
-    hipError_t e = hvx::launch_shade(*fr, *cam, *ubo, *pc, d_lights, d_cells, d_indices, *ibl, d_rgb,
-                                     (hipStream_t)stream);
+    hipError_t e = hipSuccess;
+    if (!ctx->shade_exact &&
+        (e = hvx::ensure_scratch(ctx, &ctx->shade_lights, &ctx->shade_lights_bytes, (size_t)n_lights * 32)) !=
+            hipSuccess)
+        return hip_fail(ctx, e, "shading light table allocation");
+    e = hvx::launch_shade(*fr, *cam, *ubo, *pc, d_lights, n_lights, d_cells, d_indices, *ibl, d_rgb,
+                          ctx->shade_exact != 0, static_cast<float4*>(ctx->shade_lights), (hipStream_t)stream);
     return e == hipSuccess ? HVX_OK : hip_fail(ctx, e, "hvx_cf_shade launch");
 }
 

@@ -23,6 +23,10 @@ struct hvx_ctx {
     size_t light_vs_bytes = 0;
     void* ggx_table = nullptr;    // float4 [mips * samples]: tangent-space GGX H
     size_t ggx_table_bytes = 0;
+    void* shade_lights = nullptr; // float4 [2 * lights]: compact shading light table
+    size_t shade_lights_bytes = 0;
+    // options (hvx_ctx_set_option)
+    int shade_exact = 0;
 };
 
 namespace hvx {
@@ -37,9 +41,11 @@ hipError_t launch_cull(const hvx_cf_ubo& ubo, const hvx_aabb* d_clusters, const 
                        uint32_t n_lights, uint32_t cap, hvx_light_cell* d_cells, uint32_t* d_indices,
                        uint32_t* d_total, uint32_t* d_overflow, float4* light_vs, uint32_t* lists,
                        uint32_t* counts, uint32_t* over, hipStream_t s);
+// --- shading (shade_kernels.hip); light_scratch holds 2 * n_lights float4
 hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const hvx_cf_ubo& ubo,
-                        const hvx_pbr_pc& pc, const hvx_light* d_lights, const hvx_light_cell* d_cells,
-                        const uint32_t* d_indices, const hvx_ibl_maps& ibl, float* d_rgb, hipStream_t s);
+                        const hvx_pbr_pc& pc, const hvx_light* d_lights, uint32_t n_lights,
+                        const hvx_light_cell* d_cells, const uint32_t* d_indices, const hvx_ibl_maps& ibl,
+                        float* d_rgb, bool exact, float4* light_scratch, hipStream_t s);
 
 // --- IBL (ibl_kernels.hip)
 hipError_t launch_brdf_lut(uint32_t w, uint32_t h, uint32_t n, float* d_lut, hipStream_t s);

@@ -155,6 +155,19 @@ const char* hvx_status_str(int status);
 const char* hvx_ctx_last_error(const hvx_ctx* ctx);
 int hvx_abi_version(void);
 
+/* Context options (no counterpart in the reference, whose arithmetic is fixed
+ * by the driver's shader compiler). */
+typedef enum hvx_option {
+    /* 0 (default): shading keeps the light/half vectors and their cosines
+     * exact and evaluates the remaining per-light terms with fused
+     * multiply-add and hardware reciprocal/exp2/log2 (within 1e-5 relative of
+     * the exact evaluation); 1: every operation exactly as the FP32
+     * definition of the path (bit-reproducible on any IEEE machine). */
+    HVX_OPT_SHADE_EXACT = 1
+} hvx_option;
+int hvx_ctx_set_option(hvx_ctx* ctx, int option, int64_t value);
+int hvx_ctx_get_option(const hvx_ctx* ctx, int option, int64_t* value);
+
 /* ------------------------------------------------------- clustered forward */
 
 /* Cluster AABBs for every froxel of the X*Y*Z grid of `ubo`.

@@ -135,17 +135,26 @@ def _shade_case(ctx, width, height, grid, n_lights, cap, seed, n_check, far_plan
     return got[ids], ref, ids
 
 
+@pytest.mark.parametrize("mode", ["fast", "exact"])
 @pytest.mark.parametrize("case", ["small", "cfg0", "cfg1"])
-def test_shade_matches_oracle(ctx, case):
-    if case == "small":
-        got, ref, _ = _shade_case(ctx, 320, 200, (16, 9, 24), 512, 128, 31, 4000, far_plane_rows=1)
-    elif case == "cfg0":
-        got, ref, _ = _shade_case(ctx, CFG0["width"], CFG0["height"], CFG0["grid"], CFG0["n_lights"], 128, 32, 20000)
-    else:
-        got, ref, _ = _shade_case(ctx, CFG1["width"], CFG1["height"], CFG1["grid"], CFG1["n_lights"], 128, 33, 20000)
+def test_shade_matches_oracle(ctx, case, mode):
+    ctx.set_option(abi.OPT_SHADE_EXACT, 1 if mode == "exact" else 0)
+    try:
+        if case == "small":
+            got, ref, _ = _shade_case(ctx, 320, 200, (16, 9, 24), 512, 128, 31, 4000, far_plane_rows=1)
+        elif case == "cfg0":
+            got, ref, _ = _shade_case(ctx, CFG0["width"], CFG0["height"], CFG0["grid"], CFG0["n_lights"], 128, 32,
+                                      20000)
+        else:
+            got, ref, _ = _shade_case(ctx, CFG1["width"], CFG1["height"], CFG1["grid"], CFG1["n_lights"], 128, 33,
+                                      20000)
+    finally:
+        ctx.set_option(abi.OPT_SHADE_EXACT, 0)
     assert np.all(np.isfinite(got))
     err = O.rel_err(got, ref)
-    print(f"shade {case}: max rel err {err:.3e}, exact fraction {np.mean(got == ref):.4f}")
+    print(f"shade {case} {mode}: max rel err {err:.3e}, exact fraction {np.mean(got == ref):.4f}")
+    if mode == "exact":
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
     assert err <= SHADE_TOL
 
 
@@ -156,6 +165,10 @@ def test_shade_zero_lights(ctx):
 
 def test_abi_rejects_bad_arguments(ctx):
     import torch
+    with pytest.raises(abi.HvxError) as e:
+        ctx.set_option(abi.OPT_SHADE_EXACT, 2)
+    assert e.value.status == abi.HVX_ERR_INVALID_ARG
+    assert ctx.get_option(abi.OPT_SHADE_EXACT) == 0
     _, _, ubo, _ = camera(320, 200, (4, 3, 6))
     d = torch.empty(4 * 3 * 6 * 8, dtype=torch.float32, device="cuda")
     lights = to_dev(abi.scene_lights(1, 10).view(np.uint8))
