@@ -46,6 +46,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-ibl", action="store_true", help="skip the one-off IBL stage timings")
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample duration")
+    p.add_argument("--calibrate", action="store_true",
+                   help="run one 256 MiB elementwise copy first (known HBM traffic, for counter scaling)")
     return p.parse_args()
 
 
@@ -93,6 +95,13 @@ def main():
         e1.record(stream)
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps
+
+    if args.calibrate:  # y = x + 0: reads and writes 256 MiB each
+        cx = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        cy = torch.empty_like(cx)
+        torch.add(cx, 0.0, out=cy)
+        torch.cuda.synchronize()
+        del cx, cy
 
     ibl = {}
     ibl["equirect_to_cube_ms"] = timed(lambda: ctx.equirect_to_cube(d_eq, EQW, EQH, d_cube, CUBE))

@@ -19,7 +19,8 @@ except ImportError:  # pragma: no cover - torch is present in this image
     torch = None
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG_DIR, "libhvx.so")
+# HVX_LIBRARY selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("HVX_LIBRARY") or os.path.join(PKG_DIR, "libhvx.so")
 
 HVX_OK = 0
 HVX_ERR_NULL = 1

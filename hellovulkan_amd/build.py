@@ -73,12 +73,25 @@ def build_oracle(verbose: bool = False) -> str:
     return ORACLE_LIB
 
 
+TEST_KERNELS_SRC = os.path.join(ROOT, "tests", "gpu_kernels", "fastmath_check.hip")
+TEST_KERNELS_LIB = os.path.join(ROOT, "tests", "_build", "libhvx_testkernels.so")
+
+
+def build_test_kernels(verbose: bool = False) -> str:
+    """Test-only GPU kernels (exhaustive arithmetic checks) used by tests/."""
+    os.makedirs(os.path.dirname(TEST_KERNELS_LIB), exist_ok=True)
+    tmp = TEST_KERNELS_LIB + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, TEST_KERNELS_SRC], verbose)
+    os.replace(tmp, TEST_KERNELS_LIB)
+    return TEST_KERNELS_LIB
+
+
 def build(verbose: bool = False) -> None:
-    with cf.ThreadPoolExecutor(max_workers=2) as ex:
-        a = ex.submit(build_product, verbose)
-        b = ex.submit(build_oracle, verbose)
-        a.result()
-        b.result()
+    with cf.ThreadPoolExecutor(max_workers=3) as ex:
+        futs = [ex.submit(build_product, verbose), ex.submit(build_oracle, verbose),
+                ex.submit(build_test_kernels, verbose)]
+        for f in futs:
+            f.result()
 
 
 if __name__ == "__main__":

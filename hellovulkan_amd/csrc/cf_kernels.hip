@@ -236,45 +236,114 @@ __global__ __launch_bounds__(256) void cull_kernel(hvx_cf_ubo u, const hvx_aabb*
 }
 
 // Exclusive scan of the counts in cluster order -> LightCell offsets (the
-// deterministic replacement of atomicAdd(globalIndex, count)).
-__global__ __launch_bounds__(1024) void scan_kernel(const uint32_t* __restrict__ counts,
-                                                    const uint32_t* __restrict__ over, uint32_t C,
-                                                    hvx_light_cell* __restrict__ cells, uint32_t* __restrict__ total,
-                                                    uint32_t* __restrict__ novf) {
-    __shared__ uint32_t s_sum[1024];
-    __shared__ uint32_t s_ov[1024];
-    const uint32_t t = threadIdx.x;
-    const uint32_t per = (C + 1023) / 1024;
-    const uint32_t b = min(t * per, C), e = min(b + per, C);
-    uint32_t local = 0, lov = 0;
-    for (uint32_t c = b; c < e; ++c) {
-        local += counts[c];
-        lov += over[c];
+// deterministic replacement of atomicAdd(globalIndex, count)), in three
+// phases over blocks of 1024 clusters: block sums, a scan of the block sums,
+// then each block's local scan plus its block offset.
+constexpr uint32_t kScanBlock = 1024;
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t u = __shfl_up(v, d, 64);
+        if ((int)lane >= d) v += u;
     }
-    s_sum[t] = local;
-    s_ov[t] = lov;
+    return v;
+}
+
+// 256-lane block exclusive scan; returns the block total in *total
+__device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* s_w, uint32_t* total) {
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t inc = wave_incl_scan(v, lane);
+    if (lane == 63) s_w[w] = inc;
     __syncthreads();
-    // Hillis-Steele inclusive scan over 1024 partial sums
-    for (uint32_t d = 1; d < 1024; d <<= 1) {
-        const uint32_t v = t >= d ? s_sum[t - d] : 0u;
-        const uint32_t w = t >= d ? s_ov[t - d] : 0u;
-        __syncthreads();
-        s_sum[t] += v;
-        s_ov[t] += w;
-        __syncthreads();
+    uint32_t woff = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        woff += (k < (int)w) ? s_w[k] : 0u;
+        tot += s_w[k];
     }
-    uint32_t run = s_sum[t] - local;
-    for (uint32_t c = b; c < e; ++c) {
-        const uint32_t k = counts[c];
-        hvx_light_cell cell;
-        cell.offset = run;
-        cell.count = k;
-        cells[c] = cell;
-        run += k;
+    __syncthreads();
+    *total = tot;
+    return woff + inc - v;
+}
+
+__global__ __launch_bounds__(256) void scan_partials_kernel(const uint32_t* __restrict__ counts,
+                                                            const uint32_t* __restrict__ over, uint32_t C,
+                                                            uint2* __restrict__ part) {
+    __shared__ uint32_t s_w[4], s_v[4];
+    const uint32_t b0 = blockIdx.x * kScanBlock;
+    uint32_t sum = 0, ov = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t c = b0 + r * 256 + threadIdx.x;
+        if (c < C) {
+            sum += counts[c];
+            ov += over[c];
+        }
     }
-    if (t == 1023) {
-        if (total) *total = s_sum[1023];
-        if (novf) *novf = s_ov[1023];
+    uint32_t tsum, tov;
+    block_excl_scan256(sum, s_w, &tsum);
+    block_excl_scan256(ov, s_v, &tov);
+    if (threadIdx.x == 0) part[blockIdx.x] = make_uint2(tsum, tov);
+}
+
+__global__ __launch_bounds__(1024) void scan_top_kernel(uint2* __restrict__ part, uint32_t nb,
+                                                        uint32_t* __restrict__ total, uint32_t* __restrict__ novf) {
+    __shared__ uint32_t s_w[16], s_o[16];
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    uint32_t carry = 0, ocarry = 0;
+    for (uint32_t b0 = 0; b0 < nb; b0 += 1024) {
+        const uint32_t b = b0 + t;
+        const uint2 p = b < nb ? part[b] : make_uint2(0u, 0u);
+        const uint32_t inc = wave_incl_scan(p.x, lane);
+        const uint32_t oinc = wave_incl_scan(p.y, lane);
+        if (lane == 63) {
+            s_w[w] = inc;
+            s_o[w] = oinc;
+        }
+        __syncthreads();
+        uint32_t woff = 0, tot = 0, otot = 0;
+        for (uint32_t k = 0; k < 16; ++k) {
+            woff += k < w ? s_w[k] : 0u;
+            tot += s_w[k];
+            otot += s_o[k];
+        }
+        __syncthreads();
+        if (b < nb) part[b].x = carry + woff + inc - p.x;  // exclusive offset of block b
+        carry += tot;
+        ocarry += otot;
+    }
+    if (t == 0) {
+        if (total) *total = carry;
+        if (novf) *novf = ocarry;
+    }
+}
+
+__global__ __launch_bounds__(256) void scan_apply_kernel(const uint32_t* __restrict__ counts, uint32_t C,
+                                                         const uint2* __restrict__ part,
+                                                         hvx_light_cell* __restrict__ cells) {
+    __shared__ uint32_t s_w[4];
+    const uint32_t b0 = blockIdx.x * kScanBlock;
+    // thread t owns clusters b0 + 4t .. b0 + 4t + 3 (contiguous, in order)
+    uint32_t k[4], local = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t c = b0 + threadIdx.x * 4 + r;
+        k[r] = c < C ? counts[c] : 0u;
+        local += k[r];
+    }
+    uint32_t tot;
+    uint32_t run = part[blockIdx.x].x + block_excl_scan256(local, s_w, &tot);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t c = b0 + threadIdx.x * 4 + r;
+        if (c < C) {
+            hvx_light_cell cell;
+            cell.offset = run;
+            cell.count = k[r];
+            cells[c] = cell;
+        }
+        run += k[r];
     }
 }
 
@@ -309,7 +378,11 @@ hipError_t launch_cull(const hvx_cf_ubo& ubo, const hvx_aabb* d_clusters, const 
     const uint32_t groups = ((X + kGX - 1) / kGX) * ((Y + kGY - 1) / kGY) * ((Z + kGZ - 1) / kGZ);
     hipLaunchKernelGGL(cull_kernel, dim3(groups), dim3(256), 0, s, ubo, d_clusters, light_vs, n_lights, cap,
                        d_overflow != nullptr ? 1 : 0, lists, counts, over);
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, counts, over, C, d_cells, d_total, d_overflow);
+    const uint32_t nb = (C + kScanBlock - 1) / kScanBlock;
+    uint2* part = reinterpret_cast<uint2*>(over + C);
+    hipLaunchKernelGGL(scan_partials_kernel, dim3(nb), dim3(256), 0, s, counts, over, C, part);
+    hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(1024), 0, s, part, nb, d_total, d_overflow);
+    hipLaunchKernelGGL(scan_apply_kernel, dim3(nb), dim3(256), 0, s, counts, C, part, d_cells);
     hipLaunchKernelGGL(scatter_kernel, dim3((C + 3) / 4), dim3(256), 0, s, lists, cap, C, d_cells, d_indices);
     return hipGetLastError();
 }

@@ -201,7 +201,8 @@ int hvx_cf_cull_lights(hvx_ctx* ctx, const hvx_cf_ubo* ubo, const hvx_aabb* d_cl
     if ((st = bind_device(ctx))) return st;
     hipError_t e;
     if ((e = hvx::ensure_scratch(ctx, &ctx->cull_lists, &ctx->cull_lists_bytes, C * cap * 4)) != hipSuccess ||
-        (e = hvx::ensure_scratch(ctx, &ctx->cull_meta, &ctx->cull_meta_bytes, C * 8)) != hipSuccess ||
+        (e = hvx::ensure_scratch(ctx, &ctx->cull_meta, &ctx->cull_meta_bytes, C * 8 + ((C + 1023) / 1024) * 8 + 64)) !=
+            hipSuccess ||
         (e = hvx::ensure_scratch(ctx, &ctx->light_vs, &ctx->light_vs_bytes, (size_t)n_lights * 16)) != hipSuccess)
         return hip_fail(ctx, e, "culling scratch allocation");
     uint32_t* counts = static_cast<uint32_t*>(ctx->cull_meta);

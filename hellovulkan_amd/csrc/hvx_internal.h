@@ -17,7 +17,7 @@ struct hvx_ctx {
     // growable device scratch, owned by the context
     void* cull_lists = nullptr;   // uint32 [clusters * cap] fixed-slot lists
     size_t cull_lists_bytes = 0;
-    void* cull_meta = nullptr;    // uint32 counts[C], overflow[C]
+    void* cull_meta = nullptr;    // uint32 counts[C], overflow[C], uint2 block partials
     size_t cull_meta_bytes = 0;
     void* light_vs = nullptr;     // float4 [lights]: view-space centre, r^2
     size_t light_vs_bytes = 0;

@@ -5,12 +5,15 @@
 //                                     the row's GGX half-vectors staged in LDS)
 //   Equirect2Cube.frag:87-107      -> equirect_kernel (lane per cube texel)
 //   VulkanImage::GenerateMipmap    -> mip_kernel (2x2 box, lane per texel)
-//   CubeFilterDiffuse.frag:35-136  -> diffuse_kernel (lane per output texel;
+//   CubeFilterDiffuse.frag:35-136  -> diffuse_kernel (16 lanes per output
+//                                     texel evaluate samples, one lane per
+//                                     texel and channel sums them in order;
 //                                     the phi/theta sin/cos grid in LDS)
 //   CubeFilterSpecular.frag:34-146 -> ggx_table_kernel + specular_kernel (lane
 //                                     per output texel of every mip in one
 //                                     launch; tangent-space GGX samples per mip
-//                                     precomputed once)
+//                                     precomputed once; at roughness 0 the n
+//                                     identical samples are evaluated once)
 // The per-sample sequence and every summation order are the reference's, so
 // outputs reproduce the FP32 oracle to the bit; the precomputed tables hold
 // exactly the values the per-sample code would compute (same functions, same
@@ -136,16 +139,28 @@ __global__ __launch_bounds__(256) void mip_kernel(const float4* __restrict__ src
 }
 
 // ---------------------------------------------------------------- diffuse
+// The reference sums 252 x 63 = 15,876 samples per output texel in one
+// sequential loop (phi outer, theta inner). A lane per texel leaves the GPU
+// idle (6,144 texels at 32^2), so here a workgroup owns kDTex texels and its
+// 256 lanes evaluate kDSlot consecutive samples of each texel per round; the
+// weighted values go to LDS and one "summer" lane per texel and channel adds
+// them in the reference's order. Results are bit-identical to the sequential
+// loop. Rounds are double-buffered so sample evaluation of round r+1
+// overlaps the summation of round r.
 constexpr int kMaxPhi = 512, kMaxTheta = 128;
+constexpr int kDTex = 16;               // texels per workgroup
+constexpr int kDSlot = 256 / kDTex;     // samples per texel per round
 
-__global__ __launch_bounds__(64) void diffuse_kernel(const float4* __restrict__ cube, uint32_t S,
+__global__ __launch_bounds__(256) void diffuse_kernel(const float4* __restrict__ cube, uint32_t S,
                                                      float4* __restrict__ out, uint32_t Sd) {
     __shared__ float s_phi[kMaxPhi], s_theta[kMaxTheta];
     __shared__ float s_sp[kMaxPhi], s_cp[kMaxPhi], s_st[kMaxTheta], s_ct[kMaxTheta];
+    __shared__ float s_buf[2][kDSlot][kDTex][3];
     __shared__ uint32_t s_np, s_nt;
     const float TWO_PI = kPiF * 2.0f, HALF_PI = kPiF * 0.5f;
     const float delta = 0.025f;
-    if (threadIdx.x == 0) {
+    const uint32_t l = threadIdx.x;
+    if (l == 0) {
         uint32_t np = 0;
         for (float phi = 0.0f; phi < TWO_PI && np < (uint32_t)kMaxPhi; phi = phi + delta) s_phi[np++] = phi;
         uint32_t nt = 0;
@@ -154,38 +169,57 @@ __global__ __launch_bounds__(64) void diffuse_kernel(const float4* __restrict__ 
         s_nt = nt;
     }
     __syncthreads();
-    const uint32_t np = s_np, nt = s_nt;
-    for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) {
+    const uint32_t np = s_np, nt = s_nt, total = np * nt;
+    for (uint32_t i = l; i < np; i += blockDim.x) {
         s_sp[i] = sinf_(s_phi[i]);
         s_cp[i] = cosf_(s_phi[i]);
     }
-    for (uint32_t i = threadIdx.x; i < nt; i += blockDim.x) {
+    for (uint32_t i = l; i < nt; i += blockDim.x) {
         s_st[i] = sinf_(s_theta[i]);
         s_ct[i] = cosf_(s_theta[i]);
     }
     __syncthreads();
-    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= 6u * Sd * Sd) return;
-    const uint32_t x = id % Sd, y = (id / Sd) % Sd, face = id / (Sd * Sd);
+    const uint32_t ntex = 6u * Sd * Sd;
+    // sample lane: texel l % kDTex, slot l / kDTex
+    const uint32_t ti = l % kDTex, slot = l / kDTex;
+    const uint32_t texel = min(blockIdx.x * kDTex + ti, ntex - 1);
+    const uint32_t x = texel % Sd, y = (texel / Sd) % Sd, face = texel / (Sd * Sd);
     const f3 N = texel_dir((int)face, (int)x, (int)y, (int)Sd);
     f3 up = make3(0.0f, 1.0f, 0.0f);
     const f3 right = normalize3(cross3(up, N));
     up = normalize3(cross3(N, right));
-    float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f;
-    for (uint32_t ip = 0; ip < np; ++ip) {
-        const float sp = s_sp[ip], cp = s_cp[ip];
-        for (uint32_t it = 0; it < nt; ++it) {
-            const float st = s_st[it], ct = s_ct[it];
+    // summer lane: l < 3 * kDTex -> texel l / 3, channel l % 3
+    const bool summer = l < 3u * kDTex;
+    const uint32_t st_i = l / 3, st_c = l % 3;
+    float acc = 0.0f;
+    const uint32_t rounds = (total + kDSlot - 1) / kDSlot;
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint32_t sidx = r * kDSlot + slot;
+        float (*buf)[kDTex][3] = s_buf[r & 1];
+        if (sidx < total) {
+            const uint32_t ip = sidx / nt, it = sidx % nt;
+            const float sp = s_sp[ip], cp = s_cp[ip], st = s_st[it], ct = s_ct[it];
             const float tx = st * cp, ty = st * sp, tz = ct;
             const f3 sv = add3(add3(mul3(right, tx), mul3(up, ty)), mul3(N, tz));
             const f3 c = sample_cube(cube, (int)S, sv);
-            acc0 = acc0 + (c.x * ct) * st;
-            acc1 = acc1 + (c.y * ct) * st;
-            acc2 = acc2 + (c.z * ct) * st;
+            buf[slot][ti][0] = (c.x * ct) * st;
+            buf[slot][ti][1] = (c.y * ct) * st;
+            buf[slot][ti][2] = (c.z * ct) * st;
+        }
+        __syncthreads();
+        if (summer) {
+            const uint32_t kmax = min((uint32_t)kDSlot, total - r * kDSlot);
+            for (uint32_t k = 0; k < kmax; ++k) acc = acc + buf[k][st_i][st_c];
         }
     }
-    const float inv = 1.0f / (float)(np * nt);
-    out[id] = make_float4((kPiF * acc0) * inv, (kPiF * acc1) * inv, (kPiF * acc2) * inv, 1.0f);
+    if (summer) {
+        const uint32_t t = blockIdx.x * kDTex + st_i;
+        if (t < ntex) {
+            const float inv = 1.0f / (float)total;
+            reinterpret_cast<float*>(out + t)[st_c] = (kPiF * acc) * inv;
+            if (st_c == 0) reinterpret_cast<float*>(out + t)[3] = 1.0f;
+        }
+    }
 }
 
 // ---------------------------------------------------------------- specular
@@ -222,6 +256,29 @@ __global__ __launch_bounds__(256) void specular_kernel(SpecArgs a) {
     const Frame fr = ggx_frame(N);
     const float4* tab = a.table + (size_t)m * a.n;
     float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, wsum = 0.0f;
+    if (m == 0) {
+        // mip 0 has roughness 0 (PipelineCubeFilter.cpp:305-311): cos(theta) = sqrt((1-y)/(1-y)) = 1 and sin(theta) = 0
+        // for every Hammersley sample, so each sample's half vector is
+        // normalize(N + (+-0)) and all n samples produce the same L, NoL and
+        // texel value. Evaluate it once and form the reference's sequential
+        // sums of the n identical terms.
+        const f3 Hh = ggx_world(fr, N, tab[0]);
+        const f3 L = normalize3(sub3(mul3(Hh, 2.0f * dot3(V, Hh)), V));
+        const float NoL = gmax(dot3(N, L), 0.0f);
+        if (NoL > 0.0f) {
+            const f3 c = sample_cube(a.cube, (int)a.S, L);
+            const float t0 = c.x * NoL, t1 = c.y * NoL, t2 = c.z * NoL;
+            for (uint32_t i = 0; i < a.n; ++i) {
+                acc0 = acc0 + t0;
+                acc1 = acc1 + t1;
+                acc2 = acc2 + t2;
+                wsum = wsum + NoL;
+            }
+        }
+        float4* o = a.out + local;
+        *o = make_float4(acc0 / wsum, acc1 / wsum, acc2 / wsum, 1.0f);
+        return;
+    }
     for (uint32_t i = 0; i < a.n; ++i) {
         const f3 Hh = ggx_world(fr, N, tab[i]);
         const f3 L = normalize3(sub3(mul3(Hh, 2.0f * dot3(V, Hh)), V));
@@ -267,8 +324,8 @@ hipError_t launch_cube_mips(float* d_chain, uint32_t S, uint32_t levels, hipStre
 
 hipError_t launch_diffuse(const float* d_cube, uint32_t S, float* d_out, uint32_t Sd, hipStream_t s) {
     const uint32_t T = 6u * Sd * Sd;
-    hipLaunchKernelGGL(diffuse_kernel, dim3((T + 63) / 64), dim3(64), 0, s, reinterpret_cast<const float4*>(d_cube), S,
-                       reinterpret_cast<float4*>(d_out), Sd);
+    hipLaunchKernelGGL(diffuse_kernel, dim3((T + kDTex - 1) / kDTex), dim3(256), 0, s,
+                       reinterpret_cast<const float4*>(d_cube), S, reinterpret_cast<float4*>(d_out), Sd);
     return hipGetLastError();
 }
 

@@ -31,6 +31,17 @@
 #include "hvx_device.h"
 #include "hvx_internal.h"
 
+// build-time tuning switches (defaults are the tuned configuration)
+#ifndef HVX_SHADE_ONE_RCP
+#define HVX_SHADE_ONE_RCP 1
+#endif
+#ifndef HVX_SHADE_XCD_REMAP  // measured 4% slower on MI355X (4.81 vs 4.60 ms per frame): off
+#define HVX_SHADE_XCD_REMAP 0
+#endif
+#ifndef HVX_SHADE_WAVES_PER_EU
+#define HVX_SHADE_WAVES_PER_EU 4
+#endif
+
 namespace hvx {
 namespace {
 
@@ -176,31 +187,50 @@ __device__ __forceinline__ void shade_one(const ShadeArgs& a, uint32_t idx, uint
             q[c] = (oneMinusMetal * alb[c]) * (1.0f / kPiF);
             om[c] = 1.0f - F0[c];
         }
+        // software-pipelined: the next light's record is in flight while the
+        // current one is shaded
+        float4 An = make_float4(0.f, 0.f, 0.f, 0.f), Bn = An;
+        if (count > 0) {
+            const uint32_t li = a.indices[off];
+            An = a.lpos[li];
+            Bn = a.lcol[li];
+        }
         for (uint32_t k = 0; k < count; ++k) {
-            const uint32_t li = a.indices[off + k];
-            const float4 A = a.lpos[li];
-            const float4 B = a.lcol[li];
-            // exact: L, H and the cosines
+            const float4 A = An, B = Bn;
+            if (k + 1 < count) {
+                const uint32_t li = a.indices[off + k + 1];
+                An = a.lpos[li];
+                Bn = a.lcol[li];
+            }
+            // exact: L, H and the cosines (max with 0 only differs in the sign
+            // of a zero, which no later term can observe)
             const f3 d = {A.x - wp.x, A.y - wp.y, A.z - wp.z};
-            const float s = sqrtf_(dot3(d, d));
-            const f3 L = mul3(d, 1.0f / s);
-            const f3 Hh = normalize3(add3(V, L));
-            const float NoH = gmax(dot3(N, Hh), 0.0f);
-            const float NoL = gmax(dot3(N, L), 0.0f);
-            const float HoV = gmax(dot3(Hh, V), 0.0f);
+            const float s = sqrt_rn_normal(dot3(d, d));
+            const f3 L = mul3(d, rcp_rn_normal(s));
+            const f3 h = add3(V, L);
+            const f3 Hh = mul3(h, rcp_rn_normal(sqrt_rn_normal(dot3(h, h))));
+            const float NoH = fmaxf(dot3(N, Hh), 0.0f);
+            const float NoL = fmaxf(dot3(N, L), 0.0f);
+            const float HoV = fmaxf(dot3(Hh, V), 0.0f);
             // fast: attenuation, D, G, Fresnel and the BRDF sum
-            const float att = gmax(__builtin_fmaf(-s, A.w, 1.0f), 0.0f) *
+            const float att = fmaxf(__builtin_fmaf(-s, A.w, 1.0f), 0.0f) *
                               __builtin_amdgcn_exp2f(negf * __builtin_amdgcn_logf(s));
             // den = NoH^2 (a^2 - 1) + 1 exactly as the oracle rounds it: near
             // N.H = 1 it cancels down to ~a^2, so one rounding more or less
             // moves D by up to 1e-7 / a^2
             const float den = (NoH * NoH) * a2m1 + 1.0f;
-            const float D = alpha2 * rcp_fast((PI * den) * den);
-            const float GL = NoL * rcp_fast(__builtin_fmaf(NoL, omk, alphaRough));
-            const float x = gclamp(1.0f - HoV, 0.0f, 1.0f);
+            // D * G_L * G_V / (4 NoV NoL + 1e-4) with one reciprocal of the
+            // product of the three denominators (each bounded away from 0)
+            const float gden = __builtin_fmaf(NoL, omk, alphaRough);
+            const float sden = __builtin_fmaf(nov4, NoL, 0.0001f);
+            const float x = fmaxf(1.0f - HoV, 0.0f);  // HoV >= 0 already
             const float x2 = x * x;
             const float p5 = (x2 * x2) * x;
-            const float DGr = (D * GL) * (GV * rcp_fast(__builtin_fmaf(nov4, NoL, 0.0001f)));
+#if HVX_SHADE_ONE_RCP
+            const float DGr = (alpha2 * NoL * GV) * rcp_fast((((PI * den) * den) * gden) * sden);
+#else
+            const float DGr = (alpha2 * rcp_fast((PI * den) * den)) * (NoL * rcp_fast(gden)) * (GV * rcp_fast(sden));
+#endif
             const float w = att * NoL;
             const float col[3] = {B.x, B.y, B.z};
 #pragma unroll
@@ -232,17 +262,36 @@ __device__ __forceinline__ void shade_one(const ShadeArgs& a, uint32_t idx, uint
     }
 }
 
-constexpr int kChunk = 1024;   // fragments per workgroup
+#ifndef HVX_SHADE_CHUNK
+#define HVX_SHADE_CHUNK 1024
+#endif
+constexpr int kChunk = HVX_SHADE_CHUNK;   // fragments per workgroup
 constexpr int kBuckets = 64;   // light-count classes (count / 2, saturating)
+constexpr uint32_t kXcds = 8;  // MI355X accelerator complex dies
+
 
 template <bool EXACT>
-__global__ __launch_bounds__(256) void shade_kernel(ShadeArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_WAVES_PER_EU, 8))) void shade_kernel(
+    ShadeArgs a) {
     __shared__ uint32_t s_off[kChunk];
     __shared__ uint16_t s_cnt[kChunk];
     __shared__ uint16_t s_order[kChunk];
     __shared__ uint32_t s_hist[kBuckets];
     const uint32_t nfrag = a.W * a.H;
-    const uint32_t base = blockIdx.x * kChunk;
+    // Optional XCD-aware chunk order: workgroups are dealt to the 8 XCDs
+    // round-robin, so the remap would give XCD k a contiguous run of chunks (a
+    // horizontal screen band) to confine its L2 working set. Measured slower
+    // than the plain order for this workload, so it is compiled out by default.
+    const uint32_t nblk = gridDim.x, per = (nblk + kXcds - 1) / kXcds;
+    const uint32_t xcd = blockIdx.x % kXcds, slot = blockIdx.x / kXcds;
+    const uint32_t full = nblk % kXcds == 0 ? kXcds : nblk % kXcds;  // XCDs with `per` blocks
+#if HVX_SHADE_XCD_REMAP
+    const uint32_t chunk = xcd < full ? xcd * per + slot : full * per + (xcd - full) * (per - 1) + slot;
+#else
+    const uint32_t chunk = blockIdx.x;
+    (void)xcd; (void)slot; (void)full;
+#endif
+    const uint32_t base = chunk * kChunk;
     const uint32_t t = threadIdx.x;
     if (t < kBuckets) s_hist[t] = 0;
     __syncthreads();
