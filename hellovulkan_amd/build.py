@@ -86,12 +86,27 @@ def build_test_kernels(verbose: bool = False) -> str:
     return TEST_KERNELS_LIB
 
 
+FRAME_SRC = os.path.join(PKG, "host", "hvx_frame.cpp")
+FRAME_BIN = os.path.join(PKG, "bin", "hvx_frame")
+
+
+def build_frame_driver(verbose: bool = False) -> str:
+    """The C++20 frame driver (include/hvx/pipelines.hpp over libhvx.so)."""
+    os.makedirs(os.path.dirname(FRAME_BIN), exist_ok=True)
+    tmp = FRAME_BIN + ".tmp"
+    _run([HIPCC, "-std=c++20", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), FRAME_SRC, "-L", PKG, "-lhvx",
+          "-Wl,-rpath,$ORIGIN/..", "-o", tmp], verbose)
+    os.replace(tmp, FRAME_BIN)
+    return FRAME_BIN
+
+
 def build(verbose: bool = False) -> None:
     with cf.ThreadPoolExecutor(max_workers=3) as ex:
         futs = [ex.submit(build_product, verbose), ex.submit(build_oracle, verbose),
                 ex.submit(build_test_kernels, verbose)]
         for f in futs:
             f.result()
+    build_frame_driver(verbose)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,77 @@
+"""The C++20 host layer (include/hvx/pipelines.hpp) driven by the frame
+driver hellovulkan_amd/bin/hvx_frame: its light lists and shaded frame must
+equal the same scene run through the C ABI from Python, bit for bit, and the
+hipGraph replay of the frame must equal the eager frame."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from hellovulkan_amd import abi
+from oracle import hvx_oracle as O
+from tests.helpers import camera, to_dev
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "hellovulkan_amd", "bin", "hvx_frame")
+
+W, H, GRID, NL, CAP = 320, 200, (16, 9, 24), 512, 64
+CUBE, DIFF, SPEC, MIPS, LUT = 64, 8, 32, 6, 64
+
+
+def test_frame_driver_matches_c_abi(ctx, tmp_path):
+    import torch
+    prefix = str(tmp_path / "frame")
+    cmd = [BIN, "--width", str(W), "--height", str(H), "--grid", *map(str, GRID), "--lights", str(NL), "--cap",
+           str(CAP), "--cube", str(CUBE), "--diffuse", str(DIFF), "--specular", str(SPEC), "--mips", str(MIPS),
+           "--lut", str(LUT), "--frames", "3", "--warmup", "1", "--dump", prefix]
+    print("running", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    print(info)
+    assert info["graph_matches_eager"] is True and info["graph_nodes"] >= 2
+
+    # the same scene through the C ABI from Python
+    desc, cam, ubo, ivp = camera(W, H, GRID)
+    lights = abi.scene_lights(1001, NL)
+    fr = abi.scene_fragments(1002, desc, ivp, threads=4)
+    eq = abi.scene_equirect(77, 2 * CUBE, CUBE, threads=4)
+    dev = torch.device("cuda", 0)
+    d_cube = torch.empty(abi.cube_chain_floats(CUBE, abi.mip_count(CUBE)), dtype=torch.float32, device=dev)
+    ctx.equirect_to_cube(to_dev(eq), 2 * CUBE, CUBE, d_cube, CUBE)
+    ctx.cube_mips(d_cube, CUBE, abi.mip_count(CUBE))
+    d_diff = torch.empty(6 * DIFF * DIFF * 4, dtype=torch.float32, device=dev)
+    d_spec = torch.empty(abi.cube_chain_floats(SPEC, MIPS), dtype=torch.float32, device=dev)
+    d_lut = torch.empty(LUT * LUT * 2, dtype=torch.float32, device=dev)
+    ctx.filter(abi.FILTER_DIFFUSE, d_cube, CUBE, d_diff, DIFF, 1, 0)
+    ctx.filter(abi.FILTER_SPECULAR, d_cube, CUBE, d_spec, SPEC, MIPS, 1024)
+    ctx.brdf_lut(LUT, LUT, 1024, d_lut)
+    Cn = GRID[0] * GRID[1] * GRID[2]
+    d_aabb = torch.empty(Cn * 8, dtype=torch.float32, device=dev)
+    ctx.build_aabbs(ubo, d_aabb)
+    d_lights = to_dev(lights.view(np.uint8))
+    d_cells = torch.zeros(Cn * 2, dtype=torch.int32, device=dev)
+    d_idx = torch.zeros(Cn * CAP, dtype=torch.int32, device=dev)
+    d_tot = torch.zeros(1, dtype=torch.int32, device=dev)
+    ctx.cull_lights(ubo, d_aabb, d_lights, NL, CAP, d_cells, d_idx, Cn * CAP, d_tot)
+    d_fr = {k: to_dev(v) for k, v in fr.items()}
+    m = abi.ibl_maps(d_spec, SPEC, MIPS, d_diff, DIFF, d_lut, LUT, LUT)
+    rgb = torch.empty(3 * W * H, dtype=torch.float32, device=dev)
+    ctx.shade(abi.frag_soa(d_fr, W, H, ivp), cam, ubo, abi.pbr_defaults(), d_lights, NL, d_cells, d_idx, m, rgb)
+    torch.cuda.synchronize()
+
+    tot = int(d_tot.item())
+    assert info["list_entries"] == tot
+    cells = np.fromfile(prefix + ".cells", np.uint32)
+    idx = np.fromfile(prefix + ".indices", np.uint32)
+    assert np.array_equal(cells, d_cells.cpu().numpy().view(np.uint32))
+    assert np.array_equal(idx, d_idx.cpu().numpy().view(np.uint32)[:tot])
+    ref_cells, ref_idx, ref_tot, _ = O.cull_lights(ubo, O.build_aabbs(ubo), lights, CAP)
+    assert ref_tot == tot and np.array_equal(idx, ref_idx)
+    got = np.fromfile(prefix + ".rgb", np.float32)
+    assert np.array_equal(got, rgb.cpu().numpy())
+    assert np.array_equal(got, np.fromfile(prefix + ".rgb_graph", np.float32))
