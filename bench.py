@@ -46,6 +46,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-ibl", action="store_true", help="skip the one-off IBL stage timings")
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample duration")
+    p.add_argument("--no-batch", action="store_true", help="skip the batched probes + views workload")
+    p.add_argument("--batch-probes", type=int, default=256)
     p.add_argument("--calibrate", action="store_true",
                    help="run one 256 MiB elementwise copy first (known HBM traffic, for counter scaling)")
     return p.parse_args()
@@ -214,10 +216,113 @@ def main():
         "roofline": roofline,
     }
 
+    if not args.no_batch:
+        result["batch"] = batch_bench(ctx, dev, stream, args.batch_probes)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(args, ubo, cam, desc, ivp, lights, fr, d_spec, d_diff, d_lut, F, pc)
     print(json.dumps(result), flush=True)
     ctx.close()
+
+
+def batch_bench(ctx, dev, stream, n_probes):
+    """The batched workload: n_probes procedural environment probes with cube
+    faces drawn uniformly from {64,128,256,512}, each GGX-prefiltered
+    (output face S/2, full mip chain, 1024 samples), and 16 views of the
+    3840x2160 / 32x18x48 frame around the scene with light counts
+    log-uniform in [1024, 65536], culled with cap 128. Timed three ways on
+    the same outputs: per-item entry-point launches, one eager batched run,
+    and replays of the captured batch graph (both branches in one graph)."""
+    import torch
+    from hellovulkan_amd import abi
+    from tests.helpers import TARGET
+    rng = np.random.default_rng(2024)
+    N_SAMPLES, CAP, NV = 1024, 128, 16
+    faces = rng.choice([64, 128, 256, 512], n_probes)
+    t0 = time.time()
+    probes, batch_out, per_item_out, cubes, specs = [], [], [], [], []
+    for k, S in enumerate(faces.tolist()):
+        eq = torch.from_numpy(abi.scene_equirect(5000 + k, 2 * S, S, threads=16)).to(dev)
+        cube = torch.empty(6 * S * S * 4, dtype=torch.float32, device=dev)
+        ctx.equirect_to_cube(eq, 2 * S, S, cube, S)
+        So = S // 2
+        mips = abi.mip_count(So)
+        out = torch.empty(abi.cube_chain_floats(So, mips), dtype=torch.float32, device=dev)
+        ref = torch.empty_like(out)
+        cubes.append(cube)
+        per_item_out.append(ref)
+        specs.append((S, So, mips))
+        batch_out.append(out)
+        probes.append(abi.probe(cube, S, out, So, mips))
+    W, H, grid = 3840, 2160, (32, 18, 48)
+    Cn = grid[0] * grid[1] * grid[2]
+    counts = np.exp(rng.uniform(np.log(1024), np.log(65536), NV)).astype(np.int64).tolist()
+    views, view_items, keep = [], [], []
+    for k, n in enumerate(counts):
+        ang = 2 * np.pi * k / NV
+        eye = (TARGET[0] + 6.0 * np.sin(ang), 1.0, TARGET[2] + 6.0 * np.cos(ang))
+        desc, cam, ubo, ivp = abi.camera_build(eye, TARGET, 0.785398, 0.1, 100.0, W, H, grid)
+        aabb = torch.empty(Cn * 8, dtype=torch.float32, device=dev)
+        ctx.build_aabbs(ubo, aabb)
+        lights = torch.from_numpy(abi.scene_lights(3000 + k, n).view(np.uint8)).to(dev)
+        bufs = [torch.empty(Cn * 2, dtype=torch.int32, device=dev), torch.empty(Cn * CAP, dtype=torch.int32, device=dev),
+                torch.zeros(1, dtype=torch.int32, device=dev)]
+        ref = [torch.empty_like(b) for b in bufs]
+        keep += [aabb, lights, *bufs, *ref]
+        views.append(abi.view(ubo, aabb, lights, n, CAP, bufs[0], bufs[1], Cn * CAP, bufs[2]))
+        view_items.append((ubo, aabb, lights, n, bufs, ref))
+    torch.cuda.synchronize()
+    log(f"batch inputs ready in {time.time() - t0:.1f}s ({n_probes} probes, {NV} views)")
+
+    def per_item():
+        for cube, ref, (S, So, mips) in zip(cubes, per_item_out, specs):
+            ctx.filter(abi.FILTER_SPECULAR, cube, S, ref, So, mips, N_SAMPLES)
+        for ubo, aabb, lights, n, bufs, ref in view_items:
+            ctx.cull_lights(ubo, aabb, lights, n, CAP, ref[0], ref[1], Cn * CAP, ref[2])
+
+    def ev_time(fn, s, reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        fn()
+        torch.cuda.synchronize()
+        e0.record(s)
+        for _ in range(reps):
+            fn()
+        e1.record(s)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    per_item_ms = ev_time(per_item, stream, 2)
+    b = abi.Batch(ctx, probes, N_SAMPLES, views)
+    t0 = time.time()
+    tun = b.autotune(stream)
+    tune_s = time.time() - t0
+    log(f"autotuned in {tune_s:.1f}s: tile {tun.probe_tile} x {tun.probe_wgs_per_cu}/CU, "
+        f"cull shape {tun.cull_shape} x {tun.cull_wgs_per_cu}/CU")
+    eager_ms = ev_time(lambda: b.run(stream), stream, 2)
+    gs = torch.cuda.Stream(dev)
+    b.capture(gs)
+    graph_ms = ev_time(lambda: b.launch(gs), gs, 3)
+    same = all(torch.equal(o.view(torch.int32), r.view(torch.int32)) for o, r in zip(batch_out, per_item_out))
+    for *_, bufs, ref in view_items:
+        t = int(ref[2].item())
+        same = same and int(bufs[2].item()) == t and torch.equal(bufs[0], ref[0]) and torch.equal(bufs[1][:t], ref[1][:t])
+    texels = sum(6 * sum(max(So >> m, 1) ** 2 for m in range(mips)) for _, So, mips in specs)
+    b.close()
+    return {
+        "workload": f"{n_probes} probes (faces {{64,128,256,512}} -> GGX prefilter S/2, full chain, 1024 samples) "
+                    f"+ {NV} views 3840x2160 32x18x48 (lights log-uniform [1024,65536], cap 128)",
+        "probe_faces": {str(f): int((faces == f).sum()) for f in (64, 128, 256, 512)},
+        "view_lights": counts,
+        "output_texels": texels,
+        "per_item_ms": round(per_item_ms, 3),
+        "batched_eager_ms": round(eager_ms, 3),
+        "batched_graph_ms": round(graph_ms, 3),
+        "speedup_graph_vs_per_item": round(per_item_ms / graph_ms, 3),
+        "tuning": {"probe_tile": tun.probe_tile, "probe_wgs_per_cu": tun.probe_wgs_per_cu,
+                   "cull_shape": ["4x4x1", "8x4x1", "4x4x2", "2x2x4"][tun.cull_shape],
+                   "cull_wgs_per_cu": tun.cull_wgs_per_cu, "probe_ms": round(tun.probe_ms, 3),
+                   "cull_ms": round(tun.cull_ms, 3), "autotune_s": round(tune_s, 1)},
+        "outputs_equal_per_item": bool(same),
+    }
 
 
 def cpu_baseline(args, ubo, cam, desc, ivp, lights, fr, d_spec, d_diff, d_lut, F, pc):

@@ -94,6 +94,22 @@ class CameraDesc(C.Structure):
                 ("slice_count_x", C.c_uint32), ("slice_count_y", C.c_uint32), ("slice_count_z", C.c_uint32)]
 
 
+class Probe(C.Structure):
+    _fields_ = [("cube", C.c_void_p), ("face_size", C.c_uint32), ("out_face_size", C.c_uint32),
+                ("out_mips", C.c_uint32), ("_pad", C.c_uint32), ("out_chain", C.c_void_p)]
+
+
+class View(C.Structure):
+    _fields_ = [("ubo", CFUBO), ("clusters", C.c_void_p), ("lights", C.c_void_p), ("n_lights", C.c_uint32),
+                ("cap", C.c_uint32), ("cells", C.c_void_p), ("indices", C.c_void_p),
+                ("indices_capacity", C.c_size_t), ("total", C.c_void_p)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("probe_tile", C.c_uint32), ("probe_wgs_per_cu", C.c_uint32), ("cull_shape", C.c_uint32),
+                ("cull_wgs_per_cu", C.c_uint32), ("probe_ms", C.c_float), ("cull_ms", C.c_float)]
+
+
 # numpy views of the array-of-struct layouts
 LIGHT_DTYPE = np.dtype([("position", "<f4", 4), ("color", "<f4", 4), ("radius", "<f4"), ("_pad", "<f4", 3)])
 AABB_DTYPE = np.dtype([("min_point", "<f4", 4), ("max_point", "<f4", 4)])
@@ -127,6 +143,14 @@ SIGNATURES = [
     ("hvx_cube_chain_floats", C.c_size_t, [u32, u32]),
     ("hvx_cube_level_offset_floats", C.c_size_t, [u32, u32]),
     ("hvx_mip_count", u32, [u32]),
+    ("hvx_batch_create", C.c_int, [vp, P(Probe), u32, u32, P(View), u32, P(vp)]),
+    ("hvx_batch_destroy", C.c_int, [vp]),
+    ("hvx_batch_set_tuning", C.c_int, [vp, P(BatchTuning)]),
+    ("hvx_batch_get_tuning", C.c_int, [vp, P(BatchTuning)]),
+    ("hvx_batch_autotune", C.c_int, [vp, vp]),
+    ("hvx_batch_run", C.c_int, [vp, vp]),
+    ("hvx_batch_capture", C.c_int, [vp, vp]),
+    ("hvx_batch_launch", C.c_int, [vp, vp]),
     ("hvx_camera_build", C.c_int, [P(CameraDesc), P(CameraUBO), P(CFUBO), P(C.c_float)]),
     ("hvx_scene_lights", C.c_int, [C.c_uint64, u32, vp]),
     ("hvx_scene_fragments", C.c_int, [C.c_uint64, P(CameraDesc), P(C.c_float), vp, vp, vp, vp, vp, vp, vp, vp,
@@ -308,6 +332,69 @@ class Context:
         opts.emulate_static_viewport = 1 if emulate_static_viewport else 0
         self._check(self.lib.hvx_ibl_filter(self.h, ftype, _ptr(cube), face, _ptr(out), out_face, out_mips, samples,
                                             C.byref(opts), self._stream(stream)), "hvx_ibl_filter")
+
+
+class Batch:
+    """Owns an hvx_batch: probes (GGX prefilter) and views (light culling) run
+    by the batched persistent kernels. `probes` is a list of Probe, `views` a
+    list of View; the referenced device buffers must outlive the batch."""
+
+    def __init__(self, ctx: Context, probes, sample_count: int, views):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        pa = (Probe * max(len(probes), 1))(*probes)
+        va = (View * max(len(views), 1))(*views)
+        h = C.c_void_p()
+        ctx._check(self.lib.hvx_batch_create(ctx.h, pa, len(probes), sample_count, va, len(views), C.byref(h)),
+                   "hvx_batch_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.hvx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def tuning(self) -> BatchTuning:
+        t = BatchTuning()
+        self.ctx._check(self.lib.hvx_batch_get_tuning(self.h, C.byref(t)), "hvx_batch_get_tuning")
+        return t
+
+    def set_tuning(self, probe_tile: int, probe_wgs_per_cu: int, cull_shape: int, cull_wgs_per_cu: int):
+        t = BatchTuning(probe_tile, probe_wgs_per_cu, cull_shape, cull_wgs_per_cu, 0.0, 0.0)
+        self.ctx._check(self.lib.hvx_batch_set_tuning(self.h, C.byref(t)), "hvx_batch_set_tuning")
+
+    def autotune(self, stream=None) -> BatchTuning:
+        self.ctx._check(self.lib.hvx_batch_autotune(self.h, self.ctx._stream(stream)), "hvx_batch_autotune")
+        return self.tuning
+
+    def run(self, stream=None):
+        self.ctx._check(self.lib.hvx_batch_run(self.h, self.ctx._stream(stream)), "hvx_batch_run")
+
+    def capture(self, stream):
+        self.ctx._check(self.lib.hvx_batch_capture(self.h, self.ctx._stream(stream)), "hvx_batch_capture")
+
+    def launch(self, stream):
+        self.ctx._check(self.lib.hvx_batch_launch(self.h, self.ctx._stream(stream)), "hvx_batch_launch")
+
+
+def probe(cube, face_size: int, out_chain, out_face_size: int, out_mips: int) -> Probe:
+    return Probe(_ptr(cube), face_size, out_face_size, out_mips, 0, _ptr(out_chain))
+
+
+def view(ubo: CFUBO, clusters, lights, n_lights: int, cap: int, cells, indices, indices_capacity: int,
+         total=None) -> View:
+    v = View()
+    v.ubo = ubo
+    v.clusters, v.lights, v.n_lights, v.cap = _ptr(clusters), _ptr(lights), n_lights, cap
+    v.cells, v.indices, v.indices_capacity, v.total = _ptr(cells), _ptr(indices), indices_capacity, _ptr(total)
+    return v
 
 
 def frag_soa(planes: dict, width: int, height: int, ivp) -> FragSoA:

@@ -95,10 +95,7 @@ __device__ __forceinline__ bool sphere_box(float4 p, const Box& b) {
     return sq < p.w;
 }
 
-constexpr int kGX = 4, kGY = 4, kGZ = 1;
-constexpr int kG = kGX * kGY * kGZ;       // clusters per workgroup
 constexpr int kWaves = 4;                 // 256 lanes
-constexpr int kPerWave = kG / kWaves;     // clusters per wave
 constexpr int kTile = 256;                // lights per level-1 step
 constexpr int kFlush = 256;               // staged candidates that trigger level 2
 constexpr int kCandMax = kFlush + kTile;
@@ -107,28 +104,41 @@ __device__ __forceinline__ uint32_t lane_prefix(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-__global__ __launch_bounds__(256) void cull_kernel(hvx_cf_ubo u, const hvx_aabb* __restrict__ clusters,
-                                                   const float4* __restrict__ lvs, uint32_t n, uint32_t cap,
-                                                   int want_overflow, uint32_t* __restrict__ lists,
-                                                   uint32_t* __restrict__ counts, uint32_t* __restrict__ over) {
-    __shared__ float4 s_pos[kCandMax];
-    __shared__ uint32_t s_idx[kCandMax];
-    __shared__ uint32_t s_wcnt[kWaves];
-    __shared__ uint32_t s_wdone[kWaves];
+template <int GX, int GY, int GZ> struct CullShape {
+    static constexpr int kG = GX * GY * GZ;
+    static constexpr int kPerWave = kG / kWaves;
+    static_assert(kG % kWaves == 0, "clusters per workgroup must split evenly over 4 waves");
+};
 
-    const uint32_t X = u.slice_count_x, Y = u.slice_count_y, Z = u.slice_count_z;
-    const uint32_t ngx = (X + kGX - 1) / kGX, ngy = (Y + kGY - 1) / kGY;
-    const uint32_t g = blockIdx.x;
+template <int GX, int GY, int GZ> __host__ __device__ inline uint32_t cull_groups(uint32_t X, uint32_t Y, uint32_t Z) {
+    return ((X + GX - 1) / GX) * ((Y + GY - 1) / GY) * ((Z + GZ - 1) / GZ);
+}
+
+struct CullShared {
+    float4 pos[kCandMax];
+    uint32_t idx[kCandMax];
+    uint32_t wcnt[kWaves];
+    uint32_t wdone[kWaves];
+};
+
+// Culls the GX x GY x GZ block of clusters `g` of job J (see the file header).
+template <int GX, int GY, int GZ>
+__device__ __forceinline__ void cull_group(const CullJob& J, uint32_t g, CullShared& sh) {
+    using Shape = CullShape<GX, GY, GZ>;
+    constexpr int kG = Shape::kG, kPerWave = Shape::kPerWave;
+    const uint32_t X = J.X, Y = J.Y, Z = J.Z, n = J.n, cap = J.cap;
+    const uint32_t ngx = (X + GX - 1) / GX, ngy = (Y + GY - 1) / GY;
     const uint32_t gx = g % ngx, gy = (g / ngx) % ngy, gz = g / (ngx * ngy);
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __syncthreads();  // LDS reuse across consecutive groups of a persistent workgroup
 
     // union box of the valid clusters of this block (uniform)
     Box U;
     for (int a = 0; a < 3; ++a) { U.mn[a] = hvx::kInfF; U.mx[a] = -hvx::kInfF; }
     for (int k = 0; k < kG; ++k) {
-        const uint32_t cx = gx * kGX + (k % kGX), cy = gy * kGY + (k / kGX) % kGY, cz = gz * kGZ + k / (kGX * kGY);
+        const uint32_t cx = gx * GX + (k % GX), cy = gy * GY + (k / GX) % GY, cz = gz * GZ + k / (GX * GY);
         if (cx >= X || cy >= Y || cz >= Z) continue;
-        const hvx_aabb& b = clusters[cx + cy * X + cz * X * Y];
+        const hvx_aabb& b = J.clusters[cx + cy * X + cz * X * Y];
         for (int a = 0; a < 3; ++a) {
             U.mn[a] = gmin(U.mn[a], b.min_point[a]);
             U.mx[a] = gmax(U.mx[a], b.max_point[a]);
@@ -142,14 +152,14 @@ __global__ __launch_bounds__(256) void cull_kernel(hvx_cf_ubo u, const hvx_aabb*
 #pragma unroll
     for (int j = 0; j < kPerWave; ++j) {
         const int k = (int)wave * kPerWave + j;
-        const uint32_t cx = gx * kGX + (k % kGX), cy = gy * kGY + (k / kGX) % kGY, cz = gz * kGZ + k / (kGX * kGY);
+        const uint32_t cx = gx * GX + (k % GX), cy = gy * GY + (k / GX) % GY, cz = gz * GZ + k / (GX * GY);
         const bool valid = cx < X && cy < Y && cz < Z;
         cid[j] = valid ? cx + cy * X + cz * X * Y : 0u;
         cnt[j] = 0;
         ovf[j] = 0;
         done[j] = !valid;
         if (valid) {
-            const hvx_aabb& b = clusters[cid[j]];
+            const hvx_aabb& b = J.clusters[cid[j]];
             for (int a = 0; a < 3; ++a) { cb[j].mn[a] = b.min_point[a]; cb[j].mx[a] = b.max_point[a]; }
         } else {
             for (int a = 0; a < 3; ++a) { cb[j].mn[a] = 0.0f; cb[j].mx[a] = 0.0f; }
@@ -163,23 +173,23 @@ __global__ __launch_bounds__(256) void cull_kernel(hvx_cf_ubo u, const hvx_aabb*
         bool hit = false;
         float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
         if (li < n) {
-            p = lvs[li];
+            p = J.lvs[li];
             hit = sphere_box(p, U);
         }
         const uint64_t m = __ballot(hit);
-        if (lane == 0) s_wcnt[wave] = (uint32_t)__popcll(m);
+        if (lane == 0) sh.wcnt[wave] = (uint32_t)__popcll(m);
         __syncthreads();
         uint32_t woff = 0, tot = 0;
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) {
-            const uint32_t c = s_wcnt[w];
+            const uint32_t c = sh.wcnt[w];
             woff += (w < (int)wave) ? c : 0u;
             tot += c;
         }
         if (hit) {
             const uint32_t pos = ncand + woff + lane_prefix(m);
-            s_pos[pos] = p;
-            s_idx[pos] = li;
+            sh.pos[pos] = p;
+            sh.idx[pos] = li;
         }
         __syncthreads();
         ncand += tot;
@@ -194,18 +204,18 @@ __global__ __launch_bounds__(256) void cull_kernel(hvx_cf_ubo u, const hvx_aabb*
                 for (uint32_t k0 = 0; k0 < ncand; k0 += 64) {
                     const uint32_t k = k0 + lane;
                     bool h = false;
-                    if (k < ncand) h = sphere_box(s_pos[k], cb[j]);
+                    if (k < ncand) h = sphere_box(sh.pos[k], cb[j]);
                     const uint64_t mm = __ballot(h);
                     if (cnt[j] < cap) {
                         const uint32_t pos = cnt[j] + lane_prefix(mm);
-                        if (h && pos < cap) lists[(size_t)cid[j] * cap + pos] = s_idx[k];
+                        if (h && pos < cap) J.lists[(size_t)cid[j] * cap + pos] = sh.idx[k];
                         const uint32_t nh = (uint32_t)__popcll(mm);
                         if (cnt[j] + nh > cap) ovf[j] = 1;
                         cnt[j] = min(cnt[j] + nh, cap);
                     } else if (mm != 0) {
                         ovf[j] = 1;
                     }
-                    if (cnt[j] >= cap && (ovf[j] || !want_overflow)) {
+                    if (cnt[j] >= cap && (ovf[j] || !J.want_overflow)) {
                         done[j] = true;
                         break;
                     }
@@ -213,12 +223,12 @@ __global__ __launch_bounds__(256) void cull_kernel(hvx_cf_ubo u, const hvx_aabb*
             }
             wave_done = wave_done && done[j];
         }
-        if (lane == 0) s_wdone[wave] = wave_done ? 1u : 0u;
+        if (lane == 0) sh.wdone[wave] = wave_done ? 1u : 0u;
         __syncthreads();
         ncand = 0;
         bool all_done = true;
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w) all_done = all_done && (s_wdone[w] != 0);
+        for (int w = 0; w < kWaves; ++w) all_done = all_done && (sh.wdone[w] != 0);
         if (all_done || base >= n) break;
     }
 
@@ -226,13 +236,18 @@ __global__ __launch_bounds__(256) void cull_kernel(hvx_cf_ubo u, const hvx_aabb*
 #pragma unroll
         for (int j = 0; j < kPerWave; ++j) {
             const int k = (int)wave * kPerWave + j;
-            const uint32_t cx = gx * kGX + (k % kGX), cy = gy * kGY + (k / kGX) % kGY, cz = gz * kGZ + k / (kGX * kGY);
+            const uint32_t cx = gx * GX + (k % GX), cy = gy * GY + (k / GX) % GY, cz = gz * GZ + k / (GX * GY);
             if (cx < X && cy < Y && cz < Z) {
-                counts[cid[j]] = cnt[j];
-                over[cid[j]] = ovf[j];
+                J.counts[cid[j]] = cnt[j];
+                J.over[cid[j]] = ovf[j];
             }
         }
     }
+}
+
+template <int GX, int GY, int GZ> __global__ __launch_bounds__(256) void cull_kernel(CullJob J) {
+    __shared__ CullShared sh;
+    cull_group<GX, GY, GZ>(J, blockIdx.x, sh);
 }
 
 // Exclusive scan of the counts in cluster order -> LightCell offsets (the
@@ -267,11 +282,11 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* s_w
     return woff + inc - v;
 }
 
-__global__ __launch_bounds__(256) void scan_partials_kernel(const uint32_t* __restrict__ counts,
-                                                            const uint32_t* __restrict__ over, uint32_t C,
-                                                            uint2* __restrict__ part) {
-    __shared__ uint32_t s_w[4], s_v[4];
-    const uint32_t b0 = blockIdx.x * kScanBlock;
+__device__ __forceinline__ void scan_partials_block(const uint32_t* __restrict__ counts,
+                                                    const uint32_t* __restrict__ over, uint32_t C,
+                                                    uint2* __restrict__ part, uint32_t blk, uint32_t* s_w,
+                                                    uint32_t* s_v) {
+    const uint32_t b0 = blk * kScanBlock;
     uint32_t sum = 0, ov = 0;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -284,15 +299,22 @@ __global__ __launch_bounds__(256) void scan_partials_kernel(const uint32_t* __re
     uint32_t tsum, tov;
     block_excl_scan256(sum, s_w, &tsum);
     block_excl_scan256(ov, s_v, &tov);
-    if (threadIdx.x == 0) part[blockIdx.x] = make_uint2(tsum, tov);
+    if (threadIdx.x == 0) part[blk] = make_uint2(tsum, tov);
 }
 
-__global__ __launch_bounds__(1024) void scan_top_kernel(uint2* __restrict__ part, uint32_t nb,
-                                                        uint32_t* __restrict__ total, uint32_t* __restrict__ novf) {
-    __shared__ uint32_t s_w[16], s_o[16];
-    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+__global__ __launch_bounds__(256) void scan_partials_kernel(const uint32_t* __restrict__ counts,
+                                                            const uint32_t* __restrict__ over, uint32_t C,
+                                                            uint2* __restrict__ part) {
+    __shared__ uint32_t s_w[4], s_v[4];
+    scan_partials_block(counts, over, C, part, blockIdx.x, s_w, s_v);
+}
+
+// exclusive block offsets over nb partials with a workgroup of 1024 lanes
+__device__ __forceinline__ void scan_top_block(uint2* __restrict__ part, uint32_t nb, uint32_t* __restrict__ total,
+                                               uint32_t* __restrict__ novf, uint32_t* s_w, uint32_t* s_o) {
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6, nw = blockDim.x >> 6;
     uint32_t carry = 0, ocarry = 0;
-    for (uint32_t b0 = 0; b0 < nb; b0 += 1024) {
+    for (uint32_t b0 = 0; b0 < nb; b0 += blockDim.x) {
         const uint32_t b = b0 + t;
         const uint2 p = b < nb ? part[b] : make_uint2(0u, 0u);
         const uint32_t inc = wave_incl_scan(p.x, lane);
@@ -303,7 +325,7 @@ __global__ __launch_bounds__(1024) void scan_top_kernel(uint2* __restrict__ part
         }
         __syncthreads();
         uint32_t woff = 0, tot = 0, otot = 0;
-        for (uint32_t k = 0; k < 16; ++k) {
+        for (uint32_t k = 0; k < nw; ++k) {
             woff += k < w ? s_w[k] : 0u;
             tot += s_w[k];
             otot += s_o[k];
@@ -319,11 +341,16 @@ __global__ __launch_bounds__(1024) void scan_top_kernel(uint2* __restrict__ part
     }
 }
 
-__global__ __launch_bounds__(256) void scan_apply_kernel(const uint32_t* __restrict__ counts, uint32_t C,
-                                                         const uint2* __restrict__ part,
-                                                         hvx_light_cell* __restrict__ cells) {
-    __shared__ uint32_t s_w[4];
-    const uint32_t b0 = blockIdx.x * kScanBlock;
+__global__ __launch_bounds__(1024) void scan_top_kernel(uint2* __restrict__ part, uint32_t nb,
+                                                        uint32_t* __restrict__ total, uint32_t* __restrict__ novf) {
+    __shared__ uint32_t s_w[16], s_o[16];
+    scan_top_block(part, nb, total, novf, s_w, s_o);
+}
+
+__device__ __forceinline__ void scan_apply_block(const uint32_t* __restrict__ counts, uint32_t C,
+                                                 const uint2* __restrict__ part, hvx_light_cell* __restrict__ cells,
+                                                 uint32_t blk, uint32_t* s_w) {
+    const uint32_t b0 = blk * kScanBlock;
     // thread t owns clusters b0 + 4t .. b0 + 4t + 3 (contiguous, in order)
     uint32_t k[4], local = 0;
 #pragma unroll
@@ -333,7 +360,7 @@ __global__ __launch_bounds__(256) void scan_apply_kernel(const uint32_t* __restr
         local += k[r];
     }
     uint32_t tot;
-    uint32_t run = part[blockIdx.x].x + block_excl_scan256(local, s_w, &tot);
+    uint32_t run = part[blk].x + block_excl_scan256(local, s_w, &tot);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const uint32_t c = b0 + threadIdx.x * 4 + r;
@@ -347,13 +374,92 @@ __global__ __launch_bounds__(256) void scan_apply_kernel(const uint32_t* __restr
     }
 }
 
-__global__ __launch_bounds__(256) void scatter_kernel(const uint32_t* __restrict__ lists, uint32_t cap, uint32_t C,
-                                                      const hvx_light_cell* __restrict__ cells,
-                                                      uint32_t* __restrict__ indices) {
-    const uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+__global__ __launch_bounds__(256) void scan_apply_kernel(const uint32_t* __restrict__ counts, uint32_t C,
+                                                         const uint2* __restrict__ part,
+                                                         hvx_light_cell* __restrict__ cells) {
+    __shared__ uint32_t s_w[4];
+    scan_apply_block(counts, C, part, cells, blockIdx.x, s_w);
+}
+
+__device__ __forceinline__ void scatter_group(const uint32_t* __restrict__ lists, uint32_t cap, uint32_t C,
+                                              const hvx_light_cell* __restrict__ cells, uint32_t* __restrict__ indices,
+                                              uint32_t grp) {
+    const uint32_t c = grp * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (c >= C) return;
     const hvx_light_cell cell = cells[c];
     for (uint32_t j = lane; j < cell.count; j += 64) indices[cell.offset + j] = lists[(size_t)c * cap + j];
+}
+
+__global__ __launch_bounds__(256) void scatter_kernel(const uint32_t* __restrict__ lists, uint32_t cap, uint32_t C,
+                                                      const hvx_light_cell* __restrict__ cells,
+                                                      uint32_t* __restrict__ indices) {
+    scatter_group(lists, cap, C, cells, indices, blockIdx.x);
+}
+
+// ------------------------------------------------------ batched (multi-view)
+// Every kernel below covers all views of a batch in one launch; a work item
+// finds its view by the views' running item offsets (at most kMaxViews).
+__device__ __forceinline__ uint32_t view_of(const CullBatchView* __restrict__ V, uint32_t nv, uint32_t item,
+                                            uint32_t CullBatchView::*begin) {
+    uint32_t v = 0;
+    while (v + 1 < nv && item >= V[v + 1].*begin) ++v;
+    return v;
+}
+
+__global__ __launch_bounds__(64) void batch_reset_kernel(uint32_t* __restrict__ q, uint32_t n) {
+    if (threadIdx.x < n) q[threadIdx.x] = 0u;
+}
+
+__global__ __launch_bounds__(256) void batch_light_prep_kernel(const CullBatchView* __restrict__ V, uint32_t nv,
+                                                               uint32_t total) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const CullBatchView& v = V[view_of(V, nv, i, &CullBatchView::light_begin)];
+    const uint32_t li = i - v.light_begin;
+    const hvx_light& l = v.lights[li];
+    float o[4];
+    matvec4(v.view, l.position[0], l.position[1], l.position[2], l.position[3], o);
+    const_cast<float4*>(v.job.lvs)[li] = make_float4(o[0], o[1], o[2], l.radius * l.radius);
+}
+
+// persistent: workgroups pull (view, cluster block) items from a queue
+template <int GX, int GY, int GZ>
+__global__ __launch_bounds__(256) void cull_batch_kernel(const CullBatchView* __restrict__ V, uint32_t nv,
+                                                         uint32_t total_groups, uint32_t* __restrict__ queue) {
+    __shared__ CullShared sh;
+    __shared__ uint32_t s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(queue, 1u);
+        __syncthreads();
+        const uint32_t item = s_item;
+        __syncthreads();
+        if (item >= total_groups) break;
+        const CullBatchView& v = V[view_of(V, nv, item, &CullBatchView::group_begin)];
+        cull_group<GX, GY, GZ>(v.job, item - v.group_begin, sh);
+    }
+}
+
+__global__ __launch_bounds__(256) void batch_scan_partials_kernel(const CullBatchView* __restrict__ V, uint32_t nv) {
+    __shared__ uint32_t s_w[4], s_v[4];
+    const CullBatchView& v = V[view_of(V, nv, blockIdx.x, &CullBatchView::block_begin)];
+    scan_partials_block(v.job.counts, v.job.over, v.C, v.part, blockIdx.x - v.block_begin, s_w, s_v);
+}
+
+__global__ __launch_bounds__(1024) void batch_scan_top_kernel(const CullBatchView* __restrict__ V) {
+    __shared__ uint32_t s_w[16], s_o[16];
+    const CullBatchView& v = V[blockIdx.x];
+    scan_top_block(v.part, v.nb, v.total, v.novf, s_w, s_o);
+}
+
+__global__ __launch_bounds__(256) void batch_scan_apply_kernel(const CullBatchView* __restrict__ V, uint32_t nv) {
+    __shared__ uint32_t s_w[4];
+    const CullBatchView& v = V[view_of(V, nv, blockIdx.x, &CullBatchView::block_begin)];
+    scan_apply_block(v.job.counts, v.C, v.part, v.cells, blockIdx.x - v.block_begin, s_w);
+}
+
+__global__ __launch_bounds__(256) void batch_scatter_kernel(const CullBatchView* __restrict__ V, uint32_t nv) {
+    const CullBatchView& v = V[view_of(V, nv, blockIdx.x, &CullBatchView::scatter_begin)];
+    scatter_group(v.job.lists, v.job.cap, v.C, v.cells, v.indices, blockIdx.x - v.scatter_begin);
 }
 
 }  // namespace
@@ -375,15 +481,59 @@ hipError_t launch_cull(const hvx_cf_ubo& ubo, const hvx_aabb* d_clusters, const 
         hipLaunchKernelGGL(light_prep_kernel, dim3((n_lights + 255) / 256), dim3(256), 0, s, ubo, d_lights, n_lights,
                            light_vs);
     }
-    const uint32_t groups = ((X + kGX - 1) / kGX) * ((Y + kGY - 1) / kGY) * ((Z + kGZ - 1) / kGZ);
-    hipLaunchKernelGGL(cull_kernel, dim3(groups), dim3(256), 0, s, ubo, d_clusters, light_vs, n_lights, cap,
-                       d_overflow != nullptr ? 1 : 0, lists, counts, over);
+    CullJob J;
+    J.X = X;
+    J.Y = Y;
+    J.Z = Z;
+    J.n = n_lights;
+    J.cap = cap;
+    J.want_overflow = d_overflow != nullptr ? 1u : 0u;
+    J.clusters = d_clusters;
+    J.lvs = light_vs;
+    J.lists = lists;
+    J.counts = counts;
+    J.over = over;
+    hipLaunchKernelGGL((cull_kernel<4, 4, 1>), dim3(cull_groups<4, 4, 1>(X, Y, Z)), dim3(256), 0, s, J);
     const uint32_t nb = (C + kScanBlock - 1) / kScanBlock;
     uint2* part = reinterpret_cast<uint2*>(over + C);
     hipLaunchKernelGGL(scan_partials_kernel, dim3(nb), dim3(256), 0, s, counts, over, C, part);
     hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(1024), 0, s, part, nb, d_total, d_overflow);
     hipLaunchKernelGGL(scan_apply_kernel, dim3(nb), dim3(256), 0, s, counts, C, part, d_cells);
     hipLaunchKernelGGL(scatter_kernel, dim3((C + 3) / 4), dim3(256), 0, s, lists, cap, C, d_cells, d_indices);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------ batched launchers
+uint32_t cull_batch_groups(int shape, uint32_t X, uint32_t Y, uint32_t Z) {
+    switch (shape) {
+        case 0: return cull_groups<4, 4, 1>(X, Y, Z);
+        case 1: return cull_groups<8, 4, 1>(X, Y, Z);
+        case 2: return cull_groups<4, 4, 2>(X, Y, Z);
+        default: return cull_groups<2, 2, 4>(X, Y, Z);
+    }
+}
+
+hipError_t launch_cull_batch(const CullBatchView* d_views, uint32_t nv, const CullBatchTotals& t, int shape,
+                             uint32_t persistent_wgs, uint32_t* d_queue, hipStream_t s) {
+    if (t.lights > 0)
+        hipLaunchKernelGGL(batch_light_prep_kernel, dim3((t.lights + 255) / 256), dim3(256), 0, s, d_views, nv,
+                           t.lights);
+    const dim3 grid(persistent_wgs), block(256);
+    switch (shape) {
+        case 0: hipLaunchKernelGGL((cull_batch_kernel<4, 4, 1>), grid, block, 0, s, d_views, nv, t.groups, d_queue); break;
+        case 1: hipLaunchKernelGGL((cull_batch_kernel<8, 4, 1>), grid, block, 0, s, d_views, nv, t.groups, d_queue); break;
+        case 2: hipLaunchKernelGGL((cull_batch_kernel<4, 4, 2>), grid, block, 0, s, d_views, nv, t.groups, d_queue); break;
+        default: hipLaunchKernelGGL((cull_batch_kernel<2, 2, 4>), grid, block, 0, s, d_views, nv, t.groups, d_queue); break;
+    }
+    hipLaunchKernelGGL(batch_scan_partials_kernel, dim3(t.blocks), dim3(256), 0, s, d_views, nv);
+    hipLaunchKernelGGL(batch_scan_top_kernel, dim3(nv), dim3(1024), 0, s, d_views);
+    hipLaunchKernelGGL(batch_scan_apply_kernel, dim3(t.blocks), dim3(256), 0, s, d_views, nv);
+    hipLaunchKernelGGL(batch_scatter_kernel, dim3(t.scatter), dim3(256), 0, s, d_views, nv);
+    return hipGetLastError();
+}
+
+hipError_t launch_queue_reset(uint32_t* d_queues, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, d_queues, n);
     return hipGetLastError();
 }
 

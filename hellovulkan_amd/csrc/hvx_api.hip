@@ -195,6 +195,9 @@ int hvx_cf_cull_lights(hvx_ctx* ctx, const hvx_cf_ubo* ubo, const hvx_aabb* d_cl
     if (misaligned(d_lights, 16) || misaligned(d_clusters, 16) || misaligned(d_cells, 8) || misaligned(d_indices, 4))
         return fail(ctx, HVX_ERR_ALIGN, "lights/clusters need 16-byte, cells 8-byte alignment");
     const uint64_t C = (uint64_t)ubo->slice_count_x * ubo->slice_count_y * ubo->slice_count_z;
+    if (C * cap > 0xffffffffull)
+        return fail(ctx, HVX_ERR_INVALID_ARG, "clusters*cap = %llu exceeds 32-bit list offsets",
+                    (unsigned long long)(C * cap));
     if ((uint64_t)indices_capacity < C * cap)
         return fail(ctx, HVX_ERR_CAPACITY, "indices_capacity %zu < clusters*cap = %llu", indices_capacity,
                     (unsigned long long)(C * cap));

@@ -31,6 +31,48 @@ struct hvx_ctx {
 
 namespace hvx {
 
+// One view's culling work (cf_kernels.hip).
+struct CullJob {
+    uint32_t X, Y, Z;
+    uint32_t n, cap, want_overflow;
+    const hvx_aabb* clusters;
+    const float4* lvs;
+    uint32_t* lists;
+    uint32_t* counts;
+    uint32_t* over;
+};
+
+// A view of a culling batch, device-resident: its CullJob, its inputs and
+// outputs, and the running item offsets that map batched work items to views.
+struct CullBatchView {
+    CullJob job;
+    const hvx_light* lights;
+    float view[16];
+    uint2* part;
+    hvx_light_cell* cells;
+    uint32_t* indices;
+    uint32_t* total;
+    uint32_t* novf;
+    uint32_t C, nb;
+    uint32_t light_begin, group_begin, block_begin, scatter_begin;
+};
+struct CullBatchTotals {
+    uint32_t lights, groups, blocks, scatter;
+};
+
+// One probe of a prefilter batch (ibl_kernels.hip): level-0 input cube,
+// output chain, and the offset of its mip count's GGX table.
+struct ProbeJob {
+    const float4* cube;
+    float4* out;
+    uint32_t S, So, mips, n;
+    uint32_t table;
+};
+// A work item: texels [first, first + count) of one mip of one probe.
+struct ProbeItem {
+    uint32_t probe, mip, first, count;
+};
+
 // Grows a scratch buffer to at least `bytes` (synchronises the device when it
 // has to replace an existing allocation). Returns a hipError_t.
 hipError_t ensure_scratch(hvx_ctx* ctx, void** buf, size_t* cur, size_t bytes);
@@ -41,6 +83,13 @@ hipError_t launch_cull(const hvx_cf_ubo& ubo, const hvx_aabb* d_clusters, const 
                        uint32_t n_lights, uint32_t cap, hvx_light_cell* d_cells, uint32_t* d_indices,
                        uint32_t* d_total, uint32_t* d_overflow, float4* light_vs, uint32_t* lists,
                        uint32_t* counts, uint32_t* over, hipStream_t s);
+// cull group shapes: 0 = 4x4x1, 1 = 8x4x1, 2 = 4x4x2, 3 = 2x2x4
+constexpr int kCullShapes = 4;
+uint32_t cull_batch_groups(int shape, uint32_t X, uint32_t Y, uint32_t Z);
+hipError_t launch_cull_batch(const CullBatchView* d_views, uint32_t nv, const CullBatchTotals& t, int shape,
+                             uint32_t persistent_wgs, uint32_t* d_queue, hipStream_t s);
+hipError_t launch_queue_reset(uint32_t* d_queues, uint32_t n, hipStream_t s);
+
 // --- shading (shade_kernels.hip); light_scratch holds 2 * n_lights float4
 hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const hvx_cf_ubo& ubo,
                         const hvx_pbr_pc& pc, const hvx_light* d_lights, uint32_t n_lights,
@@ -53,6 +102,9 @@ hipError_t launch_equirect_to_cube(const float* d_eq, uint32_t w, uint32_t h, fl
                                    hipStream_t s);
 hipError_t launch_cube_mips(float* d_chain, uint32_t S, uint32_t levels, hipStream_t s);
 hipError_t launch_diffuse(const float* d_cube, uint32_t S, float* d_out, uint32_t Sd, hipStream_t s);
+hipError_t launch_ggx_tables(uint32_t mips, uint32_t n, float4* table, hipStream_t s);
+hipError_t launch_probe_batch(const ProbeJob* d_jobs, const ProbeItem* d_items, uint32_t n_items,
+                              const float4* d_tables, uint32_t persistent_wgs, uint32_t* d_queue, hipStream_t s);
 hipError_t launch_specular(const float* d_cube, uint32_t S, float* d_out, uint32_t So, uint32_t mips,
                            uint32_t n, bool emulate_static_viewport, float4* ggx_table, hipStream_t s);
 

@@ -242,6 +242,52 @@ struct SpecArgs {
     const float4* table;
 };
 
+// One output texel of the GGX prefilter (CubeFilterSpecular.frag:34-75) at
+// direction-grid position (face, x, y) of a D x D viewport; `tab` holds the
+// mip's n tangent-space samples; rough0 marks the roughness-0 mip.
+__device__ __forceinline__ float4 specular_texel_value(const float4* __restrict__ cube, uint32_t S,
+                                                       const float4* __restrict__ tab, uint32_t n, bool rough0,
+                                                       int face, int x, int y, int D) {
+    const f3 N = texel_dir(face, x, y, D);
+    const f3 V = N;
+    const Frame fr = ggx_frame(N);
+    float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, wsum = 0.0f;
+    if (rough0) {
+        // roughness 0 (PipelineCubeFilter.cpp:305-311 gives mip 0 roughness 0):
+        // cos(theta) = sqrt((1-y)/(1-y)) = 1 and sin(theta) = 0 for every
+        // Hammersley sample, so each sample's half vector is normalize(N + (+-0))
+        // and all n samples produce the same L, NoL and texel value. Evaluate it
+        // once and form the reference's sequential sums of the n identical terms.
+        const f3 Hh = ggx_world(fr, N, tab[0]);
+        const f3 L = normalize3(sub3(mul3(Hh, 2.0f * dot3(V, Hh)), V));
+        const float NoL = gmax(dot3(N, L), 0.0f);
+        if (NoL > 0.0f) {
+            const f3 c = sample_cube(cube, (int)S, L);
+            const float t0 = c.x * NoL, t1 = c.y * NoL, t2 = c.z * NoL;
+            for (uint32_t i = 0; i < n; ++i) {
+                acc0 = acc0 + t0;
+                acc1 = acc1 + t1;
+                acc2 = acc2 + t2;
+                wsum = wsum + NoL;
+            }
+        }
+    } else {
+        for (uint32_t i = 0; i < n; ++i) {
+            const f3 Hh = ggx_world(fr, N, tab[i]);
+            const f3 L = normalize3(sub3(mul3(Hh, 2.0f * dot3(V, Hh)), V));
+            const float NoL = gmax(dot3(N, L), 0.0f);
+            if (NoL > 0.0f) {
+                const f3 c = sample_cube(cube, (int)S, L);
+                acc0 = acc0 + c.x * NoL;
+                acc1 = acc1 + c.y * NoL;
+                acc2 = acc2 + c.z * NoL;
+                wsum = wsum + NoL;
+            }
+        }
+    }
+    return make_float4(acc0 / wsum, acc1 / wsum, acc2 / wsum, 1.0f);
+}
+
 __global__ __launch_bounds__(256) void specular_kernel(SpecArgs a) {
     const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= a.texel_begin[a.mips]) return;
@@ -251,48 +297,36 @@ __global__ __launch_bounds__(256) void specular_kernel(SpecArgs a) {
     const uint32_t local = id - a.texel_begin[m];
     const uint32_t x = local % Sm, y = (local / Sm) % Sm, face = local / (Sm * Sm);
     const uint32_t D = a.emulate ? a.So : Sm;
-    const f3 N = texel_dir((int)face, (int)x, (int)y, (int)D);
-    const f3 V = N;
-    const Frame fr = ggx_frame(N);
-    const float4* tab = a.table + (size_t)m * a.n;
-    float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, wsum = 0.0f;
-    if (m == 0) {
-        // mip 0 has roughness 0 (PipelineCubeFilter.cpp:305-311): cos(theta) = sqrt((1-y)/(1-y)) = 1 and sin(theta) = 0
-        // for every Hammersley sample, so each sample's half vector is
-        // normalize(N + (+-0)) and all n samples produce the same L, NoL and
-        // texel value. Evaluate it once and form the reference's sequential
-        // sums of the n identical terms.
-        const f3 Hh = ggx_world(fr, N, tab[0]);
-        const f3 L = normalize3(sub3(mul3(Hh, 2.0f * dot3(V, Hh)), V));
-        const float NoL = gmax(dot3(N, L), 0.0f);
-        if (NoL > 0.0f) {
-            const f3 c = sample_cube(a.cube, (int)a.S, L);
-            const float t0 = c.x * NoL, t1 = c.y * NoL, t2 = c.z * NoL;
-            for (uint32_t i = 0; i < a.n; ++i) {
-                acc0 = acc0 + t0;
-                acc1 = acc1 + t1;
-                acc2 = acc2 + t2;
-                wsum = wsum + NoL;
-            }
-        }
-        float4* o = a.out + local;
-        *o = make_float4(acc0 / wsum, acc1 / wsum, acc2 / wsum, 1.0f);
-        return;
-    }
-    for (uint32_t i = 0; i < a.n; ++i) {
-        const f3 Hh = ggx_world(fr, N, tab[i]);
-        const f3 L = normalize3(sub3(mul3(Hh, 2.0f * dot3(V, Hh)), V));
-        const float NoL = gmax(dot3(N, L), 0.0f);
-        if (NoL > 0.0f) {
-            const f3 c = sample_cube(a.cube, (int)a.S, L);
-            acc0 = acc0 + c.x * NoL;
-            acc1 = acc1 + c.y * NoL;
-            acc2 = acc2 + c.z * NoL;
-            wsum = wsum + NoL;
+    a.out[cube_level_offset_floats(a.So, m) / 4 + local] =
+        specular_texel_value(a.cube, a.S, a.table + (size_t)m * a.n, a.n, m == 0, (int)face, (int)x, (int)y, (int)D);
+}
+
+// ------------------------------------------------- batched probe prefilter
+// Persistent: workgroups pull (probe, mip, texel range) items from a queue;
+// items are ordered by decreasing cost on the host.
+__global__ __launch_bounds__(256) void probe_batch_kernel(const ProbeJob* __restrict__ jobs,
+                                                          const ProbeItem* __restrict__ items, uint32_t n_items,
+                                                          const float4* __restrict__ tables,
+                                                          uint32_t* __restrict__ queue) {
+    __shared__ uint32_t s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(queue, 1u);
+        __syncthreads();
+        const uint32_t item = s_item;
+        __syncthreads();
+        if (item >= n_items) break;
+        const ProbeItem it = items[item];
+        const ProbeJob& j = jobs[it.probe];
+        const uint32_t m = it.mip;
+        const uint32_t Sm = max(j.So >> m, 1u);
+        const float4* tab = tables + j.table + (size_t)m * j.n;
+        float4* out = j.out + cube_level_offset_floats(j.So, m) / 4;
+        for (uint32_t t = threadIdx.x; t < it.count; t += blockDim.x) {
+            const uint32_t local = it.first + t;
+            const uint32_t x = local % Sm, y = (local / Sm) % Sm, face = local / (Sm * Sm);
+            out[local] = specular_texel_value(j.cube, j.S, tab, j.n, m == 0, (int)face, (int)x, (int)y, (int)Sm);
         }
     }
-    float4* o = a.out + cube_level_offset_floats(a.So, m) / 4 + local;
-    *o = make_float4(acc0 / wsum, acc1 / wsum, acc2 / wsum, 1.0f);
 }
 
 }  // namespace
@@ -351,6 +385,18 @@ hipError_t launch_specular(const float* d_cube, uint32_t S, float* d_out, uint32
     }
     a.table = ggx_table;
     hipLaunchKernelGGL(specular_kernel, dim3((t + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_ggx_tables(uint32_t mips, uint32_t n, float4* table, hipStream_t s) {
+    hipLaunchKernelGGL(ggx_table_kernel, dim3((mips * n + 255) / 256), dim3(256), 0, s, mips, n, table);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe_batch(const ProbeJob* d_jobs, const ProbeItem* d_items, uint32_t n_items,
+                              const float4* d_tables, uint32_t persistent_wgs, uint32_t* d_queue, hipStream_t s) {
+    hipLaunchKernelGGL(probe_batch_kernel, dim3(persistent_wgs), dim3(256), 0, s, d_jobs, d_items, n_items, d_tables,
+                       d_queue);
     return hipGetLastError();
 }
 

@@ -252,6 +252,70 @@ int hvx_ibl_filter(hvx_ctx* ctx, int type, const float* d_cube,
                    uint32_t sample_count, const hvx_filter_opts* opts,
                    void* stream);
 
+/* ---------------------------------------------------------------- batches */
+
+/* One light probe of a prefilter batch: its level-0 cube and where its GGX
+ * prefiltered chain goes (as hvx_ibl_filter type 1 writes it). */
+typedef struct hvx_probe {
+    const float* cube;         /* RGBA32F, 6 x face_size^2 (level 0 only is read) */
+    uint32_t face_size;
+    uint32_t out_face_size;
+    uint32_t out_mips;
+    uint32_t _pad;
+    float* out_chain;          /* hvx_cube_chain_floats(out_face_size, out_mips) */
+} hvx_probe;
+
+/* One clustered-forward view of a culling batch (hvx_cf_cull_lights
+ * arguments). Cluster AABBs must already be built (hvx_cf_build_aabbs). */
+typedef struct hvx_view {
+    hvx_cf_ubo ubo;
+    const hvx_aabb* clusters;
+    const hvx_light* lights;
+    uint32_t n_lights;
+    uint32_t cap;
+    hvx_light_cell* cells;
+    uint32_t* indices;
+    size_t indices_capacity;   /* >= X*Y*Z*cap */
+    uint32_t* total;           /* nullable */
+} hvx_view;
+
+/* Tile shapes of the batched kernels. hvx_batch_autotune fills the timings. */
+typedef struct hvx_batch_tuning {
+    uint32_t probe_tile;        /* output texels per probe work item, 32..8192 */
+    uint32_t probe_wgs_per_cu;  /* persistent probe workgroups per CU, 1..16 */
+    uint32_t cull_shape;        /* cluster block: 0 = 4x4x1, 1 = 8x4x1, 2 = 4x4x2, 3 = 2x2x4 */
+    uint32_t cull_wgs_per_cu;   /* persistent culling workgroups per CU, 1..16 */
+    float probe_ms;             /* measured by hvx_batch_autotune for the chosen shape */
+    float cull_ms;
+} hvx_batch_tuning;
+
+typedef struct hvx_batch hvx_batch;
+
+/* A batch of probes (GGX prefilter with sample_count samples, as
+ * hvx_ibl_filter type 1) and views (light culling, as hvx_cf_cull_lights),
+ * run by two persistent kernels that pull work items from device queues.
+ * There is no single reference entry point: it batches what
+ * ResourcesIBL::Create (ResourcesIBL.cpp:18-40) does for one probe and what
+ * PipelineLightCulling::FillCommandBuffer (PipelineLightCulling.cpp:51-119)
+ * does for one view. Every output equals the per-item entry point's output
+ * bit for bit. The arrays are copied; all scratch is allocated here, so
+ * hvx_batch_run allocates nothing and may be captured into a graph. */
+int hvx_batch_create(hvx_ctx* ctx, const hvx_probe* probes, uint32_t n_probes, uint32_t sample_count,
+                     const hvx_view* views, uint32_t n_views, hvx_batch** out);
+int hvx_batch_destroy(hvx_batch* batch);
+int hvx_batch_set_tuning(hvx_batch* batch, const hvx_batch_tuning* tuning);
+int hvx_batch_get_tuning(const hvx_batch* batch, hvx_batch_tuning* tuning);
+/* Times every candidate tile shape on this batch (synchronously, on
+ * `stream`; outputs are overwritten) and keeps the fastest. */
+int hvx_batch_autotune(hvx_batch* batch, void* stream);
+/* Runs the batch: the probe and culling kernels on forked streams, joined
+ * back into `stream` before returning control to it. */
+int hvx_batch_run(hvx_batch* batch, void* stream);
+/* Captures hvx_batch_run into a graph (re-captured after a tuning change);
+ * hvx_batch_launch replays it on `stream`. */
+int hvx_batch_capture(hvx_batch* batch, void* stream);
+int hvx_batch_launch(hvx_batch* batch, void* stream);
+
 /* ------------------------------------------------------------------ camera */
 
 /* Camera description: position/target, vertical fov (radians), clip planes,

@@ -38,6 +38,8 @@ def test_struct_layouts_match_reference():
     # CameraUBO 160 B, PushConstPBR 20 B
     assert C.sizeof(abi.Light) == 48 and C.sizeof(abi.AABB) == 32 and C.sizeof(abi.LightCell) == 8
     assert C.sizeof(abi.CFUBO) == 176 and C.sizeof(abi.CameraUBO) == 160 and C.sizeof(abi.PBRPC) == 20
+    assert C.sizeof(abi.Probe) == 32 and C.sizeof(abi.View) == 232 and C.sizeof(abi.BatchTuning) == 24
+    assert abi.View.n_lights.offset == 192 and abi.View.indices_capacity.offset == 216
     assert abi.CFUBO.screen_size.offset == 128 and abi.CFUBO.slice_count_x.offset == 152
     assert abi.Light.radius.offset == 32
 
