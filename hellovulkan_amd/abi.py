@@ -94,6 +94,12 @@ class CameraDesc(C.Structure):
                 ("slice_count_x", C.c_uint32), ("slice_count_y", C.c_uint32), ("slice_count_z", C.c_uint32)]
 
 
+class ContractMipInfo(C.Structure):
+    _fields_ = [("kind", C.c_int), ("out_face", C.c_uint32), ("in_face", C.c_uint32), ("far_face", C.c_uint32),
+                ("window_t0", C.c_float), ("window_t1", C.c_float), ("weight_bytes", C.c_uint64),
+                ("min_input_level", C.c_uint32), ("_pad", C.c_uint32)]
+
+
 class Probe(C.Structure):
     _fields_ = [("cube", C.c_void_p), ("face_size", C.c_uint32), ("out_face_size", C.c_uint32),
                 ("out_mips", C.c_uint32), ("_pad", C.c_uint32), ("out_chain", C.c_void_p)]
@@ -143,6 +149,10 @@ SIGNATURES = [
     ("hvx_cube_chain_floats", C.c_size_t, [u32, u32]),
     ("hvx_cube_level_offset_floats", C.c_size_t, [u32, u32]),
     ("hvx_mip_count", u32, [u32]),
+    ("hvx_ibl_contract_plan_create", C.c_int, [vp, C.c_int, u32, u32, u32, C.c_int, P(vp)]),
+    ("hvx_ibl_contract_plan_destroy", C.c_int, [vp]),
+    ("hvx_ibl_contract_plan_query", C.c_int, [vp, u32, P(ContractMipInfo)]),
+    ("hvx_ibl_filter_contract", C.c_int, [vp, vp, u32, vp, vp]),
     ("hvx_batch_create", C.c_int, [vp, P(Probe), u32, u32, P(View), u32, P(vp)]),
     ("hvx_batch_destroy", C.c_int, [vp]),
     ("hvx_batch_set_tuning", C.c_int, [vp, P(BatchTuning)]),
@@ -332,6 +342,45 @@ class Context:
         opts.emulate_static_viewport = 1 if emulate_static_viewport else 0
         self._check(self.lib.hvx_ibl_filter(self.h, ftype, _ptr(cube), face, _ptr(out), out_face, out_mips, samples,
                                             C.byref(opts), self._stream(stream)), "hvx_ibl_filter")
+
+
+PRECISION_FP16_SCALED, PRECISION_BF16_SPLIT = 0, 1
+CONTRACT_DELTA, CONTRACT_FULL, CONTRACT_SPLIT = 0, 1, 2
+
+
+class ContractPlan:
+    """Owns an hvx_contract_plan: the precomputed weights of the contraction
+    (MFMA) cube filter for one (type, face, out_face, out_mips)."""
+
+    def __init__(self, ctx: Context, ftype: int, face: int, out_face: int, out_mips: int,
+                 precision: int = PRECISION_FP16_SCALED):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.out_face, self.out_mips = out_face, out_mips
+        h = C.c_void_p()
+        ctx._check(self.lib.hvx_ibl_contract_plan_create(ctx.h, ftype, face, out_face, out_mips, precision,
+                                                         C.byref(h)), "hvx_ibl_contract_plan_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.hvx_ibl_contract_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self, mip: int) -> ContractMipInfo:
+        i = ContractMipInfo()
+        self.ctx._check(self.lib.hvx_ibl_contract_plan_query(self.h, mip, C.byref(i)), "hvx_ibl_contract_plan_query")
+        return i
+
+    def filter(self, chain, chain_levels: int, out, stream=None):
+        self.ctx._check(self.lib.hvx_ibl_filter_contract(self.h, _ptr(chain), chain_levels, _ptr(out),
+                                                         self.ctx._stream(stream)), "hvx_ibl_filter_contract")
 
 
 class Batch:

@@ -20,7 +20,7 @@ LIB = os.path.join(PKG, "libhvx.so")
 ORACLE_SRC = os.path.join(ROOT, "oracle", "hvx_oracle.cpp")
 ORACLE_LIB = os.path.join(ROOT, "oracle", "_build", "libhvx_oracle.so")
 
-HIP_SOURCES = ["cf_kernels.hip", "shade_kernels.hip", "ibl_kernels.hip", "hvx_api.hip", "batch_api.hip"]
+HIP_SOURCES = ["cf_kernels.hip", "shade_kernels.hip", "ibl_kernels.hip", "hvx_api.hip", "batch_api.hip", "contract.hip"]
 CPP_SOURCES = ["scene.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -73,7 +73,7 @@ def build_oracle(verbose: bool = False) -> str:
     return ORACLE_LIB
 
 
-TEST_KERNELS_SRC = os.path.join(ROOT, "tests", "gpu_kernels", "fastmath_check.hip")
+TEST_KERNELS_SRC = [os.path.join(ROOT, "tests", "gpu_kernels", f) for f in ("fastmath_check.hip", "mfma_layout.hip")]
 TEST_KERNELS_LIB = os.path.join(ROOT, "tests", "_build", "libhvx_testkernels.so")
 
 
@@ -81,7 +81,7 @@ def build_test_kernels(verbose: bool = False) -> str:
     """Test-only GPU kernels (exhaustive arithmetic checks) used by tests/."""
     os.makedirs(os.path.dirname(TEST_KERNELS_LIB), exist_ok=True)
     tmp = TEST_KERNELS_LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, TEST_KERNELS_SRC], verbose)
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *TEST_KERNELS_SRC], verbose)
     os.replace(tmp, TEST_KERNELS_LIB)
     return TEST_KERNELS_LIB
 

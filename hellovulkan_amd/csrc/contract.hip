@@ -1,0 +1,934 @@
+// contract.hip — the dense-contraction cube filter on FP16 MFMA
+// (hvx_ibl_contract_* and hvx_ibl_filter_contract of include/hvx.h).
+//
+// The operator is defined in hvx_contract.h. A plan holds, per output mip,
+// the precomputed weight matrices of the canonical face (shared by all six
+// faces), built on the GPU in FP64: the row normalisation sums are FP64
+// reductions, and each row is stored in FP16 scaled by a power of two that
+// puts its largest weight just under 2^15. A filter call stages the input
+// radiance as FP16 scaled per face by a power of two (largest value under
+// 2^15), then runs one MFMA GEMM per weight matrix:
+//
+//   C[rows, 6 faces x 4] = W[rows, K] * R[K, 6 faces x 4]    (FP32 accumulate)
+//
+// where column block f of R holds the input texels seen from output face f
+// (a permutation of the level for full-sphere sums; face f's padded plane
+// window for the near field). Rows are canonical output texels, so one
+// weight matrix serves the six faces as six column blocks.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "hvx_contract.h"
+#include "hvx_device.h"
+#include "hvx_internal.h"
+#include "hvx_mfma.h"
+
+namespace {
+
+using hvx::ContractLevel;
+using hvx::f32x4;
+using hvx::f3;
+using hvx::half8;
+using hvx::normalize3;
+using hvx::sample_cube;
+using hvx::texel_dir;
+
+enum Piece { PIECE_WHOLE = 0, PIECE_NEAR = 1, PIECE_FAR = 2 };
+
+// Dense (full-sphere) weight matrix: rows are the canonical texels of an
+// So x So face, columns every texel of a 6 x Sin x Sin level.
+struct DenseGeom {
+    int type, piece;
+    double alpha2, t0, t1;
+    uint32_t So, Sin;
+    uint32_t rows, rt, ks;  // rows = So^2, row tiles of 16, k-steps of 32
+};
+
+// Near-field window of one 16 x 16 output tile, in (unpadded) texel indices
+// of the face plane at the near level; woff = weight offset in half8 units.
+struct NearTile {
+    int32_t x0, y0;
+    uint32_t w, h;
+    uint64_t woff;
+};
+
+struct NearGeom {
+    int type;
+    double alpha2, t0, t1;
+    uint32_t So, Si, te;  // output face, near level face, tiles per edge
+};
+
+// ---------------------------------------------------------- device helpers
+__device__ __forceinline__ void local_dir(double u, double v, double d[3]) {
+    const double r = 1.0 / sqrt(1.0 + u * u + v * v);
+    d[0] = u * r;
+    d[1] = v * r;
+    d[2] = r;
+}
+
+// Canonical input texel k of a 6 x S x S level: direction and solid angle.
+__device__ __forceinline__ void canon_input(uint32_t k, uint32_t S, double l[3], double* dw) {
+    const uint32_t S2 = S * S, g = k / S2, rem = k - g * S2, y = rem / S, x = rem - y * S;
+    const double u = hvx::contract_coord((int)x, S), v = hvx::contract_coord((int)y, S);
+    double p[3];
+    hvx::contract_face_point((int)g, u, v, p);
+    const double r = 1.0 / sqrt(1.0 + u * u + v * v);
+    l[0] = p[0] * r;
+    l[1] = p[1] * r;
+    l[2] = p[2] * r;
+    *dw = hvx::contract_dw(u, v, S);
+}
+
+__device__ __forceinline__ double piece_weight(int type, int piece, double alpha2, double t0, double t1, double c,
+                                               double dw) {
+    double w = hvx::contract_weight(type, alpha2, c);
+    if (w == 0.0) return 0.0;
+    w *= dw;
+    if (piece == PIECE_NEAR) w *= hvx::contract_phi(t0, t1, c);
+    else if (piece == PIECE_FAR) w *= 1.0 - hvx::contract_phi(t0, t1, c);
+    return w;
+}
+
+__device__ __forceinline__ void dense_row_dir(const DenseGeom& G, uint32_t row, double n[3]) {
+    const uint32_t y = row / G.So, x = row - y * G.So;
+    local_dir(hvx::contract_coord((int)x, G.So), hvx::contract_coord((int)y, G.So), n);
+}
+
+__device__ __forceinline__ double dense_w(const DenseGeom& G, const double n[3], uint32_t k) {
+    double l[3], dw;
+    canon_input(k, G.Sin, l, &dw);
+    const double c = (n[0] * l[0] + n[1] * l[1]) + n[2] * l[2];
+    return piece_weight(G.type, G.piece, G.alpha2, G.t0, G.t1, c, dw);
+}
+
+__device__ __forceinline__ double near_w(const NearGeom& G, const double n[3], int px, int py) {
+    const double u = hvx::contract_coord(px, G.Si), v = hvx::contract_coord(py, G.Si);
+    double l[3];
+    local_dir(u, v, l);
+    const double c = (n[0] * l[0] + n[1] * l[1]) + n[2] * l[2];
+    return piece_weight(G.type, PIECE_NEAR, G.alpha2, G.t0, G.t1, c, hvx::contract_dw(u, v, G.Si));
+}
+
+// Power-of-two exponent s with w_max * 2^s in (2^14, 2^15].
+__host__ __device__ __forceinline__ int row_exponent(double wmax) {
+    if (!(wmax > 0.0)) return 0;
+    int e;
+    (void)frexp(wmax, &e);  // wmax = m 2^e, m in [0.5, 1)
+    return 15 - e;
+}
+
+// Deterministic block sum and max (256 threads) in double.
+__device__ void block_sum_max(double s, double m, double* sh_s, double* sh_m, double* out_s, double* out_m) {
+    const uint32_t t = threadIdx.x;
+    sh_s[t] = s;
+    sh_m[t] = m;
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (t < h) {
+            sh_s[t] = sh_s[t] + sh_s[t + h];
+            sh_m[t] = sh_m[t] < sh_m[t + h] ? sh_m[t + h] : sh_m[t];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        *out_s = sh_s[0];
+        *out_m = sh_m[0];
+    }
+}
+
+// ------------------------------------------------------ plan: dense weights
+__global__ __launch_bounds__(256) void dense_row_kernel(DenseGeom G, double* __restrict__ Z,
+                                                        double* __restrict__ wmax) {
+    __shared__ double sh_s[256], sh_m[256];
+    const uint32_t row = blockIdx.x;
+    double n[3];
+    dense_row_dir(G, row, n);
+    const uint32_t K = 6 * G.Sin * G.Sin;
+    double s = 0.0, m = 0.0;
+    for (uint32_t k = threadIdx.x; k < K; k += 256) {
+        const double w = dense_w(G, n, k);
+        s += w;
+        m = m < w ? w : m;
+    }
+    block_sum_max(s, m, sh_s, sh_m, &Z[row], &wmax[row]);
+}
+
+__global__ __launch_bounds__(256) void dense_fill_kernel(DenseGeom G, const double* __restrict__ wmax,
+                                                         half8* __restrict__ W) {
+    const uint64_t id = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (id >= (uint64_t)G.rt * G.ks * 64) return;
+    const uint32_t lane = (uint32_t)(id & 63);
+    const uint64_t t = id >> 6;
+    const uint32_t ks = (uint32_t)(t % G.ks), rt = (uint32_t)(t / G.ks);
+    const uint32_t row = rt * 16 + (lane & 15), k0 = ks * 32 + 8 * (lane >> 4);
+    half8 out;
+    if (row < G.rows) {
+        double n[3];
+        dense_row_dir(G, row, n);
+        const double sc = ldexp(1.0, row_exponent(wmax[row]));
+#pragma unroll
+        for (int q = 0; q < 8; ++q) out[q] = (_Float16)(float)(dense_w(G, n, k0 + q) * sc);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) out[q] = (_Float16)0.0f;
+    }
+    W[id] = out;
+}
+
+// rowscale = 2^-s / Z (normalised rows) or 2^-s (raw far rows; then the
+// far normaliser goes to a six-face map, identical on every face).
+__global__ __launch_bounds__(256) void dense_scale_kernel(uint32_t rows, const double* __restrict__ Z,
+                                                          const double* __restrict__ wmax, int raw,
+                                                          float* __restrict__ rowscale, float4* __restrict__ zmap) {
+    const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= rows) return;
+    const double p = ldexp(1.0, -row_exponent(wmax[r]));
+    rowscale[r] = raw ? (float)p : (float)(p / Z[r]);
+    if (raw)
+        for (uint32_t f = 0; f < 6; ++f) zmap[f * rows + r] = make_float4((float)Z[r], 0.0f, 0.0f, 0.0f);
+}
+
+// -------------------------------------------------------- plan: near weights
+__global__ __launch_bounds__(256) void near_row_kernel(NearGeom G, const NearTile* __restrict__ tiles,
+                                                       double* __restrict__ Z, double* __restrict__ wmax) {
+    __shared__ double sh_s[256], sh_m[256];
+    const uint32_t o = blockIdx.x, Y = o / G.So, X = o - Y * G.So;
+    const NearTile T = tiles[(Y / 16) * G.te + X / 16];
+    double n[3];
+    local_dir(hvx::contract_coord((int)X, G.So), hvx::contract_coord((int)Y, G.So), n);
+    const uint32_t K = T.w * T.h;
+    double s = 0.0, m = 0.0;
+    for (uint32_t k = threadIdx.x; k < K; k += 256) {
+        const uint32_t r = k / T.w, c = k - r * T.w;
+        const double w = near_w(G, n, T.x0 + (int)c, T.y0 + (int)r);
+        s += w;
+        m = m < w ? w : m;
+    }
+    block_sum_max(s, m, sh_s, sh_m, &Z[o], &wmax[o]);
+}
+
+// grid: tiles x 16 row tiles; each block writes one row tile's k-steps
+__global__ __launch_bounds__(256) void near_fill_kernel(NearGeom G, const NearTile* __restrict__ tiles,
+                                                        const double* __restrict__ wmax, half8* __restrict__ W) {
+    const uint32_t tile = blockIdx.x / 16, rt = blockIdx.x % 16;
+    const NearTile T = tiles[tile];
+    const uint32_t KS = T.h * (T.w / 32);
+    const uint32_t X0 = (tile % G.te) * 16, Y = (tile / G.te) * 16 + rt;
+    for (uint32_t idx = threadIdx.x; idx < KS * 64; idx += 256) {
+        const uint32_t ks = idx >> 6, lane = idx & 63;
+        const uint32_t X = X0 + (lane & 15), o = Y * G.So + X;
+        double n[3];
+        local_dir(hvx::contract_coord((int)X, G.So), hvx::contract_coord((int)Y, G.So), n);
+        const double sc = ldexp(1.0, row_exponent(wmax[o]));
+        const uint32_t k0 = ks * 32 + 8 * (lane >> 4), r = k0 / T.w, c0 = k0 - r * T.w;
+        half8 out;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            out[q] = (_Float16)(float)(near_w(G, n, T.x0 + (int)(c0 + q), T.y0 + (int)r) * sc);
+        W[T.woff + ((uint64_t)rt * KS + ks) * 64 + lane] = out;
+    }
+}
+
+// Z(o) = Z_near(o) + far normaliser interpolated at o (any face: the far
+// normaliser map is the same on every face).
+__global__ __launch_bounds__(256) void near_scale_kernel(NearGeom G, uint32_t Sf, const double* __restrict__ Z,
+                                                         const double* __restrict__ wmax,
+                                                         const float4* __restrict__ zmap,
+                                                         float* __restrict__ rowscale, float* __restrict__ invZ) {
+    const uint32_t o = blockIdx.x * 256 + threadIdx.x;
+    if (o >= G.So * G.So) return;
+    const uint32_t Y = o / G.So, X = o - Y * G.So;
+    const f3 zf = sample_cube(zmap, (int)Sf, texel_dir(0, (int)X, (int)Y, (int)G.So));
+    const double Zt = Z[o] + (double)zf.x;
+    rowscale[o] = (float)(ldexp(1.0, -row_exponent(wmax[o])) / Zt);
+    invZ[o] = (float)(1.0 / Zt);
+}
+
+// ------------------------------------------------------ per-call staging
+__global__ __launch_bounds__(64) void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
+    if (threadIdx.x < n) p[threadIdx.x] = v;
+}
+
+// Largest RGB magnitude per face of a level (float bits of non-negative
+// values order as unsigned integers, so atomicMax is exact and
+// order-independent).
+__global__ __launch_bounds__(256) void facemax_kernel(const float4* __restrict__ level, uint32_t S,
+                                                      uint32_t* __restrict__ fmax) {
+    __shared__ float sh[256];
+    const uint32_t f = blockIdx.y, S2 = S * S;
+    float m = 0.0f;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < S2; i += gridDim.x * 256) {
+        const float4 v = level[(size_t)f * S2 + i];
+        const float a = fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fabsf(v.z));
+        m = a > m ? a : m;  // NaN never replaces m
+    }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(&fmax[f], __float_as_uint(sh[0]));
+}
+
+// sc[f] = 2^-e_f, sc[6+f] = 2^e_f for the padded plane of face f (bounded by
+// face f and its four neighbours); sc[12] = 2^-e, sc[13] = 2^e over all faces.
+__global__ __launch_bounds__(64) void scales_kernel(const uint32_t* __restrict__ fmax, float* __restrict__ sc) {
+    const uint32_t t = threadIdx.x;
+    if (t > 6) return;
+    float m = 0.0f;
+    for (uint32_t g = 0; g < 6; ++g)
+        if (t == 6 || g != (t ^ 1u)) m = fmaxf(m, __uint_as_float(fmax[g]));
+    int e = 0;
+    if (m > 0.0f && m <= 3.0e38f) {
+        int ex;
+        (void)frexpf(m, &ex);
+        e = ex - 15;
+    }
+    if (t < 6) {
+        sc[t] = ldexpf(1.0f, -e);
+        sc[6 + t] = ldexpf(1.0f, e);
+    } else {
+        sc[12] = ldexpf(1.0f, -e);
+        sc[13] = ldexpf(1.0f, e);
+    }
+}
+
+// R[k/8][col][8]: col = 4 f + c holds channel c (< 3) of the level texel
+// that output face f sees as canonical input k; other columns are zero.
+__global__ __launch_bounds__(256) void stage_dense_kernel(const float4* __restrict__ level, uint32_t Sin,
+                                                          const int32_t* __restrict__ perm,
+                                                          const float* __restrict__ sc, half8* __restrict__ R) {
+    const uint32_t K = 6 * Sin * Sin;
+    const uint32_t id = blockIdx.x * 256 + threadIdx.x;
+    if (id >= (K / 8) * 32) return;
+    const uint32_t col = id & 31, k8 = id >> 5, f = col >> 2, c = col & 3;
+    half8 out;
+    if (col < 24 && c < 3) {
+        const float s = sc[12];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const float4 v = level[perm[(size_t)f * K + k8 * 8 + q]];
+            out[q] = (_Float16)((c == 0 ? v.x : (c == 1 ? v.y : v.z)) * s);
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) out[q] = (_Float16)0.0f;
+    }
+    R[id] = out;
+}
+
+// P[py][px/8][col][8] over the padded plane (Pw x Pw, texel index -B .. Si+B)
+// of each face: inside the face the texel itself, outside the bilinear
+// seamless sample of the level at the plane point's direction.
+__global__ __launch_bounds__(256) void stage_pad_kernel(const float4* __restrict__ level, uint32_t Si, uint32_t B,
+                                                        uint32_t Pw, const float* __restrict__ sc,
+                                                        half8* __restrict__ P) {
+    const uint32_t Pw8 = Pw / 8;
+    const uint32_t id = blockIdx.x * 256 + threadIdx.x;
+    if (id >= Pw * Pw8 * 32) return;
+    const uint32_t col = id & 31, cell = id >> 5, px8 = cell % Pw8, py = cell / Pw8;
+    const uint32_t f = col >> 2, c = col & 3;
+    half8 out;
+    if (col < 24 && c < 3) {
+        const float s = sc[f];
+        const int y = (int)py - (int)B;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int x = (int)(px8 * 8 + q) - (int)B;
+            float val;
+            if (x >= 0 && y >= 0 && x < (int)Si && y < (int)Si) {
+                const float4 v = level[((size_t)f * Si + y) * Si + x];
+                val = c == 0 ? v.x : (c == 1 ? v.y : v.z);
+            } else {
+                double p[3];
+                hvx::contract_face_point((int)f, hvx::contract_coord(x, Si), hvx::contract_coord(y, Si), p);
+                const f3 d = normalize3(f3{(float)p[0], (float)p[1], (float)p[2]});
+                const f3 t = sample_cube(level, (int)Si, d);
+                val = c == 0 ? t.x : (c == 1 ? t.y : t.z);
+            }
+            out[q] = (_Float16)(val * s);
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) out[q] = (_Float16)0.0f;
+    }
+    P[id] = out;
+}
+
+// ----------------------------------------------------------- MFMA GEMMs
+// Dense: a wave owns two row tiles (32 rows) x 32 columns over one K slice;
+// partial sums go to part[slice][row][32] and are reduced in slice order.
+__global__ __launch_bounds__(256) void dense_gemm_kernel(const half8* __restrict__ W, const half8* __restrict__ R,
+                                                         uint32_t rt, uint32_t ks, uint32_t splitk,
+                                                         float* __restrict__ part) {
+    const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const uint32_t pairs = (rt + 1) / 2;
+    if (wave >= pairs * splitk) return;  // uniform per wave; the kernel has no barriers
+    const uint32_t pair = wave / splitk, slice = wave - pair * splitk;
+    const uint32_t rt0 = 2 * pair;
+    const bool has1 = rt0 + 1 < rt;
+    const uint32_t kb = (uint32_t)((uint64_t)slice * ks / splitk), ke = (uint32_t)((uint64_t)(slice + 1) * ks / splitk);
+    const uint32_t i = lane & 15, g = lane >> 4;
+    const half8* w0 = W + (size_t)rt0 * ks * 64 + lane;
+    const half8* w1 = has1 ? w0 + (size_t)ks * 64 : w0;
+    const half8* rb = R + g * 32 + i;
+    f32x4 a00 = {0.0f, 0.0f, 0.0f, 0.0f}, a01 = a00, a10 = a00, a11 = a00;
+    for (uint32_t s = kb; s < ke; ++s) {
+        const half8 x0 = w0[(size_t)s * 64], x1 = w1[(size_t)s * 64];
+        const half8 b0 = rb[(size_t)s * 128], b1 = rb[(size_t)s * 128 + 16];
+        a00 = hvx::mfma16x16x32(x0, b0, a00);
+        a01 = hvx::mfma16x16x32(x0, b1, a01);
+        a10 = hvx::mfma16x16x32(x1, b0, a10);
+        a11 = hvx::mfma16x16x32(x1, b1, a11);
+    }
+    float* p = part + (size_t)slice * rt * 16 * 32;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t row0 = rt0 * 16 + 4 * g + r;
+        p[(size_t)row0 * 32 + i] = a00[r];
+        p[(size_t)row0 * 32 + 16 + i] = a01[r];
+        if (has1) {
+            p[(size_t)(row0 + 16) * 32 + i] = a10[r];
+            p[(size_t)(row0 + 16) * 32 + 16 + i] = a11[r];
+        }
+    }
+}
+
+// out[f][row] = (sum over slices) * rowscale[row] * 2^e  (alpha = 1 or 0)
+__global__ __launch_bounds__(256) void dense_epilogue_kernel(const float* __restrict__ part, uint32_t splitk,
+                                                             uint32_t rows, uint32_t rows_pad,
+                                                             const float* __restrict__ rowscale,
+                                                             const float* __restrict__ sc, float alpha,
+                                                             float4* __restrict__ out) {
+    const uint32_t id = blockIdx.x * 256 + threadIdx.x;
+    if (id >= rows * 6) return;
+    const uint32_t f = id / rows, row = id - f * rows;
+    float v[3] = {0.0f, 0.0f, 0.0f};
+    for (uint32_t s = 0; s < splitk; ++s) {
+        const float* p = part + ((size_t)s * rows_pad + row) * 32 + f * 4;
+        v[0] += p[0];
+        v[1] += p[1];
+        v[2] += p[2];
+    }
+    const float k = rowscale[row] * sc[13];
+    out[(size_t)f * rows + row] = make_float4(v[0] * k, v[1] * k, v[2] * k, alpha);
+}
+
+// Near field: one workgroup per 16 x 16 output tile, eight waves of two
+// output rows each; B rows come from the face planes' window.
+__global__ __launch_bounds__(512) void near_gemm_kernel(const half8* __restrict__ W, const NearTile* __restrict__ tiles,
+                                                        const half8* __restrict__ P, uint32_t Pw8, uint32_t B,
+                                                        uint32_t te, uint32_t So, const float* __restrict__ rowscale,
+                                                        const float* __restrict__ sc, float* __restrict__ out) {
+    const uint32_t tile = blockIdx.x, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const NearTile T = tiles[tile];
+    const uint32_t wc = T.w / 32, KS = T.h * wc;
+    const uint32_t i = lane & 15, g = lane >> 4, rt0 = 2 * wave;
+    const half8* w0 = W + T.woff + (size_t)rt0 * KS * 64 + lane;
+    const half8* w1 = w0 + (size_t)KS * 64;
+    const uint32_t xb = (uint32_t)(T.x0 + (int)B) / 8 + g;
+    f32x4 a00 = {0.0f, 0.0f, 0.0f, 0.0f}, a01 = a00, a10 = a00, a11 = a00;
+    uint32_t r = 0, cc = 0;
+    for (uint32_t s = 0; s < KS; ++s) {
+        const size_t pidx = (((size_t)(T.y0 + (int)B + (int)r) * Pw8 + xb + cc * 4) * 32) + i;
+        const half8 x0 = w0[(size_t)s * 64], x1 = w1[(size_t)s * 64];
+        const half8 b0 = P[pidx], b1 = P[pidx + 16];
+        a00 = hvx::mfma16x16x32(x0, b0, a00);
+        a01 = hvx::mfma16x16x32(x0, b1, a01);
+        a10 = hvx::mfma16x16x32(x1, b0, a10);
+        a11 = hvx::mfma16x16x32(x1, b1, a11);
+        if (++cc == wc) {
+            cc = 0;
+            ++r;
+        }
+    }
+    const uint32_t X0 = (tile % te) * 16, Y0 = (tile / te) * 16;
+    const size_t So2 = (size_t)So * So;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t Y = Y0 + rt0 + h;
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            const f32x4 a = h == 0 ? (nb == 0 ? a00 : a01) : (nb == 0 ? a10 : a11);
+            const uint32_t col = nb * 16 + i, f = col >> 2, c = col & 3;
+            if (col >= 24) continue;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t X = X0 + 4 * g + q, o = Y * So + X;
+                out[((size_t)f * So2 + o) * 4 + c] = c < 3 ? a[q] * rowscale[o] * sc[6 + f] : 1.0f;
+            }
+        }
+    }
+}
+
+// out(o) += far field interpolated at o / Z(o)
+__global__ __launch_bounds__(256) void far_add_kernel(const float4* __restrict__ far, uint32_t Sf, uint32_t So,
+                                                      const float* __restrict__ invZ, float4* __restrict__ out) {
+    const uint32_t id = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t So2 = So * So;
+    if (id >= 6 * So2) return;
+    const uint32_t f = id / So2, o = id - f * So2, Y = o / So, X = o - Y * So;
+    const f3 F = sample_cube(far, (int)Sf, texel_dir((int)f, (int)X, (int)Y, (int)So));
+    const float z = invZ[o];
+    float4 v = out[id];
+    v.x += F.x * z;
+    v.y += F.y * z;
+    v.z += F.z * z;
+    out[id] = v;
+}
+
+// Roughness 0: the bilinear seamless sample of level 0 at the texel direction.
+__global__ __launch_bounds__(256) void delta_kernel(const float4* __restrict__ level0, uint32_t S, uint32_t So,
+                                                    float4* __restrict__ out) {
+    const uint32_t id = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t So2 = So * So;
+    if (id >= 6 * So2) return;
+    const uint32_t f = id / So2, o = id - f * So2, Y = o / So, X = o - Y * So;
+    const f3 c = sample_cube(level0, (int)S, texel_dir((int)f, (int)X, (int)Y, (int)So));
+    out[id] = make_float4(c.x, c.y, c.z, 1.0f);
+}
+
+// -------------------------------------------------------------- host side
+uint32_t ilog2(uint32_t v) {
+    uint32_t l = 0;
+    while ((1u << (l + 1)) <= v) ++l;
+    return l;
+}
+
+// Integer frame of face f: world point of face-plane (u, v) = u E0 + v E1 + E2.
+void face_frame(int f, int E[3][3]) {
+    double z[3], a[3], b[3];
+    hvx::contract_face_point(f, 0.0, 0.0, z);
+    hvx::contract_face_point(f, 1.0, 0.0, a);
+    hvx::contract_face_point(f, 0.0, 1.0, b);
+    for (int j = 0; j < 3; ++j) {
+        E[0][j] = (int)(a[j] - z[j]);
+        E[1][j] = (int)(b[j] - z[j]);
+        E[2][j] = (int)z[j];
+    }
+}
+
+// World texel index (face-major) of the texel whose direction is output
+// face f's frame applied to canonical input k: exact integer arithmetic in
+// doubled texel units (a texel centre of face g is a E0 + b E1 + S E2 with
+// odd a, b in (-S, S)).
+int32_t perm_index(int f, uint32_t k, uint32_t S) {
+    const uint32_t S2 = S * S, g = k / S2, rem = k % S2, y = rem / S, x = rem % S;
+    const int s = (int)S, a = 2 * (int)x + 1 - s, b = 2 * (int)y + 1 - s;
+    int Eg[3][3], Ef[3][3], p[3], q[3];
+    face_frame((int)g, Eg);
+    face_frame(f, Ef);
+    for (int j = 0; j < 3; ++j) p[j] = a * Eg[0][j] + b * Eg[1][j] + s * Eg[2][j];
+    for (int j = 0; j < 3; ++j) q[j] = p[0] * Ef[0][j] + p[1] * Ef[1][j] + p[2] * Ef[2][j];
+    const int ax = std::abs(q[0]), ay = std::abs(q[1]), az = std::abs(q[2]);
+    int nf, na, nb;
+    if (ax >= ay && ax >= az) {
+        if (q[0] > 0) { nf = 0; na = -q[2]; nb = -q[1]; }
+        else { nf = 1; na = q[2]; nb = -q[1]; }
+    } else if (ay >= az) {
+        if (q[1] > 0) { nf = 2; na = q[0]; nb = q[2]; }
+        else { nf = 3; na = q[0]; nb = -q[2]; }
+    } else {
+        if (q[2] > 0) { nf = 4; na = q[0]; nb = -q[1]; }
+        else { nf = 5; na = -q[0]; nb = -q[1]; }
+    }
+    const int ni = (na + s - 1) / 2, nj = (nb + s - 1) / 2;
+    return (int32_t)(((size_t)nf * S + nj) * S + ni);
+}
+
+void orthonormal(const double n[3], double e1[3], double e2[3]) {
+    const double a[3] = {std::fabs(n[0]) < 0.9 ? 1.0 : 0.0, std::fabs(n[0]) < 0.9 ? 0.0 : 1.0, 0.0};
+    e1[0] = a[1] * n[2] - a[2] * n[1];
+    e1[1] = a[2] * n[0] - a[0] * n[2];
+    e1[2] = a[0] * n[1] - a[1] * n[0];
+    const double l = std::sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+    for (int j = 0; j < 3; ++j) e1[j] /= l;
+    e2[0] = n[1] * e1[2] - n[2] * e1[1];
+    e2[1] = n[2] * e1[0] - n[0] * e1[2];
+    e2[2] = n[0] * e1[1] - n[1] * e1[0];
+}
+
+// Windows (bounding boxes of the angular caps of radius t1 around every
+// output of a tile, on the face plane at the near level), aligned for the
+// GEMM's B loads: x0 to 8 texels, width to 32.
+std::vector<NearTile> near_tiles(uint32_t So, uint32_t Si, double t1, uint32_t* B_out) {
+    const uint32_t te = So / 16;
+    std::vector<NearTile> tiles(te * te);
+    int B = 0;
+    uint64_t woff = 0;
+    const double ct = std::cos(t1), st = std::sin(t1);
+    for (uint32_t ty = 0; ty < te; ++ty)
+        for (uint32_t tx = 0; tx < te; ++tx) {
+            double xmin = 1e30, xmax = -1e30, ymin = 1e30, ymax = -1e30;
+            for (uint32_t yy = 0; yy < 16; ++yy)
+                for (uint32_t xx = 0; xx < 16; ++xx) {
+                    const double u = hvx::contract_coord((int)(tx * 16 + xx), So);
+                    const double v = hvx::contract_coord((int)(ty * 16 + yy), So);
+                    const double r = 1.0 / std::sqrt(1.0 + u * u + v * v);
+                    const double n[3] = {u * r, v * r, r};
+                    double e1[3], e2[3];
+                    orthonormal(n, e1, e2);
+                    for (int j = 0; j < 64; ++j) {
+                        const double a = 2.0 * hvx::kContractPi * j / 64.0, ca = std::cos(a), sa = std::sin(a);
+                        double l[3];
+                        for (int q = 0; q < 3; ++q) l[q] = ct * n[q] + st * (ca * e1[q] + sa * e2[q]);
+                        const double pu = l[0] / l[2], pv = l[1] / l[2];
+                        const double px = ((pu + 1.0) * Si - 1.0) / 2.0, py = ((pv + 1.0) * Si - 1.0) / 2.0;
+                        xmin = std::min(xmin, px);
+                        xmax = std::max(xmax, px);
+                        ymin = std::min(ymin, py);
+                        ymax = std::max(ymax, py);
+                    }
+                }
+            const int x_lo = (int)std::floor(xmin) - 1, x_hi = (int)std::ceil(xmax) + 1;
+            const int y_lo = (int)std::floor(ymin) - 1, y_hi = (int)std::ceil(ymax) + 1;
+            NearTile& T = tiles[ty * te + tx];
+            T.x0 = x_lo >= 0 ? (x_lo / 8) * 8 : -(((-x_lo) + 7) / 8) * 8;
+            T.w = (uint32_t)(((x_hi + 1 - T.x0) + 31) / 32 * 32);
+            T.y0 = y_lo;
+            T.h = (uint32_t)(y_hi + 1 - y_lo);
+            T.woff = woff;
+            woff += (uint64_t)16 * T.h * (T.w / 32) * 64;
+            B = std::max({B, -T.x0, T.x0 + (int)T.w - (int)Si, -T.y0, T.y0 + (int)T.h - (int)Si});
+        }
+    *B_out = (uint32_t)((B + 7) / 8 * 8);
+    return tiles;
+}
+
+struct DenseMat {
+    DenseGeom G;
+    int stage;  // index into the staged dense levels
+    uint32_t splitk;
+    half8* W = nullptr;
+    float* rowscale = nullptr;
+};
+
+struct StageDense {
+    uint32_t Sin, level;
+    int32_t* perm = nullptr;
+    half8* R = nullptr;
+};
+
+struct StagePad {
+    uint32_t Si, level, B, Pw;
+    half8* P = nullptr;
+};
+
+struct MipPlan {
+    ContractLevel L{};
+    int dense = -1;  // FULL: the mip's matrix; SPLIT: the far matrix
+    int pad = -1;    // SPLIT: staged padded planes
+    NearGeom NG{};
+    std::vector<NearTile> tiles;
+    NearTile* d_tiles = nullptr;
+    half8* Wn = nullptr;
+    float* rowscale_n = nullptr;
+    float* invZ = nullptr;
+    float4* far_map = nullptr;
+    float4* zfar_map = nullptr;
+    uint64_t weight_bytes = 0;
+};
+
+}  // namespace
+
+struct hvx_contract_plan {
+    hvx_ctx* ctx = nullptr;
+    int type = 0, precision = 0;
+    uint32_t S = 0, So = 0, mips = 0;
+    std::vector<MipPlan> mp;
+    std::vector<DenseMat> dm;
+    std::vector<StageDense> sd;
+    std::vector<StagePad> sp;
+    uint32_t* fmax = nullptr;
+    float* sc = nullptr;
+    float* part = nullptr;
+    uint32_t min_level = 0;  // deepest input chain level read
+    std::vector<void*> allocs;
+
+    template <class T> hipError_t alloc(T** p, size_t count) {
+        void* q = nullptr;
+        const hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+        if (e == hipSuccess) allocs.push_back(q);
+        *p = static_cast<T*>(q);
+        return e;
+    }
+};
+
+namespace {
+
+int pfail(hvx_ctx* ctx, int st, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
+int pfail(hvx_ctx* ctx, int st, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->last_error = buf;
+    return st;
+}
+
+int stage_dense_index(hvx_contract_plan* P, uint32_t Sin) {
+    for (size_t i = 0; i < P->sd.size(); ++i)
+        if (P->sd[i].Sin == Sin) return (int)i;
+    StageDense s;
+    s.Sin = Sin;
+    s.level = ilog2(P->S / Sin);
+    P->sd.push_back(s);
+    return (int)P->sd.size() - 1;
+}
+
+hipError_t build_dense(hvx_contract_plan* P, DenseMat& D, int num_cus, float4* zmap) {
+    const DenseGeom& G = D.G;
+    hipError_t e;
+    double *Z = nullptr, *wm = nullptr;
+    if ((e = hipMalloc(&Z, G.rows * sizeof(double))) != hipSuccess) return e;
+    if ((e = hipMalloc(&wm, G.rows * sizeof(double))) != hipSuccess) {
+        (void)hipFree(Z);
+        return e;
+    }
+    const uint64_t n = (uint64_t)G.rt * G.ks * 64;
+    if ((e = P->alloc(&D.W, n)) == hipSuccess && (e = P->alloc(&D.rowscale, G.rows)) == hipSuccess) {
+        hipLaunchKernelGGL(dense_row_kernel, dim3(G.rows), dim3(256), 0, nullptr, G, Z, wm);
+        hipLaunchKernelGGL(dense_fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, G, wm, D.W);
+        hipLaunchKernelGGL(dense_scale_kernel, dim3((G.rows + 255) / 256), dim3(256), 0, nullptr, G.rows, Z, wm,
+                           G.piece == PIECE_FAR ? 1 : 0, D.rowscale, zmap);
+        e = hipDeviceSynchronize();
+    }
+    (void)hipFree(Z);
+    (void)hipFree(wm);
+    // K slices: enough waves to keep the weight stream in flight
+    const uint32_t pairs = (G.rt + 1) / 2;
+    const uint32_t want = (uint32_t)std::max(1, num_cus * 8) / std::max(pairs, 1u);
+    D.splitk = std::max(1u, std::min(want, std::max(1u, G.ks / 16)));
+    return e;
+}
+
+hipError_t build_near(hvx_contract_plan* P, MipPlan& M) {
+    hipError_t e;
+    const uint32_t So = M.L.So, rows = So * So;
+    uint64_t halfs8 = 0;
+    for (const NearTile& T : M.tiles) halfs8 += (uint64_t)16 * T.h * (T.w / 32) * 64;
+    double *Z = nullptr, *wm = nullptr;
+    if ((e = hipMalloc(&Z, rows * sizeof(double))) != hipSuccess) return e;
+    if ((e = hipMalloc(&wm, rows * sizeof(double))) != hipSuccess) {
+        (void)hipFree(Z);
+        return e;
+    }
+    if ((e = P->alloc(&M.d_tiles, M.tiles.size())) == hipSuccess &&
+        (e = hipMemcpy(M.d_tiles, M.tiles.data(), M.tiles.size() * sizeof(NearTile), hipMemcpyHostToDevice)) ==
+            hipSuccess &&
+        (e = P->alloc(&M.Wn, halfs8)) == hipSuccess && (e = P->alloc(&M.rowscale_n, rows)) == hipSuccess &&
+        (e = P->alloc(&M.invZ, rows)) == hipSuccess) {
+        hipLaunchKernelGGL(near_row_kernel, dim3(rows), dim3(256), 0, nullptr, M.NG, M.d_tiles, Z, wm);
+        hipLaunchKernelGGL(near_fill_kernel, dim3((unsigned)M.tiles.size() * 16), dim3(256), 0, nullptr, M.NG,
+                           M.d_tiles, wm, M.Wn);
+        hipLaunchKernelGGL(near_scale_kernel, dim3((rows + 255) / 256), dim3(256), 0, nullptr, M.NG, M.L.Sf, Z, wm,
+                           M.zfar_map, M.rowscale_n, M.invZ);
+        e = hipDeviceSynchronize();
+    }
+    M.weight_bytes += halfs8 * 16;
+    (void)hipFree(Z);
+    (void)hipFree(wm);
+    return e;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hvx_ibl_contract_plan_create(hvx_ctx* ctx, int type, uint32_t face, uint32_t out_face, uint32_t out_mips,
+                                 int precision, hvx_contract_plan** out) {
+    if (!ctx || !out) return HVX_ERR_NULL;
+    *out = nullptr;
+    if (type != HVX_FILTER_DIFFUSE && type != HVX_FILTER_SPECULAR)
+        return pfail(ctx, HVX_ERR_INVALID_ARG, "unknown filter type %d", type);
+    if (precision != HVX_PRECISION_FP16_SCALED)
+        return pfail(ctx, HVX_ERR_UNSUPPORTED, "precision %d is not implemented", precision);
+    if (type == HVX_FILTER_DIFFUSE && out_mips != 1)
+        return pfail(ctx, HVX_ERR_INVALID_ARG, "the diffuse filter writes one mip");
+    if (face < 4 || face > 8192 || (face & (face - 1)) || out_face < 1 || out_face > 4096 ||
+        (out_face & (out_face - 1)) || out_mips < 1 || out_mips > hvx_mip_count(out_face) ||
+        out_mips > hvx::kContractMaxMips)
+        return pfail(ctx, HVX_ERR_INVALID_ARG, "sizes: face %u and out_face %u must be powers of two (face >= 4), "
+                     "out_mips in [1, mips(out_face)]", face, out_face);
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || (cur != ctx->device && hipSetDevice(ctx->device) != hipSuccess))
+        return pfail(ctx, HVX_ERR_HIP, "cannot bind device %d", ctx->device);
+    hvx_contract_plan* P = new (std::nothrow) hvx_contract_plan();
+    if (!P) return HVX_ERR_HIP;
+    P->ctx = ctx;
+    P->type = type;
+    P->precision = precision;
+    P->S = face;
+    P->So = out_face;
+    P->mips = out_mips;
+    ContractLevel lv[hvx::kContractMaxMips];
+    const int ctype = type == HVX_FILTER_DIFFUSE ? hvx::CONTRACT_DIFFUSE : hvx::CONTRACT_SPECULAR;
+    hvx::contract_levels(ctype, face, out_face, out_mips, lv);
+    auto fail_destroy = [&](hipError_t e, const char* where) {
+        hvx_ibl_contract_plan_destroy(P);
+        return pfail(ctx, HVX_ERR_HIP, "%s: %s", where, hipGetErrorString(e));
+    };
+    hipError_t e;
+    size_t part_floats = 0;
+    P->mp.resize(out_mips);
+    for (uint32_t m = 0; m < out_mips; ++m) {
+        MipPlan& M = P->mp[m];
+        M.L = lv[m];
+        if (M.L.kind == hvx::CONTRACT_DELTA) continue;
+        DenseMat D;
+        D.G.type = ctype;
+        D.G.alpha2 = M.L.alpha2;
+        D.G.t0 = M.L.t0;
+        D.G.t1 = M.L.t1;
+        if (M.L.kind == hvx::CONTRACT_FULL) {
+            D.G.piece = PIECE_WHOLE;
+            D.G.So = M.L.So;
+            D.G.Sin = M.L.Si;
+        } else {
+            D.G.piece = PIECE_FAR;
+            D.G.So = M.L.Sf;
+            D.G.Sin = M.L.Sf;
+            if ((e = P->alloc(&M.far_map, 6 * (size_t)M.L.Sf * M.L.Sf)) != hipSuccess ||
+                (e = P->alloc(&M.zfar_map, 6 * (size_t)M.L.Sf * M.L.Sf)) != hipSuccess)
+                return fail_destroy(e, "far maps");
+        }
+        D.G.rows = D.G.So * D.G.So;
+        D.G.rt = (D.G.rows + 15) / 16;
+        D.G.ks = 6 * D.G.Sin * D.G.Sin / 32;
+        D.stage = stage_dense_index(P, D.G.Sin);
+        if ((e = build_dense(P, D, ctx->num_cus, M.zfar_map)) != hipSuccess) return fail_destroy(e, "dense weights");
+        M.weight_bytes += (uint64_t)D.G.rt * D.G.ks * 64 * 16;
+        part_floats = std::max(part_floats, (size_t)D.splitk * D.G.rt * 16 * 32);
+        P->dm.push_back(D);
+        M.dense = (int)P->dm.size() - 1;
+        if (M.L.kind == hvx::CONTRACT_SPLIT) {
+            M.NG = NearGeom{ctype, M.L.alpha2, M.L.t0, M.L.t1, M.L.So, M.L.Si, M.L.So / 16};
+            uint32_t B = 0;
+            M.tiles = near_tiles(M.L.So, M.L.Si, M.L.t1, &B);
+            StagePad sp;
+            sp.Si = M.L.Si;
+            sp.level = ilog2(face / M.L.Si);
+            sp.B = B;
+            sp.Pw = M.L.Si + 2 * B;
+            P->sp.push_back(sp);
+            M.pad = (int)P->sp.size() - 1;
+            if ((e = build_near(P, M)) != hipSuccess) return fail_destroy(e, "near weights");
+        }
+    }
+    // staged inputs and shared scratch
+    for (StageDense& s : P->sd) {
+        const uint32_t K = 6 * s.Sin * s.Sin;
+        std::vector<int32_t> perm((size_t)6 * K);
+        for (int f = 0; f < 6; ++f)
+            for (uint32_t k = 0; k < K; ++k) perm[(size_t)f * K + k] = perm_index(f, k, s.Sin);
+        if ((e = P->alloc(&s.perm, perm.size())) != hipSuccess ||
+            (e = hipMemcpy(s.perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = P->alloc(&s.R, (size_t)K / 8 * 32)) != hipSuccess)
+            return fail_destroy(e, "dense staging");
+        P->min_level = std::max(P->min_level, s.level);
+    }
+    for (StagePad& s : P->sp) {
+        if ((e = P->alloc(&s.P, (size_t)s.Pw * (s.Pw / 8) * 32)) != hipSuccess) return fail_destroy(e, "plane staging");
+        P->min_level = std::max(P->min_level, s.level);
+    }
+    if ((e = P->alloc(&P->fmax, 8)) != hipSuccess || (e = P->alloc(&P->sc, 16)) != hipSuccess ||
+        (e = P->alloc(&P->part, part_floats)) != hipSuccess)
+        return fail_destroy(e, "scratch");
+    if ((e = hipDeviceSynchronize()) != hipSuccess) return fail_destroy(e, "plan set-up");
+    *out = P;
+    return HVX_OK;
+}
+
+int hvx_ibl_contract_plan_destroy(hvx_contract_plan* P) {
+    if (!P) return HVX_ERR_NULL;
+    (void)hipDeviceSynchronize();
+    for (void* p : P->allocs) (void)hipFree(p);
+    delete P;
+    return HVX_OK;
+}
+
+int hvx_ibl_contract_plan_query(const hvx_contract_plan* P, uint32_t mip, hvx_contract_mip_info* info) {
+    if (!P || !info) return HVX_ERR_NULL;
+    if (mip >= P->mips) return HVX_ERR_INVALID_ARG;
+    const MipPlan& M = P->mp[mip];
+    info->kind = M.L.kind;
+    info->out_face = M.L.So;
+    info->in_face = M.L.Si;
+    info->far_face = M.L.Sf;
+    info->window_t0 = (float)M.L.t0;
+    info->window_t1 = (float)M.L.t1;
+    info->weight_bytes = M.weight_bytes;
+    info->min_input_level = P->min_level;
+    return HVX_OK;
+}
+
+int hvx_ibl_filter_contract(hvx_contract_plan* P, const float* d_chain, uint32_t chain_levels, float* d_out,
+                            void* stream) {
+    if (!P) return HVX_ERR_NULL;
+    hvx_ctx* ctx = P->ctx;
+    if (!d_chain || !d_out) return pfail(ctx, HVX_ERR_NULL, "NULL chain or output");
+    if (chain_levels <= P->min_level)
+        return pfail(ctx, HVX_ERR_INVALID_ARG, "the input chain needs %u levels (has %u); build it with "
+                     "hvx_ibl_cube_mips", P->min_level + 1, chain_levels);
+    if (((uintptr_t)d_chain | (uintptr_t)d_out) % 16) return pfail(ctx, HVX_ERR_ALIGN, "chains need 16-byte alignment");
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || (cur != ctx->device && hipSetDevice(ctx->device) != hipSuccess))
+        return pfail(ctx, HVX_ERR_HIP, "cannot bind device %d", ctx->device);
+    hipStream_t s = (hipStream_t)stream;
+    const float4* chain = reinterpret_cast<const float4*>(d_chain);
+    float4* outc = reinterpret_cast<float4*>(d_out);
+    auto level = [&](uint32_t l) { return chain + hvx::cube_level_offset_floats(P->S, l) / 4; };
+    // exponent scales from level 0 (every staged level is a box mean of it)
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, P->fmax, 6u, 0u);
+    const uint32_t S2 = P->S * P->S;
+    hipLaunchKernelGGL(facemax_kernel, dim3(std::min(64u, (S2 + 255) / 256), 6), dim3(256), 0, s, level(0), P->S,
+                       P->fmax);
+    hipLaunchKernelGGL(scales_kernel, dim3(1), dim3(64), 0, s, P->fmax, P->sc);
+    for (const StageDense& sd : P->sd) {
+        const uint32_t n = 6 * sd.Sin * sd.Sin / 8 * 32;
+        hipLaunchKernelGGL(stage_dense_kernel, dim3((n + 255) / 256), dim3(256), 0, s, level(sd.level), sd.Sin, sd.perm,
+                           P->sc, sd.R);
+    }
+    for (const StagePad& sp : P->sp) {
+        const uint32_t n = sp.Pw * (sp.Pw / 8) * 32;
+        hipLaunchKernelGGL(stage_pad_kernel, dim3((n + 255) / 256), dim3(256), 0, s, level(sp.level), sp.Si, sp.B,
+                           sp.Pw, P->sc, sp.P);
+    }
+    for (uint32_t m = 0; m < P->mips; ++m) {
+        const MipPlan& M = P->mp[m];
+        float4* dst = outc + hvx::cube_level_offset_floats(P->So, m) / 4;
+        const uint32_t So2 = M.L.So * M.L.So;
+        if (M.L.kind == hvx::CONTRACT_DELTA) {
+            hipLaunchKernelGGL(delta_kernel, dim3((6 * So2 + 255) / 256), dim3(256), 0, s, level(0), P->S, M.L.So, dst);
+            continue;
+        }
+        const DenseMat& D = P->dm[M.dense];
+        const uint32_t waves = (D.G.rt + 1) / 2 * D.splitk;
+        hipLaunchKernelGGL(dense_gemm_kernel, dim3((waves + 3) / 4), dim3(256), 0, s, D.W, P->sd[D.stage].R, D.G.rt,
+                           D.G.ks, D.splitk, P->part);
+        float4* dd = M.L.kind == hvx::CONTRACT_FULL ? dst : M.far_map;
+        hipLaunchKernelGGL(dense_epilogue_kernel, dim3((D.G.rows * 6 + 255) / 256), dim3(256), 0, s, P->part, D.splitk,
+                           D.G.rows, D.G.rt * 16, D.rowscale, P->sc, M.L.kind == hvx::CONTRACT_FULL ? 1.0f : 0.0f, dd);
+        if (M.L.kind == hvx::CONTRACT_SPLIT) {
+            const StagePad& sp = P->sp[M.pad];
+            hipLaunchKernelGGL(near_gemm_kernel, dim3((unsigned)M.tiles.size()), dim3(512), 0, s, M.Wn, M.d_tiles, sp.P,
+                               sp.Pw / 8, sp.B, M.NG.te, M.L.So, M.rowscale_n, P->sc, reinterpret_cast<float*>(dst));
+            hipLaunchKernelGGL(far_add_kernel, dim3((6 * So2 + 255) / 256), dim3(256), 0, s, M.far_map, M.L.Sf, M.L.So,
+                               M.invZ, dst);
+        }
+    }
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? HVX_OK : pfail(ctx, HVX_ERR_HIP, "hvx_ibl_filter_contract: %s", hipGetErrorString(e));
+}
+
+}  // extern "C"

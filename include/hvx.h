@@ -252,6 +252,48 @@ int hvx_ibl_filter(hvx_ctx* ctx, int type, const float* d_cube,
                    uint32_t sample_count, const hvx_filter_opts* opts,
                    void* stream);
 
+/* ------------------------------------------------- contraction (MFMA) filter */
+
+/* Arithmetic of the contraction filter's MFMA products. */
+typedef enum hvx_precision {
+    HVX_PRECISION_FP16_SCALED = 0, /* FP16 weights and radiance, power-of-two scaled per row / per face */
+    HVX_PRECISION_BF16_SPLIT = 1   /* split-BF16 compensated products (hi*hi + hi*lo + lo*hi) */
+} hvx_precision;
+
+typedef struct hvx_contract_plan hvx_contract_plan;
+
+typedef struct hvx_contract_mip_info {
+    int kind;                 /* 0 delta (roughness 0), 1 full-sphere sum, 2 near window + far field */
+    uint32_t out_face;
+    uint32_t in_face;         /* input level summed (kind 1) or near level (kind 2) */
+    uint32_t far_face;        /* kind 2: far level and far grid */
+    float window_t0, window_t1; /* kind 2: angular window radii (radians) */
+    uint64_t weight_bytes;    /* precomputed weights of this mip */
+    uint32_t min_input_level; /* deepest level of the input chain the plan reads */
+    uint32_t _pad;
+} hvx_contract_mip_info;
+
+/* The cube filters recast as dense contractions of precomputed GGX (type 1)
+ * or cosine (type 0) weight matrices with radiance texels, on FP16 MFMA with
+ * FP32 accumulation; the weights' row normalisation sums are FP64, on the
+ * GPU. This is a deterministic quadrature of the integral the reference's
+ * filters estimate (CubeFilterSpecular.frag:34-75 by 1024 GGX samples,
+ * CubeFilterDiffuse.frag:35-65 by a 252x63 grid), not a bit-compatible
+ * variant of them: hvx_ibl_filter remains the drop-in. The plan builds the
+ * weights for an input cube of face `face` and an output chain of face
+ * `out_face` with `out_mips` levels (powers of two; diffuse: one level). A
+ * plan serves any number of environments; calls sharing one plan must not
+ * overlap in time (its staging buffers are reused). */
+int hvx_ibl_contract_plan_create(hvx_ctx* ctx, int type, uint32_t face, uint32_t out_face, uint32_t out_mips,
+                                 int precision, hvx_contract_plan** out);
+int hvx_ibl_contract_plan_destroy(hvx_contract_plan* plan);
+int hvx_ibl_contract_plan_query(const hvx_contract_plan* plan, uint32_t mip, hvx_contract_mip_info* info);
+/* d_chain: the input cube with its box mip chain (hvx_ibl_cube_mips), at
+ * least min_input_level + 1 levels; d_out: hvx_cube_chain_floats(out_face,
+ * out_mips) floats. Allocates nothing; may be captured into a graph. */
+int hvx_ibl_filter_contract(hvx_contract_plan* plan, const float* d_chain, uint32_t chain_levels, float* d_out,
+                            void* stream);
+
 /* ---------------------------------------------------------------- batches */
 
 /* One light probe of a prefilter batch: its level-0 cube and where its GGX

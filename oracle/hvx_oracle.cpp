@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../include/hvx.h"
+#include "../hellovulkan_amd/csrc/hvx_contract.h"
 #include "../hellovulkan_amd/csrc/hvx_math.h"
 
 namespace {
@@ -217,18 +218,18 @@ inline Texel cube_edge_texel(int f, int i, int j, int S) {
     return point_texel(P, S);
 }
 
-template <class T>
-void fetch_cube_texel(const float* level, int S, int f, int i, int j, T out[3]) {
+template <class T, class E>
+void fetch_cube_texel(const E* level, int S, int f, int i, int j, T out[3]) {
     const bool iin = i >= 0 && i < S, jin = j >= 0 && j < S;
     auto at = [&](int ff, int ii, int jj) { return level + (((size_t)ff * S + jj) * S + ii) * 4; };
     if (iin && jin) {
-        const float* p = at(f, i, j);
+        const E* p = at(f, i, j);
         for (int c = 0; c < 3; ++c) out[c] = T(p[c]);
         return;
     }
     if (iin || jin) {
         const Texel t = cube_edge_texel(f, i, j, S);
-        const float* p = at(t.face, t.i, t.j);
+        const E* p = at(t.face, t.i, t.j);
         for (int c = 0; c < 3; ++c) out[c] = T(p[c]);
         return;
     }
@@ -236,14 +237,14 @@ void fetch_cube_texel(const float* level, int S, int f, int i, int j, T out[3]) 
     const int ci = i < 0 ? 0 : S - 1, cj = j < 0 ? 0 : S - 1;
     const Texel ta = cube_edge_texel(f, i, cj, S);
     const Texel tb = cube_edge_texel(f, ci, j, S);
-    const float* p0 = at(f, ci, cj);
-    const float* pa = at(ta.face, ta.i, ta.j);
-    const float* pb = at(tb.face, tb.i, tb.j);
+    const E* p0 = at(f, ci, cj);
+    const E* pa = at(ta.face, ta.i, ta.j);
+    const E* pb = at(tb.face, tb.i, tb.j);
     for (int c = 0; c < 3; ++c) out[c] = ((T(p0[c]) + T(pa[c])) + T(pb[c])) / T(3);
 }
 
 // Bilinear seamless sample of one cube level (RGB).
-template <class T> void sample_cube(const float* level, int S, V3<T> dir, T out[3]) {
+template <class T, class E> void sample_cube(const E* level, int S, V3<T> dir, T out[3]) {
     T s, t;
     const int f = cube_face_st(dir, &s, &t);
     const T u = s * T(S) - T(0.5);
@@ -252,10 +253,10 @@ template <class T> void sample_cube(const float* level, int S, V3<T> dir, T out[
     const T a = u - fu, b = v - fv;
     const int i0 = (int)fu, j0 = (int)fv;
     T t00[3], t10[3], t01[3], t11[3];
-    fetch_cube_texel<T>(level, S, f, i0, j0, t00);
-    fetch_cube_texel<T>(level, S, f, i0 + 1, j0, t10);
-    fetch_cube_texel<T>(level, S, f, i0, j0 + 1, t01);
-    fetch_cube_texel<T>(level, S, f, i0 + 1, j0 + 1, t11);
+    fetch_cube_texel<T, E>(level, S, f, i0, j0, t00);
+    fetch_cube_texel<T, E>(level, S, f, i0 + 1, j0, t10);
+    fetch_cube_texel<T, E>(level, S, f, i0, j0 + 1, t01);
+    fetch_cube_texel<T, E>(level, S, f, i0 + 1, j0 + 1, t11);
     const T w00 = (T(1) - a) * (T(1) - b), w10 = a * (T(1) - b);
     const T w01 = (T(1) - a) * b, w11 = a * b;
     for (int c = 0; c < 3; ++c) {
@@ -627,6 +628,189 @@ MapsHost maps_of(const hvx_ibl_maps* m) {
             m->brdf_lut, m->lut_width, m->lut_height};
 }
 
+// ====================================================== contraction filter
+// FP64 twin of the dense-contraction cube filter (hvx_contract.h gives the
+// operator and the per-mip levels). Evaluated directly in world space: for
+// an output texel direction n, the sums run over the world texels of the
+// input levels (box means of level 0 in double) and, for the near field,
+// over the face-plane grid of the output's face, scanning a box around n
+// that is verified to contain the whole angular window. No canonical-face
+// weights and no texel permutations are involved.
+
+// Box levels 0..upto of a cube given level 0 (RGBA, 4 doubles per texel).
+std::vector<std::vector<double>> box_levels(const float* level0, uint32_t S, uint32_t upto) {
+    std::vector<std::vector<double>> L(upto + 1);
+    L[0].assign(level0, level0 + (size_t)6 * S * S * 4);
+    for (uint32_t l = 1; l <= upto; ++l) {
+        const uint32_t sp = S >> (l - 1), sn = S >> l;
+        L[l].assign((size_t)6 * sn * sn * 4, 0.0);
+        for (uint32_t f = 0; f < 6; ++f)
+            for (uint32_t y = 0; y < sn; ++y)
+                for (uint32_t x = 0; x < sn; ++x)
+                    for (int c = 0; c < 4; ++c) {
+                        auto at = [&](uint32_t xx, uint32_t yy) {
+                            return L[l - 1][(((size_t)f * sp + yy) * sp + xx) * 4 + c];
+                        };
+                        L[l][(((size_t)f * sn + y) * sn + x) * 4 + c] =
+                            ((at(2 * x, 2 * y) + at(2 * x + 1, 2 * y)) + (at(2 * x, 2 * y + 1) + at(2 * x + 1, 2 * y + 1))) *
+                            0.25;
+                    }
+    }
+    return L;
+}
+
+V3<double> plane_dir(int f, double u, double v) {
+    double p[3];
+    hvx::contract_face_point(f, u, v, p);
+    return normalize(V3<double>{p[0], p[1], p[2]});
+}
+
+// sum over every texel of a level: w(n.l) * piece * dw * L  (and sum of w)
+// piece: 0 whole, 2 far (1 - phi)
+void level_sum(const std::vector<double>& lev, uint32_t Sl, int type, double alpha2, int piece, double t0, double t1,
+               V3<double> n, double acc[3], double* wsum) {
+    acc[0] = acc[1] = acc[2] = 0.0;
+    double z = 0.0;
+    for (uint32_t g = 0; g < 6; ++g)
+        for (uint32_t y = 0; y < Sl; ++y)
+            for (uint32_t x = 0; x < Sl; ++x) {
+                const double u = hvx::contract_coord((int)x, Sl), v = hvx::contract_coord((int)y, Sl);
+                const V3<double> l = plane_dir((int)g, u, v);
+                const double c = dot(n, l);
+                double w = hvx::contract_weight(type, alpha2, c);
+                if (w == 0.0) continue;
+                w *= hvx::contract_dw(u, v, Sl);
+                if (piece == 2) w *= 1.0 - hvx::contract_phi(t0, t1, c);
+                const double* t = &lev[(((size_t)g * Sl + y) * Sl + x) * 4];
+                acc[0] += w * t[0];
+                acc[1] += w * t[1];
+                acc[2] += w * t[2];
+                z += w;
+            }
+    *wsum = z;
+}
+
+// the near field of output texel (f, X, Y) of an So face: sum over the face
+// plane grid at the near level (in-face texels, else seamless bilinear
+// samples) of w * phi * dw * L
+void near_sum(const std::vector<double>& lev, uint32_t Si, int type, double alpha2, double t0, double t1, int f,
+              uint32_t X, uint32_t Y, uint32_t So, double acc[3], double* wsum) {
+    const double uo = hvx::contract_coord((int)X, So), vo = hvx::contract_coord((int)Y, So);
+    const V3<double> n = plane_dir(f, uo, vo);
+    const int cx = (int)std::floor(((uo + 1.0) * Si - 1.0) / 2.0), cy = (int)std::floor(((vo + 1.0) * Si - 1.0) / 2.0);
+    for (int R = 16;; R *= 2) {
+        // the scan box must hold the whole window: no weight on its border
+        bool border = false;
+        for (int k = -R; k <= R && !border; ++k)
+            for (int side = 0; side < 4 && !border; ++side) {
+                const int px = side < 2 ? cx + k : cx + (side == 2 ? -R : R);
+                const int py = side < 2 ? cy + (side == 0 ? -R : R) : cy + k;
+                const V3<double> l = plane_dir(f, hvx::contract_coord(px, Si), hvx::contract_coord(py, Si));
+                border = hvx::contract_phi(t0, t1, dot(n, l)) > 0.0;
+            }
+        if (border) continue;
+        acc[0] = acc[1] = acc[2] = 0.0;
+        double z = 0.0;
+        for (int py = cy - R; py <= cy + R; ++py)
+            for (int px = cx - R; px <= cx + R; ++px) {
+                const double u = hvx::contract_coord(px, Si), v = hvx::contract_coord(py, Si);
+                const V3<double> l = plane_dir(f, u, v);
+                const double c = dot(n, l);
+                const double ph = hvx::contract_phi(t0, t1, c);
+                if (ph == 0.0) continue;
+                const double w = hvx::contract_weight(type, alpha2, c) * hvx::contract_dw(u, v, Si) * ph;
+                if (w == 0.0) continue;
+                double t[3];
+                if (px >= 0 && py >= 0 && px < (int)Si && py < (int)Si) {
+                    const double* q = &lev[(((size_t)f * Si + py) * Si + px) * 4];
+                    t[0] = q[0];
+                    t[1] = q[1];
+                    t[2] = q[2];
+                } else {
+                    sample_cube<double, double>(lev.data(), (int)Si, l, t);
+                }
+                acc[0] += w * t[0];
+                acc[1] += w * t[1];
+                acc[2] += w * t[2];
+                z += w;
+            }
+        *wsum = z;
+        return;
+    }
+}
+
+void contract_f64(int type, const float* level0, uint32_t S, uint32_t So, uint32_t mips, uint32_t mip,
+                  const uint32_t* ids, uint32_t n_ids, double* out, int threads) {
+    hvx::ContractLevel lv[hvx::kContractMaxMips];
+    const int ctype = type == 0 ? hvx::CONTRACT_DIFFUSE : hvx::CONTRACT_SPECULAR;
+    if (!hvx::contract_levels(ctype, S, So, mips, lv) || mip >= mips) return;
+    const hvx::ContractLevel& L = lv[mip];
+    const uint32_t Som = L.So;
+    if (L.kind == hvx::CONTRACT_DELTA) {
+        parallel_for(n_ids, threads, [&](uint32_t i) {
+            const uint32_t id = ids[i], f = id / (Som * Som), o = id % (Som * Som);
+            const V3<double> n = texel_dir<double>((int)f, o % Som, o / Som, Som);
+            sample_cube<double, float>(level0, (int)S, n, out + 3 * (size_t)i);
+        });
+        return;
+    }
+    auto level_of = [&](uint32_t face) {
+        uint32_t l = 0;
+        while ((S >> l) > face) ++l;
+        return l;
+    };
+    const auto lev = box_levels(level0, S, level_of(L.kind == hvx::CONTRACT_SPLIT ? L.Sf : L.Si));
+    if (L.kind == hvx::CONTRACT_FULL) {
+        const auto& li = lev[level_of(L.Si)];
+        parallel_for(n_ids, threads, [&](uint32_t i) {
+            const uint32_t id = ids[i], f = id / (Som * Som), o = id % (Som * Som);
+            const V3<double> n = texel_dir<double>((int)f, o % Som, o / Som, Som);
+            double acc[3], z;
+            level_sum(li, L.Si, ctype, L.alpha2, 0, 0.0, 0.0, n, acc, &z);
+            for (int c = 0; c < 3; ++c) out[3 * (size_t)i + c] = acc[c] / z;
+        });
+        return;
+    }
+    // SPLIT: far maps (numerator RGB, normaliser in .w) at the Sf grid
+    const uint32_t Sf = L.Sf;
+    const auto& lf = lev[level_of(Sf)];
+    std::vector<double> far((size_t)6 * Sf * Sf * 4), zfar((size_t)6 * Sf * Sf * 4, 0.0);
+    parallel_for(6 * Sf * Sf, threads, [&](uint32_t t) {
+        const uint32_t g = t / (Sf * Sf), o = t % (Sf * Sf);
+        const V3<double> n = plane_dir((int)g, hvx::contract_coord((int)(o % Sf), Sf),
+                                       hvx::contract_coord((int)(o / Sf), Sf));
+        double acc[3], z;
+        level_sum(lf, Sf, ctype, L.alpha2, 2, L.t0, L.t1, n, acc, &z);
+        for (int c = 0; c < 3; ++c) far[(size_t)t * 4 + c] = acc[c];
+        far[(size_t)t * 4 + 3] = 0.0;
+        zfar[(size_t)t * 4] = z;
+    });
+    const auto& ln = lev[level_of(L.Si)];
+    parallel_for(n_ids, threads, [&](uint32_t i) {
+        const uint32_t id = ids[i], f = id / (Som * Som), o = id % (Som * Som), X = o % Som, Y = o / Som;
+        const V3<double> n = texel_dir<double>((int)f, X, Y, Som);
+        double acc[3], z, F[3], Zf[3];
+        near_sum(ln, L.Si, ctype, L.alpha2, L.t0, L.t1, (int)f, X, Y, Som, acc, &z);
+        sample_cube<double, double>(far.data(), (int)Sf, n, F);
+        sample_cube<double, double>(zfar.data(), (int)Sf, n, Zf);
+        for (int c = 0; c < 3; ++c) out[3 * (size_t)i + c] = (acc[c] + F[c]) / (z + Zf[0]);
+    });
+}
+
+// The same integral summed over every texel of level 0 (no level choice, no
+// near/far split): the accuracy reference both filters are compared with.
+void ideal_f64(int type, const float* level0, uint32_t S, double alpha2, uint32_t Som, const uint32_t* ids,
+               uint32_t n_ids, double* out, int threads) {
+    std::vector<double> lev(level0, level0 + (size_t)6 * S * S * 4);
+    parallel_for(n_ids, threads, [&](uint32_t i) {
+        const uint32_t id = ids[i], f = id / (Som * Som), o = id % (Som * Som);
+        const V3<double> n = texel_dir<double>((int)f, o % Som, o / Som, Som);
+        double acc[3], z;
+        level_sum(lev, S, type == 0 ? hvx::CONTRACT_DIFFUSE : hvx::CONTRACT_SPECULAR, alpha2, 0, 0.0, 0.0, n, acc, &z);
+        for (int c = 0; c < 3; ++c) out[3 * (size_t)i + c] = acc[c] / z;
+    });
+}
+
 }  // namespace
 
 // ======================================================================= C API
@@ -824,6 +1008,32 @@ void hvxo_ibl_specular_f64(const float* cube, uint32_t S, uint32_t So, uint32_t 
     const double rough = specular_roughness<double>(mip, mips);
     parallel_for(n_ids, threads,
                  [&](uint32_t k) { specular_texel<double>(cube, S, Sm, D, rough, n, ids[k], out + 4 * k); });
+}
+
+// ---- contraction filter: FP64 twin and the level-0 accuracy reference
+void hvxo_contract_f64(int type, const float* level0, uint32_t S, uint32_t So, uint32_t mips, uint32_t mip,
+                       const uint32_t* ids, uint32_t n_ids, double* out, int threads) {
+    contract_f64(type, level0, S, So, mips, mip, ids, n_ids, out, threads);
+}
+void hvxo_ideal_f64(int type, const float* level0, uint32_t S, double alpha2, uint32_t So_mip, const uint32_t* ids,
+                    uint32_t n_ids, double* out, int threads) {
+    ideal_f64(type, level0, S, alpha2, So_mip, ids, n_ids, out, threads);
+}
+// per-mip levels of the contraction filter: kind, So, Si, Sf per mip
+int hvxo_contract_levels(int type, uint32_t S, uint32_t So, uint32_t mips, uint32_t* out /* 4 * mips */,
+                         double* windows /* 3 * mips: alpha2, t0, t1 */) {
+    hvx::ContractLevel lv[hvx::kContractMaxMips];
+    if (!hvx::contract_levels(type == 0 ? hvx::CONTRACT_DIFFUSE : hvx::CONTRACT_SPECULAR, S, So, mips, lv)) return 0;
+    for (uint32_t m = 0; m < mips; ++m) {
+        out[4 * m] = (uint32_t)lv[m].kind;
+        out[4 * m + 1] = lv[m].So;
+        out[4 * m + 2] = lv[m].Si;
+        out[4 * m + 3] = lv[m].Sf;
+        windows[3 * m] = lv[m].alpha2;
+        windows[3 * m + 1] = lv[m].t0;
+        windows[3 * m + 2] = lv[m].t1;
+    }
+    return (int)mips;
 }
 
 }  // extern "C"

@@ -45,6 +45,9 @@ _SIGS = [
     ("hvxo_ibl_diffuse_f64", None, [vp, u32, u32, vp, u32, vp, C.c_int]),
     ("hvxo_ibl_specular_f32", None, [vp, u32, u32, u32, u32, u32, C.c_int, vp, u32, vp, C.c_int]),
     ("hvxo_ibl_specular_f64", None, [vp, u32, u32, u32, u32, u32, C.c_int, vp, u32, vp, C.c_int]),
+    ("hvxo_contract_f64", None, [C.c_int, vp, u32, u32, u32, u32, vp, u32, vp, C.c_int]),
+    ("hvxo_ideal_f64", None, [C.c_int, vp, u32, C.c_double, u32, vp, u32, vp, C.c_int]),
+    ("hvxo_contract_levels", C.c_int, [C.c_int, u32, u32, u32, vp, vp]),
 ]
 
 _lib = None
@@ -220,6 +223,41 @@ def specular(cube0: np.ndarray, S: int, So: int, mips: int, mip: int, n: int, id
     fn = lib().hvxo_ibl_specular_f32 if dtype == np.float32 else lib().hvxo_ibl_specular_f64
     fn(cube0.ctypes.data, S, So, mips, mip, n, 1 if emulate else 0, ids.ctypes.data, ids.size, out.ctypes.data,
        _threads(threads))
+    return out
+
+
+CONTRACT_DELTA, CONTRACT_FULL, CONTRACT_SPLIT = 0, 1, 2
+
+
+def contract_levels(ftype: int, S: int, So: int, mips: int) -> list[dict]:
+    """Per-mip levels of the contraction filter (kind, So, Si, Sf, alpha2, t0, t1)."""
+    lv = np.zeros(4 * mips, np.uint32)
+    win = np.zeros(3 * mips, np.float64)
+    if lib().hvxo_contract_levels(ftype, S, So, mips, lv.ctypes.data, win.ctypes.data) != mips:
+        raise ValueError("invalid contraction sizes")
+    return [dict(kind=int(lv[4 * m]), So=int(lv[4 * m + 1]), Si=int(lv[4 * m + 2]), Sf=int(lv[4 * m + 3]),
+                 alpha2=float(win[3 * m]), t0=float(win[3 * m + 1]), t1=float(win[3 * m + 2])) for m in range(mips)]
+
+
+def contract(ftype: int, level0: np.ndarray, S: int, So: int, mips: int, mip: int, ids=None,
+             threads=None) -> np.ndarray:
+    """FP64 twin of the contraction filter (RGB) at output texels `ids` of mip `mip`."""
+    level0 = _c(level0[: 6 * S * S * 4], np.float32)
+    Sm = max(So >> mip, 1)
+    ids = np.arange(6 * Sm * Sm, dtype=np.uint32) if ids is None else _c(ids, np.uint32)
+    out = np.full((ids.size, 3), np.nan, np.float64)
+    lib().hvxo_contract_f64(ftype, level0.ctypes.data, S, So, mips, mip, ids.ctypes.data, ids.size, out.ctypes.data,
+                            _threads(threads))
+    return out
+
+
+def ideal(ftype: int, level0: np.ndarray, S: int, alpha2: float, So_mip: int, ids, threads=None) -> np.ndarray:
+    """The filter integral summed over every level-0 texel (RGB), FP64."""
+    level0 = _c(level0[: 6 * S * S * 4], np.float32)
+    ids = _c(ids, np.uint32)
+    out = np.full((ids.size, 3), np.nan, np.float64)
+    lib().hvxo_ideal_f64(ftype, level0.ctypes.data, S, alpha2, So_mip, ids.ctypes.data, ids.size, out.ctypes.data,
+                         _threads(threads))
     return out
 
 

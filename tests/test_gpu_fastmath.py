@@ -40,3 +40,22 @@ def test_rcp_rn_normal_exhaustive(gpu, lo, hi):
     a, b = sorted((_bits(lo), _bits(hi)))
     mism, first = _run(a, b - a, 1)
     assert mism == 0, f"{mism} mismatches, first at bits {first:#x}"
+
+
+@pytest.mark.parametrize("bf16", [0, 1], ids=["f16", "bf16"])
+def test_mfma_16x16x32_lane_layout(gpu, bf16):
+    """The operand/result lane layout the IBL contraction kernels assume for
+    v_mfma_f32_16x16x32_{f16,bf16}: the product of exactly representable
+    operands must equal the float64 matrix product."""
+    import torch
+    lib = C.CDLL(LIB)
+    lib.hvxt_mfma_layout.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    rng = np.random.default_rng(3 + bf16)
+    # small integers times powers of two: exact in both f16 and bf16, products exact in f32
+    a = (rng.integers(-8, 9, (16, 32)) * 2.0 ** rng.integers(-3, 3, (16, 32))).astype(np.float32)
+    b = (rng.integers(-8, 9, (32, 16)) * 2.0 ** rng.integers(-3, 3, (32, 16))).astype(np.float32)
+    da, db = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    dc = torch.full((16, 16), np.nan, dtype=torch.float32, device="cuda")
+    assert lib.hvxt_mfma_layout(bf16, da.data_ptr(), db.data_ptr(), dc.data_ptr()) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(dc.cpu().numpy(), (a.astype(np.float64) @ b.astype(np.float64)).astype(np.float32))
