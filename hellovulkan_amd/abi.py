@@ -153,6 +153,14 @@ SIGNATURES = [
     ("hvx_ibl_contract_plan_destroy", C.c_int, [vp]),
     ("hvx_ibl_contract_plan_query", C.c_int, [vp, u32, P(ContractMipInfo)]),
     ("hvx_ibl_filter_contract", C.c_int, [vp, vp, u32, vp, vp]),
+    ("hvx_mlp_create", C.c_int, [vp, C.c_float, C.c_uint64, P(vp)]),
+    ("hvx_mlp_destroy", C.c_int, [vp]),
+    ("hvx_mlp_get_params", C.c_int, [vp, vp]),
+    ("hvx_mlp_set_params", C.c_int, [vp, vp]),
+    ("hvx_mlp_sample", C.c_int, [vp, vp, u32, u32, C.c_uint64, u32, vp, vp, vp]),
+    ("hvx_mlp_gradient", C.c_int, [vp, vp, vp, u32, vp, vp, vp]),
+    ("hvx_mlp_train", C.c_int, [vp, vp, vp, u32, u32, u32, C.c_float, vp, vp]),
+    ("hvx_mlp_eval", C.c_int, [vp, vp, u32, vp, vp]),
     ("hvx_batch_create", C.c_int, [vp, P(Probe), u32, u32, P(View), u32, P(vp)]),
     ("hvx_batch_destroy", C.c_int, [vp]),
     ("hvx_batch_set_tuning", C.c_int, [vp, P(BatchTuning)]),
@@ -381,6 +389,58 @@ class ContractPlan:
     def filter(self, chain, chain_levels: int, out, stream=None):
         self.ctx._check(self.lib.hvx_ibl_filter_contract(self.h, _ptr(chain), chain_levels, _ptr(out),
                                                          self.ctx._stream(stream)), "hvx_ibl_filter_contract")
+
+
+MLP_PARAMS = 16771
+
+
+class Mlp:
+    """Owns an hvx_mlp: the per-environment radiance network."""
+
+    def __init__(self, ctx: Context, lod_max: float, seed: int = 1):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        h = C.c_void_p()
+        ctx._check(self.lib.hvx_mlp_create(ctx.h, lod_max, seed, C.byref(h)), "hvx_mlp_create")
+        self.h = h
+        self.lod_max = lod_max
+
+    def close(self):
+        if self.h:
+            self.lib.hvx_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        p = np.empty(MLP_PARAMS, np.float32)
+        self.ctx._check(self.lib.hvx_mlp_get_params(self.h, p.ctypes.data), "hvx_mlp_get_params")
+        return p
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == MLP_PARAMS
+        self.ctx._check(self.lib.hvx_mlp_set_params(self.h, p.ctypes.data), "hvx_mlp_set_params")
+
+    def sample(self, chain, face: int, mips: int, seed: int, n: int, inputs, targets, stream=None):
+        self.ctx._check(self.lib.hvx_mlp_sample(self.h, _ptr(chain), face, mips, seed, n, _ptr(inputs), _ptr(targets),
+                                                self.ctx._stream(stream)), "hvx_mlp_sample")
+
+    def gradient(self, inputs, targets, batch: int, grads, loss=None, stream=None):
+        self.ctx._check(self.lib.hvx_mlp_gradient(self.h, _ptr(inputs), _ptr(targets), batch, _ptr(grads), _ptr(loss),
+                                                  self.ctx._stream(stream)), "hvx_mlp_gradient")
+
+    def train(self, inputs, targets, n: int, batch: int, steps: int, lr: float, loss_curve=None, stream=None):
+        self.ctx._check(self.lib.hvx_mlp_train(self.h, _ptr(inputs), _ptr(targets), n, batch, steps, lr,
+                                               _ptr(loss_curve), self.ctx._stream(stream)), "hvx_mlp_train")
+
+    def eval(self, inputs, n: int, rgb, stream=None):
+        self.ctx._check(self.lib.hvx_mlp_eval(self.h, _ptr(inputs), n, _ptr(rgb), self.ctx._stream(stream)),
+                        "hvx_mlp_eval")
 
 
 class Batch:

@@ -1,0 +1,581 @@
+// mlp.hip — the per-environment radiance network (hvx_mlp_* of
+// include/hvx.h), trained with hand-written BF16 MFMA forward and backward
+// kernels and FP32 master weights (Adam).
+//
+// One training step over a batch of B samples (B a multiple of 256):
+//   pe            X0 = PE(d, lod)                        bf16 [B][128] (+ transpose)
+//   gemm FWD x3   H_l = relu(H_{l-1} W_l^T + b_l)        bf16 [B][64]  (+ transpose)
+//   out           y = H3 W4^T + b4, loss, dY, dZ3 = (dY W4) * [H3 > 0], dW4/db4 partials
+//   per layer l = 3, 2, 1:
+//     gemm PART   dW_l = dZ_l^T H_{l-1}   (K = batch, split over workgroups, reduced in order)
+//     colsum      db_l = sum_b dZ_l
+//     gemm BWD    dZ_{l-1} = (dZ_l W_l) * [H_{l-1} > 0]   (l > 1)
+//   adam          FP32 master update, then BF16 copies of W1..W3 (and W2^T, W3^T)
+// Every GEMM is C = A Bt^T with both operands row-major along K, so each MFMA
+// operand load is one contiguous 16-byte read; the epilogues also write the
+// transposed activation that the weight-gradient GEMMs read that way.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "hvx_device.h"
+#include "hvx_internal.h"
+#include "hvx_mfma.h"
+#include "hvx_mlp.h"
+
+namespace {
+
+using namespace hvx;
+typedef __bf16 bf16;
+
+constexpr int kIn = hvx::kMlpIn, kH = hvx::kMlpHidden;
+constexpr uint32_t kMaxBatch = 1u << 20;
+
+// ------------------------------------------------------------------- data
+__device__ __forceinline__ uint64_t smix(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+__device__ __forceinline__ float u01(uint64_t seed, uint64_t stream, uint64_t i) {
+    return (float)((double)(smix(seed + (stream << 40) + i) >> 11) * (1.0 / 9007199254740992.0));
+}
+
+// textureLod of the prefiltered chain (trilinear, LOD clamped to [0, mips-1])
+__device__ __forceinline__ f3 chain_lod(const float4* chain, uint32_t S, uint32_t mips, f3 dir, float lod) {
+    const float q = (float)(mips - 1);
+    const float lam = gclamp(lod, 0.0f, q);
+    const float fl = floorf_(lam);
+    const uint32_t l0 = (uint32_t)fl, l1 = min(l0 + 1, mips - 1);
+    const float d = lam - fl;
+    const int s0 = (int)max(S >> l0, 1u), s1 = (int)max(S >> l1, 1u);
+    const f3 c0 = sample_cube(chain + cube_level_offset_floats(S, l0) / 4, s0, dir);
+    const f3 c1 = sample_cube(chain + cube_level_offset_floats(S, l1) / 4, s1, dir);
+    return {(1.0f - d) * c0.x + d * c1.x, (1.0f - d) * c0.y + d * c1.y, (1.0f - d) * c0.z + d * c1.z};
+}
+
+// n samples: direction uniform on the sphere, lod uniform in [0, lod_max];
+// target log(textureLod(chain, d, lod) + eps) per channel
+__global__ __launch_bounds__(256) void mlp_data_kernel(const float4* __restrict__ chain, uint32_t S, uint32_t mips,
+                                                       float lod_max, uint64_t seed, uint32_t n,
+                                                       float4* __restrict__ in, float4* __restrict__ tgt) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float z = 2.0f * u01(seed, 0, i) - 1.0f, ph = 6.2831853071795864f * u01(seed, 1, i);
+    const float r = sqrtf(fmaxf(1.0f - z * z, 0.0f));
+    const f3 d = {r * cosf(ph), r * sinf(ph), z};
+    const float lod = lod_max * u01(seed, 2, i);
+    const f3 c = chain_lod(chain, S, mips, d, lod);
+    const float e = (float)hvx::kMlpEps;
+    in[i] = make_float4(d.x, d.y, d.z, lod);
+    tgt[i] = make_float4(logf(fmaxf(c.x, 0.0f) + e), logf(fmaxf(c.y, 0.0f) + e), logf(fmaxf(c.z, 0.0f) + e), 0.0f);
+}
+
+// ----------------------------------------------------------------- encoding
+// thread per (sample, input j, frequency k); rows >= n are the encoding of 0
+__global__ __launch_bounds__(256) void mlp_pe_kernel(const float4* __restrict__ in, uint32_t n, uint32_t B,
+                                                     float lod_max, bf16* __restrict__ X, bf16* __restrict__ XT) {
+    const uint32_t id = blockIdx.x * 256 + threadIdx.x;
+    if (id >= B * 64) return;
+    const uint32_t b = id >> 6, j = (id >> 4) & 3, k = id & 15;
+    float x = 0.0f;
+    if (b < n) {
+        const float4 v = in[b];
+        x = j == 0 ? v.x : (j == 1 ? v.y : (j == 2 ? v.z : 2.0f * v.w / lod_max - 1.0f));
+    }
+    const float a = hvx::mlp_freq<float>((int)k) * x;
+    const float sv = sinf(a), cv = cosf(a);
+    const uint32_t c = j * 32 + 2 * k;
+    X[(size_t)b * kIn + c] = (bf16)sv;
+    X[(size_t)b * kIn + c + 1] = (bf16)cv;
+    XT[(size_t)c * B + b] = (bf16)sv;
+    XT[(size_t)(c + 1) * B + b] = (bf16)cv;
+}
+
+// -------------------------------------------------------------------- GEMM
+enum Epi { EPI_FWD = 0, EPI_BWD = 1, EPI_PART = 2 };
+
+struct GemmArgs {
+    const bf16* A;   // [M][Kd]
+    const bf16* Bt;  // [Nc][Kd]
+    uint32_t M, Nc, Kd, splitk;
+    const float* bias;  // FWD: [Nc]
+    const bf16* mask;   // BWD: [M][Nc], zero where <= 0
+    bf16* C;            // FWD/BWD: [M][Nc]
+    bf16* CT;           // FWD/BWD: [Nc][M]
+    float* part;        // PART: [splitk][M][Nc]
+};
+
+// A wave computes a 16 x 64 block of C over one K slice.
+template <int EPI>
+__global__ __launch_bounds__(256) void gemm_abt_kernel(GemmArgs g) {
+    const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const uint32_t rts = g.M / 16, cgs = g.Nc / 64;
+    if (wave >= rts * cgs * g.splitk) return;  // uniform per wave; no barriers below
+    const uint32_t slice = wave % g.splitk, t = wave / g.splitk, cg = t % cgs, rt = t / cgs;
+    const uint32_t ks = g.Kd / 32;
+    const uint32_t kb = slice * ks / g.splitk, ke = (slice + 1) * ks / g.splitk;
+    const uint32_t i = lane & 15, gq = lane >> 4;
+    const bf16* ap = g.A + (size_t)(rt * 16 + i) * g.Kd + 8 * gq;
+    const bf16* bp = g.Bt + (size_t)(cg * 64 + i) * g.Kd + 8 * gq;
+    const size_t bstep = (size_t)16 * g.Kd;
+    f32x4 acc[4];
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) acc[nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t s = kb; s < ke; ++s) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(ap + 32 * s);
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+            const bf16x8 b = *reinterpret_cast<const bf16x8*>(bp + nb * bstep + 32 * s);
+            acc[nb] = hvx::mfma16x16x32(a, b, acc[nb]);
+        }
+    }
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) {
+        const uint32_t col = cg * 64 + nb * 16 + i;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const uint32_t row = rt * 16 + 4 * gq + r;
+            float v = acc[nb][r];
+            if (EPI == EPI_PART) {
+                g.part[((size_t)slice * g.M + row) * g.Nc + col] = v;
+            } else {
+                if (EPI == EPI_FWD) v = fmaxf(v + g.bias[col], 0.0f);
+                else if (!((float)g.mask[(size_t)row * g.Nc + col] > 0.0f)) v = 0.0f;
+                const bf16 h = (bf16)v;
+                g.C[(size_t)row * g.Nc + col] = h;
+                g.CT[(size_t)col * g.M + row] = h;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------- output and loss
+constexpr int kOutPart = kH * 3 + 3 + 1;  // dW4, db4, loss
+
+// Training: one thread per sample (B a multiple of 256). Evaluation (TRAIN
+// false): writes radiance exp(y) - eps for samples < n.
+template <bool TRAIN>
+__global__ __launch_bounds__(256) void mlp_out_kernel(const bf16* __restrict__ H3, const float* __restrict__ P,
+                                                      const float4* __restrict__ tgt, uint32_t n, uint32_t B,
+                                                      float inv_norm, float* __restrict__ Y, bf16* __restrict__ dZ,
+                                                      bf16* __restrict__ dZT, float* __restrict__ part) {
+    __shared__ float sdY[256][3];
+    __shared__ float sl[256];
+    const uint32_t t = threadIdx.x, b = blockIdx.x * 256 + t;
+    const float* W4 = P + hvx::kMlpW4;
+    const float* b4 = P + hvx::kMlpB4;
+    float h[kH];
+#pragma unroll
+    for (int k = 0; k < kH; ++k) h[k] = (float)H3[(size_t)b * kH + k];
+    float y[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float acc = b4[c];
+        for (int k = 0; k < kH; ++k) acc += h[k] * W4[c * kH + k];
+        y[c] = acc;
+    }
+    if (!TRAIN) {
+        if (b < n)
+            for (int c = 0; c < 3; ++c) Y[(size_t)b * 3 + c] = expf(y[c]) - (float)hvx::kMlpEps;
+        return;
+    }
+    const float4 tv = tgt[b];
+    const float e0 = y[0] - tv.x, e1 = y[1] - tv.y, e2 = y[2] - tv.z;
+    sdY[t][0] = e0 * inv_norm;
+    sdY[t][1] = e1 * inv_norm;
+    sdY[t][2] = e2 * inv_norm;
+    sl[t] = (e0 * e0 + e1 * e1) + e2 * e2;
+#pragma unroll
+    for (int k = 0; k < kH; ++k) {
+        const float dh = (sdY[t][0] * W4[k] + sdY[t][1] * W4[kH + k]) + sdY[t][2] * W4[2 * kH + k];
+        const bf16 v = (bf16)(h[k] > 0.0f ? dh : 0.0f);
+        dZ[(size_t)b * kH + k] = v;
+        dZT[(size_t)k * B + b] = v;
+    }
+    __syncthreads();
+    float* pp = part + (size_t)blockIdx.x * kOutPart;
+    if (t < 3 * kH) {
+        const uint32_t c = t / kH, k = t % kH;
+        float acc = 0.0f;
+        for (uint32_t j = 0; j < 256; ++j) acc += sdY[j][c] * (float)H3[((size_t)blockIdx.x * 256 + j) * kH + k];
+        pp[t] = acc;
+    } else if (t < 3 * kH + 3) {
+        const uint32_t c = t - 3 * kH;
+        float acc = 0.0f;
+        for (uint32_t j = 0; j < 256; ++j) acc += sdY[j][c];
+        pp[t] = acc;
+    }
+    for (uint32_t hh = 128; hh > 0; hh >>= 1) {
+        __syncthreads();
+        if (t < hh) sl[t] += sl[t + hh];
+    }
+    if (t == 0) pp[kOutPart - 1] = sl[0];
+}
+
+// out[i] = sum_s part[s * stride + i] * scale, in slice order
+__global__ __launch_bounds__(256) void reduce_kernel(const float* __restrict__ part, uint32_t slices, uint32_t stride,
+                                                     uint32_t len, float scale, float* __restrict__ out) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= len) return;
+    float acc = 0.0f;
+    for (uint32_t s = 0; s < slices; ++s) acc += part[(size_t)s * stride + i];
+    out[i] = acc * scale;
+}
+
+// db[r] = sum over the row r of dZT [rows][B]
+__global__ __launch_bounds__(256) void colsum_kernel(const bf16* __restrict__ dZT, uint32_t B, float* __restrict__ db) {
+    __shared__ float sh[256];
+    const uint32_t r = blockIdx.x;
+    float acc = 0.0f;
+    for (uint32_t b = threadIdx.x; b < B; b += 256) acc += (float)dZT[(size_t)r * B + b];
+    sh[threadIdx.x] = acc;
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        __syncthreads();
+        if (threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+    }
+    if (threadIdx.x == 0) db[r] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ P, const float* __restrict__ G,
+                                                   float* __restrict__ m, float* __restrict__ v, uint32_t n, float lr,
+                                                   float bc1, float bc2) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float g = G[i];
+    const float mi = 0.9f * m[i] + 0.1f * g, vi = 0.999f * v[i] + 0.001f * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    P[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + 1e-8f);
+}
+
+// BF16 working copies: W1, W2, W3 as [out][in]; W2^T, W3^T as [in][out]
+__global__ __launch_bounds__(256) void mlp_cast_kernel(const float* __restrict__ P, bf16* __restrict__ W1,
+                                                       bf16* __restrict__ W2, bf16* __restrict__ W3,
+                                                       bf16* __restrict__ W2T, bf16* __restrict__ W3T) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < (uint32_t)(kH * kIn)) W1[i] = (bf16)P[hvx::kMlpW1 + i];
+    if (i < (uint32_t)(kH * kH)) {
+        const uint32_t o = i / kH, k = i % kH;
+        W2[i] = (bf16)P[hvx::kMlpW2 + i];
+        W3[i] = (bf16)P[hvx::kMlpW3 + i];
+        W2T[k * kH + o] = (bf16)P[hvx::kMlpW2 + i];
+        W3T[k * kH + o] = (bf16)P[hvx::kMlpW3 + i];
+    }
+}
+
+}  // namespace
+
+struct hvx_mlp {
+    hvx_ctx* ctx = nullptr;
+    float lod_max = 1.0f;
+    float *P = nullptr, *G = nullptr, *m = nullptr, *v = nullptr;
+    bf16 *W1 = nullptr, *W2 = nullptr, *W3 = nullptr, *W2T = nullptr, *W3T = nullptr;
+    uint32_t cap = 0;  // batch capacity
+    bf16 *X0 = nullptr, *X0T = nullptr, *H[3] = {}, *HT[3] = {}, *dZa = nullptr, *dZaT = nullptr, *dZb = nullptr,
+         *dZbT = nullptr;
+    float *part = nullptr, *opart = nullptr;
+    uint32_t splitk = 1;
+    uint64_t steps_done = 0;
+    std::vector<void*> batch_allocs;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+int mfail(hvx_ctx* ctx, int st, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
+int mfail(hvx_ctx* ctx, int st, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->last_error = buf;
+    return st;
+}
+
+template <class T> hipError_t dalloc(std::vector<void*>& list, T** p, size_t count) {
+    void* q = nullptr;
+    const hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) list.push_back(q);
+    *p = static_cast<T*>(q);
+    return e;
+}
+
+hipError_t ensure_batch(hvx_mlp* M, uint32_t B) {
+    if (B <= M->cap) return hipSuccess;
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return e;
+    for (void* p : M->batch_allocs) (void)hipFree(p);
+    M->batch_allocs.clear();
+    M->cap = 0;
+    auto& L = M->batch_allocs;
+    M->splitk = std::max(1u, std::min(32u, B / 512));
+    if ((e = dalloc(L, &M->X0, (size_t)B * kIn)) || (e = dalloc(L, &M->X0T, (size_t)B * kIn))) return e;
+    for (int l = 0; l < 3; ++l)
+        if ((e = dalloc(L, &M->H[l], (size_t)B * kH)) || (e = dalloc(L, &M->HT[l], (size_t)B * kH))) return e;
+    if ((e = dalloc(L, &M->dZa, (size_t)B * kH)) || (e = dalloc(L, &M->dZaT, (size_t)B * kH)) ||
+        (e = dalloc(L, &M->dZb, (size_t)B * kH)) || (e = dalloc(L, &M->dZbT, (size_t)B * kH)) ||
+        (e = dalloc(L, &M->part, (size_t)M->splitk * kH * kIn)) ||
+        (e = dalloc(L, &M->opart, (size_t)(B / 256) * kOutPart)))
+        return e;
+    M->cap = B;
+    return hipSuccess;
+}
+
+template <int EPI> void gemm(const GemmArgs& g, hipStream_t s) {
+    const uint32_t waves = (g.M / 16) * (g.Nc / 64) * g.splitk;
+    hipLaunchKernelGGL(gemm_abt_kernel<EPI>, dim3((waves + 3) / 4), dim3(256), 0, s, g);
+}
+
+// forward of B (padded) samples, n of them real
+void forward(hvx_mlp* M, const float4* in, uint32_t n, uint32_t B, hipStream_t s) {
+    hipLaunchKernelGGL(mlp_pe_kernel, dim3(B * 64 / 256), dim3(256), 0, s, in, n, B, M->lod_max, M->X0, M->X0T);
+    const bf16* Ws[3] = {M->W1, M->W2, M->W3};
+    const int bias[3] = {hvx::kMlpB1, hvx::kMlpB2, hvx::kMlpB3};
+    const bf16* prev = M->X0;
+    for (int l = 0; l < 3; ++l) {
+        GemmArgs g{};
+        g.A = prev;
+        g.Bt = Ws[l];
+        g.M = B;
+        g.Nc = kH;
+        g.Kd = l == 0 ? kIn : kH;
+        g.splitk = 1;
+        g.bias = M->P + bias[l];
+        g.C = M->H[l];
+        g.CT = M->HT[l];
+        gemm<EPI_FWD>(g, s);
+        prev = M->H[l];
+    }
+}
+
+// gradient of the mean squared log-radiance error over B samples into G (and
+// the loss into *loss)
+void gradient(hvx_mlp* M, const float4* in, const float4* tgt, uint32_t B, float* G, float* loss, hipStream_t s) {
+    forward(M, in, B, B, s);
+    const uint32_t nblk = B / 256;
+    hipLaunchKernelGGL(mlp_out_kernel<true>, dim3(nblk), dim3(256), 0, s, M->H[2], M->P, tgt, B, B,
+                       2.0f / (3.0f * (float)B), nullptr, M->dZa, M->dZaT, M->opart);
+    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(256), 0, s, M->opart, nblk, (uint32_t)kOutPart,
+                       (uint32_t)(kOutPart - 1), 1.0f, G + hvx::kMlpW4);
+    if (loss)
+        hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(256), 0, s, M->opart + kOutPart - 1, nblk, (uint32_t)kOutPart,
+                           1u, 1.0f / (3.0f * (float)B), loss);
+    const int wofs[3] = {hvx::kMlpW1, hvx::kMlpW2, hvx::kMlpW3}, bofs[3] = {hvx::kMlpB1, hvx::kMlpB2, hvx::kMlpB3};
+    const bf16* WT[3] = {nullptr, M->W2T, M->W3T};
+    bf16 *dz = M->dZa, *dzT = M->dZaT, *nz = M->dZb, *nzT = M->dZbT;
+    for (int l = 2; l >= 0; --l) {
+        const uint32_t K = l == 0 ? kIn : kH;
+        GemmArgs g{};
+        g.A = dzT;
+        g.Bt = l == 0 ? M->X0T : M->HT[l - 1];
+        g.M = kH;
+        g.Nc = K;
+        g.Kd = B;
+        g.splitk = M->splitk;
+        g.part = M->part;
+        gemm<EPI_PART>(g, s);
+        hipLaunchKernelGGL(reduce_kernel, dim3((kH * K + 255) / 256), dim3(256), 0, s, M->part, M->splitk,
+                           (uint32_t)(kH * K), (uint32_t)(kH * K), 1.0f, G + wofs[l]);
+        hipLaunchKernelGGL(colsum_kernel, dim3(kH), dim3(256), 0, s, dzT, B, G + bofs[l]);
+        if (l > 0) {
+            GemmArgs d{};
+            d.A = dz;
+            d.Bt = WT[l];
+            d.M = B;
+            d.Nc = kH;
+            d.Kd = kH;
+            d.splitk = 1;
+            d.mask = M->H[l - 1];
+            d.C = nz;
+            d.CT = nzT;
+            gemm<EPI_BWD>(d, s);
+            std::swap(dz, nz);
+            std::swap(dzT, nzT);
+        }
+    }
+}
+
+void cast(hvx_mlp* M, hipStream_t s) {
+    hipLaunchKernelGGL(mlp_cast_kernel, dim3(kH * kIn / 256), dim3(256), 0, s, M->P, M->W1, M->W2, M->W3, M->W2T,
+                       M->W3T);
+}
+
+int bind(hvx_ctx* ctx) {
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || (cur != ctx->device && hipSetDevice(ctx->device) != hipSuccess))
+        return mfail(ctx, HVX_ERR_HIP, "cannot bind device %d", ctx->device);
+    return HVX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hvx_mlp_create(hvx_ctx* ctx, float lod_max, uint64_t seed, hvx_mlp** out) {
+    if (!ctx || !out) return HVX_ERR_NULL;
+    *out = nullptr;
+    if (!(lod_max > 0.0f) || lod_max > 64.0f) return mfail(ctx, HVX_ERR_INVALID_ARG, "lod_max must be in (0, 64]");
+    int st;
+    if ((st = bind(ctx))) return st;
+    hvx_mlp* M = new (std::nothrow) hvx_mlp();
+    if (!M) return HVX_ERR_HIP;
+    M->ctx = ctx;
+    M->lod_max = lod_max;
+    hipError_t e;
+    auto& L = M->allocs;
+    const size_t np = hvx::kMlpParams;
+    if ((e = dalloc(L, &M->P, np)) || (e = dalloc(L, &M->G, np)) || (e = dalloc(L, &M->m, np)) ||
+        (e = dalloc(L, &M->v, np)) || (e = dalloc(L, &M->W1, (size_t)kH * kIn)) ||
+        (e = dalloc(L, &M->W2, (size_t)kH * kH)) || (e = dalloc(L, &M->W3, (size_t)kH * kH)) ||
+        (e = dalloc(L, &M->W2T, (size_t)kH * kH)) || (e = dalloc(L, &M->W3T, (size_t)kH * kH))) {
+        hvx_mlp_destroy(M);
+        return mfail(ctx, HVX_ERR_HIP, "hvx_mlp_create: %s", hipGetErrorString(e));
+    }
+    // uniform He initialisation (bound sqrt(6 / fan_in)), zero biases, from a
+    // SplitMix64 stream of the seed
+    std::vector<float> h(np, 0.0f);
+    uint64_t state = seed;
+    auto next = [&]() {
+        uint64_t x = (state += 0x9E3779B97F4A7C15ull);
+        x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+        x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+        x ^= x >> 31;
+        return (double)(x >> 11) * (1.0 / 9007199254740992.0);
+    };
+    auto fill = [&](int ofs, int count, int fan_in) {
+        const double bound = std::sqrt(6.0 / fan_in);
+        for (int i = 0; i < count; ++i) h[ofs + i] = (float)((2.0 * next() - 1.0) * bound);
+    };
+    fill(hvx::kMlpW1, kH * kIn, kIn);
+    fill(hvx::kMlpW2, kH * kH, kH);
+    fill(hvx::kMlpW3, kH * kH, kH);
+    fill(hvx::kMlpW4, 3 * kH, kH);
+    if ((e = hipMemcpy(M->P, h.data(), np * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemset(M->m, 0, np * 4)) != hipSuccess || (e = hipMemset(M->v, 0, np * 4)) != hipSuccess) {
+        hvx_mlp_destroy(M);
+        return mfail(ctx, HVX_ERR_HIP, "hvx_mlp_create: %s", hipGetErrorString(e));
+    }
+    cast(M, nullptr);
+    if ((e = hipDeviceSynchronize()) != hipSuccess) {
+        hvx_mlp_destroy(M);
+        return mfail(ctx, HVX_ERR_HIP, "hvx_mlp_create: %s", hipGetErrorString(e));
+    }
+    *out = M;
+    return HVX_OK;
+}
+
+int hvx_mlp_destroy(hvx_mlp* M) {
+    if (!M) return HVX_ERR_NULL;
+    (void)hipDeviceSynchronize();
+    for (void* p : M->batch_allocs) (void)hipFree(p);
+    for (void* p : M->allocs) (void)hipFree(p);
+    delete M;
+    return HVX_OK;
+}
+
+int hvx_mlp_get_params(hvx_mlp* M, float* host_params) {
+    if (!M || !host_params) return HVX_ERR_NULL;
+    const hipError_t e = hipMemcpy(host_params, M->P, hvx::kMlpParams * 4, hipMemcpyDeviceToHost);
+    return e == hipSuccess ? HVX_OK : mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_get_params: %s", hipGetErrorString(e));
+}
+
+int hvx_mlp_set_params(hvx_mlp* M, const float* host_params) {
+    if (!M || !host_params) return HVX_ERR_NULL;
+    hipError_t e = hipMemcpy(M->P, host_params, hvx::kMlpParams * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        cast(M, nullptr);
+        e = hipDeviceSynchronize();
+    }
+    return e == hipSuccess ? HVX_OK : mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_set_params: %s", hipGetErrorString(e));
+}
+
+int hvx_mlp_sample(hvx_mlp* M, const float* d_chain, uint32_t face, uint32_t mips, uint64_t seed, uint32_t n,
+                   float* d_inputs, float* d_targets, void* stream) {
+    if (!M || !d_chain || !d_inputs || !d_targets) return HVX_ERR_NULL;
+    if (face == 0 || face > 16384 || mips == 0 || mips > hvx_mip_count(face) || n == 0)
+        return mfail(M->ctx, HVX_ERR_INVALID_ARG, "hvx_mlp_sample: bad chain sizes or n");
+    int st;
+    if ((st = bind(M->ctx))) return st;
+    hipLaunchKernelGGL(mlp_data_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       reinterpret_cast<const float4*>(d_chain), face, mips, M->lod_max, seed, n,
+                       reinterpret_cast<float4*>(d_inputs), reinterpret_cast<float4*>(d_targets));
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? HVX_OK : mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_sample: %s", hipGetErrorString(e));
+}
+
+int hvx_mlp_gradient(hvx_mlp* M, const float* d_inputs, const float* d_targets, uint32_t batch, float* d_grads,
+                     float* d_loss, void* stream) {
+    if (!M || !d_inputs || !d_targets || !d_grads) return HVX_ERR_NULL;
+    if (batch == 0 || batch % 256 || batch > kMaxBatch)
+        return mfail(M->ctx, HVX_ERR_INVALID_ARG, "batch must be a positive multiple of 256 (<= 2^20)");
+    int st;
+    if ((st = bind(M->ctx))) return st;
+    hipError_t e = ensure_batch(M, batch);
+    if (e != hipSuccess) return mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_gradient: %s", hipGetErrorString(e));
+    gradient(M, reinterpret_cast<const float4*>(d_inputs), reinterpret_cast<const float4*>(d_targets), batch, d_grads,
+             d_loss, (hipStream_t)stream);
+    e = hipGetLastError();
+    return e == hipSuccess ? HVX_OK : mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_gradient: %s", hipGetErrorString(e));
+}
+
+int hvx_mlp_train(hvx_mlp* M, const float* d_inputs, const float* d_targets, uint32_t n, uint32_t batch,
+                  uint32_t steps, float lr, float* d_loss_curve, void* stream) {
+    if (!M || !d_inputs || !d_targets) return HVX_ERR_NULL;
+    if (batch == 0 || batch % 256 || batch > kMaxBatch || n < batch || !(lr > 0.0f))
+        return mfail(M->ctx, HVX_ERR_INVALID_ARG, "hvx_mlp_train: batch a multiple of 256 <= n, lr > 0");
+    int st;
+    if ((st = bind(M->ctx))) return st;
+    hipError_t e = ensure_batch(M, batch);
+    if (e != hipSuccess) return mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_train: %s", hipGetErrorString(e));
+    hipStream_t s = (hipStream_t)stream;
+    const uint32_t per_epoch = n / batch;
+    const float4* in = reinterpret_cast<const float4*>(d_inputs);
+    const float4* tg = reinterpret_cast<const float4*>(d_targets);
+    for (uint32_t k = 0; k < steps; ++k) {
+        const size_t off = (size_t)(k % per_epoch) * batch;
+        gradient(M, in + off, tg + off, batch, M->G, d_loss_curve ? d_loss_curve + k : nullptr, s);
+        // cosine decay from lr to lr / 100 over the call
+        const double ph = steps > 1 ? (double)k / (double)(steps - 1) : 0.0;
+        const float lr_k = (float)(lr * (0.01 + 0.99 * 0.5 * (1.0 + std::cos(hvx::kMlpPi * ph))));
+        ++M->steps_done;
+        const float bc1 = (float)(1.0 - std::pow(0.9, (double)M->steps_done));
+        const float bc2 = (float)(1.0 - std::pow(0.999, (double)M->steps_done));
+        hipLaunchKernelGGL(adam_kernel, dim3((hvx::kMlpParams + 255) / 256), dim3(256), 0, s, M->P, M->G, M->m, M->v,
+                           (uint32_t)hvx::kMlpParams, lr_k, bc1, bc2);
+        cast(M, s);
+    }
+    e = hipGetLastError();
+    return e == hipSuccess ? HVX_OK : mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_train: %s", hipGetErrorString(e));
+}
+
+int hvx_mlp_eval(hvx_mlp* M, const float* d_inputs, uint32_t n, float* d_rgb, void* stream) {
+    if (!M || !d_inputs || !d_rgb) return HVX_ERR_NULL;
+    if (n == 0) return HVX_OK;
+    int st;
+    if ((st = bind(M->ctx))) return st;
+    const uint32_t chunk = 1u << 16;
+    hipError_t e = ensure_batch(M, std::min(chunk, (n + 255) / 256 * 256));
+    if (e != hipSuccess) return mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_eval: %s", hipGetErrorString(e));
+    hipStream_t s = (hipStream_t)stream;
+    const float4* in = reinterpret_cast<const float4*>(d_inputs);
+    for (uint32_t b0 = 0; b0 < n; b0 += chunk) {
+        const uint32_t cnt = std::min(chunk, n - b0), B = (cnt + 255) / 256 * 256;
+        forward(M, in + b0, cnt, B, s);
+        hipLaunchKernelGGL(mlp_out_kernel<false>, dim3(B / 256), dim3(256), 0, s, M->H[2], M->P, nullptr, cnt, B, 0.0f,
+                           d_rgb + (size_t)b0 * 3, nullptr, nullptr, nullptr);
+    }
+    e = hipGetLastError();
+    return e == hipSuccess ? HVX_OK : mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_eval: %s", hipGetErrorString(e));
+}
+
+}  // extern "C"

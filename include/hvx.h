@@ -294,6 +294,38 @@ int hvx_ibl_contract_plan_query(const hvx_contract_plan* plan, uint32_t mip, hvx
 int hvx_ibl_filter_contract(hvx_contract_plan* plan, const float* d_chain, uint32_t chain_levels, float* d_out,
                             void* stream);
 
+/* --------------------------------------------- learned radiance (MLP) */
+
+/* A per-environment network standing in for the prefiltered specular chain:
+ * positional encoding (16 frequencies) of (direction, lod) -> 3 x 64 ReLU ->
+ * RGB log-radiance (the network is defined in hellovulkan_amd/csrc/hvx_mlp.h;
+ * 16771 FP32 parameters). It has no counterpart in the reference: it learns
+ * the value the shading reads with textureLod(specularMap, R, lod)
+ * (Ambient.glsl:1-29) for lod in [0, lod_max]. Trained with BF16 MFMA
+ * forward/backward kernels and FP32 master weights (Adam). */
+typedef struct hvx_mlp hvx_mlp;
+
+int hvx_mlp_create(hvx_ctx* ctx, float lod_max, uint64_t seed, hvx_mlp** out);
+int hvx_mlp_destroy(hvx_mlp* mlp);
+int hvx_mlp_get_params(hvx_mlp* mlp, float* host_params /* 16771 */);
+int hvx_mlp_set_params(hvx_mlp* mlp, const float* host_params);
+/* n training samples from a prefiltered chain (hvx_ibl_filter type 1):
+ * inputs float4 (dir.xyz, lod), targets float4 (log(rgb + 1e-3), 0). */
+int hvx_mlp_sample(hvx_mlp* mlp, const float* d_chain, uint32_t face, uint32_t mips, uint64_t seed, uint32_t n,
+                   float* d_inputs, float* d_targets, void* stream);
+/* Gradient of the mean squared log-radiance error of one batch (a multiple
+ * of 256 samples) with respect to the parameters, in their layout; *d_loss
+ * (nullable) receives the loss. */
+int hvx_mlp_gradient(hvx_mlp* mlp, const float* d_inputs, const float* d_targets, uint32_t batch, float* d_grads,
+                     float* d_loss, void* stream);
+/* `steps` Adam steps over consecutive batches of the n samples (cosine
+ * learning-rate decay from lr to lr/100); d_loss_curve (nullable) receives
+ * each step's loss. */
+int hvx_mlp_train(hvx_mlp* mlp, const float* d_inputs, const float* d_targets, uint32_t n, uint32_t batch,
+                  uint32_t steps, float lr, float* d_loss_curve, void* stream);
+/* Radiance (RGB, n x 3 floats) predicted at n inputs (dir.xyz, lod). */
+int hvx_mlp_eval(hvx_mlp* mlp, const float* d_inputs, uint32_t n, float* d_rgb, void* stream);
+
 /* ---------------------------------------------------------------- batches */
 
 /* One light probe of a prefilter batch: its level-0 cube and where its GGX

@@ -27,6 +27,7 @@
 #include "../include/hvx.h"
 #include "../hellovulkan_amd/csrc/hvx_contract.h"
 #include "../hellovulkan_amd/csrc/hvx_math.h"
+#include "../hellovulkan_amd/csrc/hvx_mlp.h"
 
 namespace {
 
@@ -811,6 +812,41 @@ void ideal_f64(int type, const float* level0, uint32_t S, double alpha2, uint32_
     });
 }
 
+// ================================================== radiance network (FP64)
+// Forward and backward of the network of hvx_mlp.h in double precision from
+// the FP32 parameters, for one sample at a time; the loss is the mean over
+// samples and channels of the squared log-radiance error.
+struct MlpAct {
+    double x[hvx::kMlpIn], h[3][hvx::kMlpHidden], y[hvx::kMlpOut];
+};
+
+void mlp_forward_one(const float* P, const float in[4], double lod_max, MlpAct& a) {
+    const double xin[4] = {in[0], in[1], in[2], 2.0 * in[3] / lod_max - 1.0};
+    for (int j = 0; j < 4; ++j)
+        for (int k = 0; k < hvx::kMlpFreqs; ++k) {
+            const double t = hvx::mlp_freq<double>(k) * xin[j];
+            a.x[j * 32 + 2 * k] = std::sin(t);
+            a.x[j * 32 + 2 * k + 1] = std::cos(t);
+        }
+    const int wo[3] = {hvx::kMlpW1, hvx::kMlpW2, hvx::kMlpW3}, bo[3] = {hvx::kMlpB1, hvx::kMlpB2, hvx::kMlpB3};
+    const double* prev = a.x;
+    int nin = hvx::kMlpIn;
+    for (int l = 0; l < 3; ++l) {
+        for (int o = 0; o < hvx::kMlpHidden; ++o) {
+            double z = P[bo[l] + o];
+            for (int i = 0; i < nin; ++i) z += (double)P[wo[l] + o * nin + i] * prev[i];
+            a.h[l][o] = z > 0.0 ? z : 0.0;
+        }
+        prev = a.h[l];
+        nin = hvx::kMlpHidden;
+    }
+    for (int c = 0; c < 3; ++c) {
+        double z = P[hvx::kMlpB4 + c];
+        for (int i = 0; i < hvx::kMlpHidden; ++i) z += (double)P[hvx::kMlpW4 + c * hvx::kMlpHidden + i] * a.h[2][i];
+        a.y[c] = z;
+    }
+}
+
 }  // namespace
 
 // ======================================================================= C API
@@ -1008,6 +1044,60 @@ void hvxo_ibl_specular_f64(const float* cube, uint32_t S, uint32_t So, uint32_t 
     const double rough = specular_roughness<double>(mip, mips);
     parallel_for(n_ids, threads,
                  [&](uint32_t k) { specular_texel<double>(cube, S, Sm, D, rough, n, ids[k], out + 4 * k); });
+}
+
+// ---- radiance network: FP64 forward (log-radiance y) and gradient
+void hvxo_mlp_forward_f64(const float* P, const float* in /* n x 4 */, uint32_t n, double lod_max, double* y) {
+    for (uint32_t s = 0; s < n; ++s) {
+        MlpAct a;
+        mlp_forward_one(P, in + 4 * (size_t)s, lod_max, a);
+        for (int c = 0; c < 3; ++c) y[3 * (size_t)s + c] = a.y[c];
+    }
+}
+void hvxo_mlp_gradient_f64(const float* P, const float* in, const float* tgt /* n x 4 */, uint32_t n, double lod_max,
+                           double* G /* kMlpParams */, double* loss) {
+    constexpr int H = hvx::kMlpHidden, I = hvx::kMlpIn;
+    std::fill(G, G + hvx::kMlpParams, 0.0);
+    double L = 0.0;
+    const double norm = 1.0 / (3.0 * n);
+    for (uint32_t s = 0; s < n; ++s) {
+        MlpAct a;
+        mlp_forward_one(P, in + 4 * (size_t)s, lod_max, a);
+        double dy[3];
+        for (int c = 0; c < 3; ++c) {
+            const double e = a.y[c] - tgt[4 * (size_t)s + c];
+            L += e * e * norm;
+            dy[c] = 2.0 * e * norm;
+        }
+        double dz[H], dprev[H];
+        for (int c = 0; c < 3; ++c) {
+            G[hvx::kMlpB4 + c] += dy[c];
+            for (int i = 0; i < H; ++i) G[hvx::kMlpW4 + c * H + i] += dy[c] * a.h[2][i];
+        }
+        for (int i = 0; i < H; ++i) {
+            double d = 0.0;
+            for (int c = 0; c < 3; ++c) d += dy[c] * (double)P[hvx::kMlpW4 + c * H + i];
+            dz[i] = a.h[2][i] > 0.0 ? d : 0.0;
+        }
+        const int wo[3] = {hvx::kMlpW1, hvx::kMlpW2, hvx::kMlpW3}, bo[3] = {hvx::kMlpB1, hvx::kMlpB2, hvx::kMlpB3};
+        for (int l = 2; l >= 0; --l) {
+            const double* prev = l == 0 ? a.x : a.h[l - 1];
+            const int nin = l == 0 ? I : H;
+            for (int o = 0; o < H; ++o) {
+                G[bo[l] + o] += dz[o];
+                for (int i = 0; i < nin; ++i) G[wo[l] + o * nin + i] += dz[o] * prev[i];
+            }
+            if (l > 0) {
+                for (int i = 0; i < H; ++i) {
+                    double d = 0.0;
+                    for (int o = 0; o < H; ++o) d += dz[o] * (double)P[wo[l] + o * H + i];
+                    dprev[i] = a.h[l - 1][i] > 0.0 ? d : 0.0;
+                }
+                std::copy(dprev, dprev + H, dz);
+            }
+        }
+    }
+    *loss = L;
 }
 
 // ---- contraction filter: FP64 twin and the level-0 accuracy reference

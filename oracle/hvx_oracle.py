@@ -48,6 +48,8 @@ _SIGS = [
     ("hvxo_contract_f64", None, [C.c_int, vp, u32, u32, u32, u32, vp, u32, vp, C.c_int]),
     ("hvxo_ideal_f64", None, [C.c_int, vp, u32, C.c_double, u32, vp, u32, vp, C.c_int]),
     ("hvxo_contract_levels", C.c_int, [C.c_int, u32, u32, u32, vp, vp]),
+    ("hvxo_mlp_forward_f64", None, [vp, vp, u32, C.c_double, vp]),
+    ("hvxo_mlp_gradient_f64", None, [vp, vp, vp, u32, C.c_double, vp, vp]),
 ]
 
 _lib = None
@@ -259,6 +261,30 @@ def ideal(ftype: int, level0: np.ndarray, S: int, alpha2: float, So_mip: int, id
     lib().hvxo_ideal_f64(ftype, level0.ctypes.data, S, alpha2, So_mip, ids.ctypes.data, ids.size, out.ctypes.data,
                          _threads(threads))
     return out
+
+
+MLP_PARAMS = 16771
+
+
+def mlp_forward(params: np.ndarray, inputs: np.ndarray, lod_max: float) -> np.ndarray:
+    """FP64 log-radiance (n x 3) of the radiance network at inputs (n x 4: dir, lod)."""
+    params = _c(params, np.float32)
+    inputs = _c(inputs, np.float32).reshape(-1, 4)
+    out = np.empty((inputs.shape[0], 3), np.float64)
+    lib().hvxo_mlp_forward_f64(params.ctypes.data, inputs.ctypes.data, inputs.shape[0], lod_max, out.ctypes.data)
+    return out
+
+
+def mlp_gradient(params, inputs, targets, lod_max: float):
+    """FP64 gradient (parameter layout) and loss of the mean squared log-radiance error."""
+    params = _c(params, np.float32)
+    inputs = _c(inputs, np.float32).reshape(-1, 4)
+    targets = _c(targets, np.float32).reshape(-1, 4)
+    g = np.empty(MLP_PARAMS, np.float64)
+    loss = np.zeros(1, np.float64)
+    lib().hvxo_mlp_gradient_f64(params.ctypes.data, inputs.ctypes.data, targets.ctypes.data, inputs.shape[0], lod_max,
+                                g.ctypes.data, loss.ctypes.data)
+    return g, float(loss[0])
 
 
 def rel_err(got, ref, floor_frac=1e-6) -> float:
