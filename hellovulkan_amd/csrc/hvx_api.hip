@@ -215,9 +215,13 @@ int hvx_cf_cull_lights(hvx_ctx* ctx, const hvx_cf_ubo* ubo, const hvx_aabb* d_cl
     return e == hipSuccess ? HVX_OK : hip_fail(ctx, e, "hvx_cf_cull_lights launch");
 }
 
-int hvx_cf_shade(hvx_ctx* ctx, const hvx_frag_soa* fr, const hvx_camera_ubo* cam, const hvx_cf_ubo* ubo,
-                 const hvx_pbr_pc* pc, const hvx_light* d_lights, uint32_t n_lights, const hvx_light_cell* d_cells,
-                 const uint32_t* d_indices, const hvx_ibl_maps* ibl, float* d_rgb, void* stream) {
+}  // extern "C"
+
+namespace {
+int shade_impl(hvx_ctx* ctx, const hvx_frag_soa* fr, const hvx_camera_ubo* cam, const hvx_cf_ubo* ubo,
+               const hvx_pbr_pc* pc, const hvx_light* d_lights, uint32_t n_lights, const hvx_light_cell* d_cells,
+               const uint32_t* d_indices, const hvx_ibl_maps* ibl, float* d_rgb, const hvx::LearnedShade* learned,
+               void* stream) {
     if (!ctx) return HVX_ERR_NULL;
     int st = check_ubo(ctx, ubo);
     if (st) return st;
@@ -246,8 +250,38 @@ int hvx_cf_shade(hvx_ctx* ctx, const hvx_frag_soa* fr, const hvx_camera_ubo* cam
             hipSuccess)
         return hip_fail(ctx, e, "shading light table allocation");
     e = hvx::launch_shade(*fr, *cam, *ubo, *pc, d_lights, n_lights, d_cells, d_indices, *ibl, d_rgb,
-                          ctx->shade_exact != 0, static_cast<float4*>(ctx->shade_lights), (hipStream_t)stream);
-    return e == hipSuccess ? HVX_OK : hip_fail(ctx, e, "hvx_cf_shade launch");
+                          ctx->shade_exact != 0, static_cast<float4*>(ctx->shade_lights), learned, (hipStream_t)stream);
+    return e == hipSuccess ? HVX_OK : hip_fail(ctx, e, "shading launch");
+}
+}  // namespace
+
+extern "C" {
+
+int hvx_cf_shade(hvx_ctx* ctx, const hvx_frag_soa* fr, const hvx_camera_ubo* cam, const hvx_cf_ubo* ubo,
+                 const hvx_pbr_pc* pc, const hvx_light* d_lights, uint32_t n_lights, const hvx_light_cell* d_cells,
+                 const uint32_t* d_indices, const hvx_ibl_maps* ibl, float* d_rgb, void* stream) {
+    return shade_impl(ctx, fr, cam, ubo, pc, d_lights, n_lights, d_cells, d_indices, ibl, d_rgb, nullptr, stream);
+}
+
+int hvx_cf_shade_learned(hvx_ctx* ctx, const hvx_frag_soa* fr, const hvx_camera_ubo* cam, const hvx_cf_ubo* ubo,
+                         const hvx_pbr_pc* pc, const hvx_light* d_lights, uint32_t n_lights,
+                         const hvx_light_cell* d_cells, const uint32_t* d_indices, const hvx_ibl_maps* ibl,
+                         const hvx_mlp* mlp, const uint8_t* d_mask, uint32_t mask_face, uint32_t lod_bins,
+                         float* d_rgb, uint32_t* d_served, void* stream) {
+    if (!ctx) return HVX_ERR_NULL;
+    if (!mlp || !d_mask || !pc) return fail(ctx, HVX_ERR_NULL, "mlp, mask and push constants are required");
+    if (mask_face == 0 || mask_face > 4096 || lod_bins == 0 || lod_bins > 256)
+        return fail(ctx, HVX_ERR_INVALID_ARG, "mask_face in [1, 4096], lod_bins in [1, 256]");
+    hvx::LearnedShade L;
+    L.mlp = hvx::mlp_view(mlp);
+    if (!(pc->max_reflection_lod >= 0.0f) || pc->max_reflection_lod > L.mlp.lod_max)
+        return fail(ctx, HVX_ERR_INVALID_ARG, "maxReflectionLod %g exceeds the network's lod range %g",
+                    pc->max_reflection_lod, L.mlp.lod_max);
+    L.mask = d_mask;
+    L.mask_face = mask_face;
+    L.lod_bins = lod_bins;
+    L.served = d_served;
+    return shade_impl(ctx, fr, cam, ubo, pc, d_lights, n_lights, d_cells, d_indices, ibl, d_rgb, &L, stream);
 }
 
 // ------------------------------------------------------------------- IBL

@@ -9,6 +9,7 @@
 #include <string>
 
 #include "../../include/hvx.h"
+#include "mlp_device.h"
 
 struct hvx_ctx {
     int device = 0;
@@ -91,10 +92,20 @@ hipError_t launch_cull_batch(const CullBatchView* d_views, uint32_t nv, const Cu
 hipError_t launch_queue_reset(uint32_t* d_queues, uint32_t n, hipStream_t s);
 
 // --- shading (shade_kernels.hip); light_scratch holds 2 * n_lights float4
+// Learned prefiltered radiance for shading: the network and its fallback mask.
+struct LearnedShade {
+    MlpView mlp;
+    const uint8_t* mask;
+    uint32_t mask_face, lod_bins;
+    uint32_t* served;
+};
+MlpView mlp_view(const hvx_mlp* mlp);
+
 hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const hvx_cf_ubo& ubo,
                         const hvx_pbr_pc& pc, const hvx_light* d_lights, uint32_t n_lights,
                         const hvx_light_cell* d_cells, const uint32_t* d_indices, const hvx_ibl_maps& ibl,
-                        float* d_rgb, bool exact, float4* light_scratch, hipStream_t s);
+                        float* d_rgb, bool exact, float4* light_scratch, const LearnedShade* learned,
+                        hipStream_t s);
 
 // --- IBL (ibl_kernels.hip)
 hipError_t launch_brdf_lut(uint32_t w, uint32_t h, uint32_t n, float* d_lut, hipStream_t s);

@@ -27,8 +27,10 @@
 
 #include "hvx_device.h"
 #include "hvx_internal.h"
+#include "hvx_contract.h"
 #include "hvx_mfma.h"
 #include "hvx_mlp.h"
+#include "mlp_device.h"
 
 namespace {
 
@@ -272,6 +274,79 @@ __global__ __launch_bounds__(256) void mlp_cast_kernel(const float* __restrict__
     }
 }
 
+// Inference through the in-shading path (mlp_device.h): each workgroup
+// stages the weights once, its four waves evaluate 16 samples per call.
+__global__ __launch_bounds__(256) void mlp_infer_kernel(MlpView v, const float4* __restrict__ in, uint32_t n,
+                                                        float* __restrict__ rgb) {
+    __shared__ MlpLds L;
+    __shared__ __bf16 tiles[4][16 * kH];
+    mlp_stage(L, v);
+    __syncthreads();
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63, i = lane & 15, g = lane >> 4;
+    for (uint32_t base = blockIdx.x * 64; base < n; base += gridDim.x * 64) {  // uniform per workgroup
+        const uint32_t sidx = base + wave * 16 + i;
+        float x[4] = {0.0f, 0.0f, 1.0f, 0.0f};
+        if (sidx < n) {
+            const float4 q = in[sidx];
+            x[0] = q.x;
+            x[1] = q.y;
+            x[2] = q.z;
+            x[3] = q.w;
+        }
+        const float y = mlp_forward_tile(L, tiles[wave], x, v.lod_max);
+        if (g < 3 && sidx < n) rgb[(size_t)sidx * 3 + g] = y;
+    }
+}
+
+// Fallback mask cells: (lod bin b, face f, texel y, x) of a mask_face cube.
+// Points of a cell: the texel's centre and four corners at the bin's two
+// ends and middle (15 points).
+constexpr uint32_t kMaskPoints = 15;
+
+__device__ __forceinline__ f3 plane_dir_f(int f, double u, double v) {
+    double p[3];
+    hvx::contract_face_point(f, u, v, p);
+    return normalize3(f3{(float)p[0], (float)p[1], (float)p[2]});
+}
+
+__global__ __launch_bounds__(256) void mask_points_kernel(uint32_t mf, uint32_t bins, float lod_max,
+                                                          float4* __restrict__ pts) {
+    const uint32_t cells = bins * 6 * mf * mf;
+    const uint32_t id = blockIdx.x * 256 + threadIdx.x;
+    if (id >= cells * kMaskPoints) return;
+    const uint32_t cell = id / kMaskPoints, p = id % kMaskPoints;
+    const uint32_t x = cell % mf, y = (cell / mf) % mf, f = (cell / (mf * mf)) % 6, b = cell / (6 * mf * mf);
+    const uint32_t corner = p % 5, li = p / 5;
+    double cx = x + 0.5, cy = y + 0.5;
+    if (corner) {
+        cx = x + ((corner & 1) ? 1.0 : 0.0);
+        cy = y + ((corner & 2) ? 1.0 : 0.0);
+    }
+    const f3 d = plane_dir_f((int)f, 2.0 * cx / mf - 1.0, 2.0 * cy / mf - 1.0);
+    const float lod = lod_max * ((float)b + 0.5f * (float)li) / (float)bins;
+    pts[id] = make_float4(d.x, d.y, d.z, lod);
+}
+
+// mask[cell] = 1 if the network is within tol (relative, every channel) of
+// the chain's textureLod value at every point of the cell
+__global__ __launch_bounds__(256) void mask_cells_kernel(const float4* __restrict__ pts, const float* __restrict__ rgb,
+                                                         const float4* __restrict__ chain, uint32_t S, uint32_t mips,
+                                                         float tol, uint32_t cells, uint8_t* __restrict__ mask,
+                                                         uint32_t* __restrict__ count) {
+    const uint32_t cell = blockIdx.x * 256 + threadIdx.x;
+    if (cell >= cells) return;
+    bool ok = true;
+    for (uint32_t p = 0; p < kMaskPoints; ++p) {
+        const float4 q = pts[(size_t)cell * kMaskPoints + p];
+        const f3 ref = chain_lod(chain, S, mips, f3{q.x, q.y, q.z}, q.w);
+        const float* m = rgb + ((size_t)cell * kMaskPoints + p) * 3;
+        const float r3[3] = {ref.x, ref.y, ref.z};
+        for (int c = 0; c < 3; ++c) ok = ok && fabsf(m[c] - r3[c]) <= tol * fabsf(r3[c]);
+    }
+    mask[cell] = ok ? 1 : 0;
+    if (ok && count) atomicAdd(count, 1u);
+}
+
 }  // namespace
 
 struct hvx_mlp {
@@ -288,6 +363,10 @@ struct hvx_mlp {
     std::vector<void*> batch_allocs;
     std::vector<void*> allocs;
 };
+
+namespace hvx {
+MlpView mlp_view(const hvx_mlp* M) { return MlpView{M->W1, M->W2, M->W3, M->P, M->lod_max}; }
+}  // namespace hvx
 
 namespace {
 
@@ -563,19 +642,44 @@ int hvx_mlp_eval(hvx_mlp* M, const float* d_inputs, uint32_t n, float* d_rgb, vo
     if (n == 0) return HVX_OK;
     int st;
     if ((st = bind(M->ctx))) return st;
-    const uint32_t chunk = 1u << 16;
-    hipError_t e = ensure_batch(M, std::min(chunk, (n + 255) / 256 * 256));
-    if (e != hipSuccess) return mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_eval: %s", hipGetErrorString(e));
-    hipStream_t s = (hipStream_t)stream;
-    const float4* in = reinterpret_cast<const float4*>(d_inputs);
-    for (uint32_t b0 = 0; b0 < n; b0 += chunk) {
-        const uint32_t cnt = std::min(chunk, n - b0), B = (cnt + 255) / 256 * 256;
-        forward(M, in + b0, cnt, B, s);
-        hipLaunchKernelGGL(mlp_out_kernel<false>, dim3(B / 256), dim3(256), 0, s, M->H[2], M->P, nullptr, cnt, B, 0.0f,
-                           d_rgb + (size_t)b0 * 3, nullptr, nullptr, nullptr);
-    }
-    e = hipGetLastError();
+    const uint32_t blocks = std::min(8192u, (n + 63) / 64);
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, hvx::mlp_view(M),
+                       reinterpret_cast<const float4*>(d_inputs), n, d_rgb);
+    const hipError_t e = hipGetLastError();
     return e == hipSuccess ? HVX_OK : mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_eval: %s", hipGetErrorString(e));
+}
+
+int hvx_mlp_build_mask(hvx_mlp* M, const float* d_chain, uint32_t face, uint32_t mips, float tol, uint32_t mask_face,
+                       uint32_t lod_bins, uint8_t* d_mask, uint32_t* d_count, void* stream) {
+    if (!M || !d_chain || !d_mask) return HVX_ERR_NULL;
+    if (face == 0 || face > 16384 || mips == 0 || mips > hvx_mip_count(face) || !(tol > 0.0f) || mask_face == 0 ||
+        mask_face > 1024 || lod_bins == 0 || lod_bins > 256)
+        return mfail(M->ctx, HVX_ERR_INVALID_ARG, "hvx_mlp_build_mask: sizes or tolerance out of range");
+    int st;
+    if ((st = bind(M->ctx))) return st;
+    const uint32_t cells = lod_bins * 6 * mask_face * mask_face;
+    const size_t npts = (size_t)cells * kMaskPoints;
+    if (npts > (1u << 28)) return mfail(M->ctx, HVX_ERR_INVALID_ARG, "hvx_mlp_build_mask: too many cells");
+    // set-up call: the point and prediction buffers are allocated here
+    float4* pts = nullptr;
+    float* rgb = nullptr;
+    hipError_t e;
+    if ((e = hipMalloc(&pts, npts * sizeof(float4))) != hipSuccess ||
+        (e = hipMalloc(&rgb, npts * 3 * sizeof(float))) != hipSuccess) {
+        if (pts) (void)hipFree(pts);
+        return mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_build_mask: %s", hipGetErrorString(e));
+    }
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(mask_points_kernel, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, mask_face, lod_bins,
+                       M->lod_max, pts);
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3(std::min(8192u, (unsigned)((npts + 63) / 64))), dim3(256), 0, s,
+                       hvx::mlp_view(M), pts, (uint32_t)npts, rgb);
+    hipLaunchKernelGGL(mask_cells_kernel, dim3((cells + 255) / 256), dim3(256), 0, s, pts, rgb,
+                       reinterpret_cast<const float4*>(d_chain), face, mips, tol, cells, d_mask, d_count);
+    e = hipStreamSynchronize(s);
+    (void)hipFree(pts);
+    (void)hipFree(rgb);
+    return e == hipSuccess ? HVX_OK : mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_build_mask: %s", hipGetErrorString(e));
 }
 
 }  // extern "C"

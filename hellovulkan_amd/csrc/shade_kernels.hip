@@ -30,6 +30,7 @@
 
 #include "hvx_device.h"
 #include "hvx_internal.h"
+#include "mlp_device.h"
 
 // build-time tuning switches (defaults are the tuned configuration)
 #ifndef HVX_SHADE_ONE_RCP
@@ -69,6 +70,11 @@ struct ShadeArgs {
     const float2* lut;
     uint32_t lut_w, lut_h;
     float* out;
+    // learned prefiltered radiance (shade_kernel<..., true>)
+    MlpView mlp;
+    const uint8_t* mask;  // [lod_bins][6][mask_face][mask_face], 1 = use the network
+    uint32_t mask_face, lod_bins;
+    uint32_t* served;     // fragments that used the network (nullable)
 };
 
 __global__ __launch_bounds__(256) void shade_light_prep_kernel(const hvx_light* __restrict__ lights, uint32_t n,
@@ -110,8 +116,37 @@ __device__ __forceinline__ uint32_t cluster_of(const ShadeArgs& a, uint32_t idx)
 
 __device__ __forceinline__ float rcp_fast(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// Reflection vector and specular level of detail of a fragment, evaluated
+// exactly as shade_one does (Ambient.glsl:17-22).
+__device__ __forceinline__ f3 frag_reflect(const ShadeArgs& a, uint32_t idx, float* lod) {
+    const uint32_t px = idx % a.W, py = idx / a.W;
+    const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
+    const float z = a.depth[idx];
+    float wh[4];
+    matvec4(a.ivp, (fx / (float)a.W) * 2.0f - 1.0f, (fy / (float)a.H) * 2.0f - 1.0f, z, 1.0f, wh);
+    const f3 wp = {wh[0] / wh[3], wh[1] / wh[3], wh[2] / wh[3]};
+    const f3 N = {a.nrm[0][idx], a.nrm[1][idx], a.nrm[2][idx]};
+    const f3 V = normalize3(sub3(make3(a.cam[0], a.cam[1], a.cam[2]), wp));
+    const f3 I = {-V.x, -V.y, -V.z};
+    const float dNI = dot3(N, I);
+    *lod = a.rough[idx] * a.pc.max_reflection_lod;
+    return sub3(I, mul3(N, 2.0f * dNI));
+}
+
+// Whether the network serves (R, lod): the fallback mask cell of the cube
+// texel of R at the mask resolution (face selection as for sampling) and of
+// lod's bin.
+__device__ __forceinline__ bool mlp_cell_ok(const ShadeArgs& a, f3 R, float lod) {
+    const CubeCoord cc = cube_select(R);
+    const uint32_t mf = a.mask_face;
+    const uint32_t x = min((uint32_t)(cc.s * (float)mf), mf - 1), y = min((uint32_t)(cc.t * (float)mf), mf - 1);
+    const uint32_t b = min((uint32_t)(lod / a.mlp.lod_max * (float)a.lod_bins), a.lod_bins - 1);
+    return a.mask[((size_t)(b * 6 + (uint32_t)cc.face) * mf + y) * mf + x] != 0;
+}
+
 template <bool EXACT>
-__device__ __forceinline__ void shade_one(const ShadeArgs& a, uint32_t idx, uint32_t off, uint32_t count) {
+__device__ __forceinline__ void shade_one(const ShadeArgs& a, uint32_t idx, uint32_t off, uint32_t count,
+                                          bool use_learned = false, f3 learned = f3{0.0f, 0.0f, 0.0f}) {
     const float PI = kPiF;
     const uint32_t nfrag = a.W * a.H;
     const uint32_t px = idx % a.W, py = idx / a.W;
@@ -248,7 +283,7 @@ __device__ __forceinline__ void shade_one(const ShadeArgs& a, uint32_t idx, uint
     const f3 I = {-V.x, -V.y, -V.z};
     const float dNI = dot3(N, I);
     const f3 R = sub3(I, mul3(N, 2.0f * dNI));
-    const f3 pre = sample_spec_lod(a, R, rough * a.pc.max_reflection_lod);
+    const f3 pre = use_learned ? learned : sample_spec_lod(a, R, rough * a.pc.max_reflection_lod);
     const float2 brdf = sample_2d_rg_repeat(a.lut, (int)a.lut_w, (int)a.lut_h, NoV, rough);
     const float irr3[3] = {irr.x, irr.y, irr.z}, pre3[3] = {pre.x, pre.y, pre.z};
 #pragma unroll
@@ -270,7 +305,7 @@ constexpr int kBuckets = 64;   // light-count classes (count / 2, saturating)
 constexpr uint32_t kXcds = 8;  // MI355X accelerator complex dies
 
 
-template <bool EXACT>
+template <bool EXACT, bool LEARNED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_WAVES_PER_EU, 8))) void shade_kernel(
     ShadeArgs a) {
     __shared__ uint32_t s_off[kChunk];
@@ -328,11 +363,58 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_W
         s_order[slot] = (uint16_t)(r * 256 + t);
     }
     __syncthreads();
+    if constexpr (LEARNED) {
+        // the network's prefiltered radiance for every fragment of the chunk:
+        // each wave evaluates 16 fragments per call, in chunk order
+        __shared__ MlpLds s_mlp;
+        __shared__ __bf16 s_tile[4][16 * kMlpHidden];
+        __shared__ float s_pre[3][kChunk];
+        __shared__ uint8_t s_use[kChunk];
+        __shared__ uint32_t s_served;
+        mlp_stage(s_mlp, a.mlp);
+        if (t == 0) s_served = 0;
+        __syncthreads();
+        const uint32_t wave = t >> 6, lane = t & 63, i = lane & 15, g = lane >> 4;
+        uint32_t mine = 0;
 #pragma unroll 1
-    for (int r = 0; r < kChunk / 256; ++r) {
-        const uint32_t local = s_order[r * 256 + t];
-        const uint32_t idx = base + local;
-        if (idx < nfrag) shade_one<EXACT>(a, idx, s_off[local], s_cnt[local]);
+        for (int it = 0; it < kChunk / 64; ++it) {
+            const uint32_t local = it * 64 + wave * 16 + i, idx = base + local;
+            float x[4] = {0.0f, 0.0f, 1.0f, 0.0f};
+            bool ok = false;
+            if (idx < nfrag) {
+                float lod;
+                const f3 R = frag_reflect(a, idx, &lod);
+                x[0] = R.x;
+                x[1] = R.y;
+                x[2] = R.z;
+                x[3] = lod;
+                ok = mlp_cell_ok(a, R, lod);
+            }
+            const float y = mlp_forward_tile(s_mlp, s_tile[wave], x, a.mlp.lod_max);
+            if (g < 3) s_pre[g][local] = y;
+            if (g == 0) {
+                s_use[local] = ok ? 1 : 0;
+                mine += ok ? 1u : 0u;
+            }
+        }
+        if (a.served && mine) atomicAdd(&s_served, mine);
+        __syncthreads();
+        if (a.served && t == 0 && s_served) atomicAdd(a.served, s_served);
+#pragma unroll 1
+        for (int r = 0; r < kChunk / 256; ++r) {
+            const uint32_t local = s_order[r * 256 + t];
+            const uint32_t idx = base + local;
+            if (idx < nfrag)
+                shade_one<EXACT>(a, idx, s_off[local], s_cnt[local], s_use[local] != 0,
+                                 f3{s_pre[0][local], s_pre[1][local], s_pre[2][local]});
+        }
+    } else {
+#pragma unroll 1
+        for (int r = 0; r < kChunk / 256; ++r) {
+            const uint32_t local = s_order[r * 256 + t];
+            const uint32_t idx = base + local;
+            if (idx < nfrag) shade_one<EXACT>(a, idx, s_off[local], s_cnt[local]);
+        }
     }
 }
 
@@ -341,8 +423,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_W
 hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const hvx_cf_ubo& ubo,
                         const hvx_pbr_pc& pc, const hvx_light* d_lights, uint32_t n_lights,
                         const hvx_light_cell* d_cells, const uint32_t* d_indices, const hvx_ibl_maps& ibl,
-                        float* d_rgb, bool exact, float4* light_scratch, hipStream_t s) {
-    ShadeArgs a;
+                        float* d_rgb, bool exact, float4* light_scratch, const LearnedShade* learned,
+                        hipStream_t s) {
+    ShadeArgs a{};
     a.depth = fr.depth;
     for (int c = 0; c < 3; ++c) {
         a.nrm[c] = fr.normal[c];
@@ -378,16 +461,25 @@ hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const
     a.lut_w = ibl.lut_width;
     a.lut_h = ibl.lut_height;
     a.out = d_rgb;
+    if (learned) {
+        a.mlp = learned->mlp;
+        a.mask = learned->mask;
+        a.mask_face = learned->mask_face;
+        a.lod_bins = learned->lod_bins;
+        a.served = learned->served;
+    }
     const uint32_t F = fr.width * fr.height;
     const dim3 grid((F + kChunk - 1) / kChunk);
     if (exact) {
-        hipLaunchKernelGGL(shade_kernel<true>, grid, dim3(256), 0, s, a);
+        if (learned) hipLaunchKernelGGL((shade_kernel<true, true>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((shade_kernel<true, false>), grid, dim3(256), 0, s, a);
     } else {
         if (n_lights > 0) {
             hipLaunchKernelGGL(shade_light_prep_kernel, dim3((n_lights + 255) / 256), dim3(256), 0, s, d_lights,
                                n_lights, pc.light_intensity, light_scratch, light_scratch + n_lights);
         }
-        hipLaunchKernelGGL(shade_kernel<false>, grid, dim3(256), 0, s, a);
+        if (learned) hipLaunchKernelGGL((shade_kernel<false, true>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((shade_kernel<false, false>), grid, dim3(256), 0, s, a);
     }
     return hipGetLastError();
 }

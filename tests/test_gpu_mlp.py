@@ -96,3 +96,78 @@ def test_training_lowers_loss(ctx):
     print(f"held-out relative error: median {np.median(rel):.3e}, 90% {np.quantile(rel, 0.9):.3e}, "
           f"within 2e-3: {np.mean(rel.max(axis=1) <= 2e-3):.4f}")
     mlp.close()
+
+
+def _frame(ctx, spec_chain=None, spec_size=32, spec_mips=6):
+    """A 320x200 frame (512 lights) culled on the GPU; IBL maps random except
+    an optional given specular chain."""
+    import torch
+    from tests.helpers import camera, random_ibl_maps
+    W, H, grid = 320, 200, (16, 9, 24)
+    desc, cam, ubo, ivp = camera(W, H, grid)
+    lights = abi.scene_lights(61, 512)
+    fr = abi.scene_fragments(62, desc, ivp)
+    Cn = grid[0] * grid[1] * grid[2]
+    aabb = torch.empty(Cn * 8, dtype=torch.float32, device="cuda")
+    ctx.build_aabbs(ubo, aabb)
+    d_lights = to_dev(lights.view(np.uint8))
+    cells = torch.zeros(Cn * 2, dtype=torch.int32, device="cuda")
+    idx = torch.zeros(Cn * 128, dtype=torch.int32, device="cuda")
+    ctx.cull_lights(ubo, aabb, d_lights, 512, 128, cells, idx, Cn * 128)
+    maps = random_ibl_maps(np.random.default_rng(63), spec_size=spec_size, spec_mips=spec_mips)
+    spec = to_dev(maps["spec"]) if spec_chain is None else spec_chain
+    keep = dict(spec=spec, diff=to_dev(maps["diff"]), lut=to_dev(maps["lut"]), cells=cells, idx=idx,
+                lights=d_lights, planes={k: to_dev(v) for k, v in fr.items()})
+    m = abi.ibl_maps(spec, spec_size, spec_mips, keep["diff"], maps["diff_size"], keep["lut"], maps["lut_w"],
+                     maps["lut_h"])
+    frs = abi.frag_soa(keep["planes"], W, H, ivp)
+    return dict(frs=frs, cam=cam, ubo=ubo, maps=m, n=W * H, keep=keep)
+
+
+def _shade(ctx, f, mlp=None, mask=None, mask_face=0, bins=0):
+    import torch
+    rgb = torch.empty(3 * f["n"], dtype=torch.float32, device="cuda")
+    pc = abi.pbr_defaults()
+    k = f["keep"]
+    served = torch.zeros(1, dtype=torch.int32, device="cuda")
+    if mlp is None:
+        ctx.shade(f["frs"], f["cam"], f["ubo"], pc, k["lights"], 512, k["cells"], k["idx"], f["maps"], rgb)
+    else:
+        ctx.shade_learned(f["frs"], f["cam"], f["ubo"], pc, k["lights"], 512, k["cells"], k["idx"], f["maps"], mlp,
+                          mask, mask_face, bins, rgb, served)
+    torch.cuda.synchronize()
+    return rgb.cpu().numpy(), int(served.item())
+
+
+def test_learned_shading_full_fallback_is_plain(ctx):
+    """An all-zero mask: every fragment falls back to the map, bit for bit."""
+    import torch
+    f = _frame(ctx)
+    mlp = abi.Mlp(ctx, 1.0, 9)
+    mask = torch.zeros(4 * 6 * 8 * 8, dtype=torch.uint8, device="cuda")
+    plain, _ = _shade(ctx, f)
+    learned, served = _shade(ctx, f, mlp, mask, 8, 4)
+    assert served == 0
+    assert np.array_equal(plain.view(np.uint32), learned.view(np.uint32))
+    mlp.close()
+
+
+def test_learned_shading_uses_network(ctx):
+    """An all-one mask and a network that predicts the constant c: equal to
+    plain shading over a specular chain filled with c."""
+    import torch
+    c = np.float32(0.7)
+    chain = torch.full((abi.cube_chain_floats(32, 6),), float(c), dtype=torch.float32, device="cuda")
+    f = _frame(ctx, spec_chain=chain)
+    mlp = abi.Mlp(ctx, 1.0, 9)
+    P = np.zeros(abi.MLP_PARAMS, np.float32)
+    P[16768:16771] = np.log(np.float64(c) + 1e-3)
+    mlp.set_params(P)
+    mask = torch.ones(2 * 6 * 4 * 4, dtype=torch.uint8, device="cuda")
+    plain, _ = _shade(ctx, f)
+    learned, served = _shade(ctx, f, mlp, mask, 4, 2)
+    assert served == f["n"]
+    err = O.rel_err(learned, plain)
+    print(f"constant network vs constant map: max rel err {err:.3e}")
+    assert err <= 1e-5
+    mlp.close()
