@@ -47,6 +47,9 @@ def parse():
     p.add_argument("--no-ibl", action="store_true", help="skip the one-off IBL stage timings")
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample duration")
     p.add_argument("--no-batch", action="store_true", help="skip the batched probes + views workload")
+    p.add_argument("--no-contract", action="store_true", help="skip the contraction (MFMA) filter timings")
+    p.add_argument("--no-learned", action="store_true", help="skip the learned-radiance (MLP) workload")
+    p.add_argument("--mlp-steps", type=int, default=3000)
     p.add_argument("--batch-probes", type=int, default=256)
     p.add_argument("--calibrate", action="store_true",
                    help="run one 256 MiB elementwise copy first (known HBM traffic, for counter scaling)")
@@ -216,12 +219,122 @@ def main():
         "roofline": roofline,
     }
 
+    if not args.no_contract and not args.no_ibl:
+        result["ibl_contract"] = contract_bench(ctx, stream, d_cube, CUBE, SPEC, SPEC_MIPS, DIFF,
+                                                ibl["specular_plain_fp32_ms"], ibl["diffuse_plain_fp32_ms"])
+    if not args.no_learned:
+        result["learned"] = learned_bench(ctx, stream, args.mlp_steps, d_spec, SPEC, SPEC_MIPS, frs, cam, ubo, pc,
+                                          d_lights, NL, d_cells, d_idx, maps, F, shade_ms)
     if not args.no_batch:
         result["batch"] = batch_bench(ctx, dev, stream, args.batch_probes)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(args, ubo, cam, desc, ivp, lights, fr, d_spec, d_diff, d_lut, F, pc)
     print(json.dumps(result), flush=True)
     ctx.close()
+
+
+def _ev_ms(fn, stream, reps):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def contract_bench(ctx, stream, d_cube, S, So, mips, Sd, plain_spec_ms, plain_diff_ms):
+    """The dense-contraction (FP16 MFMA) filters on the same 6x1024^2 cube as
+    the plain filters: plan (weight) construction once, then per-call time.
+    Their error against the FP64 twin is checked by tests/test_gpu_contract.py."""
+    import torch
+    from hellovulkan_amd import abi
+    levels = abi.mip_count(S)
+    out = {}
+    for name, ftype, so, m, plain in [("specular", abi.FILTER_SPECULAR, So, mips, plain_spec_ms),
+                                      ("diffuse", abi.FILTER_DIFFUSE, Sd, 1, plain_diff_ms)]:
+        t0 = time.time()
+        plan = abi.ContractPlan(ctx, ftype, S, so, m)
+        torch.cuda.synchronize()
+        build_s = time.time() - t0
+        dst = torch.empty(abi.cube_chain_floats(so, m), dtype=torch.float32, device=d_cube.device)
+        ms = _ev_ms(lambda: plan.filter(d_cube, levels, dst), stream, 10)
+        info = [plan.info(k) for k in range(m)]
+        out[name] = {"ms": round(ms, 4), "plain_fp32_ms": round(plain, 4), "speedup": round(plain / ms, 2),
+                     "plan_build_s": round(build_s, 3),
+                     "weight_MB": round(sum(i.weight_bytes for i in info) / 1e6, 1),
+                     "mips": [{"kind": ["delta", "full", "near+far"][i.kind], "in": i.in_face, "far": i.far_face}
+                              for i in info]}
+        plan.close()
+    log(f"contraction filters: {out['specular']['ms']} ms specular ({out['specular']['speedup']}x), "
+        f"{out['diffuse']['ms']} ms diffuse ({out['diffuse']['speedup']}x)")
+    return out
+
+
+def learned_bench(ctx, stream, steps, d_spec, SPEC, SPEC_MIPS, frs, cam, ubo, pc, d_lights, NL, d_cells, d_idx, maps,
+                  F, plain_shade_ms):
+    """The per-environment network: 4M samples of the prefiltered chain over
+    the lod range the shading reads (roughness * maxReflectionLod), BF16 MFMA
+    training, the 2e-3 fallback mask, then the frame shaded with the network
+    inferred inside the shading kernel."""
+    import torch
+    from hellovulkan_amd import abi
+    lod_max = float(pc.max_reflection_lod)
+    n, batch = 1 << 22, 16384
+    mlp = abi.Mlp(ctx, lod_max, 2024)
+    x = torch.empty(n * 4, dtype=torch.float32, device=d_spec.device)
+    t = torch.empty(n * 4, dtype=torch.float32, device=d_spec.device)
+    mlp.sample(d_spec, SPEC, SPEC_MIPS, 99, n, x, t)
+    curve = torch.zeros(steps, dtype=torch.float32, device=d_spec.device)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    mlp.train(x, t, n, batch, steps, 2e-3, curve)
+    torch.cuda.synchronize()
+    train_s = time.time() - t0
+    c = curve.cpu().numpy()
+    # held-out samples
+    nh = 1 << 18
+    xh = torch.empty(nh * 4, dtype=torch.float32, device=d_spec.device)
+    th = torch.empty(nh * 4, dtype=torch.float32, device=d_spec.device)
+    mlp.sample(d_spec, SPEC, SPEC_MIPS, 100, nh, xh, th)
+    rgb = torch.empty(nh * 3, dtype=torch.float32, device=d_spec.device)
+    infer_ms = _ev_ms(lambda: mlp.eval(xh, nh, rgb), stream, 5)
+    ref = torch.exp(th.view(-1, 4)[:, :3]) - 1e-3
+    rel = ((rgb.view(-1, 3) - ref).abs() / ref.abs().clamp_min(1e-6)).amax(dim=1)
+    relq = torch.quantile(rel, torch.tensor([0.5, 0.9], device=rel.device)).cpu().numpy()
+    # fallback mask and the learned frame
+    mf, bins = 128, 4
+    mask = torch.zeros(bins * 6 * mf * mf, dtype=torch.uint8, device=d_spec.device)
+    cnt = torch.zeros(1, dtype=torch.int32, device=d_spec.device)
+    mlp.build_mask(d_spec, SPEC, SPEC_MIPS, 2e-3, mf, bins, mask, cnt)
+    out = torch.empty(3 * F, dtype=torch.float32, device=d_spec.device)
+    served = torch.zeros(1, dtype=torch.int32, device=d_spec.device)
+    ctx.shade_learned(frs, cam, ubo, pc, d_lights, NL, d_cells, d_idx, maps, mlp, mask, mf, bins, out, served)
+    torch.cuda.synchronize()
+    served_n = int(served.item())
+    learned_ms = _ev_ms(lambda: ctx.shade_learned(frs, cam, ubo, pc, d_lights, NL, d_cells, d_idx, maps, mlp, mask,
+                                                  mf, bins, out), stream, 5)
+    res = {
+        "workload": f"network per environment over lod [0, {lod_max}] of the 6x512^2 x10 GGX chain; "
+                    f"{n} samples, batch {batch}, {steps} Adam steps; fallback tolerance 2e-3",
+        "train_s": round(train_s, 3),
+        "train_samples_per_s": round(steps * batch / train_s, 1),
+        "loss_first": float(c[0]), "loss_last": float(c[-50:].mean()),
+        "heldout_rel_err_median": float(relq[0]), "heldout_rel_err_p90": float(relq[1]),
+        "heldout_within_2e-3": round(float((rel <= 2e-3).float().mean().item()), 4),
+        "mask_cells_ok": round(int(cnt.item()) / mask.numel(), 4),
+        "served_fraction": round(served_n / F, 4),
+        "infer_Mquery_per_s": round(nh / (infer_ms * 1e-3) / 1e6, 1),
+        "shade_learned_ms": round(learned_ms, 4), "shade_plain_ms": round(plain_shade_ms, 4),
+        "network_bytes": abi.MLP_PARAMS * 4, "map_bytes": abi.cube_chain_floats(SPEC, SPEC_MIPS) * 4,
+    }
+    log(f"learned: loss {res['loss_first']:.3f} -> {res['loss_last']:.4f} in {train_s:.1f}s; served "
+        f"{res['served_fraction']:.3f}; learned shading {learned_ms:.3f} ms vs plain {plain_shade_ms:.3f} ms")
+    mlp.close()
+    return res
 
 
 def batch_bench(ctx, dev, stream, n_probes):

@@ -172,6 +172,7 @@ SIGNATURES = [
     ("hvx_batch_run", C.c_int, [vp, vp]),
     ("hvx_batch_capture", C.c_int, [vp, vp]),
     ("hvx_batch_launch", C.c_int, [vp, vp]),
+    ("hvx_hdr_decode", C.c_int, [vp, C.c_size_t, C.c_int, P(u32), P(u32), vp]),
     ("hvx_camera_build", C.c_int, [P(CameraDesc), P(CameraUBO), P(CFUBO), P(C.c_float)]),
     ("hvx_scene_lights", C.c_int, [C.c_uint64, u32, vp]),
     ("hvx_scene_fragments", C.c_int, [C.c_uint64, P(CameraDesc), P(C.c_float), vp, vp, vp, vp, vp, vp, vp, vp,
@@ -206,6 +207,22 @@ class HvxError(RuntimeError):
         if detail:
             msg += f": {detail}"
         super().__init__(msg)
+
+
+def hdr_decode(data: bytes, flip_vertically: bool = True) -> np.ndarray:
+    """RGBA32F image (height x width x 4) of a Radiance .hdr file's bytes."""
+    lib = load_library()
+    buf = np.frombuffer(data, np.uint8)
+    w, h = C.c_uint32(), C.c_uint32()
+    st = lib.hvx_hdr_decode(buf.ctypes.data, buf.size, int(flip_vertically), C.byref(w), C.byref(h), None)
+    if st:
+        raise HvxError(st, "hvx_hdr_decode")
+    out = np.empty((h.value, w.value, 4), np.float32)
+    st = lib.hvx_hdr_decode(buf.ctypes.data, buf.size, int(flip_vertically), C.byref(w), C.byref(h),
+                            out.ctypes.data)
+    if st:
+        raise HvxError(st, "hvx_hdr_decode")
+    return out
 
 
 def camera_build(eye, target, fovy, znear, zfar, width, height, grid):

@@ -21,7 +21,7 @@ ORACLE_SRC = os.path.join(ROOT, "oracle", "hvx_oracle.cpp")
 ORACLE_LIB = os.path.join(ROOT, "oracle", "_build", "libhvx_oracle.so")
 
 HIP_SOURCES = ["cf_kernels.hip", "shade_kernels.hip", "ibl_kernels.hip", "hvx_api.hip", "batch_api.hip", "contract.hip", "mlp.hip"]
-CPP_SOURCES = ["scene.cpp"]
+CPP_SOURCES = ["scene.cpp", "hdr.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [

@@ -412,6 +412,18 @@ int hvx_batch_run(hvx_batch* batch, void* stream);
 int hvx_batch_capture(hvx_batch* batch, void* stream);
 int hvx_batch_launch(hvx_batch* batch, void* stream);
 
+/* ------------------------------------------------------------ HDR input */
+
+/* Decodes a Radiance RGBE (.hdr) file held in memory into RGBA32F (alpha 1),
+ * as VulkanImage::CreateFromHDR loads the environment
+ * (VulkanImage.cpp:120-146: stbi_loadf with flip-on-load). With rgba NULL it
+ * only reads width and height; otherwise rgba receives width * height * 4
+ * floats, row 0 = the file's last scanline when flip_vertically is set.
+ * HVX_ERR_UNSUPPORTED for other pixel formats or orientations,
+ * HVX_ERR_INVALID_ARG for malformed or truncated data. Host memory only. */
+int hvx_hdr_decode(const uint8_t* data, size_t size, int flip_vertically, uint32_t* width, uint32_t* height,
+                   float* rgba);
+
 /* ------------------------------------------------------------------ camera */
 
 /* Camera description: position/target, vertical fov (radians), clip planes,
