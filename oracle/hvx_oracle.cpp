@@ -820,22 +820,38 @@ struct MlpAct {
     double x[hvx::kMlpIn], h[3][hvx::kMlpHidden], y[hvx::kMlpOut];
 };
 
-void mlp_forward_one(const float* P, const float in[4], double lod_max, MlpAct& a) {
+// Round to the nearest BF16 (ties to even) through FP32, as the kernels
+// round their GEMM operands. Identity when `on` is false.
+double bf16r(double v, bool on) {
+    if (!on) return v;
+    float f = (float)v;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// bf16: the operands the GPU kernels feed to BF16 MFMA are rounded at the
+// same points (encoding, hidden activations, W1..W3, back-propagated
+// errors), everything else stays double
+void mlp_forward_one(const float* P, const float in[4], double lod_max, MlpAct& a, bool bf16 = false) {
     const double xin[4] = {in[0], in[1], in[2], 2.0 * in[3] / lod_max - 1.0};
     for (int j = 0; j < 4; ++j)
         for (int k = 0; k < hvx::kMlpFreqs; ++k) {
             const double t = hvx::mlp_freq<double>(k) * xin[j];
-            a.x[j * 32 + 2 * k] = std::sin(t);
-            a.x[j * 32 + 2 * k + 1] = std::cos(t);
+            a.x[j * 32 + 2 * k] = bf16r(std::sin(t), bf16);
+            a.x[j * 32 + 2 * k + 1] = bf16r(std::cos(t), bf16);
         }
     const int wo[3] = {hvx::kMlpW1, hvx::kMlpW2, hvx::kMlpW3}, bo[3] = {hvx::kMlpB1, hvx::kMlpB2, hvx::kMlpB3};
     const double* prev = a.x;
     int nin = hvx::kMlpIn;
     for (int l = 0; l < 3; ++l) {
         for (int o = 0; o < hvx::kMlpHidden; ++o) {
-            double z = P[bo[l] + o];
-            for (int i = 0; i < nin; ++i) z += (double)P[wo[l] + o * nin + i] * prev[i];
-            a.h[l][o] = z > 0.0 ? z : 0.0;
+            double z = 0.0;
+            for (int i = 0; i < nin; ++i) z += bf16r(P[wo[l] + o * nin + i], bf16) * prev[i];
+            z += P[bo[l] + o];
+            a.h[l][o] = bf16r(z > 0.0 ? z : 0.0, bf16);
         }
         prev = a.h[l];
         nin = hvx::kMlpHidden;
@@ -1055,14 +1071,14 @@ void hvxo_mlp_forward_f64(const float* P, const float* in /* n x 4 */, uint32_t 
     }
 }
 void hvxo_mlp_gradient_f64(const float* P, const float* in, const float* tgt /* n x 4 */, uint32_t n, double lod_max,
-                           double* G /* kMlpParams */, double* loss) {
+                           int bf16, double* G /* kMlpParams */, double* loss) {
     constexpr int H = hvx::kMlpHidden, I = hvx::kMlpIn;
     std::fill(G, G + hvx::kMlpParams, 0.0);
     double L = 0.0;
     const double norm = 1.0 / (3.0 * n);
     for (uint32_t s = 0; s < n; ++s) {
         MlpAct a;
-        mlp_forward_one(P, in + 4 * (size_t)s, lod_max, a);
+        mlp_forward_one(P, in + 4 * (size_t)s, lod_max, a, bf16 != 0);
         double dy[3];
         for (int c = 0; c < 3; ++c) {
             const double e = a.y[c] - tgt[4 * (size_t)s + c];
@@ -1077,7 +1093,7 @@ void hvxo_mlp_gradient_f64(const float* P, const float* in, const float* tgt /* 
         for (int i = 0; i < H; ++i) {
             double d = 0.0;
             for (int c = 0; c < 3; ++c) d += dy[c] * (double)P[hvx::kMlpW4 + c * H + i];
-            dz[i] = a.h[2][i] > 0.0 ? d : 0.0;
+            dz[i] = bf16r(a.h[2][i] > 0.0 ? d : 0.0, bf16 != 0);
         }
         const int wo[3] = {hvx::kMlpW1, hvx::kMlpW2, hvx::kMlpW3}, bo[3] = {hvx::kMlpB1, hvx::kMlpB2, hvx::kMlpB3};
         for (int l = 2; l >= 0; --l) {
@@ -1090,8 +1106,8 @@ void hvxo_mlp_gradient_f64(const float* P, const float* in, const float* tgt /* 
             if (l > 0) {
                 for (int i = 0; i < H; ++i) {
                     double d = 0.0;
-                    for (int o = 0; o < H; ++o) d += dz[o] * (double)P[wo[l] + o * H + i];
-                    dprev[i] = a.h[l - 1][i] > 0.0 ? d : 0.0;
+                    for (int o = 0; o < H; ++o) d += dz[o] * bf16r(P[wo[l] + o * H + i], bf16 != 0);
+                    dprev[i] = bf16r(a.h[l - 1][i] > 0.0 ? d : 0.0, bf16 != 0);
                 }
                 std::copy(dprev, dprev + H, dz);
             }

@@ -49,7 +49,7 @@ _SIGS = [
     ("hvxo_ideal_f64", None, [C.c_int, vp, u32, C.c_double, u32, vp, u32, vp, C.c_int]),
     ("hvxo_contract_levels", C.c_int, [C.c_int, u32, u32, u32, vp, vp]),
     ("hvxo_mlp_forward_f64", None, [vp, vp, u32, C.c_double, vp]),
-    ("hvxo_mlp_gradient_f64", None, [vp, vp, vp, u32, C.c_double, vp, vp]),
+    ("hvxo_mlp_gradient_f64", None, [vp, vp, vp, u32, C.c_double, C.c_int, vp, vp]),
 ]
 
 _lib = None
@@ -275,15 +275,16 @@ def mlp_forward(params: np.ndarray, inputs: np.ndarray, lod_max: float) -> np.nd
     return out
 
 
-def mlp_gradient(params, inputs, targets, lod_max: float):
-    """FP64 gradient (parameter layout) and loss of the mean squared log-radiance error."""
+def mlp_gradient(params, inputs, targets, lod_max: float, bf16: bool = False):
+    """FP64 gradient (parameter layout) and loss of the mean squared log-radiance error;
+    bf16=True rounds the BF16 GEMM operands of the GPU kernels at the same points."""
     params = _c(params, np.float32)
     inputs = _c(inputs, np.float32).reshape(-1, 4)
     targets = _c(targets, np.float32).reshape(-1, 4)
     g = np.empty(MLP_PARAMS, np.float64)
     loss = np.zeros(1, np.float64)
     lib().hvxo_mlp_gradient_f64(params.ctypes.data, inputs.ctypes.data, targets.ctypes.data, inputs.shape[0], lod_max,
-                                g.ctypes.data, loss.ctypes.data)
+                                1 if bf16 else 0, g.ctypes.data, loss.ctypes.data)
     return g, float(loss[0])
 
 

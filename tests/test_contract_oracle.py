@@ -1,0 +1,107 @@
+"""Pins of the FP64 twin of the contraction cube filter (the checker of the
+MFMA path): constants are reproduced exactly, rotating the environment by a
+cube symmetry rotates the output, a full-sphere sum at a coarse level agrees
+with the sum over every level-0 texel for rough lobes on a smooth
+environment, and the near/far split agrees with the full-sphere sum."""
+import numpy as np
+import pytest
+
+from oracle import hvx_oracle as O
+
+
+def face_dirs(S):
+    """World directions of every texel centre of a 6 x S x S cube (face-major)."""
+    c = (2.0 * np.arange(S) + 1.0) / S - 1.0
+    u, v = np.meshgrid(c, c)  # v rows, u columns
+    one = np.ones_like(u)
+    faces = [(one, -v, -u), (-one, -v, u), (u, one, v), (u, -one, -v), (u, -v, one), (-u, -v, -one)]
+    d = np.stack([np.stack(f, axis=-1) for f in faces]).reshape(-1, 3)
+    return d / np.linalg.norm(d, axis=1, keepdims=True)
+
+
+def env_from(fn, S):
+    d = face_dirs(S)
+    rgb = fn(d)
+    return np.concatenate([rgb, np.ones((d.shape[0], 1))], axis=1).astype(np.float32).ravel()
+
+
+def smooth(d):
+    return np.stack([1.5 + 0.8 * d[:, 0] + 0.3 * d[:, 1] * d[:, 2], 1.0 + 0.5 * d[:, 1] ** 2,
+                     0.7 + 0.4 * d[:, 2] - 0.2 * d[:, 0] * d[:, 1]], axis=1)
+
+
+def test_level_table():
+    """Per-mip kinds and roughness of a 64 -> 32 x 6 chain."""
+    lv = O.contract_levels(1, 64, 32, 6)
+    assert [l["kind"] for l in lv] == [O.CONTRACT_DELTA] + [O.CONTRACT_FULL] * 5
+    assert lv[3]["alpha2"] == pytest.approx((3 / 5) ** 4)
+
+
+@pytest.mark.parametrize("S,So,mips", [(64, 32, 6), (128, 64, 7)])
+def test_constant_environment(S, So, mips):
+    lvl = np.full(6 * S * S * 4, 2.25, np.float32)
+    for m in range(mips):
+        r = O.contract(1, lvl, S, So, mips, m, np.arange(min(6 * max(So >> m, 1) ** 2, 200), dtype=np.uint32))
+        assert np.abs(r - 2.25).max() < 1e-12, m
+    r = O.contract(0, lvl, S, 16, 1, 0)
+    assert np.abs(r - 2.25).max() < 1e-12
+
+
+def test_cube_rotation_symmetry():
+    """Rotating the environment by a quarter turn about +Y (a cube symmetry)
+    rotates the filtered result: out_rot(R d) = out(d)."""
+    S, So, mips = 64, 32, 6
+    rng = np.random.default_rng(5)
+    centres = rng.normal(size=(6, 3))
+    centres /= np.linalg.norm(centres, axis=1, keepdims=True)
+
+    def env(d):
+        acc = np.full((d.shape[0], 3), 0.2)
+        for k, c in enumerate(centres):
+            acc[:, k % 3] += 3.0 * np.exp(8.0 * (d @ c - 1.0))
+        return acc
+
+    R = np.array([[0.0, 0.0, 1.0], [0.0, 1.0, 0.0], [-1.0, 0.0, 0.0]])  # quarter turn about y
+    a = env_from(env, S)
+    b = env_from(lambda d: env(d @ R), S)  # b(d) = a(R^T d)
+    dirs = face_dirs(So >> 2)
+    for m in (1, 3):
+        Sm = So >> m
+        d = face_dirs(Sm)
+        ids = np.arange(0, 6 * Sm * Sm, 7, dtype=np.uint32)
+        ra = O.contract(1, a, S, So, mips, m, ids)
+        # texel of R d (a texel centre again: R permutes the cube's texel grid)
+        rd = d[ids] @ R.T
+        D = face_dirs(Sm)
+        jd = np.argmax(rd @ D.T, axis=1).astype(np.uint32)
+        rb = O.contract(1, b, S, So, mips, m, jd)
+        assert np.abs(rb - ra).max() <= 1e-10 * np.abs(ra).max(), m
+    del dirs
+
+
+def test_full_sum_matches_level0_integral_for_rough_lobes():
+    S, So, mips = 128, 64, 7
+    lvl = env_from(smooth, S)
+    lv = O.contract_levels(1, S, So, mips)
+    for m in (4, 5, 6):
+        Sm = max(So >> m, 1)
+        ids = np.arange(0, 6 * Sm * Sm, max(1, 6 * Sm * Sm // 24), dtype=np.uint32)
+        r = O.contract(1, lvl, S, So, mips, m, ids)
+        ideal = O.ideal(1, lvl, S, lv[m]["alpha2"], Sm, ids)
+        assert np.abs(r - ideal).max() / np.abs(ideal).max() < 2e-3, m
+    r = O.contract(0, lvl, S, 16, 1, 0)
+    ideal = O.ideal(0, lvl, S, 0.0, 16, np.arange(6 * 256, dtype=np.uint32))
+    assert np.abs(r - ideal).max() / np.abs(ideal).max() < 2e-3
+
+
+def test_split_agrees_with_ideal_on_smooth_environment():
+    """The near/far split (mip 1 of 128 -> 64 x 7) against the level-0 sum."""
+    S, So, mips = 128, 64, 7
+    lv = O.contract_levels(1, S, So, mips)
+    assert lv[1]["kind"] == O.CONTRACT_SPLIT
+    lvl = env_from(smooth, S)
+    Sm = So >> 1
+    ids = np.random.default_rng(2).integers(0, 6 * Sm * Sm, 40).astype(np.uint32)
+    r = O.contract(1, lvl, S, So, mips, 1, ids)
+    ideal = O.ideal(1, lvl, S, lv[1]["alpha2"], Sm, ids)
+    assert np.abs(r - ideal).max() / np.abs(ideal).max() < 2e-3

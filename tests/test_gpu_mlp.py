@@ -14,7 +14,14 @@ pytestmark = pytest.mark.gpu
 
 BLOCKS = {"W1": (0, 8192), "b1": (8192, 8256), "W2": (8256, 12352), "b2": (12352, 12416), "W3": (12416, 16512),
           "b3": (16512, 16576), "W4": (16576, 16768), "b4": (16768, 16771)}
-GRAD_TOL = 3e-2  # relative L2 error per block, BF16 kernels vs FP64
+# relative L2 error per parameter block of the GPU gradient against the FP64
+# oracle evaluated on the same BF16-rounded operands (only FP32 accumulation
+# order and rare one-ulp BF16 rounding flips differ), and against the exact
+# FP64 gradient (BF16 operand rounding, amplified by cancellation in the
+# batch sums: direction checked by the cosine)
+GRAD_TOL_BF16 = 5e-3
+GRAD_TOL_EXACT = 1e-1
+GRAD_COS_EXACT = 0.99
 
 
 def _batch(rng, n, lod_max):
@@ -37,13 +44,17 @@ def test_gradient_matches_oracle(ctx, seed, lod_max):
     torch.cuda.synchronize()
     got = g.cpu().numpy().astype(np.float64)
     ref, ref_loss = O.mlp_gradient(mlp.params(), x, t, lod_max)
-    print(f"loss gpu {loss.item():.6f} fp64 {ref_loss:.6f}")
+    emu, emu_loss = O.mlp_gradient(mlp.params(), x, t, lod_max, bf16=True)
+    print(f"loss gpu {loss.item():.6f} fp64 {ref_loss:.6f} fp64 on bf16 operands {emu_loss:.6f}")
+    assert abs(loss.item() - emu_loss) <= 1e-4 * emu_loss
     assert abs(loss.item() - ref_loss) <= 1e-2 * ref_loss
     for name, (a, b) in BLOCKS.items():
+        e_emu = np.linalg.norm(got[a:b] - emu[a:b]) / max(np.linalg.norm(emu[a:b]), 1e-30)
         err = np.linalg.norm(got[a:b] - ref[a:b]) / max(np.linalg.norm(ref[a:b]), 1e-30)
         cos = float(got[a:b] @ ref[a:b] / max(np.linalg.norm(got[a:b]) * np.linalg.norm(ref[a:b]), 1e-30))
-        print(f"{name}: relative L2 error {err:.3e}, cosine {cos:.6f}")
-        assert err <= GRAD_TOL, name
+        print(f"{name}: relative L2 error vs bf16-operand fp64 {e_emu:.3e}; vs exact fp64 {err:.3e}, cosine {cos:.6f}")
+        assert e_emu <= GRAD_TOL_BF16, name
+        assert err <= GRAD_TOL_EXACT and cos >= GRAD_COS_EXACT, name
     mlp.close()
 
 
