@@ -161,9 +161,9 @@ SIGNATURES = [
     ("hvx_mlp_gradient", C.c_int, [vp, vp, vp, u32, vp, vp, vp]),
     ("hvx_mlp_train", C.c_int, [vp, vp, vp, u32, u32, u32, C.c_float, vp, vp]),
     ("hvx_mlp_eval", C.c_int, [vp, vp, u32, vp, vp]),
-    ("hvx_mlp_build_mask", C.c_int, [vp, vp, u32, u32, C.c_float, u32, u32, vp, vp, vp]),
+    ("hvx_mlp_cell_errors", C.c_int, [vp, vp, u32, u32, u32, u32, vp, vp]),
     ("hvx_cf_shade_learned", C.c_int, [vp, P(FragSoA), P(CameraUBO), P(CFUBO), P(PBRPC), vp, u32, vp, vp,
-                                       P(IBLMaps), vp, vp, u32, u32, vp, vp, vp]),
+                                       P(IBLMaps), vp, vp, u32, u32, C.c_float, vp, vp, vp]),
     ("hvx_batch_create", C.c_int, [vp, P(Probe), u32, u32, P(View), u32, P(vp)]),
     ("hvx_batch_destroy", C.c_int, [vp]),
     ("hvx_batch_set_tuning", C.c_int, [vp, P(BatchTuning)]),
@@ -353,11 +353,12 @@ class Context:
                                           _ptr(rgb), self._stream(stream)), "hvx_cf_shade")
 
     def shade_learned(self, frags: FragSoA, cam: CameraUBO, ubo: CFUBO, pc: PBRPC, lights, n_lights: int, cells,
-                      indices, maps: IBLMaps, mlp, mask, mask_face: int, lod_bins: int, rgb, served=None, stream=None):
+                      indices, maps: IBLMaps, mlp, cell_err, cell_face: int, lod_bins: int, tol: float, rgb,
+                      served=None, stream=None):
         self._check(self.lib.hvx_cf_shade_learned(self.h, C.byref(frags), C.byref(cam), C.byref(ubo), C.byref(pc),
                                                   _ptr(lights), n_lights, _ptr(cells), _ptr(indices), C.byref(maps),
-                                                  mlp.h, _ptr(mask), mask_face, lod_bins, _ptr(rgb), _ptr(served),
-                                                  self._stream(stream)), "hvx_cf_shade_learned")
+                                                  mlp.h, _ptr(cell_err), cell_face, lod_bins, tol, _ptr(rgb),
+                                                  _ptr(served), self._stream(stream)), "hvx_cf_shade_learned")
 
     def brdf_lut(self, w: int, h: int, samples: int, out, stream=None):
         self._check(self.lib.hvx_ibl_brdf_lut(self.h, w, h, samples, _ptr(out), self._stream(stream)),
@@ -465,11 +466,9 @@ class Mlp:
         self.ctx._check(self.lib.hvx_mlp_train(self.h, _ptr(inputs), _ptr(targets), n, batch, steps, lr,
                                                _ptr(loss_curve), self.ctx._stream(stream)), "hvx_mlp_train")
 
-    def build_mask(self, chain, face: int, mips: int, tol: float, mask_face: int, lod_bins: int, mask, count=None,
-                   stream=None):
-        self.ctx._check(self.lib.hvx_mlp_build_mask(self.h, _ptr(chain), face, mips, tol, mask_face, lod_bins,
-                                                    _ptr(mask), _ptr(count), self.ctx._stream(stream)),
-                        "hvx_mlp_build_mask")
+    def cell_errors(self, chain, face: int, mips: int, cell_face: int, lod_bins: int, err, stream=None):
+        self.ctx._check(self.lib.hvx_mlp_cell_errors(self.h, _ptr(chain), face, mips, cell_face, lod_bins, _ptr(err),
+                                                     self.ctx._stream(stream)), "hvx_mlp_cell_errors")
 
     def eval(self, inputs, n: int, rgb, stream=None):
         self.ctx._check(self.lib.hvx_mlp_eval(self.h, _ptr(inputs), n, _ptr(rgb), self.ctx._stream(stream)),

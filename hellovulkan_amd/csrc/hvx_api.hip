@@ -266,20 +266,21 @@ int hvx_cf_shade(hvx_ctx* ctx, const hvx_frag_soa* fr, const hvx_camera_ubo* cam
 int hvx_cf_shade_learned(hvx_ctx* ctx, const hvx_frag_soa* fr, const hvx_camera_ubo* cam, const hvx_cf_ubo* ubo,
                          const hvx_pbr_pc* pc, const hvx_light* d_lights, uint32_t n_lights,
                          const hvx_light_cell* d_cells, const uint32_t* d_indices, const hvx_ibl_maps* ibl,
-                         const hvx_mlp* mlp, const uint8_t* d_mask, uint32_t mask_face, uint32_t lod_bins,
-                         float* d_rgb, uint32_t* d_served, void* stream) {
+                         const hvx_mlp* mlp, const float* d_cell_err, uint32_t cell_face, uint32_t lod_bins,
+                         float tol, float* d_rgb, uint32_t* d_served, void* stream) {
     if (!ctx) return HVX_ERR_NULL;
-    if (!mlp || !d_mask || !pc) return fail(ctx, HVX_ERR_NULL, "mlp, mask and push constants are required");
-    if (mask_face == 0 || mask_face > 4096 || lod_bins == 0 || lod_bins > 256)
-        return fail(ctx, HVX_ERR_INVALID_ARG, "mask_face in [1, 4096], lod_bins in [1, 256]");
+    if (!mlp || !d_cell_err || !pc) return fail(ctx, HVX_ERR_NULL, "mlp, cell errors and push constants are required");
+    if (cell_face == 0 || cell_face > 1024 || lod_bins == 0 || lod_bins > 256 || !(tol >= 0.0f))
+        return fail(ctx, HVX_ERR_INVALID_ARG, "cell_face in [1, 1024], lod_bins in [1, 256], tol >= 0");
     hvx::LearnedShade L;
     L.mlp = hvx::mlp_view(mlp);
     if (!(pc->max_reflection_lod >= 0.0f) || pc->max_reflection_lod > L.mlp.lod_max)
         return fail(ctx, HVX_ERR_INVALID_ARG, "maxReflectionLod %g exceeds the network's lod range %g",
                     pc->max_reflection_lod, L.mlp.lod_max);
-    L.mask = d_mask;
-    L.mask_face = mask_face;
+    L.cell_err = d_cell_err;
+    L.mask_face = cell_face;
     L.lod_bins = lod_bins;
+    L.tol = tol;
     L.served = d_served;
     return shade_impl(ctx, fr, cam, ubo, pc, d_lights, n_lights, d_cells, d_indices, ibl, d_rgb, &L, stream);
 }

@@ -95,8 +95,9 @@ hipError_t launch_queue_reset(uint32_t* d_queues, uint32_t n, hipStream_t s);
 // Learned prefiltered radiance for shading: the network and its fallback mask.
 struct LearnedShade {
     MlpView mlp;
-    const uint8_t* mask;
+    const float* cell_err;
     uint32_t mask_face, lod_bins;
+    float tol;
     uint32_t* served;
 };
 MlpView mlp_view(const hvx_mlp* mlp);

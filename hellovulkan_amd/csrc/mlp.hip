@@ -327,24 +327,28 @@ __global__ __launch_bounds__(256) void mask_points_kernel(uint32_t mf, uint32_t 
     pts[id] = make_float4(d.x, d.y, d.z, lod);
 }
 
-// mask[cell] = 1 if the network is within tol (relative, every channel) of
-// the chain's textureLod value at every point of the cell
-__global__ __launch_bounds__(256) void mask_cells_kernel(const float4* __restrict__ pts, const float* __restrict__ rgb,
+// err[cell] = largest relative error (every channel, every point of the
+// cell) of the network against the chain's textureLod value; infinity where
+// the reference is 0 and the prediction is not
+__global__ __launch_bounds__(256) void cell_error_kernel(const float4* __restrict__ pts, const float* __restrict__ rgb,
                                                          const float4* __restrict__ chain, uint32_t S, uint32_t mips,
-                                                         float tol, uint32_t cells, uint8_t* __restrict__ mask,
-                                                         uint32_t* __restrict__ count) {
+                                                         uint32_t cells, float* __restrict__ err) {
     const uint32_t cell = blockIdx.x * 256 + threadIdx.x;
     if (cell >= cells) return;
-    bool ok = true;
+    float e = 0.0f;
     for (uint32_t p = 0; p < kMaskPoints; ++p) {
         const float4 q = pts[(size_t)cell * kMaskPoints + p];
         const f3 ref = chain_lod(chain, S, mips, f3{q.x, q.y, q.z}, q.w);
         const float* m = rgb + ((size_t)cell * kMaskPoints + p) * 3;
         const float r3[3] = {ref.x, ref.y, ref.z};
-        for (int c = 0; c < 3; ++c) ok = ok && fabsf(m[c] - r3[c]) <= tol * fabsf(r3[c]);
+        for (int c = 0; c < 3; ++c) {
+            const float d = fabsf(m[c] - r3[c]);
+            float rel = d == 0.0f ? 0.0f : (r3[c] != 0.0f ? d / fabsf(r3[c]) : INFINITY);
+            if (!(rel == rel)) rel = INFINITY;  // a NaN prediction fails every bound
+            e = fmaxf(e, rel);
+        }
     }
-    mask[cell] = ok ? 1 : 0;
-    if (ok && count) atomicAdd(count, 1u);
+    err[cell] = e;
 }
 
 }  // namespace
@@ -649,17 +653,18 @@ int hvx_mlp_eval(hvx_mlp* M, const float* d_inputs, uint32_t n, float* d_rgb, vo
     return e == hipSuccess ? HVX_OK : mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_eval: %s", hipGetErrorString(e));
 }
 
-int hvx_mlp_build_mask(hvx_mlp* M, const float* d_chain, uint32_t face, uint32_t mips, float tol, uint32_t mask_face,
-                       uint32_t lod_bins, uint8_t* d_mask, uint32_t* d_count, void* stream) {
-    if (!M || !d_chain || !d_mask) return HVX_ERR_NULL;
-    if (face == 0 || face > 16384 || mips == 0 || mips > hvx_mip_count(face) || !(tol > 0.0f) || mask_face == 0 ||
-        mask_face > 1024 || lod_bins == 0 || lod_bins > 256)
-        return mfail(M->ctx, HVX_ERR_INVALID_ARG, "hvx_mlp_build_mask: sizes or tolerance out of range");
+int hvx_mlp_cell_errors(hvx_mlp* M, const float* d_chain, uint32_t face, uint32_t mips, uint32_t cell_face,
+                        uint32_t lod_bins, float* d_err, void* stream) {
+    if (!M || !d_chain || !d_err) return HVX_ERR_NULL;
+    const uint32_t mask_face = cell_face;
+    if (face == 0 || face > 16384 || mips == 0 || mips > hvx_mip_count(face) || mask_face == 0 || mask_face > 1024 ||
+        lod_bins == 0 || lod_bins > 256)
+        return mfail(M->ctx, HVX_ERR_INVALID_ARG, "hvx_mlp_cell_errors: sizes out of range");
     int st;
     if ((st = bind(M->ctx))) return st;
     const uint32_t cells = lod_bins * 6 * mask_face * mask_face;
     const size_t npts = (size_t)cells * kMaskPoints;
-    if (npts > (1u << 28)) return mfail(M->ctx, HVX_ERR_INVALID_ARG, "hvx_mlp_build_mask: too many cells");
+    if (npts > (1u << 28)) return mfail(M->ctx, HVX_ERR_INVALID_ARG, "hvx_mlp_cell_errors: too many cells");
     // set-up call: the point and prediction buffers are allocated here
     float4* pts = nullptr;
     float* rgb = nullptr;
@@ -667,19 +672,19 @@ int hvx_mlp_build_mask(hvx_mlp* M, const float* d_chain, uint32_t face, uint32_t
     if ((e = hipMalloc(&pts, npts * sizeof(float4))) != hipSuccess ||
         (e = hipMalloc(&rgb, npts * 3 * sizeof(float))) != hipSuccess) {
         if (pts) (void)hipFree(pts);
-        return mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_build_mask: %s", hipGetErrorString(e));
+        return mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_cell_errors: %s", hipGetErrorString(e));
     }
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(mask_points_kernel, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, mask_face, lod_bins,
                        M->lod_max, pts);
     hipLaunchKernelGGL(mlp_infer_kernel, dim3(std::min(8192u, (unsigned)((npts + 63) / 64))), dim3(256), 0, s,
                        hvx::mlp_view(M), pts, (uint32_t)npts, rgb);
-    hipLaunchKernelGGL(mask_cells_kernel, dim3((cells + 255) / 256), dim3(256), 0, s, pts, rgb,
-                       reinterpret_cast<const float4*>(d_chain), face, mips, tol, cells, d_mask, d_count);
+    hipLaunchKernelGGL(cell_error_kernel, dim3((cells + 255) / 256), dim3(256), 0, s, pts, rgb,
+                       reinterpret_cast<const float4*>(d_chain), face, mips, cells, d_err);
     e = hipStreamSynchronize(s);
     (void)hipFree(pts);
     (void)hipFree(rgb);
-    return e == hipSuccess ? HVX_OK : mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_build_mask: %s", hipGetErrorString(e));
+    return e == hipSuccess ? HVX_OK : mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_cell_errors: %s", hipGetErrorString(e));
 }
 
 }  // extern "C"

@@ -72,9 +72,10 @@ struct ShadeArgs {
     float* out;
     // learned prefiltered radiance (shade_kernel<..., true>)
     MlpView mlp;
-    const uint8_t* mask;  // [lod_bins][6][mask_face][mask_face], 1 = use the network
+    const float* cell_err;  // [lod_bins][6][mask_face][mask_face]: the network's relative error per cell
     uint32_t mask_face, lod_bins;
-    uint32_t* served;     // fragments that used the network (nullable)
+    float mlp_tol;          // bound on the relative radiance error the network may cause
+    uint32_t* served;       // fragments that used the network (nullable)
 };
 
 __global__ __launch_bounds__(256) void shade_light_prep_kernel(const hvx_light* __restrict__ lights, uint32_t n,
@@ -133,20 +134,22 @@ __device__ __forceinline__ f3 frag_reflect(const ShadeArgs& a, uint32_t idx, flo
     return sub3(I, mul3(N, 2.0f * dNI));
 }
 
-// Whether the network serves (R, lod): the fallback mask cell of the cube
-// texel of R at the mask resolution (face selection as for sampling) and of
-// lod's bin.
-__device__ __forceinline__ bool mlp_cell_ok(const ShadeArgs& a, f3 R, float lod) {
+// The network's measured relative error for (R, lod): the cell of the cube
+// texel of R at the cell-grid resolution (face selection as for sampling)
+// and of lod's bin.
+__device__ __forceinline__ float mlp_cell_error(const ShadeArgs& a, f3 R, float lod) {
     const CubeCoord cc = cube_select(R);
     const uint32_t mf = a.mask_face;
     const uint32_t x = min((uint32_t)(cc.s * (float)mf), mf - 1), y = min((uint32_t)(cc.t * (float)mf), mf - 1);
     const uint32_t b = min((uint32_t)(lod / a.mlp.lod_max * (float)a.lod_bins), a.lod_bins - 1);
-    return a.mask[((size_t)(b * 6 + (uint32_t)cc.face) * mf + y) * mf + x] != 0;
+    return a.cell_err[((size_t)(b * 6 + (uint32_t)cc.face) * mf + y) * mf + x];
 }
 
+// Returns whether the learned radiance was used (cell_err >= 0 offers it:
+// the network's value `learned` and the measured relative error of its cell).
 template <bool EXACT>
-__device__ __forceinline__ void shade_one(const ShadeArgs& a, uint32_t idx, uint32_t off, uint32_t count,
-                                          bool use_learned = false, f3 learned = f3{0.0f, 0.0f, 0.0f}) {
+__device__ __forceinline__ bool shade_one(const ShadeArgs& a, uint32_t idx, uint32_t off, uint32_t count,
+                                          float cell_err = -1.0f, f3 learned = f3{0.0f, 0.0f, 0.0f}) {
     const float PI = kPiF;
     const uint32_t nfrag = a.W * a.H;
     const uint32_t px = idx % a.W, py = idx / a.W;
@@ -283,18 +286,53 @@ __device__ __forceinline__ void shade_one(const ShadeArgs& a, uint32_t idx, uint
     const f3 I = {-V.x, -V.y, -V.z};
     const float dNI = dot3(N, I);
     const f3 R = sub3(I, mul3(N, 2.0f * dNI));
-    const f3 pre = use_learned ? learned : sample_spec_lod(a, R, rough * a.pc.max_reflection_lod);
+    if (!(cell_err >= 0.0f)) {  // plain: the map's prefiltered radiance
+        const f3 pre = sample_spec_lod(a, R, rough * a.pc.max_reflection_lod);
+        const float2 brdf = sample_2d_rg_repeat(a.lut, (int)a.lut_w, (int)a.lut_h, NoV, rough);
+        const float irr3[3] = {irr.x, irr.y, irr.z}, pre3[3] = {pre.x, pre.y, pre.z};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float F = F0[c] + (gmax(1.0f - rough, F0[c]) - F0[c]) * p5v;
+            const float kD = (1.0f - F) * oneMinusMetal;
+            const float diffuse = irr3[c] * alb[c];
+            const float specular = pre3[c] * (F * brdf.x + brdf.y);
+            const float ambient = (kD * diffuse + specular) * 1.0f;
+            a.out[(size_t)c * nfrag + idx] = (ambient + 0.0f) + Lo[c];
+        }
+        return false;
+    }
     const float2 brdf = sample_2d_rg_repeat(a.lut, (int)a.lut_w, (int)a.lut_h, NoV, rough);
-    const float irr3[3] = {irr.x, irr.y, irr.z}, pre3[3] = {pre.x, pre.y, pre.z};
+    const float irr3[3] = {irr.x, irr.y, irr.z};
+    float kD[3], diffuse[3], sw[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const float F = F0[c] + (gmax(1.0f - rough, F0[c]) - F0[c]) * p5v;
-        const float kD = (1.0f - F) * oneMinusMetal;
-        const float diffuse = irr3[c] * alb[c];
-        const float specular = pre3[c] * (F * brdf.x + brdf.y);
-        const float ambient = (kD * diffuse + specular) * 1.0f;
+        kD[c] = (1.0f - F) * oneMinusMetal;
+        diffuse[c] = irr3[c] * alb[c];
+        sw[c] = F * brdf.x + brdf.y;  // weight of the prefiltered radiance
+    }
+    // The learned radiance replaces the map sample where the error it can
+    // cause in the shaded radiance, e/(1-e) * learned * weight per channel
+    // (e = the cell's measured relative error of the network), stays within
+    // tol of the radiance without the specular ambient term (a lower bound
+    // of the shaded value, every term being non-negative).
+    bool use = false;
+    if (cell_err < 1.0f) {
+        const float k = cell_err / (1.0f - cell_err);
+        const float l3[3] = {learned.x, learned.y, learned.z};
+        use = true;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) use = use && k * l3[c] * sw[c] <= a.mlp_tol * (kD[c] * diffuse[c] + Lo[c]);
+    }
+    const f3 pre = use ? learned : sample_spec_lod(a, R, rough * a.pc.max_reflection_lod);
+    const float pre3[3] = {pre.x, pre.y, pre.z};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float specular = pre3[c] * sw[c];
+        const float ambient = (kD[c] * diffuse[c] + specular) * 1.0f;
         a.out[(size_t)c * nfrag + idx] = (ambient + 0.0f) + Lo[c];
     }
+    return use;
 }
 
 #ifndef HVX_SHADE_CHUNK
@@ -369,18 +407,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_W
         __shared__ MlpLds s_mlp;
         __shared__ __bf16 s_tile[4][16 * kMlpHidden];
         __shared__ float s_pre[3][kChunk];
-        __shared__ uint8_t s_use[kChunk];
+        __shared__ float s_err[kChunk];
         __shared__ uint32_t s_served;
         mlp_stage(s_mlp, a.mlp);
         if (t == 0) s_served = 0;
         __syncthreads();
         const uint32_t wave = t >> 6, lane = t & 63, i = lane & 15, g = lane >> 4;
-        uint32_t mine = 0;
 #pragma unroll 1
         for (int it = 0; it < kChunk / 64; ++it) {
             const uint32_t local = it * 64 + wave * 16 + i, idx = base + local;
             float x[4] = {0.0f, 0.0f, 1.0f, 0.0f};
-            bool ok = false;
+            float err = -1.0f;
             if (idx < nfrag) {
                 float lod;
                 const f3 R = frag_reflect(a, idx, &lod);
@@ -388,25 +425,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_W
                 x[1] = R.y;
                 x[2] = R.z;
                 x[3] = lod;
-                ok = mlp_cell_ok(a, R, lod);
+                err = mlp_cell_error(a, R, lod);
             }
             const float y = mlp_forward_tile(s_mlp, s_tile[wave], x, a.mlp.lod_max);
             if (g < 3) s_pre[g][local] = y;
-            if (g == 0) {
-                s_use[local] = ok ? 1 : 0;
-                mine += ok ? 1u : 0u;
-            }
+            if (g == 0) s_err[local] = err;
         }
-        if (a.served && mine) atomicAdd(&s_served, mine);
         __syncthreads();
-        if (a.served && t == 0 && s_served) atomicAdd(a.served, s_served);
+        uint32_t mine = 0;
 #pragma unroll 1
         for (int r = 0; r < kChunk / 256; ++r) {
             const uint32_t local = s_order[r * 256 + t];
             const uint32_t idx = base + local;
             if (idx < nfrag)
-                shade_one<EXACT>(a, idx, s_off[local], s_cnt[local], s_use[local] != 0,
-                                 f3{s_pre[0][local], s_pre[1][local], s_pre[2][local]});
+                mine += shade_one<EXACT>(a, idx, s_off[local], s_cnt[local], s_err[local],
+                                         f3{s_pre[0][local], s_pre[1][local], s_pre[2][local]})
+                            ? 1u
+                            : 0u;
+        }
+        if (a.served) {
+            if (mine) atomicAdd(&s_served, mine);
+            __syncthreads();
+            if (t == 0 && s_served) atomicAdd(a.served, s_served);
         }
     } else {
 #pragma unroll 1
@@ -463,9 +503,10 @@ hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const
     a.out = d_rgb;
     if (learned) {
         a.mlp = learned->mlp;
-        a.mask = learned->mask;
+        a.cell_err = learned->cell_err;
         a.mask_face = learned->mask_face;
         a.lod_bins = learned->lod_bins;
+        a.mlp_tol = learned->tol;
         a.served = learned->served;
     }
     const uint32_t F = fr.width * fr.height;

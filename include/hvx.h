@@ -326,27 +326,30 @@ int hvx_mlp_train(hvx_mlp* mlp, const float* d_inputs, const float* d_targets, u
 /* Radiance (RGB, n x 3 floats) predicted at n inputs (dir.xyz, lod), by the
  * inference path the shading kernel uses. */
 int hvx_mlp_eval(hvx_mlp* mlp, const float* d_inputs, uint32_t n, float* d_rgb, void* stream);
-/* Fallback mask for hvx_cf_shade_learned: cell (bin b, face f, y, x) of a
- * mask_face cube and lod_bins bins over [0, lod_max] is 1 when the network is
- * within `tol` relative error (every channel) of the chain's textureLod value
- * at the cell's 15 test points (texel centre and corners at both bin ends and
- * its middle); d_mask holds lod_bins * 6 * mask_face^2 bytes; d_count
- * (nullable, zeroed by the caller) receives the number of cells set. A set-up
- * call: it allocates and synchronises the stream. */
-int hvx_mlp_build_mask(hvx_mlp* mlp, const float* d_chain, uint32_t face, uint32_t mips, float tol,
-                       uint32_t mask_face, uint32_t lod_bins, uint8_t* d_mask, uint32_t* d_count, void* stream);
+/* Measured error of the network for hvx_cf_shade_learned: for each cell
+ * (bin b, face f, texel y, x) of a cell_face cube and lod_bins bins over
+ * [0, lod_max], d_err[(b * 6 + f) * cell_face^2 + y * cell_face + x] receives
+ * the largest relative error (every channel) of the network against the
+ * chain's textureLod value at the cell's 15 test points (texel centre and
+ * corners, at both bin ends and its middle). A set-up call: it allocates
+ * and synchronises the stream. */
+int hvx_mlp_cell_errors(hvx_mlp* mlp, const float* d_chain, uint32_t face, uint32_t mips, uint32_t cell_face,
+                        uint32_t lod_bins, float* d_err, void* stream);
 /* hvx_cf_shade with the prefiltered specular radiance taken from the network
- * wherever the fallback mask allows (the mask cell of R and of
- * lod = roughness * maxReflectionLod), inferred inside the shading kernel
- * with BF16 MFMA; elsewhere the specular map is sampled as hvx_cf_shade
- * does. d_served (nullable, zeroed by the caller) counts the fragments that
- * used the network. pc->max_reflection_lod must not exceed the network's
- * lod_max. */
+ * (inferred inside the shading kernel with BF16 MFMA) wherever the error it
+ * can cause in the shaded radiance stays within `tol` relative: with e the
+ * cell's measured error, e/(1-e) * network value * (F brdf.x + brdf.y) must
+ * not exceed tol times the fragment's radiance without the specular ambient
+ * term, in every channel. Elsewhere the specular map is sampled as
+ * hvx_cf_shade does (the fallback). d_served (nullable, zeroed by the
+ * caller) counts the fragments that used the network. pc->max_reflection_lod
+ * must not exceed the network's lod_max. The cell error is measured at test
+ * points, so it estimates, not bounds, the error between them. */
 int hvx_cf_shade_learned(hvx_ctx* ctx, const hvx_frag_soa* frags, const hvx_camera_ubo* camera, const hvx_cf_ubo* ubo,
                          const hvx_pbr_pc* pc, const hvx_light* d_lights, uint32_t n_lights,
                          const hvx_light_cell* d_cells, const uint32_t* d_indices, const hvx_ibl_maps* ibl,
-                         const hvx_mlp* mlp, const uint8_t* d_mask, uint32_t mask_face, uint32_t lod_bins,
-                         float* d_rgb, uint32_t* d_served, void* stream);
+                         const hvx_mlp* mlp, const float* d_cell_err, uint32_t cell_face, uint32_t lod_bins,
+                         float tol, float* d_rgb, uint32_t* d_served, void* stream);
 
 /* ---------------------------------------------------------------- batches */
 

@@ -135,7 +135,7 @@ def _frame(ctx, spec_chain=None, spec_size=32, spec_mips=6):
     return dict(frs=frs, cam=cam, ubo=ubo, maps=m, n=W * H, keep=keep)
 
 
-def _shade(ctx, f, mlp=None, mask=None, mask_face=0, bins=0):
+def _shade(ctx, f, mlp=None, cell_err=None, cell_face=0, bins=0, tol=2e-3):
     import torch
     rgb = torch.empty(3 * f["n"], dtype=torch.float32, device="cuda")
     pc = abi.pbr_defaults()
@@ -145,26 +145,27 @@ def _shade(ctx, f, mlp=None, mask=None, mask_face=0, bins=0):
         ctx.shade(f["frs"], f["cam"], f["ubo"], pc, k["lights"], 512, k["cells"], k["idx"], f["maps"], rgb)
     else:
         ctx.shade_learned(f["frs"], f["cam"], f["ubo"], pc, k["lights"], 512, k["cells"], k["idx"], f["maps"], mlp,
-                          mask, mask_face, bins, rgb, served)
+                          cell_err, cell_face, bins, tol, rgb, served)
     torch.cuda.synchronize()
     return rgb.cpu().numpy(), int(served.item())
 
 
 def test_learned_shading_full_fallback_is_plain(ctx):
-    """An all-zero mask: every fragment falls back to the map, bit for bit."""
+    """Cells of unbounded error: every fragment falls back to the map, bit
+    for bit."""
     import torch
     f = _frame(ctx)
     mlp = abi.Mlp(ctx, 1.0, 9)
-    mask = torch.zeros(4 * 6 * 8 * 8, dtype=torch.uint8, device="cuda")
+    err = torch.full((4 * 6 * 8 * 8,), float("inf"), dtype=torch.float32, device="cuda")
     plain, _ = _shade(ctx, f)
-    learned, served = _shade(ctx, f, mlp, mask, 8, 4)
+    learned, served = _shade(ctx, f, mlp, err, 8, 4)
     assert served == 0
     assert np.array_equal(plain.view(np.uint32), learned.view(np.uint32))
     mlp.close()
 
 
 def test_learned_shading_uses_network(ctx):
-    """An all-one mask and a network that predicts the constant c: equal to
+    """Error-free cells and a network that predicts the constant c: equal to
     plain shading over a specular chain filled with c."""
     import torch
     c = np.float32(0.7)
@@ -174,11 +175,48 @@ def test_learned_shading_uses_network(ctx):
     P = np.zeros(abi.MLP_PARAMS, np.float32)
     P[16768:16771] = np.log(np.float64(c) + 1e-3)
     mlp.set_params(P)
-    mask = torch.ones(2 * 6 * 4 * 4, dtype=torch.uint8, device="cuda")
+    err = torch.zeros(2 * 6 * 4 * 4, dtype=torch.float32, device="cuda")
     plain, _ = _shade(ctx, f)
-    learned, served = _shade(ctx, f, mlp, mask, 4, 2)
+    learned, served = _shade(ctx, f, mlp, err, 4, 2)
     assert served == f["n"]
-    err = O.rel_err(learned, plain)
-    print(f"constant network vs constant map: max rel err {err:.3e}")
-    assert err <= 1e-5
+    e = O.rel_err(learned, plain)
+    print(f"constant network vs constant map: max rel err {e:.3e}")
+    assert e <= 1e-5
+    mlp.close()
+
+
+def test_trained_network_in_shading(ctx):
+    """A network trained on a chain of the plain filter, its measured cell
+    errors, then the learned frame: the fragments it serves stay within the
+    2e-3 radiance bound of plain shading (the cell error is measured at test
+    points, so a small fraction may exceed it; reported and bounded), the
+    others are plain shading bit for bit."""
+    import torch
+    S, So, mips = 64, 32, 6
+    eq = abi.scene_equirect(21, 2 * S, S)
+    cube = torch.empty(6 * S * S * 4, dtype=torch.float32, device="cuda")
+    ctx.equirect_to_cube(to_dev(eq), 2 * S, S, cube, S)
+    spec = torch.empty(abi.cube_chain_floats(So, mips), dtype=torch.float32, device="cuda")
+    ctx.filter(abi.FILTER_SPECULAR, cube, S, spec, So, mips, 256)
+    f = _frame(ctx, spec_chain=spec, spec_size=So, spec_mips=mips)
+    mlp = abi.Mlp(ctx, 1.0, 5)
+    n = 1 << 20
+    x = torch.empty(n * 4, dtype=torch.float32, device="cuda")
+    t = torch.empty(n * 4, dtype=torch.float32, device="cuda")
+    mlp.sample(spec, So, mips, 7, n, x, t)
+    mlp.train(x, t, n, 8192, 1500, 2e-3)
+    cf, bins = 32, 4
+    err = torch.empty(bins * 6 * cf * cf, dtype=torch.float32, device="cuda")
+    mlp.cell_errors(spec, So, mips, cf, bins, err)
+    plain, _ = _shade(ctx, f)
+    learned, served = _shade(ctx, f, mlp, err, cf, bins, 2e-3)
+    P3, L3 = plain.reshape(3, -1), learned.reshape(3, -1)
+    rel = (np.abs(L3 - P3) / np.maximum(np.abs(P3), 1e-30)).max(axis=0)
+    e = err.cpu().numpy()
+    print(f"cell error median {np.median(e):.3e}; served {served / f['n']:.4f} of fragments; "
+          f"fragments beyond 2e-3: {np.mean(rel > 2e-3):.2e}; max rel diff {rel.max():.3e}")
+    assert served > 0
+    changed = int(np.count_nonzero(rel))
+    assert changed <= served
+    assert np.mean(rel > 2e-3) <= 1e-3
     mlp.close()
