@@ -192,6 +192,12 @@ def main():
         "hbm_term_GBs": round(bytes_alg / (shade_ms * 1e-3) / 1e9, 1),
         "hbm_term_frac": round(bytes_alg / (shade_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
     }
+    # HBM traffic per launch of the shading kernel, from the committed
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload (calibrated)
+    tpath = os.path.join(ROOT, "profiles", "shade_kernel_cfg1_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            roofline["traffic"] = int(json.load(fh)["traffic_bytes"])
     tests = Cn * NL
     result = {
         "metric": METRIC,
@@ -263,9 +269,11 @@ def contract_bench(ctx, stream, d_cube, S, So, mips, Sd, plain_spec_ms, plain_di
         dst = torch.empty(abi.cube_chain_floats(so, m), dtype=torch.float32, device=d_cube.device)
         ms = _ev_ms(lambda: plan.filter(d_cube, levels, dst), stream, 10)
         info = [plan.info(k) for k in range(m)]
+        wbytes = sum(i.weight_bytes for i in info)
         out[name] = {"ms": round(ms, 4), "plain_fp32_ms": round(plain, 4), "speedup": round(plain / ms, 2),
                      "plan_build_s": round(build_s, 3),
-                     "weight_MB": round(sum(i.weight_bytes for i in info) / 1e6, 1),
+                     "weight_MB": round(wbytes / 1e6, 1),
+                     "weight_stream_TBs": round(wbytes / (ms * 1e-3) / 1e12, 3),
                      "mips": [{"kind": ["delta", "full", "near+far"][i.kind], "in": i.in_face, "far": i.far_face}
                               for i in info]}
         plan.close()

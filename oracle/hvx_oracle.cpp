@@ -837,9 +837,12 @@ double bf16r(double v, bool on) {
 // errors), everything else stays double
 void mlp_forward_one(const float* P, const float in[4], double lod_max, MlpAct& a, bool bf16 = false) {
     const double xin[4] = {in[0], in[1], in[2], 2.0 * in[3] / lod_max - 1.0};
+    // with bf16, the encoding's argument is formed in FP32 as the kernels
+    // form it (frequency and lod coordinate rounded to float)
+    const float xf[4] = {in[0], in[1], in[2], 2.0f * in[3] / (float)lod_max - 1.0f};
     for (int j = 0; j < 4; ++j)
         for (int k = 0; k < hvx::kMlpFreqs; ++k) {
-            const double t = hvx::mlp_freq<double>(k) * xin[j];
+            const double t = bf16 ? (double)(hvx::mlp_freq<float>(k) * xf[j]) : hvx::mlp_freq<double>(k) * xin[j];
             a.x[j * 32 + 2 * k] = bf16r(std::sin(t), bf16);
             a.x[j * 32 + 2 * k + 1] = bf16r(std::cos(t), bf16);
         }
