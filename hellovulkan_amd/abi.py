@@ -35,6 +35,7 @@ OPT_SHADE_EXACT = 1
 
 FILTER_DIFFUSE = 0
 FILTER_SPECULAR = 1
+TONEMAP_REINHARD, TONEMAP_ACES, TONEMAP_FILMIC, TONEMAP_UNCHARTED2 = 0, 1, 2, 3
 
 f4 = C.c_float * 4
 f16 = C.c_float * 16
@@ -172,6 +173,7 @@ SIGNATURES = [
     ("hvx_batch_run", C.c_int, [vp, vp]),
     ("hvx_batch_capture", C.c_int, [vp, vp]),
     ("hvx_batch_launch", C.c_int, [vp, vp]),
+    ("hvx_post_tonemap", C.c_int, [vp, vp, u32, u32, C.c_int, C.c_int, vp, vp, vp]),
     ("hvx_hdr_decode", C.c_int, [vp, C.c_size_t, C.c_int, P(u32), P(u32), vp]),
     ("hvx_camera_build", C.c_int, [P(CameraDesc), P(CameraUBO), P(CFUBO), P(C.c_float)]),
     ("hvx_scene_lights", C.c_int, [C.c_uint64, u32, vp]),
@@ -359,6 +361,11 @@ class Context:
                                                   _ptr(lights), n_lights, _ptr(cells), _ptr(indices), C.byref(maps),
                                                   mlp.h, _ptr(cell_err), cell_face, lod_bins, tol, _ptr(rgb),
                                                   _ptr(served), self._stream(stream)), "hvx_cf_shade_learned")
+
+    def tonemap(self, rgb, width: int, height: int, op: int = 0, gamma: bool = True, out_rgb=None, out_bgra8=None,
+                stream=None):
+        self._check(self.lib.hvx_post_tonemap(self.h, _ptr(rgb), width, height, op, 1 if gamma else 0, _ptr(out_rgb),
+                                              _ptr(out_bgra8), self._stream(stream)), "hvx_post_tonemap")
 
     def brdf_lut(self, w: int, h: int, samples: int, out, stream=None):
         self._check(self.lib.hvx_ibl_brdf_lut(self.h, w, h, samples, _ptr(out), self._stream(stream)),

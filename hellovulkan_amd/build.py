@@ -20,7 +20,7 @@ LIB = os.path.join(PKG, "libhvx.so")
 ORACLE_SRC = os.path.join(ROOT, "oracle", "hvx_oracle.cpp")
 ORACLE_LIB = os.path.join(ROOT, "oracle", "_build", "libhvx_oracle.so")
 
-HIP_SOURCES = ["cf_kernels.hip", "shade_kernels.hip", "ibl_kernels.hip", "hvx_api.hip", "batch_api.hip", "contract.hip", "mlp.hip"]
+HIP_SOURCES = ["cf_kernels.hip", "shade_kernels.hip", "ibl_kernels.hip", "hvx_api.hip", "batch_api.hip", "contract.hip", "mlp.hip", "post_kernels.hip"]
 CPP_SOURCES = ["scene.cpp", "hdr.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -285,6 +285,24 @@ int hvx_cf_shade_learned(hvx_ctx* ctx, const hvx_frag_soa* fr, const hvx_camera_
     return shade_impl(ctx, fr, cam, ubo, pc, d_lights, n_lights, d_cells, d_indices, ibl, d_rgb, &L, stream);
 }
 
+// -------------------------------------------------------------- tonemap
+int hvx_post_tonemap(hvx_ctx* ctx, const float* d_rgb, uint32_t width, uint32_t height, int op, int gamma,
+                     float* d_out_rgb, uint32_t* d_out_bgra8, void* stream) {
+    if (!ctx) return HVX_ERR_NULL;
+    if (!d_rgb || (!d_out_rgb && !d_out_bgra8)) return fail(ctx, HVX_ERR_NULL, "input or both outputs NULL");
+    if (width == 0 || height == 0 || (uint64_t)width * height > (1ull << 30))
+        return fail(ctx, HVX_ERR_INVALID_ARG, "image %ux%u out of range", width, height);
+    if (op < HVX_TONEMAP_REINHARD || op > HVX_TONEMAP_UNCHARTED2)
+        return fail(ctx, HVX_ERR_INVALID_ARG, "unknown tonemap operator %d", op);
+    if (misaligned(d_rgb, 4) || misaligned(d_out_rgb, 4) || misaligned(d_out_bgra8, 4))
+        return fail(ctx, HVX_ERR_ALIGN, "buffers must be 4-byte aligned");
+    int st;
+    if ((st = bind_device(ctx))) return st;
+    const hipError_t e = hvx::launch_tonemap(d_rgb, width * height, op, gamma ? 1 : 0, d_out_rgb, d_out_bgra8,
+                                             (hipStream_t)stream);
+    return e == hipSuccess ? HVX_OK : hip_fail(ctx, e, "hvx_post_tonemap launch");
+}
+
 // ------------------------------------------------------------------- IBL
 int hvx_ibl_brdf_lut(hvx_ctx* ctx, uint32_t width, uint32_t height, uint32_t sample_count, float* d_lut_rg,
                      void* stream) {

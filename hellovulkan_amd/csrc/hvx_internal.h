@@ -102,6 +102,9 @@ struct LearnedShade {
 };
 MlpView mlp_view(const hvx_mlp* mlp);
 
+hipError_t launch_tonemap(const float* d_rgb, uint32_t n, int op, int gamma, float* d_out, uint32_t* d_bgra,
+                          hipStream_t s);
+
 hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const hvx_cf_ubo& ubo,
                         const hvx_pbr_pc& pc, const hvx_light* d_lights, uint32_t n_lights,
                         const hvx_light_cell* d_cells, const uint32_t* d_indices, const hvx_ibl_maps& ibl,

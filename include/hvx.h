@@ -415,6 +415,24 @@ int hvx_batch_run(hvx_batch* batch, void* stream);
 int hvx_batch_capture(hvx_batch* batch, void* stream);
 int hvx_batch_launch(hvx_batch* batch, void* stream);
 
+/* ------------------------------------------------------------- tonemap */
+
+typedef enum hvx_tonemap_op {
+    HVX_TONEMAP_REINHARD = 0,   /* Tonemap.frag main(): Reinhard, pure white 1.0 */
+    HVX_TONEMAP_ACES = 1,
+    HVX_TONEMAP_FILMIC = 2,
+    HVX_TONEMAP_UNCHARTED2 = 3
+} hvx_tonemap_op;
+
+/* PipelineTonemap::FillCommandBuffer (PipelineTonemap.cpp:34-53) running
+ * Tonemap.frag: maps the planar FP32 RGB shading output (width*height pixels
+ * per plane) with curve `op`, then gamma 2.2 when `gamma` is set (main()
+ * uses HVX_TONEMAP_REINHARD with gamma). Writes planar FP32 RGB to d_out_rgb
+ * and/or packed B8G8R8A8_UNORM (round to nearest even, NaN -> 0, alpha 255)
+ * to d_out_bgra8; either may be NULL, not both. */
+int hvx_post_tonemap(hvx_ctx* ctx, const float* d_rgb, uint32_t width, uint32_t height, int op, int gamma,
+                     float* d_out_rgb, uint32_t* d_out_bgra8, void* stream);
+
 /* ------------------------------------------------------------ HDR input */
 
 /* Decodes a Radiance RGBE (.hdr) file held in memory into RGBA32F (alpha 1),

@@ -866,6 +866,41 @@ void mlp_forward_one(const float* P, const float in[4], double lod_max, MlpAct& 
     }
 }
 
+// ================================================================ tonemap
+// Shaders/Common/Tonemap.frag:17-94, per pixel, in the shader's order.
+template <class T> T tm_partial(T x) {
+    const T A = T(0.15f), B = T(0.50f), C = T(0.10f), D = T(0.20f), E = T(0.02f), F = T(0.30f);
+    return ((x * (A * x + C * B) + D * E) / (x * (A * x + B) + D * F)) - E / F;
+}
+
+template <class T> void tonemap_px(T c[3], int op, int gamma) {
+    if (op == 0) {  // Reinhard, pure white 1.0 (:26-35)
+        const T pw = T(1);
+        const T L = (c[0] * T(0.2126f) + c[1] * T(0.7152f)) + c[2] * T(0.0722f);
+        const T mapped = (L * (T(1) + L / (pw * pw))) / (T(1) + L);
+        const T k = mapped / L;
+        for (int j = 0; j < 3; ++j) c[j] = k * c[j];
+    } else if (op == 1) {  // ACES (:37-45)
+        const T a = T(2.51f), b = T(0.03f), cc = T(2.43f), d = T(0.59f), e = T(0.14f);
+        for (int j = 0; j < 3; ++j) {
+            const T x = c[j];
+            c[j] = gclamp((x * (a * x + b)) / (x * (cc * x + d) + e), T(0), T(1));
+        }
+    } else if (op == 2) {  // Filmic (:47-52)
+        for (int j = 0; j < 3; ++j) {
+            const T X = gmax(T(0), c[j] - T(0.004f));
+            c[j] = Mth<T>::pow((X * (T(6.2f) * X + T(0.5f))) / (X * (T(6.2f) * X + T(1.7f)) + T(0.06f)), T(2.2f));
+        }
+    } else {  // Uncharted2 (:65-73)
+        const T whiteScale = T(1) / tm_partial<T>(T(11.2f));
+        for (int j = 0; j < 3; ++j) c[j] = tm_partial<T>(c[j] * T(2)) * whiteScale;
+    }
+    if (gamma) {
+        const T inv = T(1) / T(2.2f);
+        for (int j = 0; j < 3; ++j) c[j] = Mth<T>::pow(c[j], inv);
+    }
+}
+
 }  // namespace
 
 // ======================================================================= C API
@@ -1063,6 +1098,22 @@ void hvxo_ibl_specular_f64(const float* cube, uint32_t S, uint32_t So, uint32_t 
     const double rough = specular_roughness<double>(mip, mips);
     parallel_for(n_ids, threads,
                  [&](uint32_t k) { specular_texel<double>(cube, S, Sm, D, rough, n, ids[k], out + 4 * k); });
+}
+
+// ---- tonemap (planar RGB in and out)
+void hvxo_tonemap_f32(const float* rgb, uint32_t n, int op, int gamma, float* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        float c[3] = {rgb[i], rgb[(size_t)n + i], rgb[2 * (size_t)n + i]};
+        tonemap_px<float>(c, op, gamma);
+        for (int j = 0; j < 3; ++j) out[(size_t)j * n + i] = c[j];
+    }
+}
+void hvxo_tonemap_f64(const float* rgb, uint32_t n, int op, int gamma, double* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        double c[3] = {rgb[i], rgb[(size_t)n + i], rgb[2 * (size_t)n + i]};
+        tonemap_px<double>(c, op, gamma);
+        for (int j = 0; j < 3; ++j) out[(size_t)j * n + i] = c[j];
+    }
 }
 
 // ---- radiance network: FP64 forward (log-radiance y) and gradient

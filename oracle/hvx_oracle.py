@@ -49,6 +49,8 @@ _SIGS = [
     ("hvxo_ideal_f64", None, [C.c_int, vp, u32, C.c_double, u32, vp, u32, vp, C.c_int]),
     ("hvxo_contract_levels", C.c_int, [C.c_int, u32, u32, u32, vp, vp]),
     ("hvxo_mlp_forward_f64", None, [vp, vp, u32, C.c_double, vp]),
+    ("hvxo_tonemap_f32", None, [vp, u32, C.c_int, C.c_int, vp]),
+    ("hvxo_tonemap_f64", None, [vp, u32, C.c_int, C.c_int, vp]),
     ("hvxo_mlp_gradient_f64", None, [vp, vp, vp, u32, C.c_double, C.c_int, vp, vp]),
 ]
 
@@ -264,6 +266,16 @@ def ideal(ftype: int, level0: np.ndarray, S: int, alpha2: float, So_mip: int, id
 
 
 MLP_PARAMS = 16771
+
+
+def tonemap(rgb_planar: np.ndarray, op: int, gamma: bool = True, dtype=np.float32) -> np.ndarray:
+    """Tonemap.frag of planar RGB (3 x n), FP32 (parity) or FP64."""
+    rgb = _c(rgb_planar, np.float32).reshape(3, -1)
+    n = rgb.shape[1]
+    out = np.empty((3, n), dtype)
+    fn = lib().hvxo_tonemap_f32 if dtype == np.float32 else lib().hvxo_tonemap_f64
+    fn(rgb.ctypes.data, n, op, 1 if gamma else 0, out.ctypes.data)
+    return out
 
 
 def mlp_forward(params: np.ndarray, inputs: np.ndarray, lod_max: float) -> np.ndarray:
