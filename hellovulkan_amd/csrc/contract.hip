@@ -253,15 +253,17 @@ __global__ __launch_bounds__(256) void near_scale_kernel(NearGeom G, uint32_t Sf
 }
 
 // ------------------------------------------------------ per-call staging
-__global__ __launch_bounds__(64) void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
-    if (threadIdx.x < n) p[threadIdx.x] = v;
-}
+// A filter call is eight launches: face maxima, exponent scales, all staging
+// (every dense level and padded plane), all dense GEMMs, all their
+// epilogues, all near-field tiles, all far-field additions, and the
+// roughness-0 mip. Each fused launch finds its job by running offsets in a
+// table passed by value.
+constexpr int kMaxJobs = 2 * hvx::kContractMaxMips;
+constexpr uint32_t kFaceMaxBlocks = 32;
 
-// Largest RGB magnitude per face of a level (float bits of non-negative
-// values order as unsigned integers, so atomicMax is exact and
-// order-independent).
+// partial[f][b]: largest RGB magnitude of block b's share of face f
 __global__ __launch_bounds__(256) void facemax_kernel(const float4* __restrict__ level, uint32_t S,
-                                                      uint32_t* __restrict__ fmax) {
+                                                      float* __restrict__ partial) {
     __shared__ float sh[256];
     const uint32_t f = blockIdx.y, S2 = S * S;
     float m = 0.0f;
@@ -276,17 +278,25 @@ __global__ __launch_bounds__(256) void facemax_kernel(const float4* __restrict__
         if (threadIdx.x < h) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + h]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicMax(&fmax[f], __float_as_uint(sh[0]));
+    if (threadIdx.x == 0) partial[f * gridDim.x + blockIdx.x] = sh[0];
 }
 
 // sc[f] = 2^-e_f, sc[6+f] = 2^e_f for the padded plane of face f (bounded by
 // face f and its four neighbours); sc[12] = 2^-e, sc[13] = 2^e over all faces.
-__global__ __launch_bounds__(64) void scales_kernel(const uint32_t* __restrict__ fmax, float* __restrict__ sc) {
+__global__ __launch_bounds__(64) void scales_kernel(const float* __restrict__ partial, uint32_t nb,
+                                                    float* __restrict__ sc) {
+    __shared__ float fm[6];
     const uint32_t t = threadIdx.x;
+    if (t < 6) {
+        float m = 0.0f;
+        for (uint32_t b = 0; b < nb; ++b) m = fmaxf(m, partial[t * nb + b]);
+        fm[t] = m;
+    }
+    __syncthreads();
     if (t > 6) return;
     float m = 0.0f;
     for (uint32_t g = 0; g < 6; ++g)
-        if (t == 6 || g != (t ^ 1u)) m = fmaxf(m, __uint_as_float(fmax[g]));
+        if (t == 6 || g != (t ^ 1u)) m = fmaxf(m, fm[g]);
     int e = 0;
     if (m > 0.0f && m <= 3.0e38f) {
         int ex;
@@ -302,18 +312,28 @@ __global__ __launch_bounds__(64) void scales_kernel(const uint32_t* __restrict__
     }
 }
 
+struct StageItem {
+    int kind;  // 0: a dense level (permuted), 1: padded face planes
+    uint32_t S, B, Pw;
+    size_t level_off;  // float4 offset of the level in the input chain
+    const int32_t* perm;
+    half8* dst;
+    uint32_t begin;  // first thread of the item
+};
+struct StageTable {
+    StageItem it[kMaxJobs];
+    uint32_t n, total;
+};
+
 // R[k/8][col][8]: col = 4 f + c holds channel c (< 3) of the level texel
 // that output face f sees as canonical input k; other columns are zero.
-__global__ __launch_bounds__(256) void stage_dense_kernel(const float4* __restrict__ level, uint32_t Sin,
-                                                          const int32_t* __restrict__ perm,
-                                                          const float* __restrict__ sc, half8* __restrict__ R) {
+__device__ __forceinline__ void stage_dense_body(const float4* __restrict__ level, uint32_t Sin,
+                                                 const int32_t* __restrict__ perm, float s, half8* __restrict__ R,
+                                                 uint32_t id) {
     const uint32_t K = 6 * Sin * Sin;
-    const uint32_t id = blockIdx.x * 256 + threadIdx.x;
-    if (id >= (K / 8) * 32) return;
     const uint32_t col = id & 31, k8 = id >> 5, f = col >> 2, c = col & 3;
     half8 out;
     if (col < 24 && c < 3) {
-        const float s = sc[12];
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const float4 v = level[perm[(size_t)f * K + k8 * 8 + q]];
@@ -329,12 +349,9 @@ __global__ __launch_bounds__(256) void stage_dense_kernel(const float4* __restri
 // P[py][px/8][col][8] over the padded plane (Pw x Pw, texel index -B .. Si+B)
 // of each face: inside the face the texel itself, outside the bilinear
 // seamless sample of the level at the plane point's direction.
-__global__ __launch_bounds__(256) void stage_pad_kernel(const float4* __restrict__ level, uint32_t Si, uint32_t B,
-                                                        uint32_t Pw, const float* __restrict__ sc,
-                                                        half8* __restrict__ P) {
+__device__ __forceinline__ void stage_pad_body(const float4* __restrict__ level, uint32_t Si, uint32_t B, uint32_t Pw,
+                                               const float* __restrict__ sc, half8* __restrict__ P, uint32_t id) {
     const uint32_t Pw8 = Pw / 8;
-    const uint32_t id = blockIdx.x * 256 + threadIdx.x;
-    if (id >= Pw * Pw8 * 32) return;
     const uint32_t col = id & 31, cell = id >> 5, px8 = cell % Pw8, py = cell / Pw8;
     const uint32_t f = col >> 2, c = col & 3;
     half8 out;
@@ -364,23 +381,52 @@ __global__ __launch_bounds__(256) void stage_pad_kernel(const float4* __restrict
     P[id] = out;
 }
 
+__global__ __launch_bounds__(256) void stage_all_kernel(const float4* __restrict__ chain, StageTable T,
+                                                        const float* __restrict__ sc) {
+    const uint32_t id = blockIdx.x * 256 + threadIdx.x;
+    if (id >= T.total) return;
+    uint32_t j = 0;
+    while (j + 1 < T.n && id >= T.it[j + 1].begin) ++j;
+    const StageItem& I = T.it[j];
+    const float4* level = chain + I.level_off;
+    if (I.kind == 0) stage_dense_body(level, I.S, I.perm, sc[12], I.dst, id - I.begin);
+    else stage_pad_body(level, I.S, I.B, I.Pw, sc, I.dst, id - I.begin);
+}
+
 // ----------------------------------------------------------- MFMA GEMMs
+struct DenseJob {
+    const half8* W;
+    const half8* R;
+    uint32_t rt, ks, splitk, rows;
+    float* part;            // [splitk][rt * 16][32]
+    const float* rowscale;
+    float4* out;            // plan-owned far map, or nullptr: the output chain at out_off
+    size_t out_off;
+    float alpha;
+    uint32_t wave_begin, epi_begin;
+};
+struct DenseTable {
+    DenseJob j[kMaxJobs];
+    uint32_t n, waves, epi;
+};
+
 // Dense: a wave owns two row tiles (32 rows) x 32 columns over one K slice;
 // partial sums go to part[slice][row][32] and are reduced in slice order.
-__global__ __launch_bounds__(256) void dense_gemm_kernel(const half8* __restrict__ W, const half8* __restrict__ R,
-                                                         uint32_t rt, uint32_t ks, uint32_t splitk,
-                                                         float* __restrict__ part) {
-    const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const uint32_t pairs = (rt + 1) / 2;
-    if (wave >= pairs * splitk) return;  // uniform per wave; the kernel has no barriers
+__global__ __launch_bounds__(256) void dense_gemm_kernel(DenseTable T) {
+    const uint32_t gw = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (gw >= T.waves) return;  // uniform per wave; the kernel has no barriers
+    uint32_t jj = 0;
+    while (jj + 1 < T.n && gw >= T.j[jj + 1].wave_begin) ++jj;
+    const DenseJob& J = T.j[jj];
+    const uint32_t wave = gw - J.wave_begin, rt = J.rt, ks = J.ks, splitk = J.splitk;
     const uint32_t pair = wave / splitk, slice = wave - pair * splitk;
     const uint32_t rt0 = 2 * pair;
     const bool has1 = rt0 + 1 < rt;
     const uint32_t kb = (uint32_t)((uint64_t)slice * ks / splitk), ke = (uint32_t)((uint64_t)(slice + 1) * ks / splitk);
     const uint32_t i = lane & 15, g = lane >> 4;
-    const half8* w0 = W + (size_t)rt0 * ks * 64 + lane;
+    const half8* w0 = J.W + (size_t)rt0 * ks * 64 + lane;
     const half8* w1 = has1 ? w0 + (size_t)ks * 64 : w0;
-    const half8* rb = R + g * 32 + i;
+    const half8* rb = J.R + g * 32 + i;
     f32x4 a00 = {0.0f, 0.0f, 0.0f, 0.0f}, a01 = a00, a10 = a00, a11 = a00;
     for (uint32_t s = kb; s < ke; ++s) {
         const half8 x0 = w0[(size_t)s * 64], x1 = w1[(size_t)s * 64];
@@ -390,7 +436,7 @@ __global__ __launch_bounds__(256) void dense_gemm_kernel(const half8* __restrict
         a10 = hvx::mfma16x16x32(x1, b0, a10);
         a11 = hvx::mfma16x16x32(x1, b1, a11);
     }
-    float* p = part + (size_t)slice * rt * 16 * 32;
+    float* p = J.part + (size_t)slice * rt * 16 * 32;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const uint32_t row0 = rt0 * 16 + 4 * g + r;
@@ -403,45 +449,69 @@ __global__ __launch_bounds__(256) void dense_gemm_kernel(const half8* __restrict
     }
 }
 
-// out[f][row] = (sum over slices) * rowscale[row] * 2^e  (alpha = 1 or 0)
-__global__ __launch_bounds__(256) void dense_epilogue_kernel(const float* __restrict__ part, uint32_t splitk,
-                                                             uint32_t rows, uint32_t rows_pad,
-                                                             const float* __restrict__ rowscale,
-                                                             const float* __restrict__ sc, float alpha,
-                                                             float4* __restrict__ out) {
+// out[f][row] = (sum over slices) * rowscale[row] * 2^e  (alpha 1 in the
+// chain, 0 in a far map)
+__global__ __launch_bounds__(256) void dense_epilogue_kernel(DenseTable T, const float* __restrict__ sc,
+                                                             float4* __restrict__ chain_out) {
     const uint32_t id = blockIdx.x * 256 + threadIdx.x;
-    if (id >= rows * 6) return;
-    const uint32_t f = id / rows, row = id - f * rows;
+    if (id >= T.epi) return;
+    uint32_t jj = 0;
+    while (jj + 1 < T.n && id >= T.j[jj + 1].epi_begin) ++jj;
+    const DenseJob& J = T.j[jj];
+    const uint32_t local = id - J.epi_begin, rows = J.rows, f = local / rows, row = local - f * rows;
+    const uint32_t rows_pad = J.rt * 16;
     float v[3] = {0.0f, 0.0f, 0.0f};
-    for (uint32_t s = 0; s < splitk; ++s) {
-        const float* p = part + ((size_t)s * rows_pad + row) * 32 + f * 4;
+    for (uint32_t s = 0; s < J.splitk; ++s) {
+        const float* p = J.part + ((size_t)s * rows_pad + row) * 32 + f * 4;
         v[0] += p[0];
         v[1] += p[1];
         v[2] += p[2];
     }
-    const float k = rowscale[row] * sc[13];
-    out[(size_t)f * rows + row] = make_float4(v[0] * k, v[1] * k, v[2] * k, alpha);
+    const float k = J.rowscale[row] * sc[13];
+    float4* out = J.out ? J.out : chain_out + J.out_off;
+    out[(size_t)f * rows + row] = make_float4(v[0] * k, v[1] * k, v[2] * k, J.alpha);
 }
+
+struct NearJob {
+    const half8* W;
+    const NearTile* tiles;
+    const half8* P;
+    uint32_t Pw8, B, te, So, Sf;
+    const float* rowscale;
+    const float* invZ;
+    const float4* far;
+    size_t out_off;  // float4 offset of the mip in the output chain
+    uint32_t tile_begin, texel_begin;
+};
+struct NearTable {
+    NearJob j[kMaxJobs];
+    uint32_t n, tiles, texels;
+};
+
+// the tables travel as kernel arguments (at most 4 KB)
+static_assert(sizeof(StageTable) < 4000 && sizeof(DenseTable) < 4000 && sizeof(NearTable) < 4000,
+              "launch tables must fit the kernel argument space");
 
 // Near field: one workgroup per 16 x 16 output tile, eight waves of two
 // output rows each; B rows come from the face planes' window.
-__global__ __launch_bounds__(512) void near_gemm_kernel(const half8* __restrict__ W, const NearTile* __restrict__ tiles,
-                                                        const half8* __restrict__ P, uint32_t Pw8, uint32_t B,
-                                                        uint32_t te, uint32_t So, const float* __restrict__ rowscale,
-                                                        const float* __restrict__ sc, float* __restrict__ out) {
-    const uint32_t tile = blockIdx.x, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const NearTile T = tiles[tile];
-    const uint32_t wc = T.w / 32, KS = T.h * wc;
+__global__ __launch_bounds__(512) void near_gemm_kernel(NearTable T, const float* __restrict__ sc,
+                                                        float4* __restrict__ chain_out) {
+    uint32_t jj = 0;
+    while (jj + 1 < T.n && blockIdx.x >= T.j[jj + 1].tile_begin) ++jj;
+    const NearJob& J = T.j[jj];
+    const uint32_t tile = blockIdx.x - J.tile_begin, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const NearTile Tl = J.tiles[tile];
+    const uint32_t wc = Tl.w / 32, KS = Tl.h * wc;
     const uint32_t i = lane & 15, g = lane >> 4, rt0 = 2 * wave;
-    const half8* w0 = W + T.woff + (size_t)rt0 * KS * 64 + lane;
+    const half8* w0 = J.W + Tl.woff + (size_t)rt0 * KS * 64 + lane;
     const half8* w1 = w0 + (size_t)KS * 64;
-    const uint32_t xb = (uint32_t)(T.x0 + (int)B) / 8 + g;
+    const uint32_t xb = (uint32_t)(Tl.x0 + (int)J.B) / 8 + g;
     f32x4 a00 = {0.0f, 0.0f, 0.0f, 0.0f}, a01 = a00, a10 = a00, a11 = a00;
     uint32_t r = 0, cc = 0;
     for (uint32_t s = 0; s < KS; ++s) {
-        const size_t pidx = (((size_t)(T.y0 + (int)B + (int)r) * Pw8 + xb + cc * 4) * 32) + i;
+        const size_t pidx = (((size_t)(Tl.y0 + (int)J.B + (int)r) * J.Pw8 + xb + cc * 4) * 32) + i;
         const half8 x0 = w0[(size_t)s * 64], x1 = w1[(size_t)s * 64];
-        const half8 b0 = P[pidx], b1 = P[pidx + 16];
+        const half8 b0 = J.P[pidx], b1 = J.P[pidx + 16];
         a00 = hvx::mfma16x16x32(x0, b0, a00);
         a01 = hvx::mfma16x16x32(x0, b1, a01);
         a10 = hvx::mfma16x16x32(x1, b0, a10);
@@ -451,8 +521,9 @@ __global__ __launch_bounds__(512) void near_gemm_kernel(const half8* __restrict_
             ++r;
         }
     }
-    const uint32_t X0 = (tile % te) * 16, Y0 = (tile / te) * 16;
+    const uint32_t So = J.So, X0 = (tile % J.te) * 16, Y0 = (tile / J.te) * 16;
     const size_t So2 = (size_t)So * So;
+    float* out = reinterpret_cast<float*>(chain_out + J.out_off);
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const uint32_t Y = Y0 + rt0 + h;
@@ -464,26 +535,29 @@ __global__ __launch_bounds__(512) void near_gemm_kernel(const half8* __restrict_
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const uint32_t X = X0 + 4 * g + q, o = Y * So + X;
-                out[((size_t)f * So2 + o) * 4 + c] = c < 3 ? a[q] * rowscale[o] * sc[6 + f] : 1.0f;
+                out[((size_t)f * So2 + o) * 4 + c] = c < 3 ? a[q] * J.rowscale[o] * sc[6 + f] : 1.0f;
             }
         }
     }
 }
 
 // out(o) += far field interpolated at o / Z(o)
-__global__ __launch_bounds__(256) void far_add_kernel(const float4* __restrict__ far, uint32_t Sf, uint32_t So,
-                                                      const float* __restrict__ invZ, float4* __restrict__ out) {
+__global__ __launch_bounds__(256) void far_add_kernel(NearTable T, float4* __restrict__ chain_out) {
     const uint32_t id = blockIdx.x * 256 + threadIdx.x;
-    const uint32_t So2 = So * So;
-    if (id >= 6 * So2) return;
-    const uint32_t f = id / So2, o = id - f * So2, Y = o / So, X = o - Y * So;
-    const f3 F = sample_cube(far, (int)Sf, texel_dir((int)f, (int)X, (int)Y, (int)So));
-    const float z = invZ[o];
-    float4 v = out[id];
+    if (id >= T.texels) return;
+    uint32_t jj = 0;
+    while (jj + 1 < T.n && id >= T.j[jj + 1].texel_begin) ++jj;
+    const NearJob& J = T.j[jj];
+    const uint32_t So = J.So, So2 = So * So, local = id - J.texel_begin;
+    const uint32_t f = local / So2, o = local - f * So2, Y = o / So, X = o - Y * So;
+    const f3 F = sample_cube(J.far, (int)J.Sf, texel_dir((int)f, (int)X, (int)Y, (int)So));
+    const float z = J.invZ[o];
+    float4* out = chain_out + J.out_off;
+    float4 v = out[local];
     v.x += F.x * z;
     v.y += F.y * z;
     v.z += F.z * z;
-    out[id] = v;
+    out[local] = v;
 }
 
 // Roughness 0: the bilinear seamless sample of level 0 at the texel direction.
@@ -648,10 +722,14 @@ struct hvx_contract_plan {
     std::vector<DenseMat> dm;
     std::vector<StageDense> sd;
     std::vector<StagePad> sp;
-    uint32_t* fmax = nullptr;
+    float* fmax = nullptr;  // per-face partial maxima [6][kFaceMaxBlocks]
     float* sc = nullptr;
-    float* part = nullptr;
+    float* part = nullptr;   // every dense job's K-slice partial sums
     uint32_t min_level = 0;  // deepest input chain level read
+    uint32_t delta_mip = ~0u;
+    StageTable stage{};
+    DenseTable dense{};
+    NearTable near{};
     std::vector<void*> allocs;
 
     template <class T> hipError_t alloc(T** p, size_t count) {
@@ -808,7 +886,7 @@ int hvx_ibl_contract_plan_create(hvx_ctx* ctx, int type, uint32_t face, uint32_t
         D.stage = stage_dense_index(P, D.G.Sin);
         if ((e = build_dense(P, D, ctx->num_cus, M.zfar_map)) != hipSuccess) return fail_destroy(e, "dense weights");
         M.weight_bytes += (uint64_t)D.G.rt * D.G.ks * 64 * 16;
-        part_floats = std::max(part_floats, (size_t)D.splitk * D.G.rt * 16 * 32);
+        part_floats += (size_t)D.splitk * D.G.rt * 16 * 32;
         P->dm.push_back(D);
         M.dense = (int)P->dm.size() - 1;
         if (M.L.kind == hvx::CONTRACT_SPLIT) {
@@ -841,9 +919,58 @@ int hvx_ibl_contract_plan_create(hvx_ctx* ctx, int type, uint32_t face, uint32_t
         if ((e = P->alloc(&s.P, (size_t)s.Pw * (s.Pw / 8) * 32)) != hipSuccess) return fail_destroy(e, "plane staging");
         P->min_level = std::max(P->min_level, s.level);
     }
-    if ((e = P->alloc(&P->fmax, 8)) != hipSuccess || (e = P->alloc(&P->sc, 16)) != hipSuccess ||
+    if ((e = P->alloc(&P->fmax, 6 * kFaceMaxBlocks)) != hipSuccess || (e = P->alloc(&P->sc, 16)) != hipSuccess ||
         (e = P->alloc(&P->part, part_floats)) != hipSuccess)
         return fail_destroy(e, "scratch");
+    // launch tables of a filter call: every staged level, dense job and near job
+    StageTable& ST = P->stage;
+    for (const StageDense& sd : P->sd) {
+        StageItem& I = ST.it[ST.n++];
+        I = StageItem{0, sd.Sin, 0, 0, hvx::cube_level_offset_floats(face, sd.level) / 4, sd.perm, sd.R, ST.total};
+        ST.total += 6 * sd.Sin * sd.Sin / 8 * 32;
+    }
+    for (const StagePad& sp : P->sp) {
+        StageItem& I = ST.it[ST.n++];
+        I = StageItem{1, sp.Si, sp.B, sp.Pw, hvx::cube_level_offset_floats(face, sp.level) / 4, nullptr, sp.P,
+                      ST.total};
+        ST.total += sp.Pw * (sp.Pw / 8) * 32;
+    }
+    DenseTable& DT = P->dense;
+    NearTable& NT = P->near;
+    float* part = P->part;
+    for (uint32_t m = 0; m < out_mips; ++m) {
+        const MipPlan& M = P->mp[m];
+        if (M.L.kind == hvx::CONTRACT_DELTA) {
+            P->delta_mip = m;
+            continue;
+        }
+        const DenseMat& D = P->dm[M.dense];
+        DenseJob& J = DT.j[DT.n++];
+        J.W = D.W;
+        J.R = P->sd[D.stage].R;
+        J.rt = D.G.rt;
+        J.ks = D.G.ks;
+        J.splitk = D.splitk;
+        J.rows = D.G.rows;
+        J.part = part;
+        part += (size_t)D.splitk * D.G.rt * 16 * 32;
+        J.rowscale = D.rowscale;
+        J.out = M.L.kind == hvx::CONTRACT_FULL ? nullptr : M.far_map;
+        J.out_off = hvx::cube_level_offset_floats(out_face, m) / 4;
+        J.alpha = M.L.kind == hvx::CONTRACT_FULL ? 1.0f : 0.0f;
+        J.wave_begin = DT.waves;
+        J.epi_begin = DT.epi;
+        DT.waves += (D.G.rt + 1) / 2 * D.splitk;
+        DT.epi += 6 * D.G.rows;
+        if (M.L.kind == hvx::CONTRACT_SPLIT) {
+            const StagePad& sp = P->sp[M.pad];
+            NearJob& N = NT.j[NT.n++];
+            N = NearJob{M.Wn, M.d_tiles, sp.P, sp.Pw / 8, sp.B, M.NG.te, M.L.So, M.L.Sf, M.rowscale_n, M.invZ,
+                        M.far_map, hvx::cube_level_offset_floats(out_face, m) / 4, NT.tiles, NT.texels};
+            NT.tiles += (uint32_t)M.tiles.size();
+            NT.texels += 6 * M.L.So * M.L.So;
+        }
+    }
     if ((e = hipDeviceSynchronize()) != hipSuccess) return fail_destroy(e, "plan set-up");
     *out = P;
     return HVX_OK;
@@ -887,45 +1014,27 @@ int hvx_ibl_filter_contract(hvx_contract_plan* P, const float* d_chain, uint32_t
     hipStream_t s = (hipStream_t)stream;
     const float4* chain = reinterpret_cast<const float4*>(d_chain);
     float4* outc = reinterpret_cast<float4*>(d_out);
-    auto level = [&](uint32_t l) { return chain + hvx::cube_level_offset_floats(P->S, l) / 4; };
     // exponent scales from level 0 (every staged level is a box mean of it)
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, P->fmax, 6u, 0u);
     const uint32_t S2 = P->S * P->S;
-    hipLaunchKernelGGL(facemax_kernel, dim3(std::min(64u, (S2 + 255) / 256), 6), dim3(256), 0, s, level(0), P->S,
-                       P->fmax);
-    hipLaunchKernelGGL(scales_kernel, dim3(1), dim3(64), 0, s, P->fmax, P->sc);
-    for (const StageDense& sd : P->sd) {
-        const uint32_t n = 6 * sd.Sin * sd.Sin / 8 * 32;
-        hipLaunchKernelGGL(stage_dense_kernel, dim3((n + 255) / 256), dim3(256), 0, s, level(sd.level), sd.Sin, sd.perm,
-                           P->sc, sd.R);
+    const uint32_t nb = std::min(kFaceMaxBlocks, (S2 + 255) / 256);
+    hipLaunchKernelGGL(facemax_kernel, dim3(nb, 6), dim3(256), 0, s, chain, P->S, P->fmax);
+    hipLaunchKernelGGL(scales_kernel, dim3(1), dim3(64), 0, s, P->fmax, nb, P->sc);
+    if (P->stage.total)
+        hipLaunchKernelGGL(stage_all_kernel, dim3((P->stage.total + 255) / 256), dim3(256), 0, s, chain, P->stage,
+                           P->sc);
+    if (P->dense.n) {
+        hipLaunchKernelGGL(dense_gemm_kernel, dim3((P->dense.waves + 3) / 4), dim3(256), 0, s, P->dense);
+        hipLaunchKernelGGL(dense_epilogue_kernel, dim3((P->dense.epi + 255) / 256), dim3(256), 0, s, P->dense, P->sc,
+                           outc);
     }
-    for (const StagePad& sp : P->sp) {
-        const uint32_t n = sp.Pw * (sp.Pw / 8) * 32;
-        hipLaunchKernelGGL(stage_pad_kernel, dim3((n + 255) / 256), dim3(256), 0, s, level(sp.level), sp.Si, sp.B,
-                           sp.Pw, P->sc, sp.P);
+    if (P->near.n) {
+        hipLaunchKernelGGL(near_gemm_kernel, dim3(P->near.tiles), dim3(512), 0, s, P->near, P->sc, outc);
+        hipLaunchKernelGGL(far_add_kernel, dim3((P->near.texels + 255) / 256), dim3(256), 0, s, P->near, outc);
     }
-    for (uint32_t m = 0; m < P->mips; ++m) {
-        const MipPlan& M = P->mp[m];
-        float4* dst = outc + hvx::cube_level_offset_floats(P->So, m) / 4;
-        const uint32_t So2 = M.L.So * M.L.So;
-        if (M.L.kind == hvx::CONTRACT_DELTA) {
-            hipLaunchKernelGGL(delta_kernel, dim3((6 * So2 + 255) / 256), dim3(256), 0, s, level(0), P->S, M.L.So, dst);
-            continue;
-        }
-        const DenseMat& D = P->dm[M.dense];
-        const uint32_t waves = (D.G.rt + 1) / 2 * D.splitk;
-        hipLaunchKernelGGL(dense_gemm_kernel, dim3((waves + 3) / 4), dim3(256), 0, s, D.W, P->sd[D.stage].R, D.G.rt,
-                           D.G.ks, D.splitk, P->part);
-        float4* dd = M.L.kind == hvx::CONTRACT_FULL ? dst : M.far_map;
-        hipLaunchKernelGGL(dense_epilogue_kernel, dim3((D.G.rows * 6 + 255) / 256), dim3(256), 0, s, P->part, D.splitk,
-                           D.G.rows, D.G.rt * 16, D.rowscale, P->sc, M.L.kind == hvx::CONTRACT_FULL ? 1.0f : 0.0f, dd);
-        if (M.L.kind == hvx::CONTRACT_SPLIT) {
-            const StagePad& sp = P->sp[M.pad];
-            hipLaunchKernelGGL(near_gemm_kernel, dim3((unsigned)M.tiles.size()), dim3(512), 0, s, M.Wn, M.d_tiles, sp.P,
-                               sp.Pw / 8, sp.B, M.NG.te, M.L.So, M.rowscale_n, P->sc, reinterpret_cast<float*>(dst));
-            hipLaunchKernelGGL(far_add_kernel, dim3((6 * So2 + 255) / 256), dim3(256), 0, s, M.far_map, M.L.Sf, M.L.So,
-                               M.invZ, dst);
-        }
+    if (P->delta_mip != ~0u) {
+        const uint32_t So = P->mp[P->delta_mip].L.So;
+        hipLaunchKernelGGL(delta_kernel, dim3((6 * So * So + 255) / 256), dim3(256), 0, s, chain, P->S, So,
+                           outc + hvx::cube_level_offset_floats(P->So, P->delta_mip) / 4);
     }
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? HVX_OK : pfail(ctx, HVX_ERR_HIP, "hvx_ibl_filter_contract: %s", hipGetErrorString(e));

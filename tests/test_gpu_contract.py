@@ -103,3 +103,29 @@ def test_contract_rejects_bad_arguments(ctx):
         plan.filter(chain, need - 1, out)
     assert e.value.status == abi.HVX_ERR_INVALID_ARG
     plan.close()
+
+
+def test_contract_graph_replays_equal_eager(ctx):
+    """The filter allocates nothing and resets nothing with memset, so it can
+    be captured: graph replays (with eager work between them) equal the eager
+    call bit for bit."""
+    import torch
+    S, So, mips = 128, 64, 7
+    chain, levels = _env_chain(ctx, 5, S)
+    plan = abi.ContractPlan(ctx, abi.FILTER_SPECULAR, S, So, mips)
+    ref = torch.empty(abi.cube_chain_floats(So, mips), dtype=torch.float32, device="cuda")
+    plan.filter(chain, levels, ref)
+    torch.cuda.synchronize()
+    out = torch.full_like(ref, np.nan)
+    s = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        plan.filter(chain, levels, out, stream=s)
+    for _ in range(3):
+        out.fill_(np.nan)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out.view(torch.int32), ref.view(torch.int32))
+        torch.zeros(1 << 20, device="cuda").add_(1.0)
+    plan.close()
