@@ -726,6 +726,7 @@ struct hvx_contract_plan {
     float* sc = nullptr;
     float* part = nullptr;   // every dense job's K-slice partial sums
     uint32_t min_level = 0;  // deepest input chain level read
+    uint32_t scale_level = ~0u;  // finest staged level: its maxima bound every staged value
     uint32_t delta_mip = ~0u;
     StageTable stage{};
     DenseTable dense{};
@@ -924,6 +925,8 @@ int hvx_ibl_contract_plan_create(hvx_ctx* ctx, int type, uint32_t face, uint32_t
         return fail_destroy(e, "scratch");
     // launch tables of a filter call: every staged level, dense job and near job
     StageTable& ST = P->stage;
+    for (const StageDense& sd : P->sd) P->scale_level = std::min(P->scale_level, sd.level);
+    for (const StagePad& sp : P->sp) P->scale_level = std::min(P->scale_level, sp.level);
     for (const StageDense& sd : P->sd) {
         StageItem& I = ST.it[ST.n++];
         I = StageItem{0, sd.Sin, 0, 0, hvx::cube_level_offset_floats(face, sd.level) / 4, sd.perm, sd.R, ST.total};
@@ -1014,11 +1017,15 @@ int hvx_ibl_filter_contract(hvx_contract_plan* P, const float* d_chain, uint32_t
     hipStream_t s = (hipStream_t)stream;
     const float4* chain = reinterpret_cast<const float4*>(d_chain);
     float4* outc = reinterpret_cast<float4*>(d_out);
-    // exponent scales from level 0 (every staged level is a box mean of it)
-    const uint32_t S2 = P->S * P->S;
-    const uint32_t nb = std::min(kFaceMaxBlocks, (S2 + 255) / 256);
-    hipLaunchKernelGGL(facemax_kernel, dim3(nb, 6), dim3(256), 0, s, chain, P->S, P->fmax);
-    hipLaunchKernelGGL(scales_kernel, dim3(1), dim3(64), 0, s, P->fmax, nb, P->sc);
+    // exponent scales from the finest staged level: coarser levels are its box
+    // means and padded texels its bilinear samples, so its maxima bound them
+    if (P->stage.total) {
+        const uint32_t Sl = std::max(P->S >> P->scale_level, 1u), S2 = Sl * Sl;
+        const uint32_t nb = std::min(kFaceMaxBlocks, (S2 + 255) / 256);
+        hipLaunchKernelGGL(facemax_kernel, dim3(nb, 6), dim3(256), 0, s,
+                           chain + hvx::cube_level_offset_floats(P->S, P->scale_level) / 4, Sl, P->fmax);
+        hipLaunchKernelGGL(scales_kernel, dim3(1), dim3(64), 0, s, P->fmax, nb, P->sc);
+    }
     if (P->stage.total)
         hipLaunchKernelGGL(stage_all_kernel, dim3((P->stage.total + 255) / 256), dim3(256), 0, s, chain, P->stage,
                            P->sc);
