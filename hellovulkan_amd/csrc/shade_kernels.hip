@@ -42,6 +42,9 @@
 #ifndef HVX_SHADE_WAVES_PER_EU
 #define HVX_SHADE_WAVES_PER_EU 4
 #endif
+#ifndef HVX_SHADE_PREFETCH  // lights in flight ahead of the one being shaded (1 or 2)
+#define HVX_SHADE_PREFETCH 1
+#endif
 
 namespace hvx {
 namespace {
@@ -233,13 +236,31 @@ __device__ __forceinline__ bool shade_one(const ShadeArgs& a, uint32_t idx, uint
             An = a.lpos[li];
             Bn = a.lcol[li];
         }
+#if HVX_SHADE_PREFETCH >= 2
+        float4 An2 = An, Bn2 = Bn;
+        if (count > 1) {
+            const uint32_t li = a.indices[off + 1];
+            An2 = a.lpos[li];
+            Bn2 = a.lcol[li];
+        }
+#endif
         for (uint32_t k = 0; k < count; ++k) {
             const float4 A = An, B = Bn;
+#if HVX_SHADE_PREFETCH >= 2
+            An = An2;
+            Bn = Bn2;
+            if (k + 2 < count) {
+                const uint32_t li = a.indices[off + k + 2];
+                An2 = a.lpos[li];
+                Bn2 = a.lcol[li];
+            }
+#else
             if (k + 1 < count) {
                 const uint32_t li = a.indices[off + k + 1];
                 An = a.lpos[li];
                 Bn = a.lcol[li];
             }
+#endif
             // exact: L, H and the cosines (max with 0 only differs in the sign
             // of a zero, which no later term can observe)
             const f3 d = {A.x - wp.x, A.y - wp.y, A.z - wp.z};
