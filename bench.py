@@ -260,10 +260,12 @@ def contract_bench(ctx, stream, d_cube, S, So, mips, Sd, plain_spec_ms, plain_di
     from hellovulkan_amd import abi
     levels = abi.mip_count(S)
     out = {}
-    for name, ftype, so, m, plain in [("specular", abi.FILTER_SPECULAR, So, mips, plain_spec_ms),
-                                      ("diffuse", abi.FILTER_DIFFUSE, Sd, 1, plain_diff_ms)]:
+    for name, ftype, so, m, plain, prec in [
+            ("specular", abi.FILTER_SPECULAR, So, mips, plain_spec_ms, abi.PRECISION_FP16_SCALED),
+            ("specular_split_bf16", abi.FILTER_SPECULAR, So, mips, plain_spec_ms, abi.PRECISION_BF16_SPLIT),
+            ("diffuse", abi.FILTER_DIFFUSE, Sd, 1, plain_diff_ms, abi.PRECISION_FP16_SCALED)]:
         t0 = time.time()
-        plan = abi.ContractPlan(ctx, ftype, S, so, m)
+        plan = abi.ContractPlan(ctx, ftype, S, so, m, prec)
         torch.cuda.synchronize()
         build_s = time.time() - t0
         dst = torch.empty(abi.cube_chain_floats(so, m), dtype=torch.float32, device=d_cube.device)

@@ -36,6 +36,7 @@ namespace {
 using hvx::ContractLevel;
 using hvx::f32x4;
 using hvx::f3;
+using hvx::bf16x8;
 using hvx::half8;
 using hvx::normalize3;
 using hvx::sample_cube;
@@ -215,9 +216,13 @@ __global__ __launch_bounds__(256) void near_row_kernel(NearGeom G, const NearTil
     block_sum_max(s, m, sh_s, sh_m, &Z[o], &wmax[o]);
 }
 
-// grid: tiles x 16 row tiles; each block writes one row tile's k-steps
+// grid: tiles x 16 row tiles; each block writes one row tile's k-steps.
+// SPLIT: the scaled weight v is stored as BF16 hi = bf16(v) at W and
+// lo = bf16(v - hi) at W + lo_off (hi + lo carries ~16 significant bits).
+template <bool SPLIT>
 __global__ __launch_bounds__(256) void near_fill_kernel(NearGeom G, const NearTile* __restrict__ tiles,
-                                                        const double* __restrict__ wmax, half8* __restrict__ W) {
+                                                        const double* __restrict__ wmax, half8* __restrict__ W,
+                                                        uint64_t lo_off) {
     const uint32_t tile = blockIdx.x / 16, rt = blockIdx.x % 16;
     const NearTile T = tiles[tile];
     const uint32_t KS = T.h * (T.w / 32);
@@ -229,11 +234,24 @@ __global__ __launch_bounds__(256) void near_fill_kernel(NearGeom G, const NearTi
         local_dir(hvx::contract_coord((int)X, G.So), hvx::contract_coord((int)Y, G.So), n);
         const double sc = ldexp(1.0, row_exponent(wmax[o]));
         const uint32_t k0 = ks * 32 + 8 * (lane >> 4), r = k0 / T.w, c0 = k0 - r * T.w;
-        half8 out;
+        const uint64_t at = T.woff + ((uint64_t)rt * KS + ks) * 64 + lane;
+        if constexpr (SPLIT) {
+            bf16x8 hi, lo;
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
-            out[q] = (_Float16)(float)(near_w(G, n, T.x0 + (int)(c0 + q), T.y0 + (int)r) * sc);
-        W[T.woff + ((uint64_t)rt * KS + ks) * 64 + lane] = out;
+            for (int q = 0; q < 8; ++q) {
+                const double v = near_w(G, n, T.x0 + (int)(c0 + q), T.y0 + (int)r) * sc;
+                hi[q] = (__bf16)(float)v;
+                lo[q] = (__bf16)(float)(v - (double)(float)hi[q]);
+            }
+            reinterpret_cast<bf16x8*>(W)[at] = hi;
+            reinterpret_cast<bf16x8*>(W)[at + lo_off] = lo;
+        } else {
+            half8 out;
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                out[q] = (_Float16)(float)(near_w(G, n, T.x0 + (int)(c0 + q), T.y0 + (int)r) * sc);
+            W[at] = out;
+        }
     }
 }
 
@@ -313,12 +331,12 @@ __global__ __launch_bounds__(64) void scales_kernel(const float* __restrict__ pa
 }
 
 struct StageItem {
-    int kind;  // 0: a dense level (permuted), 1: padded face planes
+    int kind;  // 0: a dense level (permuted), 1: padded face planes, 2: padded planes as BF16 hi + lo
     uint32_t S, B, Pw;
     size_t level_off;  // float4 offset of the level in the input chain
     const int32_t* perm;
     half8* dst;
-    uint32_t begin;  // first thread of the item
+    uint32_t begin;  // first thread of the item (kind 2: lo parts at dst + the item's thread count)
 };
 struct StageTable {
     StageItem it[kMaxJobs];
@@ -349,12 +367,13 @@ __device__ __forceinline__ void stage_dense_body(const float4* __restrict__ leve
 // P[py][px/8][col][8] over the padded plane (Pw x Pw, texel index -B .. Si+B)
 // of each face: inside the face the texel itself, outside the bilinear
 // seamless sample of the level at the plane point's direction.
+template <bool SPLIT>
 __device__ __forceinline__ void stage_pad_body(const float4* __restrict__ level, uint32_t Si, uint32_t B, uint32_t Pw,
                                                const float* __restrict__ sc, half8* __restrict__ P, uint32_t id) {
     const uint32_t Pw8 = Pw / 8;
     const uint32_t col = id & 31, cell = id >> 5, px8 = cell % Pw8, py = cell / Pw8;
     const uint32_t f = col >> 2, c = col & 3;
-    half8 out;
+    float vals[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     if (col < 24 && c < 3) {
         const float s = sc[f];
         const int y = (int)py - (int)B;
@@ -372,13 +391,25 @@ __device__ __forceinline__ void stage_pad_body(const float4* __restrict__ level,
                 const f3 t = sample_cube(level, (int)Si, d);
                 val = c == 0 ? t.x : (c == 1 ? t.y : t.z);
             }
-            out[q] = (_Float16)(val * s);
+            vals[q] = val * s;
         }
-    } else {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) out[q] = (_Float16)0.0f;
     }
-    P[id] = out;
+    if constexpr (SPLIT) {
+        bf16x8 hi, lo;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            hi[q] = (__bf16)vals[q];
+            lo[q] = (__bf16)(vals[q] - (float)hi[q]);
+        }
+        const size_t n = (size_t)Pw * Pw8 * 32;
+        reinterpret_cast<bf16x8*>(P)[id] = hi;
+        reinterpret_cast<bf16x8*>(P)[n + id] = lo;
+    } else {
+        half8 out;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) out[q] = (_Float16)vals[q];
+        P[id] = out;
+    }
 }
 
 __global__ __launch_bounds__(256) void stage_all_kernel(const float4* __restrict__ chain, StageTable T,
@@ -390,7 +421,8 @@ __global__ __launch_bounds__(256) void stage_all_kernel(const float4* __restrict
     const StageItem& I = T.it[j];
     const float4* level = chain + I.level_off;
     if (I.kind == 0) stage_dense_body(level, I.S, I.perm, sc[12], I.dst, id - I.begin);
-    else stage_pad_body(level, I.S, I.B, I.Pw, sc, I.dst, id - I.begin);
+    else if (I.kind == 1) stage_pad_body<false>(level, I.S, I.B, I.Pw, sc, I.dst, id - I.begin);
+    else stage_pad_body<true>(level, I.S, I.B, I.Pw, sc, I.dst, id - I.begin);
 }
 
 // ----------------------------------------------------------- MFMA GEMMs
@@ -482,6 +514,7 @@ struct NearJob {
     const float4* far;
     size_t out_off;  // float4 offset of the mip in the output chain
     uint32_t tile_begin, texel_begin;
+    uint64_t w_lo, p_lo;  // split BF16: offsets (16-byte units) of the lo parts of W and P
 };
 struct NearTable {
     NearJob j[kMaxJobs];
@@ -493,7 +526,11 @@ static_assert(sizeof(StageTable) < 4000 && sizeof(DenseTable) < 4000 && sizeof(N
               "launch tables must fit the kernel argument space");
 
 // Near field: one workgroup per 16 x 16 output tile, eight waves of two
-// output rows each; B rows come from the face planes' window.
+// output rows each; B rows come from the face planes' window. SPLIT:
+// weights and radiance are BF16 hi + lo pairs and each product is
+// hi*hi + hi*lo + lo*hi (the lo*lo term is below the FP32 accumulation's
+// resolution), three MFMAs per FP16 one.
+template <bool SPLIT>
 __global__ __launch_bounds__(512) void near_gemm_kernel(NearTable T, const float* __restrict__ sc,
                                                         float4* __restrict__ chain_out) {
     uint32_t jj = 0;
@@ -510,12 +547,32 @@ __global__ __launch_bounds__(512) void near_gemm_kernel(NearTable T, const float
     uint32_t r = 0, cc = 0;
     for (uint32_t s = 0; s < KS; ++s) {
         const size_t pidx = (((size_t)(Tl.y0 + (int)J.B + (int)r) * J.Pw8 + xb + cc * 4) * 32) + i;
-        const half8 x0 = w0[(size_t)s * 64], x1 = w1[(size_t)s * 64];
-        const half8 b0 = J.P[pidx], b1 = J.P[pidx + 16];
-        a00 = hvx::mfma16x16x32(x0, b0, a00);
-        a01 = hvx::mfma16x16x32(x0, b1, a01);
-        a10 = hvx::mfma16x16x32(x1, b0, a10);
-        a11 = hvx::mfma16x16x32(x1, b1, a11);
+        if constexpr (SPLIT) {
+            const bf16x8* wv = reinterpret_cast<const bf16x8*>(w0);
+            const bf16x8* pv = reinterpret_cast<const bf16x8*>(J.P);
+            const size_t o0 = (size_t)s * 64, o1 = o0 + (size_t)KS * 64;
+            const bf16x8 x0 = wv[o0], x1 = wv[o1], y0 = wv[o0 + J.w_lo], y1 = wv[o1 + J.w_lo];
+            const bf16x8 b0 = pv[pidx], b1 = pv[pidx + 16], c0 = pv[pidx + J.p_lo], c1 = pv[pidx + 16 + J.p_lo];
+            a00 = hvx::mfma16x16x32(x0, b0, a00);
+            a00 = hvx::mfma16x16x32(x0, c0, a00);
+            a00 = hvx::mfma16x16x32(y0, b0, a00);
+            a01 = hvx::mfma16x16x32(x0, b1, a01);
+            a01 = hvx::mfma16x16x32(x0, c1, a01);
+            a01 = hvx::mfma16x16x32(y0, b1, a01);
+            a10 = hvx::mfma16x16x32(x1, b0, a10);
+            a10 = hvx::mfma16x16x32(x1, c0, a10);
+            a10 = hvx::mfma16x16x32(y1, b0, a10);
+            a11 = hvx::mfma16x16x32(x1, b1, a11);
+            a11 = hvx::mfma16x16x32(x1, c1, a11);
+            a11 = hvx::mfma16x16x32(y1, b1, a11);
+        } else {
+            const half8 x0 = w0[(size_t)s * 64], x1 = w1[(size_t)s * 64];
+            const half8 b0 = J.P[pidx], b1 = J.P[pidx + 16];
+            a00 = hvx::mfma16x16x32(x0, b0, a00);
+            a01 = hvx::mfma16x16x32(x0, b1, a01);
+            a10 = hvx::mfma16x16x32(x1, b0, a10);
+            a11 = hvx::mfma16x16x32(x1, b1, a11);
+        }
         if (++cc == wc) {
             cc = 0;
             ++r;
@@ -710,6 +767,7 @@ struct MipPlan {
     float4* far_map = nullptr;
     float4* zfar_map = nullptr;
     uint64_t weight_bytes = 0;
+    uint64_t w_lo = 0;  // split BF16: offset of the lo weights in Wn (16-byte units)
 };
 
 }  // namespace
@@ -791,11 +849,12 @@ hipError_t build_dense(hvx_contract_plan* P, DenseMat& D, int num_cus, float4* z
     return e;
 }
 
-hipError_t build_near(hvx_contract_plan* P, MipPlan& M) {
+hipError_t build_near(hvx_contract_plan* P, MipPlan& M, bool split) {
     hipError_t e;
     const uint32_t So = M.L.So, rows = So * So;
     uint64_t halfs8 = 0;
     for (const NearTile& T : M.tiles) halfs8 += (uint64_t)16 * T.h * (T.w / 32) * 64;
+    M.w_lo = split ? halfs8 : 0;
     double *Z = nullptr, *wm = nullptr;
     if ((e = hipMalloc(&Z, rows * sizeof(double))) != hipSuccess) return e;
     if ((e = hipMalloc(&wm, rows * sizeof(double))) != hipSuccess) {
@@ -805,16 +864,20 @@ hipError_t build_near(hvx_contract_plan* P, MipPlan& M) {
     if ((e = P->alloc(&M.d_tiles, M.tiles.size())) == hipSuccess &&
         (e = hipMemcpy(M.d_tiles, M.tiles.data(), M.tiles.size() * sizeof(NearTile), hipMemcpyHostToDevice)) ==
             hipSuccess &&
-        (e = P->alloc(&M.Wn, halfs8)) == hipSuccess && (e = P->alloc(&M.rowscale_n, rows)) == hipSuccess &&
-        (e = P->alloc(&M.invZ, rows)) == hipSuccess) {
+        (e = P->alloc(&M.Wn, split ? 2 * halfs8 : halfs8)) == hipSuccess &&
+        (e = P->alloc(&M.rowscale_n, rows)) == hipSuccess && (e = P->alloc(&M.invZ, rows)) == hipSuccess) {
         hipLaunchKernelGGL(near_row_kernel, dim3(rows), dim3(256), 0, nullptr, M.NG, M.d_tiles, Z, wm);
-        hipLaunchKernelGGL(near_fill_kernel, dim3((unsigned)M.tiles.size() * 16), dim3(256), 0, nullptr, M.NG,
-                           M.d_tiles, wm, M.Wn);
+        if (split)
+            hipLaunchKernelGGL(near_fill_kernel<true>, dim3((unsigned)M.tiles.size() * 16), dim3(256), 0, nullptr,
+                               M.NG, M.d_tiles, wm, M.Wn, halfs8);
+        else
+            hipLaunchKernelGGL(near_fill_kernel<false>, dim3((unsigned)M.tiles.size() * 16), dim3(256), 0, nullptr,
+                               M.NG, M.d_tiles, wm, M.Wn, (uint64_t)0);
         hipLaunchKernelGGL(near_scale_kernel, dim3((rows + 255) / 256), dim3(256), 0, nullptr, M.NG, M.L.Sf, Z, wm,
                            M.zfar_map, M.rowscale_n, M.invZ);
         e = hipDeviceSynchronize();
     }
-    M.weight_bytes += halfs8 * 16;
+    M.weight_bytes += halfs8 * 16 * (split ? 2 : 1);
     (void)hipFree(Z);
     (void)hipFree(wm);
     return e;
@@ -830,8 +893,9 @@ int hvx_ibl_contract_plan_create(hvx_ctx* ctx, int type, uint32_t face, uint32_t
     *out = nullptr;
     if (type != HVX_FILTER_DIFFUSE && type != HVX_FILTER_SPECULAR)
         return pfail(ctx, HVX_ERR_INVALID_ARG, "unknown filter type %d", type);
-    if (precision != HVX_PRECISION_FP16_SCALED)
-        return pfail(ctx, HVX_ERR_UNSUPPORTED, "precision %d is not implemented", precision);
+    if (precision != HVX_PRECISION_FP16_SCALED && precision != HVX_PRECISION_BF16_SPLIT)
+        return pfail(ctx, HVX_ERR_INVALID_ARG, "unknown precision %d", precision);
+    const bool split = precision == HVX_PRECISION_BF16_SPLIT;
     if (type == HVX_FILTER_DIFFUSE && out_mips != 1)
         return pfail(ctx, HVX_ERR_INVALID_ARG, "the diffuse filter writes one mip");
     if (face < 4 || face > 8192 || (face & (face - 1)) || out_face < 1 || out_face > 4096 ||
@@ -901,7 +965,7 @@ int hvx_ibl_contract_plan_create(hvx_ctx* ctx, int type, uint32_t face, uint32_t
             sp.Pw = M.L.Si + 2 * B;
             P->sp.push_back(sp);
             M.pad = (int)P->sp.size() - 1;
-            if ((e = build_near(P, M)) != hipSuccess) return fail_destroy(e, "near weights");
+            if ((e = build_near(P, M, split)) != hipSuccess) return fail_destroy(e, "near weights");
         }
     }
     // staged inputs and shared scratch
@@ -917,7 +981,8 @@ int hvx_ibl_contract_plan_create(hvx_ctx* ctx, int type, uint32_t face, uint32_t
         P->min_level = std::max(P->min_level, s.level);
     }
     for (StagePad& s : P->sp) {
-        if ((e = P->alloc(&s.P, (size_t)s.Pw * (s.Pw / 8) * 32)) != hipSuccess) return fail_destroy(e, "plane staging");
+        if ((e = P->alloc(&s.P, (size_t)s.Pw * (s.Pw / 8) * 32 * (split ? 2 : 1))) != hipSuccess)
+            return fail_destroy(e, "plane staging");
         P->min_level = std::max(P->min_level, s.level);
     }
     if ((e = P->alloc(&P->fmax, 6 * kFaceMaxBlocks)) != hipSuccess || (e = P->alloc(&P->sc, 16)) != hipSuccess ||
@@ -934,8 +999,8 @@ int hvx_ibl_contract_plan_create(hvx_ctx* ctx, int type, uint32_t face, uint32_t
     }
     for (const StagePad& sp : P->sp) {
         StageItem& I = ST.it[ST.n++];
-        I = StageItem{1, sp.Si, sp.B, sp.Pw, hvx::cube_level_offset_floats(face, sp.level) / 4, nullptr, sp.P,
-                      ST.total};
+        I = StageItem{split ? 2 : 1, sp.Si, sp.B, sp.Pw, hvx::cube_level_offset_floats(face, sp.level) / 4, nullptr,
+                      sp.P, ST.total};
         ST.total += sp.Pw * (sp.Pw / 8) * 32;
     }
     DenseTable& DT = P->dense;
@@ -969,7 +1034,8 @@ int hvx_ibl_contract_plan_create(hvx_ctx* ctx, int type, uint32_t face, uint32_t
             const StagePad& sp = P->sp[M.pad];
             NearJob& N = NT.j[NT.n++];
             N = NearJob{M.Wn, M.d_tiles, sp.P, sp.Pw / 8, sp.B, M.NG.te, M.L.So, M.L.Sf, M.rowscale_n, M.invZ,
-                        M.far_map, hvx::cube_level_offset_floats(out_face, m) / 4, NT.tiles, NT.texels};
+                        M.far_map, hvx::cube_level_offset_floats(out_face, m) / 4, NT.tiles, NT.texels, M.w_lo,
+                        split ? (uint64_t)sp.Pw * (sp.Pw / 8) * 32 : 0};
             NT.tiles += (uint32_t)M.tiles.size();
             NT.texels += 6 * M.L.So * M.L.So;
         }
@@ -1035,7 +1101,10 @@ int hvx_ibl_filter_contract(hvx_contract_plan* P, const float* d_chain, uint32_t
                            outc);
     }
     if (P->near.n) {
-        hipLaunchKernelGGL(near_gemm_kernel, dim3(P->near.tiles), dim3(512), 0, s, P->near, P->sc, outc);
+        if (P->precision == HVX_PRECISION_BF16_SPLIT)
+            hipLaunchKernelGGL(near_gemm_kernel<true>, dim3(P->near.tiles), dim3(512), 0, s, P->near, P->sc, outc);
+        else
+            hipLaunchKernelGGL(near_gemm_kernel<false>, dim3(P->near.tiles), dim3(512), 0, s, P->near, P->sc, outc);
         hipLaunchKernelGGL(far_add_kernel, dim3((P->near.texels + 255) / 256), dim3(256), 0, s, P->near, outc);
     }
     if (P->delta_mip != ~0u) {

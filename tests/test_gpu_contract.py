@@ -25,10 +25,10 @@ def _env_chain(ctx, seed, S):
     return chain, levels
 
 
-def _run(ctx, ftype, S, So, mips, sub, seed=13):
+def _run(ctx, ftype, S, So, mips, sub, seed=13, precision=abi.PRECISION_FP16_SCALED, per_mip=None):
     import torch
     chain, levels = _env_chain(ctx, seed, S)
-    plan = abi.ContractPlan(ctx, ftype, S, So, mips)
+    plan = abi.ContractPlan(ctx, ftype, S, So, mips, precision)
     out = torch.full((abi.cube_chain_floats(So, mips),), np.nan, dtype=torch.float32, device="cuda")
     plan.filter(chain, levels, out)
     torch.cuda.synchronize()
@@ -50,8 +50,11 @@ def _run(ctx, ftype, S, So, mips, sub, seed=13):
             ids = np.unique(np.concatenate([[0, ids.size - 1], rng.integers(0, ids.size, sub)])).astype(np.uint32)
         ref = O.contract(ftype, lvl0, S, So, mips, m, ids)
         err = O.rel_err(got[ids, :3], ref)
-        print(f"type {ftype} {S}->{So} mip {m} kind {info.kind} Si {info.in_face} Sf {info.far_face} "
-              f"weights {info.weight_bytes / 1e6:.1f} MB: max rel err vs FP64 {err:.3e} ({ids.size} texels)")
+        print(f"type {ftype} {S}->{So} precision {precision} mip {m} kind {info.kind} Si {info.in_face} "
+              f"Sf {info.far_face} weights {info.weight_bytes / 1e6:.1f} MB: max rel err vs FP64 {err:.3e} "
+              f"({ids.size} texels)")
+        if per_mip is not None:
+            per_mip.append((info.kind, err))
         worst = max(worst, err)
     plan.close()
     return worst
@@ -60,6 +63,17 @@ def _run(ctx, ftype, S, So, mips, sub, seed=13):
 @pytest.mark.parametrize("S,So,mips,sub", [(128, 64, 7, 3000), (64, 32, 6, None)])
 def test_contract_specular_small(ctx, S, So, mips, sub):
     assert _run(ctx, abi.FILTER_SPECULAR, S, So, mips, sub) <= TOL
+
+
+@pytest.mark.parametrize("S,So,mips,sub", [(128, 64, 7, 3000), (1024, 512, 10, 40)])
+def test_contract_split_bf16(ctx, S, So, mips, sub):
+    """Split-BF16 near fields: within the bound everywhere, and the
+    low-roughness (near/far) mips an order of magnitude closer to FP64 than
+    with FP16 (their far fields stay FP16)."""
+    per = []
+    assert _run(ctx, abi.FILTER_SPECULAR, S, So, mips, sub, precision=abi.PRECISION_BF16_SPLIT, per_mip=per) <= TOL
+    split_errs = [e for k, e in per if k == abi.CONTRACT_SPLIT]
+    assert split_errs and max(split_errs) <= 1e-4
 
 
 def test_contract_diffuse_small(ctx):
@@ -94,6 +108,9 @@ def test_contract_rejects_bad_arguments(ctx):
     assert e.value.status == abi.HVX_ERR_INVALID_ARG
     with pytest.raises(abi.HvxError) as e:
         abi.ContractPlan(ctx, abi.FILTER_DIFFUSE, 64, 16, 2)
+    assert e.value.status == abi.HVX_ERR_INVALID_ARG
+    with pytest.raises(abi.HvxError) as e:
+        abi.ContractPlan(ctx, abi.FILTER_SPECULAR, 64, 32, 6, 7)
     assert e.value.status == abi.HVX_ERR_INVALID_ARG
     plan = abi.ContractPlan(ctx, abi.FILTER_SPECULAR, 64, 32, 6)
     need = plan.info(0).min_input_level + 1
