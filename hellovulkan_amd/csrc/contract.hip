@@ -162,26 +162,39 @@ __global__ __launch_bounds__(256) void dense_row_kernel(DenseGeom G, double* __r
     block_sum_max(s, m, sh_s, sh_m, &Z[row], &wmax[row]);
 }
 
+// SPLIT: BF16 hi at W, lo = bf16(v - hi) at W + lo_off (16-byte units)
+template <bool SPLIT>
 __global__ __launch_bounds__(256) void dense_fill_kernel(DenseGeom G, const double* __restrict__ wmax,
-                                                         half8* __restrict__ W) {
+                                                         half8* __restrict__ W, uint64_t lo_off) {
     const uint64_t id = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (id >= (uint64_t)G.rt * G.ks * 64) return;
     const uint32_t lane = (uint32_t)(id & 63);
     const uint64_t t = id >> 6;
     const uint32_t ks = (uint32_t)(t % G.ks), rt = (uint32_t)(t / G.ks);
     const uint32_t row = rt * 16 + (lane & 15), k0 = ks * 32 + 8 * (lane >> 4);
-    half8 out;
+    double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (row < G.rows) {
         double n[3];
         dense_row_dir(G, row, n);
         const double sc = ldexp(1.0, row_exponent(wmax[row]));
 #pragma unroll
-        for (int q = 0; q < 8; ++q) out[q] = (_Float16)(float)(dense_w(G, n, k0 + q) * sc);
-    } else {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) out[q] = (_Float16)0.0f;
+        for (int q = 0; q < 8; ++q) v[q] = dense_w(G, n, k0 + q) * sc;
     }
-    W[id] = out;
+    if constexpr (SPLIT) {
+        bf16x8 hi, lo;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            hi[q] = (__bf16)(float)v[q];
+            lo[q] = (__bf16)(float)(v[q] - (double)(float)hi[q]);
+        }
+        reinterpret_cast<bf16x8*>(W)[id] = hi;
+        reinterpret_cast<bf16x8*>(W)[id + lo_off] = lo;
+    } else {
+        half8 out;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) out[q] = (_Float16)(float)v[q];
+        W[id] = out;
+    }
 }
 
 // rowscale = 2^-s / Z (normalised rows) or 2^-s (raw far rows; then the
@@ -331,7 +344,7 @@ __global__ __launch_bounds__(64) void scales_kernel(const float* __restrict__ pa
 }
 
 struct StageItem {
-    int kind;  // 0: a dense level (permuted), 1: padded face planes, 2: padded planes as BF16 hi + lo
+    int kind;  // 0: a dense level (permuted), 1: padded face planes; 3, 2: the same as BF16 hi + lo
     uint32_t S, B, Pw;
     size_t level_off;  // float4 offset of the level in the input chain
     const int32_t* perm;
@@ -345,23 +358,35 @@ struct StageTable {
 
 // R[k/8][col][8]: col = 4 f + c holds channel c (< 3) of the level texel
 // that output face f sees as canonical input k; other columns are zero.
+template <bool SPLIT>
 __device__ __forceinline__ void stage_dense_body(const float4* __restrict__ level, uint32_t Sin,
                                                  const int32_t* __restrict__ perm, float s, half8* __restrict__ R,
                                                  uint32_t id) {
     const uint32_t K = 6 * Sin * Sin;
     const uint32_t col = id & 31, k8 = id >> 5, f = col >> 2, c = col & 3;
-    half8 out;
+    float vals[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     if (col < 24 && c < 3) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const float4 v = level[perm[(size_t)f * K + k8 * 8 + q]];
-            out[q] = (_Float16)((c == 0 ? v.x : (c == 1 ? v.y : v.z)) * s);
+            vals[q] = (c == 0 ? v.x : (c == 1 ? v.y : v.z)) * s;
         }
-    } else {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) out[q] = (_Float16)0.0f;
     }
-    R[id] = out;
+    if constexpr (SPLIT) {
+        bf16x8 hi, lo;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            hi[q] = (__bf16)vals[q];
+            lo[q] = (__bf16)(vals[q] - (float)hi[q]);
+        }
+        reinterpret_cast<bf16x8*>(R)[id] = hi;
+        reinterpret_cast<bf16x8*>(R)[(size_t)K / 8 * 32 + id] = lo;
+    } else {
+        half8 out;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) out[q] = (_Float16)vals[q];
+        R[id] = out;
+    }
 }
 
 // P[py][px/8][col][8] over the padded plane (Pw x Pw, texel index -B .. Si+B)
@@ -420,7 +445,8 @@ __global__ __launch_bounds__(256) void stage_all_kernel(const float4* __restrict
     while (j + 1 < T.n && id >= T.it[j + 1].begin) ++j;
     const StageItem& I = T.it[j];
     const float4* level = chain + I.level_off;
-    if (I.kind == 0) stage_dense_body(level, I.S, I.perm, sc[12], I.dst, id - I.begin);
+    if (I.kind == 0) stage_dense_body<false>(level, I.S, I.perm, sc[12], I.dst, id - I.begin);
+    else if (I.kind == 3) stage_dense_body<true>(level, I.S, I.perm, sc[12], I.dst, id - I.begin);
     else if (I.kind == 1) stage_pad_body<false>(level, I.S, I.B, I.Pw, sc, I.dst, id - I.begin);
     else stage_pad_body<true>(level, I.S, I.B, I.Pw, sc, I.dst, id - I.begin);
 }
@@ -436,6 +462,7 @@ struct DenseJob {
     size_t out_off;
     float alpha;
     uint32_t wave_begin, epi_begin;
+    uint64_t w_lo, r_lo;  // split BF16 (w_lo != 0): offsets (16-byte units) of the lo parts of W and R
 };
 struct DenseTable {
     DenseJob j[kMaxJobs];
@@ -460,13 +487,36 @@ __global__ __launch_bounds__(256) void dense_gemm_kernel(DenseTable T) {
     const half8* w1 = has1 ? w0 + (size_t)ks * 64 : w0;
     const half8* rb = J.R + g * 32 + i;
     f32x4 a00 = {0.0f, 0.0f, 0.0f, 0.0f}, a01 = a00, a10 = a00, a11 = a00;
-    for (uint32_t s = kb; s < ke; ++s) {
-        const half8 x0 = w0[(size_t)s * 64], x1 = w1[(size_t)s * 64];
-        const half8 b0 = rb[(size_t)s * 128], b1 = rb[(size_t)s * 128 + 16];
-        a00 = hvx::mfma16x16x32(x0, b0, a00);
-        a01 = hvx::mfma16x16x32(x0, b1, a01);
-        a10 = hvx::mfma16x16x32(x1, b0, a10);
-        a11 = hvx::mfma16x16x32(x1, b1, a11);
+    if (J.w_lo) {  // split BF16: hi*hi + hi*lo + lo*hi
+        const bf16x8* v0 = reinterpret_cast<const bf16x8*>(w0);
+        const bf16x8* v1 = reinterpret_cast<const bf16x8*>(w1);
+        const bf16x8* rv = reinterpret_cast<const bf16x8*>(rb);
+        for (uint32_t s = kb; s < ke; ++s) {
+            const size_t o = (size_t)s * 64, q = (size_t)s * 128;
+            const bf16x8 x0 = v0[o], x1 = v1[o], y0 = v0[o + J.w_lo], y1 = v1[o + J.w_lo];
+            const bf16x8 b0 = rv[q], b1 = rv[q + 16], c0 = rv[q + J.r_lo], c1 = rv[q + 16 + J.r_lo];
+            a00 = hvx::mfma16x16x32(x0, b0, a00);
+            a00 = hvx::mfma16x16x32(x0, c0, a00);
+            a00 = hvx::mfma16x16x32(y0, b0, a00);
+            a01 = hvx::mfma16x16x32(x0, b1, a01);
+            a01 = hvx::mfma16x16x32(x0, c1, a01);
+            a01 = hvx::mfma16x16x32(y0, b1, a01);
+            a10 = hvx::mfma16x16x32(x1, b0, a10);
+            a10 = hvx::mfma16x16x32(x1, c0, a10);
+            a10 = hvx::mfma16x16x32(y1, b0, a10);
+            a11 = hvx::mfma16x16x32(x1, b1, a11);
+            a11 = hvx::mfma16x16x32(x1, c1, a11);
+            a11 = hvx::mfma16x16x32(y1, b1, a11);
+        }
+    } else {
+        for (uint32_t s = kb; s < ke; ++s) {
+            const half8 x0 = w0[(size_t)s * 64], x1 = w1[(size_t)s * 64];
+            const half8 b0 = rb[(size_t)s * 128], b1 = rb[(size_t)s * 128 + 16];
+            a00 = hvx::mfma16x16x32(x0, b0, a00);
+            a01 = hvx::mfma16x16x32(x0, b1, a01);
+            a10 = hvx::mfma16x16x32(x1, b0, a10);
+            a11 = hvx::mfma16x16x32(x1, b1, a11);
+        }
     }
     float* p = J.part + (size_t)slice * rt * 16 * 32;
 #pragma unroll
@@ -738,6 +788,7 @@ std::vector<NearTile> near_tiles(uint32_t So, uint32_t Si, double t1, uint32_t* 
 struct DenseMat {
     DenseGeom G;
     int stage;  // index into the staged dense levels
+    bool split = false;  // split-BF16 weights (hi, then lo after rt * ks * 64 units)
     uint32_t splitk;
     half8* W = nullptr;
     float* rowscale = nullptr;
@@ -745,6 +796,7 @@ struct DenseMat {
 
 struct StageDense {
     uint32_t Sin, level;
+    bool split = false;  // BF16 hi + lo (lo after K / 8 * 32 units)
     int32_t* perm = nullptr;
     half8* R = nullptr;
 };
@@ -813,11 +865,12 @@ int pfail(hvx_ctx* ctx, int st, const char* fmt, ...) {
     return st;
 }
 
-int stage_dense_index(hvx_contract_plan* P, uint32_t Sin) {
+int stage_dense_index(hvx_contract_plan* P, uint32_t Sin, bool split) {
     for (size_t i = 0; i < P->sd.size(); ++i)
-        if (P->sd[i].Sin == Sin) return (int)i;
+        if (P->sd[i].Sin == Sin && P->sd[i].split == split) return (int)i;
     StageDense s;
     s.Sin = Sin;
+    s.split = split;
     s.level = ilog2(P->S / Sin);
     P->sd.push_back(s);
     return (int)P->sd.size() - 1;
@@ -833,9 +886,14 @@ hipError_t build_dense(hvx_contract_plan* P, DenseMat& D, int num_cus, float4* z
         return e;
     }
     const uint64_t n = (uint64_t)G.rt * G.ks * 64;
-    if ((e = P->alloc(&D.W, n)) == hipSuccess && (e = P->alloc(&D.rowscale, G.rows)) == hipSuccess) {
+    if ((e = P->alloc(&D.W, D.split ? 2 * n : n)) == hipSuccess && (e = P->alloc(&D.rowscale, G.rows)) == hipSuccess) {
         hipLaunchKernelGGL(dense_row_kernel, dim3(G.rows), dim3(256), 0, nullptr, G, Z, wm);
-        hipLaunchKernelGGL(dense_fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, G, wm, D.W);
+        if (D.split)
+            hipLaunchKernelGGL(dense_fill_kernel<true>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, G,
+                               wm, D.W, n);
+        else
+            hipLaunchKernelGGL(dense_fill_kernel<false>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, G,
+                               wm, D.W, (uint64_t)0);
         hipLaunchKernelGGL(dense_scale_kernel, dim3((G.rows + 255) / 256), dim3(256), 0, nullptr, G.rows, Z, wm,
                            G.piece == PIECE_FAR ? 1 : 0, D.rowscale, zmap);
         e = hipDeviceSynchronize();
@@ -948,9 +1006,10 @@ int hvx_ibl_contract_plan_create(hvx_ctx* ctx, int type, uint32_t face, uint32_t
         D.G.rows = D.G.So * D.G.So;
         D.G.rt = (D.G.rows + 15) / 16;
         D.G.ks = 6 * D.G.Sin * D.G.Sin / 32;
-        D.stage = stage_dense_index(P, D.G.Sin);
+        D.split = split && M.L.kind == hvx::CONTRACT_SPLIT;  // low-roughness mips: near and far
+        D.stage = stage_dense_index(P, D.G.Sin, D.split);
         if ((e = build_dense(P, D, ctx->num_cus, M.zfar_map)) != hipSuccess) return fail_destroy(e, "dense weights");
-        M.weight_bytes += (uint64_t)D.G.rt * D.G.ks * 64 * 16;
+        M.weight_bytes += (uint64_t)D.G.rt * D.G.ks * 64 * 16 * (D.split ? 2 : 1);
         part_floats += (size_t)D.splitk * D.G.rt * 16 * 32;
         P->dm.push_back(D);
         M.dense = (int)P->dm.size() - 1;
@@ -976,7 +1035,7 @@ int hvx_ibl_contract_plan_create(hvx_ctx* ctx, int type, uint32_t face, uint32_t
             for (uint32_t k = 0; k < K; ++k) perm[(size_t)f * K + k] = perm_index(f, k, s.Sin);
         if ((e = P->alloc(&s.perm, perm.size())) != hipSuccess ||
             (e = hipMemcpy(s.perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
-            (e = P->alloc(&s.R, (size_t)K / 8 * 32)) != hipSuccess)
+            (e = P->alloc(&s.R, (size_t)K / 8 * 32 * (s.split ? 2 : 1))) != hipSuccess)
             return fail_destroy(e, "dense staging");
         P->min_level = std::max(P->min_level, s.level);
     }
@@ -994,7 +1053,8 @@ int hvx_ibl_contract_plan_create(hvx_ctx* ctx, int type, uint32_t face, uint32_t
     for (const StagePad& sp : P->sp) P->scale_level = std::min(P->scale_level, sp.level);
     for (const StageDense& sd : P->sd) {
         StageItem& I = ST.it[ST.n++];
-        I = StageItem{0, sd.Sin, 0, 0, hvx::cube_level_offset_floats(face, sd.level) / 4, sd.perm, sd.R, ST.total};
+        I = StageItem{sd.split ? 3 : 0, sd.Sin, 0, 0, hvx::cube_level_offset_floats(face, sd.level) / 4, sd.perm,
+                      sd.R, ST.total};
         ST.total += 6 * sd.Sin * sd.Sin / 8 * 32;
     }
     for (const StagePad& sp : P->sp) {
@@ -1028,6 +1088,8 @@ int hvx_ibl_contract_plan_create(hvx_ctx* ctx, int type, uint32_t face, uint32_t
         J.alpha = M.L.kind == hvx::CONTRACT_FULL ? 1.0f : 0.0f;
         J.wave_begin = DT.waves;
         J.epi_begin = DT.epi;
+        J.w_lo = D.split ? (uint64_t)D.G.rt * D.G.ks * 64 : 0;
+        J.r_lo = D.split ? (uint64_t)6 * D.G.Sin * D.G.Sin / 8 * 32 : 0;
         DT.waves += (D.G.rt + 1) / 2 * D.splitk;
         DT.epi += 6 * D.G.rows;
         if (M.L.kind == hvx::CONTRACT_SPLIT) {

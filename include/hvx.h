@@ -257,8 +257,9 @@ int hvx_ibl_filter(hvx_ctx* ctx, int type, const float* d_cube,
 /* Arithmetic of the contraction filter's MFMA products. */
 typedef enum hvx_precision {
     HVX_PRECISION_FP16_SCALED = 0, /* FP16 weights and radiance, power-of-two scaled per row / per face */
-    HVX_PRECISION_BF16_SPLIT = 1   /* as FP16_SCALED, except the low-roughness mips' near fields: split-BF16
-                                      compensated products (hi*hi + hi*lo + lo*hi), twice the weights */
+    HVX_PRECISION_BF16_SPLIT = 1   /* as FP16_SCALED, except the low-roughness (kind 2) mips, near and far
+                                      fields: split-BF16 compensated products (hi*hi + hi*lo + lo*hi), twice
+                                      the weights of those mips */
 } hvx_precision;
 
 typedef struct hvx_contract_plan hvx_contract_plan;

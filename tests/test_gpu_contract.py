@@ -72,8 +72,11 @@ def test_contract_split_bf16(ctx, S, So, mips, sub):
     with FP16 (their far fields stay FP16)."""
     per = []
     assert _run(ctx, abi.FILTER_SPECULAR, S, So, mips, sub, precision=abi.PRECISION_BF16_SPLIT, per_mip=per) <= TOL
+    # near and far fields of the low-roughness mips carry split-BF16 products
+    # (about 16 significant bits): measured 2.7e-6 .. 4.2e-6 against the
+    # 1.5e-4 .. 3.9e-4 of the FP16 mode on the same mips
     split_errs = [e for k, e in per if k == abi.CONTRACT_SPLIT]
-    assert split_errs and max(split_errs) <= 1e-4
+    assert split_errs and max(split_errs) <= 2e-5
 
 
 def test_contract_diffuse_small(ctx):
