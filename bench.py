@@ -49,7 +49,7 @@ def parse():
     p.add_argument("--no-batch", action="store_true", help="skip the batched probes + views workload")
     p.add_argument("--no-contract", action="store_true", help="skip the contraction (MFMA) filter timings")
     p.add_argument("--no-learned", action="store_true", help="skip the learned-radiance (MLP) workload")
-    p.add_argument("--mlp-steps", type=int, default=3000)
+    p.add_argument("--mlp-steps", type=int, default=30000)
     p.add_argument("--batch-probes", type=int, default=256)
     p.add_argument("--calibrate", action="store_true",
                    help="run one 256 MiB elementwise copy first (known HBM traffic, for counter scaling)")
@@ -311,10 +311,35 @@ def learned_bench(ctx, stream, steps, d_spec, SPEC, SPEC_MIPS, frs, cam, ubo, pc
     th = torch.empty(nh * 4, dtype=torch.float32, device=d_spec.device)
     mlp.sample(d_spec, SPEC, SPEC_MIPS, 100, nh, xh, th)
     rgb = torch.empty(nh * 3, dtype=torch.float32, device=d_spec.device)
-    infer_ms = _ev_ms(lambda: mlp.eval(xh, nh, rgb), stream, 5)
+    mlp.eval(xh, nh, rgb)
+    rgb_n = torch.empty(n * 3, dtype=torch.float32, device=d_spec.device)
+    infer_ms = _ev_ms(lambda: mlp.eval(x, n, rgb_n), stream, 5)
+    del rgb_n
     ref = torch.exp(th.view(-1, 4)[:, :3]) - 1e-3
-    rel = ((rgb.view(-1, 3) - ref).abs() / ref.abs().clamp_min(1e-6)).amax(dim=1)
-    relq = torch.quantile(rel, torch.tensor([0.5, 0.9], device=rel.device)).cpu().numpy()
+
+    def rel_q(pred):
+        rel = ((pred - ref).abs() / ref.abs().clamp_min(1e-6)).amax(dim=1)
+        return rel, torch.quantile(rel, torch.tensor([0.5, 0.9, 0.99], device=rel.device)).cpu().numpy()
+
+    rel, relq = rel_q(rgb.view(-1, 3))
+    # the classical representation at comparable storage: mips 0-1 of the
+    # chain box-downsampled, sampled at the same held-out points
+    S0 = 6 * SPEC * SPEC * 4
+    L0 = d_spec[:S0].view(6, SPEC, SPEC, 4)
+    L1 = d_spec[S0:S0 + 6 * (SPEC // 2) ** 2 * 4].view(6, SPEC // 2, SPEC // 2, 4)
+
+    def down(L, s):
+        f = L.shape[1] // s
+        return L.reshape(6, s, f, s, f, 4).mean(dim=(2, 4)).reshape(-1)
+
+    eq_maps = {}
+    for Se in (16, 32, 64):
+        small = torch.cat([down(L0, Se), down(L1, Se // 2)]).contiguous()
+        xs, ts = torch.empty_like(xh), torch.empty_like(th)
+        mlp.sample(small, Se, 2, 100, nh, xs, ts)
+        q = rel_q(torch.exp(ts.view(-1, 4)[:, :3]) - 1e-3)[1]
+        eq_maps[f"{Se}^2"] = {"bytes": small.numel() * 4, "rel_err_median": float(q[0]), "rel_err_p90": float(q[1]),
+                              "rel_err_p99": float(q[2])}
     # measured cell errors, the learned frame and its difference from plain shading
     cf, bins, tol = 128, 4, 2e-3
     cell_err = torch.empty(bins * 6 * cf * cf, dtype=torch.float32, device=d_spec.device)
@@ -337,12 +362,14 @@ def learned_bench(ctx, stream, steps, d_spec, SPEC, SPEC_MIPS, frs, cam, ubo, pc
         "train_samples_per_s": round(steps * batch / train_s, 1),
         "loss_first": float(c[0]), "loss_last": float(c[-50:].mean()),
         "heldout_rel_err_median": float(relq[0]), "heldout_rel_err_p90": float(relq[1]),
+        "heldout_rel_err_p99": float(relq[2]),
+        "downsampled_map_rel_err": eq_maps,
         "heldout_within_2e-3": round(float((rel <= 2e-3).float().mean().item()), 4),
         "cell_err_median": float(cell_err.median().item()),
         "served_fraction": round(served_n / F, 4),
         "frame_max_rel_diff_vs_plain": float(diff.max().item()),
         "frame_fraction_beyond_2e-3": float((diff > tol).float().mean().item()),
-        "infer_Mquery_per_s": round(nh / (infer_ms * 1e-3) / 1e6, 1),
+        "infer_Mquery_per_s": round(n / (infer_ms * 1e-3) / 1e6, 1),
         "shade_learned_ms": round(learned_ms, 4), "shade_plain_ms": round(plain_shade_ms, 4),
         "network_bytes": abi.MLP_PARAMS * 4, "map_bytes": abi.cube_chain_floats(SPEC, SPEC_MIPS) * 4,
     }

@@ -73,7 +73,7 @@ def build_oracle(verbose: bool = False) -> str:
     return ORACLE_LIB
 
 
-TEST_KERNELS_SRC = [os.path.join(ROOT, "tests", "gpu_kernels", f) for f in ("fastmath_check.hip", "mfma_layout.hip")]
+TEST_KERNELS_SRC = [os.path.join(ROOT, "tests", "gpu_kernels", f) for f in ("fastmath_check.hip", "mfma_layout.hip", "mlp_encode.hip")]
 TEST_KERNELS_LIB = os.path.join(ROOT, "tests", "_build", "libhvx_testkernels.so")
 
 

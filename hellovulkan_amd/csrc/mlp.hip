@@ -93,8 +93,8 @@ __global__ __launch_bounds__(256) void mlp_pe_kernel(const float4* __restrict__ 
         const float4 v = in[b];
         x = j == 0 ? v.x : (j == 1 ? v.y : (j == 2 ? v.z : 2.0f * v.w / lod_max - 1.0f));
     }
-    const float a = hvx::mlp_freq<float>((int)k) * x;
-    const float sv = sinf(a), cv = cosf(a);
+    float sv, cv;
+    hvx::mlp_sincos(x, (int)k, sv, cv);
     const uint32_t c = j * 32 + 2 * k;
     X[(size_t)b * kIn + c] = (bf16)sv;
     X[(size_t)b * kIn + c + 1] = (bf16)cv;
@@ -180,11 +180,7 @@ __global__ __launch_bounds__(256) void mlp_out_kernel(const bf16* __restrict__ H
     for (int k = 0; k < kH; ++k) h[k] = (float)H3[(size_t)b * kH + k];
     float y[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        float acc = b4[c];
-        for (int k = 0; k < kH; ++k) acc += h[k] * W4[c * kH + k];
-        y[c] = acc;
-    }
+    for (int c = 0; c < 3; ++c) y[c] = hvx::mlp_output(h, W4 + c * kH, b4[c]);
     if (!TRAIN) {
         if (b < n)
             for (int c = 0; c < 3; ++c) Y[(size_t)b * 3 + c] = expf(y[c]) - (float)hvx::kMlpEps;
@@ -279,11 +275,10 @@ __global__ __launch_bounds__(256) void mlp_cast_kernel(const float* __restrict__
 __global__ __launch_bounds__(256) void mlp_infer_kernel(MlpView v, const float4* __restrict__ in, uint32_t n,
                                                         float* __restrict__ rgb) {
     __shared__ MlpLds L;
-    __shared__ __bf16 tiles[4][16 * kH];
     mlp_stage(L, v);
     __syncthreads();
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63, i = lane & 15, g = lane >> 4;
-    for (uint32_t base = blockIdx.x * 64; base < n; base += gridDim.x * 64) {  // uniform per workgroup
+    for (uint32_t base = blockIdx.x * 64; base < n; base += gridDim.x * 64) {
         const uint32_t sidx = base + wave * 16 + i;
         float x[4] = {0.0f, 0.0f, 1.0f, 0.0f};
         if (sidx < n) {
@@ -293,7 +288,7 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(MlpView v, const float4*
             x[2] = q.z;
             x[3] = q.w;
         }
-        const float y = mlp_forward_tile(L, tiles[wave], x, v.lod_max);
+        const float y = mlp_forward_tile(L, x, v.lod_max);
         if (g < 3 && sidx < n) rgb[(size_t)sidx * 3 + g] = y;
     }
 }
@@ -641,13 +636,19 @@ int hvx_mlp_train(hvx_mlp* M, const float* d_inputs, const float* d_targets, uin
     return e == hipSuccess ? HVX_OK : mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_train: %s", hipGetErrorString(e));
 }
 
+// Workgroups of mlp_infer_kernel: each stages the weights (34 KB) once and
+// loops over 64-sample steps, so a few per CU suffice.
+static uint32_t infer_blocks(const hvx_mlp* M, uint32_t n) {
+    const uint32_t cap = 4u * (uint32_t)std::max(M->ctx->num_cus, 1);
+    return std::max(1u, std::min(cap, (n + 63) / 64));
+}
+
 int hvx_mlp_eval(hvx_mlp* M, const float* d_inputs, uint32_t n, float* d_rgb, void* stream) {
     if (!M || !d_inputs || !d_rgb) return HVX_ERR_NULL;
     if (n == 0) return HVX_OK;
     int st;
     if ((st = bind(M->ctx))) return st;
-    const uint32_t blocks = std::min(8192u, (n + 63) / 64);
-    hipLaunchKernelGGL(mlp_infer_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, hvx::mlp_view(M),
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3(infer_blocks(M, n)), dim3(256), 0, (hipStream_t)stream, hvx::mlp_view(M),
                        reinterpret_cast<const float4*>(d_inputs), n, d_rgb);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? HVX_OK : mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_eval: %s", hipGetErrorString(e));
@@ -677,7 +678,7 @@ int hvx_mlp_cell_errors(hvx_mlp* M, const float* d_chain, uint32_t face, uint32_
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(mask_points_kernel, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, mask_face, lod_bins,
                        M->lod_max, pts);
-    hipLaunchKernelGGL(mlp_infer_kernel, dim3(std::min(8192u, (unsigned)((npts + 63) / 64))), dim3(256), 0, s,
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3(infer_blocks(M, npts)), dim3(256), 0, s,
                        hvx::mlp_view(M), pts, (uint32_t)npts, rgb);
     hipLaunchKernelGGL(cell_error_kernel, dim3((cells + 255) / 256), dim3(256), 0, s, pts, rgb,
                        reinterpret_cast<const float4*>(d_chain), face, mips, cells, d_err);

@@ -423,20 +423,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_W
     }
     __syncthreads();
     if constexpr (LEARNED) {
-        // the network's prefiltered radiance for every fragment of the chunk:
-        // each wave evaluates 16 fragments per call, in chunk order
+        // Each wave evaluates the network for the 64 fragments it is about to
+        // shade, as four 16-sample tiles; tile T holds the fragments of lanes
+        // 16 T .. 16 T + 15 and its results return to those lanes by shuffles.
         __shared__ MlpLds s_mlp;
-        __shared__ __bf16 s_tile[4][16 * kMlpHidden];
-        __shared__ float s_pre[3][kChunk];
-        __shared__ float s_err[kChunk];
         __shared__ uint32_t s_served;
         mlp_stage(s_mlp, a.mlp);
         if (t == 0) s_served = 0;
         __syncthreads();
-        const uint32_t wave = t >> 6, lane = t & 63, i = lane & 15, g = lane >> 4;
+        const uint32_t lane = t & 63, i = lane & 15;
+        uint32_t mine = 0;
 #pragma unroll 1
-        for (int it = 0; it < kChunk / 64; ++it) {
-            const uint32_t local = it * 64 + wave * 16 + i, idx = base + local;
+        for (int r = 0; r < kChunk / 256; ++r) {
+            const uint32_t local = s_order[r * 256 + t];
+            const uint32_t idx = base + local;
             float x[4] = {0.0f, 0.0f, 1.0f, 0.0f};
             float err = -1.0f;
             if (idx < nfrag) {
@@ -448,21 +448,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_W
                 x[3] = lod;
                 err = mlp_cell_error(a, R, lod);
             }
-            const float y = mlp_forward_tile(s_mlp, s_tile[wave], x, a.mlp.lod_max);
-            if (g < 3) s_pre[g][local] = y;
-            if (g == 0) s_err[local] = err;
-        }
-        __syncthreads();
-        uint32_t mine = 0;
+            f3 learned = {0.0f, 0.0f, 0.0f};
 #pragma unroll 1
-        for (int r = 0; r < kChunk / 256; ++r) {
-            const uint32_t local = s_order[r * 256 + t];
-            const uint32_t idx = base + local;
-            if (idx < nfrag)
-                mine += shade_one<EXACT>(a, idx, s_off[local], s_cnt[local], s_err[local],
-                                         f3{s_pre[0][local], s_pre[1][local], s_pre[2][local]})
-                            ? 1u
-                            : 0u;
+            for (int T = 0; T < 4; ++T) {
+                const int src = 16 * T + (int)i;
+                const float et = __shfl(err, src, 64);
+                // tiles whose cells all exclude the network (error >= 1) skip it
+                if (__ballot(et >= 0.0f && et < 1.0f) == 0) continue;
+                const float xt[4] = {__shfl(x[0], src, 64), __shfl(x[1], src, 64), __shfl(x[2], src, 64),
+                                     __shfl(x[3], src, 64)};
+                const float y = mlp_forward_tile(s_mlp, xt, a.mlp.lod_max);  // lane (c, i): channel c of sample i
+                const float c0 = __shfl(y, (int)i, 64), c1 = __shfl(y, 16 + (int)i, 64),
+                            c2 = __shfl(y, 32 + (int)i, 64);
+                if ((lane >> 4) == (uint32_t)T) learned = f3{c0, c1, c2};
+            }
+            if (idx < nfrag) mine += shade_one<EXACT>(a, idx, s_off[local], s_cnt[local], err, learned) ? 1u : 0u;
         }
         if (a.served) {
             if (mine) atomicAdd(&s_served, mine);

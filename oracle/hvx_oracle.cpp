@@ -837,14 +837,20 @@ double bf16r(double v, bool on) {
 // errors), everything else stays double
 void mlp_forward_one(const float* P, const float in[4], double lod_max, MlpAct& a, bool bf16 = false) {
     const double xin[4] = {in[0], in[1], in[2], 2.0 * in[3] / lod_max - 1.0};
-    // with bf16, the encoding's argument is formed in FP32 as the kernels
-    // form it (frequency and lod coordinate rounded to float)
+    // with bf16, the encoding is evaluated as the kernels evaluate it
     const float xf[4] = {in[0], in[1], in[2], 2.0f * in[3] / (float)lod_max - 1.0f};
     for (int j = 0; j < 4; ++j)
         for (int k = 0; k < hvx::kMlpFreqs; ++k) {
-            const double t = bf16 ? (double)(hvx::mlp_freq<float>(k) * xf[j]) : hvx::mlp_freq<double>(k) * xin[j];
-            a.x[j * 32 + 2 * k] = bf16r(std::sin(t), bf16);
-            a.x[j * 32 + 2 * k + 1] = bf16r(std::cos(t), bf16);
+            if (bf16) {  // the kernels' FP32 encoding, bit for bit
+                float s, c;
+                hvx::mlp_sincos(xf[j], k, s, c);
+                a.x[j * 32 + 2 * k] = bf16r(s, true);
+                a.x[j * 32 + 2 * k + 1] = bf16r(c, true);
+            } else {
+                const double t = hvx::mlp_freq<double>(k) * xin[j];
+                a.x[j * 32 + 2 * k] = std::sin(t);
+                a.x[j * 32 + 2 * k + 1] = std::cos(t);
+            }
         }
     const int wo[3] = {hvx::kMlpW1, hvx::kMlpW2, hvx::kMlpW3}, bo[3] = {hvx::kMlpB1, hvx::kMlpB2, hvx::kMlpB3};
     const double* prev = a.x;
@@ -1117,10 +1123,18 @@ void hvxo_tonemap_f64(const float* rgb, uint32_t n, int op, int gamma, double* o
 }
 
 // ---- radiance network: FP64 forward (log-radiance y) and gradient
-void hvxo_mlp_forward_f64(const float* P, const float* in /* n x 4 */, uint32_t n, double lod_max, double* y) {
+// The kernels' FP32 encoding (hvx_mlp.h) for n inputs: sc = n x 16 x (sin, cos).
+void hvxo_mlp_sincos(const float* x, uint32_t n, float* sc) {
+    for (uint32_t j = 0; j < n; ++j)
+        for (int k = 0; k < hvx::kMlpFreqs; ++k) hvx::mlp_sincos(x[j], k, sc[(j * 16 + k) * 2], sc[(j * 16 + k) * 2 + 1]);
+}
+
+// bf16: the kernels' BF16 operands and FP32 encoding (the rest in double)
+void hvxo_mlp_forward_f64(const float* P, const float* in /* n x 4 */, uint32_t n, double lod_max, int bf16,
+                          double* y) {
     for (uint32_t s = 0; s < n; ++s) {
         MlpAct a;
-        mlp_forward_one(P, in + 4 * (size_t)s, lod_max, a);
+        mlp_forward_one(P, in + 4 * (size_t)s, lod_max, a, bf16 != 0);
         for (int c = 0; c < 3; ++c) y[3 * (size_t)s + c] = a.y[c];
     }
 }

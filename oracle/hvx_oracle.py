@@ -48,7 +48,8 @@ _SIGS = [
     ("hvxo_contract_f64", None, [C.c_int, vp, u32, u32, u32, u32, vp, u32, vp, C.c_int]),
     ("hvxo_ideal_f64", None, [C.c_int, vp, u32, C.c_double, u32, vp, u32, vp, C.c_int]),
     ("hvxo_contract_levels", C.c_int, [C.c_int, u32, u32, u32, vp, vp]),
-    ("hvxo_mlp_forward_f64", None, [vp, vp, u32, C.c_double, vp]),
+    ("hvxo_mlp_forward_f64", None, [vp, vp, u32, C.c_double, C.c_int, vp]),
+    ("hvxo_mlp_sincos", None, [vp, u32, vp]),
     ("hvxo_tonemap_f32", None, [vp, u32, C.c_int, C.c_int, vp]),
     ("hvxo_tonemap_f64", None, [vp, u32, C.c_int, C.c_int, vp]),
     ("hvxo_mlp_gradient_f64", None, [vp, vp, vp, u32, C.c_double, C.c_int, vp, vp]),
@@ -278,12 +279,22 @@ def tonemap(rgb_planar: np.ndarray, op: int, gamma: bool = True, dtype=np.float3
     return out
 
 
-def mlp_forward(params: np.ndarray, inputs: np.ndarray, lod_max: float) -> np.ndarray:
-    """FP64 log-radiance (n x 3) of the radiance network at inputs (n x 4: dir, lod)."""
+def mlp_sincos(x: np.ndarray) -> np.ndarray:
+    """The kernels' FP32 encoding of hvx_mlp.h: (n, 16, 2) sin and cos of f_k x."""
+    x = _c(x, np.float32).ravel()
+    out = np.empty((x.size, 16, 2), np.float32)
+    lib().hvxo_mlp_sincos(x.ctypes.data, x.size, out.ctypes.data)
+    return out
+
+
+def mlp_forward(params: np.ndarray, inputs: np.ndarray, lod_max: float, bf16: bool = False) -> np.ndarray:
+    """FP64 log-radiance (n x 3) of the radiance network at inputs (n x 4: dir, lod);
+    bf16=True rounds the kernels' BF16 operands and uses their FP32 encoding."""
     params = _c(params, np.float32)
     inputs = _c(inputs, np.float32).reshape(-1, 4)
     out = np.empty((inputs.shape[0], 3), np.float64)
-    lib().hvxo_mlp_forward_f64(params.ctypes.data, inputs.ctypes.data, inputs.shape[0], lod_max, out.ctypes.data)
+    lib().hvxo_mlp_forward_f64(params.ctypes.data, inputs.ctypes.data, inputs.shape[0], lod_max, 1 if bf16 else 0,
+                               out.ctypes.data)
     return out
 
 

@@ -59,3 +59,23 @@ def test_mfma_16x16x32_lane_layout(gpu, bf16):
     assert lib.hvxt_mfma_layout(bf16, da.data_ptr(), db.data_ptr(), dc.data_ptr()) == 0
     torch.cuda.synchronize()
     assert np.array_equal(dc.cpu().numpy(), (a.astype(np.float64) @ b.astype(np.float64)).astype(np.float32))
+
+
+def test_mlp_encoding_matches_cpu_bits(gpu):
+    """The radiance network's FP32 encoding is the same function on the GPU
+    and in the oracle's BF16 emulation: bit-identical sin/cos for every
+    frequency over a dense grid of [-1, 1] and random floats in it."""
+    import torch
+    from oracle import hvx_oracle as O
+    lib = C.CDLL(LIB)
+    lib.hvxt_mlp_encode.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
+    rng = np.random.default_rng(17)
+    x = np.concatenate([np.linspace(-1, 1, 100001), rng.uniform(-1, 1, 100000), [0.0, -0.0, 1e-30]]).astype(np.float32)
+    dx = torch.from_numpy(x).cuda()
+    dsc = torch.full((x.size * 32,), np.nan, dtype=torch.float32, device="cuda")
+    assert lib.hvxt_mlp_encode(dx.data_ptr(), x.size, dsc.data_ptr()) == 0
+    torch.cuda.synchronize()
+    got = dsc.cpu().numpy().reshape(-1, 16, 2)
+    ref = O.mlp_sincos(x)
+    diff = np.count_nonzero(got.view(np.uint32) != ref.view(np.uint32))
+    assert diff == 0, f"{diff} of {ref.size} values differ"

@@ -62,3 +62,28 @@ def test_encoding_feature_wiring():
         f = np.pi * 2.0 ** (k / 2.0)
         ref = 1.0 + (np.cos if s else np.sin)(f * xin[:, j])
         assert np.allclose(y, ref, rtol=0, atol=1e-9), (j, k, s)
+
+
+def test_fp32_encoding_against_sin_cos():
+    """The kernels' FP32 encoding against sin/cos of f_k x in float64, f_k =
+    pi 2^(k/2): within 3e-7 absolute for even k (exact argument; k = 2, 3
+    mod 4 by one doubling) and 4e-5 for odd k (the rounded argument, up to 90.5
+    turns at k = 15) over [-1, 1], exact at x = 0 and at the even-k zeros."""
+    x = np.concatenate([np.linspace(-1, 1, 40001), [-1.0, 1.0, 0.5, -0.25, 1e-30, -1e-30]]).astype(np.float32)
+    sc = O.mlp_sincos(x)
+    k = np.arange(16)
+    f = np.pi * 2.0 ** (k // 2) * np.where(k % 2 == 1, np.sqrt(2.0), 1.0)
+    t = x.astype(np.float64)[:, None] * f[None, :]
+    err = np.maximum(np.abs(sc[:, :, 0] - np.sin(t)), np.abs(sc[:, :, 1] - np.cos(t))).max(axis=0)
+    assert err[0::2].max() <= 3e-7 and err[1::2].max() <= 4e-5, err
+    # |sin|, |cos| <= 1 and sin^2 + cos^2 = 1 to FP32 rounding
+    assert np.abs(sc).max() <= 1.0
+    assert np.abs((sc.astype(np.float64) ** 2).sum(axis=2) - 1.0).max() <= 1e-6
+    z = O.mlp_sincos(np.zeros(1, np.float32))[0]
+    assert np.all(z[:, 0] == 0.0) and np.all(z[:, 1] == 1.0)
+    # x = +-1 at even k >= 2: f_k x is a whole number of turns
+    for xv in (1.0, -1.0):
+        e = O.mlp_sincos(np.array([xv], np.float32))[0]
+        for kk in range(2, 16, 2):
+            assert abs(e[kk, 0]) == 0.0 and e[kk, 1] == 1.0
+        assert abs(e[0, 0]) == 0.0 and e[0, 1] == -1.0  # f_0 = pi
