@@ -55,6 +55,7 @@ struct DenseGeom {
 
 // Near-field window of one 16 x 16 output tile, in (unpadded) texel indices
 // of the face plane at the near level; woff = weight offset in half8 units.
+// The tile's weights are [k-step][row tile (16)][lane (64)][8].
 struct NearTile {
     int32_t x0, y0;
     uint32_t w, h;
@@ -247,7 +248,7 @@ __global__ __launch_bounds__(256) void near_fill_kernel(NearGeom G, const NearTi
         local_dir(hvx::contract_coord((int)X, G.So), hvx::contract_coord((int)Y, G.So), n);
         const double sc = ldexp(1.0, row_exponent(wmax[o]));
         const uint32_t k0 = ks * 32 + 8 * (lane >> 4), r = k0 / T.w, c0 = k0 - r * T.w;
-        const uint64_t at = T.woff + ((uint64_t)rt * KS + ks) * 64 + lane;
+        const uint64_t at = T.woff + ((uint64_t)ks * 16 + rt) * 64 + lane;  // K-step major
         if constexpr (SPLIT) {
             bf16x8 hi, lo;
 #pragma unroll
@@ -356,85 +357,61 @@ struct StageTable {
     uint32_t n, total;
 };
 
+// Writes channel values v[3] of element q of a staged 8-element operand at
+// unit `at` (and the next two units) as FP16, or as BF16 hi + lo with the lo
+// parts `lo_off` units further.
+template <bool SPLIT>
+__device__ __forceinline__ void stage_store3(half8* __restrict__ D, size_t at, uint32_t q, const float v[3],
+                                             size_t lo_off) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const size_t e = (at + c) * 8 + q;
+        if constexpr (SPLIT) {
+            const __bf16 hi = (__bf16)v[c];
+            reinterpret_cast<__bf16*>(D)[e] = hi;
+            reinterpret_cast<__bf16*>(D)[lo_off * 8 + e] = (__bf16)(v[c] - (float)hi);
+        } else {
+            reinterpret_cast<_Float16*>(D)[e] = (_Float16)v[c];
+        }
+    }
+}
+
 // R[k/8][col][8]: col = 4 f + c holds channel c (< 3) of the level texel
-// that output face f sees as canonical input k; other columns are zero.
+// that output face f sees as canonical input k; other columns are zero (from
+// plan creation). One thread per (k, f); lanes 8 apart share k / 8.
 template <bool SPLIT>
 __device__ __forceinline__ void stage_dense_body(const float4* __restrict__ level, uint32_t Sin,
                                                  const int32_t* __restrict__ perm, float s, half8* __restrict__ R,
                                                  uint32_t id) {
     const uint32_t K = 6 * Sin * Sin;
-    const uint32_t col = id & 31, k8 = id >> 5, f = col >> 2, c = col & 3;
-    float vals[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    if (col < 24 && c < 3) {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const float4 v = level[perm[(size_t)f * K + k8 * 8 + q]];
-            vals[q] = (c == 0 ? v.x : (c == 1 ? v.y : v.z)) * s;
-        }
-    }
-    if constexpr (SPLIT) {
-        bf16x8 hi, lo;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            hi[q] = (__bf16)vals[q];
-            lo[q] = (__bf16)(vals[q] - (float)hi[q]);
-        }
-        reinterpret_cast<bf16x8*>(R)[id] = hi;
-        reinterpret_cast<bf16x8*>(R)[(size_t)K / 8 * 32 + id] = lo;
-    } else {
-        half8 out;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) out[q] = (_Float16)vals[q];
-        R[id] = out;
-    }
+    const uint32_t q = id & 7, rest = id >> 3, f = rest % 6, k8 = rest / 6;
+    const float4 t = level[perm[(size_t)f * K + k8 * 8 + q]];
+    const float v[3] = {t.x * s, t.y * s, t.z * s};
+    stage_store3<SPLIT>(R, (size_t)k8 * 32 + 4 * f, q, v, (size_t)K / 8 * 32);
 }
 
 // P[py][px/8][col][8] over the padded plane (Pw x Pw, texel index -B .. Si+B)
 // of each face: inside the face the texel itself, outside the bilinear
-// seamless sample of the level at the plane point's direction.
+// seamless sample of the level at the plane point's direction. One thread
+// per (texel, face); columns 4 f + 3 and 24..31 are zero from plan creation.
 template <bool SPLIT>
 __device__ __forceinline__ void stage_pad_body(const float4* __restrict__ level, uint32_t Si, uint32_t B, uint32_t Pw,
                                                const float* __restrict__ sc, half8* __restrict__ P, uint32_t id) {
     const uint32_t Pw8 = Pw / 8;
-    const uint32_t col = id & 31, cell = id >> 5, px8 = cell % Pw8, py = cell / Pw8;
-    const uint32_t f = col >> 2, c = col & 3;
-    float vals[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    if (col < 24 && c < 3) {
-        const float s = sc[f];
-        const int y = (int)py - (int)B;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int x = (int)(px8 * 8 + q) - (int)B;
-            float val;
-            if (x >= 0 && y >= 0 && x < (int)Si && y < (int)Si) {
-                const float4 v = level[((size_t)f * Si + y) * Si + x];
-                val = c == 0 ? v.x : (c == 1 ? v.y : v.z);
-            } else {
-                double p[3];
-                hvx::contract_face_point((int)f, hvx::contract_coord(x, Si), hvx::contract_coord(y, Si), p);
-                const f3 d = normalize3(f3{(float)p[0], (float)p[1], (float)p[2]});
-                const f3 t = sample_cube(level, (int)Si, d);
-                val = c == 0 ? t.x : (c == 1 ? t.y : t.z);
-            }
-            vals[q] = val * s;
-        }
-    }
-    if constexpr (SPLIT) {
-        bf16x8 hi, lo;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            hi[q] = (__bf16)vals[q];
-            lo[q] = (__bf16)(vals[q] - (float)hi[q]);
-        }
-        const size_t n = (size_t)Pw * Pw8 * 32;
-        reinterpret_cast<bf16x8*>(P)[id] = hi;
-        reinterpret_cast<bf16x8*>(P)[n + id] = lo;
+    const uint32_t q = id & 7, rest = id >> 3, f = rest % 6, cell = rest / 6, px8 = cell % Pw8, py = cell / Pw8;
+    const int x = (int)(px8 * 8 + q) - (int)B, y = (int)py - (int)B;
+    f3 t;
+    if (x >= 0 && y >= 0 && x < (int)Si && y < (int)Si) {
+        const float4 v = level[((size_t)f * Si + y) * Si + x];
+        t = f3{v.x, v.y, v.z};
     } else {
-        half8 out;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) out[q] = (_Float16)vals[q];
-        P[id] = out;
+        double p[3];
+        hvx::contract_face_point((int)f, hvx::contract_coord(x, Si), hvx::contract_coord(y, Si), p);
+        t = sample_cube(level, (int)Si, normalize3(f3{(float)p[0], (float)p[1], (float)p[2]}));
     }
+    const float sf = sc[f];
+    const float v[3] = {t.x * sf, t.y * sf, t.z * sf};
+    stage_store3<SPLIT>(P, (size_t)cell * 32 + 4 * f, q, v, (size_t)Pw * Pw8 * 32);
 }
 
 __global__ __launch_bounds__(256) void stage_all_kernel(const float4* __restrict__ chain, StageTable T,
@@ -580,18 +557,32 @@ static_assert(sizeof(StageTable) < 4000 && sizeof(DenseTable) < 4000 && sizeof(N
 // weights and radiance are BF16 hi + lo pairs and each product is
 // hi*hi + hi*lo + lo*hi (the lo*lo term is below the FP32 accumulation's
 // resolution), three MFMAs per FP16 one.
+constexpr uint32_t kNearWaves = 8;  // waves per 16x16 output tile (two row tiles each)
+#ifndef HVX_NEAR_PARTS
+#define HVX_NEAR_PARTS 2
+#endif
+constexpr uint32_t kNearParts = HVX_NEAR_PARTS;  // workgroups per tile
+static_assert(kNearWaves % kNearParts == 0, "a tile splits into whole waves");
+
 template <bool SPLIT>
 __global__ __launch_bounds__(512) void near_gemm_kernel(NearTable T, const float* __restrict__ sc,
                                                         float4* __restrict__ chain_out) {
     uint32_t jj = 0;
-    while (jj + 1 < T.n && blockIdx.x >= T.j[jj + 1].tile_begin) ++jj;
+    // workgroup b owns half b & 1 (eight row tiles) of tile b >> 1: finer
+    // work units than whole tiles balance the unequal mips across the CUs
+    const uint32_t gt = blockIdx.x / kNearParts;
+    while (jj + 1 < T.n && gt >= T.j[jj + 1].tile_begin) ++jj;
     const NearJob& J = T.j[jj];
-    const uint32_t tile = blockIdx.x - J.tile_begin, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t tile = gt - J.tile_begin, lane = threadIdx.x & 63;
+    const uint32_t wave = (threadIdx.x >> 6) + (blockIdx.x % kNearParts) * (kNearWaves / kNearParts);
     const NearTile Tl = J.tiles[tile];
     const uint32_t wc = Tl.w / 32, KS = Tl.h * wc;
     const uint32_t i = lane & 15, g = lane >> 4, rt0 = 2 * wave;
-    const half8* w0 = J.W + Tl.woff + (size_t)rt0 * KS * 64 + lane;
-    const half8* w1 = w0 + (size_t)KS * 64;
+    // weights of a tile stored K-step major: step s of all 16 row tiles is one
+    // contiguous 16 KB block
+    constexpr size_t kStep = 16 * 64, kRow = 64;
+    const half8* w0 = J.W + Tl.woff + (size_t)rt0 * kRow + lane;
+    const half8* w1 = w0 + kRow;
     const uint32_t xb = (uint32_t)(Tl.x0 + (int)J.B) / 8 + g;
     f32x4 a00 = {0.0f, 0.0f, 0.0f, 0.0f}, a01 = a00, a10 = a00, a11 = a00;
     uint32_t r = 0, cc = 0;
@@ -600,7 +591,7 @@ __global__ __launch_bounds__(512) void near_gemm_kernel(NearTable T, const float
         if constexpr (SPLIT) {
             const bf16x8* wv = reinterpret_cast<const bf16x8*>(w0);
             const bf16x8* pv = reinterpret_cast<const bf16x8*>(J.P);
-            const size_t o0 = (size_t)s * 64, o1 = o0 + (size_t)KS * 64;
+            const size_t o0 = (size_t)s * kStep, o1 = o0 + kRow;
             const bf16x8 x0 = wv[o0], x1 = wv[o1], y0 = wv[o0 + J.w_lo], y1 = wv[o1 + J.w_lo];
             const bf16x8 b0 = pv[pidx], b1 = pv[pidx + 16], c0 = pv[pidx + J.p_lo], c1 = pv[pidx + 16 + J.p_lo];
             a00 = hvx::mfma16x16x32(x0, b0, a00);
@@ -616,7 +607,7 @@ __global__ __launch_bounds__(512) void near_gemm_kernel(NearTable T, const float
             a11 = hvx::mfma16x16x32(x1, c1, a11);
             a11 = hvx::mfma16x16x32(y1, b1, a11);
         } else {
-            const half8 x0 = w0[(size_t)s * 64], x1 = w1[(size_t)s * 64];
+            const half8 x0 = w0[(size_t)s * kStep], x1 = w1[(size_t)s * kStep];
             const half8 b0 = J.P[pidx], b1 = J.P[pidx + 16];
             a00 = hvx::mfma16x16x32(x0, b0, a00);
             a01 = hvx::mfma16x16x32(x0, b1, a01);
@@ -1035,12 +1026,14 @@ int hvx_ibl_contract_plan_create(hvx_ctx* ctx, int type, uint32_t face, uint32_t
             for (uint32_t k = 0; k < K; ++k) perm[(size_t)f * K + k] = perm_index(f, k, s.Sin);
         if ((e = P->alloc(&s.perm, perm.size())) != hipSuccess ||
             (e = hipMemcpy(s.perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
-            (e = P->alloc(&s.R, (size_t)K / 8 * 32 * (s.split ? 2 : 1))) != hipSuccess)
+            (e = P->alloc(&s.R, (size_t)K / 8 * 32 * (s.split ? 2 : 1))) != hipSuccess ||
+            (e = hipMemset(s.R, 0, (size_t)K / 8 * 32 * (s.split ? 2 : 1) * sizeof(half8))) != hipSuccess)
             return fail_destroy(e, "dense staging");
         P->min_level = std::max(P->min_level, s.level);
     }
     for (StagePad& s : P->sp) {
-        if ((e = P->alloc(&s.P, (size_t)s.Pw * (s.Pw / 8) * 32 * (split ? 2 : 1))) != hipSuccess)
+        const size_t n = (size_t)s.Pw * (s.Pw / 8) * 32 * (split ? 2 : 1);
+        if ((e = P->alloc(&s.P, n)) != hipSuccess || (e = hipMemset(s.P, 0, n * sizeof(half8))) != hipSuccess)
             return fail_destroy(e, "plane staging");
         P->min_level = std::max(P->min_level, s.level);
     }
@@ -1055,13 +1048,13 @@ int hvx_ibl_contract_plan_create(hvx_ctx* ctx, int type, uint32_t face, uint32_t
         StageItem& I = ST.it[ST.n++];
         I = StageItem{sd.split ? 3 : 0, sd.Sin, 0, 0, hvx::cube_level_offset_floats(face, sd.level) / 4, sd.perm,
                       sd.R, ST.total};
-        ST.total += 6 * sd.Sin * sd.Sin / 8 * 32;
+        ST.total += 6 * sd.Sin * sd.Sin * 6;
     }
     for (const StagePad& sp : P->sp) {
         StageItem& I = ST.it[ST.n++];
         I = StageItem{split ? 2 : 1, sp.Si, sp.B, sp.Pw, hvx::cube_level_offset_floats(face, sp.level) / 4, nullptr,
                       sp.P, ST.total};
-        ST.total += sp.Pw * (sp.Pw / 8) * 32;
+        ST.total += sp.Pw * sp.Pw * 6;
     }
     DenseTable& DT = P->dense;
     NearTable& NT = P->near;
@@ -1164,9 +1157,11 @@ int hvx_ibl_filter_contract(hvx_contract_plan* P, const float* d_chain, uint32_t
     }
     if (P->near.n) {
         if (P->precision == HVX_PRECISION_BF16_SPLIT)
-            hipLaunchKernelGGL(near_gemm_kernel<true>, dim3(P->near.tiles), dim3(512), 0, s, P->near, P->sc, outc);
+            hipLaunchKernelGGL(near_gemm_kernel<true>, dim3(P->near.tiles * kNearParts), dim3(64 * kNearWaves / kNearParts), 0,
+                               s, P->near, P->sc, outc);
         else
-            hipLaunchKernelGGL(near_gemm_kernel<false>, dim3(P->near.tiles), dim3(512), 0, s, P->near, P->sc, outc);
+            hipLaunchKernelGGL(near_gemm_kernel<false>, dim3(P->near.tiles * kNearParts), dim3(64 * kNearWaves / kNearParts), 0,
+                               s, P->near, P->sc, outc);
         hipLaunchKernelGGL(far_add_kernel, dim3((P->near.texels + 255) / 256), dim3(256), 0, s, P->near, outc);
     }
     if (P->delta_mip != ~0u) {
