@@ -461,7 +461,14 @@ def batch_bench(ctx, dev, stream, n_probes):
         t = int(ref[2].item())
         same = same and int(bufs[2].item()) == t and torch.equal(bufs[0], ref[0]) and torch.equal(bufs[1][:t], ref[1][:t])
     texels = sum(6 * sum(max(So >> m, 1) ** 2 for m in range(mips)) for _, So, mips in specs)
+    # the same batch with the probes through the FP16 MFMA contraction filter
+    # (one plan per probe size; its error against FP64 is the contraction tests')
+    t0 = time.time()
+    b.set_contract(True)
+    plans_s = time.time() - t0
+    c_eager_ms = ev_time(lambda: b.run(stream), stream, 3)
     b.close()
+    log(f"contraction-mode batch: {c_eager_ms:.2f} ms per eager run (plans built in {plans_s:.1f}s)")
     return {
         "workload": f"{n_probes} probes (faces {{64,128,256,512}} -> GGX prefilter S/2, full chain, 1024 samples) "
                     f"+ {NV} views 3840x2160 32x18x48 (lights log-uniform [1024,65536], cap 128)",
@@ -477,6 +484,8 @@ def batch_bench(ctx, dev, stream, n_probes):
                    "cull_wgs_per_cu": tun.cull_wgs_per_cu, "probe_ms": round(tun.probe_ms, 3),
                    "cull_ms": round(tun.cull_ms, 3), "autotune_s": round(tune_s, 1)},
         "outputs_equal_per_item": bool(same),
+        "contraction_mode": {"eager_ms": round(c_eager_ms, 3),
+                             "speedup_vs_sampled_graph": round(graph_ms / c_eager_ms, 2), "plans_s": round(plans_s, 2)},
     }
 
 

@@ -170,6 +170,7 @@ SIGNATURES = [
     ("hvx_batch_set_tuning", C.c_int, [vp, P(BatchTuning)]),
     ("hvx_batch_get_tuning", C.c_int, [vp, P(BatchTuning)]),
     ("hvx_batch_autotune", C.c_int, [vp, vp]),
+    ("hvx_batch_set_contract", C.c_int, [vp, C.c_int, C.c_int]),
     ("hvx_batch_run", C.c_int, [vp, vp]),
     ("hvx_batch_capture", C.c_int, [vp, vp]),
     ("hvx_batch_launch", C.c_int, [vp, vp]),
@@ -521,6 +522,11 @@ class Batch:
     def autotune(self, stream=None) -> BatchTuning:
         self.ctx._check(self.lib.hvx_batch_autotune(self.h, self.ctx._stream(stream)), "hvx_batch_autotune")
         return self.tuning
+
+    def set_contract(self, enable: bool, precision: int = 0):
+        """Probes through the MFMA contraction filter (plans per size class) or back."""
+        self.ctx._check(self.lib.hvx_batch_set_contract(self.h, 1 if enable else 0, precision),
+                        "hvx_batch_set_contract")
 
     def run(self, stream=None):
         self.ctx._check(self.lib.hvx_batch_run(self.h, self.ctx._stream(stream)), "hvx_batch_run")

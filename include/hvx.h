@@ -409,6 +409,17 @@ int hvx_batch_get_tuning(const hvx_batch* batch, hvx_batch_tuning* tuning);
 /* Times every candidate tile shape on this batch (synchronously, on
  * `stream`; outputs are overwritten) and keeps the fastest. */
 int hvx_batch_autotune(hvx_batch* batch, void* stream);
+/* Contraction mode (enable != 0): the probes are filtered by the MFMA
+ * contraction (hvx_ibl_filter_contract, `precision` an hvx_precision) instead
+ * of the sampled GGX prefilter: one plan, scratch input chain and stream per
+ * distinct (face_size, out_face_size, out_mips); per probe, level 0 is copied
+ * into the chain, its box mips are built and the plan filters it. Outputs then
+ * equal the per-probe contraction filter's bit for bit. Allocates the plans
+ * and scratch here; enable = 0 returns to the sampled prefilter. The mode runs
+ * through hvx_batch_run only: hvx_batch_capture returns HVX_ERR_UNSUPPORTED
+ * while it is enabled. No reference counterpart (it composes two additions of
+ * this library). */
+int hvx_batch_set_contract(hvx_batch* batch, int enable, int precision);
 /* Runs the batch: the probe and culling kernels on forked streams, joined
  * back into `stream` before returning control to it. */
 int hvx_batch_run(hvx_batch* batch, void* stream);
