@@ -154,6 +154,8 @@ SIGNATURES = [
     ("hvx_ibl_contract_plan_destroy", C.c_int, [vp]),
     ("hvx_ibl_contract_plan_query", C.c_int, [vp, u32, P(ContractMipInfo)]),
     ("hvx_ibl_filter_contract", C.c_int, [vp, vp, u32, vp, vp]),
+    ("hvx_ibl_contract_plan_reserve", C.c_int, [vp, u32]),
+    ("hvx_ibl_filter_contract_multi", C.c_int, [vp, u32, vp, u32, vp, vp]),
     ("hvx_mlp_create", C.c_int, [vp, C.c_float, C.c_uint64, P(vp)]),
     ("hvx_mlp_destroy", C.c_int, [vp]),
     ("hvx_mlp_get_params", C.c_int, [vp, vp]),
@@ -421,6 +423,15 @@ class ContractPlan:
         i = ContractMipInfo()
         self.ctx._check(self.lib.hvx_ibl_contract_plan_query(self.h, mip, C.byref(i)), "hvx_ibl_contract_plan_query")
         return i
+
+    def reserve(self, max_probes: int):
+        self.ctx._check(self.lib.hvx_ibl_contract_plan_reserve(self.h, max_probes), "hvx_ibl_contract_plan_reserve")
+
+    def filter_multi(self, n: int, d_chain_ptrs, chain_levels: int, d_out_ptrs, stream=None):
+        """d_chain_ptrs / d_out_ptrs: device int64 tensors of n data pointers."""
+        self.ctx._check(self.lib.hvx_ibl_filter_contract_multi(self.h, n, _ptr(d_chain_ptrs), chain_levels,
+                                                                _ptr(d_out_ptrs), self.ctx._stream(stream)),
+                        "hvx_ibl_filter_contract_multi")
 
     def filter(self, chain, chain_levels: int, out, stream=None):
         self.ctx._check(self.lib.hvx_ibl_filter_contract(self.h, _ptr(chain), chain_levels, _ptr(out),

@@ -46,12 +46,18 @@ struct ViewScratch {
 }  // namespace
 
 // Contraction mode: the probes of one (face, out_face, out_mips) class share
-// a contraction plan, a scratch input chain (level 0 copied in, box mips
-// built) and a stream; classes run concurrently.
+// a contraction plan (with per-probe scratch reserved) and a stream; their
+// input chains sit `stride` floats apart in one block (level 0 gathered in,
+// box mips built), and one multi-probe call filters them all. Classes run
+// concurrently.
 struct ContractClass {
     uint32_t S = 0, So = 0, mips = 0, levels = 0;
+    size_t stride = 0;  // floats per input chain
     hvx_contract_plan* plan = nullptr;
-    float* chain = nullptr;
+    float* chains = nullptr;
+    const float** d_src = nullptr;  // device arrays of n pointers: probe level 0,
+    float** d_in = nullptr;         // input chains,
+    float** d_out = nullptr;        // output chains
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
     std::vector<uint32_t> probes;
@@ -189,22 +195,21 @@ int upload_views(hvx_batch* b) {
     return HVX_OK;
 }
 
-// Contraction mode: per class, on its own stream forked from `s`: for each
-// probe, copy level 0 into the class chain, build the box mips, filter.
+// Contraction mode: per class, on its own stream forked from `s`: gather the
+// probes' level 0 into their chains, build the box mips of all chains, then
+// one multi-probe contraction call.
 hipError_t run_contract(hvx_batch* b, hipStream_t s) {
     hipError_t e;
     if ((e = hipEventRecord(b->cfork, s)) != hipSuccess) return e;
     for (ContractClass& c : b->classes) {
+        const uint32_t n = (uint32_t)c.probes.size();
         if ((e = hipStreamWaitEvent(c.stream, b->cfork, 0)) != hipSuccess) return e;
-        for (uint32_t i : c.probes) {
-            const hvx_probe& p = b->probes[i];
-            if ((e = hipMemcpyAsync(c.chain, p.cube, (size_t)6 * c.S * c.S * 4 * sizeof(float),
-                                    hipMemcpyDeviceToDevice, c.stream)) != hipSuccess)
-                return e;
-            if (hvx_ibl_cube_mips(b->ctx, c.chain, c.S, c.levels, c.stream) != HVX_OK ||
-                hvx_ibl_filter_contract(c.plan, c.chain, c.levels, p.out_chain, c.stream) != HVX_OK)
-                return hipErrorLaunchFailure;
-        }
+        if ((e = hvx::launch_gather_copy(c.d_src, (size_t)6 * c.S * c.S * 4, c.chains, c.stride, n, c.stream)) !=
+                hipSuccess ||
+            (e = hvx::launch_cube_mips(c.chains, c.S, c.levels, c.stream, n, c.stride)) != hipSuccess)
+            return e;
+        if (hvx_ibl_filter_contract_multi(c.plan, n, c.d_in, c.levels, c.d_out, c.stream) != HVX_OK)
+            return hipErrorLaunchFailure;
         if ((e = hipEventRecord(c.done, c.stream)) != hipSuccess || (e = hipStreamWaitEvent(s, c.done, 0)) != hipSuccess)
             return e;
     }
@@ -238,7 +243,10 @@ hipError_t run_all(hvx_batch* b, hipStream_t s) {
 void release_classes(hvx_batch* b) {
     for (ContractClass& c : b->classes) {
         if (c.plan) (void)hvx_ibl_contract_plan_destroy(c.plan);
-        if (c.chain) (void)hipFree(c.chain);
+        if (c.chains) (void)hipFree(c.chains);
+        if (c.d_src) (void)hipFree(c.d_src);
+        if (c.d_in) (void)hipFree(c.d_in);
+        if (c.d_out) (void)hipFree(c.d_out);
         if (c.done) (void)hipEventDestroy(c.done);
         if (c.stream) (void)hipStreamDestroy(c.stream);
     }
@@ -492,17 +500,41 @@ int hvx_batch_set_contract(hvx_batch* b, int enable, int precision) {
                 release_classes(b);
                 return st;
             }
-            hipError_t e;
-            if ((e = hipMalloc(reinterpret_cast<void**>(&c.chain), hvx_cube_chain_floats(c.S, c.levels) * sizeof(float))) !=
-                    hipSuccess ||
-                (e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking)) != hipSuccess ||
-                (e = hipEventCreateWithFlags(&c.done, hipEventDisableTiming)) != hipSuccess) {
-                release_classes(b);
-                return bhip(b, e, "contraction class set-up");
-            }
             it = b->classes.end() - 1;
         }
         it->probes.push_back(i);
+    }
+    for (ContractClass& c : b->classes) {
+        const uint32_t n = (uint32_t)c.probes.size();
+        c.stride = (hvx_cube_chain_floats(c.S, c.levels) + 63) & ~(size_t)63;  // 256-byte aligned chains
+        std::vector<const float*> src(n);
+        std::vector<float*> in(n), out(n);
+        int st;
+        hipError_t e;
+        if ((st = hvx_ibl_contract_plan_reserve(c.plan, n))) {
+            release_classes(b);
+            return st;
+        }
+        if ((e = hipMalloc(reinterpret_cast<void**>(&c.chains), c.stride * n * sizeof(float))) != hipSuccess ||
+            (e = hipMalloc(reinterpret_cast<void**>(&c.d_src), n * sizeof(void*))) != hipSuccess ||
+            (e = hipMalloc(reinterpret_cast<void**>(&c.d_in), n * sizeof(void*))) != hipSuccess ||
+            (e = hipMalloc(reinterpret_cast<void**>(&c.d_out), n * sizeof(void*))) != hipSuccess) {
+            release_classes(b);
+            return bhip(b, e, "contraction class buffers");
+        }
+        for (uint32_t k = 0; k < n; ++k) {
+            src[k] = b->probes[c.probes[k]].cube;
+            in[k] = c.chains + (size_t)k * c.stride;
+            out[k] = b->probes[c.probes[k]].out_chain;
+        }
+        if ((e = hipMemcpy(c.d_src, src.data(), n * sizeof(void*), hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(c.d_in, in.data(), n * sizeof(void*), hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(c.d_out, out.data(), n * sizeof(void*), hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking)) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&c.done, hipEventDisableTiming)) != hipSuccess) {
+            release_classes(b);
+            return bhip(b, e, "contraction class set-up");
+        }
     }
     // the largest classes first, so their streams start earliest
     std::stable_sort(b->classes.begin(), b->classes.end(),

@@ -56,6 +56,23 @@ struct DenseGeom {
 // Near-field window of one 16 x 16 output tile, in (unpadded) texel indices
 // of the face plane at the near level; woff = weight offset in half8 units.
 // The tile's weights are [k-step][row tile (16)][lane (64)][8].
+// The probes of one filter call: input chains and output chains (one probe:
+// the by-value pointers; several: device arrays of pointers), and where probe
+// p's copy of the per-call scratch lives: a scratch pointer of the plan moves
+// by delta + p * pstride bytes (0 and 0 for the plan's own scratch).
+struct Probes {
+    const float4* chain0;
+    float4* out0;
+    const float4* const* chains;
+    float4* const* outs;
+    uint64_t delta, pstride;
+};
+__device__ __forceinline__ const float4* probe_chain(const Probes& Q, uint32_t p) { return Q.chains ? Q.chains[p] : Q.chain0; }
+__device__ __forceinline__ float4* probe_out(const Probes& Q, uint32_t p) { return Q.outs ? Q.outs[p] : Q.out0; }
+template <class T> __device__ __forceinline__ T* scr(const Probes& Q, T* ptr, uint32_t p) {
+    return reinterpret_cast<T*>(reinterpret_cast<uintptr_t>(ptr) + Q.delta + (uint64_t)p * Q.pstride);
+}
+
 struct NearTile {
     int32_t x0, y0;
     uint32_t w, h;
@@ -292,12 +309,14 @@ __global__ __launch_bounds__(256) void near_scale_kernel(NearGeom G, uint32_t Sf
 // table passed by value.
 constexpr int kMaxJobs = 2 * hvx::kContractMaxMips;
 constexpr uint32_t kFaceMaxBlocks = 32;
+constexpr uint32_t kMaxProbes = 4096;  // probes of one multi-probe call (grid dimension y)
 
 // partial[f][b]: largest RGB magnitude of block b's share of face f
-__global__ __launch_bounds__(256) void facemax_kernel(const float4* __restrict__ level, uint32_t S,
-                                                      float* __restrict__ partial) {
+__global__ __launch_bounds__(256) void facemax_kernel(Probes Q, size_t level_off, uint32_t S, float* partial) {
     __shared__ float sh[256];
-    const uint32_t f = blockIdx.y, S2 = S * S;
+    const uint32_t f = blockIdx.y, S2 = S * S, pr = blockIdx.z;
+    const float4* level = probe_chain(Q, pr) + level_off;
+    partial = scr(Q, partial, pr);
     float m = 0.0f;
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < S2; i += gridDim.x * 256) {
         const float4 v = level[(size_t)f * S2 + i];
@@ -315,10 +334,11 @@ __global__ __launch_bounds__(256) void facemax_kernel(const float4* __restrict__
 
 // sc[f] = 2^-e_f, sc[6+f] = 2^e_f for the padded plane of face f (bounded by
 // face f and its four neighbours); sc[12] = 2^-e, sc[13] = 2^e over all faces.
-__global__ __launch_bounds__(64) void scales_kernel(const float* __restrict__ partial, uint32_t nb,
-                                                    float* __restrict__ sc) {
+__global__ __launch_bounds__(64) void scales_kernel(Probes Q, const float* partial, uint32_t nb, float* sc) {
     __shared__ float fm[6];
     const uint32_t t = threadIdx.x;
+    partial = scr(Q, partial, blockIdx.x);
+    sc = scr(Q, sc, blockIdx.x);
     if (t < 6) {
         float m = 0.0f;
         for (uint32_t b = 0; b < nb; ++b) m = fmaxf(m, partial[t * nb + b]);
@@ -414,18 +434,19 @@ __device__ __forceinline__ void stage_pad_body(const float4* __restrict__ level,
     stage_store3<SPLIT>(P, (size_t)cell * 32 + 4 * f, q, v, (size_t)Pw * Pw8 * 32);
 }
 
-__global__ __launch_bounds__(256) void stage_all_kernel(const float4* __restrict__ chain, StageTable T,
-                                                        const float* __restrict__ sc) {
-    const uint32_t id = blockIdx.x * 256 + threadIdx.x;
+__global__ __launch_bounds__(256) void stage_all_kernel(Probes Q, StageTable T, const float* sc) {
+    const uint32_t id = blockIdx.x * 256 + threadIdx.x, pr = blockIdx.y;
     if (id >= T.total) return;
     uint32_t j = 0;
     while (j + 1 < T.n && id >= T.it[j + 1].begin) ++j;
     const StageItem& I = T.it[j];
-    const float4* level = chain + I.level_off;
-    if (I.kind == 0) stage_dense_body<false>(level, I.S, I.perm, sc[12], I.dst, id - I.begin);
-    else if (I.kind == 3) stage_dense_body<true>(level, I.S, I.perm, sc[12], I.dst, id - I.begin);
-    else if (I.kind == 1) stage_pad_body<false>(level, I.S, I.B, I.Pw, sc, I.dst, id - I.begin);
-    else stage_pad_body<true>(level, I.S, I.B, I.Pw, sc, I.dst, id - I.begin);
+    const float4* level = probe_chain(Q, pr) + I.level_off;
+    sc = scr(Q, sc, pr);
+    half8* dst = scr(Q, I.dst, pr);
+    if (I.kind == 0) stage_dense_body<false>(level, I.S, I.perm, sc[12], dst, id - I.begin);
+    else if (I.kind == 3) stage_dense_body<true>(level, I.S, I.perm, sc[12], dst, id - I.begin);
+    else if (I.kind == 1) stage_pad_body<false>(level, I.S, I.B, I.Pw, sc, dst, id - I.begin);
+    else stage_pad_body<true>(level, I.S, I.B, I.Pw, sc, dst, id - I.begin);
 }
 
 // ----------------------------------------------------------- MFMA GEMMs
@@ -448,8 +469,8 @@ struct DenseTable {
 
 // Dense: a wave owns two row tiles (32 rows) x 32 columns over one K slice;
 // partial sums go to part[slice][row][32] and are reduced in slice order.
-__global__ __launch_bounds__(256) void dense_gemm_kernel(DenseTable T) {
-    const uint32_t gw = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+__global__ __launch_bounds__(256) void dense_gemm_kernel(Probes Q, DenseTable T) {
+    const uint32_t gw = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63, pr = blockIdx.y;
     if (gw >= T.waves) return;  // uniform per wave; the kernel has no barriers
     uint32_t jj = 0;
     while (jj + 1 < T.n && gw >= T.j[jj + 1].wave_begin) ++jj;
@@ -462,7 +483,7 @@ __global__ __launch_bounds__(256) void dense_gemm_kernel(DenseTable T) {
     const uint32_t i = lane & 15, g = lane >> 4;
     const half8* w0 = J.W + (size_t)rt0 * ks * 64 + lane;
     const half8* w1 = has1 ? w0 + (size_t)ks * 64 : w0;
-    const half8* rb = J.R + g * 32 + i;
+    const half8* rb = scr(Q, J.R, pr) + g * 32 + i;
     f32x4 a00 = {0.0f, 0.0f, 0.0f, 0.0f}, a01 = a00, a10 = a00, a11 = a00;
     if (J.w_lo) {  // split BF16: hi*hi + hi*lo + lo*hi
         const bf16x8* v0 = reinterpret_cast<const bf16x8*>(w0);
@@ -495,7 +516,7 @@ __global__ __launch_bounds__(256) void dense_gemm_kernel(DenseTable T) {
             a11 = hvx::mfma16x16x32(x1, b1, a11);
         }
     }
-    float* p = J.part + (size_t)slice * rt * 16 * 32;
+    float* p = scr(Q, J.part, pr) + (size_t)slice * rt * 16 * 32;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const uint32_t row0 = rt0 * 16 + 4 * g + r;
@@ -510,10 +531,10 @@ __global__ __launch_bounds__(256) void dense_gemm_kernel(DenseTable T) {
 
 // out[f][row] = (sum over slices) * rowscale[row] * 2^e  (alpha 1 in the
 // chain, 0 in a far map)
-__global__ __launch_bounds__(256) void dense_epilogue_kernel(DenseTable T, const float* __restrict__ sc,
-                                                             float4* __restrict__ chain_out) {
-    const uint32_t id = blockIdx.x * 256 + threadIdx.x;
+__global__ __launch_bounds__(256) void dense_epilogue_kernel(Probes Q, DenseTable T, const float* sc) {
+    const uint32_t id = blockIdx.x * 256 + threadIdx.x, pr = blockIdx.y;
     if (id >= T.epi) return;
+    sc = scr(Q, sc, pr);
     uint32_t jj = 0;
     while (jj + 1 < T.n && id >= T.j[jj + 1].epi_begin) ++jj;
     const DenseJob& J = T.j[jj];
@@ -521,13 +542,13 @@ __global__ __launch_bounds__(256) void dense_epilogue_kernel(DenseTable T, const
     const uint32_t rows_pad = J.rt * 16;
     float v[3] = {0.0f, 0.0f, 0.0f};
     for (uint32_t s = 0; s < J.splitk; ++s) {
-        const float* p = J.part + ((size_t)s * rows_pad + row) * 32 + f * 4;
+        const float* p = scr(Q, J.part, pr) + ((size_t)s * rows_pad + row) * 32 + f * 4;
         v[0] += p[0];
         v[1] += p[1];
         v[2] += p[2];
     }
     const float k = J.rowscale[row] * sc[13];
-    float4* out = J.out ? J.out : chain_out + J.out_off;
+    float4* out = J.out ? scr(Q, J.out, pr) : probe_out(Q, pr) + J.out_off;
     out[(size_t)f * rows + row] = make_float4(v[0] * k, v[1] * k, v[2] * k, J.alpha);
 }
 
@@ -565,8 +586,9 @@ constexpr uint32_t kNearParts = HVX_NEAR_PARTS;  // workgroups per tile
 static_assert(kNearWaves % kNearParts == 0, "a tile splits into whole waves");
 
 template <bool SPLIT>
-__global__ __launch_bounds__(512) void near_gemm_kernel(NearTable T, const float* __restrict__ sc,
-                                                        float4* __restrict__ chain_out) {
+__global__ __launch_bounds__(512) void near_gemm_kernel(Probes Q, NearTable T, const float* sc) {
+    const uint32_t pr = blockIdx.y;
+    sc = scr(Q, sc, pr);
     uint32_t jj = 0;
     // workgroup b owns half b & 1 (eight row tiles) of tile b >> 1: finer
     // work units than whole tiles balance the unequal mips across the CUs
@@ -576,6 +598,7 @@ __global__ __launch_bounds__(512) void near_gemm_kernel(NearTable T, const float
     const uint32_t tile = gt - J.tile_begin, lane = threadIdx.x & 63;
     const uint32_t wave = (threadIdx.x >> 6) + (blockIdx.x % kNearParts) * (kNearWaves / kNearParts);
     const NearTile Tl = J.tiles[tile];
+    const half8* Pp = scr(Q, J.P, pr);
     const uint32_t wc = Tl.w / 32, KS = Tl.h * wc;
     const uint32_t i = lane & 15, g = lane >> 4, rt0 = 2 * wave;
     // weights of a tile stored K-step major: step s of all 16 row tiles is one
@@ -590,7 +613,7 @@ __global__ __launch_bounds__(512) void near_gemm_kernel(NearTable T, const float
         const size_t pidx = (((size_t)(Tl.y0 + (int)J.B + (int)r) * J.Pw8 + xb + cc * 4) * 32) + i;
         if constexpr (SPLIT) {
             const bf16x8* wv = reinterpret_cast<const bf16x8*>(w0);
-            const bf16x8* pv = reinterpret_cast<const bf16x8*>(J.P);
+            const bf16x8* pv = reinterpret_cast<const bf16x8*>(Pp);
             const size_t o0 = (size_t)s * kStep, o1 = o0 + kRow;
             const bf16x8 x0 = wv[o0], x1 = wv[o1], y0 = wv[o0 + J.w_lo], y1 = wv[o1 + J.w_lo];
             const bf16x8 b0 = pv[pidx], b1 = pv[pidx + 16], c0 = pv[pidx + J.p_lo], c1 = pv[pidx + 16 + J.p_lo];
@@ -608,7 +631,7 @@ __global__ __launch_bounds__(512) void near_gemm_kernel(NearTable T, const float
             a11 = hvx::mfma16x16x32(y1, b1, a11);
         } else {
             const half8 x0 = w0[(size_t)s * kStep], x1 = w1[(size_t)s * kStep];
-            const half8 b0 = J.P[pidx], b1 = J.P[pidx + 16];
+            const half8 b0 = Pp[pidx], b1 = Pp[pidx + 16];
             a00 = hvx::mfma16x16x32(x0, b0, a00);
             a01 = hvx::mfma16x16x32(x0, b1, a01);
             a10 = hvx::mfma16x16x32(x1, b0, a10);
@@ -621,7 +644,7 @@ __global__ __launch_bounds__(512) void near_gemm_kernel(NearTable T, const float
     }
     const uint32_t So = J.So, X0 = (tile % J.te) * 16, Y0 = (tile / J.te) * 16;
     const size_t So2 = (size_t)So * So;
-    float* out = reinterpret_cast<float*>(chain_out + J.out_off);
+    float* out = reinterpret_cast<float*>(probe_out(Q, pr) + J.out_off);
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const uint32_t Y = Y0 + rt0 + h;
@@ -640,17 +663,17 @@ __global__ __launch_bounds__(512) void near_gemm_kernel(NearTable T, const float
 }
 
 // out(o) += far field interpolated at o / Z(o)
-__global__ __launch_bounds__(256) void far_add_kernel(NearTable T, float4* __restrict__ chain_out) {
-    const uint32_t id = blockIdx.x * 256 + threadIdx.x;
+__global__ __launch_bounds__(256) void far_add_kernel(Probes Q, NearTable T) {
+    const uint32_t id = blockIdx.x * 256 + threadIdx.x, pr = blockIdx.y;
     if (id >= T.texels) return;
     uint32_t jj = 0;
     while (jj + 1 < T.n && id >= T.j[jj + 1].texel_begin) ++jj;
     const NearJob& J = T.j[jj];
     const uint32_t So = J.So, So2 = So * So, local = id - J.texel_begin;
     const uint32_t f = local / So2, o = local - f * So2, Y = o / So, X = o - Y * So;
-    const f3 F = sample_cube(J.far, (int)J.Sf, texel_dir((int)f, (int)X, (int)Y, (int)So));
+    const f3 F = sample_cube(scr(Q, J.far, pr), (int)J.Sf, texel_dir((int)f, (int)X, (int)Y, (int)So));
     const float z = J.invZ[o];
-    float4* out = chain_out + J.out_off;
+    float4* out = probe_out(Q, pr) + J.out_off;
     float4 v = out[local];
     v.x += F.x * z;
     v.y += F.y * z;
@@ -659,9 +682,10 @@ __global__ __launch_bounds__(256) void far_add_kernel(NearTable T, float4* __res
 }
 
 // Roughness 0: the bilinear seamless sample of level 0 at the texel direction.
-__global__ __launch_bounds__(256) void delta_kernel(const float4* __restrict__ level0, uint32_t S, uint32_t So,
-                                                    float4* __restrict__ out) {
-    const uint32_t id = blockIdx.x * 256 + threadIdx.x;
+__global__ __launch_bounds__(256) void delta_kernel(Probes Q, uint32_t S, uint32_t So, size_t out_off) {
+    const uint32_t id = blockIdx.x * 256 + threadIdx.x, pr = blockIdx.y;
+    const float4* level0 = probe_chain(Q, pr);
+    float4* out = probe_out(Q, pr) + out_off;
     const uint32_t So2 = So * So;
     if (id >= 6 * So2) return;
     const uint32_t f = id / So2, o = id - f * So2, Y = o / So, X = o - Y * So;
@@ -789,13 +813,22 @@ struct StageDense {
     uint32_t Sin, level;
     bool split = false;  // BF16 hi + lo (lo after K / 8 * 32 units)
     int32_t* perm = nullptr;
-    half8* R = nullptr;
+    half8* R = nullptr;  // in the per-call scratch, at off
+    size_t off = 0;
 };
 
 struct StagePad {
     uint32_t Si, level, B, Pw;
-    half8* P = nullptr;
+    half8* P = nullptr;  // in the per-call scratch, at off
+    size_t off = 0;
 };
+
+// Reserves `bytes` (256-byte aligned) at the end of a scratch layout.
+size_t scratch_take(size_t* total, size_t bytes) {
+    const size_t at = (*total + 255) & ~(size_t)255;
+    *total = at + bytes;
+    return at;
+}
 
 struct MipPlan {
     ContractLevel L{};
@@ -807,7 +840,8 @@ struct MipPlan {
     half8* Wn = nullptr;
     float* rowscale_n = nullptr;
     float* invZ = nullptr;
-    float4* far_map = nullptr;
+    float4* far_map = nullptr;  // in the per-call scratch, at far_off
+    size_t far_off = 0;
     float4* zfar_map = nullptr;
     uint64_t weight_bytes = 0;
     uint64_t w_lo = 0;  // split BF16: offset of the lo weights in Wn (16-byte units)
@@ -823,6 +857,13 @@ struct hvx_contract_plan {
     std::vector<DenseMat> dm;
     std::vector<StageDense> sd;
     std::vector<StagePad> sp;
+    // Per-call scratch, carved from one block of scratch_bytes: far maps,
+    // staged levels and planes, face maxima, scales, K-slice partial sums.
+    // Multi-probe calls use mcap further copies of the block (mscratch).
+    char* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    char* mscratch = nullptr;
+    uint32_t mcap = 0;
     float* fmax = nullptr;  // per-face partial maxima [6][kFaceMaxBlocks]
     float* sc = nullptr;
     float* part = nullptr;   // every dense job's K-slice partial sums
@@ -934,6 +975,54 @@ hipError_t build_near(hvx_contract_plan* P, MipPlan& M, bool split) {
 
 }  // namespace
 
+namespace {
+
+// The eight launches of a filter call over n probes (grid dimension y, z for
+// the face maxima: probe).
+int launch_call(hvx_contract_plan* P, const Probes& Q, uint32_t n, uint32_t chain_levels, hipStream_t s,
+                const char* what) {
+    hvx_ctx* ctx = P->ctx;
+    if (chain_levels <= P->min_level)
+        return pfail(ctx, HVX_ERR_INVALID_ARG, "the input chain needs %u levels (has %u); build it with "
+                     "hvx_ibl_cube_mips", P->min_level + 1, chain_levels);
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || (cur != ctx->device && hipSetDevice(ctx->device) != hipSuccess))
+        return pfail(ctx, HVX_ERR_HIP, "cannot bind device %d", ctx->device);
+    // exponent scales from the finest staged level: coarser levels are its box
+    // means and padded texels its bilinear samples, so its maxima bound them
+    if (P->stage.total) {
+        const uint32_t Sl = std::max(P->S >> P->scale_level, 1u), S2 = Sl * Sl;
+        const uint32_t nb = std::min(kFaceMaxBlocks, (S2 + 255) / 256);
+        hipLaunchKernelGGL(facemax_kernel, dim3(nb, 6, n), dim3(256), 0, s, Q,
+                           hvx::cube_level_offset_floats(P->S, P->scale_level) / 4, Sl, P->fmax);
+        hipLaunchKernelGGL(scales_kernel, dim3(n), dim3(64), 0, s, Q, P->fmax, nb, P->sc);
+        hipLaunchKernelGGL(stage_all_kernel, dim3((P->stage.total + 255) / 256, n), dim3(256), 0, s, Q, P->stage,
+                           P->sc);
+    }
+    if (P->dense.n) {
+        hipLaunchKernelGGL(dense_gemm_kernel, dim3((P->dense.waves + 3) / 4, n), dim3(256), 0, s, Q, P->dense);
+        hipLaunchKernelGGL(dense_epilogue_kernel, dim3((P->dense.epi + 255) / 256, n), dim3(256), 0, s, Q, P->dense,
+                           P->sc);
+    }
+    if (P->near.n) {
+        const dim3 grid(P->near.tiles * kNearParts, n), block(64 * kNearWaves / kNearParts);
+        if (P->precision == HVX_PRECISION_BF16_SPLIT)
+            hipLaunchKernelGGL(near_gemm_kernel<true>, grid, block, 0, s, Q, P->near, P->sc);
+        else
+            hipLaunchKernelGGL(near_gemm_kernel<false>, grid, block, 0, s, Q, P->near, P->sc);
+        hipLaunchKernelGGL(far_add_kernel, dim3((P->near.texels + 255) / 256, n), dim3(256), 0, s, Q, P->near);
+    }
+    if (P->delta_mip != ~0u) {
+        const uint32_t So = P->mp[P->delta_mip].L.So;
+        hipLaunchKernelGGL(delta_kernel, dim3((6 * So * So + 255) / 256, n), dim3(256), 0, s, Q, P->S, So,
+                           hvx::cube_level_offset_floats(P->So, P->delta_mip) / 4);
+    }
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? HVX_OK : pfail(ctx, HVX_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+}  // namespace
+
 extern "C" {
 
 int hvx_ibl_contract_plan_create(hvx_ctx* ctx, int type, uint32_t face, uint32_t out_face, uint32_t out_mips,
@@ -971,7 +1060,7 @@ int hvx_ibl_contract_plan_create(hvx_ctx* ctx, int type, uint32_t face, uint32_t
         return pfail(ctx, HVX_ERR_HIP, "%s: %s", where, hipGetErrorString(e));
     };
     hipError_t e;
-    size_t part_floats = 0;
+    size_t part_floats = 0, sbytes = 0;
     P->mp.resize(out_mips);
     for (uint32_t m = 0; m < out_mips; ++m) {
         MipPlan& M = P->mp[m];
@@ -990,9 +1079,9 @@ int hvx_ibl_contract_plan_create(hvx_ctx* ctx, int type, uint32_t face, uint32_t
             D.G.piece = PIECE_FAR;
             D.G.So = M.L.Sf;
             D.G.Sin = M.L.Sf;
-            if ((e = P->alloc(&M.far_map, 6 * (size_t)M.L.Sf * M.L.Sf)) != hipSuccess ||
-                (e = P->alloc(&M.zfar_map, 6 * (size_t)M.L.Sf * M.L.Sf)) != hipSuccess)
-                return fail_destroy(e, "far maps");
+            M.far_off = scratch_take(&sbytes, 6 * (size_t)M.L.Sf * M.L.Sf * sizeof(float4));
+            if ((e = P->alloc(&M.zfar_map, 6 * (size_t)M.L.Sf * M.L.Sf)) != hipSuccess)
+                return fail_destroy(e, "far normaliser map");
         }
         D.G.rows = D.G.So * D.G.So;
         D.G.rt = (D.G.rows + 15) / 16;
@@ -1025,21 +1114,30 @@ int hvx_ibl_contract_plan_create(hvx_ctx* ctx, int type, uint32_t face, uint32_t
         for (int f = 0; f < 6; ++f)
             for (uint32_t k = 0; k < K; ++k) perm[(size_t)f * K + k] = perm_index(f, k, s.Sin);
         if ((e = P->alloc(&s.perm, perm.size())) != hipSuccess ||
-            (e = hipMemcpy(s.perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
-            (e = P->alloc(&s.R, (size_t)K / 8 * 32 * (s.split ? 2 : 1))) != hipSuccess ||
-            (e = hipMemset(s.R, 0, (size_t)K / 8 * 32 * (s.split ? 2 : 1) * sizeof(half8))) != hipSuccess)
+            (e = hipMemcpy(s.perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
             return fail_destroy(e, "dense staging");
+        s.off = scratch_take(&sbytes, (size_t)K / 8 * 32 * (s.split ? 2 : 1) * sizeof(half8));
         P->min_level = std::max(P->min_level, s.level);
     }
     for (StagePad& s : P->sp) {
-        const size_t n = (size_t)s.Pw * (s.Pw / 8) * 32 * (split ? 2 : 1);
-        if ((e = P->alloc(&s.P, n)) != hipSuccess || (e = hipMemset(s.P, 0, n * sizeof(half8))) != hipSuccess)
-            return fail_destroy(e, "plane staging");
+        s.off = scratch_take(&sbytes, (size_t)s.Pw * (s.Pw / 8) * 32 * (split ? 2 : 1) * sizeof(half8));
         P->min_level = std::max(P->min_level, s.level);
     }
-    if ((e = P->alloc(&P->fmax, 6 * kFaceMaxBlocks)) != hipSuccess || (e = P->alloc(&P->sc, 16)) != hipSuccess ||
-        (e = P->alloc(&P->part, part_floats)) != hipSuccess)
+    const size_t fmax_off = scratch_take(&sbytes, 6 * kFaceMaxBlocks * sizeof(float));
+    const size_t sc_off = scratch_take(&sbytes, 16 * sizeof(float));
+    const size_t part_off = scratch_take(&sbytes, part_floats * sizeof(float));
+    // one block; zeroed once (the pad columns of staged operands stay zero)
+    P->scratch_bytes = (sbytes + 255) & ~(size_t)255;
+    if ((e = P->alloc(&P->scratch, P->scratch_bytes)) != hipSuccess ||
+        (e = hipMemset(P->scratch, 0, P->scratch_bytes)) != hipSuccess)
         return fail_destroy(e, "scratch");
+    for (MipPlan& M : P->mp)
+        if (M.L.kind == hvx::CONTRACT_SPLIT) M.far_map = reinterpret_cast<float4*>(P->scratch + M.far_off);
+    for (StageDense& s : P->sd) s.R = reinterpret_cast<half8*>(P->scratch + s.off);
+    for (StagePad& s : P->sp) s.P = reinterpret_cast<half8*>(P->scratch + s.off);
+    P->fmax = reinterpret_cast<float*>(P->scratch + fmax_off);
+    P->sc = reinterpret_cast<float*>(P->scratch + sc_off);
+    P->part = reinterpret_cast<float*>(P->scratch + part_off);
     // launch tables of a filter call: every staged level, dense job and near job
     StageTable& ST = P->stage;
     for (const StageDense& sd : P->sd) P->scale_level = std::min(P->scale_level, sd.level);
@@ -1104,6 +1202,7 @@ int hvx_ibl_contract_plan_destroy(hvx_contract_plan* P) {
     if (!P) return HVX_ERR_NULL;
     (void)hipDeviceSynchronize();
     for (void* p : P->allocs) (void)hipFree(p);
+    if (P->mscratch) (void)hipFree(P->mscratch);
     delete P;
     return HVX_OK;
 }
@@ -1128,49 +1227,45 @@ int hvx_ibl_filter_contract(hvx_contract_plan* P, const float* d_chain, uint32_t
     if (!P) return HVX_ERR_NULL;
     hvx_ctx* ctx = P->ctx;
     if (!d_chain || !d_out) return pfail(ctx, HVX_ERR_NULL, "NULL chain or output");
-    if (chain_levels <= P->min_level)
-        return pfail(ctx, HVX_ERR_INVALID_ARG, "the input chain needs %u levels (has %u); build it with "
-                     "hvx_ibl_cube_mips", P->min_level + 1, chain_levels);
     if (((uintptr_t)d_chain | (uintptr_t)d_out) % 16) return pfail(ctx, HVX_ERR_ALIGN, "chains need 16-byte alignment");
-    int cur = -1;
-    if (hipGetDevice(&cur) != hipSuccess || (cur != ctx->device && hipSetDevice(ctx->device) != hipSuccess))
-        return pfail(ctx, HVX_ERR_HIP, "cannot bind device %d", ctx->device);
-    hipStream_t s = (hipStream_t)stream;
-    const float4* chain = reinterpret_cast<const float4*>(d_chain);
-    float4* outc = reinterpret_cast<float4*>(d_out);
-    // exponent scales from the finest staged level: coarser levels are its box
-    // means and padded texels its bilinear samples, so its maxima bound them
-    if (P->stage.total) {
-        const uint32_t Sl = std::max(P->S >> P->scale_level, 1u), S2 = Sl * Sl;
-        const uint32_t nb = std::min(kFaceMaxBlocks, (S2 + 255) / 256);
-        hipLaunchKernelGGL(facemax_kernel, dim3(nb, 6), dim3(256), 0, s,
-                           chain + hvx::cube_level_offset_floats(P->S, P->scale_level) / 4, Sl, P->fmax);
-        hipLaunchKernelGGL(scales_kernel, dim3(1), dim3(64), 0, s, P->fmax, nb, P->sc);
+    const Probes Q{reinterpret_cast<const float4*>(d_chain), reinterpret_cast<float4*>(d_out), nullptr, nullptr, 0, 0};
+    return launch_call(P, Q, 1, chain_levels, (hipStream_t)stream, "hvx_ibl_filter_contract");
+}
+
+int hvx_ibl_contract_plan_reserve(hvx_contract_plan* P, uint32_t max_probes) {
+    if (!P) return HVX_ERR_NULL;
+    if (max_probes == 0 || max_probes > kMaxProbes)
+        return pfail(P->ctx, HVX_ERR_INVALID_ARG, "max_probes must be in [1, %u]", kMaxProbes);
+    if (max_probes <= P->mcap) return HVX_OK;
+    (void)hipDeviceSynchronize();  // the old copies may be in use
+    if (P->mscratch) (void)hipFree(P->mscratch);
+    P->mscratch = nullptr;
+    P->mcap = 0;
+    const size_t bytes = P->scratch_bytes * max_probes;
+    hipError_t e;
+    if ((e = hipMalloc(reinterpret_cast<void**>(&P->mscratch), bytes)) != hipSuccess ||
+        (e = hipMemset(P->mscratch, 0, bytes)) != hipSuccess) {
+        if (P->mscratch) (void)hipFree(P->mscratch);
+        P->mscratch = nullptr;
+        return pfail(P->ctx, HVX_ERR_HIP, "multi-probe scratch (%zu bytes): %s", bytes, hipGetErrorString(e));
     }
-    if (P->stage.total)
-        hipLaunchKernelGGL(stage_all_kernel, dim3((P->stage.total + 255) / 256), dim3(256), 0, s, chain, P->stage,
-                           P->sc);
-    if (P->dense.n) {
-        hipLaunchKernelGGL(dense_gemm_kernel, dim3((P->dense.waves + 3) / 4), dim3(256), 0, s, P->dense);
-        hipLaunchKernelGGL(dense_epilogue_kernel, dim3((P->dense.epi + 255) / 256), dim3(256), 0, s, P->dense, P->sc,
-                           outc);
-    }
-    if (P->near.n) {
-        if (P->precision == HVX_PRECISION_BF16_SPLIT)
-            hipLaunchKernelGGL(near_gemm_kernel<true>, dim3(P->near.tiles * kNearParts), dim3(64 * kNearWaves / kNearParts), 0,
-                               s, P->near, P->sc, outc);
-        else
-            hipLaunchKernelGGL(near_gemm_kernel<false>, dim3(P->near.tiles * kNearParts), dim3(64 * kNearWaves / kNearParts), 0,
-                               s, P->near, P->sc, outc);
-        hipLaunchKernelGGL(far_add_kernel, dim3((P->near.texels + 255) / 256), dim3(256), 0, s, P->near, outc);
-    }
-    if (P->delta_mip != ~0u) {
-        const uint32_t So = P->mp[P->delta_mip].L.So;
-        hipLaunchKernelGGL(delta_kernel, dim3((6 * So * So + 255) / 256), dim3(256), 0, s, chain, P->S, So,
-                           outc + hvx::cube_level_offset_floats(P->So, P->delta_mip) / 4);
-    }
-    const hipError_t e = hipGetLastError();
-    return e == hipSuccess ? HVX_OK : pfail(ctx, HVX_ERR_HIP, "hvx_ibl_filter_contract: %s", hipGetErrorString(e));
+    P->mcap = max_probes;
+    return HVX_OK;
+}
+
+int hvx_ibl_filter_contract_multi(hvx_contract_plan* P, uint32_t n, const float* const* d_chains,
+                                  uint32_t chain_levels, float* const* d_outs, void* stream) {
+    if (!P) return HVX_ERR_NULL;
+    if (!d_chains || !d_outs) return pfail(P->ctx, HVX_ERR_NULL, "NULL pointer arrays");
+    if (n == 0) return HVX_OK;
+    if (n > P->mcap)
+        return pfail(P->ctx, HVX_ERR_CAPACITY, "%u probes but scratch reserved for %u (hvx_ibl_contract_plan_reserve)",
+                     n, P->mcap);
+    const Probes Q{nullptr, nullptr, reinterpret_cast<const float4* const*>(d_chains),
+                   reinterpret_cast<float4* const*>(d_outs),
+                   (uint64_t)(reinterpret_cast<uintptr_t>(P->mscratch) - reinterpret_cast<uintptr_t>(P->scratch)),
+                   (uint64_t)P->scratch_bytes};
+    return launch_call(P, Q, n, chain_levels, (hipStream_t)stream, "hvx_ibl_filter_contract_multi");
 }
 
 }  // extern "C"

@@ -115,7 +115,12 @@ hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const
 hipError_t launch_brdf_lut(uint32_t w, uint32_t h, uint32_t n, float* d_lut, hipStream_t s);
 hipError_t launch_equirect_to_cube(const float* d_eq, uint32_t w, uint32_t h, float* d_cube, uint32_t S,
                                    hipStream_t s);
-hipError_t launch_cube_mips(float* d_chain, uint32_t S, uint32_t levels, hipStream_t s);
+// n chains chain_stride_floats apart (one chain by default)
+hipError_t launch_cube_mips(float* d_chain, uint32_t S, uint32_t levels, hipStream_t s, uint32_t n = 1,
+                            size_t chain_stride_floats = 0);
+// d_dst + p * dst_stride_floats <- d_src[p] (a device array of n pointers), `floats` floats each
+hipError_t launch_gather_copy(const float* const* d_src, size_t floats, float* d_dst, size_t dst_stride_floats,
+                              uint32_t n, hipStream_t s);
 hipError_t launch_diffuse(const float* d_cube, uint32_t S, float* d_out, uint32_t Sd, hipStream_t s);
 hipError_t launch_ggx_tables(uint32_t mips, uint32_t n, float4* table, hipStream_t s);
 hipError_t launch_probe_batch(const ProbeJob* d_jobs, const ProbeItem* d_items, uint32_t n_items,

@@ -127,10 +127,13 @@ __global__ __launch_bounds__(256) void equirect_kernel(const float4* __restrict_
 }
 
 // ---------------------------------------------------------------- mip chain
+// grid dimension y: chains `pstride` float4s apart (one chain: y = 1)
 __global__ __launch_bounds__(256) void mip_kernel(const float4* __restrict__ src, uint32_t sp, float4* __restrict__ dst,
-                                                  uint32_t sn) {
+                                                  uint32_t sn, size_t pstride) {
     const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= 6u * sn * sn) return;
+    src += blockIdx.y * pstride;
+    dst += blockIdx.y * pstride;
     const uint32_t x = id % sn, y = (id / sn) % sn, f = id / (sn * sn);
     const float4* p = src + ((size_t)f * sp + 2 * y) * sp + 2 * x;
     const float4 a = p[0], b = p[1], c = p[sp], d = p[sp + 1];
@@ -345,14 +348,34 @@ hipError_t launch_equirect_to_cube(const float* d_eq, uint32_t w, uint32_t h, fl
     return hipGetLastError();
 }
 
-hipError_t launch_cube_mips(float* d_chain, uint32_t S, uint32_t levels, hipStream_t s) {
+hipError_t launch_cube_mips(float* d_chain, uint32_t S, uint32_t levels, hipStream_t s, uint32_t n,
+                            size_t chain_stride_floats) {
     for (uint32_t m = 1; m < levels; ++m) {
         const uint32_t sp = std::max(S >> (m - 1), 1u), sn = std::max(S >> m, 1u);
         const float4* src = reinterpret_cast<const float4*>(d_chain + cube_level_offset_floats(S, m - 1));
         float4* dst = reinterpret_cast<float4*>(d_chain + cube_level_offset_floats(S, m));
         const uint32_t T = 6u * sn * sn;
-        hipLaunchKernelGGL(mip_kernel, dim3((T + 255) / 256), dim3(256), 0, s, src, sp, dst, sn);
+        hipLaunchKernelGGL(mip_kernel, dim3((T + 255) / 256, n), dim3(256), 0, s, src, sp, dst, sn,
+                           chain_stride_floats / 4);
     }
+    return hipGetLastError();
+}
+
+// dst + p * dst_stride <- src[p], `count` float4s, for p < n (grid dimension y)
+__global__ __launch_bounds__(256) void gather_copy_kernel(const float4* const* __restrict__ src, size_t count,
+                                                          float4* __restrict__ dst, size_t dst_stride) {
+    const float4* from = src[blockIdx.y];
+    float4* to = dst + blockIdx.y * dst_stride;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (size_t)gridDim.x * 256) to[i] = from[i];
+}
+
+hipError_t launch_gather_copy(const float* const* d_src, size_t floats, float* d_dst, size_t dst_stride_floats,
+                              uint32_t n, hipStream_t s) {
+    const size_t count = floats / 4;
+    const uint32_t blocks = (uint32_t)std::min<size_t>((count + 255) / 256, 1024);
+    hipLaunchKernelGGL(gather_copy_kernel, dim3(std::max(blocks, 1u), n), dim3(256), 0, s,
+                       reinterpret_cast<const float4* const*>(d_src), count, reinterpret_cast<float4*>(d_dst),
+                       dst_stride_floats / 4);
     return hipGetLastError();
 }
 

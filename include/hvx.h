@@ -296,6 +296,17 @@ int hvx_ibl_contract_plan_query(const hvx_contract_plan* plan, uint32_t mip, hvx
 int hvx_ibl_filter_contract(hvx_contract_plan* plan, const float* d_chain, uint32_t chain_levels, float* d_out,
                             void* stream);
 
+/* Several probes of the plan's sizes in one call (the same launches, with a
+ * probe dimension): d_chains and d_outs are DEVICE arrays of n pointers
+ * (inputs as for hvx_ibl_filter_contract). Each probe's result equals the
+ * single-probe call's bit for bit. The plan must first reserve per-probe
+ * scratch for at least n probes (hvx_ibl_contract_plan_reserve, which
+ * allocates; n <= 4096); the call itself allocates nothing. No reference
+ * counterpart (batches many ResourcesIBL specular prefilters). */
+int hvx_ibl_contract_plan_reserve(hvx_contract_plan* plan, uint32_t max_probes);
+int hvx_ibl_filter_contract_multi(hvx_contract_plan* plan, uint32_t n, const float* const* d_chains,
+                                  uint32_t chain_levels, float* const* d_outs, void* stream);
+
 /* --------------------------------------------- learned radiance (MLP) */
 
 /* A per-environment network standing in for the prefiltered specular chain:

@@ -188,3 +188,35 @@ def test_contract_and_plain_against_exact_integral_cfg2(ctx):
         assert ec.max() <= 2e-2 and np.median(ec) <= 3e-3
         assert ec.max() < ep.max() and np.median(ec) < np.median(ep)
     plan.close()
+
+
+@pytest.mark.parametrize("precision", [abi.PRECISION_FP16_SCALED, abi.PRECISION_BF16_SPLIT])
+def test_contract_multi_equals_single(ctx, precision):
+    """One multi-probe call over three probes of the same sizes equals three
+    single-probe calls bit for bit; more probes than reserved is refused."""
+    import torch
+    S, So, mips = 128, 64, 7
+    levels = abi.mip_count(S)
+    rng = np.random.default_rng(41)
+    plan = abi.ContractPlan(ctx, abi.FILTER_SPECULAR, S, So, mips, precision)
+    chains, singles, multis = [], [], []
+    for k in range(3):
+        c = rng.uniform(0.0, 1.0 + 5 * k, 6 * S * S * 4).astype(np.float32)
+        chain = torch.empty(abi.cube_chain_floats(S, levels), dtype=torch.float32, device="cuda")
+        chain[:c.size].copy_(to_dev(c))
+        ctx.cube_mips(chain, S, levels)
+        out = torch.full((abi.cube_chain_floats(So, mips),), -1.0, dtype=torch.float32, device="cuda")
+        plan.filter(chain, levels, out)
+        chains.append(chain)
+        singles.append(out)
+        multis.append(torch.full_like(out, -2.0))
+    d_in = torch.tensor([c.data_ptr() for c in chains], dtype=torch.int64, device="cuda")
+    d_out = torch.tensor([o.data_ptr() for o in multis], dtype=torch.int64, device="cuda")
+    with pytest.raises(abi.HvxError):
+        plan.filter_multi(3, d_in, levels, d_out)   # nothing reserved yet
+    plan.reserve(3)
+    plan.filter_multi(3, d_in, levels, d_out)
+    torch.cuda.synchronize()
+    for k in range(3):
+        assert torch.equal(multis[k].view(torch.int32), singles[k].view(torch.int32)), f"probe {k}"
+    plan.close()
