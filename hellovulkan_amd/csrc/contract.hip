@@ -66,7 +66,12 @@ struct Probes {
     const float4* const* chains;
     float4* const* outs;
     uint64_t delta, pstride;
+    uint32_t probe_fast;  // GEMMs: grid x = probe, y = workgroup (probes sharing weights dispatch together)
 };
+// (workgroup, probe) of a GEMM launch
+__device__ __forceinline__ uint2 gemm_block(const Probes& Q) {
+    return Q.probe_fast ? make_uint2(blockIdx.y, blockIdx.x) : make_uint2(blockIdx.x, blockIdx.y);
+}
 __device__ __forceinline__ const float4* probe_chain(const Probes& Q, uint32_t p) { return Q.chains ? Q.chains[p] : Q.chain0; }
 __device__ __forceinline__ float4* probe_out(const Probes& Q, uint32_t p) { return Q.outs ? Q.outs[p] : Q.out0; }
 template <class T> __device__ __forceinline__ T* scr(const Probes& Q, T* ptr, uint32_t p) {
@@ -470,7 +475,8 @@ struct DenseTable {
 // Dense: a wave owns two row tiles (32 rows) x 32 columns over one K slice;
 // partial sums go to part[slice][row][32] and are reduced in slice order.
 __global__ __launch_bounds__(256) void dense_gemm_kernel(Probes Q, DenseTable T) {
-    const uint32_t gw = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63, pr = blockIdx.y;
+    const uint2 wb = gemm_block(Q);
+    const uint32_t gw = wb.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63, pr = wb.y;
     if (gw >= T.waves) return;  // uniform per wave; the kernel has no barriers
     uint32_t jj = 0;
     while (jj + 1 < T.n && gw >= T.j[jj + 1].wave_begin) ++jj;
@@ -587,16 +593,17 @@ static_assert(kNearWaves % kNearParts == 0, "a tile splits into whole waves");
 
 template <bool SPLIT>
 __global__ __launch_bounds__(512) void near_gemm_kernel(Probes Q, NearTable T, const float* sc) {
-    const uint32_t pr = blockIdx.y;
+    const uint2 wb = gemm_block(Q);
+    const uint32_t pr = wb.y;
     sc = scr(Q, sc, pr);
     uint32_t jj = 0;
     // workgroup b owns half b & 1 (eight row tiles) of tile b >> 1: finer
     // work units than whole tiles balance the unequal mips across the CUs
-    const uint32_t gt = blockIdx.x / kNearParts;
+    const uint32_t gt = wb.x / kNearParts;
     while (jj + 1 < T.n && gt >= T.j[jj + 1].tile_begin) ++jj;
     const NearJob& J = T.j[jj];
     const uint32_t tile = gt - J.tile_begin, lane = threadIdx.x & 63;
-    const uint32_t wave = (threadIdx.x >> 6) + (blockIdx.x % kNearParts) * (kNearWaves / kNearParts);
+    const uint32_t wave = (threadIdx.x >> 6) + (wb.x % kNearParts) * (kNearWaves / kNearParts);
     const NearTile Tl = J.tiles[tile];
     const half8* Pp = scr(Q, J.P, pr);
     const uint32_t wc = Tl.w / 32, KS = Tl.h * wc;
@@ -999,17 +1006,23 @@ int launch_call(hvx_contract_plan* P, const Probes& Q, uint32_t n, uint32_t chai
         hipLaunchKernelGGL(stage_all_kernel, dim3((P->stage.total + 255) / 256, n), dim3(256), 0, s, Q, P->stage,
                            P->sc);
     }
+    // GEMMs with several probes: probe index fastest, so the probes' workgroups
+    // for one weight block run together and share it in the caches
+    Probes G = Q;
+    const uint32_t dense_wgs = (P->dense.waves + 3) / 4, near_wgs = P->near.tiles * kNearParts;
+    G.probe_fast = n > 1 && std::max(dense_wgs, near_wgs) <= 65535u;
     if (P->dense.n) {
-        hipLaunchKernelGGL(dense_gemm_kernel, dim3((P->dense.waves + 3) / 4, n), dim3(256), 0, s, Q, P->dense);
+        hipLaunchKernelGGL(dense_gemm_kernel, G.probe_fast ? dim3(n, dense_wgs) : dim3(dense_wgs, n), dim3(256), 0, s,
+                           G, P->dense);
         hipLaunchKernelGGL(dense_epilogue_kernel, dim3((P->dense.epi + 255) / 256, n), dim3(256), 0, s, Q, P->dense,
                            P->sc);
     }
     if (P->near.n) {
-        const dim3 grid(P->near.tiles * kNearParts, n), block(64 * kNearWaves / kNearParts);
+        const dim3 grid = G.probe_fast ? dim3(n, near_wgs) : dim3(near_wgs, n), block(64 * kNearWaves / kNearParts);
         if (P->precision == HVX_PRECISION_BF16_SPLIT)
-            hipLaunchKernelGGL(near_gemm_kernel<true>, grid, block, 0, s, Q, P->near, P->sc);
+            hipLaunchKernelGGL(near_gemm_kernel<true>, grid, block, 0, s, G, P->near, P->sc);
         else
-            hipLaunchKernelGGL(near_gemm_kernel<false>, grid, block, 0, s, Q, P->near, P->sc);
+            hipLaunchKernelGGL(near_gemm_kernel<false>, grid, block, 0, s, G, P->near, P->sc);
         hipLaunchKernelGGL(far_add_kernel, dim3((P->near.texels + 255) / 256, n), dim3(256), 0, s, Q, P->near);
     }
     if (P->delta_mip != ~0u) {
@@ -1228,7 +1241,7 @@ int hvx_ibl_filter_contract(hvx_contract_plan* P, const float* d_chain, uint32_t
     hvx_ctx* ctx = P->ctx;
     if (!d_chain || !d_out) return pfail(ctx, HVX_ERR_NULL, "NULL chain or output");
     if (((uintptr_t)d_chain | (uintptr_t)d_out) % 16) return pfail(ctx, HVX_ERR_ALIGN, "chains need 16-byte alignment");
-    const Probes Q{reinterpret_cast<const float4*>(d_chain), reinterpret_cast<float4*>(d_out), nullptr, nullptr, 0, 0};
+    const Probes Q{reinterpret_cast<const float4*>(d_chain), reinterpret_cast<float4*>(d_out), nullptr, nullptr, 0, 0, 0};
     return launch_call(P, Q, 1, chain_levels, (hipStream_t)stream, "hvx_ibl_filter_contract");
 }
 
@@ -1264,7 +1277,7 @@ int hvx_ibl_filter_contract_multi(hvx_contract_plan* P, uint32_t n, const float*
     const Probes Q{nullptr, nullptr, reinterpret_cast<const float4* const*>(d_chains),
                    reinterpret_cast<float4* const*>(d_outs),
                    (uint64_t)(reinterpret_cast<uintptr_t>(P->mscratch) - reinterpret_cast<uintptr_t>(P->scratch)),
-                   (uint64_t)P->scratch_bytes};
+                   (uint64_t)P->scratch_bytes, 0};
     return launch_call(P, Q, n, chain_levels, (hipStream_t)stream, "hvx_ibl_filter_contract_multi");
 }
 
