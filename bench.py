@@ -49,7 +49,7 @@ def parse():
     p.add_argument("--no-batch", action="store_true", help="skip the batched probes + views workload")
     p.add_argument("--no-contract", action="store_true", help="skip the contraction (MFMA) filter timings")
     p.add_argument("--no-learned", action="store_true", help="skip the learned-radiance (MLP) workload")
-    p.add_argument("--mlp-steps", type=int, default=30000)
+    p.add_argument("--mlp-steps", type=int, default=15000)
     p.add_argument("--batch-probes", type=int, default=256)
     p.add_argument("--calibrate", action="store_true",
                    help="run one 256 MiB elementwise copy first (known HBM traffic, for counter scaling)")
@@ -293,7 +293,7 @@ def learned_bench(ctx, stream, steps, d_spec, SPEC, SPEC_MIPS, frs, cam, ubo, pc
     import torch
     from hellovulkan_amd import abi
     lod_max = float(pc.max_reflection_lod)
-    n, batch = 1 << 22, 16384
+    n, batch = 1 << 22, 32768
     mlp = abi.Mlp(ctx, lod_max, 2024)
     x = torch.empty(n * 4, dtype=torch.float32, device=d_spec.device)
     t = torch.empty(n * 4, dtype=torch.float32, device=d_spec.device)
@@ -301,7 +301,7 @@ def learned_bench(ctx, stream, steps, d_spec, SPEC, SPEC_MIPS, frs, cam, ubo, pc
     curve = torch.zeros(steps, dtype=torch.float32, device=d_spec.device)
     torch.cuda.synchronize()
     t0 = time.time()
-    mlp.train(x, t, n, batch, steps, 2e-3, curve)
+    mlp.train(x, t, n, batch, steps, 4e-3, curve)
     torch.cuda.synchronize()
     train_s = time.time() - t0
     c = curve.cpu().numpy()

@@ -83,23 +83,42 @@ __global__ __launch_bounds__(256) void mlp_data_kernel(const float4* __restrict_
 
 // ----------------------------------------------------------------- encoding
 // thread per (sample, input j, frequency k); rows >= n are the encoding of 0
-__global__ __launch_bounds__(256) void mlp_pe_kernel(const float4* __restrict__ in, uint32_t n, uint32_t B,
-                                                     float lod_max, bf16* __restrict__ X, bf16* __restrict__ XT) {
-    const uint32_t id = blockIdx.x * 256 + threadIdx.x;
-    if (id >= B * 64) return;
-    const uint32_t b = id >> 6, j = (id >> 4) & 3, k = id & 15;
-    float x = 0.0f;
+// One thread per sample: its 128 features (row of X, sixteen 16-byte
+// stores) and their transposed copies (XT, coalesced across the samples).
+__global__ __launch_bounds__(64) void mlp_pe_kernel(const float4* __restrict__ in, uint32_t n, uint32_t B,
+                                                    float lod_max, bf16* __restrict__ X, bf16* __restrict__ XT) {
+    const uint32_t b = blockIdx.x * 64 + threadIdx.x;
+    if (b >= B) return;
+    float xin[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     if (b < n) {
         const float4 v = in[b];
-        x = j == 0 ? v.x : (j == 1 ? v.y : (j == 2 ? v.z : 2.0f * v.w / lod_max - 1.0f));
+        xin[0] = v.x;
+        xin[1] = v.y;
+        xin[2] = v.z;
+        xin[3] = 2.0f * v.w / lod_max - 1.0f;
     }
-    float sv, cv;
-    hvx::mlp_sincos(x, (int)k, sv, cv);
-    const uint32_t c = j * 32 + 2 * k;
-    X[(size_t)b * kIn + c] = (bf16)sv;
-    X[(size_t)b * kIn + c + 1] = (bf16)cv;
-    XT[(size_t)c * B + b] = (bf16)sv;
-    XT[(size_t)(c + 1) * B + b] = (bf16)cv;
+    bf16x8* xrow = reinterpret_cast<bf16x8*>(X + (size_t)b * kIn);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            // frequencies 4 g .. 4 g + 3 of input j, as mlp_sincos evaluates them
+            float sv[4], cv[4];
+            hvx::mlp_sincos_direct(xin[j], 4 * g, sv[0], cv[0]);
+            hvx::mlp_sincos_direct(xin[j], 4 * g + 1, sv[1], cv[1]);
+            hvx::mlp_sincos_double(sv[0], cv[0], sv[2], cv[2]);
+            hvx::mlp_sincos_double(sv[1], cv[1], sv[3], cv[3]);
+            bf16x8 pk;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                pk[2 * q] = (bf16)sv[q];
+                pk[2 * q + 1] = (bf16)cv[q];
+            }
+            xrow[4 * j + g] = pk;
+            const uint32_t c0 = 32 * j + 8 * g;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) XT[(size_t)(c0 + q) * B + b] = pk[q];
+        }
 }
 
 // -------------------------------------------------------------------- GEMM
@@ -143,6 +162,7 @@ __global__ __launch_bounds__(256) void gemm_abt_kernel(GemmArgs g) {
 #pragma unroll
     for (int nb = 0; nb < 4; ++nb) {
         const uint32_t col = cg * 64 + nb * 16 + i;
+        bf16 ht[4];  // rows 4 gq .. 4 gq + 3 of column col: one 8-byte store into CT
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const uint32_t row = rt * 16 + 4 * gq + r;
@@ -154,8 +174,12 @@ __global__ __launch_bounds__(256) void gemm_abt_kernel(GemmArgs g) {
                 else if (!((float)g.mask[(size_t)row * g.Nc + col] > 0.0f)) v = 0.0f;
                 const bf16 h = (bf16)v;
                 g.C[(size_t)row * g.Nc + col] = h;
-                g.CT[(size_t)col * g.M + row] = h;
+                ht[r] = h;
             }
+        }
+        if (EPI != EPI_PART) {
+            typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+            *reinterpret_cast<bf16x4*>(g.CT + (size_t)col * g.M + rt * 16 + 4 * gq) = bf16x4{ht[0], ht[1], ht[2], ht[3]};
         }
     }
 }
@@ -172,12 +196,21 @@ __global__ __launch_bounds__(256) void mlp_out_kernel(const bf16* __restrict__ H
                                                       bf16* __restrict__ dZT, float* __restrict__ part) {
     __shared__ float sdY[256][3];
     __shared__ float sl[256];
+    __shared__ bf16 sh3[kH][256 + 8];  // the block's H3 transposed (padded rows)
     const uint32_t t = threadIdx.x, b = blockIdx.x * 256 + t;
     const float* W4 = P + hvx::kMlpW4;
     const float* b4 = P + hvx::kMlpB4;
     float h[kH];
+    const bf16x8* hrow = reinterpret_cast<const bf16x8*>(H3 + (size_t)b * kH);
 #pragma unroll
-    for (int k = 0; k < kH; ++k) h[k] = (float)H3[(size_t)b * kH + k];
+    for (int k8 = 0; k8 < kH / 8; ++k8) {
+        const bf16x8 v = hrow[k8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            h[8 * k8 + q] = (float)v[q];
+            if (TRAIN) sh3[8 * k8 + q][t] = v[q];
+        }
+    }
     float y[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) y[c] = hvx::mlp_output(h, W4 + c * kH, b4[c]);
@@ -193,18 +226,24 @@ __global__ __launch_bounds__(256) void mlp_out_kernel(const bf16* __restrict__ H
     sdY[t][2] = e2 * inv_norm;
     sl[t] = (e0 * e0 + e1 * e1) + e2 * e2;
 #pragma unroll
-    for (int k = 0; k < kH; ++k) {
-        const float dh = (sdY[t][0] * W4[k] + sdY[t][1] * W4[kH + k]) + sdY[t][2] * W4[2 * kH + k];
-        const bf16 v = (bf16)(h[k] > 0.0f ? dh : 0.0f);
-        dZ[(size_t)b * kH + k] = v;
-        dZT[(size_t)k * B + b] = v;
+    for (int k8 = 0; k8 < kH / 8; ++k8) {
+        bf16x8 pk;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int k = 8 * k8 + q;
+            const float dh = (sdY[t][0] * W4[k] + sdY[t][1] * W4[kH + k]) + sdY[t][2] * W4[2 * kH + k];
+            const bf16 v = (bf16)(h[k] > 0.0f ? dh : 0.0f);
+            pk[q] = v;
+            dZT[(size_t)k * B + b] = v;
+        }
+        reinterpret_cast<bf16x8*>(dZ + (size_t)b * kH)[k8] = pk;
     }
     __syncthreads();
     float* pp = part + (size_t)blockIdx.x * kOutPart;
     if (t < 3 * kH) {
         const uint32_t c = t / kH, k = t % kH;
         float acc = 0.0f;
-        for (uint32_t j = 0; j < 256; ++j) acc += sdY[j][c] * (float)H3[((size_t)blockIdx.x * 256 + j) * kH + k];
+        for (uint32_t j = 0; j < 256; ++j) acc += sdY[j][c] * (float)sh3[k][j];
         pp[t] = acc;
     } else if (t < 3 * kH + 3) {
         const uint32_t c = t - 3 * kH;
@@ -225,7 +264,8 @@ __global__ __launch_bounds__(256) void reduce_kernel(const float* __restrict__ p
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= len) return;
     float acc = 0.0f;
-    for (uint32_t s = 0; s < slices; ++s) acc += part[(size_t)s * stride + i];
+#pragma unroll 8
+    for (uint32_t s = 0; s < slices; ++s) acc += part[(size_t)s * stride + i];  // loads in flight, sums in order
     out[i] = acc * scale;
 }
 
@@ -234,7 +274,13 @@ __global__ __launch_bounds__(256) void colsum_kernel(const bf16* __restrict__ dZ
     __shared__ float sh[256];
     const uint32_t r = blockIdx.x;
     float acc = 0.0f;
-    for (uint32_t b = threadIdx.x; b < B; b += 256) acc += (float)dZT[(size_t)r * B + b];
+    const bf16x8* row = reinterpret_cast<const bf16x8*>(dZT + (size_t)r * B);  // B is a multiple of 256
+#pragma unroll 4
+    for (uint32_t b = threadIdx.x; b < B / 8; b += 256) {
+        const bf16x8 v = row[b];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc += (float)v[q];
+    }
     sh[threadIdx.x] = acc;
     for (uint32_t h = 128; h > 0; h >>= 1) {
         __syncthreads();
@@ -416,7 +462,7 @@ template <int EPI> void gemm(const GemmArgs& g, hipStream_t s) {
 
 // forward of B (padded) samples, n of them real
 void forward(hvx_mlp* M, const float4* in, uint32_t n, uint32_t B, hipStream_t s) {
-    hipLaunchKernelGGL(mlp_pe_kernel, dim3(B * 64 / 256), dim3(256), 0, s, in, n, B, M->lod_max, M->X0, M->X0T);
+    hipLaunchKernelGGL(mlp_pe_kernel, dim3((B + 63) / 64), dim3(64), 0, s, in, n, B, M->lod_max, M->X0, M->X0T);
     const bf16* Ws[3] = {M->W1, M->W2, M->W3};
     const int bias[3] = {hvx::kMlpB1, hvx::kMlpB2, hvx::kMlpB3};
     const bf16* prev = M->X0;
