@@ -10,10 +10,16 @@
 //   hvx_frame [--width W --height H --grid X Y Z --lights N --cap C]
 //             [--cube S --diffuse S --specular S --mips M --lut S]
 //             [--frames K --warmup W] [--dump PREFIX]
+//             [--contract PRECISION] [--tonemap OP [--gamma 0|1]]
 //
+// --contract builds the specular chain with the MFMA contraction filter
+// (hvx_precision) instead of the sampled prefilter; --tonemap adds the post
+// step (hvx_tonemap_op) after shading, eagerly and in the graph.
 // --dump writes PREFIX.cells (hvx_light_cell[C]), PREFIX.indices (u32, the
-// packed total), PREFIX.rgb (float[3][W*H], eager frame) and PREFIX.rgb_graph
-// (graph replay) for comparison against the C ABI driven from Python.
+// packed total), PREFIX.rgb (float[3][W*H], eager frame), PREFIX.rgb_graph
+// (graph replay), PREFIX.spec (the specular chain) and with --tonemap
+// PREFIX.bgra8 / PREFIX.bgra8_graph, for comparison against the C ABI
+// driven from Python.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -34,6 +40,7 @@ struct Args {
     uint32_t cube = 1024, diffuse = 32, specular = 512, mips = 10, lut = 512;
     uint32_t frames = 20, warmup = 3;
     uint64_t light_seed = 1001, frag_seed = 1002, env_seed = 77;
+    int contract = -1, tonemap = -1, gamma = 1;
     std::string dump;
 };
 
@@ -60,6 +67,9 @@ Args parse(int argc, char** argv) {
         else if (k == "--light-seed") a.light_seed = next();
         else if (k == "--frag-seed") a.frag_seed = next();
         else if (k == "--dump") { if (i + 1 >= argc) throw std::runtime_error("--dump PREFIX"); a.dump = argv[++i]; }
+        else if (k == "--contract") a.contract = (int)next();
+        else if (k == "--tonemap") a.tonemap = (int)next();
+        else if (k == "--gamma") a.gamma = (int)next();
         else throw std::runtime_error("unknown argument " + k);
     }
     return a;
@@ -142,6 +152,12 @@ int main(int argc, char** argv) {
         ibl.lutWidth_ = ibl.lutHeight_ = a.lut;
         const auto t_ibl0 = std::chrono::steady_clock::now();
         hvx::CreateIBL(ctx, ibl, d_eq.data(), ew, eh, stream);
+        if (a.contract >= 0) {  // the specular chain through the contraction filter instead
+            hvx::PipelineCubeFilterContract filter(ctx, &ibl.environmentCubemap_, ibl.cubeSize_, stream);
+            filter.OffscreenRender(ibl.specularCubemap_, hvx::CubeFilterType::Specular, ibl.specularSize_,
+                                   ibl.specularMips_, (hvx_precision)a.contract, stream);
+            hvx::hip_check(hipStreamSynchronize(stream), "contraction sync");
+        }
         hvx::hip_check(hipStreamSynchronize(stream), "ibl sync");
         const double ibl_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_ibl0).count();
@@ -150,9 +166,16 @@ int main(int argc, char** argv) {
         hvx::ResourcesClusterForward resCF;
         resCF.CreateBuffers(a.gx, a.gy, a.gz, a.cap);
         hvx::DeviceBuffer<float> rgb(3 * F), rgb_graph(3 * F);
+        hvx::DeviceBuffer<uint32_t> bgra(a.tonemap >= 0 ? F : 0), bgra_graph(a.tonemap >= 0 ? F : 0);
         hvx::PipelineAABBGenerator aabb(ctx, &resCF);
         hvx::PipelineLightCulling cull(ctx, &resLight, &resCF);
         hvx::PipelinePBRClusterForward pbr(ctx, &frags, &resLight, &resCF, &ibl, &rgb);
+        hvx::PipelineTonemap tone(ctx, &rgb, a.width, a.height, &bgra), tone_g(ctx, &rgb_graph, a.width, a.height,
+                                                                               &bgra_graph);
+        if (a.tonemap >= 0) {
+            tone.SetOperator((hvx_tonemap_op)a.tonemap, a.gamma != 0);
+            tone_g.SetOperator((hvx_tonemap_op)a.tonemap, a.gamma != 0);
+        }
         aabb.SetClusterForwardUBO(ubo);
         cull.ResetGlobalIndex();
         cull.SetClusterForwardUBO(ubo);
@@ -164,15 +187,13 @@ int main(int argc, char** argv) {
         hvx::hip_check(hipEventCreate(&e0), "hipEventCreate");
         hvx::hip_check(hipEventCreate(&e1), "hipEventCreate");
         // eager frames
-        for (uint32_t i = 0; i < a.warmup; ++i) {
-            cull.FillCommandBuffer(stream);
-            pbr.FillCommandBuffer(stream);
-        }
+        std::vector<hvx::PipelineBase*> eager = {&cull, &pbr};
+        if (a.tonemap >= 0) eager.push_back(&tone);
+        for (uint32_t i = 0; i < a.warmup; ++i)
+            for (hvx::PipelineBase* p : eager) p->FillCommandBuffer(stream);
         hvx::hip_check(hipEventRecord(e0, stream), "hipEventRecord");
-        for (uint32_t i = 0; i < a.frames; ++i) {
-            cull.FillCommandBuffer(stream);
-            pbr.FillCommandBuffer(stream);
-        }
+        for (uint32_t i = 0; i < a.frames; ++i)
+            for (hvx::PipelineBase* p : eager) p->FillCommandBuffer(stream);
         hvx::hip_check(hipEventRecord(e1, stream), "hipEventRecord");
         hvx::hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
         const float eager_ms = elapsed_ms(e0, e1) / (float)a.frames;
@@ -181,7 +202,9 @@ int main(int argc, char** argv) {
         hvx::PipelinePBRClusterForward pbr_g(ctx, &frags, &resLight, &resCF, &ibl, &rgb_graph);
         pbr_g.SetClusterForwardUBO(ubo);
         pbr_g.SetCameraUBO(cam);
-        hvx::FrameGraph graph({&cull, &pbr_g});
+        std::vector<hvx::PipelineBase*> staged = {&cull, &pbr_g};
+        if (a.tonemap >= 0) staged.push_back(&tone_g);
+        hvx::FrameGraph graph(staged);
         graph.Capture();
         for (uint32_t i = 0; i < a.warmup; ++i) graph.Launch();
         hvx::hip_check(hipEventRecord(e0, graph.stream()), "hipEventRecord");
@@ -196,7 +219,13 @@ int main(int argc, char** argv) {
         std::vector<float> h_rgb(3 * F), h_rgb_g(3 * F);
         rgb.download(h_rgb.data(), h_rgb.size());
         rgb_graph.download(h_rgb_g.data(), h_rgb_g.size());
-        const bool graph_matches = std::memcmp(h_rgb.data(), h_rgb_g.data(), h_rgb.size() * sizeof(float)) == 0;
+        bool graph_matches = std::memcmp(h_rgb.data(), h_rgb_g.data(), h_rgb.size() * sizeof(float)) == 0;
+        std::vector<uint32_t> h_bgra(bgra.size()), h_bgra_g(bgra_graph.size());
+        if (a.tonemap >= 0) {
+            bgra.download(h_bgra.data(), F);
+            bgra_graph.download(h_bgra_g.data(), F);
+            graph_matches = graph_matches && h_bgra == h_bgra_g;
+        }
         double rgb_sum = 0.0;
         for (float v : h_rgb) rgb_sum += v;
         if (!a.dump.empty()) {
@@ -208,13 +237,20 @@ int main(int argc, char** argv) {
             write_file(a.dump + ".indices", idx);
             write_file(a.dump + ".rgb", h_rgb);
             write_file(a.dump + ".rgb_graph", h_rgb_g);
+            std::vector<float> spec(ibl.specularCubemap_.size());
+            ibl.specularCubemap_.download(spec.data(), spec.size());
+            write_file(a.dump + ".spec", spec);
+            if (a.tonemap >= 0) {
+                write_file(a.dump + ".bgra8", h_bgra);
+                write_file(a.dump + ".bgra8_graph", h_bgra_g);
+            }
         }
         std::printf(
             "{\"eager_ms_per_frame\": %.4f, \"graph_ms_per_frame\": %.4f, \"graph_nodes\": %zu, "
             "\"graph_matches_eager\": %s, \"mfrag_per_s_graph\": %.2f, \"ibl_build_ms\": %.2f, \"list_entries\": %u, "
-            "\"rgb_sum\": %.6e, \"fragments\": %zu}\n",
+            "\"rgb_sum\": %.6e, \"fragments\": %zu, \"contract\": %d, \"tonemap\": %d}\n",
             eager_ms, graph_ms, graph.node_count(), graph_matches ? "true" : "false",
-            (double)F / (graph_ms * 1e-3) / 1e6, ibl_ms, total, rgb_sum, F);
+            (double)F / (graph_ms * 1e-3) / 1e6, ibl_ms, total, rgb_sum, F, a.contract, a.tonemap);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         (void)hipStreamDestroy(stream);

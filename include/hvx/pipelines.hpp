@@ -315,6 +315,117 @@ public:
     void FillCommandBuffer(hipStream_t) override {}
 };
 
+// Owner of a contraction-filter plan (hvx_ibl_contract_plan_*): the weights
+// of one (type, face, out_face, out_mips, precision), built once.
+class ContractPlan {
+public:
+    ContractPlan(Context& ctx, CubeFilterType type, uint32_t face, uint32_t outFace, uint32_t outMips,
+                 hvx_precision precision = HVX_PRECISION_FP16_SCALED)
+        : ctx_(ctx), type_(type), face_(face), outFace_(outFace), outMips_(outMips), precision_(precision) {
+        ctx_.check(hvx_ibl_contract_plan_create(ctx_.get(), (int)type, face, outFace, outMips, precision, &plan_),
+                   "hvx_ibl_contract_plan_create");
+    }
+    ~ContractPlan() {
+        if (plan_) (void)hvx_ibl_contract_plan_destroy(plan_);
+    }
+    ContractPlan(const ContractPlan&) = delete;
+    ContractPlan& operator=(const ContractPlan&) = delete;
+    bool matches(CubeFilterType t, uint32_t face, uint32_t outFace, uint32_t mips, hvx_precision pr) const {
+        return t == type_ && face == face_ && outFace == outFace_ && mips == outMips_ && pr == precision_;
+    }
+    // chain: the input cube with its box mips (levels levels)
+    void Filter(const float* chain, uint32_t levels, float* out, hipStream_t stream = nullptr) {
+        ctx_.check(hvx_ibl_filter_contract(plan_, chain, levels, out, stream), "hvx_ibl_filter_contract");
+    }
+    hvx_contract_mip_info Info(uint32_t mip) const {
+        hvx_contract_mip_info i{};
+        ctx_.check(hvx_ibl_contract_plan_query(plan_, mip, &i), "hvx_ibl_contract_plan_query");
+        return i;
+    }
+
+private:
+    Context& ctx_;
+    CubeFilterType type_;
+    uint32_t face_, outFace_, outMips_;
+    hvx_precision precision_;
+    hvx_contract_plan* plan_ = nullptr;
+};
+
+// PipelineCubeFilter's interface over the MFMA contraction filter (no
+// reference counterpart: a deterministic quadrature of the same integral in
+// place of the sampled estimate). Plans are built on first use and kept.
+class PipelineCubeFilterContract final : public PipelineBase {
+public:
+    PipelineCubeFilterContract(Context& ctx, DeviceBuffer<float>* inputCube, uint32_t faceSize,
+                               hipStream_t stream = nullptr)
+        : PipelineBase(ctx), in_(inputCube), size_(faceSize), levels_(hvx_mip_count(faceSize)) {
+        ctx_.check(hvx_ibl_cube_mips(ctx_.get(), in_->data(), size_, levels_, stream), "hvx_ibl_cube_mips");
+    }
+    void OffscreenRender(DeviceBuffer<float>& out, CubeFilterType type, uint32_t outSize, uint32_t mips,
+                         hvx_precision precision = HVX_PRECISION_FP16_SCALED, hipStream_t stream = nullptr) {
+        ContractPlan* plan = nullptr;
+        for (auto& p : plans_)
+            if (p->matches(type, size_, outSize, mips, precision)) plan = p.get();
+        if (!plan) {
+            plans_.push_back(std::make_unique<ContractPlan>(ctx_, type, size_, outSize, mips, precision));
+            plan = plans_.back().get();
+        }
+        out.resize(hvx_cube_chain_floats(outSize, mips));
+        plan->Filter(in_->data(), levels_, out.data(), stream);
+    }
+    void FillCommandBuffer(hipStream_t) override {}
+
+private:
+    DeviceBuffer<float>* in_;
+    uint32_t size_, levels_;
+    std::vector<std::unique_ptr<ContractPlan>> plans_;
+};
+
+// Owner of a per-environment radiance network (hvx_mlp_*): trained on
+// samples of a prefiltered specular chain, with its measured cell errors for
+// learned shading (no reference counterpart).
+class RadianceNetwork {
+public:
+    RadianceNetwork(Context& ctx, float lodMax, uint64_t seed) : ctx_(ctx) {
+        ctx_.check(hvx_mlp_create(ctx_.get(), lodMax, seed, &mlp_), "hvx_mlp_create");
+    }
+    ~RadianceNetwork() {
+        if (mlp_) (void)hvx_mlp_destroy(mlp_);
+    }
+    RadianceNetwork(const RadianceNetwork&) = delete;
+    RadianceNetwork& operator=(const RadianceNetwork&) = delete;
+    // samples of the chain, then `steps` Adam steps of `batch` samples
+    void Train(const ResourcesIBL& ibl, uint32_t samples, uint32_t batch, uint32_t steps, float lr, uint64_t seed,
+               hipStream_t stream = nullptr) {
+        inputs_.resize((size_t)samples * 4);
+        targets_.resize((size_t)samples * 4);
+        ctx_.check(hvx_mlp_sample(mlp_, ibl.specularCubemap_.data(), ibl.specularSize_, ibl.specularMips_, seed,
+                                  samples, inputs_.data(), targets_.data(), stream),
+                   "hvx_mlp_sample");
+        ctx_.check(hvx_mlp_train(mlp_, inputs_.data(), targets_.data(), samples, batch, steps, lr, nullptr, stream),
+                   "hvx_mlp_train");
+    }
+    // the fallback mask: measured error per (lod bin, face, cell)
+    void MeasureCells(const ResourcesIBL& ibl, uint32_t cellFace, uint32_t lodBins, hipStream_t stream = nullptr) {
+        cellFace_ = cellFace;
+        lodBins_ = lodBins;
+        cellErr_.resize((size_t)lodBins * 6 * cellFace * cellFace);
+        ctx_.check(hvx_mlp_cell_errors(mlp_, ibl.specularCubemap_.data(), ibl.specularSize_, ibl.specularMips_, cellFace,
+                                       lodBins, cellErr_.data(), stream),
+                   "hvx_mlp_cell_errors");
+    }
+    const hvx_mlp* get() const { return mlp_; }
+    const float* cellErrors() const { return cellErr_.data(); }
+    uint32_t cellFace() const { return cellFace_; }
+    uint32_t lodBins() const { return lodBins_; }
+
+private:
+    Context& ctx_;
+    hvx_mlp* mlp_ = nullptr;
+    DeviceBuffer<float> inputs_, targets_, cellErr_;
+    uint32_t cellFace_ = 0, lodBins_ = 0;
+};
+
 // Builds every map of ResourcesIBL from an equirect (ResourcesIBL::Create).
 inline void CreateIBL(Context& ctx, ResourcesIBL& ibl, const float* d_equirect, uint32_t w, uint32_t h,
                       hipStream_t stream = nullptr) {
@@ -337,13 +448,28 @@ public:
         : PipelineBase(ctx), frags_(frags), lights_(lights), cf_(cf), ibl_(ibl), rgb_(rgb) {}
     void SetClusterForwardUBO(const hvx_cf_ubo& ubo) { ubo_ = ubo; }
     void SetPBRPushConstants(const hvx_pbr_pc& pc) { pc_ = pc; }
+    // Learned specular IBL (hvx_cf_shade_learned) with fallback tolerance
+    // `tol`; nullptr returns to the prefiltered map. served (nullable) counts
+    // the fragments the network served.
+    void SetLearned(const RadianceNetwork* net, float tol = 2e-3f, uint32_t* d_served = nullptr) {
+        net_ = net;
+        tol_ = tol;
+        served_ = d_served;
+    }
     void FillCommandBuffer(hipStream_t stream) override {
         const hvx_frag_soa f = frags_->view();
         const hvx_ibl_maps m = ibl_->maps();
-        ctx_.check(hvx_cf_shade(ctx_.get(), &f, &cameraUBO_, &ubo_, &pc_, lights_->device(), lights_->GetLightCount(),
-                                cf_->lightCellsBuffer_.data(), cf_->lightIndicesBuffer_.data(), &m, rgb_->data(),
-                                stream),
-                   "hvx_cf_shade");
+        if (net_)
+            ctx_.check(hvx_cf_shade_learned(ctx_.get(), &f, &cameraUBO_, &ubo_, &pc_, lights_->device(),
+                                            lights_->GetLightCount(), cf_->lightCellsBuffer_.data(),
+                                            cf_->lightIndicesBuffer_.data(), &m, net_->get(), net_->cellErrors(),
+                                            net_->cellFace(), net_->lodBins(), tol_, rgb_->data(), served_, stream),
+                       "hvx_cf_shade_learned");
+        else
+            ctx_.check(hvx_cf_shade(ctx_.get(), &f, &cameraUBO_, &ubo_, &pc_, lights_->device(),
+                                    lights_->GetLightCount(), cf_->lightCellsBuffer_.data(),
+                                    cf_->lightIndicesBuffer_.data(), &m, rgb_->data(), stream),
+                       "hvx_cf_shade");
     }
 
 private:
@@ -354,6 +480,36 @@ private:
     DeviceBuffer<float>* rgb_;
     hvx_cf_ubo ubo_{};
     hvx_pbr_pc pc_{1.75f, 0.01f, 1.0f, 0.1f, 0.01f};  // UIData.h:22-29, AppPBRClusterForward.cpp:21
+    const RadianceNetwork* net_ = nullptr;
+    float tol_ = 2e-3f;
+    uint32_t* served_ = nullptr;
+};
+
+// The post step after shading (Shaders/Common/Tonemap.frag:11-94): tone
+// mapping of the planar FP32 frame into packed BGRA8 (and, optionally, the
+// mapped FP32 planes).
+class PipelineTonemap final : public PipelineBase {
+public:
+    PipelineTonemap(Context& ctx, const DeviceBuffer<float>* rgb, uint32_t width, uint32_t height,
+                    DeviceBuffer<uint32_t>* bgra8, DeviceBuffer<float>* mapped = nullptr)
+        : PipelineBase(ctx), rgb_(rgb), w_(width), h_(height), bgra8_(bgra8), mapped_(mapped) {}
+    void SetOperator(hvx_tonemap_op op, bool gamma = true) {
+        op_ = op;
+        gamma_ = gamma;
+    }
+    void FillCommandBuffer(hipStream_t stream) override {
+        ctx_.check(hvx_post_tonemap(ctx_.get(), rgb_->data(), w_, h_, (int)op_, gamma_ ? 1 : 0,
+                                    mapped_ ? mapped_->data() : nullptr, bgra8_ ? bgra8_->data() : nullptr, stream),
+                   "hvx_post_tonemap");
+    }
+
+private:
+    const DeviceBuffer<float>* rgb_;
+    uint32_t w_, h_;
+    DeviceBuffer<uint32_t>* bgra8_;
+    DeviceBuffer<float>* mapped_;
+    hvx_tonemap_op op_ = HVX_TONEMAP_REINHARD;
+    bool gamma_ = true;
 };
 
 // Captures the stages' FillCommandBuffer calls (in order) into one hipGraph

@@ -75,3 +75,60 @@ def test_frame_driver_matches_c_abi(ctx, tmp_path):
     got = np.fromfile(prefix + ".rgb", np.float32)
     assert np.array_equal(got, rgb.cpu().numpy())
     assert np.array_equal(got, np.fromfile(prefix + ".rgb_graph", np.float32))
+
+
+def test_frame_driver_contraction_and_tonemap(ctx, tmp_path):
+    """The C++ layer's contraction-filter stage and tone-mapping post step:
+    the driver's specular chain (FP16 contraction), shaded frame and BGRA8
+    ACES output equal the same stages run through the C ABI from Python, and
+    the graph replay (culling, shading, tone mapping) equals the eager frame."""
+    import torch
+    prefix = str(tmp_path / "frame")
+    cmd = [BIN, "--width", str(W), "--height", str(H), "--grid", *map(str, GRID), "--lights", str(NL), "--cap",
+           str(CAP), "--cube", str(CUBE), "--diffuse", str(DIFF), "--specular", str(SPEC), "--mips", str(MIPS),
+           "--lut", str(LUT), "--frames", "2", "--warmup", "1", "--contract", "0", "--tonemap", "1", "--gamma", "1",
+           "--dump", prefix]
+    print("running", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    print(info)
+    assert info["graph_matches_eager"] is True and info["contract"] == 0 and info["tonemap"] == 1
+
+    desc, cam, ubo, ivp = camera(W, H, GRID)
+    lights = abi.scene_lights(1001, NL)
+    fr = abi.scene_fragments(1002, desc, ivp, threads=4)
+    eq = abi.scene_equirect(77, 2 * CUBE, CUBE, threads=4)
+    dev = torch.device("cuda", 0)
+    levels = abi.mip_count(CUBE)
+    d_cube = torch.empty(abi.cube_chain_floats(CUBE, levels), dtype=torch.float32, device=dev)
+    ctx.equirect_to_cube(to_dev(eq), 2 * CUBE, CUBE, d_cube, CUBE)
+    ctx.cube_mips(d_cube, CUBE, levels)
+    d_diff = torch.empty(6 * DIFF * DIFF * 4, dtype=torch.float32, device=dev)
+    d_spec = torch.empty(abi.cube_chain_floats(SPEC, MIPS), dtype=torch.float32, device=dev)
+    d_lut = torch.empty(LUT * LUT * 2, dtype=torch.float32, device=dev)
+    ctx.filter(abi.FILTER_DIFFUSE, d_cube, CUBE, d_diff, DIFF, 1, 0)
+    plan = abi.ContractPlan(ctx, abi.FILTER_SPECULAR, CUBE, SPEC, MIPS)
+    plan.filter(d_cube, levels, d_spec)
+    ctx.brdf_lut(LUT, LUT, 1024, d_lut)
+    Cn = GRID[0] * GRID[1] * GRID[2]
+    d_aabb = torch.empty(Cn * 8, dtype=torch.float32, device=dev)
+    ctx.build_aabbs(ubo, d_aabb)
+    d_lights = to_dev(lights.view(np.uint8))
+    d_cells = torch.zeros(Cn * 2, dtype=torch.int32, device=dev)
+    d_idx = torch.zeros(Cn * CAP, dtype=torch.int32, device=dev)
+    ctx.cull_lights(ubo, d_aabb, d_lights, NL, CAP, d_cells, d_idx, Cn * CAP)
+    d_fr = {k: to_dev(v) for k, v in fr.items()}
+    m = abi.ibl_maps(d_spec, SPEC, MIPS, d_diff, DIFF, d_lut, LUT, LUT)
+    rgb = torch.empty(3 * W * H, dtype=torch.float32, device=dev)
+    ctx.shade(abi.frag_soa(d_fr, W, H, ivp), cam, ubo, abi.pbr_defaults(), d_lights, NL, d_cells, d_idx, m, rgb)
+    bgra = torch.empty(W * H, dtype=torch.int32, device=dev)
+    ctx.tonemap(rgb, W, H, 1, True, None, bgra)
+    torch.cuda.synchronize()
+    plan.close()
+    assert np.array_equal(np.fromfile(prefix + ".spec", np.float32), d_spec.cpu().numpy())
+    assert np.array_equal(np.fromfile(prefix + ".rgb", np.float32), rgb.cpu().numpy())
+    got = np.fromfile(prefix + ".bgra8", np.uint32)
+    assert np.array_equal(got, bgra.cpu().numpy().view(np.uint32))
+    assert np.array_equal(got, np.fromfile(prefix + ".bgra8_graph", np.uint32))
+    assert len(np.unique(got)) > 100  # a real image, not a constant
