@@ -66,7 +66,8 @@ struct Probes {
     const float4* const* chains;
     float4* const* outs;
     uint64_t delta, pstride;
-    uint32_t probe_fast;  // GEMMs: grid x = probe, y = workgroup (probes sharing weights dispatch together)
+    uint32_t probe_fast;  // GEMMs: grid x = probe group, y = workgroup (groups sharing weights dispatch together)
+    uint32_t n;           // probes in the call
 };
 // (workgroup, probe) of a GEMM launch
 __device__ __forceinline__ uint2 gemm_block(const Probes& Q) {
@@ -315,6 +316,10 @@ __global__ __launch_bounds__(256) void near_scale_kernel(NearGeom G, uint32_t Sf
 constexpr int kMaxJobs = 2 * hvx::kContractMaxMips;
 constexpr uint32_t kFaceMaxBlocks = 32;
 constexpr uint32_t kMaxProbes = 4096;  // probes of one multi-probe call (grid dimension y)
+#ifndef HVX_PROBE_GROUP
+#define HVX_PROBE_GROUP 4
+#endif
+constexpr int kProbeGroup = HVX_PROBE_GROUP;  // probes per wave in multi-probe GEMMs
 
 // partial[f][b]: largest RGB magnitude of block b's share of face f
 __global__ __launch_bounds__(256) void facemax_kernel(Probes Q, size_t level_off, uint32_t S, float* partial) {
@@ -474,10 +479,15 @@ struct DenseTable {
 
 // Dense: a wave owns two row tiles (32 rows) x 32 columns over one K slice;
 // partial sums go to part[slice][row][32] and are reduced in slice order.
+// PG probes per wave (grid dimension: probe group): each weight fragment is
+// loaded once and multiplied into every probe of the group; each probe's
+// accumulation is the same sequence as with PG = 1.
+template <int PG>
 __global__ __launch_bounds__(256) void dense_gemm_kernel(Probes Q, DenseTable T) {
     const uint2 wb = gemm_block(Q);
-    const uint32_t gw = wb.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63, pr = wb.y;
+    const uint32_t gw = wb.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63, p0 = wb.y * PG;
     if (gw >= T.waves) return;  // uniform per wave; the kernel has no barriers
+    const uint32_t np = min((uint32_t)PG, Q.n - p0);
     uint32_t jj = 0;
     while (jj + 1 < T.n && gw >= T.j[jj + 1].wave_begin) ++jj;
     const DenseJob& J = T.j[jj];
@@ -489,48 +499,64 @@ __global__ __launch_bounds__(256) void dense_gemm_kernel(Probes Q, DenseTable T)
     const uint32_t i = lane & 15, g = lane >> 4;
     const half8* w0 = J.W + (size_t)rt0 * ks * 64 + lane;
     const half8* w1 = has1 ? w0 + (size_t)ks * 64 : w0;
-    const half8* rb = scr(Q, J.R, pr) + g * 32 + i;
-    f32x4 a00 = {0.0f, 0.0f, 0.0f, 0.0f}, a01 = a00, a10 = a00, a11 = a00;
+    const half8* rb[PG];
+#pragma unroll
+    for (int q = 0; q < PG; ++q) rb[q] = scr(Q, J.R, p0 + ((uint32_t)q < np ? q : 0)) + g * 32 + i;
+    f32x4 acc[PG][4];
+#pragma unroll
+    for (int q = 0; q < PG; ++q)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc[q][k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     if (J.w_lo) {  // split BF16: hi*hi + hi*lo + lo*hi
         const bf16x8* v0 = reinterpret_cast<const bf16x8*>(w0);
         const bf16x8* v1 = reinterpret_cast<const bf16x8*>(w1);
-        const bf16x8* rv = reinterpret_cast<const bf16x8*>(rb);
         for (uint32_t s = kb; s < ke; ++s) {
-            const size_t o = (size_t)s * 64, q = (size_t)s * 128;
+            const size_t o = (size_t)s * 64, qo = (size_t)s * 128;
             const bf16x8 x0 = v0[o], x1 = v1[o], y0 = v0[o + J.w_lo], y1 = v1[o + J.w_lo];
-            const bf16x8 b0 = rv[q], b1 = rv[q + 16], c0 = rv[q + J.r_lo], c1 = rv[q + 16 + J.r_lo];
-            a00 = hvx::mfma16x16x32(x0, b0, a00);
-            a00 = hvx::mfma16x16x32(x0, c0, a00);
-            a00 = hvx::mfma16x16x32(y0, b0, a00);
-            a01 = hvx::mfma16x16x32(x0, b1, a01);
-            a01 = hvx::mfma16x16x32(x0, c1, a01);
-            a01 = hvx::mfma16x16x32(y0, b1, a01);
-            a10 = hvx::mfma16x16x32(x1, b0, a10);
-            a10 = hvx::mfma16x16x32(x1, c0, a10);
-            a10 = hvx::mfma16x16x32(y1, b0, a10);
-            a11 = hvx::mfma16x16x32(x1, b1, a11);
-            a11 = hvx::mfma16x16x32(x1, c1, a11);
-            a11 = hvx::mfma16x16x32(y1, b1, a11);
+#pragma unroll
+            for (int q = 0; q < PG; ++q) {
+                const bf16x8* rv = reinterpret_cast<const bf16x8*>(rb[q]);
+                const bf16x8 b0 = rv[qo], b1 = rv[qo + 16], c0 = rv[qo + J.r_lo], c1 = rv[qo + 16 + J.r_lo];
+                acc[q][0] = hvx::mfma16x16x32(x0, b0, acc[q][0]);
+                acc[q][0] = hvx::mfma16x16x32(x0, c0, acc[q][0]);
+                acc[q][0] = hvx::mfma16x16x32(y0, b0, acc[q][0]);
+                acc[q][1] = hvx::mfma16x16x32(x0, b1, acc[q][1]);
+                acc[q][1] = hvx::mfma16x16x32(x0, c1, acc[q][1]);
+                acc[q][1] = hvx::mfma16x16x32(y0, b1, acc[q][1]);
+                acc[q][2] = hvx::mfma16x16x32(x1, b0, acc[q][2]);
+                acc[q][2] = hvx::mfma16x16x32(x1, c0, acc[q][2]);
+                acc[q][2] = hvx::mfma16x16x32(y1, b0, acc[q][2]);
+                acc[q][3] = hvx::mfma16x16x32(x1, b1, acc[q][3]);
+                acc[q][3] = hvx::mfma16x16x32(x1, c1, acc[q][3]);
+                acc[q][3] = hvx::mfma16x16x32(y1, b1, acc[q][3]);
+            }
         }
     } else {
         for (uint32_t s = kb; s < ke; ++s) {
             const half8 x0 = w0[(size_t)s * 64], x1 = w1[(size_t)s * 64];
-            const half8 b0 = rb[(size_t)s * 128], b1 = rb[(size_t)s * 128 + 16];
-            a00 = hvx::mfma16x16x32(x0, b0, a00);
-            a01 = hvx::mfma16x16x32(x0, b1, a01);
-            a10 = hvx::mfma16x16x32(x1, b0, a10);
-            a11 = hvx::mfma16x16x32(x1, b1, a11);
+#pragma unroll
+            for (int q = 0; q < PG; ++q) {
+                const half8 b0 = rb[q][(size_t)s * 128], b1 = rb[q][(size_t)s * 128 + 16];
+                acc[q][0] = hvx::mfma16x16x32(x0, b0, acc[q][0]);
+                acc[q][1] = hvx::mfma16x16x32(x0, b1, acc[q][1]);
+                acc[q][2] = hvx::mfma16x16x32(x1, b0, acc[q][2]);
+                acc[q][3] = hvx::mfma16x16x32(x1, b1, acc[q][3]);
+            }
         }
     }
-    float* p = scr(Q, J.part, pr) + (size_t)slice * rt * 16 * 32;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const uint32_t row0 = rt0 * 16 + 4 * g + r;
-        p[(size_t)row0 * 32 + i] = a00[r];
-        p[(size_t)row0 * 32 + 16 + i] = a01[r];
-        if (has1) {
-            p[(size_t)(row0 + 16) * 32 + i] = a10[r];
-            p[(size_t)(row0 + 16) * 32 + 16 + i] = a11[r];
+    for (int q = 0; q < PG; ++q) {
+        if ((uint32_t)q >= np) break;
+        float* p = scr(Q, J.part, p0 + q) + (size_t)slice * rt * 16 * 32;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const uint32_t row0 = rt0 * 16 + 4 * g + r;
+            p[(size_t)row0 * 32 + i] = acc[q][0][r];
+            p[(size_t)row0 * 32 + 16 + i] = acc[q][1][r];
+            if (has1) {
+                p[(size_t)(row0 + 16) * 32 + i] = acc[q][2][r];
+                p[(size_t)(row0 + 16) * 32 + 16 + i] = acc[q][3][r];
+            }
         }
     }
 }
@@ -591,11 +617,10 @@ constexpr uint32_t kNearWaves = 8;  // waves per 16x16 output tile (two row tile
 constexpr uint32_t kNearParts = HVX_NEAR_PARTS;  // workgroups per tile
 static_assert(kNearWaves % kNearParts == 0, "a tile splits into whole waves");
 
-template <bool SPLIT>
+template <bool SPLIT, int PG>
 __global__ __launch_bounds__(512) void near_gemm_kernel(Probes Q, NearTable T, const float* sc) {
     const uint2 wb = gemm_block(Q);
-    const uint32_t pr = wb.y;
-    sc = scr(Q, sc, pr);
+    const uint32_t p0 = wb.y * PG, np = min((uint32_t)PG, Q.n - p0);
     uint32_t jj = 0;
     // workgroup b owns half b & 1 (eight row tiles) of tile b >> 1: finer
     // work units than whole tiles balance the unequal mips across the CUs
@@ -605,7 +630,11 @@ __global__ __launch_bounds__(512) void near_gemm_kernel(Probes Q, NearTable T, c
     const uint32_t tile = gt - J.tile_begin, lane = threadIdx.x & 63;
     const uint32_t wave = (threadIdx.x >> 6) + (wb.x % kNearParts) * (kNearWaves / kNearParts);
     const NearTile Tl = J.tiles[tile];
-    const half8* Pp = scr(Q, J.P, pr);
+    // PG probes per wave: each weight fragment is loaded once for the group;
+    // each probe's accumulation is the same sequence as with PG = 1
+    const half8* Pp[PG];
+#pragma unroll
+    for (int q = 0; q < PG; ++q) Pp[q] = scr(Q, J.P, p0 + ((uint32_t)q < np ? q : 0));
     const uint32_t wc = Tl.w / 32, KS = Tl.h * wc;
     const uint32_t i = lane & 15, g = lane >> 4, rt0 = 2 * wave;
     // weights of a tile stored K-step major: step s of all 16 row tiles is one
@@ -614,35 +643,45 @@ __global__ __launch_bounds__(512) void near_gemm_kernel(Probes Q, NearTable T, c
     const half8* w0 = J.W + Tl.woff + (size_t)rt0 * kRow + lane;
     const half8* w1 = w0 + kRow;
     const uint32_t xb = (uint32_t)(Tl.x0 + (int)J.B) / 8 + g;
-    f32x4 a00 = {0.0f, 0.0f, 0.0f, 0.0f}, a01 = a00, a10 = a00, a11 = a00;
+    f32x4 acc[PG][4];
+#pragma unroll
+    for (int q = 0; q < PG; ++q)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc[q][k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     uint32_t r = 0, cc = 0;
     for (uint32_t s = 0; s < KS; ++s) {
         const size_t pidx = (((size_t)(Tl.y0 + (int)J.B + (int)r) * J.Pw8 + xb + cc * 4) * 32) + i;
         if constexpr (SPLIT) {
             const bf16x8* wv = reinterpret_cast<const bf16x8*>(w0);
-            const bf16x8* pv = reinterpret_cast<const bf16x8*>(Pp);
             const size_t o0 = (size_t)s * kStep, o1 = o0 + kRow;
             const bf16x8 x0 = wv[o0], x1 = wv[o1], y0 = wv[o0 + J.w_lo], y1 = wv[o1 + J.w_lo];
-            const bf16x8 b0 = pv[pidx], b1 = pv[pidx + 16], c0 = pv[pidx + J.p_lo], c1 = pv[pidx + 16 + J.p_lo];
-            a00 = hvx::mfma16x16x32(x0, b0, a00);
-            a00 = hvx::mfma16x16x32(x0, c0, a00);
-            a00 = hvx::mfma16x16x32(y0, b0, a00);
-            a01 = hvx::mfma16x16x32(x0, b1, a01);
-            a01 = hvx::mfma16x16x32(x0, c1, a01);
-            a01 = hvx::mfma16x16x32(y0, b1, a01);
-            a10 = hvx::mfma16x16x32(x1, b0, a10);
-            a10 = hvx::mfma16x16x32(x1, c0, a10);
-            a10 = hvx::mfma16x16x32(y1, b0, a10);
-            a11 = hvx::mfma16x16x32(x1, b1, a11);
-            a11 = hvx::mfma16x16x32(x1, c1, a11);
-            a11 = hvx::mfma16x16x32(y1, b1, a11);
+#pragma unroll
+            for (int q = 0; q < PG; ++q) {
+                const bf16x8* pv = reinterpret_cast<const bf16x8*>(Pp[q]);
+                const bf16x8 b0 = pv[pidx], b1 = pv[pidx + 16], c0 = pv[pidx + J.p_lo], c1 = pv[pidx + 16 + J.p_lo];
+                acc[q][0] = hvx::mfma16x16x32(x0, b0, acc[q][0]);
+                acc[q][0] = hvx::mfma16x16x32(x0, c0, acc[q][0]);
+                acc[q][0] = hvx::mfma16x16x32(y0, b0, acc[q][0]);
+                acc[q][1] = hvx::mfma16x16x32(x0, b1, acc[q][1]);
+                acc[q][1] = hvx::mfma16x16x32(x0, c1, acc[q][1]);
+                acc[q][1] = hvx::mfma16x16x32(y0, b1, acc[q][1]);
+                acc[q][2] = hvx::mfma16x16x32(x1, b0, acc[q][2]);
+                acc[q][2] = hvx::mfma16x16x32(x1, c0, acc[q][2]);
+                acc[q][2] = hvx::mfma16x16x32(y1, b0, acc[q][2]);
+                acc[q][3] = hvx::mfma16x16x32(x1, b1, acc[q][3]);
+                acc[q][3] = hvx::mfma16x16x32(x1, c1, acc[q][3]);
+                acc[q][3] = hvx::mfma16x16x32(y1, b1, acc[q][3]);
+            }
         } else {
             const half8 x0 = w0[(size_t)s * kStep], x1 = w1[(size_t)s * kStep];
-            const half8 b0 = Pp[pidx], b1 = Pp[pidx + 16];
-            a00 = hvx::mfma16x16x32(x0, b0, a00);
-            a01 = hvx::mfma16x16x32(x0, b1, a01);
-            a10 = hvx::mfma16x16x32(x1, b0, a10);
-            a11 = hvx::mfma16x16x32(x1, b1, a11);
+#pragma unroll
+            for (int q = 0; q < PG; ++q) {
+                const half8 b0 = Pp[q][pidx], b1 = Pp[q][pidx + 16];
+                acc[q][0] = hvx::mfma16x16x32(x0, b0, acc[q][0]);
+                acc[q][1] = hvx::mfma16x16x32(x0, b1, acc[q][1]);
+                acc[q][2] = hvx::mfma16x16x32(x1, b0, acc[q][2]);
+                acc[q][3] = hvx::mfma16x16x32(x1, b1, acc[q][3]);
+            }
         }
         if (++cc == wc) {
             cc = 0;
@@ -651,19 +690,24 @@ __global__ __launch_bounds__(512) void near_gemm_kernel(Probes Q, NearTable T, c
     }
     const uint32_t So = J.So, X0 = (tile % J.te) * 16, Y0 = (tile / J.te) * 16;
     const size_t So2 = (size_t)So * So;
-    float* out = reinterpret_cast<float*>(probe_out(Q, pr) + J.out_off);
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const uint32_t Y = Y0 + rt0 + h;
+    for (int q = 0; q < PG; ++q) {
+        if ((uint32_t)q >= np) break;
+        float* out = reinterpret_cast<float*>(probe_out(Q, p0 + q) + J.out_off);
+        const float* sq = scr(Q, sc, p0 + q);
 #pragma unroll
-        for (int nb = 0; nb < 2; ++nb) {
-            const f32x4 a = h == 0 ? (nb == 0 ? a00 : a01) : (nb == 0 ? a10 : a11);
-            const uint32_t col = nb * 16 + i, f = col >> 2, c = col & 3;
-            if (col >= 24) continue;
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t Y = Y0 + rt0 + h;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t X = X0 + 4 * g + q, o = Y * So + X;
-                out[((size_t)f * So2 + o) * 4 + c] = c < 3 ? a[q] * J.rowscale[o] * sc[6 + f] : 1.0f;
+            for (int nb = 0; nb < 2; ++nb) {
+                const f32x4 a = acc[q][2 * h + nb];
+                const uint32_t col = nb * 16 + i, f = col >> 2, c = col & 3;
+                if (col >= 24) continue;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const uint32_t X = X0 + 4 * g + e, o = Y * So + X;
+                    out[((size_t)f * So2 + o) * 4 + c] = c < 3 ? a[e] * J.rowscale[o] * sq[6 + f] : 1.0f;
+                }
             }
         }
     }
@@ -1006,23 +1050,35 @@ int launch_call(hvx_contract_plan* P, const Probes& Q, uint32_t n, uint32_t chai
         hipLaunchKernelGGL(stage_all_kernel, dim3((P->stage.total + 255) / 256, n), dim3(256), 0, s, Q, P->stage,
                            P->sc);
     }
-    // GEMMs with several probes: probe index fastest, so the probes' workgroups
-    // for one weight block run together and share it in the caches
+    // GEMMs with several probes: groups of kProbeGroup probes per wave share
+    // each weight fragment, and the group index is the fastest grid dimension,
+    // so groups reading one weight block run together and share it in the caches
     Probes G = Q;
     const uint32_t dense_wgs = (P->dense.waves + 3) / 4, near_wgs = P->near.tiles * kNearParts;
-    G.probe_fast = n > 1 && std::max(dense_wgs, near_wgs) <= 65535u;
+    const bool grouped = n > 1;
+    const uint32_t groups = grouped ? (n + kProbeGroup - 1) / kProbeGroup : 1;
+    G.probe_fast = grouped && std::max(dense_wgs, near_wgs) <= 65535u;
     if (P->dense.n) {
-        hipLaunchKernelGGL(dense_gemm_kernel, G.probe_fast ? dim3(n, dense_wgs) : dim3(dense_wgs, n), dim3(256), 0, s,
-                           G, P->dense);
+        const dim3 grid = G.probe_fast ? dim3(groups, dense_wgs) : dim3(dense_wgs, groups);
+        if (grouped)
+            hipLaunchKernelGGL(dense_gemm_kernel<kProbeGroup>, grid, dim3(256), 0, s, G, P->dense);
+        else
+            hipLaunchKernelGGL(dense_gemm_kernel<1>, grid, dim3(256), 0, s, G, P->dense);
         hipLaunchKernelGGL(dense_epilogue_kernel, dim3((P->dense.epi + 255) / 256, n), dim3(256), 0, s, Q, P->dense,
                            P->sc);
     }
     if (P->near.n) {
-        const dim3 grid = G.probe_fast ? dim3(n, near_wgs) : dim3(near_wgs, n), block(64 * kNearWaves / kNearParts);
-        if (P->precision == HVX_PRECISION_BF16_SPLIT)
-            hipLaunchKernelGGL(near_gemm_kernel<true>, grid, block, 0, s, G, P->near, P->sc);
+        const dim3 grid = G.probe_fast ? dim3(groups, near_wgs) : dim3(near_wgs, groups),
+                   block(64 * kNearWaves / kNearParts);
+        const bool split = P->precision == HVX_PRECISION_BF16_SPLIT;
+        if (grouped && split)
+            hipLaunchKernelGGL((near_gemm_kernel<true, kProbeGroup>), grid, block, 0, s, G, P->near, P->sc);
+        else if (grouped)
+            hipLaunchKernelGGL((near_gemm_kernel<false, kProbeGroup>), grid, block, 0, s, G, P->near, P->sc);
+        else if (split)
+            hipLaunchKernelGGL((near_gemm_kernel<true, 1>), grid, block, 0, s, G, P->near, P->sc);
         else
-            hipLaunchKernelGGL(near_gemm_kernel<false>, grid, block, 0, s, G, P->near, P->sc);
+            hipLaunchKernelGGL((near_gemm_kernel<false, 1>), grid, block, 0, s, G, P->near, P->sc);
         hipLaunchKernelGGL(far_add_kernel, dim3((P->near.texels + 255) / 256, n), dim3(256), 0, s, Q, P->near);
     }
     if (P->delta_mip != ~0u) {
@@ -1241,7 +1297,7 @@ int hvx_ibl_filter_contract(hvx_contract_plan* P, const float* d_chain, uint32_t
     hvx_ctx* ctx = P->ctx;
     if (!d_chain || !d_out) return pfail(ctx, HVX_ERR_NULL, "NULL chain or output");
     if (((uintptr_t)d_chain | (uintptr_t)d_out) % 16) return pfail(ctx, HVX_ERR_ALIGN, "chains need 16-byte alignment");
-    const Probes Q{reinterpret_cast<const float4*>(d_chain), reinterpret_cast<float4*>(d_out), nullptr, nullptr, 0, 0, 0};
+    const Probes Q{reinterpret_cast<const float4*>(d_chain), reinterpret_cast<float4*>(d_out), nullptr, nullptr, 0, 0, 0, 1};
     return launch_call(P, Q, 1, chain_levels, (hipStream_t)stream, "hvx_ibl_filter_contract");
 }
 
@@ -1277,7 +1333,7 @@ int hvx_ibl_filter_contract_multi(hvx_contract_plan* P, uint32_t n, const float*
     const Probes Q{nullptr, nullptr, reinterpret_cast<const float4* const*>(d_chains),
                    reinterpret_cast<float4* const*>(d_outs),
                    (uint64_t)(reinterpret_cast<uintptr_t>(P->mscratch) - reinterpret_cast<uintptr_t>(P->scratch)),
-                   (uint64_t)P->scratch_bytes, 0};
+                   (uint64_t)P->scratch_bytes, 0, n};
     return launch_call(P, Q, n, chain_levels, (hipStream_t)stream, "hvx_ibl_filter_contract_multi");
 }
 
