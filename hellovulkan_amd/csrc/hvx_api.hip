@@ -147,7 +147,8 @@ int hvx_ctx_destroy(hvx_ctx* ctx) {
     if (!ctx) return HVX_ERR_NULL;
     if (bind_device(ctx) == HVX_OK) {
         (void)hipDeviceSynchronize();
-        void* bufs[] = {ctx->cull_lists, ctx->cull_meta, ctx->light_vs, ctx->ggx_table, ctx->shade_lights};
+        void* bufs[] = {ctx->cull_lists,   ctx->cull_meta,    ctx->light_vs,
+                        ctx->ggx_table,    ctx->shade_lights, ctx->shade_bins};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
     }
@@ -249,8 +250,13 @@ int shade_impl(hvx_ctx* ctx, const hvx_frag_soa* fr, const hvx_camera_ubo* cam, 
         (e = hvx::ensure_scratch(ctx, &ctx->shade_lights, &ctx->shade_lights_bytes, (size_t)n_lights * 32)) !=
             hipSuccess)
         return hip_fail(ctx, e, "shading light table allocation");
+    const bool binned = hvx::shade_binned(*ubo);
+    if (binned && (e = hvx::ensure_scratch(ctx, &ctx->shade_bins, &ctx->shade_bins_bytes,
+                                           hvx::shade_bin_scratch_bytes(*fr, *ubo))) != hipSuccess)
+        return hip_fail(ctx, e, "shading bin allocation");
     e = hvx::launch_shade(*fr, *cam, *ubo, *pc, d_lights, n_lights, d_cells, d_indices, *ibl, d_rgb,
-                          ctx->shade_exact != 0, static_cast<float4*>(ctx->shade_lights), learned, (hipStream_t)stream);
+                          ctx->shade_exact != 0, static_cast<float4*>(ctx->shade_lights),
+                          binned ? static_cast<uint32_t*>(ctx->shade_bins) : nullptr, learned, (hipStream_t)stream);
     return e == hipSuccess ? HVX_OK : hip_fail(ctx, e, "shading launch");
 }
 }  // namespace

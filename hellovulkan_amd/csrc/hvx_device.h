@@ -63,15 +63,15 @@ __device__ __forceinline__ float sqrt_rn_normal(float x) {
 }
 
 // Correctly rounded reciprocal of a normal b with |b| in [2^-125, 2^125]:
-// hardware estimate plus two Newton-Raphson corrections with fused
-// multiply-add, y' = y + y (1 - b y) (Markstein). Verified equal to 1.0f / b on
-// every float of [1, 2) on gfx950 (the map b -> 1/b commutes with exact power
-// of two scaling in that range).
+// hardware estimate (within 1 ulp) plus one Newton-Raphson correction with
+// fused multiply-add, y' = y + y (1 - b y): the fused residual is rounded
+// once, so y' carries only an O(ulp^2) error before its single rounding.
+// Verified equal to 1.0f / b on every float of [1, 2) and of three more
+// binades on gfx950 (the map b -> 1/b commutes with exact power of two
+// scaling in that range; tests/test_gpu_fastmath.py).
 __device__ __forceinline__ float rcp_rn_normal(float b) {
-    float y = __builtin_amdgcn_rcpf(b);
-    float e = __builtin_fmaf(-b, y, 1.0f);
-    y = __builtin_fmaf(e, y, y);
-    e = __builtin_fmaf(-b, y, 1.0f);
+    const float y = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, y, 1.0f);
     return __builtin_fmaf(e, y, y);
 }
 

@@ -26,6 +26,8 @@ struct hvx_ctx {
     size_t ggx_table_bytes = 0;
     void* shade_lights = nullptr; // float4 [2 * lights]: compact shading light table
     size_t shade_lights_bytes = 0;
+    void* shade_bins = nullptr;   // uint32 [fragments + X + Y + 2]: cluster-binned fragment order
+    size_t shade_bins_bytes = 0;
     // options (hvx_ctx_set_option)
     int shade_exact = 0;
 };
@@ -108,8 +110,12 @@ hipError_t launch_tonemap(const float* d_rgb, uint32_t n, int op, int gamma, flo
 hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const hvx_cf_ubo& ubo,
                         const hvx_pbr_pc& pc, const hvx_light* d_lights, uint32_t n_lights,
                         const hvx_light_cell* d_cells, const uint32_t* d_indices, const hvx_ibl_maps& ibl,
-                        float* d_rgb, bool exact, float4* light_scratch, const LearnedShade* learned,
-                        hipStream_t s);
+                        float* d_rgb, bool exact, float4* light_scratch, uint32_t* bin_scratch,
+                        const LearnedShade* learned, hipStream_t s);
+// Whether shading bins fragments by cluster for this grid, and the scratch
+// it needs for that (the fragment order and the tile bounds).
+bool shade_binned(const hvx_cf_ubo& ubo);
+size_t shade_bin_scratch_bytes(const hvx_frag_soa& fr, const hvx_cf_ubo& ubo);
 
 // --- IBL (ibl_kernels.hip)
 hipError_t launch_brdf_lut(uint32_t w, uint32_t h, uint32_t n, float* d_lut, hipStream_t s);

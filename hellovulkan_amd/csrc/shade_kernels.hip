@@ -5,26 +5,31 @@
 // lookup, per-light loop, ambient), ClusteredForward/Radiance.glsl:2-56,
 // PBR/PBRHeader.glsl:16-64, PBR/Ambient.glsl:1-29.
 //
-// Scheduling. A workgroup owns 1024 consecutive fragments. It first resolves
-// every fragment's cluster and light count, then counting-sorts the
-// fragments by light count in LDS and hands them to lanes in that order, so
-// the 64 lanes of a wave walk light lists of nearly equal length (with depth
-// drawn independently per pixel, neighbouring pixels sit in unrelated
-// clusters; unsorted, a wave runs to its longest list). Each fragment is still
-// shaded start to finish by one lane, summing its lights in list order.
+// Scheduling. Fragments are first binned by cluster (bin_kernel): one
+// workgroup per screen tile counting-sorts the tile's pixels by depth slice,
+// so the fragment order lists each cluster's fragments together. A shading
+// workgroup takes 1024 consecutive entries of that order, resolves their
+// light lists, counting-sorts them by list length in LDS and hands them to
+// lanes in that order. The lanes of a wave then mostly share one light list:
+// their light loads hit one or two cache lines per instruction instead of 64
+// (with depth drawn independently per pixel, neighbouring pixels sit in
+// unrelated clusters, and per-lane gathers of light records bound the loop
+// by the L1 tag rate rather than by arithmetic). Each fragment is still
+// shaded start to finish by one lane, summing its lights in list order, so
+// the order of the fragments changes no result.
 //
 // Arithmetic, two variants:
 //  exact  every expression as the FP32 oracle evaluates it (IEEE operations,
 //         contraction off, correctly rounded divide and square root, the
 //         shared log2/exp2/pow) — bit-identical to the oracle.
-//  fast   (default) the light vector L, half vector H and N.H, N.L, H.V stay
-//         exact, and so does D's denominator N.H^2 (a^2-1) + 1, because GGX's
+//  fast   (default) the light vector L, half vector H and N.H stay exact, and
+//         so does D's denominator N.H^2 (a^2-1) + 1, because GGX's
 //         D = a^2 / (pi (N.H^2 (a^2-1) + 1)^2) amplifies an error in N.H or in
 //         that sum by up to ~1/a^2 (a^2 ~ 6e-6 at roughness 0.05);
-//         the remaining per-light terms (attenuation, D, G, Fresnel, BRDF
-//         combination) use fused multiply-add, hardware reciprocal and
-//         hardware exp2/log2, reading a per-call compact light table
-//         (position + 1/radius, colour * intensity). Every light term is
+//         the remaining per-light terms (N.L, H.V, attenuation, D, G,
+//         Fresnel, BRDF combination) use fused multiply-add, hardware
+//         reciprocal and hardware exp2/log2, reading a per-call compact light
+//         table (position + 1/radius, colour * intensity). Every light term is
 //         non-negative, so the sum keeps the terms' ~1e-6 relative accuracy.
 #include <hip/hip_runtime.h>
 
@@ -33,17 +38,11 @@
 #include "mlp_device.h"
 
 // build-time tuning switches (defaults are the tuned configuration)
-#ifndef HVX_SHADE_ONE_RCP
-#define HVX_SHADE_ONE_RCP 1
-#endif
-#ifndef HVX_SHADE_XCD_REMAP  // measured 4% slower on MI355X (4.81 vs 4.60 ms per frame): off
+#ifndef HVX_SHADE_XCD_REMAP  // measured no faster on MI355X: off
 #define HVX_SHADE_XCD_REMAP 0
 #endif
 #ifndef HVX_SHADE_WAVES_PER_EU
 #define HVX_SHADE_WAVES_PER_EU 4
-#endif
-#ifndef HVX_SHADE_PREFETCH  // lights in flight ahead of the one being shaded (1 or 2)
-#define HVX_SHADE_PREFETCH 1
 #endif
 
 namespace hvx {
@@ -79,6 +78,10 @@ struct ShadeArgs {
     uint32_t mask_face, lod_bins;
     float mlp_tol;          // bound on the relative radiance error the network may cause
     uint32_t* served;       // fragments that used the network (nullable)
+    // fragment order grouped by cluster (nullable: identity order) and the
+    // tile bounds it is built from: X + 1 column starts, then Y + 1 row starts
+    uint32_t* order;
+    uint32_t* tiles;
 };
 
 __global__ __launch_bounds__(256) void shade_light_prep_kernel(const hvx_light* __restrict__ lights, uint32_t n,
@@ -104,18 +107,26 @@ __device__ __forceinline__ f3 sample_spec_lod(const ShadeArgs& a, f3 dir, float 
     return {(1.0f - d) * c0.x + d * c1.x, (1.0f - d) * c0.y + d * c1.y, (1.0f - d) * c0.z + d * c1.z};
 }
 
-// Scene.frag:55-78: LinearDepth, log2 slice, tile -> cluster index
-__device__ __forceinline__ uint32_t cluster_of(const ShadeArgs& a, uint32_t idx) {
-    const uint32_t px = idx % a.W, py = idx / a.W;
-    const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
+// Scene.frag:55-78: LinearDepth, log2 slice, tile -> cluster index. The tile
+// column and row are non-decreasing in the pixel column and row, so a tile is
+// a rectangle of pixels.
+__device__ __forceinline__ uint32_t tile_x(const ShadeArgs& a, uint32_t px) {
+    const float tsx = a.scr_x / (float)a.X;
+    return min((uint32_t)(((float)px + 0.5f) / tsx), a.X - 1);
+}
+__device__ __forceinline__ uint32_t tile_y(const ShadeArgs& a, uint32_t py) {
+    const float tsy = a.scr_y / (float)a.Y;
+    return min((uint32_t)(((float)py + 0.5f) / tsy), a.Y - 1);
+}
+__device__ __forceinline__ uint32_t slice_of(const ShadeArgs& a, uint32_t idx) {
     const float z = a.depth[idx];
     const float lin = (a.nearv * a.farv) / (a.farv + z * (a.nearv - a.farv));
     const float zf = gmax(log2f_(lin) * a.scale + a.bias, 0.0f);
-    uint32_t zi = (uint32_t)zf;
-    if (zi >= a.Z) zi = a.Z - 1;
-    const float tsx = a.scr_x / (float)a.X, tsy = a.scr_y / (float)a.Y;
-    const uint32_t cx = min((uint32_t)(fx / tsx), a.X - 1), cy = min((uint32_t)(fy / tsy), a.Y - 1);
-    return cx + cy * a.X + zi * a.X * a.Y;
+    const uint32_t zi = (uint32_t)zf;
+    return zi >= a.Z ? a.Z - 1 : zi;
+}
+__device__ __forceinline__ uint32_t cluster_of(const ShadeArgs& a, uint32_t idx) {
+    return tile_x(a, idx % a.W) + tile_y(a, idx / a.W) * a.X + slice_of(a, idx) * a.X * a.Y;
 }
 
 __device__ __forceinline__ float rcp_fast(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -222,56 +233,27 @@ __device__ __forceinline__ bool shade_one(const ShadeArgs& a, uint32_t idx, uint
         const float a2m1 = alpha2 - 1.0f;
         const float nov4 = 4.0f * NoV;
         const float negf = -a.pc.light_falloff;
+        const float aGVpi = (alpha2 * GV) * (1.0f / kPiF);
         float q[3], om[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             q[c] = (oneMinusMetal * alb[c]) * (1.0f / kPiF);
             om[c] = 1.0f - F0[c];
         }
-        // software-pipelined: the next light's record is in flight while the
-        // current one is shaded
-        float4 An = make_float4(0.f, 0.f, 0.f, 0.f), Bn = An;
-        if (count > 0) {
-            const uint32_t li = a.indices[off];
-            An = a.lpos[li];
-            Bn = a.lcol[li];
-        }
-#if HVX_SHADE_PREFETCH >= 2
-        float4 An2 = An, Bn2 = Bn;
-        if (count > 1) {
-            const uint32_t li = a.indices[off + 1];
-            An2 = a.lpos[li];
-            Bn2 = a.lcol[li];
-        }
-#endif
-        for (uint32_t k = 0; k < count; ++k) {
-            const float4 A = An, B = Bn;
-#if HVX_SHADE_PREFETCH >= 2
-            An = An2;
-            Bn = Bn2;
-            if (k + 2 < count) {
-                const uint32_t li = a.indices[off + k + 2];
-                An2 = a.lpos[li];
-                Bn2 = a.lcol[li];
-            }
-#else
-            if (k + 1 < count) {
-                const uint32_t li = a.indices[off + k + 1];
-                An = a.lpos[li];
-                Bn = a.lcol[li];
-            }
-#endif
-            // exact: L, H and the cosines (max with 0 only differs in the sign
-            // of a zero, which no later term can observe)
+        // one light's term, added to Lo
+        auto light = [&](const float4 A, const float4 B) {
+            // exact: L, H and N.H (max with 0 only differs in the sign of a
+            // zero, which no later term can observe)
             const f3 d = {A.x - wp.x, A.y - wp.y, A.z - wp.z};
             const float s = sqrt_rn_normal(dot3(d, d));
             const f3 L = mul3(d, rcp_rn_normal(s));
             const f3 h = add3(V, L);
             const f3 Hh = mul3(h, rcp_rn_normal(sqrt_rn_normal(dot3(h, h))));
             const float NoH = fmaxf(dot3(N, Hh), 0.0f);
-            const float NoL = fmaxf(dot3(N, L), 0.0f);
-            const float HoV = fmaxf(dot3(Hh, V), 0.0f);
-            // fast: attenuation, D, G, Fresnel and the BRDF sum
+            const float NoL = fmaxf(__builtin_fmaf(N.z, L.z, __builtin_fmaf(N.y, L.y, N.x * L.x)), 0.0f);
+            const float HoV = __builtin_fmaf(Hh.z, V.z, __builtin_fmaf(Hh.y, V.y, Hh.x * V.x));
+            // fast: N.L and H.V fused (they enter only smooth factors), then
+            // attenuation, D, G, Fresnel and the BRDF sum
             const float att = fmaxf(__builtin_fmaf(-s, A.w, 1.0f), 0.0f) *
                               __builtin_amdgcn_exp2f(negf * __builtin_amdgcn_logf(s));
             // den = NoH^2 (a^2 - 1) + 1 exactly as the oracle rounds it: near
@@ -279,24 +261,74 @@ __device__ __forceinline__ bool shade_one(const ShadeArgs& a, uint32_t idx, uint
             // moves D by up to 1e-7 / a^2
             const float den = (NoH * NoH) * a2m1 + 1.0f;
             // D * G_L * G_V / (4 NoV NoL + 1e-4) with one reciprocal of the
-            // product of the three denominators (each bounded away from 0)
+            // product of the three denominators (each bounded away from 0);
+            // a^2 G_V / pi is a per-fragment constant
             const float gden = __builtin_fmaf(NoL, omk, alphaRough);
             const float sden = __builtin_fmaf(nov4, NoL, 0.0001f);
-            const float x = fmaxf(1.0f - HoV, 0.0f);  // HoV >= 0 already
+            const float x = 1.0f - HoV;
             const float x2 = x * x;
             const float p5 = (x2 * x2) * x;
-#if HVX_SHADE_ONE_RCP
-            const float DGr = (alpha2 * NoL * GV) * rcp_fast((((PI * den) * den) * gden) * sden);
-#else
-            const float DGr = (alpha2 * rcp_fast((PI * den) * den)) * (NoL * rcp_fast(gden)) * (GV * rcp_fast(sden));
-#endif
+            const float DGr = (aGVpi * NoL) * rcp_fast(((den * den) * gden) * sden);
             const float w = att * NoL;
             const float col[3] = {B.x, B.y, B.z};
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const float F = __builtin_fmaf(om[c], p5, F0[c]);
-                const float brdf = __builtin_fmaf(1.0f - F, q[c], DGr * F);
+                const float brdf = __builtin_fmaf(F, DGr - q[c], q[c]);  // (1 - F) q + F DGr
                 Lo[c] = __builtin_fmaf(brdf * col[c], w, Lo[c]);
+            }
+        };
+        const uint32_t off0 = __builtin_amdgcn_readfirstlane(off), cnt0 = __builtin_amdgcn_readfirstlane(count);
+        if (__all((off == off0) & (count == cnt0))) {
+            // Every active lane walks the same list (the usual case with
+            // cluster-binned fragments): its entries and light records are
+            // wave-uniform and come through the scalar cache into SGPRs.
+            // Both tables are read-only for the kernel's lifetime, which the
+            // constant address space states.
+            typedef const __attribute__((address_space(4))) float cfloat;
+            typedef const __attribute__((address_space(4))) uint32_t cuint;
+            cfloat* P = (cfloat*)a.lpos;
+            cfloat* Cl = (cfloat*)a.lcol;
+            cuint* I = (cuint*)(a.indices + off0);
+#pragma unroll 2
+            for (uint32_t k = 0; k < cnt0; ++k) {
+                const uint32_t li = I[k];
+                cfloat* p = P + 4 * (size_t)li;
+                cfloat* c = Cl + 4 * (size_t)li;
+                light(make_float4(p[0], p[1], p[2], p[3]), make_float4(c[0], c[1], c[2], 0.0f));
+            }
+        } else {
+            // Lanes on different lists: per-lane loads, unrolled by two and
+            // software-pipelined (while lights k, k+1 are shaded, the records
+            // of k+2, k+3 and the list entries of k+4, k+5 are in flight).
+            // Positions past the end of a list re-read its last entry
+            // (count >= 1 inside the loop), which keeps the loads
+            // unconditional; the empty asm keeps them in this iteration
+            // (without it they are sunk to their first use, one iteration
+            // later).
+            float4 A0n = make_float4(0.f, 0.f, 0.f, 0.f), B0n = A0n, A1n = A0n, B1n = A0n;
+            uint32_t i2 = 0, i3 = 0;
+            const uint32_t c1 = count > 0 ? count - 1 : 0;
+            if (count > 0) {
+                const uint32_t l0 = a.indices[off], l1 = a.indices[off + min(1u, c1)];
+                A0n = a.lpos[l0];
+                B0n = a.lcol[l0];
+                A1n = a.lpos[l1];
+                B1n = a.lcol[l1];
+                i2 = a.indices[off + min(2u, c1)];
+                i3 = a.indices[off + min(3u, c1)];
+            }
+            for (uint32_t k = 0; k < count; k += 2) {
+                const float4 A0 = A0n, B0 = B0n, A1 = A1n, B1 = B1n;
+                A0n = a.lpos[i2];
+                B0n = a.lcol[i2];
+                A1n = a.lpos[i3];
+                B1n = a.lcol[i3];
+                i2 = a.indices[off + min(k + 4, c1)];
+                i3 = a.indices[off + min(k + 5, c1)];
+                __asm__ volatile("" ::: "memory");
+                light(A0, B0);
+                if (k + 1 < count) light(A1, B1);
             }
         }
     }
@@ -363,10 +395,82 @@ constexpr int kChunk = HVX_SHADE_CHUNK;   // fragments per workgroup
 constexpr int kBuckets = 64;   // light-count classes (count / 2, saturating)
 constexpr uint32_t kXcds = 8;  // MI355X accelerator complex dies
 
+// ------------------------------------------------------- cluster binning
+constexpr uint32_t kBinThreads = 1024;
+constexpr uint32_t kBinMaxZ = 1024;  // depth slices a tile's LDS histogram holds
+
+// Tile bounds: tiles[c] = first pixel column of tile column c (c <= X, with
+// tiles[X] = W), then the same for rows. One thread per pixel column or row;
+// tile_x is non-decreasing, so the thread of the first column of a tile
+// writes that tile's start (and the starts of any empty tiles before it).
+__global__ __launch_bounds__(256) void tile_bounds_kernel(ShadeArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.W + a.H) return;
+    const bool col = i < a.W;
+    const uint32_t p = col ? i : i - a.W, n = col ? a.W : a.H, T = col ? a.X : a.Y;
+    uint32_t* b = col ? a.tiles : a.tiles + a.X + 1;
+    const uint32_t c = col ? tile_x(a, p) : tile_y(a, p);
+    const uint32_t first = p == 0 ? 0 : (col ? tile_x(a, p - 1) : tile_y(a, p - 1)) + 1;
+    for (uint32_t t = first; t <= c; ++t) b[t] = p;
+    if (p == n - 1)
+        for (uint32_t t = c + 1; t <= T; ++t) b[t] = n;
+}
+
+// One workgroup per screen tile: counting sort of the tile's pixels by depth
+// slice into the tile's segment of the order. Tiles are laid out row band by
+// row band, so tile (cx, cy) starts at y0 * W + (y1 - y0) * x0. Within a
+// slice the order is that of the LDS atomics, which changes no result.
+__global__ __launch_bounds__(kBinThreads) void bin_kernel(ShadeArgs a) {
+    __shared__ uint32_t s_hist[kBinMaxZ];
+    __shared__ uint32_t s_wave[kBinThreads / 64];
+    const uint32_t cx = blockIdx.x % a.X, cy = blockIdx.x / a.X, t = threadIdx.x;
+    const uint32_t x0 = a.tiles[cx], x1 = a.tiles[cx + 1];
+    const uint32_t y0 = a.tiles[a.X + 1 + cy], y1 = a.tiles[a.X + 1 + cy + 1];
+    const uint32_t w = x1 - x0, area = w * (y1 - y0);
+    if (area == 0) return;  // uniform over the workgroup
+    const uint32_t start = y0 * a.W + (y1 - y0) * x0;
+    for (uint32_t z = t; z < a.Z; z += kBinThreads) s_hist[z] = 0;
+    __syncthreads();
+    for (uint32_t i = t; i < area; i += kBinThreads) {
+        const uint32_t idx = (y0 + i / w) * a.W + x0 + i % w;
+        atomicAdd(&s_hist[slice_of(a, idx)], 1u);
+    }
+    __syncthreads();
+    // exclusive scan of the Z <= kBinThreads slice counts: per wave, then
+    // over the wave totals
+    const uint32_t v = t < a.Z ? s_hist[t] : 0u, lane = t & 63, wv = t >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t u = __shfl_up(inc, d, 64);
+        if ((int)lane >= d) inc += u;
+    }
+    if (lane == 63) s_wave[wv] = inc;
+    __syncthreads();
+    if (t < 64) {
+        const uint32_t wt = t < kBinThreads / 64 ? s_wave[t] : 0u;
+        uint32_t wi = wt;
+#pragma unroll
+        for (int d = 1; d < 16; d <<= 1) {
+            const uint32_t u = __shfl_up(wi, d, 64);
+            if ((int)t >= d) wi += u;
+        }
+        if (t < kBinThreads / 64) s_wave[t] = wi - wt;
+    }
+    __syncthreads();
+    if (t < a.Z) s_hist[t] = start + s_wave[wv] + inc - v;
+    __syncthreads();
+    for (uint32_t i = t; i < area; i += kBinThreads) {
+        const uint32_t idx = (y0 + i / w) * a.W + x0 + i % w;
+        a.order[atomicAdd(&s_hist[slice_of(a, idx)], 1u)] = idx;
+    }
+}
+
 
 template <bool EXACT, bool LEARNED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_WAVES_PER_EU, 8))) void shade_kernel(
     ShadeArgs a) {
+    __shared__ uint32_t s_idx[kChunk];
     __shared__ uint32_t s_off[kChunk];
     __shared__ uint16_t s_cnt[kChunk];
     __shared__ uint16_t s_order[kChunk];
@@ -374,8 +478,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_W
     const uint32_t nfrag = a.W * a.H;
     // Optional XCD-aware chunk order: workgroups are dealt to the 8 XCDs
     // round-robin, so the remap would give XCD k a contiguous run of chunks (a
-    // horizontal screen band) to confine its L2 working set. Measured slower
-    // than the plain order for this workload, so it is compiled out by default.
+    // horizontal screen band) to confine its L2 working set. Measured no
+    // faster than the plain order for this workload, so it is compiled out.
     const uint32_t nblk = gridDim.x, per = (nblk + kXcds - 1) / kXcds;
     const uint32_t xcd = blockIdx.x % kXcds, slot = blockIdx.x / kXcds;
     const uint32_t full = nblk % kXcds == 0 ? kXcds : nblk % kXcds;  // XCDs with `per` blocks
@@ -392,7 +496,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_W
     uint32_t bucket[kChunk / 256];
 #pragma unroll
     for (int r = 0; r < kChunk / 256; ++r) {
-        const uint32_t local = r * 256 + t, idx = base + local;
+        const uint32_t local = r * 256 + t, pos = base + local;
+        const uint32_t idx = pos >= nfrag ? nfrag : a.order ? a.order[pos] : pos;
+        s_idx[local] = idx;
         uint32_t off = 0, cnt = 0;
         if (idx < nfrag) {
             const hvx_light_cell cell = a.cells[cluster_of(a, idx)];
@@ -402,26 +508,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_W
         s_off[local] = off;
         s_cnt[local] = (uint16_t)cnt;
         bucket[r] = min(cnt >> 1, (uint32_t)kBuckets - 1);
-        atomicAdd(&s_hist[bucket[r]], 1u);
+        if (a.order) s_order[local] = (uint16_t)local;
+        else atomicAdd(&s_hist[bucket[r]], 1u);
     }
     __syncthreads();
-    if (t < 64) {  // exclusive scan of the bucket sizes (one wave)
-        const uint32_t v = s_hist[t];
-        uint32_t inc = v;
+    // Unbinned fragments: counting sort by light count, so the lanes of a
+    // wave walk lists of nearly equal length. Binned fragments keep their
+    // order, which groups each cluster's fragments.
+    if (!a.order) {
+        if (t < 64) {  // exclusive scan of the bucket sizes (one wave)
+            const uint32_t v = s_hist[t];
+            uint32_t inc = v;
 #pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t u = __shfl_up(inc, d, 64);
-            if ((int)t >= d) inc += u;
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t u = __shfl_up(inc, d, 64);
+                if ((int)t >= d) inc += u;
+            }
+            s_hist[t] = inc - v;
         }
-        s_hist[t] = inc - v;
-    }
-    __syncthreads();
+        __syncthreads();
 #pragma unroll
-    for (int r = 0; r < kChunk / 256; ++r) {
-        const uint32_t slot = atomicAdd(&s_hist[bucket[r]], 1u);
-        s_order[slot] = (uint16_t)(r * 256 + t);
+        for (int r = 0; r < kChunk / 256; ++r) {
+            const uint32_t slot = atomicAdd(&s_hist[bucket[r]], 1u);
+            s_order[slot] = (uint16_t)(r * 256 + t);
+        }
+        __syncthreads();
     }
-    __syncthreads();
     if constexpr (LEARNED) {
         // Each wave evaluates the network for the 64 fragments it is about to
         // shade, as four 16-sample tiles; tile T holds the fragments of lanes
@@ -436,7 +548,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_W
 #pragma unroll 1
         for (int r = 0; r < kChunk / 256; ++r) {
             const uint32_t local = s_order[r * 256 + t];
-            const uint32_t idx = base + local;
+            const uint32_t idx = s_idx[local];
             float x[4] = {0.0f, 0.0f, 1.0f, 0.0f};
             float err = -1.0f;
             if (idx < nfrag) {
@@ -473,7 +585,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_W
 #pragma unroll 1
         for (int r = 0; r < kChunk / 256; ++r) {
             const uint32_t local = s_order[r * 256 + t];
-            const uint32_t idx = base + local;
+            const uint32_t idx = s_idx[local];
             if (idx < nfrag) shade_one<EXACT>(a, idx, s_off[local], s_cnt[local]);
         }
     }
@@ -481,11 +593,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_W
 
 }  // namespace
 
+bool shade_binned(const hvx_cf_ubo& ubo) { return ubo.slice_count_z <= kBinMaxZ; }
+
+size_t shade_bin_scratch_bytes(const hvx_frag_soa& fr, const hvx_cf_ubo& ubo) {
+    return ((size_t)fr.width * fr.height + ubo.slice_count_x + ubo.slice_count_y + 2) * sizeof(uint32_t);
+}
+
 hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const hvx_cf_ubo& ubo,
                         const hvx_pbr_pc& pc, const hvx_light* d_lights, uint32_t n_lights,
                         const hvx_light_cell* d_cells, const uint32_t* d_indices, const hvx_ibl_maps& ibl,
-                        float* d_rgb, bool exact, float4* light_scratch, const LearnedShade* learned,
-                        hipStream_t s) {
+                        float* d_rgb, bool exact, float4* light_scratch, uint32_t* bin_scratch,
+                        const LearnedShade* learned, hipStream_t s) {
     ShadeArgs a{};
     a.depth = fr.depth;
     for (int c = 0; c < 3; ++c) {
@@ -532,6 +650,12 @@ hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const
     }
     const uint32_t F = fr.width * fr.height;
     const dim3 grid((F + kChunk - 1) / kChunk);
+    if (bin_scratch && shade_binned(ubo)) {
+        a.order = bin_scratch;
+        a.tiles = bin_scratch + F;
+        hipLaunchKernelGGL(tile_bounds_kernel, dim3((fr.width + fr.height + 255) / 256), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(bin_kernel, dim3(a.X * a.Y), dim3(kBinThreads), 0, s, a);
+    }
     if (exact) {
         if (learned) hipLaunchKernelGGL((shade_kernel<true, true>), grid, dim3(256), 0, s, a);
         else hipLaunchKernelGGL((shade_kernel<true, false>), grid, dim3(256), 0, s, a);
