@@ -93,7 +93,7 @@ hipError_t launch_cull_batch(const CullBatchView* d_views, uint32_t nv, const Cu
                              uint32_t persistent_wgs, uint32_t* d_queue, hipStream_t s);
 hipError_t launch_queue_reset(uint32_t* d_queues, uint32_t n, hipStream_t s);
 
-// --- shading (shade_kernels.hip); light_scratch holds 2 * n_lights float4
+// --- shading (shade_kernels.hip); light_scratch holds 2 * n_lights float4 (a 32-byte record per light)
 // Learned prefiltered radiance for shading: the network and its fallback mask.
 struct LearnedShade {
     MlpView mlp;

@@ -61,8 +61,7 @@ struct ShadeArgs {
     uint32_t X, Y, Z;
     hvx_pbr_pc pc;
     const hvx_light* lights;
-    const float4* lpos;   // fast: (x, y, z, 1/radius)
-    const float4* lcol;   // fast: (r, g, b) * lightIntensity
+    const float4* lrec;   // fast: per light (x, y, z, 1/radius), ((r, g, b) * lightIntensity, 0)
     const hvx_light_cell* cells;
     const uint32_t* indices;
     const float4* spec;
@@ -85,13 +84,12 @@ struct ShadeArgs {
 };
 
 __global__ __launch_bounds__(256) void shade_light_prep_kernel(const hvx_light* __restrict__ lights, uint32_t n,
-                                                               float intensity, float4* __restrict__ lpos,
-                                                               float4* __restrict__ lcol) {
+                                                               float intensity, float4* __restrict__ lrec) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const hvx_light& l = lights[i];
-    lpos[i] = make_float4(l.position[0], l.position[1], l.position[2], 1.0f / l.radius);
-    lcol[i] = make_float4(l.color[0] * intensity, l.color[1] * intensity, l.color[2] * intensity, 0.0f);
+    lrec[2 * i] = make_float4(l.position[0], l.position[1], l.position[2], 1.0f / l.radius);
+    lrec[2 * i + 1] = make_float4(l.color[0] * intensity, l.color[1] * intensity, l.color[2] * intensity, 0.0f);
 }
 
 __device__ __forceinline__ f3 sample_spec_lod(const ShadeArgs& a, f3 dir, float lod) {
@@ -287,15 +285,32 @@ __device__ __forceinline__ bool shade_one(const ShadeArgs& a, uint32_t idx, uint
             // constant address space states.
             typedef const __attribute__((address_space(4))) float cfloat;
             typedef const __attribute__((address_space(4))) uint32_t cuint;
-            cfloat* P = (cfloat*)a.lpos;
-            cfloat* Cl = (cfloat*)a.lcol;
+            cfloat* R = (cfloat*)a.lrec;
             cuint* I = (cuint*)(a.indices + off0);
-#pragma unroll 2
-            for (uint32_t k = 0; k < cnt0; ++k) {
-                const uint32_t li = I[k];
-                cfloat* p = P + 4 * (size_t)li;
-                cfloat* c = Cl + 4 * (size_t)li;
-                light(make_float4(p[0], p[1], p[2], p[3]), make_float4(c[0], c[1], c[2], 0.0f));
+            // one 32-byte light record per scalar load; software-pipelined
+            // like the per-lane loop below (records of k+2, k+3 and entries
+            // of k+4, k+5 in flight while lights k, k+1 are shaded)
+            auto rec = [&](uint32_t li, float4& A, float4& B) {
+                cfloat* r = R + 8 * (size_t)li;
+                A = make_float4(r[0], r[1], r[2], r[3]);
+                B = make_float4(r[4], r[5], r[6], r[7]);
+            };
+            if (cnt0 > 0) {
+                const uint32_t c1 = cnt0 - 1;
+                float4 A0n, B0n, A1n, B1n;
+                rec(I[0], A0n, B0n);
+                rec(I[min(1u, c1)], A1n, B1n);
+                uint32_t i2 = I[min(2u, c1)], i3 = I[min(3u, c1)];
+                for (uint32_t k = 0; k < cnt0; k += 2) {
+                    const float4 A0 = A0n, B0 = B0n, A1 = A1n, B1 = B1n;
+                    rec(i2, A0n, B0n);
+                    rec(i3, A1n, B1n);
+                    i2 = I[min(k + 4, c1)];
+                    i3 = I[min(k + 5, c1)];
+                    __asm__ volatile("" ::: "memory");
+                    light(A0, B0);
+                    if (k + 1 < cnt0) light(A1, B1);
+                }
             }
         } else {
             // Lanes on different lists: per-lane loads, unrolled by two and
@@ -311,19 +326,19 @@ __device__ __forceinline__ bool shade_one(const ShadeArgs& a, uint32_t idx, uint
             const uint32_t c1 = count > 0 ? count - 1 : 0;
             if (count > 0) {
                 const uint32_t l0 = a.indices[off], l1 = a.indices[off + min(1u, c1)];
-                A0n = a.lpos[l0];
-                B0n = a.lcol[l0];
-                A1n = a.lpos[l1];
-                B1n = a.lcol[l1];
+                A0n = a.lrec[2 * l0];
+                B0n = a.lrec[2 * l0 + 1];
+                A1n = a.lrec[2 * l1];
+                B1n = a.lrec[2 * l1 + 1];
                 i2 = a.indices[off + min(2u, c1)];
                 i3 = a.indices[off + min(3u, c1)];
             }
             for (uint32_t k = 0; k < count; k += 2) {
                 const float4 A0 = A0n, B0 = B0n, A1 = A1n, B1 = B1n;
-                A0n = a.lpos[i2];
-                B0n = a.lcol[i2];
-                A1n = a.lpos[i3];
-                B1n = a.lcol[i3];
+                A0n = a.lrec[2 * i2];
+                B0n = a.lrec[2 * i2 + 1];
+                A1n = a.lrec[2 * i3];
+                B1n = a.lrec[2 * i3 + 1];
                 i2 = a.indices[off + min(k + 4, c1)];
                 i3 = a.indices[off + min(k + 5, c1)];
                 __asm__ volatile("" ::: "memory");
@@ -627,8 +642,7 @@ hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const
     a.Z = ubo.slice_count_z;
     a.pc = pc;
     a.lights = d_lights;
-    a.lpos = light_scratch;
-    a.lcol = light_scratch + n_lights;
+    a.lrec = light_scratch;
     a.cells = d_cells;
     a.indices = d_indices;
     a.spec = reinterpret_cast<const float4*>(ibl.specular);
@@ -662,7 +676,7 @@ hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const
     } else {
         if (n_lights > 0) {
             hipLaunchKernelGGL(shade_light_prep_kernel, dim3((n_lights + 255) / 256), dim3(256), 0, s, d_lights,
-                               n_lights, pc.light_intensity, light_scratch, light_scratch + n_lights);
+                               n_lights, pc.light_intensity, light_scratch);
         }
         if (learned) hipLaunchKernelGGL((shade_kernel<false, true>), grid, dim3(256), 0, s, a);
         else hipLaunchKernelGGL((shade_kernel<false, false>), grid, dim3(256), 0, s, a);
