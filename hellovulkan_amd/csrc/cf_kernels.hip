@@ -8,17 +8,18 @@
 //
 // Culling. The reference runs one invocation per cluster that walks every
 // light and keeps the first `cap` hits. Here a workgroup owns a 4x4x1 block
-// of clusters. Lights stream through in tiles of 256 (one per lane); each lane
-// tests its light against the union AABB of the block (a conservative filter:
-// the per-axis distance terms of the union box are never larger than a member
-// box's, term by term and under rounding, so a light that fails the union
-// test fails every member test). Survivors are compacted into an LDS
-// candidate list in ascending light order with a wavefront ballot and a
-// workgroup prefix sum. Once enough candidates are staged, every wave tests its
-// clusters against them, 64 candidates per step, and appends hits with a
-// second ballot + mbcnt prefix, keeping the first `cap` in ascending order —
-// exactly the reference's list. The per-test arithmetic is the reference's,
-// in the same order, with contraction off, so lists match it bit for bit.
+// of clusters. Lights stream through in steps of 1024 (four per lane); each
+// lane tests its lights against the union AABB of the block (a conservative
+// filter: the per-axis distance terms of the union box are never larger than
+// a member box's, term by term and under rounding, so a light that fails the
+// union test fails every member test). Survivors are compacted into an LDS
+// candidate list in ascending light order with wavefront ballots and a
+// workgroup prefix sum (two barriers per step). Once enough candidates are
+// staged, every wave tests its clusters against them, 64 candidates per step,
+// and appends hits with a second ballot + mbcnt prefix, keeping the first
+// `cap` in ascending order — exactly the reference's list. The per-test
+// arithmetic is the reference's, in the same order, with contraction off, so
+// lists match it bit for bit.
 #include <hip/hip_runtime.h>
 
 #include "hvx_device.h"
@@ -96,7 +97,11 @@ __device__ __forceinline__ bool sphere_box(float4 p, const Box& b) {
 }
 
 constexpr int kWaves = 4;                 // 256 lanes
-constexpr int kTile = 256;                // lights per level-1 step
+#ifndef HVX_CULL_LPL
+#define HVX_CULL_LPL 4
+#endif
+constexpr int kLPL = HVX_CULL_LPL;        // lights per lane per level-1 step
+constexpr int kTile = 256 * kLPL;         // lights per level-1 step
 constexpr int kFlush = 256;               // staged candidates that trigger level 2
 constexpr int kCandMax = kFlush + kTile;
 
@@ -117,7 +122,7 @@ template <int GX, int GY, int GZ> __host__ __device__ inline uint32_t cull_group
 struct CullShared {
     float4 pos[kCandMax];
     uint32_t idx[kCandMax];
-    uint32_t wcnt[kWaves];
+    uint32_t wcnt[kLPL][kWaves];
     uint32_t wdone[kWaves];
 };
 
@@ -166,62 +171,105 @@ __device__ __forceinline__ void cull_group(const CullJob& J, uint32_t g, CullSha
         }
     }
 
-    uint32_t base = 0, ncand = 0;
-    for (;;) {
-        // ---- level 1: one light per lane against the union box
-        const uint32_t li = base + tid;
-        bool hit = false;
-        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (li < n) {
-            p = J.lvs[li];
-            hit = sphere_box(p, U);
-        }
-        const uint64_t m = __ballot(hit);
-        if (lane == 0) sh.wcnt[wave] = (uint32_t)__popcll(m);
-        __syncthreads();
-        uint32_t woff = 0, tot = 0;
+    // union box of this wave's valid clusters (a wave with none never tests)
+    Box UW;
+    for (int a = 0; a < 3; ++a) { UW.mn[a] = hvx::kInfF; UW.mx[a] = -hvx::kInfF; }
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w) {
-            const uint32_t c = sh.wcnt[w];
-            woff += (w < (int)wave) ? c : 0u;
-            tot += c;
+    for (int j = 0; j < kPerWave; ++j)
+        if (!done[j])
+            for (int a = 0; a < 3; ++a) {
+                UW.mn[a] = gmin(UW.mn[a], cb[j].mn[a]);
+                UW.mx[a] = gmax(UW.mx[a], cb[j].mx[a]);
+            }
+
+    uint32_t base = 0, ncand = 0;
+    // The next step's lights are loaded while this step is filtered. Loads
+    // past the last light re-read it (the light table always holds at least
+    // one entry), which keeps them unconditional; the lanes' hits are masked.
+    const uint32_t last = n > 0 ? n - 1 : 0u;
+    float4 pn[kLPL];
+#pragma unroll
+    for (int j = 0; j < kLPL; ++j) pn[j] = J.lvs[min(256u * j + tid, last)];
+    for (;;) {
+        // ---- level 1: kLPL lights per lane against the union box; sub-step
+        // j covers lights base + 256 j .. base + 256 j + 255, so candidates
+        // are compacted in (sub-step, wave, lane) order = ascending light order
+        float4 p[kLPL];
+        bool hit[kLPL];
+        uint64_t m[kLPL];
+#pragma unroll
+        for (int j = 0; j < kLPL; ++j) {
+            p[j] = pn[j];
+            pn[j] = J.lvs[min(base + kTile + 256u * j + tid, last)];
         }
-        if (hit) {
-            const uint32_t pos = ncand + woff + lane_prefix(m);
-            sh.pos[pos] = p;
-            sh.idx[pos] = li;
+        __asm__ volatile("" ::: "memory");  // keeps the prefetch in this step
+#pragma unroll
+        for (int j = 0; j < kLPL; ++j) {
+            const uint32_t li = base + 256u * j + tid;
+            hit[j] = li < n && sphere_box(p[j], U);
+        }
+#pragma unroll
+        for (int j = 0; j < kLPL; ++j) {
+            m[j] = __ballot(hit[j]);
+            if (lane == 0) sh.wcnt[j][wave] = (uint32_t)__popcll(m[j]);
+        }
+        __syncthreads();
+        uint32_t off[kLPL], tot = 0;
+#pragma unroll
+        for (int j = 0; j < kLPL; ++j) {
+            off[j] = 0;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                const uint32_t c = sh.wcnt[j][w];
+                if (w == (int)wave) off[j] = tot;
+                tot += c;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kLPL; ++j) {
+            if (hit[j]) {
+                const uint32_t pos = ncand + off[j] + lane_prefix(m[j]);
+                sh.pos[pos] = p[j];
+                sh.idx[pos] = base + 256u * j + tid;
+            }
         }
         __syncthreads();
         ncand += tot;
         base += kTile;
         if (ncand < (uint32_t)kFlush && base < n) continue;
 
-        // ---- level 2: each wave tests its clusters against the staged candidates
+        // ---- level 2: each wave tests its clusters against the staged
+        // candidates, 64 per step: a lane reads its candidate once, skips
+        // the step if no candidate meets the wave's union box, and otherwise
+        // tests it against each cluster the wave still fills
         bool wave_done = true;
 #pragma unroll
-        for (int j = 0; j < kPerWave; ++j) {
-            if (!done[j]) {
-                for (uint32_t k0 = 0; k0 < ncand; k0 += 64) {
-                    const uint32_t k = k0 + lane;
-                    bool h = false;
-                    if (k < ncand) h = sphere_box(sh.pos[k], cb[j]);
+        for (int j = 0; j < kPerWave; ++j) wave_done = wave_done && done[j];
+        for (uint32_t k0 = 0; k0 < ncand && !wave_done; k0 += 64) {
+            const uint32_t k = k0 + lane;
+            const bool valid = k < ncand;
+            const float4 pk = valid ? sh.pos[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+            if (__ballot(valid && sphere_box(pk, UW)) == 0) continue;
+            const uint32_t ik = valid ? sh.idx[k] : 0u;
+            wave_done = true;
+#pragma unroll
+            for (int j = 0; j < kPerWave; ++j) {
+                if (!done[j]) {
+                    const bool h = valid && sphere_box(pk, cb[j]);
                     const uint64_t mm = __ballot(h);
                     if (cnt[j] < cap) {
                         const uint32_t pos = cnt[j] + lane_prefix(mm);
-                        if (h && pos < cap) J.lists[(size_t)cid[j] * cap + pos] = sh.idx[k];
+                        if (h && pos < cap) J.lists[(size_t)cid[j] * cap + pos] = ik;
                         const uint32_t nh = (uint32_t)__popcll(mm);
                         if (cnt[j] + nh > cap) ovf[j] = 1;
                         cnt[j] = min(cnt[j] + nh, cap);
                     } else if (mm != 0) {
                         ovf[j] = 1;
                     }
-                    if (cnt[j] >= cap && (ovf[j] || !J.want_overflow)) {
-                        done[j] = true;
-                        break;
-                    }
+                    if (cnt[j] >= cap && (ovf[j] || !J.want_overflow)) done[j] = true;
                 }
+                wave_done = wave_done && done[j];
             }
-            wave_done = wave_done && done[j];
         }
         if (lane == 0) sh.wdone[wave] = wave_done ? 1u : 0u;
         __syncthreads();
