@@ -129,8 +129,11 @@ def _shade_case(ctx, width, height, grid, n_lights, cap, seed, n_check, far_plan
     torch.cuda.synchronize()
     got = rgb.cpu().numpy().reshape(3, -1).T
     F = width * height
-    ids = np.unique(np.concatenate([np.arange(min(F, 64)), rng.integers(0, F, n_check),
-                                    np.arange(max(0, F - 64), F)])).astype(np.uint32)
+    if n_check is None:  # every fragment
+        ids = np.arange(F, dtype=np.uint32)
+    else:
+        ids = np.unique(np.concatenate([np.arange(min(F, 64)), rng.integers(0, F, n_check),
+                                        np.arange(max(0, F - 64), F)])).astype(np.uint32)
     ref = O.shade(fr, width, height, ivp, cam, ubo, pc, lights, cells, idx, maps, ids)
     return got[ids], ref, ids
 
@@ -156,6 +159,27 @@ def test_shade_matches_oracle(ctx, case, mode):
     if mode == "exact":
         assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
     assert err <= SHADE_TOL
+
+
+@pytest.mark.parametrize("mode", ["fast", "exact"])
+@pytest.mark.parametrize("width,height,grid", [(7, 5, (16, 9, 24)), (100, 37, (3, 7, 5)), (96, 64, (2, 2, 1100))],
+                         ids=["more-tiles-than-pixels", "uneven-tiles", "unbinned-1100-slices"])
+def test_shade_binning_grids(ctx, width, height, grid, mode):
+    """Cluster grids the fragment binning treats specially: screen tiles with
+    no pixels, tiles of unequal size, and more depth slices than a tile's
+    histogram holds (those frames are shaded unbinned). Every fragment of
+    these small frames is checked, so one the binning dropped or shaded twice
+    would show."""
+    ctx.set_option(abi.OPT_SHADE_EXACT, 1 if mode == "exact" else 0)
+    try:
+        got, ref, ids = _shade_case(ctx, width, height, grid, 256, 64, 51, None)
+    finally:
+        ctx.set_option(abi.OPT_SHADE_EXACT, 0)
+    assert len(ids) == width * height
+    assert np.all(np.isfinite(got))
+    if mode == "exact":
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    assert O.rel_err(got, ref) <= SHADE_TOL
 
 
 def test_shade_zero_lights(ctx):
