@@ -616,6 +616,7 @@ constexpr uint32_t kNearWaves = 8;  // waves per 16x16 output tile (two row tile
 #endif
 constexpr uint32_t kNearParts = HVX_NEAR_PARTS;  // workgroups per tile
 static_assert(kNearWaves % kNearParts == 0, "a tile splits into whole waves");
+static_assert(64 * kNearWaves / kNearParts == 256, "the near GEMM's LDS staging assumes 256-lane workgroups");
 
 template <bool SPLIT, int PG>
 __global__ __launch_bounds__(512) void near_gemm_kernel(Probes Q, NearTable T, const float* sc) {
@@ -687,6 +688,141 @@ __global__ __launch_bounds__(512) void near_gemm_kernel(Probes Q, NearTable T, c
             cc = 0;
             ++r;
         }
+    }
+    const uint32_t So = J.So, X0 = (tile % J.te) * 16, Y0 = (tile / J.te) * 16;
+    const size_t So2 = (size_t)So * So;
+#pragma unroll
+    for (int q = 0; q < PG; ++q) {
+        if ((uint32_t)q >= np) break;
+        float* out = reinterpret_cast<float*>(probe_out(Q, p0 + q) + J.out_off);
+        const float* sq = scr(Q, sc, p0 + q);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t Y = Y0 + rt0 + h;
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb) {
+                const f32x4 a = acc[q][2 * h + nb];
+                const uint32_t col = nb * 16 + i, f = col >> 2, c = col & 3;
+                if (col >= 24) continue;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const uint32_t X = X0 + 4 * g + e, o = Y * So + X;
+                    out[((size_t)f * So2 + o) * 4 + c] = c < 3 ? a[e] * J.rowscale[o] * sq[6 + f] : 1.0f;
+                }
+            }
+        }
+    }
+}
+
+template <bool SPLIT, int PG>
+__global__ __launch_bounds__(512) void near_gemm_staged_kernel(Probes Q, NearTable T, const float* sc) {
+    const uint2 wb = gemm_block(Q);
+    const uint32_t p0 = wb.y * PG, np = min((uint32_t)PG, Q.n - p0);
+    uint32_t jj = 0;
+    // workgroup b owns half b & 1 (eight row tiles) of tile b >> 1: finer
+    // work units than whole tiles balance the unequal mips across the CUs
+    const uint32_t gt = wb.x / kNearParts;
+    while (jj + 1 < T.n && gt >= T.j[jj + 1].tile_begin) ++jj;
+    const NearJob& J = T.j[jj];
+    const uint32_t tile = gt - J.tile_begin, lane = threadIdx.x & 63;
+    const uint32_t wave = (threadIdx.x >> 6) + (wb.x % kNearParts) * (kNearWaves / kNearParts);
+    const NearTile Tl = J.tiles[tile];
+    // PG probes per wave: each weight fragment is loaded once for the group;
+    // each probe's accumulation is the same sequence as with PG = 1
+    const half8* Pp[PG];
+#pragma unroll
+    for (int q = 0; q < PG; ++q) Pp[q] = scr(Q, J.P, p0 + ((uint32_t)q < np ? q : 0));
+    const uint32_t wc = Tl.w / 32, KS = Tl.h * wc;
+    const uint32_t i = lane & 15, g = lane >> 4, rt0 = 2 * wave;
+    // weights of a tile stored K-step major: step s of all 16 row tiles is one
+    // contiguous 16 KB block
+    constexpr size_t kStep = 16 * 64, kRow = 64;
+    const half8* w0 = J.W + Tl.woff + (size_t)rt0 * kRow + lane;
+    const half8* w1 = w0 + kRow;
+    const uint32_t xb0 = (uint32_t)(Tl.x0 + (int)J.B) / 8;
+    // The B operand of a K step (for each probe: 4 x 32 consecutive half8 of
+    // the staged plane, plus as many lo parts when split) is the same for
+    // every wave of the workgroup: it is staged once per step in LDS, double
+    // buffered, the next step's copy loaded while this step's MFMAs run. With
+    // PG probes per wave this cuts the workgroup's operand reads PG-fold per
+    // step, which pays for the barrier (cfg3 contraction mode 8.8 -> 8.4 ms);
+    // with one probe it does not (cfg2 specular 0.366 -> 0.392 ms), so
+    // single-probe calls use near_gemm_kernel.
+    constexpr int kB = PG * 128 * (SPLIT ? 2 : 1);  // half8 per step
+    constexpr int kPer = (kB + 255) / 256;           // per thread (256 lanes)
+    static_assert(kB % 256 == 0 || kB < 256, "the B block splits evenly over the workgroup");
+    __shared__ half8 sB[2][kB];
+    const uint32_t t = threadIdx.x;
+    auto b_load = [&](uint32_t r, uint32_t cc, half8 (&v)[kPer]) {
+        const size_t base = ((size_t)(Tl.y0 + (int)J.B + (int)r) * J.Pw8 + xb0 + cc * 4) * 32;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const uint32_t e = t + 256u * k, lo = e / (PG * 128), q = (e / 128) % PG, o = e % 128;
+            if (e < (uint32_t)kB) v[k] = Pp[q][base + o + (lo ? J.p_lo : 0)];
+        }
+    };
+    half8 nx[kPer];
+    b_load(0, 0, nx);
+#pragma unroll
+    for (int k = 0; k < kPer; ++k)
+        if (t + 256u * k < (uint32_t)kB) sB[0][t + 256u * k] = nx[k];
+    __syncthreads();
+    f32x4 acc[PG][4];
+#pragma unroll
+    for (int q = 0; q < PG; ++q)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc[q][k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    uint32_t r = 0, cc = 0;
+    for (uint32_t s = 0; s < KS; ++s) {
+        uint32_t rn = r, cn = cc + 1;
+        if (cn == wc) {
+            cn = 0;
+            ++rn;
+        }
+        const bool more = s + 1 < KS;
+        if (more) b_load(rn, cn, nx);
+        const half8* sb = sB[s & 1];
+        if constexpr (SPLIT) {
+            const bf16x8* wv = reinterpret_cast<const bf16x8*>(w0);
+            const size_t o0 = (size_t)s * kStep, o1 = o0 + kRow;
+            const bf16x8 x0 = wv[o0], x1 = wv[o1], y0 = wv[o0 + J.w_lo], y1 = wv[o1 + J.w_lo];
+#pragma unroll
+            for (int q = 0; q < PG; ++q) {
+                const bf16x8* pv = reinterpret_cast<const bf16x8*>(sb + q * 128 + g * 32 + i);
+                const bf16x8 b0 = pv[0], b1 = pv[16], c0 = pv[PG * 128], c1 = pv[PG * 128 + 16];
+                acc[q][0] = hvx::mfma16x16x32(x0, b0, acc[q][0]);
+                acc[q][0] = hvx::mfma16x16x32(x0, c0, acc[q][0]);
+                acc[q][0] = hvx::mfma16x16x32(y0, b0, acc[q][0]);
+                acc[q][1] = hvx::mfma16x16x32(x0, b1, acc[q][1]);
+                acc[q][1] = hvx::mfma16x16x32(x0, c1, acc[q][1]);
+                acc[q][1] = hvx::mfma16x16x32(y0, b1, acc[q][1]);
+                acc[q][2] = hvx::mfma16x16x32(x1, b0, acc[q][2]);
+                acc[q][2] = hvx::mfma16x16x32(x1, c0, acc[q][2]);
+                acc[q][2] = hvx::mfma16x16x32(y1, b0, acc[q][2]);
+                acc[q][3] = hvx::mfma16x16x32(x1, b1, acc[q][3]);
+                acc[q][3] = hvx::mfma16x16x32(x1, c1, acc[q][3]);
+                acc[q][3] = hvx::mfma16x16x32(y1, b1, acc[q][3]);
+            }
+        } else {
+            const half8 x0 = w0[(size_t)s * kStep], x1 = w1[(size_t)s * kStep];
+#pragma unroll
+            for (int q = 0; q < PG; ++q) {
+                const half8* pv = sb + q * 128 + g * 32 + i;
+                const half8 b0 = pv[0], b1 = pv[16];
+                acc[q][0] = hvx::mfma16x16x32(x0, b0, acc[q][0]);
+                acc[q][1] = hvx::mfma16x16x32(x0, b1, acc[q][1]);
+                acc[q][2] = hvx::mfma16x16x32(x1, b0, acc[q][2]);
+                acc[q][3] = hvx::mfma16x16x32(x1, b1, acc[q][3]);
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int k = 0; k < kPer; ++k)
+                if (t + 256u * k < (uint32_t)kB) sB[(s + 1) & 1][t + 256u * k] = nx[k];
+        }
+        __syncthreads();
+        r = rn;
+        cc = cn;
     }
     const uint32_t So = J.So, X0 = (tile % J.te) * 16, Y0 = (tile / J.te) * 16;
     const size_t So2 = (size_t)So * So;
@@ -1072,9 +1208,9 @@ int launch_call(hvx_contract_plan* P, const Probes& Q, uint32_t n, uint32_t chai
                    block(64 * kNearWaves / kNearParts);
         const bool split = P->precision == HVX_PRECISION_BF16_SPLIT;
         if (grouped && split)
-            hipLaunchKernelGGL((near_gemm_kernel<true, kProbeGroup>), grid, block, 0, s, G, P->near, P->sc);
+            hipLaunchKernelGGL((near_gemm_staged_kernel<true, kProbeGroup>), grid, block, 0, s, G, P->near, P->sc);
         else if (grouped)
-            hipLaunchKernelGGL((near_gemm_kernel<false, kProbeGroup>), grid, block, 0, s, G, P->near, P->sc);
+            hipLaunchKernelGGL((near_gemm_staged_kernel<false, kProbeGroup>), grid, block, 0, s, G, P->near, P->sc);
         else if (split)
             hipLaunchKernelGGL((near_gemm_kernel<true, 1>), grid, block, 0, s, G, P->near, P->sc);
         else
