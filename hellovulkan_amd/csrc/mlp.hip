@@ -269,6 +269,30 @@ __global__ __launch_bounds__(256) void reduce_kernel(const float* __restrict__ p
     out[i] = acc * scale;
 }
 
+// out[i] = scale * sum_s part[s * stride + i] for many slices: a workgroup
+// takes 64 outputs and 16 slice ranges (one per wave), each summed in slice
+// order, then adds the 16 range sums in a fixed tree
+constexpr uint32_t kRedGroups = 16;
+__global__ __launch_bounds__(64 * kRedGroups) void reduce_slices_kernel(const float* __restrict__ part,
+                                                                         uint32_t slices, uint32_t stride,
+                                                                         uint32_t len, float scale,
+                                                                         float* __restrict__ out) {
+    __shared__ float sh[kRedGroups][64];
+    const uint32_t l = threadIdx.x & 63, gq = threadIdx.x >> 6, i = blockIdx.x * 64 + l;
+    float acc = 0.0f;
+    if (i < len) {
+        const uint32_t s0 = gq * slices / kRedGroups, s1 = (gq + 1) * slices / kRedGroups;
+#pragma unroll 8
+        for (uint32_t s = s0; s < s1; ++s) acc += part[(size_t)s * stride + i];
+    }
+    sh[gq][l] = acc;
+    for (uint32_t h = kRedGroups / 2; h > 0; h >>= 1) {
+        __syncthreads();
+        if (gq < h) sh[gq][l] += sh[gq + h][l];
+    }
+    if (gq == 0 && i < len) out[i] = sh[0][l] * scale;
+}
+
 // db[r] = sum over the row r of dZT [rows][B]
 __global__ __launch_bounds__(256) void colsum_kernel(const bf16* __restrict__ dZT, uint32_t B, float* __restrict__ db) {
     __shared__ float sh[256];
@@ -442,7 +466,9 @@ hipError_t ensure_batch(hvx_mlp* M, uint32_t B) {
     M->batch_allocs.clear();
     M->cap = 0;
     auto& L = M->batch_allocs;
-    M->splitk = std::max(1u, std::min(32u, B / 512));
+    // weight gradients are [64][<=128] GEMMs over K = B: split K into slices
+    // of 128 samples (up to 256) so that the GEMM has 1-2 k waves
+    M->splitk = std::max(1u, std::min(256u, B / 128));
     if ((e = dalloc(L, &M->X0, (size_t)B * kIn)) || (e = dalloc(L, &M->X0T, (size_t)B * kIn))) return e;
     for (int l = 0; l < 3; ++l)
         if ((e = dalloc(L, &M->H[l], (size_t)B * kH)) || (e = dalloc(L, &M->HT[l], (size_t)B * kH))) return e;
@@ -508,8 +534,8 @@ void gradient(hvx_mlp* M, const float4* in, const float4* tgt, uint32_t B, float
         g.splitk = M->splitk;
         g.part = M->part;
         gemm<EPI_PART>(g, s);
-        hipLaunchKernelGGL(reduce_kernel, dim3((kH * K + 255) / 256), dim3(256), 0, s, M->part, M->splitk,
-                           (uint32_t)(kH * K), (uint32_t)(kH * K), 1.0f, G + wofs[l]);
+        hipLaunchKernelGGL(reduce_slices_kernel, dim3((kH * K + 63) / 64), dim3(64 * kRedGroups), 0, s, M->part,
+                           M->splitk, (uint32_t)(kH * K), (uint32_t)(kH * K), 1.0f, G + wofs[l]);
         hipLaunchKernelGGL(colsum_kernel, dim3(kH), dim3(256), 0, s, dzT, B, G + bofs[l]);
         if (l > 0) {
             GemmArgs d{};
