@@ -3,14 +3,15 @@
 // kernels and FP32 master weights (Adam).
 //
 // One training step over a batch of B samples (B a multiple of 256):
-//   pe            X0 = PE(d, lod)                        bf16 [B][128] (+ transpose)
+//   pe            X0 = PE(d, lod)                        bf16 [B][128] (+ transpose);
+//                 once per training call when the call revisits its samples
 //   gemm FWD x3   H_l = relu(H_{l-1} W_l^T + b_l)        bf16 [B][64]  (+ transpose)
 //   out           y = H3 W4^T + b4, loss, dY, dZ3 = (dY W4) * [H3 > 0], dW4/db4 partials
 //   per layer l = 3, 2, 1:
-//     gemm PART   dW_l = dZ_l^T H_{l-1}   (K = batch, split over workgroups, reduced in order)
-//     colsum      db_l = sum_b dZ_l
+//     gemm PART   dW_l = dZ_l^T H_{l-1}, db_l = sum_b dZ_l   (K = batch, split into
+//                 128-sample slices over waves, reduced in slice order)
 //     gemm BWD    dZ_{l-1} = (dZ_l W_l) * [H_{l-1} > 0]   (l > 1)
-//   adam          FP32 master update, then BF16 copies of W1..W3 (and W2^T, W3^T)
+//   adam          FP32 master update and the BF16 copies of W1..W3 (and W2^T, W3^T)
 // Every GEMM is C = A Bt^T with both operands row-major along K, so each MFMA
 // operand load is one contiguous 16-byte read; the epilogues also write the
 // transposed activation that the weight-gradient GEMMs read that way.
@@ -128,11 +129,14 @@ struct GemmArgs {
     const bf16* A;   // [M][Kd]
     const bf16* Bt;  // [Nc][Kd]
     uint32_t M, Nc, Kd, splitk;
+    uint32_t lda, ldb;  // row strides of A and Bt in elements (0: Kd)
     const float* bias;  // FWD: [Nc]
     const bf16* mask;   // BWD: [M][Nc], zero where <= 0
     bf16* C;            // FWD/BWD: [M][Nc]
     bf16* CT;           // FWD/BWD: [Nc][M]
-    float* part;        // PART: [splitk][M][Nc]
+    float* part;        // PART: per slice [M][Nc] then (colsum) [M], slices pstride apart
+    uint32_t pstride;   // PART: floats per slice
+    uint32_t colsum;    // PART: also sum A's rows over K (the bias gradient)
 };
 
 // A wave computes a 16 x 64 block of C over one K slice.
@@ -145,12 +149,17 @@ __global__ __launch_bounds__(256) void gemm_abt_kernel(GemmArgs g) {
     const uint32_t ks = g.Kd / 32;
     const uint32_t kb = slice * ks / g.splitk, ke = (slice + 1) * ks / g.splitk;
     const uint32_t i = lane & 15, gq = lane >> 4;
-    const bf16* ap = g.A + (size_t)(rt * 16 + i) * g.Kd + 8 * gq;
-    const bf16* bp = g.Bt + (size_t)(cg * 64 + i) * g.Kd + 8 * gq;
-    const size_t bstep = (size_t)16 * g.Kd;
+    const size_t lda = g.lda ? g.lda : g.Kd, ldb = g.ldb ? g.ldb : g.Kd;
+    const bf16* ap = g.A + (size_t)(rt * 16 + i) * lda + 8 * gq;
+    const bf16* bp = g.Bt + (size_t)(cg * 64 + i) * ldb + 8 * gq;
+    const size_t bstep = (size_t)16 * ldb;
     f32x4 acc[4];
 #pragma unroll
     for (int nb = 0; nb < 4; ++nb) acc[nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    // PART with colsum: the waves of column group 0 also sum the A elements
+    // they load (row rt * 16 + i, this lane's 8 of every 32 K)
+    const bool rowsum = EPI == EPI_PART && g.colsum && cg == 0;
+    float rs = 0.0f;
     for (uint32_t s = kb; s < ke; ++s) {
         const bf16x8 a = *reinterpret_cast<const bf16x8*>(ap + 32 * s);
 #pragma unroll
@@ -158,6 +167,17 @@ __global__ __launch_bounds__(256) void gemm_abt_kernel(GemmArgs g) {
             const bf16x8 b = *reinterpret_cast<const bf16x8*>(bp + nb * bstep + 32 * s);
             acc[nb] = hvx::mfma16x16x32(a, b, acc[nb]);
         }
+        if (rowsum) {
+            float t = 0.0f;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) t += (float)a[q];
+            rs += t;
+        }
+    }
+    if (rowsum) {  // the four lanes of row i hold disjoint K subsets
+        rs += __shfl_xor(rs, 16, 64);
+        rs += __shfl_xor(rs, 32, 64);
+        if (gq == 0) g.part[(size_t)slice * g.pstride + g.M * g.Nc + rt * 16 + i] = rs;
     }
 #pragma unroll
     for (int nb = 0; nb < 4; ++nb) {
@@ -168,7 +188,7 @@ __global__ __launch_bounds__(256) void gemm_abt_kernel(GemmArgs g) {
             const uint32_t row = rt * 16 + 4 * gq + r;
             float v = acc[nb][r];
             if (EPI == EPI_PART) {
-                g.part[((size_t)slice * g.M + row) * g.Nc + col] = v;
+                g.part[(size_t)slice * g.pstride + (size_t)row * g.Nc + col] = v;
             } else {
                 if (EPI == EPI_FWD) v = fmaxf(v + g.bias[col], 0.0f);
                 else if (!((float)g.mask[(size_t)row * g.Nc + col] > 0.0f)) v = 0.0f;
@@ -258,15 +278,20 @@ __global__ __launch_bounds__(256) void mlp_out_kernel(const bf16* __restrict__ H
     if (t == 0) pp[kOutPart - 1] = sl[0];
 }
 
-// out[i] = sum_s part[s * stride + i] * scale, in slice order
-__global__ __launch_bounds__(256) void reduce_kernel(const float* __restrict__ part, uint32_t slices, uint32_t stride,
-                                                     uint32_t len, float scale, float* __restrict__ out) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= len) return;
+
+// The output layer's block partials (kOutPart per block: dW4, db4, loss),
+// summed in block order: dW4 and db4 into out, the loss (scaled) into *loss
+// when asked for
+__global__ __launch_bounds__(256) void reduce_out_kernel(const float* __restrict__ part, uint32_t blocks,
+                                                         float* __restrict__ out, float loss_scale,
+                                                         float* __restrict__ loss) {
+    const uint32_t i = threadIdx.x;
+    if (i >= (uint32_t)kOutPart || (i == (uint32_t)kOutPart - 1 && !loss)) return;
     float acc = 0.0f;
 #pragma unroll 8
-    for (uint32_t s = 0; s < slices; ++s) acc += part[(size_t)s * stride + i];  // loads in flight, sums in order
-    out[i] = acc * scale;
+    for (uint32_t s = 0; s < blocks; ++s) acc += part[(size_t)s * kOutPart + i];
+    if (i < (uint32_t)kOutPart - 1) out[i] = acc;
+    else *loss = acc * loss_scale;
 }
 
 // out[i] = scale * sum_s part[s * stride + i] for many slices: a workgroup
@@ -293,36 +318,34 @@ __global__ __launch_bounds__(64 * kRedGroups) void reduce_slices_kernel(const fl
     if (gq == 0 && i < len) out[i] = sh[0][l] * scale;
 }
 
-// db[r] = sum over the row r of dZT [rows][B]
-__global__ __launch_bounds__(256) void colsum_kernel(const bf16* __restrict__ dZT, uint32_t B, float* __restrict__ db) {
-    __shared__ float sh[256];
-    const uint32_t r = blockIdx.x;
-    float acc = 0.0f;
-    const bf16x8* row = reinterpret_cast<const bf16x8*>(dZT + (size_t)r * B);  // B is a multiple of 256
-#pragma unroll 4
-    for (uint32_t b = threadIdx.x; b < B / 8; b += 256) {
-        const bf16x8 v = row[b];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) acc += (float)v[q];
-    }
-    sh[threadIdx.x] = acc;
-    for (uint32_t h = 128; h > 0; h >>= 1) {
-        __syncthreads();
-        if (threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
-    }
-    if (threadIdx.x == 0) db[r] = sh[0];
-}
 
+// Adam step of every parameter; the updated weights also refresh their BF16
+// working copies (as mlp_cast_kernel does), so a step needs no cast launch.
 __global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ P, const float* __restrict__ G,
                                                    float* __restrict__ m, float* __restrict__ v, uint32_t n, float lr,
-                                                   float bc1, float bc2) {
+                                                   float bc1, float bc2, bf16* __restrict__ W1, bf16* __restrict__ W2,
+                                                   bf16* __restrict__ W3, bf16* __restrict__ W2T,
+                                                   bf16* __restrict__ W3T) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const float g = G[i];
     const float mi = 0.9f * m[i] + 0.1f * g, vi = 0.999f * v[i] + 0.001f * g * g;
     m[i] = mi;
     v[i] = vi;
-    P[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + 1e-8f);
+    const float p = P[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + 1e-8f);
+    P[i] = p;
+    const bf16 h = (bf16)p;
+    if (i - (uint32_t)hvx::kMlpW1 < (uint32_t)(kH * kIn)) W1[i - hvx::kMlpW1] = h;
+    if (i - (uint32_t)hvx::kMlpW2 < (uint32_t)(kH * kH)) {
+        const uint32_t j = i - hvx::kMlpW2;
+        W2[j] = h;
+        W2T[(j % kH) * kH + j / kH] = h;
+    }
+    if (i - (uint32_t)hvx::kMlpW3 < (uint32_t)(kH * kH)) {
+        const uint32_t j = i - hvx::kMlpW3;
+        W3[j] = h;
+        W3T[(j % kH) * kH + j / kH] = h;
+    }
 }
 
 // BF16 working copies: W1, W2, W3 as [out][in]; W2^T, W3^T as [in][out]
@@ -429,6 +452,9 @@ struct hvx_mlp {
     float *part = nullptr, *opart = nullptr;
     uint32_t splitk = 1;
     uint64_t steps_done = 0;
+    // a training call's inputs encoded once: [n][kIn] and [kIn][enc_cap]
+    bf16 *enc = nullptr, *encT = nullptr;
+    size_t enc_cap = 0;
     std::vector<void*> batch_allocs;
     std::vector<void*> allocs;
 };
@@ -474,7 +500,7 @@ hipError_t ensure_batch(hvx_mlp* M, uint32_t B) {
         if ((e = dalloc(L, &M->H[l], (size_t)B * kH)) || (e = dalloc(L, &M->HT[l], (size_t)B * kH))) return e;
     if ((e = dalloc(L, &M->dZa, (size_t)B * kH)) || (e = dalloc(L, &M->dZaT, (size_t)B * kH)) ||
         (e = dalloc(L, &M->dZb, (size_t)B * kH)) || (e = dalloc(L, &M->dZbT, (size_t)B * kH)) ||
-        (e = dalloc(L, &M->part, (size_t)M->splitk * kH * kIn)) ||
+        (e = dalloc(L, &M->part, (size_t)M->splitk * (kH * kIn + kH))) ||
         (e = dalloc(L, &M->opart, (size_t)(B / 256) * kOutPart)))
         return e;
     M->cap = B;
@@ -486,12 +512,14 @@ template <int EPI> void gemm(const GemmArgs& g, hipStream_t s) {
     hipLaunchKernelGGL(gemm_abt_kernel<EPI>, dim3((waves + 3) / 4), dim3(256), 0, s, g);
 }
 
-// forward of B (padded) samples, n of them real
-void forward(hvx_mlp* M, const float4* in, uint32_t n, uint32_t B, hipStream_t s) {
-    hipLaunchKernelGGL(mlp_pe_kernel, dim3((B + 63) / 64), dim3(64), 0, s, in, n, B, M->lod_max, M->X0, M->X0T);
+// forward of B (padded) samples, n of them real; x0: their encoding, if
+// already made
+void forward(hvx_mlp* M, const float4* in, uint32_t n, uint32_t B, hipStream_t s, const bf16* x0 = nullptr) {
+    if (!x0)
+        hipLaunchKernelGGL(mlp_pe_kernel, dim3((B + 63) / 64), dim3(64), 0, s, in, n, B, M->lod_max, M->X0, M->X0T);
     const bf16* Ws[3] = {M->W1, M->W2, M->W3};
     const int bias[3] = {hvx::kMlpB1, hvx::kMlpB2, hvx::kMlpB3};
-    const bf16* prev = M->X0;
+    const bf16* prev = x0 ? x0 : M->X0;
     for (int l = 0; l < 3; ++l) {
         GemmArgs g{};
         g.A = prev;
@@ -509,34 +537,39 @@ void forward(hvx_mlp* M, const float4* in, uint32_t n, uint32_t B, hipStream_t s
 }
 
 // gradient of the mean squared log-radiance error over B samples into G (and
-// the loss into *loss)
-void gradient(hvx_mlp* M, const float4* in, const float4* tgt, uint32_t B, float* G, float* loss, hipStream_t s) {
-    forward(M, in, B, B, s);
+// the loss into *loss); x0 / x0t (row stride ldx): the samples' encoding, if
+// already made
+void gradient(hvx_mlp* M, const float4* in, const float4* tgt, uint32_t B, float* G, float* loss, hipStream_t s,
+              const bf16* x0 = nullptr, const bf16* x0t = nullptr, uint32_t ldx = 0) {
+    forward(M, in, B, B, s, x0);
     const uint32_t nblk = B / 256;
     hipLaunchKernelGGL(mlp_out_kernel<true>, dim3(nblk), dim3(256), 0, s, M->H[2], M->P, tgt, B, B,
                        2.0f / (3.0f * (float)B), nullptr, M->dZa, M->dZaT, M->opart);
-    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(256), 0, s, M->opart, nblk, (uint32_t)kOutPart,
-                       (uint32_t)(kOutPart - 1), 1.0f, G + hvx::kMlpW4);
-    if (loss)
-        hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(256), 0, s, M->opart + kOutPart - 1, nblk, (uint32_t)kOutPart,
-                           1u, 1.0f / (3.0f * (float)B), loss);
-    const int wofs[3] = {hvx::kMlpW1, hvx::kMlpW2, hvx::kMlpW3}, bofs[3] = {hvx::kMlpB1, hvx::kMlpB2, hvx::kMlpB3};
+    hipLaunchKernelGGL(reduce_out_kernel, dim3(1), dim3(256), 0, s, M->opart, nblk, G + hvx::kMlpW4,
+                       1.0f / (3.0f * (float)B), loss);
+    const int wofs[3] = {hvx::kMlpW1, hvx::kMlpW2, hvx::kMlpW3};
     const bf16* WT[3] = {nullptr, M->W2T, M->W3T};
     bf16 *dz = M->dZa, *dzT = M->dZaT, *nz = M->dZb, *nzT = M->dZbT;
     for (int l = 2; l >= 0; --l) {
         const uint32_t K = l == 0 ? kIn : kH;
         GemmArgs g{};
         g.A = dzT;
-        g.Bt = l == 0 ? M->X0T : M->HT[l - 1];
+        g.Bt = l == 0 ? (x0t ? x0t : M->X0T) : M->HT[l - 1];
+        g.ldb = l == 0 && x0t ? ldx : 0;
         g.M = kH;
         g.Nc = K;
         g.Kd = B;
         g.splitk = M->splitk;
         g.part = M->part;
+        g.pstride = kH * K + kH;
+        g.colsum = 1;
         gemm<EPI_PART>(g, s);
-        hipLaunchKernelGGL(reduce_slices_kernel, dim3((kH * K + 63) / 64), dim3(64 * kRedGroups), 0, s, M->part,
-                           M->splitk, (uint32_t)(kH * K), (uint32_t)(kH * K), 1.0f, G + wofs[l]);
-        hipLaunchKernelGGL(colsum_kernel, dim3(kH), dim3(256), 0, s, dzT, B, G + bofs[l]);
+        // W_l and b_l are adjacent in the parameter vector: one reduction
+        static_assert(hvx::kMlpB1 == hvx::kMlpW1 + kH * kIn && hvx::kMlpB2 == hvx::kMlpW2 + kH * kH &&
+                          hvx::kMlpB3 == hvx::kMlpW3 + kH * kH,
+                      "each layer's bias follows its weights");
+        hipLaunchKernelGGL(reduce_slices_kernel, dim3((kH * K + kH + 63) / 64), dim3(64 * kRedGroups), 0, s,
+                           M->part, M->splitk, (uint32_t)(kH * K + kH), (uint32_t)(kH * K + kH), 1.0f, G + wofs[l]);
         if (l > 0) {
             GemmArgs d{};
             d.A = dz;
@@ -629,6 +662,8 @@ int hvx_mlp_destroy(hvx_mlp* M) {
     (void)hipDeviceSynchronize();
     for (void* p : M->batch_allocs) (void)hipFree(p);
     for (void* p : M->allocs) (void)hipFree(p);
+    if (M->enc) (void)hipFree(M->enc);
+    if (M->encT) (void)hipFree(M->encT);
     delete M;
     return HVX_OK;
 }
@@ -691,9 +726,35 @@ int hvx_mlp_train(hvx_mlp* M, const float* d_inputs, const float* d_targets, uin
     const uint32_t per_epoch = n / batch;
     const float4* in = reinterpret_cast<const float4*>(d_inputs);
     const float4* tg = reinterpret_cast<const float4*>(d_targets);
+    // Encode the call's samples once (2 x 256 B per sample) when they are
+    // revisited, instead of once per step; without the memory, each step
+    // encodes its batch.
+    const size_t npad = ((size_t)per_epoch * batch + 255) / 256 * 256;
+    if (steps > per_epoch && M->enc_cap < npad) {
+        if (M->enc) (void)hipFree(M->enc);
+        if (M->encT) (void)hipFree(M->encT);
+        M->enc = M->encT = nullptr;
+        M->enc_cap = 0;
+        if (hipMalloc(reinterpret_cast<void**>(&M->enc), npad * kIn * sizeof(bf16)) == hipSuccess &&
+            hipMalloc(reinterpret_cast<void**>(&M->encT), npad * kIn * sizeof(bf16)) == hipSuccess) {
+            M->enc_cap = npad;
+        } else {
+            if (M->enc) (void)hipFree(M->enc);
+            M->enc = nullptr;
+            (void)hipGetLastError();  // a failed allocation leaves the slower path, not an error
+        }
+    }
+    const bool pre = steps > per_epoch && M->enc_cap >= npad;
+    if (pre)
+        hipLaunchKernelGGL(mlp_pe_kernel, dim3((uint32_t)(npad / 64)), dim3(64), 0, s, in, per_epoch * batch,
+                           (uint32_t)npad, M->lod_max, M->enc, M->encT);
     for (uint32_t k = 0; k < steps; ++k) {
         const size_t off = (size_t)(k % per_epoch) * batch;
-        gradient(M, in + off, tg + off, batch, M->G, d_loss_curve ? d_loss_curve + k : nullptr, s);
+        if (pre)
+            gradient(M, in + off, tg + off, batch, M->G, d_loss_curve ? d_loss_curve + k : nullptr, s,
+                     M->enc + off * kIn, M->encT + off, (uint32_t)npad);
+        else
+            gradient(M, in + off, tg + off, batch, M->G, d_loss_curve ? d_loss_curve + k : nullptr, s);
         // cosine decay from lr to lr / 100 over the call
         const double ph = steps > 1 ? (double)k / (double)(steps - 1) : 0.0;
         const float lr_k = (float)(lr * (0.01 + 0.99 * 0.5 * (1.0 + std::cos(hvx::kMlpPi * ph))));
@@ -701,8 +762,7 @@ int hvx_mlp_train(hvx_mlp* M, const float* d_inputs, const float* d_targets, uin
         const float bc1 = (float)(1.0 - std::pow(0.9, (double)M->steps_done));
         const float bc2 = (float)(1.0 - std::pow(0.999, (double)M->steps_done));
         hipLaunchKernelGGL(adam_kernel, dim3((hvx::kMlpParams + 255) / 256), dim3(256), 0, s, M->P, M->G, M->m, M->v,
-                           (uint32_t)hvx::kMlpParams, lr_k, bc1, bc2);
-        cast(M, s);
+                           (uint32_t)hvx::kMlpParams, lr_k, bc1, bc2, M->W1, M->W2, M->W3, M->W2T, M->W3T);
     }
     e = hipGetLastError();
     return e == hipSuccess ? HVX_OK : mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_train: %s", hipGetErrorString(e));
