@@ -14,12 +14,12 @@
 // a member box's, term by term and under rounding, so a light that fails the
 // union test fails every member test). Survivors are compacted into an LDS
 // candidate list in ascending light order with wavefront ballots and a
-// workgroup prefix sum (two barriers per step). Once enough candidates are
-// staged, every wave tests its clusters against them, 64 candidates per step,
-// and appends hits with a second ballot + mbcnt prefix, keeping the first
-// `cap` in ascending order — exactly the reference's list. The per-test
-// arithmetic is the reference's, in the same order, with contraction off, so
-// lists match it bit for bit.
+// workgroup prefix sum (one barrier per step, the counts double-buffered).
+// Once enough candidates are staged, every wave tests its clusters against
+// them, 64 candidates per step, and appends hits with a second ballot + mbcnt
+// prefix, keeping the first `cap` in ascending order — exactly the
+// reference's list. The per-test arithmetic is the reference's, in the same
+// order, with contraction off, so lists match it bit for bit.
 #include <hip/hip_runtime.h>
 
 #include "hvx_device.h"
@@ -122,7 +122,7 @@ template <int GX, int GY, int GZ> __host__ __device__ inline uint32_t cull_group
 struct CullShared {
     float4 pos[kCandMax];
     uint32_t idx[kCandMax];
-    uint32_t wcnt[kLPL][kWaves];
+    uint32_t wcnt[2][kLPL][kWaves];  // double-buffered by step parity
     uint32_t wdone[kWaves];
 };
 
@@ -182,7 +182,7 @@ __device__ __forceinline__ void cull_group(const CullJob& J, uint32_t g, CullSha
                 UW.mx[a] = gmax(UW.mx[a], cb[j].mx[a]);
             }
 
-    uint32_t base = 0, ncand = 0;
+    uint32_t base = 0, ncand = 0, ph = 0;
     // The next step's lights are loaded while this step is filtered. Loads
     // past the last light re-read it (the light table always holds at least
     // one entry), which keeps them unconditional; the lanes' hits are masked.
@@ -211,7 +211,7 @@ __device__ __forceinline__ void cull_group(const CullJob& J, uint32_t g, CullSha
 #pragma unroll
         for (int j = 0; j < kLPL; ++j) {
             m[j] = __ballot(hit[j]);
-            if (lane == 0) sh.wcnt[j][wave] = (uint32_t)__popcll(m[j]);
+            if (lane == 0) sh.wcnt[ph][j][wave] = (uint32_t)__popcll(m[j]);
         }
         __syncthreads();
         uint32_t off[kLPL], tot = 0;
@@ -220,7 +220,7 @@ __device__ __forceinline__ void cull_group(const CullJob& J, uint32_t g, CullSha
             off[j] = 0;
 #pragma unroll
             for (int w = 0; w < kWaves; ++w) {
-                const uint32_t c = sh.wcnt[j][w];
+                const uint32_t c = sh.wcnt[ph][j][w];
                 if (w == (int)wave) off[j] = tot;
                 tot += c;
             }
@@ -233,10 +233,13 @@ __device__ __forceinline__ void cull_group(const CullJob& J, uint32_t g, CullSha
                 sh.idx[pos] = base + 256u * j + tid;
             }
         }
-        __syncthreads();
+        // the next step's counts go to the other buffer, so only level 2
+        // (which reads every wave's candidates) needs a barrier here
+        ph ^= 1u;
         ncand += tot;
         base += kTile;
         if (ncand < (uint32_t)kFlush && base < n) continue;
+        __syncthreads();
 
         // ---- level 2: each wave tests its clusters against the staged
         // candidates, 64 per step: a lane reads its candidate once, skips
