@@ -81,6 +81,10 @@ struct ShadeArgs {
     // tile bounds it is built from: X + 1 column starts, then Y + 1 row starts
     uint32_t* order;
     uint32_t* tiles;
+    // binned output (nullable): RGB planes in fragment-order positions, which
+    // unbin_kernel moves to pixel order
+    float* out_binned;
+    uint32_t unbin_cap;  // floats of unbin_kernel's LDS tile image
 };
 
 __global__ __launch_bounds__(256) void shade_light_prep_kernel(const hvx_light* __restrict__ lights, uint32_t n,
@@ -160,10 +164,14 @@ __device__ __forceinline__ float mlp_cell_error(const ShadeArgs& a, f3 R, float 
 // Returns whether the learned radiance was used (cell_err >= 0 offers it:
 // the network's value `learned` and the measured relative error of its cell).
 template <bool EXACT>
-__device__ __forceinline__ bool shade_one(const ShadeArgs& a, uint32_t idx, uint32_t off, uint32_t count,
-                                          float cell_err = -1.0f, f3 learned = f3{0.0f, 0.0f, 0.0f}) {
+__device__ __forceinline__ bool shade_one(const ShadeArgs& a, uint32_t idx, uint32_t oidx, uint32_t off,
+                                          uint32_t count, float cell_err = -1.0f,
+                                          f3 learned = f3{0.0f, 0.0f, 0.0f}) {
     const float PI = kPiF;
     const uint32_t nfrag = a.W * a.H;
+    // the RGB goes to pixel idx of out, or to position oidx of the binned planes
+    float* const dst = a.out_binned ? a.out_binned : a.out;
+    const uint32_t di = a.out_binned ? oidx : idx;
     const uint32_t px = idx % a.W, py = idx / a.W;
     const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
     const float z = a.depth[idx];
@@ -365,7 +373,7 @@ __device__ __forceinline__ bool shade_one(const ShadeArgs& a, uint32_t idx, uint
             const float diffuse = irr3[c] * alb[c];
             const float specular = pre3[c] * (F * brdf.x + brdf.y);
             const float ambient = (kD * diffuse + specular) * 1.0f;
-            a.out[(size_t)c * nfrag + idx] = (ambient + 0.0f) + Lo[c];
+            dst[(size_t)c * nfrag + di] = (ambient + 0.0f) + Lo[c];
         }
         return false;
     }
@@ -398,7 +406,7 @@ __device__ __forceinline__ bool shade_one(const ShadeArgs& a, uint32_t idx, uint
     for (int c = 0; c < 3; ++c) {
         const float specular = pre3[c] * sw[c];
         const float ambient = (kD[c] * diffuse[c] + specular) * 1.0f;
-        a.out[(size_t)c * nfrag + idx] = (ambient + 0.0f) + Lo[c];
+        dst[(size_t)c * nfrag + di] = (ambient + 0.0f) + Lo[c];
     }
     return use;
 }
@@ -429,6 +437,36 @@ __global__ __launch_bounds__(256) void tile_bounds_kernel(ShadeArgs a) {
     for (uint32_t t = first; t <= c; ++t) b[t] = p;
     if (p == n - 1)
         for (uint32_t t = c + 1; t <= T; ++t) b[t] = n;
+}
+
+// One workgroup per screen tile: moves the tile's segment of the binned RGB
+// planes to pixel order through an LDS image of the tile, one plane at a
+// time, so that both the shading kernel's stores (fragment order) and these
+// (tile rows) are coalesced.
+__global__ __launch_bounds__(kBinThreads) void unbin_kernel(ShadeArgs a) {
+    extern __shared__ float s_img[];
+    const uint32_t cx = blockIdx.x % a.X, cy = blockIdx.x / a.X, t = threadIdx.x;
+    const uint32_t x0 = a.tiles[cx], x1 = a.tiles[cx + 1];
+    const uint32_t y0 = a.tiles[a.X + 1 + cy], y1 = a.tiles[a.X + 1 + cy + 1];
+    const uint32_t w = x1 - x0, area = w * (y1 - y0), nfrag = a.W * a.H;
+    if (area == 0) return;  // uniform over the workgroup
+    const uint32_t start = y0 * a.W + (y1 - y0) * x0;
+    if (area > a.unbin_cap) {  // larger than the bound the image was sized by: scatter directly
+        for (uint32_t c = 0; c < 3; ++c)
+            for (uint32_t i = t; i < area; i += kBinThreads)
+                a.out[(size_t)c * nfrag + a.order[start + i]] = a.out_binned[(size_t)c * nfrag + start + i];
+        return;
+    }
+    for (uint32_t c = 0; c < 3; ++c) {
+        for (uint32_t i = t; i < area; i += kBinThreads) {
+            const uint32_t idx = a.order[start + i];
+            s_img[(idx / a.W - y0) * w + (idx % a.W - x0)] = a.out_binned[(size_t)c * nfrag + start + i];
+        }
+        __syncthreads();
+        for (uint32_t i = t; i < area; i += kBinThreads)
+            a.out[(size_t)c * nfrag + (y0 + i / w) * a.W + x0 + i % w] = s_img[i];
+        __syncthreads();
+    }
 }
 
 // One workgroup per screen tile: counting sort of the tile's pixels by depth
@@ -590,7 +628,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_W
                             c2 = __shfl(y, 32 + (int)i, 64);
                 if ((lane >> 4) == (uint32_t)T) learned = f3{c0, c1, c2};
             }
-            if (idx < nfrag) mine += shade_one<EXACT>(a, idx, s_off[local], s_cnt[local], err, learned) ? 1u : 0u;
+            if (idx < nfrag)
+                mine += shade_one<EXACT>(a, idx, base + local, s_off[local], s_cnt[local], err, learned) ? 1u : 0u;
         }
         if (a.served) {
             if (mine) atomicAdd(&s_served, mine);
@@ -602,7 +641,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_W
         for (int r = 0; r < kChunk / 256; ++r) {
             const uint32_t local = s_order[r * 256 + t];
             const uint32_t idx = s_idx[local];
-            if (idx < nfrag) shade_one<EXACT>(a, idx, s_off[local], s_cnt[local]);
+            if (idx < nfrag) shade_one<EXACT>(a, idx, base + local, s_off[local], s_cnt[local]);
         }
     }
 }
@@ -611,8 +650,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_W
 
 bool shade_binned(const hvx_cf_ubo& ubo) { return ubo.slice_count_z <= kBinMaxZ; }
 
+// LDS bytes of unbin_kernel's tile image: a bound on the largest tile (a tile
+// spans at most ceil(W / X) + 1 columns and ceil(H / Y) + 1 rows)
+static size_t unbin_lds_bytes(const hvx_frag_soa& fr, const hvx_cf_ubo& ubo) {
+    const size_t tw = (fr.width + ubo.slice_count_x - 1) / ubo.slice_count_x + 1;
+    const size_t th = (fr.height + ubo.slice_count_y - 1) / ubo.slice_count_y + 1;
+    return tw * th * sizeof(float);
+}
+constexpr size_t kUnbinMaxLds = 64 * 1024;
+
 size_t shade_bin_scratch_bytes(const hvx_frag_soa& fr, const hvx_cf_ubo& ubo) {
-    return ((size_t)fr.width * fr.height + ubo.slice_count_x + ubo.slice_count_y + 2) * sizeof(uint32_t);
+    const size_t F = (size_t)fr.width * fr.height;
+    const size_t out = unbin_lds_bytes(fr, ubo) <= kUnbinMaxLds ? 3 * F : 0;  // binned RGB planes
+    return (F + ubo.slice_count_x + ubo.slice_count_y + 2 + out) * sizeof(uint32_t);
 }
 
 hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const hvx_cf_ubo& ubo,
@@ -665,9 +715,15 @@ hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const
     }
     const uint32_t F = fr.width * fr.height;
     const dim3 grid((F + kChunk - 1) / kChunk);
-    if (bin_scratch && shade_binned(ubo)) {
+    const size_t unbin_lds = unbin_lds_bytes(fr, ubo);
+    const bool binned = bin_scratch && shade_binned(ubo), staged = binned && unbin_lds <= kUnbinMaxLds;
+    if (binned) {
         a.order = bin_scratch;
         a.tiles = bin_scratch + F;
+        if (staged) {
+            a.out_binned = reinterpret_cast<float*>(bin_scratch + F + a.X + a.Y + 2);
+            a.unbin_cap = (uint32_t)(unbin_lds / sizeof(float));
+        }
         hipLaunchKernelGGL(tile_bounds_kernel, dim3((fr.width + fr.height + 255) / 256), dim3(256), 0, s, a);
         hipLaunchKernelGGL(bin_kernel, dim3(a.X * a.Y), dim3(kBinThreads), 0, s, a);
     }
@@ -682,6 +738,7 @@ hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const
         if (learned) hipLaunchKernelGGL((shade_kernel<false, true>), grid, dim3(256), 0, s, a);
         else hipLaunchKernelGGL((shade_kernel<false, false>), grid, dim3(256), 0, s, a);
     }
+    if (staged) hipLaunchKernelGGL(unbin_kernel, dim3(a.X * a.Y), dim3(kBinThreads), unbin_lds, s, a);
     return hipGetLastError();
 }
 
