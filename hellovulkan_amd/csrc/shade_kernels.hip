@@ -8,15 +8,17 @@
 // Scheduling. Fragments are first binned by cluster (bin_kernel): one
 // workgroup per screen tile counting-sorts the tile's pixels by depth slice,
 // so the fragment order lists each cluster's fragments together. A shading
-// workgroup takes 1024 consecutive entries of that order, resolves their
-// light lists, counting-sorts them by list length in LDS and hands them to
-// lanes in that order. The lanes of a wave then mostly share one light list:
-// their light loads hit one or two cache lines per instruction instead of 64
-// (with depth drawn independently per pixel, neighbouring pixels sit in
-// unrelated clusters, and per-lane gathers of light records bound the loop
-// by the L1 tag rate rather than by arithmetic). Each fragment is still
-// shaded start to finish by one lane, summing its lights in list order, so
-// the order of the fragments changes no result.
+// workgroup takes 1024 consecutive entries of that order and hands them to
+// lanes in order, so the lanes of a wave mostly share one light list: their
+// list entries and light records are wave-uniform and come through the
+// scalar cache (with depth drawn independently per pixel, neighbouring pixels
+// sit in unrelated clusters, and per-lane gathers of light records bound the
+// loop by the L1 tag rate rather than by arithmetic). Grids too deep to bin
+// are counting-sorted by light count within each workgroup instead. Each
+// fragment is still shaded start to finish by one lane, summing its lights in
+// list order, so the order of the fragments changes no result. The RGB is
+// written at the fragments' binned positions (coalesced) and unbin_kernel
+// moves it to pixel order through an LDS image of each tile.
 //
 // Arithmetic, two variants:
 //  exact  every expression as the FP32 oracle evaluates it (IEEE operations,
