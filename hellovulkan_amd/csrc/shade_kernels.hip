@@ -87,7 +87,13 @@ struct ShadeArgs {
     // unbin_kernel moves to pixel order
     float* out_binned;
     uint32_t unbin_cap;  // floats of unbin_kernel's LDS tile image
+    uint32_t rclasses;   // binning sub-key classes per slice: 2 splits by roughness, else 1
 };
+
+// Fragments at least this rough take hardware-rsqrt light and half vectors
+// (fast variant, waves whose every lane qualifies): GGX D amplifies an error
+// in N.H by at most ~1/a^2 <= 4.2 there (a = roughness^2 >= 0.49).
+constexpr float kRoughFastH = 0.7f;
 
 __global__ __launch_bounds__(256) void shade_light_prep_kernel(const hvx_light* __restrict__ lights, uint32_t n,
                                                                float intensity, float4* __restrict__ lrec) {
@@ -249,15 +255,30 @@ __device__ __forceinline__ bool shade_one(const ShadeArgs& a, uint32_t idx, uint
             om[c] = 1.0f - F0[c];
         }
         // one light's term, added to Lo
+        // wave-uniform: every lane rough enough for the fast light vectors
+        const bool fastH = __all(rough >= kRoughFastH);
         auto light = [&](const float4 A, const float4 B) {
-            // exact: L, H and N.H (max with 0 only differs in the sign of a
-            // zero, which no later term can observe)
             const f3 d = {A.x - wp.x, A.y - wp.y, A.z - wp.z};
-            const float s = sqrt_rn_normal(dot3(d, d));
-            const f3 L = mul3(d, rcp_rn_normal(s));
-            const f3 h = add3(V, L);
-            const f3 Hh = mul3(h, rcp_rn_normal(sqrt_rn_normal(dot3(h, h))));
-            const float NoH = fmaxf(dot3(N, Hh), 0.0f);
+            float s, NoH;
+            f3 L, Hh;
+            if (fastH) {
+                // L and H from the hardware reciprocal square root (a few ulp)
+                const float dd = __builtin_fmaf(d.z, d.z, __builtin_fmaf(d.y, d.y, d.x * d.x));
+                const float r = __builtin_amdgcn_rsqf(dd);
+                s = dd * r;
+                L = mul3(d, r);
+                const f3 h = add3(V, L);
+                Hh = mul3(h, __builtin_amdgcn_rsqf(__builtin_fmaf(h.z, h.z, __builtin_fmaf(h.y, h.y, h.x * h.x))));
+                NoH = fmaxf(__builtin_fmaf(N.z, Hh.z, __builtin_fmaf(N.y, Hh.y, N.x * Hh.x)), 0.0f);
+            } else {
+                // exact: L, H and N.H (max with 0 only differs in the sign of
+                // a zero, which no later term can observe)
+                s = sqrt_rn_normal(dot3(d, d));
+                L = mul3(d, rcp_rn_normal(s));
+                const f3 h = add3(V, L);
+                Hh = mul3(h, rcp_rn_normal(sqrt_rn_normal(dot3(h, h))));
+                NoH = fmaxf(dot3(N, Hh), 0.0f);
+            }
             const float NoL = fmaxf(__builtin_fmaf(N.z, L.z, __builtin_fmaf(N.y, L.y, N.x * L.x)), 0.0f);
             const float HoV = __builtin_fmaf(Hh.z, V.z, __builtin_fmaf(Hh.y, V.y, Hh.x * V.x));
             // fast: N.L and H.V fused (they enter only smooth factors), then
@@ -471,6 +492,14 @@ __global__ __launch_bounds__(kBinThreads) void unbin_kernel(ShadeArgs a) {
     }
 }
 
+// Binning key of a fragment: its depth slice, then (rclasses = 2) whether it
+// is rough enough for the fast light vectors, so a cluster's fragments are
+// contiguous and its rough ones fill whole waves.
+__device__ __forceinline__ uint32_t bin_of(const ShadeArgs& a, uint32_t idx) {
+    const uint32_t z = slice_of(a, idx);
+    return a.rclasses == 2 ? 2 * z + (a.rough[idx] >= kRoughFastH ? 1u : 0u) : z;
+}
+
 // One workgroup per screen tile: counting sort of the tile's pixels by depth
 // slice into the tile's segment of the order. Tiles are laid out row band by
 // row band, so tile (cx, cy) starts at y0 * W + (y1 - y0) * x0. Within a
@@ -484,16 +513,17 @@ __global__ __launch_bounds__(kBinThreads) void bin_kernel(ShadeArgs a) {
     const uint32_t w = x1 - x0, area = w * (y1 - y0);
     if (area == 0) return;  // uniform over the workgroup
     const uint32_t start = y0 * a.W + (y1 - y0) * x0;
-    for (uint32_t z = t; z < a.Z; z += kBinThreads) s_hist[z] = 0;
+    for (uint32_t z = t; z < a.Z * a.rclasses; z += kBinThreads) s_hist[z] = 0;
     __syncthreads();
+    const uint32_t nb = a.Z * a.rclasses;  // bins: (slice, roughness class)
     for (uint32_t i = t; i < area; i += kBinThreads) {
         const uint32_t idx = (y0 + i / w) * a.W + x0 + i % w;
-        atomicAdd(&s_hist[slice_of(a, idx)], 1u);
+        atomicAdd(&s_hist[bin_of(a, idx)], 1u);
     }
     __syncthreads();
     // exclusive scan of the Z <= kBinThreads slice counts: per wave, then
     // over the wave totals
-    const uint32_t v = t < a.Z ? s_hist[t] : 0u, lane = t & 63, wv = t >> 6;
+    const uint32_t v = t < nb ? s_hist[t] : 0u, lane = t & 63, wv = t >> 6;
     uint32_t inc = v;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -513,11 +543,11 @@ __global__ __launch_bounds__(kBinThreads) void bin_kernel(ShadeArgs a) {
         if (t < kBinThreads / 64) s_wave[t] = wi - wt;
     }
     __syncthreads();
-    if (t < a.Z) s_hist[t] = start + s_wave[wv] + inc - v;
+    if (t < nb) s_hist[t] = start + s_wave[wv] + inc - v;
     __syncthreads();
     for (uint32_t i = t; i < area; i += kBinThreads) {
         const uint32_t idx = (y0 + i / w) * a.W + x0 + i % w;
-        a.order[atomicAdd(&s_hist[slice_of(a, idx)], 1u)] = idx;
+        a.order[atomicAdd(&s_hist[bin_of(a, idx)], 1u)] = idx;
     }
 }
 
@@ -722,6 +752,7 @@ hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const
     if (binned) {
         a.order = bin_scratch;
         a.tiles = bin_scratch + F;
+        a.rclasses = !exact && 2 * a.Z <= kBinMaxZ ? 2u : 1u;
         if (staged) {
             a.out_binned = reinterpret_cast<float*>(bin_scratch + F + a.X + a.Y + 2);
             a.unbin_cap = (uint32_t)(unbin_lds / sizeof(float));
