@@ -333,7 +333,9 @@ int hvx_mlp_gradient(hvx_mlp* mlp, const float* d_inputs, const float* d_targets
                      float* d_loss, void* stream);
 /* `steps` Adam steps over consecutive batches of the n samples (cosine
  * learning-rate decay from lr to lr/100); d_loss_curve (nullable) receives
- * each step's loss. */
+ * each step's loss. A call that revisits its samples (steps > n / batch)
+ * encodes them once into network-owned memory (512 B per sample, kept for
+ * later calls); without that memory each step encodes its batch. */
 int hvx_mlp_train(hvx_mlp* mlp, const float* d_inputs, const float* d_targets, uint32_t n, uint32_t batch,
                   uint32_t steps, float lr, float* d_loss_curve, void* stream);
 /* Radiance (RGB, n x 3 floats) predicted at n inputs (dir.xyz, lod), by the
