@@ -482,6 +482,15 @@ struct DenseTable {
 // PG probes per wave (grid dimension: probe group): each weight fragment is
 // loaded once and multiplied into every probe of the group; each probe's
 // accumulation is the same sequence as with PG = 1.
+// Wave w of a job: four consecutive waves take four row-tile pairs of one K
+// slice (a job's pairs are padded to a multiple of four), so a workgroup's
+// waves share the slice's B operand.
+__device__ __forceinline__ void dense_wave(const DenseJob& J, uint32_t w, uint32_t& pair, uint32_t& slice) {
+    const uint32_t per = 4 * J.splitk, grp = w / per, r = w - grp * per;
+    slice = r / 4;
+    pair = grp * 4 + (r & 3);
+}
+
 template <int PG>
 __global__ __launch_bounds__(256) void dense_gemm_kernel(Probes Q, DenseTable T) {
     const uint2 wb = gemm_block(Q);
@@ -491,9 +500,11 @@ __global__ __launch_bounds__(256) void dense_gemm_kernel(Probes Q, DenseTable T)
     uint32_t jj = 0;
     while (jj + 1 < T.n && gw >= T.j[jj + 1].wave_begin) ++jj;
     const DenseJob& J = T.j[jj];
-    const uint32_t wave = gw - J.wave_begin, rt = J.rt, ks = J.ks, splitk = J.splitk;
-    const uint32_t pair = wave / splitk, slice = wave - pair * splitk;
+    const uint32_t rt = J.rt, ks = J.ks, splitk = J.splitk;
+    uint32_t pair, slice;
+    dense_wave(J, gw - J.wave_begin, pair, slice);
     const uint32_t rt0 = 2 * pair;
+    if (rt0 >= rt) return;  // padding wave (uniform per wave)
     const bool has1 = rt0 + 1 < rt;
     const uint32_t kb = (uint32_t)((uint64_t)slice * ks / splitk), ke = (uint32_t)((uint64_t)(slice + 1) * ks / splitk);
     const uint32_t i = lane & 15, g = lane >> 4;
@@ -544,6 +555,122 @@ __global__ __launch_bounds__(256) void dense_gemm_kernel(Probes Q, DenseTable T)
             }
         }
     }
+#pragma unroll
+    for (int q = 0; q < PG; ++q) {
+        if ((uint32_t)q >= np) break;
+        float* p = scr(Q, J.part, p0 + q) + (size_t)slice * rt * 16 * 32;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const uint32_t row0 = rt0 * 16 + 4 * g + r;
+            p[(size_t)row0 * 32 + i] = acc[q][0][r];
+            p[(size_t)row0 * 32 + 16 + i] = acc[q][1][r];
+            if (has1) {
+                p[(size_t)(row0 + 16) * 32 + i] = acc[q][2][r];
+                p[(size_t)(row0 + 16) * 32 + 16 + i] = acc[q][3][r];
+            }
+        }
+    }
+}
+
+// The grouped dense GEMM with each K step's B operand (PG probes' 4 x 32
+// half8, plus the lo parts when split) staged once per workgroup in LDS,
+// double buffered, as near_gemm_staged_kernel does: the workgroup's four
+// waves share the K slice. Padding waves stage and synchronise but compute
+// nothing.
+template <int PG>
+__global__ __launch_bounds__(256) void dense_gemm_staged_kernel(Probes Q, DenseTable T) {
+    const uint2 wb = gemm_block(Q);
+    const uint32_t gw0 = wb.x * 4;
+    if (gw0 >= T.waves) return;  // uniform per workgroup: jobs hold multiples of four waves
+    const uint32_t t = threadIdx.x, gw = gw0 + (t >> 6), lane = t & 63, p0 = wb.y * PG;
+    const uint32_t np = min((uint32_t)PG, Q.n - p0);
+    uint32_t jj = 0;
+    while (jj + 1 < T.n && gw0 >= T.j[jj + 1].wave_begin) ++jj;
+    const DenseJob& J = T.j[jj];
+    const uint32_t rt = J.rt, ks = J.ks, splitk = J.splitk;
+    uint32_t pair, slice;
+    dense_wave(J, gw - J.wave_begin, pair, slice);
+    const uint32_t rt0 = 2 * pair;
+    const bool active = rt0 < rt, has1 = rt0 + 1 < rt;
+    const uint32_t kb = (uint32_t)((uint64_t)slice * ks / splitk), ke = (uint32_t)((uint64_t)(slice + 1) * ks / splitk);
+    const uint32_t i = lane & 15, g = lane >> 4;
+    const half8* w0 = J.W + (active ? (size_t)rt0 * ks * 64 + lane : 0);
+    const half8* w1 = has1 ? w0 + (size_t)ks * 64 : w0;
+    const half8* Rp[PG];
+#pragma unroll
+    for (int q = 0; q < PG; ++q) Rp[q] = scr(Q, J.R, p0 + ((uint32_t)q < np ? q : 0));
+    const bool split = J.w_lo != 0;
+    const uint32_t nB = PG * 128 * (split ? 2u : 1u);  // half8 per step
+    constexpr uint32_t kMaxB = PG * 128 * 2, kPer = (kMaxB + 255) / 256;
+    __shared__ half8 sB[2][kMaxB];
+    auto b_load = [&](uint32_t s, half8 (&v)[kPer]) {
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            const uint32_t e = t + 256u * k, lo = e / (PG * 128), q = (e / 128) % PG, o = e % 128;
+            if (e < nB) v[k] = Rp[q][(size_t)s * 128 + o + (lo ? J.r_lo : 0)];
+        }
+    };
+    f32x4 acc[PG][4];
+#pragma unroll
+    for (int q = 0; q < PG; ++q)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc[q][k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    half8 nx[kPer];
+    if (kb < ke) {
+        b_load(kb, nx);
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k)
+            if (t + 256u * k < nB) sB[0][t + 256u * k] = nx[k];
+    }
+    __syncthreads();
+    for (uint32_t s = kb; s < ke; ++s) {
+        const bool more = s + 1 < ke;
+        if (more) b_load(s + 1, nx);
+        const half8* sb = sB[(s - kb) & 1];
+        if (active) {
+            if (split) {
+                const bf16x8* v0 = reinterpret_cast<const bf16x8*>(w0);
+                const bf16x8* v1 = reinterpret_cast<const bf16x8*>(w1);
+                const size_t o = (size_t)s * 64;
+                const bf16x8 x0 = v0[o], x1 = v1[o], y0 = v0[o + J.w_lo], y1 = v1[o + J.w_lo];
+#pragma unroll
+                for (int q = 0; q < PG; ++q) {
+                    const bf16x8* rv = reinterpret_cast<const bf16x8*>(sb + q * 128 + g * 32 + i);
+                    const bf16x8 b0 = rv[0], b1 = rv[16], c0 = rv[PG * 128], c1 = rv[PG * 128 + 16];
+                    acc[q][0] = hvx::mfma16x16x32(x0, b0, acc[q][0]);
+                    acc[q][0] = hvx::mfma16x16x32(x0, c0, acc[q][0]);
+                    acc[q][0] = hvx::mfma16x16x32(y0, b0, acc[q][0]);
+                    acc[q][1] = hvx::mfma16x16x32(x0, b1, acc[q][1]);
+                    acc[q][1] = hvx::mfma16x16x32(x0, c1, acc[q][1]);
+                    acc[q][1] = hvx::mfma16x16x32(y0, b1, acc[q][1]);
+                    acc[q][2] = hvx::mfma16x16x32(x1, b0, acc[q][2]);
+                    acc[q][2] = hvx::mfma16x16x32(x1, c0, acc[q][2]);
+                    acc[q][2] = hvx::mfma16x16x32(y1, b0, acc[q][2]);
+                    acc[q][3] = hvx::mfma16x16x32(x1, b1, acc[q][3]);
+                    acc[q][3] = hvx::mfma16x16x32(x1, c1, acc[q][3]);
+                    acc[q][3] = hvx::mfma16x16x32(y1, b1, acc[q][3]);
+                }
+            } else {
+                const half8 x0 = w0[(size_t)s * 64], x1 = w1[(size_t)s * 64];
+#pragma unroll
+                for (int q = 0; q < PG; ++q) {
+                    const half8* rv = sb + q * 128 + g * 32 + i;
+                    const half8 b0 = rv[0], b1 = rv[16];
+                    acc[q][0] = hvx::mfma16x16x32(x0, b0, acc[q][0]);
+                    acc[q][1] = hvx::mfma16x16x32(x0, b1, acc[q][1]);
+                    acc[q][2] = hvx::mfma16x16x32(x1, b0, acc[q][2]);
+                    acc[q][3] = hvx::mfma16x16x32(x1, b1, acc[q][3]);
+                }
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (uint32_t k = 0; k < kPer; ++k)
+                if (t + 256u * k < nB) sB[(s + 1 - kb) & 1][t + 256u * k] = nx[k];
+        }
+        __syncthreads();
+    }
+    if (!active) return;
 #pragma unroll
     for (int q = 0; q < PG; ++q) {
         if ((uint32_t)q >= np) break;
@@ -1197,7 +1324,7 @@ int launch_call(hvx_contract_plan* P, const Probes& Q, uint32_t n, uint32_t chai
     if (P->dense.n) {
         const dim3 grid = G.probe_fast ? dim3(groups, dense_wgs) : dim3(dense_wgs, groups);
         if (grouped)
-            hipLaunchKernelGGL(dense_gemm_kernel<kProbeGroup>, grid, dim3(256), 0, s, G, P->dense);
+            hipLaunchKernelGGL(dense_gemm_staged_kernel<kProbeGroup>, grid, dim3(256), 0, s, G, P->dense);
         else
             hipLaunchKernelGGL(dense_gemm_kernel<1>, grid, dim3(256), 0, s, G, P->dense);
         hipLaunchKernelGGL(dense_epilogue_kernel, dim3((P->dense.epi + 255) / 256, n), dim3(256), 0, s, Q, P->dense,
@@ -1386,7 +1513,7 @@ int hvx_ibl_contract_plan_create(hvx_ctx* ctx, int type, uint32_t face, uint32_t
         J.epi_begin = DT.epi;
         J.w_lo = D.split ? (uint64_t)D.G.rt * D.G.ks * 64 : 0;
         J.r_lo = D.split ? (uint64_t)6 * D.G.Sin * D.G.Sin / 8 * 32 : 0;
-        DT.waves += (D.G.rt + 1) / 2 * D.splitk;
+        DT.waves += ((D.G.rt + 1) / 2 + 3) / 4 * 4 * D.splitk;  // pairs padded to a multiple of four
         DT.epi += 6 * D.G.rows;
         if (M.L.kind == hvx::CONTRACT_SPLIT) {
             const StagePad& sp = P->sp[M.pad];
