@@ -254,9 +254,9 @@ __device__ __forceinline__ bool shade_one(const ShadeArgs& a, uint32_t idx, uint
             q[c] = (oneMinusMetal * alb[c]) * (1.0f / kPiF);
             om[c] = 1.0f - F0[c];
         }
-        // one light's term, added to Lo
         // wave-uniform: every lane rough enough for the fast light vectors
         const bool fastH = __all(rough >= kRoughFastH);
+        // one light's term, added to Lo
         auto light = [&](const float4 A, const float4 B) {
             const f3 d = {A.x - wp.x, A.y - wp.y, A.z - wp.z};
             float s, NoH;
