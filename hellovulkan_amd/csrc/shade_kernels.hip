@@ -434,10 +434,21 @@ __device__ __forceinline__ bool shade_one(const ShadeArgs& a, uint32_t idx, uint
     return use;
 }
 
+// Fragments per workgroup (cfg1, 3840x2160): plain shading 64 (2.07 ms; 128:
+// 2.09, 256: 2.11, 1024: 2.34); learned shading 256 (3.35 ms; 128: 4.69, 64:
+// 8.12), since each of its workgroups first stages the network's weights.
 #ifndef HVX_SHADE_CHUNK
-#define HVX_SHADE_CHUNK 1024
+#define HVX_SHADE_CHUNK 64
 #endif
-constexpr int kChunk = HVX_SHADE_CHUNK;   // fragments per workgroup
+#ifndef HVX_SHADE_CHUNK_LEARNED
+#define HVX_SHADE_CHUNK_LEARNED 256
+#endif
+template <bool LEARNED> struct ShadeChunk {
+    static constexpr int kChunk = LEARNED ? HVX_SHADE_CHUNK_LEARNED : HVX_SHADE_CHUNK;
+    static constexpr int kThreads = kChunk < 256 ? kChunk : 256;  // lanes per workgroup
+    static constexpr int kRounds = kChunk / kThreads;            // fragments per lane
+    static_assert(kChunk % kThreads == 0 && kThreads % 64 == 0, "whole waves, whole rounds");
+};
 constexpr int kBuckets = 64;   // light-count classes (count / 2, saturating)
 constexpr uint32_t kXcds = 8;  // MI355X accelerator complex dies
 
@@ -555,6 +566,8 @@ __global__ __launch_bounds__(kBinThreads) void bin_kernel(ShadeArgs a) {
 template <bool EXACT, bool LEARNED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_WAVES_PER_EU, 8))) void shade_kernel(
     ShadeArgs a) {
+    constexpr int kChunk = ShadeChunk<LEARNED>::kChunk, kThreads = ShadeChunk<LEARNED>::kThreads,
+                  kRounds = ShadeChunk<LEARNED>::kRounds;
     __shared__ uint32_t s_idx[kChunk];
     __shared__ uint32_t s_off[kChunk];
     __shared__ uint16_t s_cnt[kChunk];
@@ -579,10 +592,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_W
     const uint32_t t = threadIdx.x;
     if (t < kBuckets) s_hist[t] = 0;
     __syncthreads();
-    uint32_t bucket[kChunk / 256];
+    uint32_t bucket[kRounds];
 #pragma unroll
-    for (int r = 0; r < kChunk / 256; ++r) {
-        const uint32_t local = r * 256 + t, pos = base + local;
+    for (int r = 0; r < kRounds; ++r) {
+        const uint32_t local = r * kThreads + t, pos = base + local;
         const uint32_t idx = pos >= nfrag ? nfrag : a.order ? a.order[pos] : pos;
         s_idx[local] = idx;
         uint32_t off = 0, cnt = 0;
@@ -614,9 +627,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_W
         }
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < kChunk / 256; ++r) {
+        for (int r = 0; r < kRounds; ++r) {
             const uint32_t slot = atomicAdd(&s_hist[bucket[r]], 1u);
-            s_order[slot] = (uint16_t)(r * 256 + t);
+            s_order[slot] = (uint16_t)(r * kThreads + t);
         }
         __syncthreads();
     }
@@ -632,8 +645,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_W
         const uint32_t lane = t & 63, i = lane & 15;
         uint32_t mine = 0;
 #pragma unroll 1
-        for (int r = 0; r < kChunk / 256; ++r) {
-            const uint32_t local = s_order[r * 256 + t];
+        for (int r = 0; r < kRounds; ++r) {
+            const uint32_t local = s_order[r * kThreads + t];
             const uint32_t idx = s_idx[local];
             float x[4] = {0.0f, 0.0f, 1.0f, 0.0f};
             float err = -1.0f;
@@ -670,8 +683,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_W
         }
     } else {
 #pragma unroll 1
-        for (int r = 0; r < kChunk / 256; ++r) {
-            const uint32_t local = s_order[r * 256 + t];
+        for (int r = 0; r < kRounds; ++r) {
+            const uint32_t local = s_order[r * kThreads + t];
             const uint32_t idx = s_idx[local];
             if (idx < nfrag) shade_one<EXACT>(a, idx, base + local, s_off[local], s_cnt[local]);
         }
@@ -746,7 +759,8 @@ hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const
         a.served = learned->served;
     }
     const uint32_t F = fr.width * fr.height;
-    const dim3 grid((F + kChunk - 1) / kChunk);
+    const int chunk = learned ? ShadeChunk<true>::kChunk : ShadeChunk<false>::kChunk;
+    const dim3 grid((F + chunk - 1) / chunk), block(learned ? ShadeChunk<true>::kThreads : ShadeChunk<false>::kThreads);
     const size_t unbin_lds = unbin_lds_bytes(fr, ubo);
     const bool binned = bin_scratch && shade_binned(ubo), staged = binned && unbin_lds <= kUnbinMaxLds;
     if (binned) {
@@ -761,15 +775,15 @@ hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const
         hipLaunchKernelGGL(bin_kernel, dim3(a.X * a.Y), dim3(kBinThreads), 0, s, a);
     }
     if (exact) {
-        if (learned) hipLaunchKernelGGL((shade_kernel<true, true>), grid, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((shade_kernel<true, false>), grid, dim3(256), 0, s, a);
+        if (learned) hipLaunchKernelGGL((shade_kernel<true, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((shade_kernel<true, false>), grid, block, 0, s, a);
     } else {
         if (n_lights > 0) {
             hipLaunchKernelGGL(shade_light_prep_kernel, dim3((n_lights + 255) / 256), dim3(256), 0, s, d_lights,
                                n_lights, pc.light_intensity, light_scratch);
         }
-        if (learned) hipLaunchKernelGGL((shade_kernel<false, true>), grid, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((shade_kernel<false, false>), grid, dim3(256), 0, s, a);
+        if (learned) hipLaunchKernelGGL((shade_kernel<false, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((shade_kernel<false, false>), grid, block, 0, s, a);
     }
     if (staged) hipLaunchKernelGGL(unbin_kernel, dim3(a.X * a.Y), dim3(kBinThreads), unbin_lds, s, a);
     return hipGetLastError();
