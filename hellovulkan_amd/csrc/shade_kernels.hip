@@ -8,8 +8,8 @@
 // Scheduling. Fragments are first binned by cluster (bin_kernel): one
 // workgroup per screen tile counting-sorts the tile's pixels by depth slice,
 // so the fragment order lists each cluster's fragments together. A shading
-// workgroup takes 1024 consecutive entries of that order and hands them to
-// lanes in order, so the lanes of a wave mostly share one light list: their
+// workgroup takes 64 consecutive entries of that order (256 in the learned
+// variant) and hands them to lanes in order, so the lanes of a wave mostly share one light list: their
 // list entries and light records are wave-uniform and come through the
 // scalar cache (with depth drawn independently per pixel, neighbouring pixels
 // sit in unrelated clusters, and per-lane gathers of light records bound the
