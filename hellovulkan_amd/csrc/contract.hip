@@ -24,6 +24,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "hvx_contract.h"
@@ -477,6 +478,72 @@ struct DenseTable {
     uint32_t n, waves, epi;
 };
 
+// One K step of a wave's two row tiles x 32 columns: the weight fragments of
+// both row tiles (x[0..1]; split: hi, then lo in x[2..3]) and PG probes' two
+// 16-column B fragments (b[q][0..1]; split: hi, then lo in b[q][2..3]).
+template <bool SPLIT, int PG>
+struct KStep {
+    using V = std::conditional_t<SPLIT, bf16x8, half8>;
+    static constexpr int kW = SPLIT ? 4 : 2, kB = SPLIT ? 4 : 2;
+    V x[kW];
+    V b[PG][kB];
+};
+
+// acc[q][2 * h + nb] += row tile h x column block nb; split: hi*hi + hi*lo +
+// lo*hi (the lo*lo term is below the FP32 accumulation's resolution)
+template <bool SPLIT, int PG>
+__device__ __forceinline__ void kstep_mac(f32x4 (&acc)[PG][4], const KStep<SPLIT, PG>& o) {
+#pragma unroll
+    for (int q = 0; q < PG; ++q) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb) {
+                f32x4& a = acc[q][2 * h + nb];
+                a = hvx::mfma16x16x32(o.x[h], o.b[q][nb], a);
+                if constexpr (SPLIT) {
+                    a = hvx::mfma16x16x32(o.x[h], o.b[q][2 + nb], a);
+                    a = hvx::mfma16x16x32(o.x[2 + h], o.b[q][nb], a);
+                }
+            }
+    }
+}
+
+// Runs n >= 1 K steps as a software pipeline over groups of U steps: a
+// group's loads are issued before the MFMAs of the group loaded before it.
+// load_next(KStep&) loads the next step and advances a cursor that stops at
+// the last step (the group loaded ahead of the final one repeats it, unused).
+// The accumulation order is that of a plain loop over the steps.
+template <uint32_t U, bool SPLIT, int PG, class LoadNext>
+__device__ __forceinline__ void kstep_pipeline(f32x4 (&acc)[PG][4], uint32_t n, LoadNext load_next) {
+    KStep<SPLIT, PG> cur[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) load_next(cur[u]);
+    uint32_t s = 0;
+    for (; s + U <= n; s += U) {
+        KStep<SPLIT, PG> nx[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) load_next(nx[u]);
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) kstep_mac<SPLIT, PG>(acc, cur[u]);
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) cur[u] = nx[u];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u)
+        if (s + u < n) kstep_mac<SPLIT, PG>(acc, cur[u]);
+}
+
+#ifndef HVX_KSTEP_UNROLL
+#define HVX_KSTEP_UNROLL 8
+#endif
+// FP16 K steps whose loads are in flight together (half as many split-BF16
+// steps, which load twice the fragments): the near field's largest mip runs
+// about two waves per SIMD, too few to stream the weights otherwise
+constexpr uint32_t kStepUnroll = HVX_KSTEP_UNROLL;
+template <bool SPLIT>
+constexpr uint32_t kstep_unroll() { return SPLIT ? (kStepUnroll + 1) / 2 : kStepUnroll; }
+
 // Dense: a wave owns two row tiles (32 rows) x 32 columns over one K slice;
 // partial sums go to part[slice][row][32] and are reduced in slice order.
 // PG probes per wave (grid dimension: probe group): each weight fragment is
@@ -518,43 +585,42 @@ __global__ __launch_bounds__(256) void dense_gemm_kernel(Probes Q, DenseTable T)
     for (int q = 0; q < PG; ++q)
 #pragma unroll
         for (int k = 0; k < 4; ++k) acc[q][k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    if (J.w_lo) {  // split BF16: hi*hi + hi*lo + lo*hi
-        const bf16x8* v0 = reinterpret_cast<const bf16x8*>(w0);
-        const bf16x8* v1 = reinterpret_cast<const bf16x8*>(w1);
-        for (uint32_t s = kb; s < ke; ++s) {
-            const size_t o = (size_t)s * 64, qo = (size_t)s * 128;
-            const bf16x8 x0 = v0[o], x1 = v1[o], y0 = v0[o + J.w_lo], y1 = v1[o + J.w_lo];
+    // steps [kb, ke) of the slice; an empty slice (ke == kb) loads step kb,
+    // which exists (kb < ks), and accumulates nothing
+    auto run = [&](auto split_tag) {
+        constexpr bool SPLIT = decltype(split_tag)::value;
+        using Op = KStep<SPLIT, PG>;
+        using V = typename Op::V;
+        using GV = const __attribute__((address_space(1))) V*;
+        const GV v0 = (GV)reinterpret_cast<const V*>(w0), v1 = (GV)reinterpret_cast<const V*>(w1);
+        GV rv[PG];
+#pragma unroll
+        for (int q = 0; q < PG; ++q) rv[q] = (GV)reinterpret_cast<const V*>(rb[q]);
+        uint32_t ls = kb;
+        kstep_pipeline<kstep_unroll<SPLIT>(), SPLIT, PG>(acc, ke - kb, [&](Op& o) {
+            const size_t wo = (size_t)ls * 64, qo = (size_t)ls * 128;
+            o.x[0] = v0[wo];
+            o.x[1] = v1[wo];
+            if constexpr (SPLIT) {
+                o.x[2] = v0[wo + J.w_lo];
+                o.x[3] = v1[wo + J.w_lo];
+            }
 #pragma unroll
             for (int q = 0; q < PG; ++q) {
-                const bf16x8* rv = reinterpret_cast<const bf16x8*>(rb[q]);
-                const bf16x8 b0 = rv[qo], b1 = rv[qo + 16], c0 = rv[qo + J.r_lo], c1 = rv[qo + 16 + J.r_lo];
-                acc[q][0] = hvx::mfma16x16x32(x0, b0, acc[q][0]);
-                acc[q][0] = hvx::mfma16x16x32(x0, c0, acc[q][0]);
-                acc[q][0] = hvx::mfma16x16x32(y0, b0, acc[q][0]);
-                acc[q][1] = hvx::mfma16x16x32(x0, b1, acc[q][1]);
-                acc[q][1] = hvx::mfma16x16x32(x0, c1, acc[q][1]);
-                acc[q][1] = hvx::mfma16x16x32(y0, b1, acc[q][1]);
-                acc[q][2] = hvx::mfma16x16x32(x1, b0, acc[q][2]);
-                acc[q][2] = hvx::mfma16x16x32(x1, c0, acc[q][2]);
-                acc[q][2] = hvx::mfma16x16x32(y1, b0, acc[q][2]);
-                acc[q][3] = hvx::mfma16x16x32(x1, b1, acc[q][3]);
-                acc[q][3] = hvx::mfma16x16x32(x1, c1, acc[q][3]);
-                acc[q][3] = hvx::mfma16x16x32(y1, b1, acc[q][3]);
+                o.b[q][0] = rv[q][qo];
+                o.b[q][1] = rv[q][qo + 16];
+                if constexpr (SPLIT) {
+                    o.b[q][2] = rv[q][qo + J.r_lo];
+                    o.b[q][3] = rv[q][qo + 16 + J.r_lo];
+                }
             }
-        }
-    } else {
-        for (uint32_t s = kb; s < ke; ++s) {
-            const half8 x0 = w0[(size_t)s * 64], x1 = w1[(size_t)s * 64];
-#pragma unroll
-            for (int q = 0; q < PG; ++q) {
-                const half8 b0 = rb[q][(size_t)s * 128], b1 = rb[q][(size_t)s * 128 + 16];
-                acc[q][0] = hvx::mfma16x16x32(x0, b0, acc[q][0]);
-                acc[q][1] = hvx::mfma16x16x32(x0, b1, acc[q][1]);
-                acc[q][2] = hvx::mfma16x16x32(x1, b0, acc[q][2]);
-                acc[q][3] = hvx::mfma16x16x32(x1, b1, acc[q][3]);
-            }
-        }
-    }
+            ls += ls + 1 < ke ? 1u : 0u;
+        });
+    };
+    if (J.w_lo)  // split BF16
+        run(std::true_type{});
+    else
+        run(std::false_type{});
 #pragma unroll
     for (int q = 0; q < PG; ++q) {
         if ((uint32_t)q >= np) break;
@@ -769,53 +835,50 @@ __global__ __launch_bounds__(512) void near_gemm_kernel(Probes Q, NearTable T, c
     // contiguous 16 KB block
     constexpr size_t kStep = 16 * 64, kRow = 64;
     const half8* w0 = J.W + Tl.woff + (size_t)rt0 * kRow + lane;
-    const half8* w1 = w0 + kRow;
     const uint32_t xb = (uint32_t)(Tl.x0 + (int)J.B) / 8 + g;
     f32x4 acc[PG][4];
 #pragma unroll
     for (int q = 0; q < PG; ++q)
 #pragma unroll
         for (int k = 0; k < 4; ++k) acc[q][k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    uint32_t r = 0, cc = 0;
-    for (uint32_t s = 0; s < KS; ++s) {
-        const size_t pidx = (((size_t)(Tl.y0 + (int)J.B + (int)r) * J.Pw8 + xb + cc * 4) * 32) + i;
+    // operands read through global (not generic) pointers, so each MFMA
+    // waits only for its own loads
+    using Op = KStep<SPLIT, PG>;
+    using V = typename Op::V;
+    using GV = const __attribute__((address_space(1))) V*;
+    const GV wv = (GV)reinterpret_cast<const V*>(w0);
+    GV pv[PG];
+#pragma unroll
+    for (int q = 0; q < PG; ++q) pv[q] = (GV)reinterpret_cast<const V*>(Pp[q]);
+    // load cursor: step ls at window row lr, 32-column block lcc. A tile's
+    // window spans at least 3 rows and 32 columns (the host pads the
+    // footprint by a texel on each side), so KS >= 3.
+    uint32_t ls = 0, lr = 0, lcc = 0;
+    kstep_pipeline<kstep_unroll<SPLIT>(), SPLIT, PG>(acc, KS, [&](Op& o) {
+        const size_t pidx = (((size_t)(Tl.y0 + (int)J.B + (int)lr) * J.Pw8 + xb + lcc * 4) * 32) + i;
+        const size_t o0 = (size_t)ls * kStep, o1 = o0 + kRow;
+        o.x[0] = wv[o0];
+        o.x[1] = wv[o1];
         if constexpr (SPLIT) {
-            const bf16x8* wv = reinterpret_cast<const bf16x8*>(w0);
-            const size_t o0 = (size_t)s * kStep, o1 = o0 + kRow;
-            const bf16x8 x0 = wv[o0], x1 = wv[o1], y0 = wv[o0 + J.w_lo], y1 = wv[o1 + J.w_lo];
+            o.x[2] = wv[o0 + J.w_lo];
+            o.x[3] = wv[o1 + J.w_lo];
+        }
 #pragma unroll
-            for (int q = 0; q < PG; ++q) {
-                const bf16x8* pv = reinterpret_cast<const bf16x8*>(Pp[q]);
-                const bf16x8 b0 = pv[pidx], b1 = pv[pidx + 16], c0 = pv[pidx + J.p_lo], c1 = pv[pidx + 16 + J.p_lo];
-                acc[q][0] = hvx::mfma16x16x32(x0, b0, acc[q][0]);
-                acc[q][0] = hvx::mfma16x16x32(x0, c0, acc[q][0]);
-                acc[q][0] = hvx::mfma16x16x32(y0, b0, acc[q][0]);
-                acc[q][1] = hvx::mfma16x16x32(x0, b1, acc[q][1]);
-                acc[q][1] = hvx::mfma16x16x32(x0, c1, acc[q][1]);
-                acc[q][1] = hvx::mfma16x16x32(y0, b1, acc[q][1]);
-                acc[q][2] = hvx::mfma16x16x32(x1, b0, acc[q][2]);
-                acc[q][2] = hvx::mfma16x16x32(x1, c0, acc[q][2]);
-                acc[q][2] = hvx::mfma16x16x32(y1, b0, acc[q][2]);
-                acc[q][3] = hvx::mfma16x16x32(x1, b1, acc[q][3]);
-                acc[q][3] = hvx::mfma16x16x32(x1, c1, acc[q][3]);
-                acc[q][3] = hvx::mfma16x16x32(y1, b1, acc[q][3]);
-            }
-        } else {
-            const half8 x0 = w0[(size_t)s * kStep], x1 = w1[(size_t)s * kStep];
-#pragma unroll
-            for (int q = 0; q < PG; ++q) {
-                const half8 b0 = Pp[q][pidx], b1 = Pp[q][pidx + 16];
-                acc[q][0] = hvx::mfma16x16x32(x0, b0, acc[q][0]);
-                acc[q][1] = hvx::mfma16x16x32(x0, b1, acc[q][1]);
-                acc[q][2] = hvx::mfma16x16x32(x1, b0, acc[q][2]);
-                acc[q][3] = hvx::mfma16x16x32(x1, b1, acc[q][3]);
+        for (int q = 0; q < PG; ++q) {
+            o.b[q][0] = pv[q][pidx];
+            o.b[q][1] = pv[q][pidx + 16];
+            if constexpr (SPLIT) {
+                o.b[q][2] = pv[q][pidx + J.p_lo];
+                o.b[q][3] = pv[q][pidx + 16 + J.p_lo];
             }
         }
-        if (++cc == wc) {
-            cc = 0;
-            ++r;
-        }
-    }
+        const uint32_t step = ls + 1 < KS ? 1u : 0u;  // selects, not branches, between the loads
+        ls += step;
+        lcc += step;
+        const bool wrap = lcc == wc;
+        lr += wrap ? 1u : 0u;
+        lcc = wrap ? 0u : lcc;
+    });
     const uint32_t So = J.So, X0 = (tile % J.te) * 16, Y0 = (tile / J.te) * 16;
     const size_t So2 = (size_t)So * So;
 #pragma unroll
