@@ -478,32 +478,33 @@ struct DenseTable {
     uint32_t n, waves, epi;
 };
 
-// One K step of a wave's two row tiles x 32 columns: the weight fragments of
-// both row tiles (x[0..1]; split: hi, then lo in x[2..3]) and PG probes' two
-// 16-column B fragments (b[q][0..1]; split: hi, then lo in b[q][2..3]).
-template <bool SPLIT, int PG>
+// One K step of a wave's RT (1 or 2) row tiles x 32 columns: the weight
+// fragments of the row tiles (x[0..RT-1]; split: hi, then lo in
+// x[RT..2RT-1]) and PG probes' two 16-column B fragments (b[q][0..1]; split:
+// hi, then lo in b[q][2..3]).
+template <bool SPLIT, int PG, int RT = 2>
 struct KStep {
     using V = std::conditional_t<SPLIT, bf16x8, half8>;
-    static constexpr int kW = SPLIT ? 4 : 2, kB = SPLIT ? 4 : 2;
+    static constexpr int kW = SPLIT ? 2 * RT : RT, kB = SPLIT ? 4 : 2;
     V x[kW];
     V b[PG][kB];
 };
 
 // acc[q][2 * h + nb] += row tile h x column block nb; split: hi*hi + hi*lo +
 // lo*hi (the lo*lo term is below the FP32 accumulation's resolution)
-template <bool SPLIT, int PG>
-__device__ __forceinline__ void kstep_mac(f32x4 (&acc)[PG][4], const KStep<SPLIT, PG>& o) {
+template <bool SPLIT, int PG, int RT>
+__device__ __forceinline__ void kstep_mac(f32x4 (&acc)[PG][2 * RT], const KStep<SPLIT, PG, RT>& o) {
 #pragma unroll
     for (int q = 0; q < PG; ++q) {
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
+        for (int h = 0; h < RT; ++h)
 #pragma unroll
             for (int nb = 0; nb < 2; ++nb) {
                 f32x4& a = acc[q][2 * h + nb];
                 a = hvx::mfma16x16x32(o.x[h], o.b[q][nb], a);
                 if constexpr (SPLIT) {
                     a = hvx::mfma16x16x32(o.x[h], o.b[q][2 + nb], a);
-                    a = hvx::mfma16x16x32(o.x[2 + h], o.b[q][nb], a);
+                    a = hvx::mfma16x16x32(o.x[RT + h], o.b[q][nb], a);
                 }
             }
     }
@@ -514,24 +515,24 @@ __device__ __forceinline__ void kstep_mac(f32x4 (&acc)[PG][4], const KStep<SPLIT
 // load_next(KStep&) loads the next step and advances a cursor that stops at
 // the last step (the group loaded ahead of the final one repeats it, unused).
 // The accumulation order is that of a plain loop over the steps.
-template <uint32_t U, bool SPLIT, int PG, class LoadNext>
-__device__ __forceinline__ void kstep_pipeline(f32x4 (&acc)[PG][4], uint32_t n, LoadNext load_next) {
-    KStep<SPLIT, PG> cur[U];
+template <uint32_t U, bool SPLIT, int PG, int RT, class LoadNext>
+__device__ __forceinline__ void kstep_pipeline(f32x4 (&acc)[PG][2 * RT], uint32_t n, LoadNext load_next) {
+    KStep<SPLIT, PG, RT> cur[U];
 #pragma unroll
     for (uint32_t u = 0; u < U; ++u) load_next(cur[u]);
     uint32_t s = 0;
     for (; s + U <= n; s += U) {
-        KStep<SPLIT, PG> nx[U];
+        KStep<SPLIT, PG, RT> nx[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) load_next(nx[u]);
 #pragma unroll
-        for (uint32_t u = 0; u < U; ++u) kstep_mac<SPLIT, PG>(acc, cur[u]);
+        for (uint32_t u = 0; u < U; ++u) kstep_mac<SPLIT, PG, RT>(acc, cur[u]);
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) cur[u] = nx[u];
     }
 #pragma unroll
     for (uint32_t u = 0; u < U; ++u)
-        if (s + u < n) kstep_mac<SPLIT, PG>(acc, cur[u]);
+        if (s + u < n) kstep_mac<SPLIT, PG, RT>(acc, cur[u]);
 }
 
 #ifndef HVX_KSTEP_UNROLL
@@ -597,7 +598,7 @@ __global__ __launch_bounds__(256) void dense_gemm_kernel(Probes Q, DenseTable T)
 #pragma unroll
         for (int q = 0; q < PG; ++q) rv[q] = (GV)reinterpret_cast<const V*>(rb[q]);
         uint32_t ls = kb;
-        kstep_pipeline<kstep_unroll<SPLIT>(), SPLIT, PG>(acc, ke - kb, [&](Op& o) {
+        kstep_pipeline<kstep_unroll<SPLIT>(), SPLIT, PG, 2>(acc, ke - kb, [&](Op& o) {
             const size_t wo = (size_t)ls * 64, qo = (size_t)ls * 128;
             o.x[0] = v0[wo];
             o.x[1] = v1[wo];
@@ -810,19 +811,32 @@ constexpr uint32_t kNearWaves = 8;  // waves per 16x16 output tile (two row tile
 constexpr uint32_t kNearParts = HVX_NEAR_PARTS;  // workgroups per tile
 static_assert(kNearWaves % kNearParts == 0, "a tile splits into whole waves");
 static_assert(64 * kNearWaves / kNearParts == 256, "the near GEMM's LDS staging assumes 256-lane workgroups");
+#ifndef HVX_NEAR_ROWS
+#define HVX_NEAR_ROWS 2
+#endif
+// The single-probe near GEMM: row tiles per wave (a tile's 16 row tiles take
+// 16 / kNearRows waves, in 256-lane workgroups). One row tile per wave
+// doubles the waves that stream the weights but loads each B fragment twice
+// as often (from L2); four row tiles per wave halve the B loads and the
+// waves. Measured on the 512^2 GGX chain (FP16 near GEMM): two 203 us, one
+// 230 us, four 217 us. Each output's accumulation order is the same for all.
+constexpr uint32_t kNearRows = HVX_NEAR_ROWS;
+constexpr uint32_t kNearSingleParts = 16 / kNearRows / 4;
+static_assert(kNearRows == 1 || kNearRows == 2 || kNearRows == 4, "1, 2 or 4 row tiles per wave");
 
 template <bool SPLIT, int PG>
 __global__ __launch_bounds__(512) void near_gemm_kernel(Probes Q, NearTable T, const float* sc) {
     const uint2 wb = gemm_block(Q);
     const uint32_t p0 = wb.y * PG, np = min((uint32_t)PG, Q.n - p0);
     uint32_t jj = 0;
-    // workgroup b owns half b & 1 (eight row tiles) of tile b >> 1: finer
-    // work units than whole tiles balance the unequal mips across the CUs
-    const uint32_t gt = wb.x / kNearParts;
+    // workgroup b owns part b % kNearSingleParts (four waves' row tiles) of
+    // tile b / kNearSingleParts: finer work units than whole tiles balance
+    // the unequal mips across the CUs
+    const uint32_t gt = wb.x / kNearSingleParts;
     while (jj + 1 < T.n && gt >= T.j[jj + 1].tile_begin) ++jj;
     const NearJob& J = T.j[jj];
     const uint32_t tile = gt - J.tile_begin, lane = threadIdx.x & 63;
-    const uint32_t wave = (threadIdx.x >> 6) + (wb.x % kNearParts) * (kNearWaves / kNearParts);
+    const uint32_t wave = (threadIdx.x >> 6) + (wb.x % kNearSingleParts) * 4;
     const NearTile Tl = J.tiles[tile];
     // PG probes per wave: each weight fragment is loaded once for the group;
     // each probe's accumulation is the same sequence as with PG = 1
@@ -830,20 +844,20 @@ __global__ __launch_bounds__(512) void near_gemm_kernel(Probes Q, NearTable T, c
 #pragma unroll
     for (int q = 0; q < PG; ++q) Pp[q] = scr(Q, J.P, p0 + ((uint32_t)q < np ? q : 0));
     const uint32_t wc = Tl.w / 32, KS = Tl.h * wc;
-    const uint32_t i = lane & 15, g = lane >> 4, rt0 = 2 * wave;
+    const uint32_t i = lane & 15, g = lane >> 4, rt0 = kNearRows * wave;
     // weights of a tile stored K-step major: step s of all 16 row tiles is one
     // contiguous 16 KB block
     constexpr size_t kStep = 16 * 64, kRow = 64;
     const half8* w0 = J.W + Tl.woff + (size_t)rt0 * kRow + lane;
     const uint32_t xb = (uint32_t)(Tl.x0 + (int)J.B) / 8 + g;
-    f32x4 acc[PG][4];
+    f32x4 acc[PG][2 * kNearRows];
 #pragma unroll
     for (int q = 0; q < PG; ++q)
 #pragma unroll
-        for (int k = 0; k < 4; ++k) acc[q][k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (uint32_t k = 0; k < 2 * kNearRows; ++k) acc[q][k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     // operands read through global (not generic) pointers, so each MFMA
     // waits only for its own loads
-    using Op = KStep<SPLIT, PG>;
+    using Op = KStep<SPLIT, PG, kNearRows>;
     using V = typename Op::V;
     using GV = const __attribute__((address_space(1))) V*;
     const GV wv = (GV)reinterpret_cast<const V*>(w0);
@@ -854,14 +868,13 @@ __global__ __launch_bounds__(512) void near_gemm_kernel(Probes Q, NearTable T, c
     // window spans at least 3 rows and 32 columns (the host pads the
     // footprint by a texel on each side), so KS >= 3.
     uint32_t ls = 0, lr = 0, lcc = 0;
-    kstep_pipeline<kstep_unroll<SPLIT>(), SPLIT, PG>(acc, KS, [&](Op& o) {
+    kstep_pipeline<kstep_unroll<SPLIT>(), SPLIT, PG, kNearRows>(acc, KS, [&](Op& o) {
         const size_t pidx = (((size_t)(Tl.y0 + (int)J.B + (int)lr) * J.Pw8 + xb + lcc * 4) * 32) + i;
-        const size_t o0 = (size_t)ls * kStep, o1 = o0 + kRow;
-        o.x[0] = wv[o0];
-        o.x[1] = wv[o1];
-        if constexpr (SPLIT) {
-            o.x[2] = wv[o0 + J.w_lo];
-            o.x[3] = wv[o1 + J.w_lo];
+        const size_t o0 = (size_t)ls * kStep;
+#pragma unroll
+        for (uint32_t h = 0; h < kNearRows; ++h) {
+            o.x[h] = wv[o0 + h * kRow];
+            if constexpr (SPLIT) o.x[kNearRows + h] = wv[o0 + h * kRow + J.w_lo];
         }
 #pragma unroll
         for (int q = 0; q < PG; ++q) {
@@ -887,7 +900,7 @@ __global__ __launch_bounds__(512) void near_gemm_kernel(Probes Q, NearTable T, c
         float* out = reinterpret_cast<float*>(probe_out(Q, p0 + q) + J.out_off);
         const float* sq = scr(Q, sc, p0 + q);
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        for (uint32_t h = 0; h < kNearRows; ++h) {
             const uint32_t Y = Y0 + rt0 + h;
 #pragma unroll
             for (int nb = 0; nb < 2; ++nb) {
@@ -1380,8 +1393,9 @@ int launch_call(hvx_contract_plan* P, const Probes& Q, uint32_t n, uint32_t chai
     // each weight fragment, and the group index is the fastest grid dimension,
     // so groups reading one weight block run together and share it in the caches
     Probes G = Q;
-    const uint32_t dense_wgs = (P->dense.waves + 3) / 4, near_wgs = P->near.tiles * kNearParts;
     const bool grouped = n > 1;
+    const uint32_t dense_wgs = (P->dense.waves + 3) / 4,
+                   near_wgs = P->near.tiles * (grouped ? kNearParts : kNearSingleParts);
     const uint32_t groups = grouped ? (n + kProbeGroup - 1) / kProbeGroup : 1;
     G.probe_fast = grouped && std::max(dense_wgs, near_wgs) <= 65535u;
     if (P->dense.n) {
