@@ -666,77 +666,117 @@ __global__ __launch_bounds__(256) void dense_gemm_staged_kernel(Probes Q, DenseT
     const half8* Rp[PG];
 #pragma unroll
     for (int q = 0; q < PG; ++q) Rp[q] = scr(Q, J.R, p0 + ((uint32_t)q < np ? q : 0));
-    const bool split = J.w_lo != 0;
-    const uint32_t nB = PG * 128 * (split ? 2u : 1u);  // half8 per step
-    constexpr uint32_t kMaxB = PG * 128 * 2, kPer = (kMaxB + 255) / 256;
+    constexpr uint32_t kMaxB = PG * 128 * 2;  // half8 per step, split
     __shared__ half8 sB[2][kMaxB];
-    auto b_load = [&](uint32_t s, half8 (&v)[kPer]) {
+    // the B operand is read through global (not generic) pointers: generic
+    // loads count against the LDS counter too, so each step's LDS reads
+    // would wait for the next step's B loads
+    using GH = const __attribute__((address_space(1))) half8*;
+    GH Rg[PG];
 #pragma unroll
-        for (uint32_t k = 0; k < kPer; ++k) {
-            const uint32_t e = t + 256u * k, lo = e / (PG * 128), q = (e / 128) % PG, o = e % 128;
-            if (e < nB) v[k] = Rp[q][(size_t)s * 128 + o + (lo ? J.r_lo : 0)];
-        }
-    };
+    for (int q = 0; q < PG; ++q) Rg[q] = (GH)Rp[q];
     f32x4 acc[PG][4];
 #pragma unroll
     for (int q = 0; q < PG; ++q)
 #pragma unroll
         for (int k = 0; k < 4; ++k) acc[q][k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    half8 nx[kPer];
-    if (kb < ke) {
-        b_load(kb, nx);
+    // the job's precision is uniform per workgroup; each variant has its own
+    // loop so that its loads are unconditional (conditional loads make the
+    // waits conservative)
+    auto run = [&](auto split_tag) {
+        constexpr bool SPLIT = decltype(split_tag)::value;
+        constexpr uint32_t nB = PG * 128 * (SPLIT ? 2u : 1u), kPer = (nB + 255) / 256;
+        // element e = t + 256 k of a step's block: lo part e / (PG * 128),
+        // probe (e / 128) % PG, unit e % 128; with whole 256-element rounds
+        // and an even PG the part is the same for the whole round
+        constexpr bool kWhole = nB % 256 == 0 && (PG * 128) % 256 == 0;
+        auto b_load = [&](uint32_t s, half8 (&v)[kPer]) {
 #pragma unroll
-        for (uint32_t k = 0; k < kPer; ++k)
-            if (t + 256u * k < nB) sB[0][t + 256u * k] = nx[k];
-    }
-    __syncthreads();
-    for (uint32_t s = kb; s < ke; ++s) {
-        const bool more = s + 1 < ke;
-        if (more) b_load(s + 1, nx);
-        const half8* sb = sB[(s - kb) & 1];
-        if (active) {
-            if (split) {
-                const bf16x8* v0 = reinterpret_cast<const bf16x8*>(w0);
-                const bf16x8* v1 = reinterpret_cast<const bf16x8*>(w1);
-                const size_t o = (size_t)s * 64;
-                const bf16x8 x0 = v0[o], x1 = v1[o], y0 = v0[o + J.w_lo], y1 = v1[o + J.w_lo];
-#pragma unroll
-                for (int q = 0; q < PG; ++q) {
-                    const bf16x8* rv = reinterpret_cast<const bf16x8*>(sb + q * 128 + g * 32 + i);
-                    const bf16x8 b0 = rv[0], b1 = rv[16], c0 = rv[PG * 128], c1 = rv[PG * 128 + 16];
-                    acc[q][0] = hvx::mfma16x16x32(x0, b0, acc[q][0]);
-                    acc[q][0] = hvx::mfma16x16x32(x0, c0, acc[q][0]);
-                    acc[q][0] = hvx::mfma16x16x32(y0, b0, acc[q][0]);
-                    acc[q][1] = hvx::mfma16x16x32(x0, b1, acc[q][1]);
-                    acc[q][1] = hvx::mfma16x16x32(x0, c1, acc[q][1]);
-                    acc[q][1] = hvx::mfma16x16x32(y0, b1, acc[q][1]);
-                    acc[q][2] = hvx::mfma16x16x32(x1, b0, acc[q][2]);
-                    acc[q][2] = hvx::mfma16x16x32(x1, c0, acc[q][2]);
-                    acc[q][2] = hvx::mfma16x16x32(y1, b0, acc[q][2]);
-                    acc[q][3] = hvx::mfma16x16x32(x1, b1, acc[q][3]);
-                    acc[q][3] = hvx::mfma16x16x32(x1, c1, acc[q][3]);
-                    acc[q][3] = hvx::mfma16x16x32(y1, b1, acc[q][3]);
-                }
-            } else {
-                const half8 x0 = w0[(size_t)s * 64], x1 = w1[(size_t)s * 64];
-#pragma unroll
-                for (int q = 0; q < PG; ++q) {
-                    const half8* rv = sb + q * 128 + g * 32 + i;
-                    const half8 b0 = rv[0], b1 = rv[16];
-                    acc[q][0] = hvx::mfma16x16x32(x0, b0, acc[q][0]);
-                    acc[q][1] = hvx::mfma16x16x32(x0, b1, acc[q][1]);
-                    acc[q][2] = hvx::mfma16x16x32(x1, b0, acc[q][2]);
-                    acc[q][3] = hvx::mfma16x16x32(x1, b1, acc[q][3]);
+            for (uint32_t k = 0; k < kPer; ++k) {
+                const uint32_t e = t + 256u * k, q = (e / 128) % PG, o = e % 128;
+                if constexpr (kWhole) {
+                    const uint32_t lo = (256u * k) / (PG * 128);
+                    v[k] = Rg[q][(size_t)s * 128 + o + (lo ? J.r_lo : 0)];
+                } else {
+                    const uint32_t lo = e / (PG * 128);
+                    if (e < nB) v[k] = Rg[q][(size_t)s * 128 + o + (lo ? J.r_lo : 0)];
                 }
             }
-        }
-        if (more) {
+        };
+        auto b_store = [&](uint32_t buf, const half8 (&v)[kPer]) {
 #pragma unroll
             for (uint32_t k = 0; k < kPer; ++k)
-                if (t + 256u * k < nB) sB[(s + 1 - kb) & 1][t + 256u * k] = nx[k];
+                if (kWhole || t + 256u * k < nB) sB[buf][t + 256u * k] = v[k];
+        };
+        half8 nx[kPer];
+        if (kb < ke) {
+            b_load(kb, nx);
+            b_store(0, nx);
         }
         __syncthreads();
-    }
+        for (uint32_t s = kb; s < ke; ++s) {
+            const bool more = s + 1 < ke;
+            // this step's weights are loaded (every wave; an inactive wave's
+            // pointers are the job's first row) before the next step's B
+            // operand: the load counter completes in order, so the MFMAs'
+            // wait for the weights leaves the B loads in flight
+            const size_t o = (size_t)s * 64;
+            half8 xw[SPLIT ? 4 : 2];
+            xw[0] = w0[o];
+            xw[1] = w1[o];
+            if constexpr (SPLIT) {
+                xw[2] = w0[o + J.w_lo];
+                xw[3] = w1[o + J.w_lo];
+            }
+            b_load(more ? s + 1 : s, nx);       // the last step reloads itself
+            __asm__ volatile("" ::: "memory");  // keeps the B loads ahead of the MFMAs
+            const half8* sb = sB[(s - kb) & 1];
+            if (active) {
+                if constexpr (SPLIT) {
+                    const bf16x8 x0 = __builtin_bit_cast(bf16x8, xw[0]), x1 = __builtin_bit_cast(bf16x8, xw[1]);
+                    const bf16x8 y0 = __builtin_bit_cast(bf16x8, xw[2]), y1 = __builtin_bit_cast(bf16x8, xw[3]);
+#pragma unroll
+                    for (int q = 0; q < PG; ++q) {
+                        const bf16x8* rv = reinterpret_cast<const bf16x8*>(sb + q * 128 + g * 32 + i);
+                        const bf16x8 b0 = rv[0], b1 = rv[16], c0 = rv[PG * 128], c1 = rv[PG * 128 + 16];
+                        acc[q][0] = hvx::mfma16x16x32(x0, b0, acc[q][0]);
+                        acc[q][0] = hvx::mfma16x16x32(x0, c0, acc[q][0]);
+                        acc[q][0] = hvx::mfma16x16x32(y0, b0, acc[q][0]);
+                        acc[q][1] = hvx::mfma16x16x32(x0, b1, acc[q][1]);
+                        acc[q][1] = hvx::mfma16x16x32(x0, c1, acc[q][1]);
+                        acc[q][1] = hvx::mfma16x16x32(y0, b1, acc[q][1]);
+                        acc[q][2] = hvx::mfma16x16x32(x1, b0, acc[q][2]);
+                        acc[q][2] = hvx::mfma16x16x32(x1, c0, acc[q][2]);
+                        acc[q][2] = hvx::mfma16x16x32(y1, b0, acc[q][2]);
+                        acc[q][3] = hvx::mfma16x16x32(x1, b1, acc[q][3]);
+                        acc[q][3] = hvx::mfma16x16x32(x1, c1, acc[q][3]);
+                        acc[q][3] = hvx::mfma16x16x32(y1, b1, acc[q][3]);
+                    }
+                } else {
+                    const half8 x0 = xw[0], x1 = xw[1];
+#pragma unroll
+                    for (int q = 0; q < PG; ++q) {
+                        const half8* rv = sb + q * 128 + g * 32 + i;
+                        const half8 b0 = rv[0], b1 = rv[16];
+                        acc[q][0] = hvx::mfma16x16x32(x0, b0, acc[q][0]);
+                        acc[q][1] = hvx::mfma16x16x32(x0, b1, acc[q][1]);
+                        acc[q][2] = hvx::mfma16x16x32(x1, b0, acc[q][2]);
+                        acc[q][3] = hvx::mfma16x16x32(x1, b1, acc[q][3]);
+                    }
+                }
+            }
+            // stored unconditionally: after the last step the buffer is no
+            // longer read (its readers passed the previous barrier), and a
+            // store under a branch would draw the B loads into the branch,
+            // after the MFMAs
+            b_store((s + 1 - kb) & 1, nx);
+            __syncthreads();
+        }
+    };
+    if (J.w_lo != 0)  // split BF16
+        run(std::true_type{});
+    else
+        run(std::false_type{});
     if (!active) return;
 #pragma unroll
     for (int q = 0; q < PG; ++q) {
@@ -918,7 +958,7 @@ __global__ __launch_bounds__(512) void near_gemm_kernel(Probes Q, NearTable T, c
 }
 
 template <bool SPLIT, int PG>
-__global__ __launch_bounds__(512) void near_gemm_staged_kernel(Probes Q, NearTable T, const float* sc) {
+__global__ __launch_bounds__(256) void near_gemm_staged_kernel(Probes Q, NearTable T, const float* sc) {
     const uint2 wb = gemm_block(Q);
     const uint32_t p0 = wb.y * PG, np = min((uint32_t)PG, Q.n - p0);
     uint32_t jj = 0;
@@ -941,7 +981,6 @@ __global__ __launch_bounds__(512) void near_gemm_staged_kernel(Probes Q, NearTab
     // contiguous 16 KB block
     constexpr size_t kStep = 16 * 64, kRow = 64;
     const half8* w0 = J.W + Tl.woff + (size_t)rt0 * kRow + lane;
-    const half8* w1 = w0 + kRow;
     const uint32_t xb0 = (uint32_t)(Tl.x0 + (int)J.B) / 8;
     // The B operand of a K step (for each probe: 4 x 32 consecutive half8 of
     // the staged plane, plus as many lo parts when split) is the same for
@@ -956,12 +995,30 @@ __global__ __launch_bounds__(512) void near_gemm_staged_kernel(Probes Q, NearTab
     static_assert(kB % 256 == 0 || kB < 256, "the B block splits evenly over the workgroup");
     __shared__ half8 sB[2][kB];
     const uint32_t t = threadIdx.x;
+    // the B operand is read through global (not generic) pointers: generic
+    // loads count against the LDS counter too, so each step's LDS reads
+    // would wait for the next step's B loads
+    using GH = const __attribute__((address_space(1))) half8*;
+    GH Pg[PG];
+#pragma unroll
+    for (int q = 0; q < PG; ++q) Pg[q] = (GH)Pp[q];
+    // element e = t + 256 k of the step's block: lo part e / (PG * 128),
+    // probe (e / 128) % PG, unit e % 128. With whole 256-element rounds and
+    // an even PG the part is the same for the whole round, so every lane
+    // loads unconditionally (conditional loads make the waits conservative).
+    const uint32_t p_lo = J.p_lo;
     auto b_load = [&](uint32_t r, uint32_t cc, half8 (&v)[kPer]) {
         const size_t base = ((size_t)(Tl.y0 + (int)J.B + (int)r) * J.Pw8 + xb0 + cc * 4) * 32;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
-            const uint32_t e = t + 256u * k, lo = e / (PG * 128), q = (e / 128) % PG, o = e % 128;
-            if (e < (uint32_t)kB) v[k] = Pp[q][base + o + (lo ? J.p_lo : 0)];
+            const uint32_t e = t + 256u * k, q = (e / 128) % PG, o = e % 128;
+            if constexpr (kB % 256 == 0 && (PG * 128) % 256 == 0) {
+                const uint32_t lo = (256u * k) / (PG * 128);
+                v[k] = Pg[q][base + o + (lo ? p_lo : 0)];
+            } else {
+                const uint32_t lo = e / (PG * 128);
+                if (e < (uint32_t)kB) v[k] = Pg[q][base + o + (lo ? p_lo : 0)];
+            }
         }
     };
     half8 nx[kPer];
@@ -983,12 +1040,24 @@ __global__ __launch_bounds__(512) void near_gemm_staged_kernel(Probes Q, NearTab
             ++rn;
         }
         const bool more = s + 1 < KS;
-        if (more) b_load(rn, cn, nx);
+        // this step's weights are loaded before the next step's B operand:
+        // the load counter completes in order, so the MFMAs' wait for the
+        // weights leaves the B loads in flight
+        using V = std::conditional_t<SPLIT, bf16x8, half8>;
+        const V* wv = reinterpret_cast<const V*>(w0);
+        const size_t o0 = (size_t)s * kStep, o1 = o0 + kRow;
+        V xw[SPLIT ? 4 : 2];
+        xw[0] = wv[o0];
+        xw[1] = wv[o1];
+        if constexpr (SPLIT) {
+            xw[2] = wv[o0 + J.w_lo];
+            xw[3] = wv[o1 + J.w_lo];
+        }
+        b_load(more ? rn : r, more ? cn : cc, nx);  // the last step reloads itself
+        __asm__ volatile("" ::: "memory");          // keeps the B loads ahead of the MFMAs
         const half8* sb = sB[s & 1];
         if constexpr (SPLIT) {
-            const bf16x8* wv = reinterpret_cast<const bf16x8*>(w0);
-            const size_t o0 = (size_t)s * kStep, o1 = o0 + kRow;
-            const bf16x8 x0 = wv[o0], x1 = wv[o1], y0 = wv[o0 + J.w_lo], y1 = wv[o1 + J.w_lo];
+            const bf16x8 x0 = xw[0], x1 = xw[1], y0 = xw[2], y1 = xw[3];
 #pragma unroll
             for (int q = 0; q < PG; ++q) {
                 const bf16x8* pv = reinterpret_cast<const bf16x8*>(sb + q * 128 + g * 32 + i);
@@ -1007,7 +1076,7 @@ __global__ __launch_bounds__(512) void near_gemm_staged_kernel(Probes Q, NearTab
                 acc[q][3] = hvx::mfma16x16x32(y1, b1, acc[q][3]);
             }
         } else {
-            const half8 x0 = w0[(size_t)s * kStep], x1 = w1[(size_t)s * kStep];
+            const half8 x0 = xw[0], x1 = xw[1];
 #pragma unroll
             for (int q = 0; q < PG; ++q) {
                 const half8* pv = sb + q * 128 + g * 32 + i;
@@ -1018,11 +1087,12 @@ __global__ __launch_bounds__(512) void near_gemm_staged_kernel(Probes Q, NearTab
                 acc[q][3] = hvx::mfma16x16x32(x1, b1, acc[q][3]);
             }
         }
-        if (more) {
+        // stored unconditionally: after the last step the buffer is no longer
+        // read (its readers passed the previous barrier), and a store under a
+        // branch would draw the B loads into the branch, after the MFMAs
 #pragma unroll
-            for (int k = 0; k < kPer; ++k)
-                if (t + 256u * k < (uint32_t)kB) sB[(s + 1) & 1][t + 256u * k] = nx[k];
-        }
+        for (int k = 0; k < kPer; ++k)
+            if (t + 256u * k < (uint32_t)kB) sB[(s + 1) & 1][t + 256u * k] = nx[k];
         __syncthreads();
         r = rn;
         cc = cn;
