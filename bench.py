@@ -68,19 +68,19 @@ def main():
         sys.exit(2)
     import torch
     from hellovulkan_amd import abi
-    from tests.helpers import CFG1, camera
+    from tests.helpers import UHD_FRAME, camera
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     ctx = abi.Context(0)
     stream = torch.cuda.current_stream(dev)
 
-    W, H, grid, NL, CAP = CFG1["width"], CFG1["height"], CFG1["grid"], CFG1["n_lights"], CFG1["cap"]
+    W, H, grid, NL, CAP = UHD_FRAME["width"], UHD_FRAME["height"], UHD_FRAME["grid"], UHD_FRAME["n_lights"], UHD_FRAME["cap"]
     F = W * H
     Cn = grid[0] * grid[1] * grid[2]
     desc, cam, ubo, ivp = camera(W, H, grid)
 
-    # ---------------- IBL maps (cfg2 sizes) built by the library's own stages
+    # ---------------- IBL maps (2048x1024 equirect, 1024^2 cube, 512^2 x 10 chain) built by the library's own stages
     t0 = time.time()
     EQW, EQH, CUBE, DIFF, SPEC, SPEC_MIPS, LUT, LUT_N, SPEC_N = 2048, 1024, 1024, 32, 512, 10, 512, 1024, 1024
     eq = abi.scene_equirect(77, EQW, EQH, threads=16)
@@ -194,7 +194,7 @@ def main():
     }
     # HBM traffic per launch of the shading kernel, from the committed
     # rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload (calibrated)
-    tpath = os.path.join(ROOT, "profiles", "shade_kernel_cfg1_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "shade_kernel_3840x2160_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
             roofline["traffic"] = int(json.load(fh)["traffic_bytes"])

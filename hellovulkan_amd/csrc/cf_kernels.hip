@@ -516,7 +516,8 @@ __global__ __launch_bounds__(256) void batch_scatter_kernel(const CullBatchView*
 }  // namespace
 
 // ================================================================ launchers
-// cluster block of the single-view culling kernel (cfg1: 4x4x1 0.186 ms,
+// cluster block of the single-view culling kernel (3840x2160, 32x18x48
+// clusters, 65,536 lights: 4x4x1 0.186 ms,
 // 8x4x1 0.191, 4x4x2 0.245, 2x2x4 0.245, 8x8x1 0.307)
 #ifndef HVX_CULL_GX
 #define HVX_CULL_GX 4

@@ -987,8 +987,8 @@ __global__ __launch_bounds__(256) void near_gemm_staged_kernel(Probes Q, NearTab
     // every wave of the workgroup: it is staged once per step in LDS, double
     // buffered, the next step's copy loaded while this step's MFMAs run. With
     // PG probes per wave this cuts the workgroup's operand reads PG-fold per
-    // step, which pays for the barrier (cfg3 contraction mode 8.8 -> 8.4 ms);
-    // with one probe it does not (cfg2 specular 0.366 -> 0.392 ms), so
+    // step, which pays for the barrier (256-probe batch, contraction mode: 8.8 -> 8.4 ms);
+    // with one probe it does not (512^2 x 10 GGX chain: 0.366 -> 0.392 ms), so
     // single-probe calls use near_gemm_kernel.
     constexpr int kB = PG * 128 * (SPLIT ? 2 : 1);  // half8 per step
     constexpr int kPer = (kB + 255) / 256;           // per thread (256 lanes)

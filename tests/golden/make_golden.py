@@ -18,7 +18,7 @@ sys.path.insert(0, ROOT)
 
 from hellovulkan_amd import abi  # noqa: E402
 from oracle import hvx_oracle as O  # noqa: E402
-from tests.helpers import CFG0, camera  # noqa: E402
+from tests.helpers import SMALL_FRAME, camera  # noqa: E402
 
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden.npz")
 
@@ -46,10 +46,10 @@ def ibl_inputs():
 
 def compute() -> dict:
     g = {}
-    # cfg0 light culling (16x9x24 clusters, 2560x1600, 1024 lights, cap 128)
-    _, _, ubo, _ = camera(CFG0["width"], CFG0["height"], CFG0["grid"])
-    lights = abi.scene_lights(7, CFG0["n_lights"])
-    cells, idx, tot, ovf = O.cull_lights(ubo, O.build_aabbs(ubo), lights, CFG0["cap"])
+    # light culling of the 2560x1600 frame (16x9x24 clusters, 2560x1600, 1024 lights, cap 128)
+    _, _, ubo, _ = camera(SMALL_FRAME["width"], SMALL_FRAME["height"], SMALL_FRAME["grid"])
+    lights = abi.scene_lights(7, SMALL_FRAME["n_lights"])
+    cells, idx, tot, ovf = O.cull_lights(ubo, O.build_aabbs(ubo), lights, SMALL_FRAME["cap"])
     g["cull_cells"] = cells.view(np.uint32).reshape(-1)
     g["cull_total"] = np.array([tot, ovf], np.int64)
     g["cull_head"] = idx[:4096].astype(np.uint32)

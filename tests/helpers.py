@@ -12,8 +12,8 @@ FOVY = 0.785398              # CameraConfig::Zoom (Configs.h:38)
 NEAR, FAR = 0.1, 100.0       # Configs.h:42-43
 
 # workload configurations: (width, height, grid, n_lights, cap)
-CFG0 = dict(width=2560, height=1600, grid=(16, 9, 24), n_lights=1024, cap=128)
-CFG1 = dict(width=3840, height=2160, grid=(32, 18, 48), n_lights=65536, cap=128)
+SMALL_FRAME = dict(width=2560, height=1600, grid=(16, 9, 24), n_lights=1024, cap=128)
+UHD_FRAME = dict(width=3840, height=2160, grid=(32, 18, 48), n_lights=65536, cap=128)
 
 
 def camera(width, height, grid):

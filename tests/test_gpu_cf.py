@@ -8,7 +8,7 @@ import pytest
 
 from hellovulkan_amd import abi
 from oracle import hvx_oracle as O
-from tests.helpers import CFG0, CFG1, camera, cells_to_lists, random_ibl_maps, to_dev
+from tests.helpers import SMALL_FRAME, UHD_FRAME, camera, cells_to_lists, random_ibl_maps, to_dev
 
 pytestmark = pytest.mark.gpu
 
@@ -40,7 +40,7 @@ def gpu_cull(ctx, ubo, d_aabbs, lights_np, cap, with_overflow=True):
     return cells_np, idx.cpu().numpy().view(np.uint32)[:t], t, int(ov.item()), idx, d_lights, cells
 
 
-@pytest.mark.parametrize("cfg", [CFG0, CFG1], ids=["cfg0", "cfg1"])
+@pytest.mark.parametrize("cfg", [SMALL_FRAME, UHD_FRAME], ids=["2560x1600", "3840x2160"])
 def test_aabbs_bit_exact(ctx, cfg):
     _, _, ubo, _ = camera(cfg["width"], cfg["height"], cfg["grid"])
     got = gpu_aabbs(ctx, ubo).cpu().numpy().reshape(-1, 8)
@@ -49,7 +49,7 @@ def test_aabbs_bit_exact(ctx, cfg):
         f"max |diff| {np.abs(got - ref).max()}"
 
 
-@pytest.mark.parametrize("cfg,seed", [(CFG0, 11), (CFG1, 12)], ids=["cfg0", "cfg1"])
+@pytest.mark.parametrize("cfg,seed", [(SMALL_FRAME, 11), (UHD_FRAME, 12)], ids=["2560x1600", "3840x2160"])
 def test_cull_lists_bit_exact(ctx, cfg, seed):
     _, _, ubo, _ = camera(cfg["width"], cfg["height"], cfg["grid"])
     lights = abi.scene_lights(seed, cfg["n_lights"])
@@ -93,7 +93,7 @@ def test_cull_saturated_overflow(ctx):
 
 def test_cull_deterministic_and_no_overrun(ctx):
     import torch
-    cfg = CFG0
+    cfg = SMALL_FRAME
     _, _, ubo, _ = camera(cfg["width"], cfg["height"], cfg["grid"])
     lights = abi.scene_lights(21, 3000)
     d_aabbs = gpu_aabbs(ctx, ubo)
@@ -139,17 +139,17 @@ def _shade_case(ctx, width, height, grid, n_lights, cap, seed, n_check, far_plan
 
 
 @pytest.mark.parametrize("mode", ["fast", "exact"])
-@pytest.mark.parametrize("case", ["small", "cfg0", "cfg1"])
+@pytest.mark.parametrize("case", ["small", "2560x1600", "3840x2160"])
 def test_shade_matches_oracle(ctx, case, mode):
     ctx.set_option(abi.OPT_SHADE_EXACT, 1 if mode == "exact" else 0)
     try:
         if case == "small":
             got, ref, _ = _shade_case(ctx, 320, 200, (16, 9, 24), 512, 128, 31, 4000, far_plane_rows=1)
-        elif case == "cfg0":
-            got, ref, _ = _shade_case(ctx, CFG0["width"], CFG0["height"], CFG0["grid"], CFG0["n_lights"], 128, 32,
+        elif case == "2560x1600":
+            got, ref, _ = _shade_case(ctx, SMALL_FRAME["width"], SMALL_FRAME["height"], SMALL_FRAME["grid"], SMALL_FRAME["n_lights"], 128, 32,
                                       20000)
         else:
-            got, ref, _ = _shade_case(ctx, CFG1["width"], CFG1["height"], CFG1["grid"], CFG1["n_lights"], 128, 33,
+            got, ref, _ = _shade_case(ctx, UHD_FRAME["width"], UHD_FRAME["height"], UHD_FRAME["grid"], UHD_FRAME["n_lights"], 128, 33,
                                       20000)
     finally:
         ctx.set_option(abi.OPT_SHADE_EXACT, 0)

@@ -151,8 +151,8 @@ def test_contract_graph_replays_equal_eager(ctx):
     plan.close()
 
 
-def test_contract_and_plain_against_exact_integral_cfg2(ctx):
-    """The cfg2 environment (2048x1024 procedural equirect with a sun disk of
+def test_contract_and_plain_against_exact_integral_512_chain(ctx):
+    """The benchmark environment (2048x1024 procedural equirect with a sun disk of
     radiance 1e4, 6x1024^2 cube, 512^2 x 10 GGX chain): the FP16 contraction
     filter and the plain 1024-sample filter against the filter integral summed
     over every level-0 texel in FP64, on 24 texels of each rough mip. The

@@ -11,7 +11,7 @@ import pytest
 
 from hellovulkan_amd import abi
 from oracle import hvx_oracle as O
-from tests.helpers import CFG0, camera, to_dev
+from tests.helpers import SMALL_FRAME, camera, to_dev
 
 pytestmark = pytest.mark.gpu
 
@@ -26,18 +26,18 @@ def gold():
     return np.load(GOLDEN)
 
 
-def test_golden_culling_cfg0(ctx, gold):
+def test_golden_culling_2560x1600(ctx, gold):
     import torch
-    _, _, ubo, _ = camera(CFG0["width"], CFG0["height"], CFG0["grid"])
+    _, _, ubo, _ = camera(SMALL_FRAME["width"], SMALL_FRAME["height"], SMALL_FRAME["grid"])
     Cn = 16 * 9 * 24
-    lights = abi.scene_lights(7, CFG0["n_lights"])
+    lights = abi.scene_lights(7, SMALL_FRAME["n_lights"])
     aabb = torch.empty(Cn * 8, dtype=torch.float32, device="cuda")
     ctx.build_aabbs(ubo, aabb)
     cells = torch.zeros(Cn * 2, dtype=torch.int32, device="cuda")
-    idx = torch.zeros(Cn * CFG0["cap"], dtype=torch.int32, device="cuda")
+    idx = torch.zeros(Cn * SMALL_FRAME["cap"], dtype=torch.int32, device="cuda")
     tot = torch.zeros(1, dtype=torch.int32, device="cuda")
-    ctx.cull_lights(ubo, aabb, to_dev(lights.view(np.uint8)), CFG0["n_lights"], CFG0["cap"], cells, idx,
-                    Cn * CFG0["cap"], tot)
+    ctx.cull_lights(ubo, aabb, to_dev(lights.view(np.uint8)), SMALL_FRAME["n_lights"], SMALL_FRAME["cap"], cells, idx,
+                    Cn * SMALL_FRAME["cap"], tot)
     torch.cuda.synchronize()
     t = int(tot.item())
     assert t == int(gold["cull_total"][0])

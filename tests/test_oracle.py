@@ -10,7 +10,7 @@ import pytest
 
 from hellovulkan_amd import abi
 from oracle import hvx_oracle as O
-from tests.helpers import CFG0, FAR, FOVY, NEAR, camera, cells_to_lists
+from tests.helpers import SMALL_FRAME, FAR, FOVY, NEAR, camera, cells_to_lists
 
 F32 = np.float32
 
@@ -83,9 +83,9 @@ def test_aabbs_against_independent_frustum():
 
 
 def test_aabbs_invariants():
-    _, _, ubo, _ = camera(CFG0["width"], CFG0["height"], CFG0["grid"])
+    _, _, ubo, _ = camera(SMALL_FRAME["width"], SMALL_FRAME["height"], SMALL_FRAME["grid"])
     a = O.build_aabbs(ubo)
-    X, Y, Z = CFG0["grid"]
+    X, Y, Z = SMALL_FRAME["grid"]
     assert np.all(a[:, 0:3] <= a[:, 4:7])
     a = a.reshape(Z, Y, X, 8)
     # adjacent z-slices share their separating plane (same pow() input)
