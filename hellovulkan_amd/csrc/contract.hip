@@ -687,8 +687,8 @@ __global__ __launch_bounds__(256) void dense_gemm_staged_kernel(Probes Q, DenseT
         constexpr bool SPLIT = decltype(split_tag)::value;
         constexpr uint32_t nB = PG * 128 * (SPLIT ? 2u : 1u), kPer = (nB + 255) / 256;
         // element e = t + 256 k of a step's block: lo part e / (PG * 128),
-        // probe (e / 128) % PG, unit e % 128; with whole 256-element rounds
-        // and an even PG the part is the same for the whole round
+        // probe (e / 128) % PG, unit e % 128; with whole 256-element passes
+        // and an even PG the part is the same for the whole pass
         constexpr bool kWhole = nB % 256 == 0 && (PG * 128) % 256 == 0;
         auto b_load = [&](uint32_t s, half8 (&v)[kPer]) {
 #pragma unroll
@@ -1011,8 +1011,8 @@ __global__ __launch_bounds__(256) void near_gemm_staged_kernel(Probes Q, NearTab
 #pragma unroll
     for (int q = 0; q < PG; ++q) Pg[q] = (GH)Pp[q];
     // element e = t + 256 k of the step's block: lo part e / (PG * 128),
-    // probe (e / 128) % PG, unit e % 128. With whole 256-element rounds and
-    // an even PG the part is the same for the whole round, so every lane
+    // probe (e / 128) % PG, unit e % 128. With whole 256-element passes and
+    // an even PG the part is the same for the whole pass, so every lane
     // loads unconditionally (conditional loads make the waits conservative).
     const uint32_t p_lo = J.p_lo;
     auto b_load = [&](uint32_t r, uint32_t cc, half8 (&v)[kPer]) {
