@@ -30,6 +30,11 @@ HVX_ERR_ALIGN = 4
 HVX_ERR_HIP = 5
 HVX_ERR_UNSUPPORTED = 6
 HVX_ERR_NO_DEVICE = 7
+HVX_ERR_STALE_GRAPH = 8
+
+# hvx_cf_cull_lights_batch flags
+CULL_BATCH_REFERENCE_CAP = 1
+CULL_BATCH_REFERENCE_TAIL = 2
 
 OPT_SHADE_EXACT = 1
 
@@ -139,8 +144,10 @@ SIGNATURES = [
     ("hvx_ctx_last_error", C.c_char_p, [vp]),
     ("hvx_ctx_set_option", C.c_int, [vp, C.c_int, C.c_int64]),
     ("hvx_ctx_get_option", C.c_int, [vp, C.c_int, P(C.c_int64)]),
+    ("hvx_ctx_scratch_generation", C.c_uint64, [vp]),
     ("hvx_cf_build_aabbs", C.c_int, [vp, P(CFUBO), vp, vp]),
     ("hvx_cf_cull_lights", C.c_int, [vp, P(CFUBO), vp, vp, u32, u32, vp, vp, C.c_size_t, vp, vp, vp]),
+    ("hvx_cf_cull_lights_batch", C.c_int, [vp, P(CFUBO), vp, vp, u32, u32, u32, vp, vp, C.c_size_t, vp, vp]),
     ("hvx_cf_shade", C.c_int, [vp, P(FragSoA), P(CameraUBO), P(CFUBO), P(PBRPC), vp, u32, vp, vp,
                                P(IBLMaps), vp, vp]),
     ("hvx_ibl_brdf_lut", C.c_int, [vp, u32, u32, u32, vp, vp]),
@@ -350,6 +357,17 @@ class Context:
         self._check(self.lib.hvx_cf_cull_lights(self.h, C.byref(ubo), _ptr(clusters), _ptr(lights), n_lights, cap,
                                                 _ptr(cells), _ptr(indices), indices_capacity, _ptr(total),
                                                 _ptr(overflow), self._stream(stream)), "hvx_cf_cull_lights")
+
+    def cull_lights_batch(self, ubo: CFUBO, clusters, lights, n_lights: int, cap: int, flags: int, cells, indices,
+                          indices_capacity: int, total=None, stream=None):
+        """LightCullingBatch.comp semantics (flags: CULL_BATCH_REFERENCE_*)."""
+        self._check(self.lib.hvx_cf_cull_lights_batch(self.h, C.byref(ubo), _ptr(clusters), _ptr(lights), n_lights,
+                                                      cap, flags, _ptr(cells), _ptr(indices), indices_capacity,
+                                                      _ptr(total), self._stream(stream)), "hvx_cf_cull_lights_batch")
+
+    @property
+    def scratch_generation(self) -> int:
+        return int(self.lib.hvx_ctx_scratch_generation(self.h))
 
     def shade(self, frags: FragSoA, cam: CameraUBO, ubo: CFUBO, pc: PBRPC, lights, n_lights: int, cells, indices,
               maps: IBLMaps, rgb, stream=None):

@@ -447,6 +447,77 @@ __global__ __launch_bounds__(256) void scatter_kernel(const uint32_t* __restrict
     scatter_group(lists, cap, C, cells, indices, blockIdx.x);
 }
 
+// ------------------------------------- LightCullingBatch.comp semantics
+// LightCullingBatch.comp:61-106: one lane per cluster (cluster = local index
+// + kCullLdsBatch * workgroup), the lights staged in LDS in batches of
+// kCullLdsBatch, every lane testing its cluster against the whole batch in
+// slot order. The LDS entries hold the view-space centre and r^2 of each
+// light (light_prep_kernel), the values the shader recomputes per test with
+// the same arithmetic. Pass 0 counts, pass 1 writes at the scanned offsets;
+// both make the same tests in the same order.
+struct CullLdsJob {
+    const hvx_aabb* clusters;
+    const float4* lvs;
+    uint32_t C, n, cap, flags;
+    uint32_t* counts;
+    uint32_t* over;
+    const hvx_light_cell* cells;
+    uint32_t* indices;
+    size_t capacity;
+};
+
+template <bool WRITE>
+__global__ __launch_bounds__(kCullLdsBatch) void cull_lds_batch_kernel(CullLdsJob J) {
+    constexpr uint32_t T = kCullLdsBatch;
+    __shared__ float4 sl[T];
+    const uint32_t t = threadIdx.x, c = blockIdx.x * T + t;
+    const bool valid = c < J.C;
+    const bool ref_cap = (J.flags & HVX_CULL_BATCH_REFERENCE_CAP) != 0u;
+    const bool ref_tail = (J.flags & HVX_CULL_BATCH_REFERENCE_TAIL) != 0u;
+    Box b;
+    if (valid) {
+        const hvx_aabb& a = J.clusters[c];
+        for (int k = 0; k < 3; ++k) { b.mn[k] = a.min_point[k]; b.mx[k] = a.max_point[k]; }
+    } else {
+        for (int k = 0; k < 3; ++k) { b.mn[k] = 0.0f; b.mx[k] = 0.0f; }
+    }
+    // slots the shader never writes are undefined there; zeros here (radius 0
+    // never passes the strict test)
+    sl[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const uint32_t base = WRITE && valid ? J.cells[c].offset : 0u;
+    uint32_t cnt = 0, ovf = 0;
+    bool done = !valid;
+    const uint32_t nb = (J.n + T - 1) / T;
+    for (uint32_t batch = 0; batch < nb; ++batch) {
+        // every lane has finished the previous batch before its slots are
+        // overwritten (the shader has no barrier here, :86-93); a workgroup
+        // whose clusters are all full stops
+        if (__syncthreads_and(done ? 1 : 0)) break;
+        const uint32_t li = batch * T + t;
+        if (li < J.n) sl[t] = J.lvs[li];  // :79-82: the tail keeps the previous batch
+        __syncthreads();
+        const uint32_t m = (ref_tail || batch + 1 < nb) ? T : J.n - batch * T;  // slots tested
+        if (done) continue;
+        for (uint32_t i = 0; i < m; ++i) {
+            if (!sphere_box(sl[i], b)) continue;
+            if (!ref_cap && cnt >= J.cap) {  // a hit past the cap: the list is full
+                ovf = 1;
+                done = true;
+                break;
+            }
+            if (WRITE) {
+                const size_t pos = (size_t)base + cnt;
+                if (pos < J.capacity) J.indices[pos] = batch * T + i;
+            }
+            ++cnt;
+        }
+    }
+    if (!WRITE && valid) {
+        J.counts[c] = cnt;
+        J.over[c] = ovf;
+    }
+}
+
 // ------------------------------------------------------ batched (multi-view)
 // Every kernel below covers all views of a batch in one launch; a work item
 // finds its view by the views' running item offsets (at most kMaxViews).
@@ -561,6 +632,26 @@ hipError_t launch_cull(const hvx_cf_ubo& ubo, const hvx_aabb* d_clusters, const 
     hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(1024), 0, s, part, nb, d_total, d_overflow);
     hipLaunchKernelGGL(scan_apply_kernel, dim3(nb), dim3(256), 0, s, counts, C, part, d_cells);
     hipLaunchKernelGGL(scatter_kernel, dim3((C + 3) / 4), dim3(256), 0, s, lists, cap, C, d_cells, d_indices);
+    return hipGetLastError();
+}
+
+hipError_t launch_cull_lds_batch(const hvx_cf_ubo& ubo, const hvx_aabb* d_clusters, const hvx_light* d_lights,
+                                 uint32_t n_lights, uint32_t cap, uint32_t flags, hvx_light_cell* d_cells,
+                                 uint32_t* d_indices, size_t indices_capacity, uint32_t* d_total, float4* light_vs,
+                                 uint32_t* counts, uint32_t* over, hipStream_t s) {
+    const uint32_t C = ubo.slice_count_x * ubo.slice_count_y * ubo.slice_count_z;
+    if (n_lights > 0)
+        hipLaunchKernelGGL(light_prep_kernel, dim3((n_lights + 255) / 256), dim3(256), 0, s, ubo, d_lights, n_lights,
+                           light_vs);
+    CullLdsJob J{d_clusters, light_vs, C, n_lights, cap, flags, counts, over, d_cells, d_indices, indices_capacity};
+    const dim3 grid((C + kCullLdsBatch - 1) / kCullLdsBatch), block(kCullLdsBatch);
+    hipLaunchKernelGGL(cull_lds_batch_kernel<false>, grid, block, 0, s, J);
+    const uint32_t nb = (C + kScanBlock - 1) / kScanBlock;
+    uint2* part = reinterpret_cast<uint2*>(over + C);
+    hipLaunchKernelGGL(scan_partials_kernel, dim3(nb), dim3(256), 0, s, counts, over, C, part);
+    hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(1024), 0, s, part, nb, d_total, nullptr);
+    hipLaunchKernelGGL(scan_apply_kernel, dim3(nb), dim3(256), 0, s, counts, C, part, d_cells);
+    hipLaunchKernelGGL(cull_lds_batch_kernel<true>, grid, block, 0, s, J);
     return hipGetLastError();
 }
 

@@ -67,8 +67,19 @@ int check_ubo(hvx_ctx* ctx, const hvx_cf_ubo* u) {
 }  // namespace
 
 namespace hvx {
-hipError_t ensure_scratch(hvx_ctx* ctx, void** buf, size_t* cur, size_t bytes) {
+bool stream_capturing(hipStream_t stream) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &st) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return st != hipStreamCaptureStatusNone;
+}
+
+hipError_t ensure_scratch(hvx_ctx* ctx, void** buf, size_t* cur, size_t bytes, hipStream_t stream) {
     if (*cur >= bytes && *buf) return hipSuccess;
+    if (stream_capturing(stream)) return hipErrorStreamCaptureUnsupported;
+    ctx->scratch_gen += 1;
     if (*buf) {
         hipError_t e = hipDeviceSynchronize();
         if (e != hipSuccess) return e;
@@ -88,9 +99,23 @@ hipError_t ensure_scratch(hvx_ctx* ctx, void** buf, size_t* cur, size_t bytes) {
 }
 }  // namespace hvx
 
+namespace {
+// Status of a failed scratch growth: a refusal under graph capture, or the
+// runtime's error.
+int scratch_fail(hvx_ctx* ctx, hipError_t e, const char* what) {
+    if (e == hipErrorStreamCaptureUnsupported)
+        return fail(ctx, HVX_ERR_UNSUPPORTED,
+                    "%s: the call needs more scratch while its stream is being captured; run it once eagerly at "
+                    "this size before capturing", what);
+    return hip_fail(ctx, e, what);
+}
+}  // namespace
+
 extern "C" {
 
 int hvx_abi_version(void) { return HVX_ABI_VERSION; }
+
+uint64_t hvx_ctx_scratch_generation(const hvx_ctx* ctx) { return ctx ? ctx->scratch_gen : 0; }
 
 const char* hvx_status_str(int s) {
     switch (s) {
@@ -102,6 +127,7 @@ const char* hvx_status_str(int s) {
         case HVX_ERR_HIP: return "HIP runtime error";
         case HVX_ERR_UNSUPPORTED: return "unsupported option";
         case HVX_ERR_NO_DEVICE: return "no gfx950 device";
+        case HVX_ERR_STALE_GRAPH: return "graph captured before a scratch reallocation";
         default: return "unknown status";
     }
 }
@@ -203,17 +229,57 @@ int hvx_cf_cull_lights(hvx_ctx* ctx, const hvx_cf_ubo* ubo, const hvx_aabb* d_cl
         return fail(ctx, HVX_ERR_CAPACITY, "indices_capacity %zu < clusters*cap = %llu", indices_capacity,
                     (unsigned long long)(C * cap));
     if ((st = bind_device(ctx))) return st;
+    const hipStream_t s = (hipStream_t)stream;
     hipError_t e;
-    if ((e = hvx::ensure_scratch(ctx, &ctx->cull_lists, &ctx->cull_lists_bytes, C * cap * 4)) != hipSuccess ||
-        (e = hvx::ensure_scratch(ctx, &ctx->cull_meta, &ctx->cull_meta_bytes, C * 8 + ((C + 1023) / 1024) * 8 + 64)) !=
-            hipSuccess ||
-        (e = hvx::ensure_scratch(ctx, &ctx->light_vs, &ctx->light_vs_bytes, (size_t)n_lights * 16)) != hipSuccess)
-        return hip_fail(ctx, e, "culling scratch allocation");
+    if ((e = hvx::ensure_scratch(ctx, &ctx->cull_lists, &ctx->cull_lists_bytes, C * cap * 4, s)) != hipSuccess ||
+        (e = hvx::ensure_scratch(ctx, &ctx->cull_meta, &ctx->cull_meta_bytes, C * 8 + ((C + 1023) / 1024) * 8 + 64,
+                                 s)) != hipSuccess ||
+        (e = hvx::ensure_scratch(ctx, &ctx->light_vs, &ctx->light_vs_bytes, (size_t)n_lights * 16, s)) != hipSuccess)
+        return scratch_fail(ctx, e, "culling scratch allocation");
     uint32_t* counts = static_cast<uint32_t*>(ctx->cull_meta);
     e = hvx::launch_cull(*ubo, d_clusters, d_lights, n_lights, cap, d_cells, d_indices, d_total, d_overflow_count,
                          static_cast<float4*>(ctx->light_vs), static_cast<uint32_t*>(ctx->cull_lists), counts,
-                         counts + C, (hipStream_t)stream);
+                         counts + C, s);
     return e == hipSuccess ? HVX_OK : hip_fail(ctx, e, "hvx_cf_cull_lights launch");
+}
+
+int hvx_cf_cull_lights_batch(hvx_ctx* ctx, const hvx_cf_ubo* ubo, const hvx_aabb* d_clusters,
+                             const hvx_light* d_lights, uint32_t n_lights, uint32_t cap, uint32_t flags,
+                             hvx_light_cell* d_cells, uint32_t* d_indices, size_t indices_capacity, uint32_t* d_total,
+                             void* stream) {
+    if (!ctx) return HVX_ERR_NULL;
+    int st = check_ubo(ctx, ubo);
+    if (st) return st;
+    if (!d_clusters || !d_cells || !d_indices) return fail(ctx, HVX_ERR_NULL, "clusters/cells/indices NULL");
+    if (n_lights > 0 && !d_lights) return fail(ctx, HVX_ERR_NULL, "d_lights is NULL");
+    if (cap < 1 || cap > 1024) return fail(ctx, HVX_ERR_INVALID_ARG, "cap %u not in [1, 1024]", cap);
+    if (flags & ~(HVX_CULL_BATCH_REFERENCE_CAP | HVX_CULL_BATCH_REFERENCE_TAIL))
+        return fail(ctx, HVX_ERR_INVALID_ARG, "unknown flags 0x%x", flags);
+    if (n_lights > (1u << 28)) return fail(ctx, HVX_ERR_INVALID_ARG, "too many lights (%u)", n_lights);
+    if (misaligned(d_lights, 16) || misaligned(d_clusters, 16) || misaligned(d_cells, 8) || misaligned(d_indices, 4))
+        return fail(ctx, HVX_ERR_ALIGN, "lights/clusters need 16-byte, cells 8-byte alignment");
+    const uint64_t C = (uint64_t)ubo->slice_count_x * ubo->slice_count_y * ubo->slice_count_z;
+    if (C * cap > 0xffffffffull)
+        return fail(ctx, HVX_ERR_INVALID_ARG, "clusters*cap = %llu exceeds 32-bit list offsets",
+                    (unsigned long long)(C * cap));
+    if ((flags & HVX_CULL_BATCH_REFERENCE_CAP) && C * (uint64_t)(n_lights + hvx::kCullLdsBatch) > 0xffffffffull)
+        return fail(ctx, HVX_ERR_INVALID_ARG, "uncapped lists of %llu clusters x %u lights exceed 32-bit offsets",
+                    (unsigned long long)C, n_lights);
+    if ((uint64_t)indices_capacity < C * cap)
+        return fail(ctx, HVX_ERR_CAPACITY, "indices_capacity %zu < clusters*cap = %llu", indices_capacity,
+                    (unsigned long long)(C * cap));
+    if ((st = bind_device(ctx))) return st;
+    const hipStream_t s = (hipStream_t)stream;
+    hipError_t e;
+    if ((e = hvx::ensure_scratch(ctx, &ctx->cull_meta, &ctx->cull_meta_bytes, C * 8 + ((C + 1023) / 1024) * 8 + 64,
+                                 s)) != hipSuccess ||
+        (e = hvx::ensure_scratch(ctx, &ctx->light_vs, &ctx->light_vs_bytes, (size_t)n_lights * 16, s)) != hipSuccess)
+        return scratch_fail(ctx, e, "culling scratch allocation");
+    uint32_t* counts = static_cast<uint32_t*>(ctx->cull_meta);
+    e = hvx::launch_cull_lds_batch(*ubo, d_clusters, d_lights, n_lights, cap, flags, d_cells, d_indices,
+                                   indices_capacity, d_total, static_cast<float4*>(ctx->light_vs), counts, counts + C,
+                                   s);
+    return e == hipSuccess ? HVX_OK : hip_fail(ctx, e, "hvx_cf_cull_lights_batch launch");
 }
 
 }  // extern "C"
@@ -247,13 +313,14 @@ int shade_impl(hvx_ctx* ctx, const hvx_frag_soa* fr, const hvx_camera_ubo* cam, 
     if ((st = bind_device(ctx))) return st;
     hipError_t e = hipSuccess;
     if (!ctx->shade_exact &&
-        (e = hvx::ensure_scratch(ctx, &ctx->shade_lights, &ctx->shade_lights_bytes, (size_t)n_lights * 32)) !=
-            hipSuccess)
-        return hip_fail(ctx, e, "shading light table allocation");
+        (e = hvx::ensure_scratch(ctx, &ctx->shade_lights, &ctx->shade_lights_bytes, (size_t)n_lights * 32,
+                                 (hipStream_t)stream)) != hipSuccess)
+        return scratch_fail(ctx, e, "shading light table allocation");
     const bool binned = hvx::shade_binned(*ubo);
     if (binned && (e = hvx::ensure_scratch(ctx, &ctx->shade_bins, &ctx->shade_bins_bytes,
-                                           hvx::shade_bin_scratch_bytes(*fr, *ubo))) != hipSuccess)
-        return hip_fail(ctx, e, "shading bin allocation");
+                                           hvx::shade_bin_scratch_bytes(*fr, *ubo), (hipStream_t)stream)) !=
+                      hipSuccess)
+        return scratch_fail(ctx, e, "shading bin allocation");
     e = hvx::launch_shade(*fr, *cam, *ubo, *pc, d_lights, n_lights, d_cells, d_indices, *ibl, d_rgb,
                           ctx->shade_exact != 0, static_cast<float4*>(ctx->shade_lights),
                           binned ? static_cast<uint32_t*>(ctx->shade_bins) : nullptr, learned, (hipStream_t)stream);
@@ -373,8 +440,9 @@ int hvx_ibl_filter(hvx_ctx* ctx, int type, const float* d_cube, uint32_t face_si
     if (sample_count == 0 || sample_count > (1u << 20))
         return fail(ctx, HVX_ERR_INVALID_ARG, "sample_count %u not in [1, 2^20]", sample_count);
     if ((st = bind_device(ctx))) return st;
-    hipError_t e = hvx::ensure_scratch(ctx, &ctx->ggx_table, &ctx->ggx_table_bytes, (size_t)out_mips * sample_count * 16);
-    if (e != hipSuccess) return hip_fail(ctx, e, "GGX table allocation");
+    hipError_t e = hvx::ensure_scratch(ctx, &ctx->ggx_table, &ctx->ggx_table_bytes, (size_t)out_mips * sample_count * 16,
+                                       (hipStream_t)stream);
+    if (e != hipSuccess) return scratch_fail(ctx, e, "GGX table allocation");
     e = hvx::launch_specular(d_cube, face_size, d_out, out_face_size, out_mips, sample_count, emulate,
                              static_cast<float4*>(ctx->ggx_table), (hipStream_t)stream);
     return e == hipSuccess ? HVX_OK : hip_fail(ctx, e, "specular launch");

@@ -28,6 +28,9 @@ struct hvx_ctx {
     size_t shade_lights_bytes = 0;
     void* shade_bins = nullptr;   // uint32 [fragments + X + Y + 2]: cluster-binned fragment order
     size_t shade_bins_bytes = 0;
+    // bumped whenever a scratch buffer is (re)allocated: a graph captured
+    // before the bump may hold pointers to freed scratch
+    uint64_t scratch_gen = 0;
     // options (hvx_ctx_set_option)
     int shade_exact = 0;
 };
@@ -77,8 +80,12 @@ struct ProbeItem {
 };
 
 // Grows a scratch buffer to at least `bytes` (synchronises the device when it
-// has to replace an existing allocation). Returns a hipError_t.
-hipError_t ensure_scratch(hvx_ctx* ctx, void** buf, size_t* cur, size_t bytes);
+// has to replace an existing allocation). A growth while `stream` is being
+// captured into a graph is refused with hipErrorStreamCaptureUnsupported
+// (allocation and synchronisation are illegal there). Returns a hipError_t.
+hipError_t ensure_scratch(hvx_ctx* ctx, void** buf, size_t* cur, size_t bytes, hipStream_t stream);
+// Whether `stream` is being captured into a graph.
+bool stream_capturing(hipStream_t stream);
 
 // --- clustered forward (cf_kernels.hip)
 hipError_t launch_build_aabbs(const hvx_cf_ubo& ubo, hvx_aabb* d_clusters, hipStream_t s);
@@ -86,6 +93,15 @@ hipError_t launch_cull(const hvx_cf_ubo& ubo, const hvx_aabb* d_clusters, const 
                        uint32_t n_lights, uint32_t cap, hvx_light_cell* d_cells, uint32_t* d_indices,
                        uint32_t* d_total, uint32_t* d_overflow, float4* light_vs, uint32_t* lists,
                        uint32_t* counts, uint32_t* over, hipStream_t s);
+// LightCullingBatch.comp semantics (one lane per cluster, lights staged in
+// LDS batches of kCullLdsBatch): pass 0 writes per-cluster counts and
+// overflow flags, pass 1 (after the scan) writes the packed lists. flags are
+// HVX_CULL_BATCH_REFERENCE_* of hvx.h.
+constexpr uint32_t kCullLdsBatch = 576;
+hipError_t launch_cull_lds_batch(const hvx_cf_ubo& ubo, const hvx_aabb* d_clusters, const hvx_light* d_lights,
+                                 uint32_t n_lights, uint32_t cap, uint32_t flags, hvx_light_cell* d_cells,
+                                 uint32_t* d_indices, size_t indices_capacity, uint32_t* d_total, float4* light_vs,
+                                 uint32_t* counts, uint32_t* over, hipStream_t s);
 // cull group shapes: 0 = 4x4x1, 1 = 8x4x1, 2 = 4x4x2, 3 = 2x2x4
 constexpr int kCullShapes = 4;
 uint32_t cull_batch_groups(int shape, uint32_t X, uint32_t Y, uint32_t Z);

@@ -10,7 +10,12 @@
 //   hvx_frame [--width W --height H --grid X Y Z --lights N --cap C]
 //             [--cube S --diffuse S --specular S --mips M --lut S]
 //             [--frames K --warmup W] [--dump PREFIX]
-//             [--contract PRECISION] [--tonemap OP [--gamma 0|1]]
+//             [--contract PRECISION] [--tonemap OP [--gamma 0|1]] [--orbit K]
+//
+// --orbit K then moves the camera for K more frames (AppPBRClusterForward::
+// UpdateUBOs pushes new camera and cluster UBOs every frame): the eye circles
+// the target, each frame runs eagerly and from the same executable graph,
+// updated in place by FrameGraph::Capture, and the two frames must be equal.
 //
 // --contract builds the specular chain with the MFMA contraction filter
 // (hvx_precision) instead of the sampled prefilter; --tonemap adds the post
@@ -23,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -38,7 +44,7 @@ namespace {
 struct Args {
     uint32_t width = 3840, height = 2160, gx = 32, gy = 18, gz = 48, lights = 65536, cap = 128;
     uint32_t cube = 1024, diffuse = 32, specular = 512, mips = 10, lut = 512;
-    uint32_t frames = 20, warmup = 3;
+    uint32_t frames = 20, warmup = 3, orbit = 0;
     uint64_t light_seed = 1001, frag_seed = 1002, env_seed = 77;
     int contract = -1, tonemap = -1, gamma = 1;
     std::string dump;
@@ -64,6 +70,7 @@ Args parse(int argc, char** argv) {
         else if (k == "--lut") a.lut = next();
         else if (k == "--frames") a.frames = next();
         else if (k == "--warmup") a.warmup = next();
+        else if (k == "--orbit") a.orbit = next();
         else if (k == "--light-seed") a.light_seed = next();
         else if (k == "--frag-seed") a.frag_seed = next();
         else if (k == "--dump") { if (i + 1 >= argc) throw std::runtime_error("--dump PREFIX"); a.dump = argv[++i]; }
@@ -214,6 +221,59 @@ int main(int argc, char** argv) {
         const float graph_ms = elapsed_ms(e0, e1) / (float)a.frames;
         hvx::hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
 
+        // moving camera: new UBOs per frame through the same executable graph
+        std::vector<float> o_eager(3 * F), o_graph(3 * F);
+        bool orbit_match = true;
+        std::vector<double> orbit_sums;
+        for (uint32_t k = 0; k < a.orbit; ++k) {
+            const float ang = 0.35f * (float)(k + 1), radius = 5.0f;
+            hvx_camera_desc d = desc;
+            d.eye[0] = target[0] + radius * std::sin(ang);
+            d.eye[1] = eye[1] + 0.25f * (float)k;
+            d.eye[2] = target[2] + radius * std::cos(ang);
+            hvx_camera_ubo c2{};
+            hvx_cf_ubo u2{};
+            float ivp2[16];
+            ctx.check(hvx_camera_build(&d, &c2, &u2, ivp2), "hvx_camera_build");
+            frags.SetInvViewProj(ivp2);
+            cull.SetClusterForwardUBO(u2);
+            for (hvx::PipelinePBRClusterForward* p : {&pbr, &pbr_g}) {
+                p->SetClusterForwardUBO(u2);
+                p->SetCameraUBO(c2);
+            }
+            for (hvx::PipelineBase* p : eager) p->FillCommandBuffer(stream);
+            hvx::hip_check(hipStreamSynchronize(stream), "orbit eager sync");
+            graph.Capture();
+            graph.Launch();
+            hvx::hip_check(hipStreamSynchronize(graph.stream()), "orbit graph sync");
+            rgb.download(o_eager.data(), o_eager.size());
+            rgb_graph.download(o_graph.data(), o_graph.size());
+            orbit_match = orbit_match && std::memcmp(o_eager.data(), o_graph.data(), o_eager.size() * 4) == 0;
+            double sum = 0.0;
+            for (float v : o_eager) sum += v;
+            orbit_sums.push_back(sum);
+        }
+        std::string sums = "[";
+        for (size_t k = 0; k < orbit_sums.size(); ++k) {
+            char b[32];
+            std::snprintf(b, sizeof(b), "%s%.6e", k ? ", " : "", orbit_sums[k]);
+            sums += b;
+        }
+        sums += "]";
+        if (a.orbit) {  // back to the first camera for the outputs below
+            frags.SetInvViewProj(ivp);
+            cull.SetClusterForwardUBO(ubo);
+            for (hvx::PipelinePBRClusterForward* p : {&pbr, &pbr_g}) {
+                p->SetClusterForwardUBO(ubo);
+                p->SetCameraUBO(cam);
+            }
+            for (hvx::PipelineBase* p : eager) p->FillCommandBuffer(stream);
+            hvx::hip_check(hipStreamSynchronize(stream), "eager sync");
+            graph.Capture();
+            graph.Launch();
+            hvx::hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        }
+
         uint32_t total = 0;
         resCF.globalIndex_.download(&total, 1);
         std::vector<float> h_rgb(3 * F), h_rgb_g(3 * F);
@@ -248,13 +308,15 @@ int main(int argc, char** argv) {
         std::printf(
             "{\"eager_ms_per_frame\": %.4f, \"graph_ms_per_frame\": %.4f, \"graph_nodes\": %zu, "
             "\"graph_matches_eager\": %s, \"mfrag_per_s_graph\": %.2f, \"ibl_build_ms\": %.2f, \"list_entries\": %u, "
-            "\"rgb_sum\": %.6e, \"fragments\": %zu, \"contract\": %d, \"tonemap\": %d}\n",
+            "\"rgb_sum\": %.6e, \"fragments\": %zu, \"contract\": %d, \"tonemap\": %d, \"orbit_frames\": %u, "
+            "\"orbit_frames_match\": %s, \"orbit_rgb_sums\": %s, \"graph_updates_in_place\": %zu}\n",
             eager_ms, graph_ms, graph.node_count(), graph_matches ? "true" : "false",
-            (double)F / (graph_ms * 1e-3) / 1e6, ibl_ms, total, rgb_sum, F, a.contract, a.tonemap);
+            (double)F / (graph_ms * 1e-3) / 1e6, ibl_ms, total, rgb_sum, F, a.contract, a.tonemap, a.orbit,
+            orbit_match ? "true" : "false", sums.c_str(), graph.update_count());
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         (void)hipStreamDestroy(stream);
-        return graph_matches ? 0 : 3;
+        return graph_matches && orbit_match ? 0 : 3;
     } catch (const hvx::Error& e) {
         std::fprintf(stderr, "hvx_frame: %s\n", e.what());
         return 2;

@@ -7,7 +7,18 @@
  * HelloVulkan/Header/Pipelines/PipelineBase.h:24-95). The caller owns all
  * device memory; every call is asynchronous on the HIP stream passed as
  * `void* stream` (NULL = the default stream), returns an hvx_status code and
- * never throws. A context must not be used by two host threads at once.
+ * never throws. A context must not be used by two host threads at once, and
+ * the stages of one context run in order on one stream: they share the
+ * context's scratch buffers, so calls on different streams race (use one
+ * context per stream for concurrent work).
+ *
+ * Graph capture: a call that would grow the context's scratch while its
+ * stream is being captured is refused (HVX_ERR_UNSUPPORTED), since
+ * allocation and synchronisation are illegal there; run each stage once
+ * eagerly at its largest size first. A graph holds the scratch pointers it
+ * was captured with: when a later eager call grows the scratch (the value
+ * of hvx_ctx_scratch_generation changes) those pointers are freed, and the
+ * graph must be captured again before it is replayed.
  *
  * Struct layouts mirror the reference's GPU-visible layouts byte for byte
  * (std430 / std140 as declared in the GLSL and the C++ UBO structs) so that a
@@ -33,8 +44,11 @@ typedef enum hvx_status {
     HVX_ERR_CAPACITY = 3,      /* an output buffer is too small                */
     HVX_ERR_ALIGN = 4,         /* a device pointer is misaligned               */
     HVX_ERR_HIP = 5,           /* the HIP runtime reported an error            */
-    HVX_ERR_UNSUPPORTED = 6,   /* option combination not implemented           */
-    HVX_ERR_NO_DEVICE = 7      /* no usable gfx950 device                      */
+    HVX_ERR_UNSUPPORTED = 6,   /* option combination not implemented, or a call
+                                  that would allocate during graph capture      */
+    HVX_ERR_NO_DEVICE = 7,     /* no usable gfx950 device                      */
+    HVX_ERR_STALE_GRAPH = 8    /* a captured graph refers to context scratch
+                                  that has since been reallocated              */
 } hvx_status;
 
 /* ------------------------------------------------------------------ types  */
@@ -167,6 +181,10 @@ typedef enum hvx_option {
 } hvx_option;
 int hvx_ctx_set_option(hvx_ctx* ctx, int option, int64_t value);
 int hvx_ctx_get_option(const hvx_ctx* ctx, int option, int64_t* value);
+/* Counter of the context's scratch (re)allocations. A graph captured at one
+ * value may be replayed only while the value is unchanged. No counterpart in
+ * the reference (its descriptor sets bind fixed buffers). */
+uint64_t hvx_ctx_scratch_generation(const hvx_ctx* ctx);
 
 /* ------------------------------------------------------- clustered forward */
 
@@ -201,6 +219,38 @@ int hvx_cf_cull_lights(hvx_ctx* ctx, const hvx_cf_ubo* ubo,
                        uint32_t* d_indices, size_t indices_capacity,
                        uint32_t* d_total, uint32_t* d_overflow_count,
                        void* stream);
+
+/* The batched variant of the culling shader, LightCullingBatch.comp:61-106
+ * (the reference builds it but dispatches LightCulling.comp instead,
+ * PipelineLightCulling.cpp:25-26, :80-83): one invocation per cluster,
+ * cluster index = local index + 576 * workgroup (with the reference's
+ * 16x9x4 workgroups over a 16x9xZ grid this is x + y*X + z*X*Y, the index
+ * of every other stage), and the lights staged through shared memory in
+ * batches of 576, each invocation testing its cluster against the whole
+ * batch (strict d^2 < r^2, as LightCulling.comp). Offsets are the exclusive
+ * prefix sum of the counts (the reference: atomicAdd).
+ * flags == 0 fixes the shader's two defects: the list stops at `cap`
+ * and entries of the last batch past n_lights are not tested. The lists are
+ * then those of hvx_cf_cull_lights, bit for bit. The flags restore the
+ * reference's behaviour for comparison (such lists must not be shaded):
+ *   HVX_CULL_BATCH_REFERENCE_CAP:  no cap (:86-92 writes past the 150-entry
+ *       array): counts are not limited; entries whose packed position is
+ *       >= indices_capacity are not written, and *d_total receives the
+ *       unlimited sum, so a caller sees when its buffer was too small;
+ *   HVX_CULL_BATCH_REFERENCE_TAIL: the last batch's slots past n_lights keep
+ *       the previous batch's lights (never written, :79-82) and are tested,
+ *       emitting indices >= n_lights; slots never written hold zeros (the
+ *       shader leaves them undefined), whose radius 0 never passes the test.
+ * d_overflow_count is not produced; other arguments as hvx_cf_cull_lights. */
+#define HVX_CULL_BATCH_REFERENCE_CAP 1u
+#define HVX_CULL_BATCH_REFERENCE_TAIL 2u
+int hvx_cf_cull_lights_batch(hvx_ctx* ctx, const hvx_cf_ubo* ubo,
+                             const hvx_aabb* d_clusters,
+                             const hvx_light* d_lights, uint32_t n_lights,
+                             uint32_t cap, uint32_t flags,
+                             hvx_light_cell* d_cells,
+                             uint32_t* d_indices, size_t indices_capacity,
+                             uint32_t* d_total, void* stream);
 
 /* Clustered Cook-Torrance + split-sum IBL shading of SoA fragments.
  * Replaces PipelinePBRClusterForward::FillCommandBuffer

@@ -156,6 +156,9 @@ public:
         metallic_.resize(n);
         std::memcpy(ivp_, invViewProj, sizeof(ivp_));
     }
+    // a new camera over the same G-buffer planes (world positions are rebuilt
+    // from pixel centre and depth with this matrix)
+    void SetInvViewProj(const float invViewProj[16]) { std::memcpy(ivp_, invViewProj, sizeof(ivp_)); }
     hvx_frag_soa view() const {
         hvx_frag_soa f{};
         f.depth = depth_.data();
@@ -192,6 +195,7 @@ public:
     virtual void FillCommandBuffer(hipStream_t stream) = 0;
     virtual void OnWindowResized() {}
     virtual void SetCameraUBO(const hvx_camera_ubo& ubo) { cameraUBO_ = ubo; }
+    Context& context() const { return ctx_; }
 
 protected:
     Context& ctx_;
@@ -514,10 +518,12 @@ private:
 
 // Captures the stages' FillCommandBuffer calls (in order) into one hipGraph
 // on a capture stream and replays it (AppBase::FillCommandBuffer's role). The
-// stages must have run once eagerly first so their scratch is allocated (no
-// allocation is captured). Call Capture() again after any stage parameter
-// changes (camera/UBO, push constants): the executable graph is updated in
-// place when the topology is unchanged.
+// stages must have run once eagerly first so their scratch is allocated (the
+// library refuses to allocate under capture). Call Capture() again after any
+// stage parameter changes (camera/UBO, push constants): the executable graph
+// is updated in place when the topology is unchanged. Launch() refuses a
+// graph whose contexts have reallocated scratch since the capture (the
+// graph's pointers to it are freed): capture again.
 class FrameGraph {
 public:
     explicit FrameGraph(std::vector<PipelineBase*> stages) : stages_(std::move(stages)) {
@@ -532,6 +538,8 @@ public:
     FrameGraph& operator=(const FrameGraph&) = delete;
 
     void Capture() {
+        gens_.clear();
+        for (PipelineBase* p : stages_) gens_.push_back(hvx_ctx_scratch_generation(p->context().get()));
         hipGraph_t g = nullptr;
         hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         try {
@@ -548,6 +556,7 @@ public:
             if (hipGraphExecUpdate(exec_, g, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
                 (void)hipGraphDestroy(graph_);
                 graph_ = g;
+                updates_ += 1;
                 return;
             }
             (void)hipGetLastError();
@@ -558,7 +567,16 @@ public:
         graph_ = g;
         hip_check(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
     }
-    void Launch() { hip_check(hipGraphLaunch(exec_, stream_), "hipGraphLaunch"); }
+    void Launch() {
+        if (!exec_) throw Error(HVX_ERR_INVALID_ARG, "FrameGraph::Launch before Capture");
+        for (size_t i = 0; i < stages_.size(); ++i)
+            if (hvx_ctx_scratch_generation(stages_[i]->context().get()) != gens_[i])
+                throw Error(HVX_ERR_STALE_GRAPH, "FrameGraph::Launch: a stage's context reallocated its scratch "
+                                                 "after the capture; Capture() again");
+        hip_check(hipGraphLaunch(exec_, stream_), "hipGraphLaunch");
+    }
+    // captures that updated the executable graph in place
+    size_t update_count() const { return updates_; }
     hipStream_t stream() const { return stream_; }
     size_t node_count() const {
         size_t n = 0;
@@ -568,9 +586,11 @@ public:
 
 private:
     std::vector<PipelineBase*> stages_;
+    std::vector<uint64_t> gens_;
     hipStream_t stream_ = nullptr;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
+    size_t updates_ = 0;
 };
 
 }  // namespace hvx

@@ -1018,6 +1018,63 @@ void hvxo_cf_cull_lights(const hvx_cf_ubo* u, const float* aabbs, const hvx_ligh
     if (overflow) *overflow = novf;
 }
 
+// ---- LightCullingBatch.comp:61-106, the batched variant: workgroups of T
+// invocations (the shader: 16x9x4 = 576), invocation l of workgroup g culls
+// cluster g*T + l (:67-68); the workgroup's shared array of T lights
+// (:20) is refilled per batch of T lights, slot t by invocation t only when
+// batch*T + t < n (:75-82; slots never written hold zero lights here, which
+// the shader leaves undefined); every invocation tests all T slots in order,
+// recomputing posView = cameraView * position per test (:52-59, :86-93).
+// flags (hvx.h): without HVX_CULL_BATCH_REFERENCE_CAP a list stops at `cap`,
+// with it the list keeps every hit (the shader writes past its 150-entry
+// array, :90); without HVX_CULL_BATCH_REFERENCE_TAIL only slots holding
+// lights of the current batch are tested. Offsets: exclusive scan in
+// cluster order (the shader: atomicAdd, :98); entries at packed positions >=
+// capacity are dropped; *total = sum of counts.
+void hvxo_cf_cull_lights_batch(const hvx_cf_ubo* u, const float* aabbs, const hvx_light* lights, uint32_t n,
+                               uint32_t cap, uint32_t flags, uint32_t T, hvx_light_cell* cells, uint32_t* indices,
+                               uint64_t capacity, uint32_t* total, int threads) {
+    const uint32_t C = u->slice_count_x * u->slice_count_y * u->slice_count_z;
+    const bool ref_cap = (flags & HVX_CULL_BATCH_REFERENCE_CAP) != 0u;
+    const bool ref_tail = (flags & HVX_CULL_BATCH_REFERENCE_TAIL) != 0u;
+    const uint32_t groups = (C + T - 1) / T, batches = (n + T - 1) / T;
+    std::vector<std::vector<uint32_t>> lists(C);
+    parallel_for(groups, threads, [&](uint32_t g) {
+        std::vector<hvx_light> shared(T);
+        std::memset(shared.data(), 0, T * sizeof(hvx_light));
+        std::vector<uint8_t> full(T, 0);
+        for (uint32_t batch = 0; batch < batches; ++batch) {
+            for (uint32_t t = 0; t < T; ++t)
+                if (batch * T + t < n) shared[t] = lights[batch * T + t];
+            for (uint32_t l = 0; l < T; ++l) {
+                const uint32_t c = g * T + l;
+                if (c >= C || full[l]) continue;
+                const float* box = aabbs + (size_t)c * 8;
+                for (uint32_t i = 0; i < T; ++i) {
+                    if (!ref_tail && batch * T + i >= n) break;
+                    float o[4];
+                    matvec<float>(u->camera_view, shared[i].position, o);
+                    if (!test_sphere_aabb(o, shared[i].radius, box)) continue;
+                    if (!ref_cap && lists[c].size() >= cap) {
+                        full[l] = 1;
+                        break;
+                    }
+                    lists[c].push_back(batch * T + i);
+                }
+            }
+        }
+    });
+    uint64_t off = 0;
+    for (uint32_t c = 0; c < C; ++c) {
+        cells[c].offset = (uint32_t)off;
+        cells[c].count = (uint32_t)lists[c].size();
+        for (size_t k = 0; k < lists[c].size(); ++k)
+            if (off + k < capacity) indices[off + k] = lists[c][k];
+        off += lists[c].size();
+    }
+    if (total) *total = (uint32_t)off;
+}
+
 // ---- a5-a8: shading of selected fragments (out: 3 per id)
 void hvxo_cf_shade_f32(const hvx_frag_soa* fr, const hvx_camera_ubo* cam, const hvx_cf_ubo* u,
                        const hvx_pbr_pc* pc, const hvx_light* lights, const hvx_light_cell* cells,

@@ -32,6 +32,7 @@ _SIGS = [
     ("hvxo_cf_build_aabbs_f32", None, [P(CFUBO), vp]),
     ("hvxo_cf_build_aabbs_f64", None, [P(CFUBO), vp]),
     ("hvxo_cf_cull_lights", None, [P(CFUBO), vp, vp, u32, u32, vp, vp, vp, vp, C.c_int]),
+    ("hvxo_cf_cull_lights_batch", None, [P(CFUBO), vp, vp, u32, u32, u32, u32, vp, vp, C.c_uint64, vp, C.c_int]),
     ("hvxo_cf_shade_f32", None, [P(FragSoA), P(CameraUBO), P(CFUBO), P(PBRPC), vp, vp, vp, P(IBLMaps), vp, u32, vp,
                                  C.c_int]),
     ("hvxo_cf_shade_f64", None, [P(FragSoA), P(CameraUBO), P(CFUBO), P(PBRPC), vp, vp, vp, P(IBLMaps), vp, u32, vp,
@@ -160,6 +161,25 @@ def cull_lights(ubo: CFUBO, aabbs: np.ndarray, lights: np.ndarray, cap: int, thr
                               cells.ctypes.data, idx.ctypes.data, tot.ctypes.data, ov.ctypes.data,
                               _threads(threads))
     return cells, idx[: int(tot[0])].copy(), int(tot[0]), int(ov[0])
+
+
+def cull_lights_batch(ubo: CFUBO, aabbs: np.ndarray, lights: np.ndarray, cap: int, flags: int = 0, batch: int = 576,
+                      capacity=None, threads=None):
+    """LightCullingBatch.comp restated (flags: hvx.h HVX_CULL_BATCH_REFERENCE_*).
+    Returns (cells[C] CELL_DTYPE, indices[min(total, capacity)] u32, total)."""
+    Cn = ubo.slice_count_x * ubo.slice_count_y * ubo.slice_count_z
+    aabbs = _c(aabbs, np.float32).reshape(Cn, 8)
+    lights = np.ascontiguousarray(lights, LIGHT_DTYPE)
+    if capacity is None:
+        capacity = Cn * (max(cap, lights.shape[0] + batch) if flags & 1 else cap)
+    cells = np.zeros(Cn, CELL_DTYPE)
+    idx = np.zeros(max(capacity, 1), np.uint32)
+    tot = np.zeros(1, np.uint32)
+    lib().hvxo_cf_cull_lights_batch(C.byref(ubo), aabbs.ctypes.data, lights.ctypes.data, lights.shape[0], cap, flags,
+                                    batch, cells.ctypes.data, idx.ctypes.data, capacity, tot.ctypes.data,
+                                    _threads(threads))
+    t = int(tot[0])
+    return cells, idx[: min(t, capacity)].copy(), t
 
 
 def shade(frags: dict, width, height, ivp, cam: CameraUBO, ubo: CFUBO, pc: PBRPC, lights, cells, indices,

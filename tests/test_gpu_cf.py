@@ -215,3 +215,119 @@ def test_abi_rejects_bad_arguments(ctx):
     with pytest.raises(abi.HvxError) as e:
         ctx.build_aabbs(ubo, d[1:])  # 4-byte offset: misaligned
     assert e.value.status == abi.HVX_ERR_ALIGN
+
+
+# ------------------------------------------- LightCullingBatch.comp variant
+def gpu_cull_batch(ctx, ubo, d_aabbs, lights_np, cap, flags, capacity=None):
+    import torch
+    Cn = ubo.slice_count_x * ubo.slice_count_y * ubo.slice_count_z
+    n = len(lights_np)
+    capacity = capacity or Cn * cap
+    d_lights = to_dev(lights_np.view(np.uint8)) if n else torch.empty(48, dtype=torch.uint8, device="cuda")
+    cells = torch.zeros(Cn * 2, dtype=torch.int32, device="cuda")
+    idx = torch.full((capacity,), -1, dtype=torch.int32, device="cuda")
+    tot = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ctx.cull_lights_batch(ubo, d_aabbs, d_lights, n, cap, flags, cells, idx, capacity, tot)
+    torch.cuda.synchronize()
+    t = int(tot.item())
+    return cells.cpu().numpy().view(abi.CELL_DTYPE), idx.cpu().numpy().view(np.uint32), t
+
+
+@pytest.mark.parametrize("cfg,seed", [(SMALL_FRAME, 11), (UHD_FRAME, 12)], ids=["2560x1600", "3840x2160"])
+def test_cull_batch_variant_equals_cull_lights(ctx, cfg, seed):
+    """LightCullingBatch.comp with its cap and tail defects fixed produces
+    hvx_cf_cull_lights's lists bit for bit (cells, packed indices, total)."""
+    _, _, ubo, _ = camera(cfg["width"], cfg["height"], cfg["grid"])
+    lights = abi.scene_lights(seed, cfg["n_lights"])
+    d_aabbs = gpu_aabbs(ctx, ubo)
+    cells, idx, tot, *_ = gpu_cull(ctx, ubo, d_aabbs, lights, cfg["cap"])
+    b_cells, b_idx, b_tot = gpu_cull_batch(ctx, ubo, d_aabbs, lights, cfg["cap"], 0)
+    assert b_tot == tot
+    assert np.array_equal(b_cells.view(np.uint32), cells.view(np.uint32))
+    assert np.array_equal(b_idx[:tot], idx)
+    assert np.all(b_idx[tot:].view(np.int32) == -1)
+
+
+@pytest.mark.parametrize("n_lights,cap,flags", [
+    (1000, 128, abi.CULL_BATCH_REFERENCE_TAIL), (1000, 16, abi.CULL_BATCH_REFERENCE_CAP),
+    (1000, 16, abi.CULL_BATCH_REFERENCE_CAP | abi.CULL_BATCH_REFERENCE_TAIL), (300, 8, abi.CULL_BATCH_REFERENCE_TAIL),
+    (0, 8, 3), (1, 1, 0), (577, 3, 0), (1152, 2, 3)])
+def test_cull_batch_variant_against_oracle(ctx, n_lights, cap, flags):
+    """Every switch combination against the oracle's restatement of
+    LightCullingBatch.comp at the 16x9x24 grid it was written for: cells,
+    total, and the entries that fit the index buffer."""
+    _, _, ubo, _ = camera(2560, 1600, (16, 9, 24))
+    lights = abi.scene_lights(300 + n_lights, n_lights)
+    Cn = 16 * 9 * 24
+    ref_cells, ref_idx, ref_tot = O.cull_lights_batch(ubo, O.build_aabbs(ubo), lights, cap, flags,
+                                                      capacity=Cn * cap)
+    cells, idx, tot = gpu_cull_batch(ctx, ubo, gpu_aabbs(ctx, ubo), lights, cap, flags)
+    assert tot == ref_tot
+    assert np.array_equal(cells.view(np.uint32), ref_cells.view(np.uint32))
+    kept = min(tot, Cn * cap)
+    assert np.array_equal(idx[:kept], ref_idx[:kept])
+    if flags & abi.CULL_BATCH_REFERENCE_CAP and n_lights >= 1000:
+        assert int(cells["count"].max()) > cap  # lists past the cap
+    if flags & abi.CULL_BATCH_REFERENCE_TAIL and n_lights % 576 and n_lights > 576:
+        assert int(idx[:kept].max()) >= n_lights  # stale slots emitted
+
+
+def test_cull_batch_variant_rejects_bad_flags(ctx):
+    _, _, ubo, _ = camera(320, 200, (4, 3, 6))
+    lights = abi.scene_lights(5, 10)
+    with pytest.raises(abi.HvxError) as e:
+        gpu_cull_batch(ctx, ubo, gpu_aabbs(ctx, ubo), lights, 4, 4)
+    assert e.value.status == abi.HVX_ERR_INVALID_ARG
+
+
+# ------------------------------------------------ scratch under graph capture
+def test_scratch_growth_refused_under_capture_and_counted(gpu):
+    """A fresh context's first culling call inside a graph capture would have
+    to allocate: it is refused with HVX_ERR_UNSUPPORTED (nothing launched,
+    the capture stays valid). Eager calls that grow the scratch bump the
+    generation counter; calls that fit do not, and those may be captured."""
+    import torch
+    c = abi.Context(0)
+    try:
+        _, _, ubo, _ = camera(320, 200, (4, 3, 6))
+        lights = abi.scene_lights(5, 200)
+        d_aabbs = gpu_aabbs(c, ubo)
+        d_lights = to_dev(lights.view(np.uint8))
+        Cn = 4 * 3 * 6
+        cells = torch.zeros(Cn * 2, dtype=torch.int32, device="cuda")
+        idx = torch.zeros(Cn * 32, dtype=torch.int32, device="cuda")
+        tot = torch.zeros(1, dtype=torch.int32, device="cuda")
+        g0 = c.scratch_generation
+        s = torch.cuda.Stream()
+        x = torch.zeros(16, device="cuda")
+        g = torch.cuda.CUDAGraph()
+        err = None
+        with torch.cuda.graph(g, stream=s):
+            try:
+                c.cull_lights(ubo, d_aabbs, d_lights, 200, 32, cells, idx, Cn * 32, tot, stream=s)
+            except abi.HvxError as e:
+                err = e
+            x.add_(1.0)
+        assert err is not None and err.status == abi.HVX_ERR_UNSUPPORTED, err
+        assert c.scratch_generation == g0
+        g.replay()
+        torch.cuda.synchronize()
+        assert float(x[0]) == 1.0  # the capture itself stayed valid
+        c.cull_lights(ubo, d_aabbs, d_lights, 200, 32, cells, idx, Cn * 32, tot)
+        g1 = c.scratch_generation
+        assert g1 > g0
+        c.cull_lights(ubo, d_aabbs, d_lights, 100, 32, cells, idx, Cn * 32, tot)
+        assert c.scratch_generation == g1  # fits: no reallocation
+        ref = (cells.clone(), idx.clone(), tot.clone())
+        g2 = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g2, stream=s):
+            c.cull_lights(ubo, d_aabbs, d_lights, 100, 32, cells, idx, Cn * 32, tot, stream=s)
+        for _ in range(2):
+            cells.zero_(), idx.zero_(), tot.zero_()
+            torch.cuda.synchronize()
+            g2.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(cells, ref[0]) and torch.equal(idx, ref[1]) and torch.equal(tot, ref[2])
+        torch.cuda.synchronize()
+    finally:
+        c.close()
