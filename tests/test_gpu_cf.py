@@ -316,18 +316,24 @@ def test_scratch_growth_refused_under_capture_and_counted(gpu):
         c.cull_lights(ubo, d_aabbs, d_lights, 200, 32, cells, idx, Cn * 32, tot)
         g1 = c.scratch_generation
         assert g1 > g0
+        # the lists fill only the first `tot` slots of the index buffer: clear
+        # it so the slots past them compare equal too
+        idx.zero_()
         c.cull_lights(ubo, d_aabbs, d_lights, 100, 32, cells, idx, Cn * 32, tot)
         assert c.scratch_generation == g1  # fits: no reallocation
         ref = (cells.clone(), idx.clone(), tot.clone())
         g2 = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g2, stream=s):
             c.cull_lights(ubo, d_aabbs, d_lights, 100, 32, cells, idx, Cn * 32, tot, stream=s)
-        for _ in range(2):
+        for r in range(3):
             cells.zero_(), idx.zero_(), tot.zero_()
             torch.cuda.synchronize()
             g2.replay()
             torch.cuda.synchronize()
-            assert torch.equal(cells, ref[0]) and torch.equal(idx, ref[1]) and torch.equal(tot, ref[2])
+            assert torch.equal(cells, ref[0]) and torch.equal(idx, ref[1]) and torch.equal(tot, ref[2]), f"replay {r}"
+            # eager work on the same context between replays
+            c.cull_lights(ubo, d_aabbs, d_lights, 60, 32, cells, idx, Cn * 32, tot)
+            torch.cuda.synchronize()
         torch.cuda.synchronize()
     finally:
         c.close()
