@@ -147,6 +147,9 @@ SIGNATURES = [
     ("hvx_ctx_scratch_generation", C.c_uint64, [vp]),
     ("hvx_cf_build_aabbs", C.c_int, [vp, P(CFUBO), vp, vp]),
     ("hvx_cf_cull_lights", C.c_int, [vp, P(CFUBO), vp, vp, u32, u32, vp, vp, C.c_size_t, vp, vp, vp]),
+    ("hvx_cf_cull_lights_noalloc", C.c_int, [vp, P(CFUBO), vp, vp, u32, u32, vp, vp, C.c_size_t, vp, vp, vp]),
+    ("hvx_cf_cull_noalloc_scratch_bytes", C.c_size_t, [u32, u32, u32]),
+    ("hvx_cf_cull_noalloc_scratch_limit", C.c_size_t, []),
     ("hvx_cf_cull_lights_batch", C.c_int, [vp, P(CFUBO), vp, vp, u32, u32, u32, vp, vp, C.c_size_t, vp, vp]),
     ("hvx_cf_shade", C.c_int, [vp, P(FragSoA), P(CameraUBO), P(CFUBO), P(PBRPC), vp, u32, vp, vp,
                                P(IBLMaps), vp, vp]),
@@ -357,6 +360,14 @@ class Context:
         self._check(self.lib.hvx_cf_cull_lights(self.h, C.byref(ubo), _ptr(clusters), _ptr(lights), n_lights, cap,
                                                 _ptr(cells), _ptr(indices), indices_capacity, _ptr(total),
                                                 _ptr(overflow), self._stream(stream)), "hvx_cf_cull_lights")
+
+    def cull_lights_noalloc(self, ubo: CFUBO, clusters, lights, n_lights: int, cap: int, cells, indices,
+                            indices_capacity: int, total=None, overflow=None, stream=None):
+        """hvx_cf_cull_lights on the library's fixed scratch (allocates nothing)."""
+        self._check(self.lib.hvx_cf_cull_lights_noalloc(self.h, C.byref(ubo), _ptr(clusters), _ptr(lights), n_lights,
+                                                        cap, _ptr(cells), _ptr(indices), indices_capacity,
+                                                        _ptr(total), _ptr(overflow), self._stream(stream)),
+                    "hvx_cf_cull_lights_noalloc")
 
     def cull_lights_batch(self, ubo: CFUBO, clusters, lights, n_lights: int, cap: int, flags: int, cells, indices,
                           indices_capacity: int, total=None, stream=None):

@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 
@@ -99,7 +100,42 @@ hipError_t ensure_scratch(hvx_ctx* ctx, void** buf, size_t* cur, size_t bytes, h
 }
 }  // namespace hvx
 
+// Fixed culling scratch of hvx_cf_cull_lights_noalloc: fixed-slot lists,
+// counts/overflow/partials and view-space lights of one call, carved in that
+// order at 256-byte boundaries. Every GPU the library runs on holds its own
+// copy for the life of the process. In host code `fixed_cull_scratch` names a
+// host-side stand-in, never device memory: host code uses the address
+// hipGetSymbolAddress returns for the device (hvx_ctx::fixed_scratch).
+constexpr size_t kFixedCullScratchBytes = size_t(24) << 20;
+static __device__ __align__(256) unsigned char fixed_cull_scratch[kFixedCullScratchBytes];
+
 namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_fixed_mutex;
+unsigned char* g_fixed_addr[kMaxDevices] = {};
+
+// Device address of `device`'s fixed scratch, looked up on the first request
+// for that device (with the device current) and kept for the process.
+hipError_t fixed_scratch_address(int device, unsigned char** out) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_fixed_mutex);
+    if (!g_fixed_addr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(fixed_cull_scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess) return e;
+        g_fixed_addr[device] = static_cast<unsigned char*>(p);
+    }
+    *out = g_fixed_addr[device];
+    return hipSuccess;
+}
+
+size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
 // Status of a failed scratch growth: a refusal under graph capture, or the
 // runtime's error.
 int scratch_fail(hvx_ctx* ctx, hipError_t e, const char* what) {
@@ -165,6 +201,13 @@ int hvx_ctx_create(int device, hvx_ctx** out) {
     if (!c) return HVX_ERR_HIP;
     c->device = device;
     c->num_cus = prop.multiProcessorCount;
+    // the device's fixed culling scratch; without it only
+    // hvx_cf_cull_lights_noalloc is unavailable (it reports the error)
+    const hipError_t e = fixed_scratch_address(device, &c->fixed_scratch);
+    if (e != hipSuccess) {
+        c->fixed_scratch = nullptr;
+        c->fixed_scratch_error = hipGetErrorString(e);
+    }
     *out = c;
     return HVX_OK;
 }
@@ -209,10 +252,14 @@ int hvx_cf_build_aabbs(hvx_ctx* ctx, const hvx_cf_ubo* ubo, hvx_aabb* d_clusters
     return e == hipSuccess ? HVX_OK : hip_fail(ctx, e, "hvx_cf_build_aabbs launch");
 }
 
-int hvx_cf_cull_lights(hvx_ctx* ctx, const hvx_cf_ubo* ubo, const hvx_aabb* d_clusters, const hvx_light* d_lights,
-                       uint32_t n_lights, uint32_t cap, hvx_light_cell* d_cells, uint32_t* d_indices,
-                       size_t indices_capacity, uint32_t* d_total, uint32_t* d_overflow_count, void* stream) {
-    if (!ctx) return HVX_ERR_NULL;
+}  // extern "C"
+
+namespace {
+// Argument checks shared by hvx_cf_cull_lights and its allocation-free
+// variant; on success *C_out is the cluster count.
+int check_cull_args(hvx_ctx* ctx, const hvx_cf_ubo* ubo, const hvx_aabb* d_clusters, const hvx_light* d_lights,
+                    uint32_t n_lights, uint32_t cap, const hvx_light_cell* d_cells, const uint32_t* d_indices,
+                    size_t indices_capacity, uint64_t* C_out) {
     int st = check_ubo(ctx, ubo);
     if (st) return st;
     if (!d_clusters || !d_cells || !d_indices) return fail(ctx, HVX_ERR_NULL, "clusters/cells/indices NULL");
@@ -228,6 +275,20 @@ int hvx_cf_cull_lights(hvx_ctx* ctx, const hvx_cf_ubo* ubo, const hvx_aabb* d_cl
     if ((uint64_t)indices_capacity < C * cap)
         return fail(ctx, HVX_ERR_CAPACITY, "indices_capacity %zu < clusters*cap = %llu", indices_capacity,
                     (unsigned long long)(C * cap));
+    *C_out = C;
+    return HVX_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int hvx_cf_cull_lights(hvx_ctx* ctx, const hvx_cf_ubo* ubo, const hvx_aabb* d_clusters, const hvx_light* d_lights,
+                       uint32_t n_lights, uint32_t cap, hvx_light_cell* d_cells, uint32_t* d_indices,
+                       size_t indices_capacity, uint32_t* d_total, uint32_t* d_overflow_count, void* stream) {
+    if (!ctx) return HVX_ERR_NULL;
+    uint64_t C = 0;
+    int st = check_cull_args(ctx, ubo, d_clusters, d_lights, n_lights, cap, d_cells, d_indices, indices_capacity, &C);
+    if (st) return st;
     if ((st = bind_device(ctx))) return st;
     const hipStream_t s = (hipStream_t)stream;
     hipError_t e;
@@ -241,6 +302,43 @@ int hvx_cf_cull_lights(hvx_ctx* ctx, const hvx_cf_ubo* ubo, const hvx_aabb* d_cl
                          static_cast<float4*>(ctx->light_vs), static_cast<uint32_t*>(ctx->cull_lists), counts,
                          counts + C, s);
     return e == hipSuccess ? HVX_OK : hip_fail(ctx, e, "hvx_cf_cull_lights launch");
+}
+
+size_t hvx_cf_cull_noalloc_scratch_bytes(uint32_t clusters, uint32_t cap, uint32_t n_lights) {
+    const size_t C = clusters;
+    return align256(C * cap * 4) + align256(C * 8 + ((C + 1023) / 1024) * 8 + 64) + align256((size_t)n_lights * 16);
+}
+
+size_t hvx_cf_cull_noalloc_scratch_limit(void) { return kFixedCullScratchBytes; }
+
+int hvx_cf_cull_lights_noalloc(hvx_ctx* ctx, const hvx_cf_ubo* ubo, const hvx_aabb* d_clusters,
+                               const hvx_light* d_lights, uint32_t n_lights, uint32_t cap, hvx_light_cell* d_cells,
+                               uint32_t* d_indices, size_t indices_capacity, uint32_t* d_total,
+                               uint32_t* d_overflow_count, void* stream) {
+    if (!ctx) return HVX_ERR_NULL;
+    uint64_t C = 0;
+    int st = check_cull_args(ctx, ubo, d_clusters, d_lights, n_lights, cap, d_cells, d_indices, indices_capacity, &C);
+    if (st) return st;
+    if (!ctx->fixed_scratch)
+        return fail(ctx, HVX_ERR_HIP, "the fixed culling scratch of device %d is unavailable: %s", ctx->device,
+                    ctx->fixed_scratch_error.c_str());
+    const size_t lists_b = align256(C * cap * 4), meta_b = align256(C * 8 + ((C + 1023) / 1024) * 8 + 64);
+    const size_t need = hvx_cf_cull_noalloc_scratch_bytes((uint32_t)C, cap, n_lights);
+    if (need > kFixedCullScratchBytes)
+        return fail(ctx, HVX_ERR_CAPACITY,
+                    "%llu clusters x cap %u with %u lights need %zu bytes of culling scratch; the fixed scratch of "
+                    "hvx_cf_cull_lights_noalloc holds %zu (use hvx_cf_cull_lights)",
+                    (unsigned long long)C, cap, n_lights, need, kFixedCullScratchBytes);
+    if ((st = bind_device(ctx))) return st;
+    // lists, counts + overflow + partials, view-space lights: every word the
+    // kernels read is written earlier in the same call, so nothing is cleared
+    unsigned char* base = ctx->fixed_scratch;
+    uint32_t* lists = reinterpret_cast<uint32_t*>(base);
+    uint32_t* counts = reinterpret_cast<uint32_t*>(base + lists_b);
+    float4* lvs = reinterpret_cast<float4*>(base + lists_b + meta_b);
+    const hipError_t e = hvx::launch_cull(*ubo, d_clusters, d_lights, n_lights, cap, d_cells, d_indices, d_total,
+                                          d_overflow_count, lvs, lists, counts, counts + C, (hipStream_t)stream);
+    return e == hipSuccess ? HVX_OK : hip_fail(ctx, e, "hvx_cf_cull_lights_noalloc launch");
 }
 
 int hvx_cf_cull_lights_batch(hvx_ctx* ctx, const hvx_cf_ubo* ubo, const hvx_aabb* d_clusters,

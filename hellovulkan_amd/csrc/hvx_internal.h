@@ -31,6 +31,10 @@ struct hvx_ctx {
     // bumped whenever a scratch buffer is (re)allocated: a graph captured
     // before the bump may hold pointers to freed scratch
     uint64_t scratch_gen = 0;
+    // device address of the library's fixed culling scratch on `device`
+    // (hvx_cf_cull_lights_noalloc), looked up once per device at creation
+    unsigned char* fixed_scratch = nullptr;
+    std::string fixed_scratch_error;  // why the lookup failed, if it did
     // options (hvx_ctx_set_option)
     int shade_exact = 0;
 };

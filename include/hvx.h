@@ -220,6 +220,33 @@ int hvx_cf_cull_lights(hvx_ctx* ctx, const hvx_cf_ubo* ubo,
                        uint32_t* d_total, uint32_t* d_overflow_count,
                        void* stream);
 
+/* hvx_cf_cull_lights without any allocation: no counterpart in the
+ * reference. It is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight per-frame serving loops, hosts that forbid
+ * hipMalloc). Same arguments, checks, kernels and results as
+ * hvx_cf_cull_lights; its scratch (fixed-slot lists, counts, view-space
+ * lights) is a fixed device array of the library, one per GPU, held until
+ * the process ends, instead of the context's growable scratch. Calls whose
+ * scratch (hvx_cf_cull_noalloc_scratch_bytes) exceeds
+ * hvx_cf_cull_noalloc_scratch_limit() (24 MiB: e.g. 32x18x48 clusters, cap
+ * 128 and 65,536 lights need 15.4 MB) are refused with HVX_ERR_CAPACITY
+ * before anything is launched. Every call on one GPU shares that GPU's
+ * array, so calls that overlap on one GPU (other streams, other contexts,
+ * other host threads) race: serialise them. It never calls hipMalloc,
+ * hipFree or a synchronising runtime function. */
+int hvx_cf_cull_lights_noalloc(hvx_ctx* ctx, const hvx_cf_ubo* ubo,
+                               const hvx_aabb* d_clusters,
+                               const hvx_light* d_lights, uint32_t n_lights,
+                               uint32_t cap,
+                               hvx_light_cell* d_cells,
+                               uint32_t* d_indices, size_t indices_capacity,
+                               uint32_t* d_total, uint32_t* d_overflow_count,
+                               void* stream);
+/* Bytes of fixed scratch a hvx_cf_cull_lights_noalloc call needs, and the
+ * size of the fixed scratch (the limit). */
+size_t hvx_cf_cull_noalloc_scratch_bytes(uint32_t clusters, uint32_t cap, uint32_t n_lights);
+size_t hvx_cf_cull_noalloc_scratch_limit(void);
+
 /* The batched variant of the culling shader, LightCullingBatch.comp:61-106
  * (the reference builds it but dispatches LightCulling.comp instead,
  * PipelineLightCulling.cpp:25-26, :80-83): one invocation per cluster,
