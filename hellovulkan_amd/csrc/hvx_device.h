@@ -75,6 +75,11 @@ __device__ __forceinline__ float rcp_rn_normal(float b) {
     return __builtin_fmaf(e, y, y);
 }
 
+// normalize3 of a vector whose squared length is a normal float in
+// [2^-96, 2^126] (every near-unit vector): the same correctly rounded square
+// root and reciprocal, by the fast paths above, so the result is bit-identical.
+__device__ __forceinline__ f3 normalize3_rn(f3 a) { return mul3(a, rcp_rn_normal(sqrt_rn_normal(dot3(a, a)))); }
+
 // ------------------------------------------------------------ cube sampling
 struct CubeCoord {
     int face;
@@ -115,7 +120,7 @@ __device__ __forceinline__ bool edge_fold(int& c, int& mj, int S) {
 
 // Texel (face, i, j) with at most one of i, j one step outside [0, S):
 // returns the linear texel index of the texel across the edge.
-__device__ __noinline__ int cube_edge_index(int f, int i, int j, int S) {
+__device__ __forceinline__ int cube_edge_index_inl(int f, int i, int j, int S) {
     const int a = 2 * i + 1 - S, b = 2 * j + 1 - S;
     // 3D point (doubled-integer units) of the texel centre on face f's plane
     int p0, p1, p2;
@@ -148,6 +153,7 @@ __device__ __noinline__ int cube_edge_index(int f, int i, int j, int S) {
     const int ni = (na + S - 1) >> 1, nj = (nb + S - 1) >> 1;
     return (nf * S + nj) * S + ni;
 }
+__device__ __noinline__ int cube_edge_index(int f, int i, int j, int S) { return cube_edge_index_inl(f, i, j, S); }
 
 // RGB of texel (f, i, j) of a cube level with seamless edges.
 __device__ __noinline__ f3 cube_texel_seamless(const float4* __restrict__ level, int S, int f, int i, int j) {
@@ -197,6 +203,70 @@ __device__ __forceinline__ f3 sample_cube(const float4* __restrict__ level, int 
     return {((w00 * t00.x + w10 * t10.x) + w01 * t01.x) + w11 * t11.x,
             ((w00 * t00.y + w10 * t10.y) + w01 * t01.y) + w11 * t11.y,
             ((w00 * t00.z + w10 * t10.z) + w01 * t01.z) + w11 * t11.z};
+}
+
+// The same bilinear seamless sample in two steps, so the texel loads of one
+// sample can be in flight while other work runs: cube_taps finds the four
+// taps' texel indices and weights (and issues nothing), the caller loads the
+// taps, cube_combine forms the sum exactly as sample_cube does. A tap at a
+// cube corner (both coordinates off the face) is the mean of three texels;
+// such samples are marked and cube_combine recomputes them with sample_cube.
+struct CubeTaps {
+    int idx[4];  // taps (i0,j0), (i0+1,j0), (i0,j0+1), (i0+1,j0+1); a corner tap holds 0
+    float w00, w10, w01, w11;
+    bool corner;
+};
+
+// Linear texel index of tap (f, i, j), i and j at most one step off the
+// face; -1 for a corner tap.
+__device__ __forceinline__ int cube_tap_index(int S, int f, int i, int j) {
+    i = i < -1 ? -1 : (i > S ? S : i);
+    j = j < -1 ? -1 : (j > S ? S : j);
+    const bool iin = (unsigned)i < (unsigned)S, jin = (unsigned)j < (unsigned)S;
+    if (iin && jin) return (f * S + j) * S + i;
+    if (iin || jin) return cube_edge_index_inl(f, i, j, S);
+    return -1;
+}
+
+__device__ __forceinline__ CubeTaps cube_taps(int S, f3 dir) {
+    const CubeCoord cc = cube_select(dir);
+    const float u = cc.s * (float)S - 0.5f;
+    const float v = cc.t * (float)S - 0.5f;
+    const float fu = floorf_(u), fv = floorf_(v);
+    const float a = u - fu, b = v - fv;
+    const int i0 = (int)fu, j0 = (int)fv;
+    CubeTaps t;
+    t.w00 = (1.0f - a) * (1.0f - b);
+    t.w10 = a * (1.0f - b);
+    t.w01 = (1.0f - a) * b;
+    t.w11 = a * b;
+    t.corner = false;
+    if (i0 >= 0 && j0 >= 0 && i0 + 1 < S && j0 + 1 < S) {
+        const int base = (cc.face * S + j0) * S + i0;
+        t.idx[0] = base;
+        t.idx[1] = base + 1;
+        t.idx[2] = base + S;
+        t.idx[3] = base + S + 1;
+    } else {
+        const int di[4] = {0, 1, 0, 1}, dj[4] = {0, 0, 1, 1};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int id = cube_tap_index(S, cc.face, i0 + di[k], j0 + dj[k]);
+            t.corner = t.corner || id < 0;
+            t.idx[k] = id < 0 ? 0 : id;
+        }
+    }
+    return t;
+}
+
+// The bilinear sum of loaded taps q (sample_cube's expression); a sample
+// with a corner tap is resampled with sample_cube.
+__device__ __forceinline__ f3 cube_combine(const float4* __restrict__ level, int S, f3 dir, const CubeTaps& t,
+                                           const float4 q[4]) {
+    if (t.corner) return sample_cube(level, S, dir);
+    return {((t.w00 * q[0].x + t.w10 * q[1].x) + t.w01 * q[2].x) + t.w11 * q[3].x,
+            ((t.w00 * q[0].y + t.w10 * q[1].y) + t.w01 * q[2].y) + t.w11 * q[3].y,
+            ((t.w00 * q[0].z + t.w10 * q[1].z) + t.w01 * q[2].z) + t.w11 * q[3].z};
 }
 
 // ------------------------------------------------------------- 2D sampling

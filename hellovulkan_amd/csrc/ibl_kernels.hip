@@ -9,11 +9,15 @@
 //                                     texel evaluate samples, one lane per
 //                                     texel and channel sums them in order;
 //                                     the phi/theta sin/cos grid in LDS)
-//   CubeFilterSpecular.frag:34-146 -> ggx_table_kernel + specular_kernel (lane
-//                                     per output texel of every mip in one
-//                                     launch; tangent-space GGX samples per mip
-//                                     precomputed once; at roughness 0 the n
-//                                     identical samples are evaluated once)
+//   CubeFilterSpecular.frag:34-146 -> ggx_table_kernel (tangent-space GGX
+//                                     samples per mip, computed once),
+//                                     specular_delta_kernel (mip 0, roughness
+//                                     0: the n identical samples evaluated
+//                                     once, lane per texel) and
+//                                     specular_split_kernel (mips >= 1: each
+//                                     texel's samples spread over 16 lanes, the
+//                                     terms summed in sample order through
+//                                     LDS; tap loads one round ahead)
 // The per-sample sequence and every summation order are the reference's, so
 // outputs reproduce the FP32 oracle to the bit; the precomputed tables hold
 // exactly the values the per-sample code would compute (same functions, same
@@ -236,14 +240,6 @@ __global__ __launch_bounds__(256) void ggx_table_kernel(uint32_t mips, uint32_t 
 }
 
 constexpr int kMaxMips = 16;
-struct SpecArgs {
-    const float4* cube;
-    uint32_t S;
-    float4* out;
-    uint32_t So, mips, n, emulate;
-    uint32_t texel_begin[kMaxMips + 1];  // flattened texel ranges per mip
-    const float4* table;
-};
 
 // One output texel of the GGX prefilter (CubeFilterSpecular.frag:34-75) at
 // direction-grid position (face, x, y) of a D x D viewport; `tab` holds the
@@ -291,17 +287,150 @@ __device__ __forceinline__ float4 specular_texel_value(const float4* __restrict_
     return make_float4(acc0 / wsum, acc1 / wsum, acc2 / wsum, 1.0f);
 }
 
-__global__ __launch_bounds__(256) void specular_kernel(SpecArgs a) {
+// Mip 0 has roughness 0 (PipelineCubeFilter.cpp:305-311): one lane per texel.
+__global__ __launch_bounds__(256) void specular_delta_kernel(const float4* __restrict__ cube, uint32_t S,
+                                                             float4* __restrict__ out, uint32_t So,
+                                                             const float4* __restrict__ table, uint32_t n) {
     const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= a.texel_begin[a.mips]) return;
-    uint32_t m = 0;
-    while (id >= a.texel_begin[m + 1]) ++m;
-    const uint32_t Sm = max(a.So >> m, 1u);
-    const uint32_t local = id - a.texel_begin[m];
-    const uint32_t x = local % Sm, y = (local / Sm) % Sm, face = local / (Sm * Sm);
-    const uint32_t D = a.emulate ? a.So : Sm;
-    a.out[cube_level_offset_floats(a.So, m) / 4 + local] =
-        specular_texel_value(a.cube, a.S, a.table + (size_t)m * a.n, a.n, m == 0, (int)face, (int)x, (int)y, (int)D);
+    if (id >= 6u * So * So) return;
+    const uint32_t x = id % So, y = (id / So) % So, face = id / (So * So);
+    out[id] = specular_texel_value(cube, S, table, n, true, (int)face, (int)x, (int)y, (int)So);
+}
+
+// Mips of roughness > 0, samples spread over lanes. A wave owns kSpecTPW
+// texels of a tile and evaluates kSpecSPR = 64 / kSpecTPW consecutive
+// samples of each per round; every lane writes its sample's four terms
+// (c * NoL, NoL; -0 for a rejected sample, the exact identity of x + y) to
+// LDS, and one "summer" lane per texel and term adds that round's terms in
+// sample order, so every sum is the reference's sequential sum (bit-exact).
+// Few texels per wave keep the texels in flight per XCD few, so the GGX lobes
+// being read at any time share the XCD's L2: tiles are taken in 16x16-tile
+// supertiles, and the workgroups of one XCD (workgroup b runs on XCD b % 8)
+// get one contiguous run of tiles.
+#ifndef HVX_SPEC_TPW
+#define HVX_SPEC_TPW 4
+#endif
+constexpr int kSpecTPW = HVX_SPEC_TPW;                   // texels per wave: 4, 8 or 16
+constexpr int kSpecSPR = 64 / kSpecTPW;                  // samples per texel per round
+constexpr int kSpecTW = kSpecTPW >= 8 ? 8 : 4;           // tile width (texels)
+constexpr int kSpecRPW = kSpecTPW / kSpecTW;             // texel rows per wave
+constexpr int kSpecTH = 4 * kSpecRPW;                    // tile height: four waves
+constexpr int kSpecRow = 4 * kSpecSPR + 4;               // floats per texel row of the LDS buffer (padded)
+static_assert(kSpecTPW == 4 || kSpecTPW == 8 || kSpecTPW == 16, "HVX_SPEC_TPW is 4, 8 or 16");
+
+struct SpecSplitArgs {
+    const float4* cube;
+    uint32_t S;
+    float4* out;
+    uint32_t So, mips, n, emulate;
+    uint32_t n_tiles, per_xcd;               // tiles (padded supertiles) and tiles per XCD run
+    uint32_t tile_begin[kMaxMips + 1];       // first tile of mip m (mips 1..), tile_begin[1] = 0
+    uint32_t tiles_x[kMaxMips], tiles_y[kMaxMips], super_x[kMaxMips], super_n[kMaxMips];
+    const float4* table;
+};
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One GGX sample of the specular loop, split so its texel loads overlap other
+// work: spec_begin forms L and NoL and issues the four tap loads (always, so
+// the loads are not under a branch; a rejected sample's taps are valid
+// texels whose values are not used), spec_term forms the sample's terms.
+struct SpecSample {
+    f3 L;
+    float nol;
+    CubeTaps taps;
+    float4 q[4];
+};
+
+__device__ __forceinline__ void spec_begin(SpecSample& p, const float4* __restrict__ cube, int S, const Frame& fr,
+                                           f3 N, float4 h) {
+    // both vectors are unit length up to rounding: normalize3_rn is exact here
+    const f3 Hh = normalize3_rn(add3(add3(mul3(fr.t, h.x), mul3(fr.b, h.y)), mul3(N, h.z)));  // ggx_world
+    p.L = normalize3_rn(sub3(mul3(Hh, 2.0f * dot3(N, Hh)), N));                            // V = N
+    p.nol = gmax(dot3(N, p.L), 0.0f);
+    p.taps = cube_taps(S, p.L);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) p.q[k] = cube[p.taps.idx[k]];
+}
+
+// (c * NoL, NoL) of a sample that counts (NoL > 0); -0 in every term of one
+// that does not (a rejected or padding sample): x + (-0) == x for every x.
+__device__ __forceinline__ float4 spec_term(const SpecSample& p, bool live, const float4* __restrict__ cube, int S) {
+    if (!(live && p.nol > 0.0f)) return make_float4(-0.0f, -0.0f, -0.0f, -0.0f);
+    const f3 c = cube_combine(cube, S, p.L, p.taps, p.q);
+    return make_float4(c.x * p.nol, c.y * p.nol, c.z * p.nol, p.nol);
+}
+
+#ifndef HVX_SPEC_WAVES_PER_EU
+#define HVX_SPEC_WAVES_PER_EU 4
+#endif
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SPEC_WAVES_PER_EU, 8))) void
+specular_split_kernel(SpecSplitArgs a) {
+    __shared__ __align__(16) float s_buf[4][2][kSpecTPW][kSpecRow];
+    const uint32_t b = blockIdx.x;
+    const uint32_t tile = (b % 8u) * a.per_xcd + b / 8u;
+    if (tile >= a.n_tiles) return;
+    uint32_t m = 1;
+    while (tile >= a.tile_begin[m + 1]) ++m;
+    const uint32_t local = tile - a.tile_begin[m];
+    const uint32_t per_face = a.super_n[m] * 256u;
+    const uint32_t face = local / per_face, k = local % per_face;
+    const uint32_t ks = k / 256u, kw = k % 256u;
+    const uint32_t tx = (ks % a.super_x[m]) * 16u + kw % 16u, ty = (ks / a.super_x[m]) * 16u + kw / 16u;
+    if (tx >= a.tiles_x[m] || ty >= a.tiles_y[m]) return;  // padding of a partial supertile
+    const uint32_t Sm = max(a.So >> m, 1u), D = a.emulate ? a.So : Sm;
+    const uint32_t wave = threadIdx.x / 64u, lane = threadIdx.x % 64u;
+    // evaluating lane: texel t of the wave, sample slot s
+    const uint32_t t = lane / kSpecSPR, s = lane % kSpecSPR;
+    const uint32_t x = min(tx * kSpecTW + t % kSpecTW, Sm - 1),
+                   y = min(ty * kSpecTH + wave * kSpecRPW + t / kSpecTW, Sm - 1);
+    const f3 N = texel_dir((int)face, (int)x, (int)y, (int)D);
+    const Frame fr = ggx_frame(N);
+    const float4* __restrict__ tab = a.table + (size_t)m * a.n;
+    const int S = (int)a.S;
+    const uint32_t n = a.n;
+    // summing lane: texel st of the wave, term sc (R, G, B, NoL)
+    const bool summer = lane < 4u * kSpecTPW;
+    const uint32_t st = lane / 4u, sc = lane % 4u;
+    float acc = 0.0f;
+    float* const wbuf = &s_buf[wave][0][t][4 * s];
+    const float* const rbuf = &s_buf[wave][0][st][sc];
+    constexpr uint32_t kHalf = kSpecTPW * kSpecRow;  // floats between the two buffers
+    // rounds in pairs (A: even, B: odd); sample i of round r is r * SPR + s,
+    // table entries are fetched two rounds ahead and tap loads one round
+    // ahead of their use
+    const uint32_t rounds = (n + kSpecSPR - 1) / kSpecSPR;
+    auto tab_at = [&](uint32_t r) { return tab[min(r * kSpecSPR + s, n - 1)]; };
+    auto sum_round = [&](uint32_t parity, float4 term) {
+        *reinterpret_cast<float4*>(wbuf + parity * kHalf) = term;
+        wave_lds_sync();
+        if (summer) {
+#pragma unroll
+            for (int q = 0; q < kSpecSPR; ++q) acc = acc + rbuf[parity * kHalf + 4 * q];
+        }
+    };
+    SpecSample A, B;
+    float4 hA = tab_at(0), hB = tab_at(1);
+    spec_begin(A, a.cube, S, fr, N, hA);
+    hA = tab_at(2);
+    for (uint32_t r = 0; r < rounds; r += 2) {
+        spec_begin(B, a.cube, S, fr, N, hB);
+        hB = tab_at(r + 3);
+        sum_round(0, spec_term(A, r * kSpecSPR + s < n, a.cube, S));
+        spec_begin(A, a.cube, S, fr, N, hA);
+        hA = tab_at(r + 4);
+        sum_round(1, spec_term(B, (r + 1) * kSpecSPR + s < n, a.cube, S));
+    }
+    const float wsum = __shfl(acc, (int)((lane & ~3u) | 3u));
+    const uint32_t xs = tx * kSpecTW + st % kSpecTW, ys = ty * kSpecTH + wave * kSpecRPW + st / kSpecTW;
+    if (summer && xs < Sm && ys < Sm) {
+        float* o = reinterpret_cast<float*>(a.out + cube_level_offset_floats(a.So, m) / 4 + (face * Sm + ys) * Sm + xs);
+        o[sc] = sc == 3u ? 1.0f : acc / wsum;
+    }
 }
 
 // ------------------------------------------------- batched probe prefilter
@@ -390,24 +519,38 @@ hipError_t launch_specular(const float* d_cube, uint32_t S, float* d_out, uint32
                            bool emulate_static_viewport, float4* ggx_table, hipStream_t s) {
     if (mips > (uint32_t)kMaxMips) return hipErrorInvalidValue;
     hipLaunchKernelGGL(ggx_table_kernel, dim3((mips * n + 255) / 256), dim3(256), 0, s, mips, n, ggx_table);
-    SpecArgs a;
-    a.cube = reinterpret_cast<const float4*>(d_cube);
-    a.S = S;
-    a.out = reinterpret_cast<float4*>(d_out);
-    a.So = So;
-    a.mips = mips;
-    a.n = n;
-    a.emulate = emulate_static_viewport ? 1u : 0u;
-    uint32_t t = 0;
-    for (uint32_t m = 0; m <= (uint32_t)kMaxMips; ++m) {
-        a.texel_begin[m] = t;
-        if (m < mips) {
-            const uint32_t sm = std::max(So >> m, 1u);
-            t += 6u * sm * sm;
+    const float4* cube = reinterpret_cast<const float4*>(d_cube);
+    float4* out = reinterpret_cast<float4*>(d_out);
+    // mip 0 (roughness 0) covers its whole face whether or not the viewport
+    // is emulated: its size is the base size
+    hipLaunchKernelGGL(specular_delta_kernel, dim3((6u * So * So + 255) / 256), dim3(256), 0, s, cube, S, out, So,
+                       ggx_table, n);
+    if (mips > 1) {
+        SpecSplitArgs a{};
+        a.cube = cube;
+        a.S = S;
+        a.out = out;
+        a.So = So;
+        a.mips = mips;
+        a.n = n;
+        a.emulate = emulate_static_viewport ? 1u : 0u;
+        a.table = ggx_table;
+        uint32_t t = 0;
+        for (uint32_t m = 1; m <= (uint32_t)kMaxMips; ++m) {
+            a.tile_begin[m] = t;
+            if (m < mips && m < (uint32_t)kMaxMips) {
+                const uint32_t sm = std::max(So >> m, 1u);
+                a.tiles_x[m] = (sm + kSpecTW - 1) / kSpecTW;
+                a.tiles_y[m] = (sm + kSpecTH - 1) / kSpecTH;
+                a.super_x[m] = (a.tiles_x[m] + 15) / 16;
+                a.super_n[m] = a.super_x[m] * ((a.tiles_y[m] + 15) / 16);
+                t += 6u * a.super_n[m] * 256u;
+            }
         }
+        a.n_tiles = t;
+        a.per_xcd = (t + 7) / 8;
+        hipLaunchKernelGGL(specular_split_kernel, dim3(a.per_xcd * 8), dim3(256), 0, s, a);
     }
-    a.table = ggx_table;
-    hipLaunchKernelGGL(specular_kernel, dim3((t + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
