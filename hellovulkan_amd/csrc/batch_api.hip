@@ -72,11 +72,12 @@ struct hvx_batch {
     DevBuf<hvx::ProbeJob> d_jobs;
     DevBuf<float4> d_tables;
     DevBuf<hvx::ProbeItem> d_items;
+    DevBuf<uint32_t> d_xcd_begin;  // [9]: item runs of the per-XCD probe queues
     uint32_t n_items = 0;
     std::vector<ViewScratch> vscratch;
     DevBuf<hvx::CullBatchView> d_views;
     hvx::CullBatchTotals totals{};
-    DevBuf<uint32_t> d_queues;  // [0] probe queue, [1] culling queue
+    DevBuf<uint32_t> d_queues;  // [0, 8) per-XCD probe queues, [8] culling queue
     hvx_batch_tuning tuning{256, 4, 0, 8, 0.0f, 0.0f};
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr, cfork = nullptr;
@@ -107,40 +108,20 @@ int bhip(hvx_batch* b, hipError_t e, const char* where) {
 
 bool misaligned(const void* p, size_t a) { return (reinterpret_cast<uintptr_t>(p) % a) != 0; }
 
-// Work items of tile `tile` texels, ordered by decreasing cost (a roughness-0
-// mip costs one sample per texel, the others `n`).
-std::vector<hvx::ProbeItem> make_items(const hvx_batch& b, uint32_t tile) {
-    struct Costed {
-        hvx::ProbeItem it;
-        uint64_t cost;
-    };
-    std::vector<Costed> v;
-    for (uint32_t p = 0; p < (uint32_t)b.probes.size(); ++p) {
-        const hvx_probe& pr = b.probes[p];
-        for (uint32_t m = 0; m < pr.out_mips; ++m) {
-            const uint32_t sm = std::max(pr.out_face_size >> m, 1u);
-            const uint32_t texels = 6u * sm * sm;
-            for (uint32_t f = 0; f < texels; f += tile) {
-                const uint32_t c = std::min(tile, texels - f);
-                v.push_back({{p, m, f, c}, (uint64_t)c * (m == 0 ? 1u : b.n)});
-            }
-        }
-    }
-    std::stable_sort(v.begin(), v.end(), [](const Costed& a, const Costed& c) { return a.cost > c.cost; });
-    std::vector<hvx::ProbeItem> out(v.size());
-    for (size_t i = 0; i < v.size(); ++i) out[i] = v[i].it;
-    return out;
-}
-
 int upload_items(hvx_batch* b) {
-    const std::vector<hvx::ProbeItem> items = make_items(*b, b->tuning.probe_tile);
+    std::vector<hvx::ProbeItem> items;
+    uint32_t xcd_begin[9];
+    hvx::make_probe_items(b->jobs.data(), (uint32_t)b->jobs.size(), b->tuning.probe_tile, items, xcd_begin);
     hipError_t e;
     if (b->d_items.n < items.size() && (e = b->d_items.alloc(items.size())) != hipSuccess)
         return bhip(b, e, "batch item allocation");
+    if (!b->d_xcd_begin.p && (e = b->d_xcd_begin.alloc(9)) != hipSuccess) return bhip(b, e, "batch item runs");
     if (!items.empty() &&
         (e = hipMemcpy(b->d_items.p, items.data(), items.size() * sizeof(hvx::ProbeItem), hipMemcpyHostToDevice)) !=
             hipSuccess)
         return bhip(b, e, "batch item upload");
+    if ((e = hipMemcpy(b->d_xcd_begin.p, xcd_begin, sizeof(xcd_begin), hipMemcpyHostToDevice)) != hipSuccess)
+        return bhip(b, e, "batch item runs upload");
     b->n_items = (uint32_t)items.size();
     return HVX_OK;
 }
@@ -219,19 +200,19 @@ hipError_t run_contract(hvx_batch* b, hipStream_t s) {
 hipError_t run_probes(hvx_batch* b, hipStream_t s) {
     if (!b->classes.empty()) return run_contract(b, s);
     if (b->n_items == 0) return hipSuccess;
-    return hvx::launch_probe_batch(b->d_jobs.p, b->d_items.p, b->n_items, b->d_tables.p,
+    return hvx::launch_probe_batch(b->d_jobs.p, b->d_items.p, b->d_xcd_begin.p, b->d_tables.p,
                                    (uint32_t)b->ctx->num_cus * b->tuning.probe_wgs_per_cu, b->d_queues.p, s);
 }
 
 hipError_t run_culls(hvx_batch* b, hipStream_t s) {
     if (b->views.empty()) return hipSuccess;
     return hvx::launch_cull_batch(b->d_views.p, (uint32_t)b->views.size(), b->totals, (int)b->tuning.cull_shape,
-                                  (uint32_t)b->ctx->num_cus * b->tuning.cull_wgs_per_cu, b->d_queues.p + 1, s);
+                                  (uint32_t)b->ctx->num_cus * b->tuning.cull_wgs_per_cu, b->d_queues.p + hvx::kProbeQueues, s);
 }
 
 hipError_t run_all(hvx_batch* b, hipStream_t s) {
     hipError_t e;
-    if ((e = hvx::launch_queue_reset(b->d_queues.p, 2, s)) != hipSuccess) return e;
+    if ((e = hvx::launch_queue_reset(b->d_queues.p, hvx::kProbeQueues + 1, s)) != hipSuccess) return e;
     if ((e = hipEventRecord(b->fork, s)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(b->side, b->fork, 0)) != hipSuccess) return e;
     if ((e = run_probes(b, b->side)) != hipSuccess) return e;
@@ -270,7 +251,7 @@ float time_ms(hvx_batch* b, hipStream_t s, bool probes, int reps) {
     float best = std::numeric_limits<float>::infinity();
     for (int r = 0; r < reps; ++r) {
         (void)hipEventRecord(e0, s);
-        hipError_t e = hvx::launch_queue_reset(b->d_queues.p, 2, s);
+        hipError_t e = hvx::launch_queue_reset(b->d_queues.p, hvx::kProbeQueues + 1, s);
         if (e == hipSuccess) e = probes ? run_probes(b, s) : run_culls(b, s);
         (void)hipEventRecord(e1, s);
         if (e != hipSuccess || hipEventSynchronize(e1) != hipSuccess) break;
@@ -378,7 +359,7 @@ int hvx_batch_create(hvx_ctx* ctx, const hvx_probe* probes, uint32_t n_probes, u
             (e = s.lvs.alloc(v.n_lights)) != hipSuccess)
             return fail_destroy(bhip(b, e, "view scratch"));
     }
-    if ((e = b->d_queues.alloc(2)) != hipSuccess) return fail_destroy(bhip(b, e, "queues"));
+    if ((e = b->d_queues.alloc(hvx::kProbeQueues + 1)) != hipSuccess) return fail_destroy(bhip(b, e, "queues"));
     if ((e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&b->join, hipEventDisableTiming)) != hipSuccess ||
@@ -404,6 +385,7 @@ int hvx_batch_destroy(hvx_batch* b) {
     b->d_jobs.release();
     b->d_tables.release();
     b->d_items.release();
+    b->d_xcd_begin.release();
     b->d_views.release();
     b->d_queues.release();
     for (ViewScratch& s : b->vscratch) {

@@ -7,6 +7,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/hvx.h"
 #include "mlp_device.h"
@@ -78,7 +79,8 @@ struct ProbeJob {
     uint32_t S, So, mips, n;
     uint32_t table;
 };
-// A work item: texels [first, first + count) of one mip of one probe.
+// A work item of one mip of one probe: texels [first, first + count) of mip
+// 0, or padded 4x4-texel tiles [first, first + count) of a mip >= 1.
 struct ProbeItem {
     uint32_t probe, mip, first, count;
 };
@@ -149,8 +151,14 @@ hipError_t launch_gather_copy(const float* const* d_src, size_t floats, float* d
                               uint32_t n, hipStream_t s);
 hipError_t launch_diffuse(const float* d_cube, uint32_t S, float* d_out, uint32_t Sd, hipStream_t s);
 hipError_t launch_ggx_tables(uint32_t mips, uint32_t n, float4* table, hipStream_t s);
-hipError_t launch_probe_batch(const ProbeJob* d_jobs, const ProbeItem* d_items, uint32_t n_items,
-                              const float4* d_tables, uint32_t persistent_wgs, uint32_t* d_queue, hipStream_t s);
+// d_queues: the eight per-XCD probe queues (zeroed before each run)
+hipError_t launch_probe_batch(const ProbeJob* d_jobs, const ProbeItem* d_items, const uint32_t* d_xcd_begin,
+                              const float4* d_tables, uint32_t persistent_wgs, uint32_t* d_queues, hipStream_t s);
+// Work items of a probe batch (probe, mip, tile order; items of about
+// `texels` output texels) and the item runs of the eight XCD queues.
+void make_probe_items(const ProbeJob* jobs, uint32_t n_probes, uint32_t texels, std::vector<ProbeItem>& items,
+                      uint32_t xcd_begin[9]);
+constexpr uint32_t kProbeQueues = 8;  // per-XCD probe queues; the culling queue follows them
 hipError_t launch_specular(const float* d_cube, uint32_t S, float* d_out, uint32_t So, uint32_t mips,
                            uint32_t n, bool emulate_static_viewport, float4* ggx_table, hipStream_t s);
 

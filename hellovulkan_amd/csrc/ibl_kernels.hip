@@ -323,9 +323,8 @@ struct SpecSplitArgs {
     uint32_t S;
     float4* out;
     uint32_t So, mips, n, emulate;
-    uint32_t n_tiles, per_xcd;               // tiles (padded supertiles) and tiles per XCD run
-    uint32_t tile_begin[kMaxMips + 1];       // first tile of mip m (mips 1..), tile_begin[1] = 0
-    uint32_t tiles_x[kMaxMips], tiles_y[kMaxMips], super_x[kMaxMips], super_n[kMaxMips];
+    uint32_t n_tiles, per_xcd;          // padded tiles of mips 1.. and tiles per XCD run
+    uint32_t tile_begin[kMaxMips + 1];  // first tile of mip m (mips 1..), tile_begin[1] = 0
     const float4* table;
 };
 
@@ -365,24 +364,39 @@ __device__ __forceinline__ float4 spec_term(const SpecSample& p, bool live, cons
     return make_float4(c.x * p.nol, c.y * p.nol, c.z * p.nol, p.nol);
 }
 
-#ifndef HVX_SPEC_WAVES_PER_EU
-#define HVX_SPEC_WAVES_PER_EU 4
-#endif
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SPEC_WAVES_PER_EU, 8))) void
-specular_split_kernel(SpecSplitArgs a) {
-    __shared__ __align__(16) float s_buf[4][2][kSpecTPW][kSpecRow];
-    const uint32_t b = blockIdx.x;
-    const uint32_t tile = (b % 8u) * a.per_xcd + b / 8u;
-    if (tile >= a.n_tiles) return;
-    uint32_t m = 1;
-    while (tile >= a.tile_begin[m + 1]) ++m;
-    const uint32_t local = tile - a.tile_begin[m];
-    const uint32_t per_face = a.super_n[m] * 256u;
-    const uint32_t face = local / per_face, k = local % per_face;
-    const uint32_t ks = k / 256u, kw = k % 256u;
-    const uint32_t tx = (ks % a.super_x[m]) * 16u + kw % 16u, ty = (ks / a.super_x[m]) * 16u + kw / 16u;
-    if (tx >= a.tiles_x[m] || ty >= a.tiles_y[m]) return;  // padding of a partial supertile
-    const uint32_t Sm = max(a.So >> m, 1u), D = a.emulate ? a.So : Sm;
+// Tile geometry of one mip of size Sm: tiles of kSpecTW x kSpecTH texels,
+// taken in 16 x 16-tile supertiles (row-major supertiles, row-major tiles in
+// each); padded tile k of a face maps to (tx, ty), outside the face when the
+// supertile is partial.
+struct SpecTileGeom {
+    uint32_t tiles_x, tiles_y, super_x, per_face;  // per_face: padded tiles per face
+};
+__host__ __device__ inline SpecTileGeom spec_tile_geom(uint32_t Sm) {
+    SpecTileGeom g;
+    g.tiles_x = (Sm + kSpecTW - 1) / kSpecTW;
+    g.tiles_y = (Sm + kSpecTH - 1) / kSpecTH;
+    g.super_x = (g.tiles_x + 15) / 16;
+    g.per_face = g.super_x * ((g.tiles_y + 15) / 16) * 256u;
+    return g;
+}
+// Padded tile index -> (face, tx, ty); false for a padding slot.
+__host__ __device__ inline bool spec_tile_at(const SpecTileGeom& g, uint32_t k, uint32_t& face, uint32_t& tx,
+                                             uint32_t& ty) {
+    face = k / g.per_face;
+    const uint32_t kk = k % g.per_face, ks = kk / 256u, kw = kk % 256u;
+    tx = (ks % g.super_x) * 16u + kw % 16u;
+    ty = (ks / g.super_x) * 16u + kw / 16u;
+    return tx < g.tiles_x && ty < g.tiles_y;
+}
+
+// One tile of the sample-split specular loop (four waves, each independent):
+// texels of face `face` of a mip of size Sm whose directions lie on a D x D
+// grid, out_level that mip's texels, tab its n tangent-space samples, s_buf
+// the workgroup's [4][2][kSpecTPW][kSpecRow] summation buffer.
+__device__ __forceinline__ void specular_split_tile(const float4* __restrict__ cube, int S,
+                                                    float4* __restrict__ out_level, uint32_t Sm, uint32_t D,
+                                                    const float4* __restrict__ tab, uint32_t n, uint32_t face,
+                                                    uint32_t tx, uint32_t ty, float* s_buf) {
     const uint32_t wave = threadIdx.x / 64u, lane = threadIdx.x % 64u;
     // evaluating lane: texel t of the wave, sample slot s
     const uint32_t t = lane / kSpecSPR, s = lane % kSpecSPR;
@@ -390,16 +404,13 @@ specular_split_kernel(SpecSplitArgs a) {
                    y = min(ty * kSpecTH + wave * kSpecRPW + t / kSpecTW, Sm - 1);
     const f3 N = texel_dir((int)face, (int)x, (int)y, (int)D);
     const Frame fr = ggx_frame(N);
-    const float4* __restrict__ tab = a.table + (size_t)m * a.n;
-    const int S = (int)a.S;
-    const uint32_t n = a.n;
     // summing lane: texel st of the wave, term sc (R, G, B, NoL)
     const bool summer = lane < 4u * kSpecTPW;
     const uint32_t st = lane / 4u, sc = lane % 4u;
     float acc = 0.0f;
-    float* const wbuf = &s_buf[wave][0][t][4 * s];
-    const float* const rbuf = &s_buf[wave][0][st][sc];
     constexpr uint32_t kHalf = kSpecTPW * kSpecRow;  // floats between the two buffers
+    float* const wbuf = s_buf + wave * 2u * kHalf + t * kSpecRow + 4u * s;
+    const float* const rbuf = s_buf + wave * 2u * kHalf + st * kSpecRow + sc;
     // rounds in pairs (A: even, B: odd); sample i of round r is r * SPR + s,
     // table entries are fetched two rounds ahead and tap loads one round
     // ahead of their use
@@ -415,48 +426,86 @@ specular_split_kernel(SpecSplitArgs a) {
     };
     SpecSample A, B;
     float4 hA = tab_at(0), hB = tab_at(1);
-    spec_begin(A, a.cube, S, fr, N, hA);
+    spec_begin(A, cube, S, fr, N, hA);
     hA = tab_at(2);
     for (uint32_t r = 0; r < rounds; r += 2) {
-        spec_begin(B, a.cube, S, fr, N, hB);
+        spec_begin(B, cube, S, fr, N, hB);
         hB = tab_at(r + 3);
-        sum_round(0, spec_term(A, r * kSpecSPR + s < n, a.cube, S));
-        spec_begin(A, a.cube, S, fr, N, hA);
+        sum_round(0, spec_term(A, r * kSpecSPR + s < n, cube, S));
+        spec_begin(A, cube, S, fr, N, hA);
         hA = tab_at(r + 4);
-        sum_round(1, spec_term(B, (r + 1) * kSpecSPR + s < n, a.cube, S));
+        sum_round(1, spec_term(B, (r + 1) * kSpecSPR + s < n, cube, S));
     }
     const float wsum = __shfl(acc, (int)((lane & ~3u) | 3u));
     const uint32_t xs = tx * kSpecTW + st % kSpecTW, ys = ty * kSpecTH + wave * kSpecRPW + st / kSpecTW;
     if (summer && xs < Sm && ys < Sm) {
-        float* o = reinterpret_cast<float*>(a.out + cube_level_offset_floats(a.So, m) / 4 + (face * Sm + ys) * Sm + xs);
+        float* o = reinterpret_cast<float*>(out_level + (face * Sm + ys) * Sm + xs);
         o[sc] = sc == 3u ? 1.0f : acc / wsum;
     }
+    // the next tile of this workgroup reuses the buffers: keep its first
+    // writes after these reads
+    wave_lds_sync();
+}
+
+#ifndef HVX_SPEC_WAVES_PER_EU
+#define HVX_SPEC_WAVES_PER_EU 4
+#endif
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SPEC_WAVES_PER_EU, 8))) void
+specular_split_kernel(SpecSplitArgs a) {
+    __shared__ __align__(16) float s_buf[4][2][kSpecTPW][kSpecRow];
+    const uint32_t b = blockIdx.x;
+    const uint32_t tile = (b % 8u) * a.per_xcd + b / 8u;
+    if (tile >= a.n_tiles) return;
+    uint32_t m = 1;
+    while (tile >= a.tile_begin[m + 1]) ++m;
+    const uint32_t Sm = max(a.So >> m, 1u);
+    uint32_t face, tx, ty;
+    if (!spec_tile_at(spec_tile_geom(Sm), tile - a.tile_begin[m], face, tx, ty)) return;  // padding
+    specular_split_tile(a.cube, (int)a.S, a.out + cube_level_offset_floats(a.So, m) / 4, Sm, a.emulate ? a.So : Sm,
+                        a.table + (size_t)m * a.n, a.n, face, tx, ty, &s_buf[0][0][0][0]);
 }
 
 // ------------------------------------------------- batched probe prefilter
-// Persistent: workgroups pull (probe, mip, texel range) items from a queue;
-// items are ordered by decreasing cost on the host.
-__global__ __launch_bounds__(256) void probe_batch_kernel(const ProbeJob* __restrict__ jobs,
-                                                          const ProbeItem* __restrict__ items, uint32_t n_items,
-                                                          const float4* __restrict__ tables,
-                                                          uint32_t* __restrict__ queue) {
+// Persistent: the workgroups on XCD x (workgroup b runs on XCD b % 8) pull
+// items from queue x, which serves the contiguous run [xcd_begin[x],
+// xcd_begin[x+1]) of the item list; the host orders items by probe, mip and
+// tile and cuts the runs at equal estimated cost, so each XCD works through
+// neighbouring tiles of a few probes and shares their texels in its L2.
+// Items of mip 0 are texel ranges (lane per texel, as specular_delta_kernel);
+// other items are runs of padded tiles (specular_split_tile).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SPEC_WAVES_PER_EU, 8))) void
+probe_batch_kernel(const ProbeJob* __restrict__ jobs, const ProbeItem* __restrict__ items,
+                   const uint32_t* __restrict__ xcd_begin, const float4* __restrict__ tables,
+                   uint32_t* __restrict__ queues) {
+    __shared__ __align__(16) float s_buf[4][2][kSpecTPW][kSpecRow];
     __shared__ uint32_t s_item;
+    const uint32_t x = blockIdx.x % 8u;
+    const uint32_t begin = xcd_begin[x], end = xcd_begin[x + 1];
     for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(queue, 1u);
+        if (threadIdx.x == 0) s_item = begin + atomicAdd(queues + x, 1u);
         __syncthreads();
         const uint32_t item = s_item;
         __syncthreads();
-        if (item >= n_items) break;
+        if (item >= end) break;
         const ProbeItem it = items[item];
         const ProbeJob& j = jobs[it.probe];
         const uint32_t m = it.mip;
         const uint32_t Sm = max(j.So >> m, 1u);
         const float4* tab = tables + j.table + (size_t)m * j.n;
         float4* out = j.out + cube_level_offset_floats(j.So, m) / 4;
-        for (uint32_t t = threadIdx.x; t < it.count; t += blockDim.x) {
-            const uint32_t local = it.first + t;
-            const uint32_t x = local % Sm, y = (local / Sm) % Sm, face = local / (Sm * Sm);
-            out[local] = specular_texel_value(j.cube, j.S, tab, j.n, m == 0, (int)face, (int)x, (int)y, (int)Sm);
+        if (m == 0) {
+            for (uint32_t t = threadIdx.x; t < it.count; t += blockDim.x) {
+                const uint32_t local = it.first + t;
+                const uint32_t tx = local % Sm, ty = (local / Sm) % Sm, face = local / (Sm * Sm);
+                out[local] = specular_texel_value(j.cube, j.S, tab, j.n, true, (int)face, (int)tx, (int)ty, (int)Sm);
+            }
+        } else {
+            const SpecTileGeom g = spec_tile_geom(Sm);
+            for (uint32_t k = it.first; k < it.first + it.count; ++k) {
+                uint32_t face, tx, ty;
+                if (spec_tile_at(g, k, face, tx, ty))
+                    specular_split_tile(j.cube, (int)j.S, out, Sm, Sm, tab, j.n, face, tx, ty, &s_buf[0][0][0][0]);
+            }
         }
     }
 }
@@ -538,14 +587,7 @@ hipError_t launch_specular(const float* d_cube, uint32_t S, float* d_out, uint32
         uint32_t t = 0;
         for (uint32_t m = 1; m <= (uint32_t)kMaxMips; ++m) {
             a.tile_begin[m] = t;
-            if (m < mips && m < (uint32_t)kMaxMips) {
-                const uint32_t sm = std::max(So >> m, 1u);
-                a.tiles_x[m] = (sm + kSpecTW - 1) / kSpecTW;
-                a.tiles_y[m] = (sm + kSpecTH - 1) / kSpecTH;
-                a.super_x[m] = (a.tiles_x[m] + 15) / 16;
-                a.super_n[m] = a.super_x[m] * ((a.tiles_y[m] + 15) / 16);
-                t += 6u * a.super_n[m] * 256u;
-            }
+            if (m < mips) t += 6u * spec_tile_geom(std::max(So >> m, 1u)).per_face;
         }
         a.n_tiles = t;
         a.per_xcd = (t + 7) / 8;
@@ -559,11 +601,63 @@ hipError_t launch_ggx_tables(uint32_t mips, uint32_t n, float4* table, hipStream
     return hipGetLastError();
 }
 
-hipError_t launch_probe_batch(const ProbeJob* d_jobs, const ProbeItem* d_items, uint32_t n_items,
-                              const float4* d_tables, uint32_t persistent_wgs, uint32_t* d_queue, hipStream_t s) {
-    hipLaunchKernelGGL(probe_batch_kernel, dim3(persistent_wgs), dim3(256), 0, s, d_jobs, d_items, n_items, d_tables,
-                       d_queue);
+hipError_t launch_probe_batch(const ProbeJob* d_jobs, const ProbeItem* d_items, const uint32_t* d_xcd_begin,
+                              const float4* d_tables, uint32_t persistent_wgs, uint32_t* d_queues, hipStream_t s) {
+    const uint32_t grid = std::max((persistent_wgs + 7) / 8, 1u) * 8;  // every XCD gets workgroups
+    hipLaunchKernelGGL(probe_batch_kernel, dim3(grid), dim3(256), 0, s, d_jobs, d_items, d_xcd_begin, d_tables,
+                       d_queues);
     return hipGetLastError();
+}
+
+// Batch work items (probe, mip, range) for the probes of `jobs`, in probe,
+// mip, tile order: mip 0 as ranges of `texels` texels, other mips as runs of
+// padded tiles holding about texels / 16 tiles each; xcd_begin[0..8] cuts the
+// list into eight runs of equal estimated cost.
+void make_probe_items(const ProbeJob* jobs, uint32_t n_probes, uint32_t texels, std::vector<ProbeItem>& items,
+                      uint32_t xcd_begin[9]) {
+    items.clear();
+    std::vector<double> cost;
+    const uint32_t per_item = std::max(texels / (uint32_t)(kSpecTW * kSpecTH), 1u);
+    for (uint32_t p = 0; p < n_probes; ++p) {
+        const ProbeJob& j = jobs[p];
+        for (uint32_t m = 0; m < j.mips; ++m) {
+            const uint32_t Sm = std::max(j.So >> m, 1u);
+            if (m == 0) {
+                const uint32_t T = 6u * Sm * Sm;
+                for (uint32_t f = 0; f < T; f += texels) {
+                    const uint32_t c = std::min(texels, T - f);
+                    items.push_back({p, 0, f, c});
+                    cost.push_back((double)c * (200.0 + 4.0 * j.n));  // one sample and n sums per texel
+                }
+                continue;
+            }
+            const SpecTileGeom g = spec_tile_geom(Sm);
+            const uint32_t slots = 6u * g.per_face;
+            uint32_t k = 0;
+            while (k < slots) {
+                uint32_t first = k, valid = 0;
+                for (; k < slots && valid < per_item; ++k) {
+                    uint32_t f, tx, ty;
+                    if (spec_tile_at(g, k, f, tx, ty)) valid += 1;
+                }
+                if (valid) {
+                    items.push_back({p, m, first, k - first});
+                    cost.push_back((double)valid * kSpecTW * kSpecTH * j.n * 216.0);
+                }
+            }
+        }
+    }
+    double total = 0.0;
+    for (double c : cost) total += c;
+    uint32_t x = 1;
+    double run = 0.0;
+    xcd_begin[0] = 0;
+    for (uint32_t i = 0; i < (uint32_t)items.size() && x < 8; ++i) {
+        run += cost[i];
+        while (x < 8 && run >= total * x / 8.0) xcd_begin[x++] = i + 1;
+    }
+    while (x < 8) xcd_begin[x++] = (uint32_t)items.size();
+    xcd_begin[8] = (uint32_t)items.size();
 }
 
 }  // namespace hvx
