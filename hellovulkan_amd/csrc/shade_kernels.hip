@@ -26,8 +26,9 @@
 //         shared log2/exp2/pow) — bit-identical to the oracle.
 //  fast   (default) the light vector L, half vector H and N.H stay exact, and
 //         so does D's denominator N.H^2 (a^2-1) + 1, because GGX's
-//         D = a^2 / (pi (N.H^2 (a^2-1) + 1)^2) amplifies an error in N.H or in
-//         that sum by up to ~1/a^2 (a^2 ~ 6e-6 at roughness 0.05);
+//         D = a^2 / (pi (N.H^2 (a^2-1) + 1)^2) amplifies a relative error in
+//         N.H by up to 4 (1-a^2) / a^2 (at N.H = 1) and an absolute error in
+//         that sum by up to 2 / a^2 (a^2 ~ 6e-6 at roughness 0.05);
 //         the remaining per-light terms (N.L, H.V, attenuation, D, G,
 //         Fresnel, BRDF combination) use fused multiply-add, hardware
 //         reciprocal and hardware exp2/log2, reading a per-call compact light
@@ -45,6 +46,9 @@
 #endif
 #ifndef HVX_SHADE_WAVES_PER_EU
 #define HVX_SHADE_WAVES_PER_EU 4
+#endif
+#ifndef HVX_SHADE_SKIP  // waves skip lights out of range of all their lanes
+#define HVX_SHADE_SKIP 1
 #endif
 
 namespace hvx {
@@ -91,9 +95,16 @@ struct ShadeArgs {
 };
 
 // Fragments at least this rough take hardware-rsqrt light and half vectors
-// (fast variant, waves whose every lane qualifies): GGX D amplifies an error
-// in N.H by at most ~1/a^2 <= 4.2 there (a = roughness^2 >= 0.49).
+// (fast variant, waves whose every lane qualifies): d ln D / d(N.H) is
+// 4 c (1-a^2) / (1 - c^2 (1-a^2)) at N.H = c, at most 4 (1-a^2) / a^2 <= 12.7
+// there (a = roughness^2 >= 0.49), so an N.H error of a few ulp (<= 5e-7
+// relative) moves D by at most 6.3e-6 relative, within the 1e-5 bound.
 constexpr float kRoughFastH = 0.7f;
+
+// Squared distance from which a light's attenuation max(1 - s / r, 0) is 0
+// for certain: r^2 (1 + 2^-16), so that s, computed from the squared
+// distance dd >= it with a few ulp of error, still satisfies s * (1/r) >= 1.
+constexpr float kOutOfRangeMargin = 1.0f + 1.0f / 65536.0f;
 
 __global__ __launch_bounds__(256) void shade_light_prep_kernel(const hvx_light* __restrict__ lights, uint32_t n,
                                                                float intensity, float4* __restrict__ lrec) {
@@ -101,7 +112,8 @@ __global__ __launch_bounds__(256) void shade_light_prep_kernel(const hvx_light* 
     if (i >= n) return;
     const hvx_light& l = lights[i];
     lrec[2 * i] = make_float4(l.position[0], l.position[1], l.position[2], 1.0f / l.radius);
-    lrec[2 * i + 1] = make_float4(l.color[0] * intensity, l.color[1] * intensity, l.color[2] * intensity, 0.0f);
+    lrec[2 * i + 1] = make_float4(l.color[0] * intensity, l.color[1] * intensity, l.color[2] * intensity,
+                                  (l.radius * l.radius) * kOutOfRangeMargin);
 }
 
 __device__ __forceinline__ f3 sample_spec_lod(const ShadeArgs& a, f3 dir, float lod) {
@@ -261,9 +273,13 @@ __device__ __forceinline__ bool shade_one(const ShadeArgs& a, uint32_t idx, uint
             const f3 d = {A.x - wp.x, A.y - wp.y, A.z - wp.z};
             float s, NoH;
             f3 L, Hh;
+            // A light out of range of every lane (squared distance at least
+            // B.w = r^2 (1 + 2^-16)) has attenuation exactly 0 in each: its
+            // terms add nothing, so the wave skips it.
             if (fastH) {
                 // L and H from the hardware reciprocal square root (a few ulp)
                 const float dd = __builtin_fmaf(d.z, d.z, __builtin_fmaf(d.y, d.y, d.x * d.x));
+                if (HVX_SHADE_SKIP && !__any(dd < B.w)) return;
                 const float r = __builtin_amdgcn_rsqf(dd);
                 s = dd * r;
                 L = mul3(d, r);
@@ -273,7 +289,9 @@ __device__ __forceinline__ bool shade_one(const ShadeArgs& a, uint32_t idx, uint
             } else {
                 // exact: L, H and N.H (max with 0 only differs in the sign of
                 // a zero, which no later term can observe)
-                s = sqrt_rn_normal(dot3(d, d));
+                const float dd = dot3(d, d);
+                if (HVX_SHADE_SKIP && !__any(dd < B.w)) return;
+                s = sqrt_rn_normal(dd);
                 L = mul3(d, rcp_rn_normal(s));
                 const f3 h = add3(V, L);
                 Hh = mul3(h, rcp_rn_normal(sqrt_rn_normal(dot3(h, h))));
