@@ -105,3 +105,53 @@ def test_split_agrees_with_ideal_on_smooth_environment():
     r = O.contract(1, lvl, S, So, mips, 1, ids)
     ideal = O.ideal(1, lvl, S, lv[1]["alpha2"], Sm, ids)
     assert np.abs(r - ideal).max() / np.abs(ideal).max() < 2e-3
+
+
+def test_ideal_weight_is_the_reference_estimand():
+    """The contraction's weight w(c) = D_GGX(h(c)) c (c = n.l, V = R = N) is
+    the integrand the reference's estimator converges to: importance sampling
+    H from D(h)(n.h) and weighting by n.l (CubeFilterSpecular.frag:46-72)
+    estimates the integral of L(l) D(h) n.l over the integral of D(h) n.l.
+    On a smooth environment the FP64 estimator with 16,384 Hammersley samples
+    and the FP64 sum over every level-0 texel (hvxo_ideal_f64, which uses
+    the product's weight) agree within 2e-4 at roughness 0.4, 0.6 and 1.0.
+    Sums with a plausibly wrong weight (D alone, without n.l; or D at
+    n.h^2 = c instead of (1 + c) / 2) miss the estimator by more than 10x
+    that on the rough lobes, where the weight's shape matters."""
+    S, So, mips, n = 128, 64, 6, 16384
+    lvl = env_from(smooth, S)
+    dirs = face_dirs(S)
+    # solid angle of each texel: (2/S)^2 / (1 + u^2 + v^2)^(3/2)
+    c = (2.0 * np.arange(S) + 1.0) / S - 1.0
+    u, v = np.meshgrid(c, c)
+    dw = np.tile(((2.0 / S) ** 2 / (1 + u * u + v * v) ** 1.5).ravel(), 6)
+    rgb = lvl.reshape(-1, 4)[:, :3].astype(np.float64)
+    rng = np.random.default_rng(11)
+    miss = {"no n.l": 0.0, "D at c": 0.0}
+    for m in (2, 3, 5):
+        rough = m / (mips - 1)
+        a2 = (rough * rough) ** 2
+        Sm = So >> m
+        ids = np.unique(rng.integers(0, 6 * Sm * Sm, 12)).astype(np.uint32)
+        est = O.specular(lvl, S, So, mips, m, n, ids, dtype=np.float64)[:, :3]
+        ref = O.ideal(1, lvl, S, a2, Sm, ids)
+        scale = np.abs(ref).max()
+        err = np.abs(est - ref).max() / scale
+        assert err < 2e-4, (m, err)
+        # the same sum in numpy: the weight the product uses, then wrong ones
+        N = face_dirs(Sm)[ids]
+        cl = N @ dirs.T  # n . l per (output, texel)
+
+        def weighted(w):
+            w = np.where(cl > 0, w, 0.0) * dw
+            return (w @ rgb) / w.sum(axis=1, keepdims=True)
+
+        def D_of(nh2):
+            return a2 / (np.pi * (nh2 * (a2 - 1) + 1) ** 2)
+
+        right = weighted(D_of(0.5 * (1 + cl)) * cl)
+        assert np.abs(right - ref).max() <= 1e-9 * scale
+        for name, wrong in (("no n.l", weighted(D_of(0.5 * (1 + cl)))), ("D at c", weighted(D_of(cl) * cl))):
+            miss[name] = max(miss[name], np.abs(est - wrong).max() / scale)
+    for name, e in miss.items():
+        assert e > 2e-3, (name, e)

@@ -424,6 +424,55 @@ def test_shading_closed_form(rough, metal):
     assert np.allclose(got32, got, rtol=2e-6)
 
 
+def test_shading_closed_form_generic_angles():
+    """Radiance.glsl / PBRHeader.glsl at a generic (N, V, L): metal = 1 and
+    albedo = 1 make F0 = 1, so F == 1 and kD == 0 in both the light term and
+    the ambient term, and the light's term reduces to
+    D(N.H) G_L(N.L) G_V(N.V) / (4 N.V N.L + 1e-4) * rad * N.L. The normal and
+    light are placed so that N.H, H.V, N.L and N.V all differ (by more than
+    10 %), so a swap of N.H with H.V in D or of N.L with N.V in G or in the
+    denominator changes the value by far more than the tolerance."""
+    rough = 0.45
+    fr, ivp, cam, ubo, lights, cells, idx, wp = _one_fragment_scene(rough, True, (1.0, 1.0, 1.0), 2.0, 6.0,
+                                                                     (1.0, 0.5, 0.25))
+    n = np.array([-0.6, -0.6, 1.0]); n /= np.linalg.norm(n)
+    fr["nx"][:], fr["ny"][:], fr["nz"][:] = n.astype(F32)
+    n = np.array([fr["nx"][0], fr["ny"][0], fr["nz"][0]], np.float64)
+    ldir = np.array([0.3, 0.4, 0.8]); ldir /= np.linalg.norm(ldir)
+    lpos = (wp + 2.0 * ldir).astype(F32)
+    lights["position"][0, :3] = lpos
+    cd, cs, bx, by = (0.3, 0.2, 0.1), (0.6, 0.5, 0.4), 0.7, 0.05
+    maps = _const_maps(cd, cs, bx, by)
+    pc = abi.pbr_defaults()
+    got = O.shade(fr, 1, 1, ivp, cam, ubo, pc, lights, cells, idx, maps, [0], dtype=np.float64)[0]
+    cp = np.array(cam.position[:3], np.float64)
+    V = (cp - wp) / np.linalg.norm(cp - wp)
+    d = lpos.astype(np.float64) - wp
+    dist = np.linalg.norm(d)
+    L = d / dist
+    Hh = (V + L) / np.linalg.norm(V + L)
+    NoH, NoL, NoV, HoV = n @ Hh, n @ L, n @ V, Hh @ V
+    vals = sorted([NoH, NoL, NoV, HoV])
+    assert min(b - a for a, b in zip(vals, vals[1:])) > 0.1 and vals[0] > 0.1
+    a2 = (rough * rough) ** 2
+    D = a2 / (np.pi * (NoH * NoH * (a2 - 1) + 1) ** 2)
+    k = (rough + 1) ** 2 / 8
+    G = (NoL / (NoL * (1 - k) + k)) * (NoV / (NoV * (1 - k) + k))
+    spec = D * G / (4 * NoV * NoL + 1e-4)
+    att = max(1 - dist / lights["radius"][0], 0) / dist ** pc.light_falloff
+    rad = np.array([1.0, 0.5, 0.25]) * att * pc.light_intensity
+    Lo = 1.0 * pc.albedo_multiplier + spec * rad * NoL
+    amb = np.array(cs) * (bx + by)  # F = 1, kD = 0: only the prefiltered term
+    assert np.allclose(got, amb + Lo, rtol=1e-7), (got, amb + Lo)
+    # the mistakes this pins: each moves the light term by more than 5 %
+    D_swap = a2 / (np.pi * (HoV * HoV * (a2 - 1) + 1) ** 2)
+    spec_swap = D * G / (4 * NoV * NoV + 1e-4)
+    for wrong in (D_swap * G / (4 * NoV * NoL + 1e-4), spec_swap):
+        assert abs(wrong - spec) > 0.05 * spec
+    got32 = O.shade(fr, 1, 1, ivp, cam, ubo, pc, lights, cells, idx, maps, [0])[0]
+    assert np.allclose(got32, got, rtol=2e-6)
+
+
 def test_shading_light_beyond_radius_contributes_nothing():
     fr, ivp, cam, ubo, lights, cells, idx, _ = _one_fragment_scene(0.5, False, (0.5, 0.5, 0.5), 3.0, 2.5, (1, 1, 1))
     maps = _const_maps((0.1, 0.1, 0.1), (0.2, 0.2, 0.2), 0.5, 0.1)
