@@ -215,16 +215,19 @@ def test_batch_rejects_bad_arguments(ctx, scene):
 
 
 CONTRACT_PROBES = [(16, 8, 4), (32, 16, 5), (64, 32, 6), (64, 32, 6), (128, 64, 7), (32, 16, 5)]
+# classes of 6 and 9 probes: the multi-probe GEMMs' second and third probe groups
+CONTRACT_LARGE_CLASSES = [(32, 16, 5)] * 6 + [(64, 32, 6)] * 9
 
 
 @pytest.mark.parametrize("precision", [abi.PRECISION_FP16_SCALED, abi.PRECISION_BF16_SPLIT])
-def test_batch_contraction_mode_matches_per_probe(ctx, precision):
+@pytest.mark.parametrize("probes", [CONTRACT_PROBES, CONTRACT_LARGE_CLASSES], ids=["mixed", "large-classes"])
+def test_batch_contraction_mode_matches_per_probe(ctx, precision, probes):
     """Contraction mode: every probe equals the per-probe contraction filter
     (its plan over the probe's level 0 and box mips) bit for bit on repeated
     eager runs, with the views still culled as per-item; switching back
     restores the sampled prefilter's outputs."""
     import torch
-    sc = Scene(ctx, probes=CONTRACT_PROBES, views=VIEWS[:2])
+    sc = Scene(ctx, probes=probes, views=VIEWS[:2])
     refs = []
     for cube, (S, So, mips) in zip(sc.probe_in, sc.probe_specs):
         levels = abi.mip_count(S)

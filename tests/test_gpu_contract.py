@@ -191,16 +191,20 @@ def test_contract_and_plain_against_exact_integral_512_chain(ctx):
 
 
 @pytest.mark.parametrize("precision", [abi.PRECISION_FP16_SCALED, abi.PRECISION_BF16_SPLIT])
-def test_contract_multi_equals_single(ctx, precision):
-    """One multi-probe call over three probes of the same sizes equals three
-    single-probe calls bit for bit; more probes than reserved is refused."""
+@pytest.mark.parametrize("nprobe", [3, 4, 9])
+def test_contract_multi_equals_single(ctx, precision, nprobe):
+    """One multi-probe call over n probes of the same sizes equals n
+    single-probe calls bit for bit; more probes than reserved is refused.
+    The grouped GEMMs take probes in groups of four: n = 3 is one partial
+    group, 4 one full group, 9 two full groups and a tail (group indices past
+    the first)."""
     import torch
     S, So, mips = 128, 64, 7
     levels = abi.mip_count(S)
     rng = np.random.default_rng(41)
     plan = abi.ContractPlan(ctx, abi.FILTER_SPECULAR, S, So, mips, precision)
     chains, singles, multis = [], [], []
-    for k in range(3):
+    for k in range(nprobe):
         c = rng.uniform(0.0, 1.0 + 5 * k, 6 * S * S * 4).astype(np.float32)
         chain = torch.empty(abi.cube_chain_floats(S, levels), dtype=torch.float32, device="cuda")
         chain[:c.size].copy_(to_dev(c))
@@ -213,10 +217,10 @@ def test_contract_multi_equals_single(ctx, precision):
     d_in = torch.tensor([c.data_ptr() for c in chains], dtype=torch.int64, device="cuda")
     d_out = torch.tensor([o.data_ptr() for o in multis], dtype=torch.int64, device="cuda")
     with pytest.raises(abi.HvxError):
-        plan.filter_multi(3, d_in, levels, d_out)   # nothing reserved yet
-    plan.reserve(3)
-    plan.filter_multi(3, d_in, levels, d_out)
+        plan.filter_multi(nprobe, d_in, levels, d_out)   # nothing reserved yet
+    plan.reserve(nprobe)
+    plan.filter_multi(nprobe, d_in, levels, d_out)
     torch.cuda.synchronize()
-    for k in range(3):
+    for k in range(nprobe):
         assert torch.equal(multis[k].view(torch.int32), singles[k].view(torch.int32)), f"probe {k}"
     plan.close()
