@@ -35,7 +35,6 @@ PEAK_HBM_GBS = 8000.0      # MI355X HBM3E peak (datasheet)
 # fragment set-up + ambient, 140 per listed light; 45 bytes of fragment data
 # in and out (depth 4, normal 12, albedo 12, roughness 4, metallic 1, RGB 12)
 OPS_FRAG, OPS_PAIR, BYTES_FRAG = 160, 140, 45
-OPS_TEST = 16              # FP32 ops per cluster-light test
 
 
 def parse():
@@ -219,22 +218,24 @@ def main():
         "shade_ms": round(shade_ms, 4),
         "aabb_ms": round(aabb_ms, 4),
         "cluster_light_tests_per_s": round(tests / (cull_ms * 1e-3), 1),
-        "cull_valu_frac": round(tests * OPS_TEST / (cull_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
         "list_entries": int(d_tot.item()),
         "ibl_stage_ms": {k: round(v, 4) for k, v in ibl.items()},
         "roofline": roofline,
     }
 
+    contract_out = {}
     if not args.no_contract and not args.no_ibl:
         result["ibl_contract"] = contract_bench(ctx, stream, d_cube, CUBE, SPEC, SPEC_MIPS, DIFF,
-                                                ibl["specular_plain_fp32_ms"], ibl["diffuse_plain_fp32_ms"])
+                                                ibl["specular_plain_fp32_ms"], ibl["diffuse_plain_fp32_ms"],
+                                                contract_out)
     if not args.no_learned:
         result["learned"] = learned_bench(ctx, stream, args.mlp_steps, d_spec, SPEC, SPEC_MIPS, frs, cam, ubo, pc,
                                           d_lights, NL, d_cells, d_idx, maps, F, shade_ms)
     if not args.no_batch:
         result["batch"] = batch_bench(ctx, dev, stream, args.batch_probes)
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(args, ubo, cam, desc, ivp, lights, fr, d_spec, d_diff, d_lut, F, pc)
+        result["cpu_baseline"] = cpu_baseline(args, ubo, cam, desc, ivp, lights, fr, d_cube, CUBE, d_spec, d_diff,
+                                              d_lut, F, pc, contract_out)
     print(json.dumps(result), flush=True)
     ctx.close()
 
@@ -252,10 +253,10 @@ def _ev_ms(fn, stream, reps):
     return e0.elapsed_time(e1) / reps
 
 
-def contract_bench(ctx, stream, d_cube, S, So, mips, Sd, plain_spec_ms, plain_diff_ms):
+def contract_bench(ctx, stream, d_cube, S, So, mips, Sd, plain_spec_ms, plain_diff_ms, outputs):
     """The dense-contraction (FP16 MFMA) filters on the same 6x1024^2 cube as
     the plain filters: plan (weight) construction once, then per-call time.
-    Their error against the FP64 twin is checked by tests/test_gpu_contract.py."""
+    Their outputs are kept in `outputs` for the error report of the CPU leg."""
     import torch
     from hellovulkan_amd import abi
     levels = abi.mip_count(S)
@@ -270,6 +271,7 @@ def contract_bench(ctx, stream, d_cube, S, So, mips, Sd, plain_spec_ms, plain_di
         build_s = time.time() - t0
         dst = torch.empty(abi.cube_chain_floats(so, m), dtype=torch.float32, device=d_cube.device)
         ms = _ev_ms(lambda: plan.filter(d_cube, levels, dst), stream, 10)
+        outputs[name] = dst
         info = [plan.info(k) for k in range(m)]
         wbytes = sum(i.weight_bytes for i in info)
         out[name] = {"ms": round(ms, 4), "plain_fp32_ms": round(plain, 4), "speedup": round(plain / ms, 2),
@@ -489,41 +491,198 @@ def batch_bench(ctx, dev, stream, n_probes):
     }
 
 
-def cpu_baseline(args, ubo, cam, desc, ivp, lights, fr, d_spec, d_diff, d_lut, F, pc):
-    """The FP32 oracle on the host cores: full light culling of the same frame,
-    then shading of a fixed deterministic fragment sample; the frame rate is
-    F / (cull time + per-fragment shading time * F)."""
+def host_info():
+    """The host the CPU leg runs on: logical CPUs, those this process may use, model."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"nproc": os.cpu_count(), "usable_cpus": usable, "model": model}
+
+
+def _median_runs(fn, runs):
+    ts = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts)), ts
+
+
+def cpu_baseline(args, ubo, cam, desc, ivp, lights, fr, d_cube, CUBE, d_spec, d_diff, d_lut, F, pc, contract_out):
+    """The FP32 oracle (-O2, -ffp-contract=off) on the host cores, and the
+    per-mip errors of the GPU filters against the FP64 oracle.
+
+    value: the 3840x2160 frame (full light culling + shading of a fixed
+    seeded fragment sample, scaled to every fragment), median of 5 runs on
+    all usable cores (at most 16). Also single-thread, the 2560x1600 /
+    1,024-light culling pass and the 512^2 x 1024 BRDF LUT (a row sample,
+    scaled), and per-filter times of the 1024^2 -> 512^2 x 10 / 32^2 filters
+    (texel samples, scaled). Every scaled figure says what it was scaled from."""
     from oracle import hvx_oracle as O
-    threads = max(1, min(16, os.cpu_count() or 1))
-    t = time.perf_counter()
+    from tests.helpers import SMALL_FRAME, camera
+    host = host_info()
+    threads = max(1, min(16, host["usable_cpus"]))
+    rng = np.random.default_rng(5)
     aabbs = O.build_aabbs(ubo)
-    cells, idx, _, _ = O.cull_lights(ubo, aabbs, lights, 128, threads=threads)
-    t_cull = time.perf_counter() - t
     maps = dict(spec=d_spec.cpu().numpy(), spec_size=512, spec_mips=10, diff=d_diff.cpu().numpy(), diff_size=32,
                 lut=d_lut.cpu().numpy(), lut_w=512, lut_h=512)
-    rng = np.random.default_rng(5)
-    n = 20000
-    ids = np.sort(rng.choice(F, n, replace=False)).astype(np.uint32)
+    out = {"host": host}
+
+    # ---- the 3840x2160 frame: culling + shading sample, all cores, median of 5
+    culled = {}
+
+    def cull_all():
+        culled["r"] = O.cull_lights(ubo, aabbs, lights, 128, threads=threads)
+
+    t_cull, cull_runs = _median_runs(cull_all, 5)
+    cells, idx, _, _ = culled["r"]
+    probe_ids = np.sort(rng.choice(F, 1000, replace=False)).astype(np.uint32)
     t = time.perf_counter()
-    O.shade(fr, desc.width, desc.height, ivp, cam, ubo, pc, lights, cells, idx, maps, ids[:2000], threads=threads)
-    per = (time.perf_counter() - t) / 2000
-    n = int(min(F, max(2000, (args.cpu_seconds - t_cull) / max(per, 1e-9))))
+    O.shade(fr, desc.width, desc.height, ivp, cam, ubo, pc, lights, cells, idx, maps, probe_ids, threads=threads)
+    per = (time.perf_counter() - t) / probe_ids.size
+    n = int(min(F, max(2000, (args.cpu_seconds / 5.0) / max(per, 1e-9))))
     ids = np.sort(rng.choice(F, n, replace=False)).astype(np.uint32)
-    t = time.perf_counter()
-    O.shade(fr, desc.width, desc.height, ivp, cam, ubo, pc, lights, cells, idx, maps, ids, threads=threads)
-    t_shade = time.perf_counter() - t
+    t_shade, _ = _median_runs(lambda: O.shade(fr, desc.width, desc.height, ivp, cam, ubo, pc, lights, cells, idx,
+                                              maps, ids, threads=threads), 5)
     frame_s = t_cull + t_shade / n * F
-    return {
+    # single thread: one culling pass and a 300-fragment shading sample
+    t = time.perf_counter()
+    O.cull_lights(ubo, aabbs, lights, 128, threads=1)
+    t_cull1 = time.perf_counter() - t
+    ids1 = ids[:300]
+    t = time.perf_counter()
+    O.shade(fr, desc.width, desc.height, ivp, cam, ubo, pc, lights, cells, idx, maps, ids1, threads=1)
+    t_shade1 = time.perf_counter() - t
+    frame1_s = t_cull1 + t_shade1 / ids1.size * F
+    out.update({
         "value": round(F / frame_s / 1e6, 4),
         "unit": "Mfrag/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"FP32 oracle (-O2 -ffp-contract=off), {threads} threads: full culling of the frame "
-                  f"({t_cull:.2f} s) + shading of {n} seeded random fragments ({t_shade:.2f} s), frame time = "
+        "sample": f"FP32 oracle (-O2 -ffp-contract=off) on {threads} threads of a {host['usable_cpus']}-CPU share "
+                  f"({host['model']}): median of 5 runs of the full culling of the 3840x2160 frame "
+                  f"({t_cull:.2f} s) and of shading {n} seeded random fragments ({t_shade:.2f} s); frame time = "
                   f"cull + shading time scaled to {F} fragments",
         "cull_s": round(t_cull, 3),
+        "cull_runs_s": [round(x, 3) for x in cull_runs],
         "cluster_light_tests_per_s": round(27648 * 65536 / t_cull, 1),
+        "single_thread": {"value": round(F / frame1_s / 1e6, 4), "unit": "Mfrag/s", "cull_s": round(t_cull1, 3),
+                          "sample": f"one run: full culling + shading of {ids1.size} fragments, scaled"},
+    })
+
+    # ---- the 2560x1600 / 1,024-light culling pass and the 512^2 BRDF LUT
+    _, _, ubo0, _ = camera(SMALL_FRAME["width"], SMALL_FRAME["height"], SMALL_FRAME["grid"])
+    lights0 = abi_scene_lights(11, SMALL_FRAME["n_lights"])
+    aabbs0 = O.build_aabbs(ubo0)
+    C0 = SMALL_FRAME["grid"][0] * SMALL_FRAME["grid"][1] * SMALL_FRAME["grid"][2]
+    c0_all, _ = _median_runs(lambda: O.cull_lights(ubo0, aabbs0, lights0, 128, threads=threads), 5)
+    c0_one, _ = _median_runs(lambda: O.cull_lights(ubo0, aabbs0, lights0, 128, threads=1), 5)
+    rows = 8
+    lut_ids = np.arange(rows * 512, dtype=np.uint32) + 252 * 512  # 8 middle rows
+    lut_all, _ = _median_runs(lambda: O.brdf_lut(512, 512, 1024, lut_ids, threads=threads), 3)
+    lut_one, _ = _median_runs(lambda: O.brdf_lut(512, 512, 1024, lut_ids, threads=1), 3)
+    out["small_frame"] = {
+        "workload": "2560x1600, 16x9x24 clusters, 1024 lights, cap 128; BRDF LUT 512^2 x 1024 samples",
+        "cull_ms": round(c0_all * 1e3, 3), "cull_ms_1thread": round(c0_one * 1e3, 3),
+        "cluster_light_tests_per_s": round(C0 * SMALL_FRAME["n_lights"] / c0_all, 1),
+        "cluster_light_tests_per_s_1thread": round(C0 * SMALL_FRAME["n_lights"] / c0_one, 1),
+        "brdf_lut_ms": round(lut_all * 512 / rows * 1e3, 1), "brdf_lut_ms_1thread": round(lut_one * 512 / rows * 1e3, 1),
+        "sample": f"culling: median of 5 full passes; LUT: median of 3 runs of {rows} of 512 rows, scaled",
     }
+
+    # ---- the 1024^2 -> 512^2 x 10 / 32^2 filters: CPU time and per-mip error of the GPU outputs
+    cube0 = d_cube[:6 * CUBE * CUBE * 4].cpu().numpy()
+    spec = maps["spec"]
+    SPEC, MIPS, NS = 512, 10, 1024
+    per_mip, spec_ms_all, spec_ms_one = [], 0.0, 0.0
+    outs = {k: v.cpu().numpy() for k, v in contract_out.items()}
+    lv = O.contract_levels(1, CUBE, SPEC, MIPS)
+    for m in range(MIPS):
+        Sm = max(SPEC >> m, 1)
+        T = 6 * Sm * Sm
+        ids = np.unique(rng.integers(0, T, min(T, 64))).astype(np.uint32)
+        t = time.perf_counter()
+        ref32 = O.specular(cube0, CUBE, SPEC, MIPS, m, NS, ids, threads=threads)
+        spec_ms_all += (time.perf_counter() - t) / ids.size * T * 1e3
+        t = time.perf_counter()
+        O.specular(cube0, CUBE, SPEC, MIPS, m, NS, ids[:16], threads=1)
+        spec_ms_one += (time.perf_counter() - t) / min(16, ids.size) * T * 1e3
+        ref64 = O.specular(cube0, CUBE, SPEC, MIPS, m, NS, ids, dtype=np.float64, threads=threads)[:, :3]
+        off = abi_cube_level_offset(SPEC, m)
+        plain = spec[off: off + 4 * T].reshape(-1, 4)[ids, :3].astype(np.float64)
+        row = {"mip": m, "texels_checked": int(ids.size),
+               "plain_vs_fp32_oracle_bitexact": bool(np.array_equal(
+                   spec[off: off + 4 * T].reshape(-1, 4)[ids].view(np.uint32), ref32.view(np.uint32))),
+               "plain_fp32_vs_fp64_oracle": _max_rel(plain, ref64)}
+        if "specular" in outs:
+            twin = O.contract(1, cube0, CUBE, SPEC, MIPS, m, ids, threads=threads)
+            exact_ids = ids[:8]
+            ideal = O.ideal(1, cube0, CUBE, lv[m]["alpha2"], Sm, exact_ids, threads=threads) if m > 0 else None
+            for key, name in (("specular", "fp16"), ("specular_split_bf16", "split_bf16")):
+                if key not in outs:
+                    continue
+                got = outs[key][off: off + 4 * T].reshape(-1, 4)[ids, :3].astype(np.float64)
+                row[f"contract_{name}_vs_fp64_twin"] = _max_rel(got, twin)
+                if ideal is not None:
+                    row[f"contract_{name}_vs_exact_integral"] = _max_rel(got[:exact_ids.size], ideal)
+            if ideal is not None:
+                row["plain_fp32_vs_exact_integral"] = _max_rel(plain[:exact_ids.size], ideal)
+        per_mip.append(row)
+    dids = np.unique(rng.integers(0, 6 * 32 * 32, 48)).astype(np.uint32)
+    t = time.perf_counter()
+    dref32 = O.diffuse(cube0, CUBE, 32, dids, threads=threads)
+    diff_ms_all = (time.perf_counter() - t) / dids.size * 6 * 32 * 32 * 1e3
+    t = time.perf_counter()
+    O.diffuse(cube0, CUBE, 32, dids[:4], threads=1)
+    diff_ms_one = (time.perf_counter() - t) / 4 * 6 * 32 * 32 * 1e3
+    dref64 = O.diffuse(cube0, CUBE, 32, dids, dtype=np.float64, threads=threads)[:, :3]
+    dplain = maps["diff"].reshape(-1, 4)[dids]
+    diffuse_err = {"texels_checked": int(dids.size),
+                   "plain_vs_fp32_oracle_bitexact": bool(np.array_equal(dplain.view(np.uint32),
+                                                                        dref32.view(np.uint32))),
+                   "plain_fp32_vs_fp64_oracle": _max_rel(dplain[:, :3].astype(np.float64), dref64)}
+    if "diffuse" in outs:
+        got = outs["diffuse"].reshape(-1, 4)[dids, :3].astype(np.float64)
+        diffuse_err["contract_fp16_vs_fp64_twin"] = _max_rel(got, O.contract(0, cube0, CUBE, 32, 1, 0, dids,
+                                                                             threads=threads))
+        diffuse_err["contract_fp16_vs_exact_integral"] = _max_rel(got[:8], O.ideal(0, cube0, CUBE, 0.0, 32, dids[:8],
+                                                                                     threads=threads))
+    out["ibl_filters"] = {
+        "workload": "6x1024^2 cube -> 10-mip GGX chain from 512^2 (1024 samples) and 6x32^2 irradiance",
+        "specular_ms": round(spec_ms_all, 1), "specular_ms_1thread": round(spec_ms_one, 1),
+        "diffuse_ms": round(diff_ms_all, 1), "diffuse_ms_1thread": round(diff_ms_one, 1),
+        "sample": "up to 64 texels per specular mip and 48 irradiance texels (16 and 4 on one thread), scaled to "
+                  "every texel; errors: max over the checked texels and channels of |x - ref| / |ref|, exact "
+                  "integral (sum over every level-0 texel, FP64) on the first 8",
+        "specular_per_mip_err": per_mip,
+        "diffuse_err": diffuse_err,
+    }
+    return out
+
+
+def _max_rel(got, ref):
+    ref = np.asarray(ref, np.float64)
+    return float(np.max(np.abs(np.asarray(got, np.float64) - ref) / np.maximum(np.abs(ref), 1e-30)))
+
+
+def abi_scene_lights(seed, n):
+    from hellovulkan_amd import abi
+    return abi.scene_lights(seed, n)
+
+
+def abi_cube_level_offset(S, m):
+    from hellovulkan_amd import abi
+    return abi.cube_level_offset_floats(S, m)
 
 
 if __name__ == "__main__":
