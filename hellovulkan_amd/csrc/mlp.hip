@@ -387,9 +387,10 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(MlpView v, const float4*
 }
 
 // Fallback mask cells: (lod bin b, face f, texel y, x) of a mask_face cube.
-// Points of a cell: the texel's centre and four corners at the bin's two
-// ends and middle (15 points).
-constexpr uint32_t kMaskPoints = 15;
+// Points of a cell: a kMaskGrid x kMaskGrid grid over the texel (corners,
+// edge and inner points) at the bin's two ends and middle.
+constexpr uint32_t kMaskGrid = 5;
+constexpr uint32_t kMaskPoints = kMaskGrid * kMaskGrid * 3;
 
 __device__ __forceinline__ f3 plane_dir_f(int f, double u, double v) {
     double p[3];
@@ -404,12 +405,8 @@ __global__ __launch_bounds__(256) void mask_points_kernel(uint32_t mf, uint32_t 
     if (id >= cells * kMaskPoints) return;
     const uint32_t cell = id / kMaskPoints, p = id % kMaskPoints;
     const uint32_t x = cell % mf, y = (cell / mf) % mf, f = (cell / (mf * mf)) % 6, b = cell / (6 * mf * mf);
-    const uint32_t corner = p % 5, li = p / 5;
-    double cx = x + 0.5, cy = y + 0.5;
-    if (corner) {
-        cx = x + ((corner & 1) ? 1.0 : 0.0);
-        cy = y + ((corner & 2) ? 1.0 : 0.0);
-    }
+    const uint32_t gp = p % (kMaskGrid * kMaskGrid), li = p / (kMaskGrid * kMaskGrid);
+    const double cx = x + (double)(gp % kMaskGrid) / (kMaskGrid - 1), cy = y + (double)(gp / kMaskGrid) / (kMaskGrid - 1);
     const f3 d = plane_dir_f((int)f, 2.0 * cx / mf - 1.0, 2.0 * cy / mf - 1.0);
     const float lod = lod_max * ((float)b + 0.5f * (float)li) / (float)bins;
     pts[id] = make_float4(d.x, d.y, d.z, lod);
@@ -437,6 +434,16 @@ __global__ __launch_bounds__(256) void cell_error_kernel(const float4* __restric
         }
     }
     err[cell] = e;
+}
+
+// The error the fallback decides on: the largest measured error of the cell
+// times kCellSafety, a margin for the error between the test points, which
+// is not measured.
+constexpr float kCellSafety = 2.0f;
+__global__ __launch_bounds__(256) void cell_margin_kernel(const float* __restrict__ raw, uint32_t cells,
+                                                          float* __restrict__ err) {
+    const uint32_t cell = blockIdx.x * 256 + threadIdx.x;
+    if (cell < cells) err[cell] = raw[cell] * kCellSafety;
 }
 
 }  // namespace
@@ -798,13 +805,16 @@ int hvx_mlp_cell_errors(hvx_mlp* M, const float* d_chain, uint32_t face, uint32_
     const uint32_t cells = lod_bins * 6 * mask_face * mask_face;
     const size_t npts = (size_t)cells * kMaskPoints;
     if (npts > (1u << 28)) return mfail(M->ctx, HVX_ERR_INVALID_ARG, "hvx_mlp_cell_errors: too many cells");
-    // set-up call: the point and prediction buffers are allocated here
+    // set-up call: the point, prediction and raw-error buffers are allocated here
     float4* pts = nullptr;
     float* rgb = nullptr;
+    float* raw = nullptr;
     hipError_t e;
     if ((e = hipMalloc(&pts, npts * sizeof(float4))) != hipSuccess ||
-        (e = hipMalloc(&rgb, npts * 3 * sizeof(float))) != hipSuccess) {
+        (e = hipMalloc(&rgb, npts * 3 * sizeof(float))) != hipSuccess ||
+        (e = hipMalloc(&raw, (size_t)cells * sizeof(float))) != hipSuccess) {
         if (pts) (void)hipFree(pts);
+        if (rgb) (void)hipFree(rgb);
         return mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_cell_errors: %s", hipGetErrorString(e));
     }
     hipStream_t s = (hipStream_t)stream;
@@ -813,10 +823,12 @@ int hvx_mlp_cell_errors(hvx_mlp* M, const float* d_chain, uint32_t face, uint32_
     hipLaunchKernelGGL(mlp_infer_kernel, dim3(infer_blocks(M, npts)), dim3(256), 0, s,
                        hvx::mlp_view(M), pts, (uint32_t)npts, rgb);
     hipLaunchKernelGGL(cell_error_kernel, dim3((cells + 255) / 256), dim3(256), 0, s, pts, rgb,
-                       reinterpret_cast<const float4*>(d_chain), face, mips, cells, d_err);
+                       reinterpret_cast<const float4*>(d_chain), face, mips, cells, raw);
+    hipLaunchKernelGGL(cell_margin_kernel, dim3((cells + 255) / 256), dim3(256), 0, s, raw, cells, d_err);
     e = hipStreamSynchronize(s);
     (void)hipFree(pts);
     (void)hipFree(rgb);
+    (void)hipFree(raw);
     return e == hipSuccess ? HVX_OK : mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_cell_errors: %s", hipGetErrorString(e));
 }
 

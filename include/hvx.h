@@ -418,13 +418,15 @@ int hvx_mlp_train(hvx_mlp* mlp, const float* d_inputs, const float* d_targets, u
 /* Radiance (RGB, n x 3 floats) predicted at n inputs (dir.xyz, lod), by the
  * inference path the shading kernel uses. */
 int hvx_mlp_eval(hvx_mlp* mlp, const float* d_inputs, uint32_t n, float* d_rgb, void* stream);
-/* Measured error of the network for hvx_cf_shade_learned: for each cell
- * (bin b, face f, texel y, x) of a cell_face cube and lod_bins bins over
- * [0, lod_max], d_err[(b * 6 + f) * cell_face^2 + y * cell_face + x] receives
- * the largest relative error (every channel) of the network against the
- * chain's textureLod value at the cell's 15 test points (texel centre and
- * corners, at both bin ends and its middle). A set-up call: it allocates
- * and synchronises the stream. */
+/* Error of the network for hvx_cf_shade_learned: for each cell (bin b,
+ * face f, texel y, x) of a cell_face cube and lod_bins bins over
+ * [0, lod_max], d_err[(b * 6 + f) * cell_face^2 + y * cell_face + x]
+ * receives twice the largest relative error (every channel) of the
+ * network against the chain's textureLod value at the cell's 75 test points
+ * (a 5 x 5 grid over the texel, corners included, at both bin ends and its
+ * middle): a margin over the measured error, since the error between test
+ * points is not measured. A set-up call: it allocates and synchronises the
+ * stream. */
 int hvx_mlp_cell_errors(hvx_mlp* mlp, const float* d_chain, uint32_t face, uint32_t mips, uint32_t cell_face,
                         uint32_t lod_bins, float* d_err, void* stream);
 /* hvx_cf_shade with the prefiltered specular radiance taken from the network

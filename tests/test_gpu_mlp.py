@@ -195,8 +195,9 @@ def test_trained_network_in_shading(ctx):
     """A network trained on a chain of the plain filter, its measured cell
     errors, then the learned frame: the fragments it serves stay within the
     2e-3 radiance bound of plain shading (the cell error is measured at test
-    points, so a small fraction may exceed it; reported and bounded), the
-    others are plain shading bit for bit."""
+    points with a margin, not bounded between them, so a tiny fraction could
+    exceed it: reported and limited to 1e-4), the others are plain shading
+    bit for bit."""
     import torch
     S, So, mips = 64, 32, 6
     eq = abi.scene_equirect(21, 2 * S, S)
@@ -210,7 +211,7 @@ def test_trained_network_in_shading(ctx):
     x = torch.empty(n * 4, dtype=torch.float32, device="cuda")
     t = torch.empty(n * 4, dtype=torch.float32, device="cuda")
     mlp.sample(spec, So, mips, 7, n, x, t)
-    mlp.train(x, t, n, 8192, 1500, 2e-3)
+    mlp.train(x, t, n, 8192, 4000, 2e-3)
     cf, bins = 32, 4
     err = torch.empty(bins * 6 * cf * cf, dtype=torch.float32, device="cuda")
     mlp.cell_errors(spec, So, mips, cf, bins, err)
@@ -224,7 +225,7 @@ def test_trained_network_in_shading(ctx):
     assert served > 0
     changed = int(np.count_nonzero(rel))
     assert changed <= served
-    assert np.mean(rel > 2e-3) <= 1e-3
+    assert np.mean(rel > 2e-3) <= 1e-4
     mlp.close()
 
 

@@ -266,5 +266,8 @@ def test_learned_shading_3840x2160_unserved_fragments_unchanged(ctx):
         assert 0 < n_served < F
         assert n_changed <= n_served  # nothing the network did not serve changed
         assert n_changed >= 0.9 * n_served
+        # the margin on the measured cell errors keeps every served fragment
+        # within the bound on this frame
+        assert int((rel > tol).sum().item()) == 0
     finally:
         mlp.close()
