@@ -37,6 +37,7 @@ CULL_BATCH_REFERENCE_CAP = 1
 CULL_BATCH_REFERENCE_TAIL = 2
 
 OPT_SHADE_EXACT = 1
+OPT_SHADE_KERNEL_EVENTS = 2
 
 FILTER_DIFFUSE = 0
 FILTER_SPECULAR = 1
@@ -144,6 +145,7 @@ SIGNATURES = [
     ("hvx_ctx_last_error", C.c_char_p, [vp]),
     ("hvx_ctx_set_option", C.c_int, [vp, C.c_int, C.c_int64]),
     ("hvx_ctx_get_option", C.c_int, [vp, C.c_int, P(C.c_int64)]),
+    ("hvx_ctx_kernel_times", C.c_int, [vp, P(C.c_float), u32, P(u32)]),
     ("hvx_ctx_scratch_generation", C.c_uint64, [vp]),
     ("hvx_cf_build_aabbs", C.c_int, [vp, P(CFUBO), vp, vp]),
     ("hvx_cf_cull_lights", C.c_int, [vp, P(CFUBO), vp, vp, u32, u32, vp, vp, C.c_size_t, vp, vp, vp]),
@@ -345,6 +347,14 @@ class Context:
 
     def set_option(self, option: int, value: int):
         self._check(self.lib.hvx_ctx_set_option(self.h, option, value), "hvx_ctx_set_option")
+
+    def kernel_times(self, max_n: int = 256) -> list:
+        """Durations (ms) of the shading-kernel launches recorded since the
+        last call (HVX_OPT_SHADE_KERNEL_EVENTS)."""
+        buf = (C.c_float * max(1, max_n))()
+        n = C.c_uint32()
+        self._check(self.lib.hvx_ctx_kernel_times(self.h, buf, max_n, C.byref(n)), "hvx_ctx_kernel_times")
+        return [float(buf[k]) for k in range(n.value)]
 
     def get_option(self, option: int) -> int:
         v = C.c_int64()

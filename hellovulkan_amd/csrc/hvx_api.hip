@@ -177,6 +177,27 @@ int hvx_ctx_set_option(hvx_ctx* ctx, int option, int64_t value) {
             if (value != 0 && value != 1) return fail(ctx, HVX_ERR_INVALID_ARG, "HVX_OPT_SHADE_EXACT is 0 or 1");
             ctx->shade_exact = (int)value;
             return HVX_OK;
+        case HVX_OPT_SHADE_KERNEL_EVENTS: {
+            if (value < 0 || value > 256)
+                return fail(ctx, HVX_ERR_INVALID_ARG, "HVX_OPT_SHADE_KERNEL_EVENTS is in [0, 256]");
+            int st;
+            if ((st = bind_device(ctx))) return st;
+            (void)hipDeviceSynchronize();  // the old events may be pending
+            for (hipEvent_t e : ctx->kev) (void)hipEventDestroy(e);
+            ctx->kev.clear();
+            ctx->kev_next = ctx->kev_count = 0;
+            for (int64_t k = 0; k < 2 * value; ++k) {
+                hipEvent_t e = nullptr;
+                const hipError_t err = hipEventCreate(&e);
+                if (err != hipSuccess) {
+                    for (hipEvent_t x : ctx->kev) (void)hipEventDestroy(x);
+                    ctx->kev.clear();
+                    return hip_fail(ctx, err, "HVX_OPT_SHADE_KERNEL_EVENTS");
+                }
+                ctx->kev.push_back(e);
+            }
+            return HVX_OK;
+        }
         default: return fail(ctx, HVX_ERR_INVALID_ARG, "unknown option %d", option);
     }
 }
@@ -185,8 +206,27 @@ int hvx_ctx_get_option(const hvx_ctx* ctx, int option, int64_t* value) {
     if (!ctx || !value) return HVX_ERR_NULL;
     switch (option) {
         case HVX_OPT_SHADE_EXACT: *value = ctx->shade_exact; return HVX_OK;
+        case HVX_OPT_SHADE_KERNEL_EVENTS: *value = (int64_t)(ctx->kev.size() / 2); return HVX_OK;
         default: return HVX_ERR_INVALID_ARG;
     }
+}
+
+int hvx_ctx_kernel_times(hvx_ctx* ctx, float* ms_out, uint32_t max, uint32_t* n_out) {
+    if (!ctx || !n_out || (max && !ms_out)) return HVX_ERR_NULL;
+    *n_out = 0;
+    const uint32_t pairs = (uint32_t)(ctx->kev.size() / 2);
+    if (pairs == 0) return fail(ctx, HVX_ERR_INVALID_ARG, "HVX_OPT_SHADE_KERNEL_EVENTS is off");
+    const uint32_t n = std::min(ctx->kev_count, pairs), first = (ctx->kev_next + pairs - n) % pairs;
+    const uint32_t out = std::min(n, max);
+    for (uint32_t k = 0; k < out; ++k) {
+        const uint32_t slot = (first + (n - out) + k) % pairs;
+        hipError_t e = hipEventSynchronize(ctx->kev[2 * slot + 1]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms_out[k], ctx->kev[2 * slot], ctx->kev[2 * slot + 1]);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hvx_ctx_kernel_times");
+    }
+    ctx->kev_count = 0;
+    *n_out = out;
+    return HVX_OK;
 }
 
 int hvx_ctx_create(int device, hvx_ctx** out) {
@@ -220,6 +260,7 @@ int hvx_ctx_destroy(hvx_ctx* ctx) {
                         ctx->ggx_table,    ctx->shade_lights, ctx->shade_bins};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
+        for (hipEvent_t e : ctx->kev) (void)hipEventDestroy(e);
     }
     delete ctx;
     return HVX_OK;
@@ -419,9 +460,18 @@ int shade_impl(hvx_ctx* ctx, const hvx_frag_soa* fr, const hvx_camera_ubo* cam, 
                                            hvx::shade_bin_scratch_bytes(*fr, *ubo), (hipStream_t)stream)) !=
                       hipSuccess)
         return scratch_fail(ctx, e, "shading bin allocation");
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (!ctx->kev.empty()) {
+        const uint32_t pairs = (uint32_t)(ctx->kev.size() / 2), slot = ctx->kev_next;
+        ev0 = ctx->kev[2 * slot];
+        ev1 = ctx->kev[2 * slot + 1];
+        ctx->kev_next = (slot + 1) % pairs;
+        ctx->kev_count = std::min(ctx->kev_count + 1, pairs);
+    }
     e = hvx::launch_shade(*fr, *cam, *ubo, *pc, d_lights, n_lights, d_cells, d_indices, *ibl, d_rgb,
                           ctx->shade_exact != 0, static_cast<float4*>(ctx->shade_lights),
-                          binned ? static_cast<uint32_t*>(ctx->shade_bins) : nullptr, learned, (hipStream_t)stream);
+                          binned ? static_cast<uint32_t*>(ctx->shade_bins) : nullptr, learned, (hipStream_t)stream,
+                          ev0, ev1);
     return e == hipSuccess ? HVX_OK : hip_fail(ctx, e, "shading launch");
 }
 }  // namespace

@@ -38,6 +38,10 @@ struct hvx_ctx {
     std::string fixed_scratch_error;  // why the lookup failed, if it did
     // options (hvx_ctx_set_option)
     int shade_exact = 0;
+    // HVX_OPT_SHADE_KERNEL_EVENTS: event pairs around shading-kernel launches
+    // (a ring of kev.size() / 2 pairs; kev_count recorded since the last read)
+    std::vector<hipEvent_t> kev;
+    uint32_t kev_next = 0, kev_count = 0;
 };
 
 namespace hvx {
@@ -133,7 +137,8 @@ hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const
                         const hvx_pbr_pc& pc, const hvx_light* d_lights, uint32_t n_lights,
                         const hvx_light_cell* d_cells, const uint32_t* d_indices, const hvx_ibl_maps& ibl,
                         float* d_rgb, bool exact, float4* light_scratch, uint32_t* bin_scratch,
-                        const LearnedShade* learned, hipStream_t s);
+                        const LearnedShade* learned, hipStream_t s, hipEvent_t ev_begin = nullptr,
+                        hipEvent_t ev_end = nullptr);
 // Whether shading bins fragments by cluster for this grid, and the scratch
 // it needs for that (the fragment order and the tile bounds).
 bool shade_binned(const hvx_cf_ubo& ubo);

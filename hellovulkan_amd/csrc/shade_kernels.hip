@@ -732,7 +732,7 @@ hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const
                         const hvx_pbr_pc& pc, const hvx_light* d_lights, uint32_t n_lights,
                         const hvx_light_cell* d_cells, const uint32_t* d_indices, const hvx_ibl_maps& ibl,
                         float* d_rgb, bool exact, float4* light_scratch, uint32_t* bin_scratch,
-                        const LearnedShade* learned, hipStream_t s) {
+                        const LearnedShade* learned, hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end) {
     ShadeArgs a{};
     a.depth = fr.depth;
     for (int c = 0; c < 3; ++c) {
@@ -792,17 +792,18 @@ hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const
         hipLaunchKernelGGL(tile_bounds_kernel, dim3((fr.width + fr.height + 255) / 256), dim3(256), 0, s, a);
         hipLaunchKernelGGL(bin_kernel, dim3(a.X * a.Y), dim3(kBinThreads), 0, s, a);
     }
+    if (!exact && n_lights > 0)
+        hipLaunchKernelGGL(shade_light_prep_kernel, dim3((n_lights + 255) / 256), dim3(256), 0, s, d_lights, n_lights,
+                           pc.light_intensity, light_scratch);
+    if (ev_begin) (void)hipEventRecord(ev_begin, s);
     if (exact) {
         if (learned) hipLaunchKernelGGL((shade_kernel<true, true>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((shade_kernel<true, false>), grid, block, 0, s, a);
     } else {
-        if (n_lights > 0) {
-            hipLaunchKernelGGL(shade_light_prep_kernel, dim3((n_lights + 255) / 256), dim3(256), 0, s, d_lights,
-                               n_lights, pc.light_intensity, light_scratch);
-        }
         if (learned) hipLaunchKernelGGL((shade_kernel<false, true>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((shade_kernel<false, false>), grid, block, 0, s, a);
     }
+    if (ev_end) (void)hipEventRecord(ev_end, s);
     if (staged) hipLaunchKernelGGL(unbin_kernel, dim3(a.X * a.Y), dim3(kBinThreads), unbin_lds, s, a);
     return hipGetLastError();
 }

@@ -177,10 +177,20 @@ typedef enum hvx_option {
      * multiply-add and hardware reciprocal/exp2/log2 (within 1e-5 relative of
      * the exact evaluation); 1: every operation exactly as the FP32
      * definition of the path (bit-reproducible on any IEEE machine). */
-    HVX_OPT_SHADE_EXACT = 1
+    HVX_OPT_SHADE_EXACT = 1,
+    /* 0 (default): off; k in [1, 256]: record a pair of HIP events around
+     * each launch of the shading kernel itself (not the binning and light
+     * table launches of the stage), keeping the last k pairs, for
+     * hvx_ctx_kernel_times. For measurement. */
+    HVX_OPT_SHADE_KERNEL_EVENTS = 2
 } hvx_option;
 int hvx_ctx_set_option(hvx_ctx* ctx, int option, int64_t value);
 int hvx_ctx_get_option(const hvx_ctx* ctx, int option, int64_t* value);
+/* Durations (ms, oldest first) of the shading-kernel launches recorded
+ * since the last call (at most the HVX_OPT_SHADE_KERNEL_EVENTS depth):
+ * waits for the last recorded launch, writes min(recorded, max) values and
+ * their count, and clears the record. No counterpart in the reference. */
+int hvx_ctx_kernel_times(hvx_ctx* ctx, float* ms_out, uint32_t max, uint32_t* n_out);
 /* Counter of the context's scratch (re)allocations. A graph captured at one
  * value may be replayed only while the value is unchanged. No counterpart in
  * the reference (its descriptor sets bind fixed buffers). */

@@ -337,3 +337,50 @@ def test_scratch_growth_refused_under_capture_and_counted(gpu):
         torch.cuda.synchronize()
     finally:
         c.close()
+
+
+def test_shade_kernel_event_ring(gpu):
+    """HVX_OPT_SHADE_KERNEL_EVENTS: each shading call records one duration
+    (of the shading kernel alone, so within the call's own time), reading
+    clears the record, the ring keeps the last k calls, and out-of-range
+    depths are refused."""
+    import torch
+    c = abi.Context(0)
+    try:
+        W, H, grid = 320, 200, (16, 9, 24)
+        desc, cam, ubo, ivp = camera(W, H, grid)
+        Cn = grid[0] * grid[1] * grid[2]
+        lights = abi.scene_lights(3, 256)
+        fr = abi.scene_fragments(4, desc, ivp, threads=4)
+        maps = random_ibl_maps(np.random.default_rng(2))
+        d_aabbs = gpu_aabbs(c, ubo)
+        d_lights = to_dev(lights.view(np.uint8))
+        cells = torch.empty(Cn * 2, dtype=torch.int32, device="cuda")
+        idx = torch.empty(Cn * 64, dtype=torch.int32, device="cuda")
+        c.cull_lights(ubo, d_aabbs, d_lights, 256, 64, cells, idx, Cn * 64)
+        d_fr = {k: to_dev(v) for k, v in fr.items()}
+        dm = {k: to_dev(maps[k]) for k in ("spec", "diff", "lut")}
+        m = abi.ibl_maps(dm["spec"], maps["spec_size"], maps["spec_mips"], dm["diff"], maps["diff_size"], dm["lut"],
+                         maps["lut_w"], maps["lut_h"])
+        rgb = torch.empty(3 * W * H, dtype=torch.float32, device="cuda")
+        frs = abi.frag_soa(d_fr, W, H, ivp)
+        for bad in (-1, 257):
+            with pytest.raises(abi.HvxError):
+                c.set_option(abi.OPT_SHADE_KERNEL_EVENTS, bad)
+        c.set_option(abi.OPT_SHADE_KERNEL_EVENTS, 3)
+        assert c.get_option(abi.OPT_SHADE_KERNEL_EVENTS) == 3
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            c.shade(frs, cam, ubo, abi.pbr_defaults(), d_lights, 256, cells, idx, m, rgb)
+        e1.record()
+        torch.cuda.synchronize()
+        times = c.kernel_times()
+        assert len(times) == 3 and all(t > 0.0 for t in times)
+        assert sum(times) <= e0.elapsed_time(e1)
+        assert c.kernel_times() == []
+        c.set_option(abi.OPT_SHADE_KERNEL_EVENTS, 0)
+        with pytest.raises(abi.HvxError):
+            c.kernel_times()
+    finally:
+        c.close()
