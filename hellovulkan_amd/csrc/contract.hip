@@ -845,10 +845,7 @@ static_assert(sizeof(StageTable) < 4000 && sizeof(DenseTable) < 4000 && sizeof(N
 // hi*hi + hi*lo + lo*hi (the lo*lo term is below the FP32 accumulation's
 // resolution), three MFMAs per FP16 one.
 constexpr uint32_t kNearWaves = 8;  // waves per 16x16 output tile (two row tiles each)
-#ifndef HVX_NEAR_PARTS
-#define HVX_NEAR_PARTS 2
-#endif
-constexpr uint32_t kNearParts = HVX_NEAR_PARTS;  // workgroups per tile
+constexpr uint32_t kNearParts = 2;  // workgroups per tile (fixed by the 256-lane LDS staging below)
 static_assert(kNearWaves % kNearParts == 0, "a tile splits into whole waves");
 static_assert(64 * kNearWaves / kNearParts == 256, "the near GEMM's LDS staging assumes 256-lane workgroups");
 #ifndef HVX_NEAR_ROWS
@@ -863,14 +860,6 @@ static_assert(64 * kNearWaves / kNearParts == 256, "the near GEMM's LDS staging 
 constexpr uint32_t kNearRows = HVX_NEAR_ROWS;
 constexpr uint32_t kNearSingleParts = 16 / kNearRows / 4;
 static_assert(kNearRows == 1 || kNearRows == 2 || kNearRows == 4, "1, 2 or 4 row tiles per wave");
-#ifndef HVX_SINGLE_STAGED
-#define HVX_SINGLE_STAGED 0
-#endif
-// single-probe calls through the LDS-staged GEMMs (as multi-probe calls)
-// instead of near_gemm_kernel / dense_gemm_kernel. Measured on the 512^2 x 10
-// GGX chain: FP16 near GEMM 203 -> 246 us, split-BF16 386 -> 388 us, dense
-// 63.0 -> 62.4 us, so off.
-constexpr bool kSingleStaged = HVX_SINGLE_STAGED != 0;
 
 template <bool SPLIT, int PG>
 __global__ __launch_bounds__(512) void near_gemm_kernel(Probes Q, NearTable T, const float* sc) {
@@ -1471,17 +1460,18 @@ int launch_call(hvx_contract_plan* P, const Probes& Q, uint32_t n, uint32_t chai
     // each weight fragment, and the group index is the fastest grid dimension,
     // so groups reading one weight block run together and share it in the caches
     Probes G = Q;
-    const bool grouped = n > 1, staged = grouped || kSingleStaged;
+    // single-probe calls use the unstaged GEMMs: staging the B operand in LDS
+    // pays only when several probes share it (measured on the 512^2 x 10 GGX
+    // chain: FP16 near GEMM 203 us unstaged, 246 us staged)
+    const bool grouped = n > 1;
     const uint32_t dense_wgs = (P->dense.waves + 3) / 4,
-                   near_wgs = P->near.tiles * (staged ? kNearParts : kNearSingleParts);
+                   near_wgs = P->near.tiles * (grouped ? kNearParts : kNearSingleParts);
     const uint32_t groups = grouped ? (n + kProbeGroup - 1) / kProbeGroup : 1;
     G.probe_fast = grouped && std::max(dense_wgs, near_wgs) <= 65535u;
     if (P->dense.n) {
         const dim3 grid = G.probe_fast ? dim3(groups, dense_wgs) : dim3(dense_wgs, groups);
         if (grouped)
             hipLaunchKernelGGL(dense_gemm_staged_kernel<kProbeGroup>, grid, dim3(256), 0, s, G, P->dense);
-        else if (staged)
-            hipLaunchKernelGGL(dense_gemm_staged_kernel<1>, grid, dim3(256), 0, s, G, P->dense);
         else
             hipLaunchKernelGGL(dense_gemm_kernel<1>, grid, dim3(256), 0, s, G, P->dense);
         hipLaunchKernelGGL(dense_epilogue_kernel, dim3((P->dense.epi + 255) / 256, n), dim3(256), 0, s, Q, P->dense,
@@ -1495,10 +1485,6 @@ int launch_call(hvx_contract_plan* P, const Probes& Q, uint32_t n, uint32_t chai
             hipLaunchKernelGGL((near_gemm_staged_kernel<true, kProbeGroup>), grid, block, 0, s, G, P->near, P->sc);
         else if (grouped)
             hipLaunchKernelGGL((near_gemm_staged_kernel<false, kProbeGroup>), grid, block, 0, s, G, P->near, P->sc);
-        else if (staged && split)
-            hipLaunchKernelGGL((near_gemm_staged_kernel<true, 1>), grid, block, 0, s, G, P->near, P->sc);
-        else if (staged)
-            hipLaunchKernelGGL((near_gemm_staged_kernel<false, 1>), grid, block, 0, s, G, P->near, P->sc);
         else if (split)
             hipLaunchKernelGGL((near_gemm_kernel<true, 1>), grid, block, 0, s, G, P->near, P->sc);
         else
