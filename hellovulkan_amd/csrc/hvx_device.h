@@ -48,36 +48,35 @@ __device__ __forceinline__ void matvec4(const float* m, float x, float y, float 
 }
 
 // ------------------------------------------- correctly rounded fast paths
-// Correctly rounded square root of a positive normal x in [2^-96, 2^126]:
-// the hardware estimate (within 1 ulp) corrected by the exact residuals of
-// its two neighbours — the sequence the compiler emits for sqrtf, without the
-// subnormal rescaling and special-value selects. Verified equal to sqrtf on
-// every float of [1, 4) on gfx950 (tests/test_gpu_fastmath.py).
+// Correctly rounded square root of x in [2^-102, FLT_MAX]: Markstein's
+// iteration from the hardware reciprocal square root y ~ 1/sqrt(x):
+// s0 = rn(x y), then one correction s0 + (x - s0^2) (y/2) with the residual
+// fused, so it is exact. Five operations where correcting the hardware square
+// root by both neighbours' residuals takes nine. Verified equal to sqrtf on
+// every float of [2^-102, 2^128) on gfx950 (tests/test_gpu_fastmath.py);
+// below 2^-102 the residual is subnormal and the result can be an ulp off.
 __device__ __forceinline__ float sqrt_rn_normal(float x) {
-    const float s = __builtin_amdgcn_sqrtf(x);
-    const float dn = u2f(f2u(s) - 1u), up = u2f(f2u(s) + 1u);
-    const float rdn = __builtin_fmaf(-dn, s, x), rup = __builtin_fmaf(-up, s, x);
-    float r = rdn <= 0.0f ? dn : s;
-    r = rup > 0.0f ? up : r;
-    return r;
+    const float y = __builtin_amdgcn_rsqf(x);
+    const float s0 = x * y;
+    const float r = __builtin_fmaf(-s0, s0, x);
+    return __builtin_fmaf(r, 0.5f * y, s0);
 }
 
 // Correctly rounded reciprocal of a normal b with |b| in [2^-125, 2^125]:
 // hardware estimate (within 1 ulp) plus one Newton-Raphson correction with
 // fused multiply-add, y' = y + y (1 - b y): the fused residual is rounded
 // once, so y' carries only an O(ulp^2) error before its single rounding.
-// Verified equal to 1.0f / b on every float of [1, 2) and of three more
-// binades on gfx950 (the map b -> 1/b commutes with exact power of two
-// scaling in that range; tests/test_gpu_fastmath.py).
+// Verified equal to 1.0f / b on every float b with |b| in [2^-125, 2^125)
+// on gfx950 (tests/test_gpu_fastmath.py).
 __device__ __forceinline__ float rcp_rn_normal(float b) {
     const float y = __builtin_amdgcn_rcpf(b);
     const float e = __builtin_fmaf(-b, y, 1.0f);
     return __builtin_fmaf(e, y, y);
 }
 
-// normalize3 of a vector whose squared length is a normal float in
-// [2^-96, 2^126] (every near-unit vector): the same correctly rounded square
-// root and reciprocal, by the fast paths above, so the result is bit-identical.
+// normalize3 of a vector whose squared length is a float in [2^-102, FLT_MAX]
+// (every near-unit vector): the same correctly rounded square root and
+// reciprocal, by the fast paths above, so the result is bit-identical.
 __device__ __forceinline__ f3 normalize3_rn(f3 a) { return mul3(a, rcp_rn_normal(sqrt_rn_normal(dot3(a, a)))); }
 
 // ------------------------------------------------------------ cube sampling
