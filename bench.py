@@ -35,6 +35,8 @@ PEAK_HBM_GBS = 8000.0      # MI355X HBM3E peak (datasheet)
 # fragment set-up + ambient, 140 per listed light; 45 bytes of fragment data
 # in and out (depth 4, normal 12, albedo 12, roughness 4, metallic 1, RGB 12)
 OPS_FRAG, OPS_PAIR, BYTES_FRAG = 160, 140, 45
+# texels the contraction filters' bright-texel split may sum exactly
+HOT_TEXELS = 2048
 
 
 def parse():
@@ -267,12 +269,16 @@ def contract_bench(ctx, stream, d_cube, S, So, mips, Sd, plain_spec_ms, plain_di
     from hellovulkan_amd import abi
     levels = abi.mip_count(S)
     out = {}
-    for name, ftype, so, m, plain, prec in [
-            ("specular", abi.FILTER_SPECULAR, So, mips, plain_spec_ms, abi.PRECISION_FP16_SCALED),
-            ("specular_split_bf16", abi.FILTER_SPECULAR, So, mips, plain_spec_ms, abi.PRECISION_BF16_SPLIT),
-            ("diffuse", abi.FILTER_DIFFUSE, Sd, 1, plain_diff_ms, abi.PRECISION_FP16_SCALED)]:
+    for name, ftype, so, m, plain, prec, hot in [
+            ("specular", abi.FILTER_SPECULAR, So, mips, plain_spec_ms, abi.PRECISION_FP16_SCALED, 0),
+            ("specular_split_bf16", abi.FILTER_SPECULAR, So, mips, plain_spec_ms, abi.PRECISION_BF16_SPLIT, 0),
+            ("specular_hot", abi.FILTER_SPECULAR, So, mips, plain_spec_ms, abi.PRECISION_FP16_SCALED, HOT_TEXELS),
+            ("diffuse", abi.FILTER_DIFFUSE, Sd, 1, plain_diff_ms, abi.PRECISION_FP16_SCALED, 0),
+            ("diffuse_hot", abi.FILTER_DIFFUSE, Sd, 1, plain_diff_ms, abi.PRECISION_FP16_SCALED, HOT_TEXELS)]:
         t0 = time.time()
         plan = abi.ContractPlan(ctx, ftype, S, so, m, prec)
+        if hot:
+            plan.set_hot(hot)
         torch.cuda.synchronize()
         build_s = time.time() - t0
         dst = torch.empty(abi.cube_chain_floats(so, m), dtype=torch.float32, device=d_cube.device)
@@ -286,8 +292,12 @@ def contract_bench(ctx, stream, d_cube, S, So, mips, Sd, plain_spec_ms, plain_di
                      "weight_stream_TBs": round(wbytes / (ms * 1e-3) / 1e12, 3),
                      "mips": [{"kind": ["delta", "full", "near+far"][i.kind], "in": i.in_face, "far": i.far_face}
                               for i in info]}
+        if hot:
+            T, n_hot = plan.hot_info()
+            out[name].update(bright_texels_max=hot, threshold=T, bright_texels=n_hot)
         plan.close()
     log(f"contraction filters: {out['specular']['ms']} ms specular ({out['specular']['speedup']}x), "
+        f"{out['specular_hot']['ms']} ms with the bright-texel split ({out['specular_hot']['speedup']}x), "
         f"{out['diffuse']['ms']} ms diffuse ({out['diffuse']['speedup']}x)")
     return out
 
@@ -632,13 +642,15 @@ def cpu_baseline(args, ubo, cam, desc, ivp, lights, fr, d_cube, CUBE, d_spec, d_
                "plain_fp32_vs_fp64_oracle": _max_rel(plain, ref64)}
         if "specular" in outs:
             twin = O.contract(1, cube0, CUBE, SPEC, MIPS, m, ids, threads=threads)
+            twin_hot = O.contract(1, cube0, CUBE, SPEC, MIPS, m, ids, threads=threads, max_hot=HOT_TEXELS)
             exact_ids = ids[:8]
             ideal = O.ideal(1, cube0, CUBE, lv[m]["alpha2"], Sm, exact_ids, threads=threads) if m > 0 else None
-            for key, name in (("specular", "fp16"), ("specular_split_bf16", "split_bf16")):
+            for key, name in (("specular", "fp16"), ("specular_split_bf16", "split_bf16"),
+                              ("specular_hot", "fp16_bright_split")):
                 if key not in outs:
                     continue
                 got = outs[key][off: off + 4 * T].reshape(-1, 4)[ids, :3].astype(np.float64)
-                row[f"contract_{name}_vs_fp64_twin"] = _max_rel(got, twin)
+                row[f"contract_{name}_vs_fp64_twin"] = _max_rel(got, twin_hot if key == "specular_hot" else twin)
                 if ideal is not None:
                     row[f"contract_{name}_vs_exact_integral"] = _max_rel(got[:exact_ids.size], ideal)
             if ideal is not None:
@@ -658,11 +670,14 @@ def cpu_baseline(args, ubo, cam, desc, ivp, lights, fr, d_cube, CUBE, d_spec, d_
                                                                         dref32.view(np.uint32))),
                    "plain_fp32_vs_fp64_oracle": _max_rel(dplain[:, :3].astype(np.float64), dref64)}
     if "diffuse" in outs:
-        got = outs["diffuse"].reshape(-1, 4)[dids, :3].astype(np.float64)
-        diffuse_err["contract_fp16_vs_fp64_twin"] = _max_rel(got, O.contract(0, cube0, CUBE, 32, 1, 0, dids,
-                                                                             threads=threads))
-        diffuse_err["contract_fp16_vs_exact_integral"] = _max_rel(got[:8], O.ideal(0, cube0, CUBE, 0.0, 32, dids[:8],
-                                                                                     threads=threads))
+        dideal = O.ideal(0, cube0, CUBE, 0.0, 32, dids[:8], threads=threads)
+        for key, name, hot in (("diffuse", "fp16", 0), ("diffuse_hot", "fp16_bright_split", HOT_TEXELS)):
+            if key not in outs:
+                continue
+            got = outs[key].reshape(-1, 4)[dids, :3].astype(np.float64)
+            diffuse_err[f"contract_{name}_vs_fp64_twin"] = _max_rel(got, O.contract(0, cube0, CUBE, 32, 1, 0, dids,
+                                                                                    threads=threads, max_hot=hot))
+            diffuse_err[f"contract_{name}_vs_exact_integral"] = _max_rel(got[:8], dideal)
     out["ibl_filters"] = {
         "workload": "6x1024^2 cube -> 10-mip GGX chain from 512^2 (1024 samples) and 6x32^2 irradiance",
         "specular_ms": round(spec_ms_all, 1), "specular_ms_1thread": round(spec_ms_one, 1),

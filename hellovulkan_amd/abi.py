@@ -168,6 +168,8 @@ SIGNATURES = [
     ("hvx_ibl_filter_contract", C.c_int, [vp, vp, u32, vp, vp]),
     ("hvx_ibl_contract_plan_reserve", C.c_int, [vp, u32]),
     ("hvx_ibl_filter_contract_multi", C.c_int, [vp, u32, vp, u32, vp, vp]),
+    ("hvx_ibl_contract_plan_set_hot", C.c_int, [vp, u32]),
+    ("hvx_ibl_contract_hot_info", C.c_int, [vp, P(C.c_float), P(u32)]),
     ("hvx_mlp_create", C.c_int, [vp, C.c_float, C.c_uint64, P(vp)]),
     ("hvx_mlp_destroy", C.c_int, [vp]),
     ("hvx_mlp_get_params", C.c_int, [vp, vp]),
@@ -471,6 +473,16 @@ class ContractPlan:
         self.ctx._check(self.lib.hvx_ibl_filter_contract_multi(self.h, n, _ptr(d_chain_ptrs), chain_levels,
                                                                 _ptr(d_out_ptrs), self.ctx._stream(stream)),
                         "hvx_ibl_filter_contract_multi")
+
+    def set_hot(self, max_hot: int):
+        """Bright-texel split for single-probe calls (0: off)."""
+        self.ctx._check(self.lib.hvx_ibl_contract_plan_set_hot(self.h, max_hot), "hvx_ibl_contract_plan_set_hot")
+
+    def hot_info(self):
+        """(threshold, hot texels) of the last split call."""
+        t, n = C.c_float(), C.c_uint32()
+        self.ctx._check(self.lib.hvx_ibl_contract_hot_info(self.h, C.byref(t), C.byref(n)), "hvx_ibl_contract_hot_info")
+        return t.value, n.value
 
     def filter(self, chain, chain_levels: int, out, stream=None):
         self.ctx._check(self.lib.hvx_ibl_filter_contract(self.h, _ptr(chain), chain_levels, _ptr(out),

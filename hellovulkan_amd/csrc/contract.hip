@@ -1150,6 +1150,182 @@ __global__ __launch_bounds__(256) void delta_kernel(Probes Q, uint32_t S, uint32
     out[id] = make_float4(c.x, c.y, c.z, 1.0f);
 }
 
+// --------------------------------------------------- bright-texel split
+// (hvx_contract.h): threshold from a histogram of the largest channel's float
+// pattern, clamped level 0, the hot texels in texel order, and their exact
+// level-0 sum added to every non-delta output texel.
+struct HotArgs {
+    const float4* level0;  // the caller's level 0
+    float4* clamped;       // level 0 of the clamped chain
+    uint32_t S, K;
+    uint32_t* hist;        // kHotBins counts, zero between calls
+    uint32_t* cnt;         // [0] hot texels found, [1] hot texels listed
+    float* T;
+    uint32_t* idx;         // kHotMax texel indices
+    float4* list;          // per listed texel: (direction, 0), (solid angle * excess, 0)
+};
+
+__device__ __forceinline__ float hot_largest(float4 t) { return fmaxf(fmaxf(t.x, t.y), t.z); }
+
+__global__ __launch_bounds__(256) void hot_hist_kernel(HotArgs H) {
+    __shared__ uint32_t sh[hvx::kHotBins];
+    for (uint32_t b = threadIdx.x; b < hvx::kHotBins; b += 256) sh[b] = 0u;
+    __syncthreads();
+    const size_t n = 6ull * H.S * H.S;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const float m = hot_largest(H.level0[i]);
+        if (m > 0.0f) atomicAdd(&sh[hvx::contract_hot_bin(__float_as_uint(m))], 1u);
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < hvx::kHotBins; b += 256)
+        if (sh[b]) atomicAdd(&H.hist[b], sh[b]);
+}
+
+// One workgroup of 1024: c(b) = texels in bins >= b (a scan over the bins
+// from the top, two per thread), T from the number of bins with c(b) > K;
+// clears the histogram and the counters for the next call.
+__global__ __launch_bounds__(1024) void hot_select_kernel(HotArgs H) {
+    __shared__ uint32_t s_wave[16];
+    const uint32_t t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const uint32_t b_hi = hvx::kHotBins - 1 - 2 * t, b_lo = b_hi - 1;  // thread t: bins 2047-2t, 2046-2t
+    const uint32_t h_hi = H.hist[b_hi], h_lo = H.hist[b_lo];
+    H.hist[b_hi] = 0u;
+    H.hist[b_lo] = 0u;
+    uint32_t inc = h_hi + h_lo;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t u = __shfl_up(inc, d, 64);
+        if ((int)lane >= d) inc += u;
+    }
+    if (lane == 63) s_wave[wv] = inc;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t w = 0; w < wv; ++w) before += s_wave[w];
+    const uint32_t c_lo = before + inc, c_hi = c_lo - h_lo;  // c(b_lo), c(b_hi)
+    const int over = __syncthreads_count(c_hi > H.K) + __syncthreads_count(c_lo > H.K);
+    if (t == 0) {
+        H.T[0] = __uint_as_float(hvx::contract_hot_threshold_bits((uint32_t)over));
+        H.cnt[0] = 0u;
+        H.cnt[1] = 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void hot_compact_kernel(HotArgs H) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= 6ull * H.S * H.S) return;
+    const float T = H.T[0];
+    float4 v = H.level0[i];
+    const float m = hot_largest(v);
+    if (m > 0.0f && m >= T) {
+        const uint32_t k = atomicAdd(&H.cnt[0], 1u);
+        if (k < H.K) H.idx[k] = (uint32_t)i;
+        v.x = v.x > T ? T : v.x;
+        v.y = v.y > T ? T : v.y;
+        v.z = v.z > T ? T : v.z;
+    }
+    H.clamped[i] = v;
+}
+
+// One workgroup of 1024: sorts the found indices (bitonic, in LDS) so the
+// sum order is the texel order, then writes each texel's direction and
+// solid angle times excess (FP64 geometry, as the oracle's level-0 sum).
+__global__ __launch_bounds__(1024) void hot_list_kernel(HotArgs H) {
+    __shared__ uint32_t s[hvx::kHotMax];
+    const uint32_t n = min(H.cnt[0], H.K), t = threadIdx.x;
+    for (uint32_t j = t; j < hvx::kHotMax; j += 1024) s[j] = j < n ? H.idx[j] : 0xffffffffu;
+    __syncthreads();
+    for (uint32_t size = 2; size <= hvx::kHotMax; size <<= 1)
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t j = t; j < hvx::kHotMax / 2; j += 1024) {
+                const uint32_t lo = 2 * j - (j & (stride - 1)), hi = lo + stride;
+                const bool up = (lo & size) == 0;
+                const uint32_t a = s[lo], b = s[hi];
+                if ((a > b) == up) {
+                    s[lo] = b;
+                    s[hi] = a;
+                }
+            }
+            __syncthreads();
+        }
+    const float T = H.T[0];
+    const uint32_t S2 = H.S * H.S;
+    for (uint32_t k = t; k < n; k += 1024) {
+        const uint32_t i = s[k], g = i / S2, r = i - g * S2, y = r / H.S, x = r - y * H.S;
+        const double u = hvx::contract_coord((int)x, H.S), v = hvx::contract_coord((int)y, H.S);
+        double p[3];
+        hvx::contract_face_point((int)g, u, v, p);
+        const double rl = 1.0 / sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]), dw = hvx::contract_dw(u, v, H.S);
+        const float4 c = H.level0[i];
+        const float ex = c.x > T ? c.x - T : 0.0f, ey = c.y > T ? c.y - T : 0.0f, ez = c.z > T ? c.z - T : 0.0f;
+        H.list[2 * k] = make_float4((float)(p[0] * rl), (float)(p[1] * rl), (float)(p[2] * rl), 0.0f);
+        H.list[2 * k + 1] = make_float4((float)(dw * ex), (float)(dw * ey), (float)(dw * ez), 0.0f);
+    }
+    if (t == 0) H.cnt[1] = n;
+}
+
+struct HotMip {
+    uint32_t So, first;  // output face; first thread of the mip (a multiple of 256: one mip per workgroup)
+    float alpha2, h1;    // GGX a^2 and (1 - a^2) / 2
+    float scale;         // a^2 / pi / (weight integral); diffuse: 1 / pi
+    uint32_t _pad;
+    size_t out_off;      // the mip in the output chain, float4 units
+};
+struct HotAdd {
+    HotMip m[hvx::kContractMaxMips];
+    uint32_t nm, blocks;
+    const float4* list;
+    const uint32_t* cnt;
+    float4* out;
+};
+
+// out(o) += scale * sum_k w'(n_o . l_k) E_k over the listed hot texels, with
+// w' = c / d^2 (GGX, d = a^2 + (1 - c)(1 - a^2)/2 = N.H^2 (a^2 - 1) + 1) or
+// c (cosine), 0 for c <= 0; 1 - c is formed as |n - l|^2 / 2, which keeps its
+// relative accuracy for nearby directions. The list is staged through LDS.
+template <bool DIFFUSE>
+__global__ __launch_bounds__(256) void hot_add_kernel(HotAdd A) {
+    constexpr uint32_t kChunk = 512;
+    __shared__ float4 sl[2 * kChunk];
+    const uint32_t b0 = blockIdx.x * 256;
+    uint32_t mi = 0;
+    while (mi + 1 < A.nm && b0 >= A.m[mi + 1].first) ++mi;  // uniform over the workgroup
+    const HotMip M = A.m[mi];
+    const uint32_t So2 = M.So * M.So, loc0 = b0 + threadIdx.x - M.first;
+    const bool live = loc0 < 6 * So2;
+    const uint32_t loc = live ? loc0 : 0u, f = loc / So2, o = loc - f * So2, Y = o / M.So, X = o - Y * M.So;
+    const f3 n = texel_dir((int)f, (int)X, (int)Y, (int)M.So);
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    const uint32_t nh = A.cnt[1];
+    for (uint32_t base = 0; base < nh; base += kChunk) {
+        const uint32_t cnt = min(kChunk, nh - base);
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < 2 * cnt; j += 256) sl[j] = A.list[2 * base + j];
+        __syncthreads();
+        for (uint32_t j = 0; j < cnt; ++j) {
+            const float4 L = sl[2 * j], E = sl[2 * j + 1];
+            const float dx = n.x - L.x, dy = n.y - L.y, dz = n.z - L.z;
+            const float omc = 0.5f * __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+            const float c = 1.0f - omc;
+            float w;
+            if constexpr (DIFFUSE) {
+                w = c;
+            } else {
+                const float d = __builtin_fmaf(omc, M.h1, M.alpha2);
+                w = c * __builtin_amdgcn_rcpf(d * d);
+            }
+            w = c > 0.0f ? w : 0.0f;
+            ax = __builtin_fmaf(w, E.x, ax);
+            ay = __builtin_fmaf(w, E.y, ay);
+            az = __builtin_fmaf(w, E.z, az);
+        }
+    }
+    if (!live) return;
+    float4& r = A.out[M.out_off + loc];
+    r.x = __builtin_fmaf(ax, M.scale, r.x);
+    r.y = __builtin_fmaf(ay, M.scale, r.y);
+    r.z = __builtin_fmaf(az, M.scale, r.z);
+}
+
 // -------------------------------------------------------------- host side
 uint32_t ilog2(uint32_t v) {
     uint32_t l = 0;
@@ -1330,6 +1506,14 @@ struct hvx_contract_plan {
     StageTable stage{};
     DenseTable dense{};
     NearTable near{};
+    // bright-texel split (hvx_ibl_contract_plan_set_hot): off while hot_k == 0
+    uint32_t hot_k = 0;
+    float* hot_chain = nullptr;  // the clamped chain, min_level + 1 levels
+    uint32_t* hot_hist = nullptr;
+    uint32_t* hot_cnt = nullptr;
+    float* hot_T = nullptr;
+    uint32_t* hot_idx = nullptr;
+    float4* hot_list = nullptr;
     std::vector<void*> allocs;
 
     template <class T> hipError_t alloc(T** p, size_t count) {
@@ -1435,9 +1619,10 @@ hipError_t build_near(hvx_contract_plan* P, MipPlan& M, bool split) {
 namespace {
 
 // The eight launches of a filter call over n probes (grid dimension y, z for
-// the face maxima: probe).
+// the face maxima: probe). delta_chain (single probe, nullable): the chain the
+// delta mip samples instead of the probe's chain.
 int launch_call(hvx_contract_plan* P, const Probes& Q, uint32_t n, uint32_t chain_levels, hipStream_t s,
-                const char* what) {
+                const char* what, const float4* delta_chain = nullptr) {
     hvx_ctx* ctx = P->ctx;
     if (chain_levels <= P->min_level)
         return pfail(ctx, HVX_ERR_INVALID_ARG, "the input chain needs %u levels (has %u); build it with "
@@ -1493,11 +1678,69 @@ int launch_call(hvx_contract_plan* P, const Probes& Q, uint32_t n, uint32_t chai
     }
     if (P->delta_mip != ~0u) {
         const uint32_t So = P->mp[P->delta_mip].L.So;
-        hipLaunchKernelGGL(delta_kernel, dim3((6 * So * So + 255) / 256, n), dim3(256), 0, s, Q, P->S, So,
+        Probes D = Q;
+        if (delta_chain) D.chain0 = delta_chain;
+        hipLaunchKernelGGL(delta_kernel, dim3((6 * So * So + 255) / 256, n), dim3(256), 0, s, D, P->S, So,
                            hvx::cube_level_offset_floats(P->So, P->delta_mip) / 4);
     }
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? HVX_OK : pfail(ctx, HVX_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+// A single-probe call with the bright-texel split: threshold, clamped level 0
+// and its box levels, the hot list, the contraction of the clamped chain (the
+// delta mip samples the caller's chain), then the hot texels' exact sum.
+int launch_hot_call(hvx_contract_plan* P, const float* d_chain, uint32_t chain_levels, float* d_out, hipStream_t s) {
+    hvx_ctx* ctx = P->ctx;
+    if (chain_levels <= P->min_level)
+        return pfail(ctx, HVX_ERR_INVALID_ARG, "the input chain needs %u levels (has %u); build it with "
+                     "hvx_ibl_cube_mips", P->min_level + 1, chain_levels);
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || (cur != ctx->device && hipSetDevice(ctx->device) != hipSuccess))
+        return pfail(ctx, HVX_ERR_HIP, "cannot bind device %d", ctx->device);
+    const HotArgs H{reinterpret_cast<const float4*>(d_chain), reinterpret_cast<float4*>(P->hot_chain), P->S,
+                    P->hot_k, P->hot_hist, P->hot_cnt, P->hot_T, P->hot_idx, P->hot_list};
+    const size_t texels = 6ull * P->S * P->S;
+    hipLaunchKernelGGL(hot_hist_kernel, dim3((uint32_t)std::min<size_t>(2048, (texels + 255) / 256)), dim3(256), 0, s,
+                       H);
+    hipLaunchKernelGGL(hot_select_kernel, dim3(1), dim3(1024), 0, s, H);
+    hipLaunchKernelGGL(hot_compact_kernel, dim3((uint32_t)((texels + 255) / 256)), dim3(256), 0, s, H);
+    hipLaunchKernelGGL(hot_list_kernel, dim3(1), dim3(1024), 0, s, H);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return pfail(ctx, HVX_ERR_HIP, "bright-texel split: %s", hipGetErrorString(e));
+    if (P->min_level > 0 && (e = hvx::launch_cube_mips(P->hot_chain, P->S, P->min_level + 1, s)) != hipSuccess)
+        return pfail(ctx, HVX_ERR_HIP, "clamped chain levels: %s", hipGetErrorString(e));
+    const Probes Q{reinterpret_cast<const float4*>(P->hot_chain), reinterpret_cast<float4*>(d_out), nullptr, nullptr,
+                   0, 0, 0, 1};
+    int st = launch_call(P, Q, 1, chain_levels, s, "hvx_ibl_filter_contract", reinterpret_cast<const float4*>(d_chain));
+    if (st) return st;
+    HotAdd A{};
+    uint32_t first = 0;
+    const int ctype = P->type == HVX_FILTER_DIFFUSE ? hvx::CONTRACT_DIFFUSE : hvx::CONTRACT_SPECULAR;
+    for (uint32_t m = 0; m < P->mips; ++m) {
+        const ContractLevel& L = P->mp[m].L;
+        if (L.kind == hvx::CONTRACT_DELTA) continue;
+        HotMip& M = A.m[A.nm++];
+        M.So = L.So;
+        M.first = first;
+        M.alpha2 = (float)L.alpha2;
+        M.h1 = (float)(0.5 * (1.0 - L.alpha2));
+        const double z = hvx::contract_weight_integral(ctype, L.alpha2);
+        M.scale = (float)((ctype == hvx::CONTRACT_DIFFUSE ? 1.0 : L.alpha2 / hvx::kContractPi) / z);
+        M.out_off = hvx::cube_level_offset_floats(P->So, m) / 4;
+        first += (6 * L.So * L.So + 255) / 256 * 256;
+    }
+    if (A.nm == 0) return HVX_OK;
+    A.blocks = first / 256;
+    A.list = P->hot_list;
+    A.cnt = P->hot_cnt;
+    A.out = reinterpret_cast<float4*>(d_out);
+    if (ctype == hvx::CONTRACT_DIFFUSE)
+        hipLaunchKernelGGL(hot_add_kernel<true>, dim3(A.blocks), dim3(256), 0, s, A);
+    else
+        hipLaunchKernelGGL(hot_add_kernel<false>, dim3(A.blocks), dim3(256), 0, s, A);
+    e = hipGetLastError();
+    return e == hipSuccess ? HVX_OK : pfail(ctx, HVX_ERR_HIP, "bright-texel sum: %s", hipGetErrorString(e));
 }
 
 }  // namespace
@@ -1707,8 +1950,40 @@ int hvx_ibl_filter_contract(hvx_contract_plan* P, const float* d_chain, uint32_t
     hvx_ctx* ctx = P->ctx;
     if (!d_chain || !d_out) return pfail(ctx, HVX_ERR_NULL, "NULL chain or output");
     if (((uintptr_t)d_chain | (uintptr_t)d_out) % 16) return pfail(ctx, HVX_ERR_ALIGN, "chains need 16-byte alignment");
+    if (P->hot_k) return launch_hot_call(P, d_chain, chain_levels, d_out, (hipStream_t)stream);
     const Probes Q{reinterpret_cast<const float4*>(d_chain), reinterpret_cast<float4*>(d_out), nullptr, nullptr, 0, 0, 0, 1};
     return launch_call(P, Q, 1, chain_levels, (hipStream_t)stream, "hvx_ibl_filter_contract");
+}
+
+int hvx_ibl_contract_plan_set_hot(hvx_contract_plan* P, uint32_t max_hot) {
+    if (!P) return HVX_ERR_NULL;
+    if (max_hot > hvx::kHotMax)
+        return pfail(P->ctx, HVX_ERR_INVALID_ARG, "max_hot must be in [0, %u]", hvx::kHotMax);
+    if (max_hot && !P->hot_chain) {
+        hipError_t e;
+        if ((e = P->alloc(&P->hot_chain, hvx::cube_level_offset_floats(P->S, P->min_level + 1))) != hipSuccess ||
+            (e = P->alloc(&P->hot_hist, hvx::kHotBins)) != hipSuccess ||
+            (e = P->alloc(&P->hot_cnt, 2)) != hipSuccess || (e = P->alloc(&P->hot_T, 1)) != hipSuccess ||
+            (e = P->alloc(&P->hot_idx, hvx::kHotMax)) != hipSuccess ||
+            (e = P->alloc(&P->hot_list, 2 * hvx::kHotMax)) != hipSuccess ||
+            (e = hipMemset(P->hot_hist, 0, hvx::kHotBins * sizeof(uint32_t))) != hipSuccess ||
+            (e = hipMemset(P->hot_cnt, 0, 2 * sizeof(uint32_t))) != hipSuccess ||
+            (e = hipDeviceSynchronize()) != hipSuccess) {
+            P->hot_chain = nullptr;  // the partial allocations are released with the plan
+            return pfail(P->ctx, HVX_ERR_HIP, "bright-texel split buffers: %s", hipGetErrorString(e));
+        }
+    }
+    P->hot_k = max_hot;
+    return HVX_OK;
+}
+
+int hvx_ibl_contract_hot_info(hvx_contract_plan* P, float* threshold, uint32_t* n_hot) {
+    if (!P || !threshold || !n_hot) return HVX_ERR_NULL;
+    if (!P->hot_chain) return pfail(P->ctx, HVX_ERR_INVALID_ARG, "the bright-texel split was never enabled");
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(threshold, P->hot_T, sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(n_hot, P->hot_cnt + 1, sizeof(uint32_t), hipMemcpyDeviceToHost);
+    return e == hipSuccess ? HVX_OK : pfail(P->ctx, HVX_ERR_HIP, "hvx_ibl_contract_hot_info: %s", hipGetErrorString(e));
 }
 
 int hvx_ibl_contract_plan_reserve(hvx_contract_plan* P, uint32_t max_probes) {
@@ -1740,6 +2015,8 @@ int hvx_ibl_filter_contract_multi(hvx_contract_plan* P, uint32_t n, const float*
     if (n > P->mcap)
         return pfail(P->ctx, HVX_ERR_CAPACITY, "%u probes but scratch reserved for %u (hvx_ibl_contract_plan_reserve)",
                      n, P->mcap);
+    if (P->hot_k)
+        return pfail(P->ctx, HVX_ERR_UNSUPPORTED, "the bright-texel split serves single-probe calls only");
     const Probes Q{nullptr, nullptr, reinterpret_cast<const float4* const*>(d_chains),
                    reinterpret_cast<float4* const*>(d_outs),
                    (uint64_t)(reinterpret_cast<uintptr_t>(P->mscratch) - reinterpret_cast<uintptr_t>(P->scratch)),

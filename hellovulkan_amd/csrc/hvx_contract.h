@@ -109,6 +109,50 @@ HVX_CONTRACT_HD inline void contract_face_point(int f, double u, double v, doubl
     }
 }
 
+// Integral of w over the sphere, 2 pi int_0^1 w(c) dc: pi for the cosine
+// weight; for GGX 2 a^2 I with I = int_0^1 c / (A + b c)^2 dc, A = (1 + a^2)/2,
+// b = (a^2 - 1)/2 (so A + b = a^2): in closed form
+// I = (ln(a^2 / A) + A / a^2 - 1) / b^2, or, where |b / A| is small and that
+// form cancels, the series I = sum_n (n + 1) / (n + 2) (-b / A)^n / A^2.
+HVX_CONTRACT_HD inline double contract_weight_integral(int type, double alpha2) {
+    if (type == CONTRACT_DIFFUSE) return kContractPi;
+    const double A = 0.5 * (1.0 + alpha2), b = 0.5 * (alpha2 - 1.0), r = -b / A;
+    double I;
+    if (r > -0.125 && r < 0.125) {
+        I = 0.0;
+        double p = 1.0;
+        for (int n = 0; n < 40; ++n, p *= r) I += (double)(n + 1) / (double)(n + 2) * p;
+        I /= A * A;
+    } else {
+        I = (log(alpha2 / A) + A / alpha2 - 1.0) / (b * b);
+    }
+    return 2.0 * alpha2 * I;
+}
+
+// ------------------------------------------- bright-texel split (optional)
+// Level 0 is split per channel into min(L, T) + (L - T)+ at a threshold T
+// that at most K texels reach. The contraction filters the clamped part (its
+// box levels rebuilt from the clamped level 0); the excess of those texels is
+// summed exactly over their level-0 texel centres and added to every output
+// texel of the non-delta mips:
+//     hot(o) = sum_k w(n_o . l_k) dw_k (L_k - T)+ / contract_weight_integral.
+// Delta mips sample the unsplit level 0. T: texels with a positive largest
+// channel m are binned by the float pattern of m shifted right by
+// kHotBinShift (monotone in m, bins 2^(1/8) wide); with c(b) the texels in
+// bins >= b, T is the lower edge of the lowest bin b with c(b) <= K, and a
+// texel is hot when m >= T (m > 0). When the brightest occupied bin alone
+// holds more than K texels, T is the edge above it and nothing is hot.
+// Inputs must be finite.
+constexpr uint32_t kHotBinShift = 20;
+constexpr uint32_t kHotBins = 2048;  // patterns of positive floats up to +inf (0x7f800000 >> 20 = 2040)
+constexpr uint32_t kHotMax = 4096;   // largest K
+
+HVX_CONTRACT_HD inline uint32_t contract_hot_bin(uint32_t float_bits) { return float_bits >> kHotBinShift; }
+// T as a float pattern from the first qualifying bin (past the +inf bin: +inf)
+HVX_CONTRACT_HD inline uint32_t contract_hot_threshold_bits(uint32_t first_bin) {
+    return first_bin > 2040u ? 0x7f800000u : first_bin << kHotBinShift;
+}
+
 // Per-mip contraction levels for an input cube of face S (a power of two)
 // and an output chain of face So (a power of two) with `mips` levels. The
 // roughness of mip m is m / (mips - 1) (PipelineCubeFilter.cpp:305-311).

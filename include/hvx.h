@@ -383,6 +383,22 @@ int hvx_ibl_contract_plan_query(const hvx_contract_plan* plan, uint32_t mip, hvx
 int hvx_ibl_filter_contract(hvx_contract_plan* plan, const float* d_chain, uint32_t chain_levels, float* d_out,
                             void* stream);
 
+/* Bright-texel split for single-probe calls (max_hot in [1, 4096]; 0 turns
+ * it off). Level 0 is split per channel into min(L, T) + (L - T)+ at the
+ * threshold T that at most max_hot texels reach (largest channel binned in
+ * steps of 2^(1/8)); the contraction filters the clamped part, and the
+ * excess of those texels is summed exactly over their level-0 texel centres,
+ * w(n.l) dw (L - T)+, over the closed-form integral of w, and added to every
+ * non-delta output texel. A sun-like source then no longer passes through
+ * the box-filtered levels. The first call that enables it allocates the
+ * clamped chain and the lists; filter calls still allocate nothing.
+ * hvx_ibl_filter_contract_multi refuses plans with the split on. No
+ * reference counterpart. */
+int hvx_ibl_contract_plan_set_hot(hvx_contract_plan* plan, uint32_t max_hot);
+/* Threshold and number of hot texels of the plan's last split call (waits
+ * for the device). */
+int hvx_ibl_contract_hot_info(hvx_contract_plan* plan, float* threshold, uint32_t* n_hot);
+
 /* Several probes of the plan's sizes in one call (the same launches, with a
  * probe dimension): d_chains and d_outs are DEVICE arrays of n pointers
  * (inputs as for hvx_ibl_filter_contract). Each probe's result equals the

@@ -740,8 +740,104 @@ void near_sum(const std::vector<double>& lev, uint32_t Si, int type, double alph
     }
 }
 
+// ------------------------------------------------ bright-texel split
+// The optional split of hvx_contract.h: the threshold T that at most K
+// texels reach (largest channel binned by its float pattern), the clamped
+// level 0, and the excess of the hot texels with their level-0 directions
+// and solid angles.
+float hot_largest(const float* t) { return std::fmax(std::fmax(t[0], t[1]), t[2]); }
+
+float hot_threshold(const float* level0, uint32_t S, uint32_t K) {
+    std::vector<uint64_t> hist(hvx::kHotBins, 0);
+    const size_t n = (size_t)6 * S * S;
+    for (size_t i = 0; i < n; ++i) {
+        const float m = hot_largest(level0 + 4 * i);
+        uint32_t bits;
+        std::memcpy(&bits, &m, 4);
+        if (m > 0.0f) ++hist[hvx::contract_hot_bin(bits)];
+    }
+    // c(b) = texels in bins >= b is non-increasing in b: scan from the top
+    uint32_t first = 0;
+    uint64_t above = 0;
+    for (uint32_t b = hvx::kHotBins; b-- > 0;) {
+        above += hist[b];
+        if (above > K) {
+            first = b + 1;
+            break;
+        }
+    }
+    const uint32_t tb = hvx::contract_hot_threshold_bits(first);
+    float T;
+    std::memcpy(&T, &tb, 4);
+    return T;
+}
+
+struct HotTexel {
+    V3<double> l;
+    double dw, e[3];  // solid angle, excess per channel
+};
+
+// clamped level 0 (RGBA) and the hot texels, in texel order
+void hot_split(const float* level0, uint32_t S, float T, std::vector<float>& clamped, std::vector<HotTexel>& hot) {
+    const size_t n = (size_t)6 * S * S;
+    clamped.assign(level0, level0 + 4 * n);
+    hot.clear();
+    for (size_t i = 0; i < n; ++i) {
+        const float* t = level0 + 4 * i;
+        const float m = hot_largest(t);
+        if (!(m > 0.0f && m >= T)) continue;
+        const uint32_t g = (uint32_t)(i / ((size_t)S * S)), r = (uint32_t)(i % ((size_t)S * S));
+        const double u = hvx::contract_coord((int)(r % S), S), v = hvx::contract_coord((int)(r / S), S);
+        HotTexel h;
+        h.l = plane_dir((int)g, u, v);
+        h.dw = hvx::contract_dw(u, v, S);
+        for (int c = 0; c < 3; ++c) {
+            h.e[c] = t[c] > T ? (double)t[c] - (double)T : 0.0;
+            clamped[4 * i + c] = t[c] > T ? T : t[c];
+        }
+        hot.push_back(h);
+    }
+}
+
 void contract_f64(int type, const float* level0, uint32_t S, uint32_t So, uint32_t mips, uint32_t mip,
-                  const uint32_t* ids, uint32_t n_ids, double* out, int threads) {
+                  const uint32_t* ids, uint32_t n_ids, double* out, int threads, uint32_t max_hot = 0);
+
+// The contraction with the bright-texel split: the clamped contraction plus
+// the excess summed at level 0 over the weight's integral (non-delta mips).
+void contract_hot_f64(int type, const float* level0, uint32_t S, uint32_t So, uint32_t mips, uint32_t mip,
+                      const uint32_t* ids, uint32_t n_ids, double* out, int threads, uint32_t K) {
+    hvx::ContractLevel lv[hvx::kContractMaxMips];
+    const int ctype = type == 0 ? hvx::CONTRACT_DIFFUSE : hvx::CONTRACT_SPECULAR;
+    if (!hvx::contract_levels(ctype, S, So, mips, lv) || mip >= mips) return;
+    if (lv[mip].kind == hvx::CONTRACT_DELTA) {
+        contract_f64(type, level0, S, So, mips, mip, ids, n_ids, out, threads);
+        return;
+    }
+    const float T = hot_threshold(level0, S, K);
+    std::vector<float> clamped;
+    std::vector<HotTexel> hot;
+    hot_split(level0, S, T, clamped, hot);
+    contract_f64(type, clamped.data(), S, So, mips, mip, ids, n_ids, out, threads);
+    const double alpha2 = lv[mip].alpha2, inv_z = 1.0 / hvx::contract_weight_integral(ctype, alpha2);
+    const uint32_t Som = lv[mip].So;
+    parallel_for(n_ids, threads, [&](uint32_t i) {
+        const uint32_t id = ids[i], f = id / (Som * Som), o = id % (Som * Som);
+        const V3<double> n = texel_dir<double>((int)f, o % Som, o / Som, Som);
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (const HotTexel& h : hot) {
+            const double w = hvx::contract_weight(ctype, alpha2, dot(n, h.l)) * h.dw;
+            for (int c = 0; c < 3; ++c) acc[c] += w * h.e[c];
+        }
+        for (int c = 0; c < 3; ++c) out[3 * (size_t)i + c] += acc[c] * inv_z;
+    });
+}
+
+void contract_f64(int type, const float* level0, uint32_t S, uint32_t So, uint32_t mips, uint32_t mip,
+                  const uint32_t* ids, uint32_t n_ids, double* out, int threads, uint32_t max_hot) {
+    if (max_hot) {
+        contract_hot_f64(type, level0, S, So, mips, mip, ids, n_ids, out, threads, max_hot);
+        return;
+    }
     hvx::ContractLevel lv[hvx::kContractMaxMips];
     const int ctype = type == 0 ? hvx::CONTRACT_DIFFUSE : hvx::CONTRACT_SPECULAR;
     if (!hvx::contract_levels(ctype, S, So, mips, lv) || mip >= mips) return;
@@ -1243,8 +1339,13 @@ void hvxo_mlp_gradient_f64(const float* P, const float* in, const float* tgt /* 
 
 // ---- contraction filter: FP64 twin and the level-0 accuracy reference
 void hvxo_contract_f64(int type, const float* level0, uint32_t S, uint32_t So, uint32_t mips, uint32_t mip,
-                       const uint32_t* ids, uint32_t n_ids, double* out, int threads) {
-    contract_f64(type, level0, S, So, mips, mip, ids, n_ids, out, threads);
+                       const uint32_t* ids, uint32_t n_ids, double* out, uint32_t max_hot, int threads) {
+    contract_f64(type, level0, S, So, mips, mip, ids, n_ids, out, threads, max_hot);
+}
+// threshold of the bright-texel split over level 0 (hvx_contract.h)
+float hvxo_hot_threshold(const float* level0, uint32_t S, uint32_t max_hot) { return hot_threshold(level0, S, max_hot); }
+double hvxo_weight_integral(int type, double alpha2) {
+    return hvx::contract_weight_integral(type == 0 ? hvx::CONTRACT_DIFFUSE : hvx::CONTRACT_SPECULAR, alpha2);
 }
 void hvxo_ideal_f64(int type, const float* level0, uint32_t S, double alpha2, uint32_t So_mip, const uint32_t* ids,
                     uint32_t n_ids, double* out, int threads) {

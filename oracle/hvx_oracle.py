@@ -46,7 +46,9 @@ _SIGS = [
     ("hvxo_ibl_diffuse_f64", None, [vp, u32, u32, vp, u32, vp, C.c_int]),
     ("hvxo_ibl_specular_f32", None, [vp, u32, u32, u32, u32, u32, C.c_int, vp, u32, vp, C.c_int]),
     ("hvxo_ibl_specular_f64", None, [vp, u32, u32, u32, u32, u32, C.c_int, vp, u32, vp, C.c_int]),
-    ("hvxo_contract_f64", None, [C.c_int, vp, u32, u32, u32, u32, vp, u32, vp, C.c_int]),
+    ("hvxo_contract_f64", None, [C.c_int, vp, u32, u32, u32, u32, vp, u32, vp, u32, C.c_int]),
+    ("hvxo_hot_threshold", C.c_float, [vp, u32, u32]),
+    ("hvxo_weight_integral", C.c_double, [C.c_int, C.c_double]),
     ("hvxo_ideal_f64", None, [C.c_int, vp, u32, C.c_double, u32, vp, u32, vp, C.c_int]),
     ("hvxo_contract_levels", C.c_int, [C.c_int, u32, u32, u32, vp, vp]),
     ("hvxo_mlp_forward_f64", None, [vp, vp, u32, C.c_double, C.c_int, vp]),
@@ -265,15 +267,27 @@ def contract_levels(ftype: int, S: int, So: int, mips: int) -> list[dict]:
 
 
 def contract(ftype: int, level0: np.ndarray, S: int, So: int, mips: int, mip: int, ids=None,
-             threads=None) -> np.ndarray:
-    """FP64 twin of the contraction filter (RGB) at output texels `ids` of mip `mip`."""
+             threads=None, max_hot: int = 0) -> np.ndarray:
+    """FP64 twin of the contraction filter (RGB) at output texels `ids` of mip `mip`;
+    max_hot > 0: with the bright-texel split at that many texels."""
     level0 = _c(level0[: 6 * S * S * 4], np.float32)
     Sm = max(So >> mip, 1)
     ids = np.arange(6 * Sm * Sm, dtype=np.uint32) if ids is None else _c(ids, np.uint32)
     out = np.full((ids.size, 3), np.nan, np.float64)
     lib().hvxo_contract_f64(ftype, level0.ctypes.data, S, So, mips, mip, ids.ctypes.data, ids.size, out.ctypes.data,
-                            _threads(threads))
+                            max_hot, _threads(threads))
     return out
+
+
+def hot_threshold(level0: np.ndarray, S: int, max_hot: int) -> float:
+    """Threshold T of the bright-texel split (+inf: no texel split off)."""
+    level0 = _c(level0[: 6 * S * S * 4], np.float32)
+    return float(lib().hvxo_hot_threshold(level0.ctypes.data, S, max_hot))
+
+
+def weight_integral(ftype: int, alpha2: float) -> float:
+    """Integral of the filter weight over the sphere (closed form / series)."""
+    return float(lib().hvxo_weight_integral(ftype, alpha2))
 
 
 def ideal(ftype: int, level0: np.ndarray, S: int, alpha2: float, So_mip: int, ids, threads=None) -> np.ndarray:

@@ -155,3 +155,127 @@ def test_ideal_weight_is_the_reference_estimand():
             miss[name] = max(miss[name], np.abs(est - wrong).max() / scale)
     for name, e in miss.items():
         assert e > 2e-3, (name, e)
+
+
+# ----------------------------------------------------------- bright-texel split
+def _w(alpha2, c, diffuse=False):
+    c = np.asarray(c, np.float64)
+    if diffuse:
+        return np.where(c > 0, c, 0.0)
+    d = 0.5 * (1.0 + c) * (alpha2 - 1.0) + 1.0
+    return np.where(c > 0, alpha2 / (np.pi * d * d) * c, 0.0)
+
+
+def _dw(S):
+    c = (2.0 * np.arange(S) + 1.0) / S - 1.0
+    u, v = np.meshgrid(c, c)
+    r2 = 1.0 + u * u + v * v
+    return np.tile(((2.0 / S) ** 2 / (r2 * np.sqrt(r2))).ravel(), 6)
+
+
+@pytest.mark.parametrize("alpha2", [1.5e-4, 2.4e-3, 1.23e-2, 0.1, 0.6, 0.9, 0.99, 0.9999, 1.0])
+def test_weight_integral_matches_quadrature(alpha2):
+    """The closed form (series near a^2 = 1) equals 2 pi int_0^1 w(c) dc by
+    adaptive quadrature; the cosine weight integrates to pi."""
+    from scipy.integrate import quad
+    pts = [1.0 - 4.0 * alpha2] if alpha2 < 0.2 else None
+    num = 2.0 * np.pi * quad(lambda c: float(_w(alpha2, c)), 0.0, 1.0, epsabs=0, epsrel=1e-13, limit=500,
+                             points=pts)[0]
+    assert O.weight_integral(1, alpha2) == pytest.approx(num, rel=1e-10)
+    assert O.weight_integral(0, alpha2) == pytest.approx(np.pi, rel=1e-15)
+
+
+def test_weight_integral_matches_level0_sum():
+    """Summed over every texel centre of a 6 x 256^2 level, the weight of a
+    lobe a few texels wide is within 1e-5 of the closed-form integral."""
+    S = 256
+    d, dw = face_dirs(S), _dw(S)
+    rng = np.random.default_rng(3)
+    for alpha2 in (2.4e-3, 1.23e-2, 0.1):
+        for _ in range(3):
+            n = rng.normal(size=3)
+            n /= np.linalg.norm(n)
+            z = float((_w(alpha2, d @ n) * dw).sum())
+            assert z == pytest.approx(O.weight_integral(1, alpha2), rel=1e-5)
+
+
+def _largest(level0):
+    return np.maximum(np.maximum(level0[0::4], level0[1::4]), level0[2::4])
+
+
+@pytest.mark.parametrize("K", [0, 1, 7, 64, 100000])
+def test_hot_threshold_rule(K):
+    """T is the lower edge of the lowest 2^(1/8)-wide bin (float pattern >> 20)
+    such that at most K texels have a largest channel at or above it: at most
+    K texels reach T, and more than K reach the next lower bin edge; 0 when
+    every positive texel fits (K = 0: the edge above the brightest bin)."""
+    S = 16
+    rng = np.random.default_rng(K + 1)
+    lvl = rng.uniform(0.0, 2.0, 6 * S * S * 4).astype(np.float32)
+    lvl[4 * rng.integers(0, 6 * S * S, 40)] = rng.uniform(50.0, 5000.0, 40).astype(np.float32)
+    lvl[4 * np.arange(5)] = 0.0
+    lvl[4 * np.arange(5) + 1] = 0.0
+    lvl[4 * np.arange(5) + 2] = 0.0
+    m = _largest(lvl)
+    T = O.hot_threshold(lvl, S, K)
+    hot = (m > 0) & (m >= T)
+    assert hot.sum() <= K
+    if T == 0.0:
+        assert (m > 0).sum() <= K
+    elif np.isinf(T):
+        top = (m.view(np.uint32) >> 20) == (m.view(np.uint32) >> 20).max()
+        assert top.sum() > K
+    else:
+        tb = np.float32(T).view(np.uint32)
+        assert tb & ((1 << 20) - 1) == 0
+        lower = np.uint32(tb - (1 << 20)).view(np.float32)
+        assert ((m > 0) & (m >= lower)).sum() > K
+
+
+def test_hot_split_with_every_texel_hot_is_the_level0_sum():
+    """When every positive texel is split off (T = 0), the clamped part is zero
+    and the result is the exact level-0 sum over the closed-form integral: the
+    level-0 sum times (discrete normaliser / integral)."""
+    S, So, mips = 16, 8, 4
+    rng = np.random.default_rng(11)
+    lvl = rng.uniform(0.1, 3.0, 6 * S * S * 4).astype(np.float32)
+    lvl[4 * 77] = 800.0
+    assert O.hot_threshold(lvl, S, 4096) == 0.0
+    d, dw = face_dirs(S), _dw(S)
+    lv = O.contract_levels(1, S, So, mips)
+    for m in range(1, mips):
+        Sm = So >> m
+        ids = np.arange(6 * Sm * Sm, dtype=np.uint32)
+        got = O.contract(1, lvl, S, So, mips, m, ids, max_hot=4096)
+        ref = O.ideal(1, lvl, S, lv[m]["alpha2"], Sm, ids)
+        zc = O.weight_integral(1, lv[m]["alpha2"])
+        n_o = face_dirs(Sm)
+        z = (_w(lv[m]["alpha2"], n_o @ d.T) * dw[None, :]).sum(axis=1)
+        np.testing.assert_allclose(got * (zc / z)[:, None], ref, rtol=1e-11)
+    got = O.contract(0, lvl, S, 8, 1, 0, max_hot=4096)
+    ref = O.ideal(0, lvl, S, 0.0, 8, np.arange(6 * 64, dtype=np.uint32))
+    z = (_w(0, face_dirs(8) @ d.T, diffuse=True) * dw[None, :]).sum(axis=1)
+    np.testing.assert_allclose(got * (np.pi / z)[:, None], ref, rtol=1e-11)
+
+
+def test_hot_split_removes_a_point_source_error():
+    """A single texel of radiance 1e5 on a smooth environment: the split
+    contraction agrees with the level-0 sum at least 5x better than the
+    plain contraction around it, and the delta mip is untouched."""
+    S, So, mips = 128, 64, 7
+    lvl = env_from(smooth, S)
+    src = 6 * S * S // 3 + S * 40 + 70
+    lvl[4 * src:4 * src + 3] = 1e5
+    n_src = face_dirs(S)[src]
+    lv = O.contract_levels(1, S, So, mips)
+    for m in (1, 2, 3):
+        Sm = So >> m
+        nd = face_dirs(Sm)
+        ids = np.argsort(-(nd @ n_src))[:24].astype(np.uint32)
+        ref = O.ideal(1, lvl, S, lv[m]["alpha2"], Sm, ids)
+        e_plain = np.abs(O.contract(1, lvl, S, So, mips, m, ids) / ref - 1).max()
+        e_hot = np.abs(O.contract(1, lvl, S, So, mips, m, ids, max_hot=16) / ref - 1).max()
+        assert e_hot * 5 < e_plain, (m, e_hot, e_plain)
+    ids = np.arange(6 * So * So, dtype=np.uint32)
+    np.testing.assert_array_equal(O.contract(1, lvl, S, So, mips, 0, ids, max_hot=16),
+                                  O.contract(1, lvl, S, So, mips, 0, ids))

@@ -224,3 +224,126 @@ def test_contract_multi_equals_single(ctx, precision, nprobe):
     for k in range(nprobe):
         assert torch.equal(multis[k].view(torch.int32), singles[k].view(torch.int32)), f"probe {k}"
     plan.close()
+
+
+# ----------------------------------------------------------- bright-texel split
+def _sun_ids(Sm, n_sun, k=6):
+    """Output texels of an Sm face chain nearest the direction n_sun."""
+    c = (2.0 * np.arange(Sm) + 1.0) / Sm - 1.0
+    u, v = np.meshgrid(c, c)
+    one = np.ones_like(u)
+    faces = [(one, -v, -u), (-one, -v, u), (u, one, v), (u, -one, -v), (u, -v, one), (-u, -v, -one)]
+    d = np.stack([np.stack(f, axis=-1) for f in faces]).reshape(-1, 3)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.argsort(-(d @ n_sun))[:k].astype(np.uint32)
+
+
+SUN = np.array([np.cos(np.radians(35)) * np.cos(np.radians(40)), np.sin(np.radians(35)),
+                np.cos(np.radians(35)) * np.sin(np.radians(40))])
+
+
+def test_contract_hot_split_workload(ctx):
+    """The 2048x1024 -> 1024^2 -> 512^2 x 10 workload with the bright-texel
+    split at 2048 texels: the threshold and hot count equal the oracle's;
+    every mip is within 2e-3 of the split FP64 twin; against the level-0
+    sum (24 random texels and the 6 nearest the sun per rough mip) the split
+    contraction's worst error is no larger than the unsplit one's, and at
+    the texels nearest the sun it is lower on every rough mip; the delta mip
+    is bit-identical to the unsplit call's; graph replays equal the eager
+    call."""
+    import torch
+    S, So, mips, K = 1024, 512, 10, 2048
+    eq = abi.scene_equirect(77, 2048, 1024, threads=16)
+    levels = abi.mip_count(S)
+    cube = torch.empty(abi.cube_chain_floats(S, levels), dtype=torch.float32, device="cuda")
+    ctx.equirect_to_cube(to_dev(eq), 2048, 1024, cube, S)
+    ctx.cube_mips(cube, S, levels)
+    plan = abi.ContractPlan(ctx, abi.FILTER_SPECULAR, S, So, mips)
+    base = torch.empty(abi.cube_chain_floats(So, mips), dtype=torch.float32, device="cuda")
+    plan.filter(cube, levels, base)
+    plan.set_hot(K)
+    hot = torch.full_like(base, np.nan)
+    plan.filter(cube, levels, hot)
+    torch.cuda.synchronize()
+    T, n_hot = plan.hot_info()
+    lvl0 = cube[: 6 * S * S * 4].cpu().numpy()
+    T_ref = O.hot_threshold(lvl0, S, K)
+    m = np.maximum(np.maximum(lvl0[0::4], lvl0[1::4]), lvl0[2::4])
+    assert T == T_ref and n_hot == int(((m > 0) & (m >= T_ref)).sum()) and 0 < n_hot <= K
+    print(f"threshold {T}, hot texels {n_hot}")
+    Hb, Hh = base.cpu().numpy(), hot.cpu().numpy()
+    lv = O.contract_levels(abi.FILTER_SPECULAR, S, So, mips)
+    rng = np.random.default_rng(5)
+    for mi in range(mips):
+        Sm, off = So >> mi, abi.cube_chain_floats(So, mi)
+        n = 6 * Sm * Sm
+        b = Hb[off:off + 4 * n].reshape(-1, 4)[:, :3]
+        h = Hh[off:off + 4 * n].reshape(-1, 4)[:, :3]
+        if lv[mi]["kind"] == 0:
+            assert np.array_equal(b, h)
+            continue
+        sun = _sun_ids(Sm, SUN)
+        ids = np.concatenate([sun, np.setdiff1d(rng.integers(0, n, 24), sun)]).astype(np.uint32)
+        twin = O.contract(abi.FILTER_SPECULAR, lvl0, S, So, mips, mi, ids, threads=16, max_hot=K)
+        e_twin = np.abs(h[ids] - twin) / np.abs(twin)
+        ref = O.ideal(abi.FILTER_SPECULAR, lvl0, S, lv[mi]["alpha2"], Sm, ids, threads=16)
+        eh = np.abs(h[ids] - ref) / np.abs(ref)
+        eb = np.abs(b[ids] - ref) / np.abs(ref)
+        ns = sun.size
+        print(f"mip {mi}: split vs twin {e_twin.max():.2e}; vs level-0 sum: split {eh.max():.2e} "
+              f"(median {np.median(eh):.1e}, at the sun {eh[:ns].max():.2e}), unsplit {eb.max():.2e} "
+              f"(at the sun {eb[:ns].max():.2e})")
+        assert e_twin.max() <= TOL
+        assert eh.max() <= eb.max() * (1 + 1e-3)
+        assert eh[:ns].max() < eb[:ns].max()
+    # graph replays of the split call equal the eager result
+    s = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    out_g = torch.full_like(base, np.nan)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g, stream=s):
+        plan.filter(cube, levels, out_g, stream=s)
+    for _ in range(3):
+        out_g.fill_(np.nan)
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out_g, hot)
+        plan.filter(cube, levels, base)  # eager work between replays
+        torch.cuda.synchronize()
+    plan.close()
+
+
+def test_contract_hot_split_arguments(ctx):
+    """set_hot accepts 0..4096; multi-probe calls are refused while it is on
+    and accepted again after it is turned off; a small diffuse plan with the
+    split stays within 2e-3 of its split twin."""
+    import torch
+    S, So, mips = 128, 64, 7
+    plan = abi.ContractPlan(ctx, abi.FILTER_SPECULAR, S, So, mips)
+    with pytest.raises(abi.HvxError):
+        plan.set_hot(4097)
+    with pytest.raises(abi.HvxError):
+        plan.hot_info()
+    plan.reserve(2)
+    chain, levels = _env_chain(ctx, 21, S)
+    outs = [torch.empty(abi.cube_chain_floats(So, mips), dtype=torch.float32, device="cuda") for _ in range(2)]
+    cp = torch.tensor([chain.data_ptr()] * 2, dtype=torch.int64, device="cuda")
+    op = torch.tensor([o.data_ptr() for o in outs], dtype=torch.int64, device="cuda")
+    plan.set_hot(64)
+    with pytest.raises(abi.HvxError):
+        plan.filter_multi(2, cp, levels, op)
+    plan.set_hot(0)
+    plan.filter_multi(2, cp, levels, op)
+    torch.cuda.synchronize()
+    plan.close()
+    dplan = abi.ContractPlan(ctx, abi.FILTER_DIFFUSE, S, 32, 1)
+    dplan.set_hot(256)
+    out = torch.full((abi.cube_chain_floats(32, 1),), np.nan, dtype=torch.float32, device="cuda")
+    dplan.filter(chain, levels, out)
+    torch.cuda.synchronize()
+    lvl0 = chain[: 6 * S * S * 4].cpu().numpy()
+    ids = np.arange(6 * 32 * 32, dtype=np.uint32)
+    twin = O.contract(abi.FILTER_DIFFUSE, lvl0, S, 32, 1, 0, ids, max_hot=256)
+    got = out.cpu().numpy().reshape(-1, 4)[:, :3]
+    assert (np.abs(got - twin) / np.abs(twin)).max() <= TOL
+    dplan.close()
