@@ -1159,10 +1159,11 @@ struct HotArgs {
     float4* clamped;       // level 0 of the clamped chain
     uint32_t S, K;
     uint32_t* hist;        // kHotBins counts, zero between calls
-    uint32_t* cnt;         // [0] hot texels found, [1] hot texels listed
+    uint32_t* cnt;         // [0] hot texels found, [1] hot texels listed, [2] blocks
     float* T;
-    uint32_t* idx;         // kHotMax texel indices
+    uint32_t* idx;         // kHotMax texel indices; after the sort, the member range of each block
     float4* list;          // per listed texel: (direction, 0), (solid angle * excess, 0)
+    float4* groups;        // per 4x4 block: (mean direction, radius), (summed excess, member range)
 };
 
 __device__ __forceinline__ float hot_largest(float4 t) { return fmaxf(fmaxf(t.x, t.y), t.z); }
@@ -1171,10 +1172,30 @@ __global__ __launch_bounds__(256) void hot_hist_kernel(HotArgs H) {
     __shared__ uint32_t sh[hvx::kHotBins];
     for (uint32_t b = threadIdx.x; b < hvx::kHotBins; b += 256) sh[b] = 0u;
     __syncthreads();
-    const size_t n = 6ull * H.S * H.S;
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
-        const float m = hot_largest(H.level0[i]);
-        if (m > 0.0f) atomicAdd(&sh[hvx::contract_hot_bin(__float_as_uint(m))], 1u);
+    const size_t n = 6ull * H.S * H.S, stride = (size_t)gridDim.x * 256;
+    const size_t i0 = (size_t)blockIdx.x * 256 + threadIdx.x;
+    // four texels in flight per thread; the bins are then counted one by one
+    for (size_t i = i0; i < n; i += 4 * stride) {
+        float m4[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const size_t k = i + q * stride;
+            m4[q] = k < n ? hot_largest(H.level0[k]) : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float m = m4[q];
+            const uint32_t bin = m > 0.0f ? hvx::contract_hot_bin(__float_as_uint(m)) : 0xffffffffu;
+            // neighbouring texels mostly share a bin: one LDS atomic per wave then
+            const uint32_t b0 = __builtin_amdgcn_readfirstlane(bin);
+            if (__all(bin == b0)) {
+                const unsigned long long act = __ballot(1);
+                if (b0 != 0xffffffffu && (threadIdx.x & 63) == (uint32_t)__ffsll(act) - 1)
+                    atomicAdd(&sh[b0], (uint32_t)__popcll(act));
+            } else if (bin != 0xffffffffu) {
+                atomicAdd(&sh[bin], 1u);
+            }
+        }
     }
     __syncthreads();
     for (uint32_t b = threadIdx.x; b < hvx::kHotBins; b += 256)
@@ -1207,6 +1228,7 @@ __global__ __launch_bounds__(1024) void hot_select_kernel(HotArgs H) {
         H.T[0] = __uint_as_float(hvx::contract_hot_threshold_bits((uint32_t)over));
         H.cnt[0] = 0u;
         H.cnt[1] = 0u;
+        H.cnt[2] = 0u;
     }
 }
 
@@ -1226,13 +1248,26 @@ __global__ __launch_bounds__(256) void hot_compact_kernel(HotArgs H) {
     H.clamped[i] = v;
 }
 
-// One workgroup of 1024: sorts the found indices (bitonic, in LDS) so the
-// sum order is the texel order, then writes each texel's direction and
-// solid angle times excess (FP64 geometry, as the oracle's level-0 sum).
+// One workgroup of 1024: sorts the found texels by (face, 4x4 block, texel
+// in block) so that the sum order is fixed and each 4x4 block's texels are
+// contiguous, then writes each texel's direction and solid angle times
+// excess (FP64 geometry, as the oracle's level-0 sum) and the member range
+// of every block.
+__device__ __forceinline__ uint32_t hot_key(uint32_t i, uint32_t S) {
+    const uint32_t S2 = S * S, g = i / S2, r = i - g * S2, y = r / S, x = r - y * S, Q = S >> 2;
+    return (((g * Q + (y >> 2)) * Q + (x >> 2)) << 4) | ((y & 3u) << 2) | (x & 3u);
+}
+__device__ __forceinline__ uint32_t hot_index(uint32_t key, uint32_t S) {
+    const uint32_t Q = S >> 2, blk = key >> 4, bx = blk % Q, by = (blk / Q) % Q, g = blk / (Q * Q);
+    const uint32_t y = (by << 2) | ((key >> 2) & 3u), x = (bx << 2) | (key & 3u);
+    return (g * S + y) * S + x;
+}
+
 __global__ __launch_bounds__(1024) void hot_list_kernel(HotArgs H) {
     __shared__ uint32_t s[hvx::kHotMax];
-    const uint32_t n = min(H.cnt[0], H.K), t = threadIdx.x;
-    for (uint32_t j = t; j < hvx::kHotMax; j += 1024) s[j] = j < n ? H.idx[j] : 0xffffffffu;
+    __shared__ uint32_t s_wave[16];
+    const uint32_t n = min(H.cnt[0], H.K), t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    for (uint32_t j = t; j < hvx::kHotMax; j += 1024) s[j] = j < n ? hot_key(H.idx[j], H.S) : 0xffffffffu;  // S >= 4
     __syncthreads();
     for (uint32_t size = 2; size <= hvx::kHotMax; size <<= 1)
         for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
@@ -1250,7 +1285,7 @@ __global__ __launch_bounds__(1024) void hot_list_kernel(HotArgs H) {
     const float T = H.T[0];
     const uint32_t S2 = H.S * H.S;
     for (uint32_t k = t; k < n; k += 1024) {
-        const uint32_t i = s[k], g = i / S2, r = i - g * S2, y = r / H.S, x = r - y * H.S;
+        const uint32_t i = hot_index(s[k], H.S), g = i / S2, r = i - g * S2, y = r / H.S, x = r - y * H.S;
         const double u = hvx::contract_coord((int)x, H.S), v = hvx::contract_coord((int)y, H.S);
         double p[3];
         hvx::contract_face_point((int)g, u, v, p);
@@ -1260,11 +1295,73 @@ __global__ __launch_bounds__(1024) void hot_list_kernel(HotArgs H) {
         H.list[2 * k] = make_float4((float)(p[0] * rl), (float)(p[1] * rl), (float)(p[2] * rl), 0.0f);
         H.list[2 * k + 1] = make_float4((float)(dw * ex), (float)(dw * ey), (float)(dw * ez), 0.0f);
     }
-    if (t == 0) H.cnt[1] = n;
+    // block starts: thread t owns entries 4t .. 4t+3; exclusive scan of the counts
+    uint32_t starts = 0;
+    for (uint32_t q = 0; q < 4; ++q) {
+        const uint32_t k = 4 * t + q;
+        starts += (k < n && (k == 0 || (s[k] >> 4) != (s[k - 1] >> 4))) ? 1u : 0u;
+    }
+    uint32_t inc = starts;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t u = __shfl_up(inc, d, 64);
+        if ((int)lane >= d) inc += u;
+    }
+    if (lane == 63) s_wave[wv] = inc;
+    __syncthreads();
+    uint32_t gi = inc - starts;
+    for (uint32_t w = 0; w < wv; ++w) gi += s_wave[w];
+    for (uint32_t q = 0; q < 4; ++q) {
+        const uint32_t k = 4 * t + q;
+        if (!(k < n && (k == 0 || (s[k] >> 4) != (s[k - 1] >> 4)))) continue;
+        uint32_t e = k + 1;
+        while (e < n && (s[e] >> 4) == (s[k] >> 4)) ++e;
+        H.idx[gi++] = k | ((e - k) << 16);  // the found indices are consumed: idx now holds block ranges
+    }
+    if (t == 1023) {
+        H.cnt[1] = n;
+        H.cnt[2] = gi;
+    }
+}
+
+// One thread per block of hot texels: excess-weighted mean direction, the
+// angular radius of the members around it, the summed excess.
+__global__ __launch_bounds__(256) void hot_group_kernel(HotArgs H) {
+    const uint32_t gi = blockIdx.x * 256 + threadIdx.x;
+    if (gi >= H.cnt[2]) return;
+    const uint32_t pk = H.idx[gi], k = pk & 0xffffu, e = k + (pk >> 16);
+    double E[3] = {0.0, 0.0, 0.0}, m[3] = {0.0, 0.0, 0.0};
+    for (uint32_t j = k; j < e; ++j) {
+        const float4 L = H.list[2 * j], X = H.list[2 * j + 1];
+        const double wgt = (double)X.x + (double)X.y + (double)X.z;
+        E[0] += X.x;
+        E[1] += X.y;
+        E[2] += X.z;
+        m[0] += wgt * L.x;
+        m[1] += wgt * L.y;
+        m[2] += wgt * L.z;
+    }
+    double ml = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+    if (!(ml > 0.0)) {  // no excess in the block: any member direction
+        const float4 L = H.list[2 * k];
+        m[0] = L.x;
+        m[1] = L.y;
+        m[2] = L.z;
+        ml = 1.0;
+    }
+    double rad = 0.0;
+    for (uint32_t j = k; j < e; ++j) {
+        const float4 L = H.list[2 * j];
+        const double c = (m[0] * L.x + m[1] * L.y + m[2] * L.z) / ml;
+        rad = fmax(rad, acos(fmin(1.0, fmax(-1.0, c))));
+    }
+    H.groups[2 * gi] = make_float4((float)(m[0] / ml), (float)(m[1] / ml), (float)(m[2] / ml),
+                                   (float)(rad * (1.0 + 1e-6) + 1e-7));
+    H.groups[2 * gi + 1] = make_float4((float)E[0], (float)E[1], (float)E[2], __uint_as_float(pk));
 }
 
 struct HotMip {
-    uint32_t So, first;  // output face; first thread of the mip (a multiple of 256: one mip per workgroup)
+    uint32_t So, first;  // output face; first output of the mip in the launch (a multiple of 64: one mip per workgroup)
     float alpha2, h1;    // GGX a^2 and (1 - a^2) / 2
     float scale;         // a^2 / pi / (weight integral); diffuse: 1 / pi
     uint32_t _pad;
@@ -1274,52 +1371,147 @@ struct HotAdd {
     HotMip m[hvx::kContractMaxMips];
     uint32_t nm, blocks;
     const float4* list;
+    const float4* groups;
     const uint32_t* cnt;
     float4* out;
+    float theta_near;  // blocks farther than this (radians) from every output direction may enter as one entry
 };
 
-// out(o) += scale * sum_k w'(n_o . l_k) E_k over the listed hot texels, with
-// w' = c / d^2 (GGX, d = a^2 + (1 - c)(1 - a^2)/2 = N.H^2 (a^2 - 1) + 1) or
-// c (cosine), 0 for c <= 0; 1 - c is formed as |n - l|^2 / 2, which keeps its
-// relative accuracy for nearby directions. The list is staged through LDS.
+// w'(n . l) E: w' = c / d^2 (GGX, d = a^2 + (1 - c)(1 - a^2)/2 = N.H^2 (a^2 - 1)
+// + 1) or c (cosine), 0 for c <= 0; 1 - c is formed as |n - l|^2 / 2, which
+// keeps its relative accuracy for nearby directions.
+template <bool DIFFUSE>
+__device__ __forceinline__ void hot_term(const f3& n, const float4 L, const float4 E, float a2, float h1, float& ax,
+                                         float& ay, float& az) {
+    const float dx = n.x - L.x, dy = n.y - L.y, dz = n.z - L.z;
+    const float q = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));  // 2 (1 - c)
+    const float c = __builtin_fmaf(q, -0.5f, 1.0f);
+    float w;
+    if constexpr (DIFFUSE) {
+        w = fmaxf(c, 0.0f);
+    } else {
+        const float d = __builtin_fmaf(q, h1, a2);  // h1 holds (1 - a^2) / 4 here
+        w = fmaxf(c, 0.0f) * __builtin_amdgcn_rcpf(d * d);
+    }
+    ax = __builtin_fmaf(w, E.x, ax);
+    ay = __builtin_fmaf(w, E.y, ay);
+    az = __builtin_fmaf(w, E.z, az);
+}
+
+// out(o) += scale * sum over the hot texels of w'(n_o . l_k) E_k. A workgroup
+// holds 64 output texels of one mip. A 4x4 block of hot texels whose cone
+// lies more than theta_near outside the workgroup's cone of output
+// directions, and wholly in front of every one of them (c > 0: clear of the
+// kink of w at c = 0), enters as one entry at its excess-weighted mean
+// direction (the terms linear in the offsets cancel; the rest is of relative
+// order 10 (block extent / distance)^2); a block wholly behind every output
+// direction adds exactly 0 and is skipped; every other block is summed texel
+// by texel.
 template <bool DIFFUSE>
 __global__ __launch_bounds__(256) void hot_add_kernel(HotAdd A) {
-    constexpr uint32_t kChunk = 512;
-    __shared__ float4 sl[2 * kChunk];
-    const uint32_t b0 = blockIdx.x * 256;
+    constexpr uint32_t kG = 32, kE = kG * 16;  // blocks per round; entries they can expand to
+    __shared__ float4 se[2 * kE];
+    __shared__ uint32_t s_ne;
+    __shared__ float s_acc[3][3][64];
+    // 64 output texels per workgroup (an 8 x 8 tile of one face when the face
+    // has at least 8 texels), one per lane; the four waves split the entries
+    const uint32_t b0 = blockIdx.x * 64, t = threadIdx.x, lane = t & 63, wv = t >> 6;
     uint32_t mi = 0;
     while (mi + 1 < A.nm && b0 >= A.m[mi + 1].first) ++mi;  // uniform over the workgroup
     const HotMip M = A.m[mi];
-    const uint32_t So2 = M.So * M.So, loc0 = b0 + threadIdx.x - M.first;
+    const uint32_t So2 = M.So * M.So, loc0 = b0 + lane - M.first;
     const bool live = loc0 < 6 * So2;
-    const uint32_t loc = live ? loc0 : 0u, f = loc / So2, o = loc - f * So2, Y = o / M.So, X = o - Y * M.So;
-    const f3 n = texel_dir((int)f, (int)X, (int)Y, (int)M.So);
-    float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    const uint32_t nh = A.cnt[1];
-    for (uint32_t base = 0; base < nh; base += kChunk) {
-        const uint32_t cnt = min(kChunk, nh - base);
-        __syncthreads();
-        for (uint32_t j = threadIdx.x; j < 2 * cnt; j += 256) sl[j] = A.list[2 * base + j];
-        __syncthreads();
-        for (uint32_t j = 0; j < cnt; ++j) {
-            const float4 L = sl[2 * j], E = sl[2 * j + 1];
-            const float dx = n.x - L.x, dy = n.y - L.y, dz = n.z - L.z;
-            const float omc = 0.5f * __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-            const float c = 1.0f - omc;
-            float w;
-            if constexpr (DIFFUSE) {
-                w = c;
-            } else {
-                const float d = __builtin_fmaf(omc, M.h1, M.alpha2);
-                w = c * __builtin_amdgcn_rcpf(d * d);
-            }
-            w = c > 0.0f ? w : 0.0f;
-            ax = __builtin_fmaf(w, E.x, ax);
-            ay = __builtin_fmaf(w, E.y, ay);
-            az = __builtin_fmaf(w, E.z, az);
+    uint32_t f, X, Y;
+    {
+        const uint32_t l = live ? loc0 : 0u;
+        f = l / So2;
+        const uint32_t w = l - f * So2;
+        if (M.So >= 8) {  // So^2 is a multiple of 64
+            const uint32_t te = M.So >> 3, tile = w >> 6, e = w & 63u;
+            X = (tile % te) * 8 + (e & 7u);
+            Y = (tile / te) * 8 + (e >> 3);
+        } else {
+            Y = w / M.So;
+            X = w - Y * M.So;
         }
     }
-    if (!live) return;
+    const uint32_t loc = (f * M.So + Y) * M.So + X;
+    const f3 n = texel_dir((int)f, (int)X, (int)Y, (int)M.So);
+    // the workgroup's cone (every wave holds the same 64 directions): mean
+    // direction of the live texels, smallest cosine to it
+    float cx = live ? n.x : 0.0f, cy = live ? n.y : 0.0f, cz = live ? n.z : 0.0f;
+    for (int d = 32; d > 0; d >>= 1) {
+        cx += __shfl_xor(cx, d, 64);
+        cy += __shfl_xor(cy, d, 64);
+        cz += __shfl_xor(cz, d, 64);
+    }
+    const float cl = sqrtf(cx * cx + cy * cy + cz * cz);
+    const bool cone = cl > 1e-3f;  // else: directions all round the sphere, no cone
+    if (cone) {
+        cx /= cl;
+        cy /= cl;
+        cz /= cl;
+    }
+    float cmin = live ? n.x * cx + n.y * cy + n.z * cz : 1.0f;
+    for (int d = 32; d > 0; d >>= 1) cmin = fminf(cmin, __shfl_xor(cmin, d, 64));
+    // radius of the output cone, rounded up (a few 1e-6 rad)
+    const float rad_o = acosf(fmaxf(-1.0f, fminf(1.0f, cmin))) * 1.0001f + 1e-5f;
+    const float kHalfPi = 1.57079633f;
+    const float h1 = DIFFUSE ? 0.0f : 0.5f * M.h1;
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    const uint32_t ng = A.cnt[2];
+    for (uint32_t base = 0; base < ng; base += kG) {
+        const uint32_t cg = min(kG, ng - base);
+        __syncthreads();  // the previous round's entries are consumed
+        if (wv == 0) {
+            // the first wave sorts this round's blocks into skipped (0 entries),
+            // aggregated (1) and summed texel by texel (its members), and
+            // expands them into the entry list at their prefix offsets
+            uint32_t mode = 2u, cnt = 0u, pk = 0u;
+            float4 G0 = make_float4(0.f, 0.f, 0.f, 0.f), G1 = G0;
+            if (lane < cg) {
+                G0 = A.groups[2 * (base + lane)];
+                G1 = A.groups[2 * (base + lane) + 1];
+                pk = __float_as_uint(G1.w);
+                const float ang = acosf(fmaxf(-1.0f, fminf(1.0f, G0.x * cx + G0.y * cy + G0.z * cz)));
+                const float lo = ang * 0.9999f - 1e-5f - rad_o - G0.w, hi = ang * 1.0001f + 1e-5f + rad_o + G0.w;
+                mode = !cone ? 0u : lo > kHalfPi + 1e-4f ? 2u : (lo > A.theta_near && hi < kHalfPi - 1e-4f) ? 1u : 0u;
+                cnt = mode == 2u ? 0u : mode == 1u ? 1u : (pk >> 16);
+            }
+            uint32_t inc = cnt;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t u = __shfl_up(inc, d, 64);
+                if ((int)lane >= d) inc += u;
+            }
+            const uint32_t off = inc - cnt;
+            if (lane == 63) s_ne = inc;
+            if (mode == 1u) {
+                se[2 * off] = G0;
+                se[2 * off + 1] = G1;
+            } else if (mode == 0u) {
+                const uint32_t k0 = pk & 0xffffu;
+                for (uint32_t j = 0; j < cnt; ++j) {
+                    se[2 * (off + j)] = A.list[2 * (k0 + j)];
+                    se[2 * (off + j) + 1] = A.list[2 * (k0 + j) + 1];
+                }
+            }
+        }
+        __syncthreads();
+        const uint32_t ne = s_ne;
+        for (uint32_t j = wv; j < ne; j += 4) hot_term<DIFFUSE>(n, se[2 * j], se[2 * j + 1], M.alpha2, h1, ax, ay, az);
+    }
+    // the waves' partial sums, added in wave order
+    if (wv > 0) {
+        s_acc[0][wv - 1][lane] = ax;
+        s_acc[1][wv - 1][lane] = ay;
+        s_acc[2][wv - 1][lane] = az;
+    }
+    __syncthreads();
+    if (wv != 0 || !live) return;
+    ax = ((ax + s_acc[0][0][lane]) + s_acc[0][1][lane]) + s_acc[0][2][lane];
+    ay = ((ay + s_acc[1][0][lane]) + s_acc[1][1][lane]) + s_acc[1][2][lane];
+    az = ((az + s_acc[2][0][lane]) + s_acc[2][1][lane]) + s_acc[2][2][lane];
     float4& r = A.out[M.out_off + loc];
     r.x = __builtin_fmaf(ax, M.scale, r.x);
     r.y = __builtin_fmaf(ay, M.scale, r.y);
@@ -1514,6 +1706,7 @@ struct hvx_contract_plan {
     float* hot_T = nullptr;
     uint32_t* hot_idx = nullptr;
     float4* hot_list = nullptr;
+    float4* hot_groups = nullptr;
     std::vector<void*> allocs;
 
     template <class T> hipError_t alloc(T** p, size_t count) {
@@ -1687,6 +1880,13 @@ int launch_call(hvx_contract_plan* P, const Probes& Q, uint32_t n, uint32_t chai
     return e == hipSuccess ? HVX_OK : pfail(ctx, HVX_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
+// Hot-texel blocks at least max(0.5, 300 level-0 texel widths) radians
+// outside a workgroup's cone of output directions enter its sum as one
+// entry: the relative error of that entry is about 10 (block extent /
+// distance)^2, under 2e-4 for 4x4 level-0 blocks (rms extent about 1.3
+// texel widths); faces below 1024 therefore aggregate little or nothing.
+float hot_near(uint32_t S) { return std::max(0.5f, 300.0f * 1.5707963f / (float)S); }
+
 // A single-probe call with the bright-texel split: threshold, clamped level 0
 // and its box levels, the hot list, the contraction of the clamped chain (the
 // delta mip samples the caller's chain), then the hot texels' exact sum.
@@ -1699,13 +1899,14 @@ int launch_hot_call(hvx_contract_plan* P, const float* d_chain, uint32_t chain_l
     if (hipGetDevice(&cur) != hipSuccess || (cur != ctx->device && hipSetDevice(ctx->device) != hipSuccess))
         return pfail(ctx, HVX_ERR_HIP, "cannot bind device %d", ctx->device);
     const HotArgs H{reinterpret_cast<const float4*>(d_chain), reinterpret_cast<float4*>(P->hot_chain), P->S,
-                    P->hot_k, P->hot_hist, P->hot_cnt, P->hot_T, P->hot_idx, P->hot_list};
+                    P->hot_k, P->hot_hist, P->hot_cnt, P->hot_T, P->hot_idx, P->hot_list, P->hot_groups};
     const size_t texels = 6ull * P->S * P->S;
-    hipLaunchKernelGGL(hot_hist_kernel, dim3((uint32_t)std::min<size_t>(2048, (texels + 255) / 256)), dim3(256), 0, s,
-                       H);
+    hipLaunchKernelGGL(hot_hist_kernel, dim3((uint32_t)std::min<size_t>(1024, (texels + 1023) / 1024)), dim3(256), 0,
+                       s, H);
     hipLaunchKernelGGL(hot_select_kernel, dim3(1), dim3(1024), 0, s, H);
     hipLaunchKernelGGL(hot_compact_kernel, dim3((uint32_t)((texels + 255) / 256)), dim3(256), 0, s, H);
     hipLaunchKernelGGL(hot_list_kernel, dim3(1), dim3(1024), 0, s, H);
+    hipLaunchKernelGGL(hot_group_kernel, dim3((hvx::kHotMax + 255) / 256), dim3(256), 0, s, H);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return pfail(ctx, HVX_ERR_HIP, "bright-texel split: %s", hipGetErrorString(e));
     if (P->min_level > 0 && (e = hvx::launch_cube_mips(P->hot_chain, P->S, P->min_level + 1, s)) != hipSuccess)
@@ -1728,12 +1929,14 @@ int launch_hot_call(hvx_contract_plan* P, const float* d_chain, uint32_t chain_l
         const double z = hvx::contract_weight_integral(ctype, L.alpha2);
         M.scale = (float)((ctype == hvx::CONTRACT_DIFFUSE ? 1.0 : L.alpha2 / hvx::kContractPi) / z);
         M.out_off = hvx::cube_level_offset_floats(P->So, m) / 4;
-        first += (6 * L.So * L.So + 255) / 256 * 256;
+        first += (6 * L.So * L.So + 63) / 64 * 64;
     }
     if (A.nm == 0) return HVX_OK;
-    A.blocks = first / 256;
+    A.blocks = first / 64;
     A.list = P->hot_list;
+    A.groups = P->hot_groups;
     A.cnt = P->hot_cnt;
+    A.theta_near = hot_near(P->S);
     A.out = reinterpret_cast<float4*>(d_out);
     if (ctype == hvx::CONTRACT_DIFFUSE)
         hipLaunchKernelGGL(hot_add_kernel<true>, dim3(A.blocks), dim3(256), 0, s, A);
@@ -1963,11 +2166,12 @@ int hvx_ibl_contract_plan_set_hot(hvx_contract_plan* P, uint32_t max_hot) {
         hipError_t e;
         if ((e = P->alloc(&P->hot_chain, hvx::cube_level_offset_floats(P->S, P->min_level + 1))) != hipSuccess ||
             (e = P->alloc(&P->hot_hist, hvx::kHotBins)) != hipSuccess ||
-            (e = P->alloc(&P->hot_cnt, 2)) != hipSuccess || (e = P->alloc(&P->hot_T, 1)) != hipSuccess ||
+            (e = P->alloc(&P->hot_cnt, 3)) != hipSuccess || (e = P->alloc(&P->hot_T, 1)) != hipSuccess ||
             (e = P->alloc(&P->hot_idx, hvx::kHotMax)) != hipSuccess ||
             (e = P->alloc(&P->hot_list, 2 * hvx::kHotMax)) != hipSuccess ||
+            (e = P->alloc(&P->hot_groups, 2 * hvx::kHotMax)) != hipSuccess ||
             (e = hipMemset(P->hot_hist, 0, hvx::kHotBins * sizeof(uint32_t))) != hipSuccess ||
-            (e = hipMemset(P->hot_cnt, 0, 2 * sizeof(uint32_t))) != hipSuccess ||
+            (e = hipMemset(P->hot_cnt, 0, 3 * sizeof(uint32_t))) != hipSuccess ||
             (e = hipDeviceSynchronize()) != hipSuccess) {
             P->hot_chain = nullptr;  // the partial allocations are released with the plan
             return pfail(P->ctx, HVX_ERR_HIP, "bright-texel split buffers: %s", hipGetErrorString(e));

@@ -50,7 +50,12 @@ constexpr double kContractPi = 3.14159265358979323846;
 constexpr uint32_t kContractMaxMips = 16;
 // largest dense (FULL) weight matrix of one mip, in entries, before the
 // near/far split is used instead
-constexpr double kContractFullBudget = 33554432.0;  // 2^25
+constexpr double kContractFullBudget = 134217728.0;  // 2^27
+// coarsest input level of a specular mip: the GGX tails that carry a few
+// percent of the weight out to a radian are summed against texels of at
+// most pi/128 (measured against the level-0 sum on the sun-and-sky
+// environment: mip 3 of 512^2 x 10 within 1.5e-3 at 64^2, 5.8e-3 at 32^2)
+constexpr uint32_t kContractMinIn = 64;
 constexpr uint32_t kContractTile = 16;               // near-field output tile edge
 
 struct ContractLevel {
@@ -176,7 +181,7 @@ inline uint32_t contract_levels(int type, uint32_t S, uint32_t So, uint32_t mips
         }
         // input texels of about half the lobe's half width (1.29 alpha)
         uint32_t Si = contract_pow2_at_least((kContractPi / 2.0) / (0.65 * alpha));
-        Si = Si < 32 ? 32 : Si;
+        Si = Si < kContractMinIn ? kContractMinIn : Si;
         Si = Si > S ? S : Si;
         L.Si = Si;
         const double full = (double)L.So * L.So * 6.0 * Si * Si;
@@ -185,9 +190,11 @@ inline uint32_t contract_levels(int type, uint32_t S, uint32_t So, uint32_t mips
         Sf = Sf > Si / 2 ? Si / 2 : Sf;
         L.kind = CONTRACT_SPLIT;
         L.Sf = Sf;
-        const double df = (kContractPi / 2.0) / (double)Sf;  // mean angular texel size of the far level
-        L.t0 = 2.5 * df;
-        L.t1 = 5.0 * df;
+        // the window ends 20 mean near-level texels out (10 to 20 far texels
+        // at Sf = Si / 4): the far level then only sees the tail past it
+        const double di = (kContractPi / 2.0) / (double)Si;  // mean angular texel size of the near level
+        L.t0 = 10.0 * di;
+        L.t1 = 20.0 * di;
     }
     return mips;
 }

@@ -95,8 +95,8 @@ def test_full_sum_matches_level0_integral_for_rough_lobes():
 
 
 def test_split_agrees_with_ideal_on_smooth_environment():
-    """The near/far split (mip 1 of 128 -> 64 x 7) against the level-0 sum."""
-    S, So, mips = 128, 64, 7
+    """The near/far split (mip 1 of 256 -> 128 x 8) against the level-0 sum."""
+    S, So, mips = 256, 128, 8
     lv = O.contract_levels(1, S, So, mips)
     assert lv[1]["kind"] == O.CONTRACT_SPLIT
     lvl = env_from(smooth, S)
@@ -261,8 +261,9 @@ def test_hot_split_with_every_texel_hot_is_the_level0_sum():
 def test_hot_split_removes_a_point_source_error():
     """A single texel of radiance 1e5 on a smooth environment: the split
     contraction agrees with the level-0 sum at least 5x better than the
-    plain contraction around it, and the delta mip is untouched."""
-    S, So, mips = 128, 64, 7
+    plain contraction around it (on mips that sum coarser levels than level
+    0), and the delta mip is untouched."""
+    S, So, mips = 256, 128, 8
     lvl = env_from(smooth, S)
     src = 6 * S * S // 3 + S * 40 + 70
     lvl[4 * src:4 * src + 3] = 1e5

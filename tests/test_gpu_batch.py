@@ -217,10 +217,13 @@ def test_batch_rejects_bad_arguments(ctx, scene):
 CONTRACT_PROBES = [(16, 8, 4), (32, 16, 5), (64, 32, 6), (64, 32, 6), (128, 64, 7), (32, 16, 5)]
 # classes of 6 and 9 probes: the multi-probe GEMMs' second and third probe groups
 CONTRACT_LARGE_CLASSES = [(32, 16, 5)] * 6 + [(64, 32, 6)] * 9
+# 256^2 -> 128^2 x 8: mip 1 is a near/far mip (five probes, two groups)
+CONTRACT_SPLIT_CLASS = [(256, 128, 8)] * 5
 
 
 @pytest.mark.parametrize("precision", [abi.PRECISION_FP16_SCALED, abi.PRECISION_BF16_SPLIT])
-@pytest.mark.parametrize("probes", [CONTRACT_PROBES, CONTRACT_LARGE_CLASSES], ids=["mixed", "large-classes"])
+@pytest.mark.parametrize("probes", [CONTRACT_PROBES, CONTRACT_LARGE_CLASSES, CONTRACT_SPLIT_CLASS],
+                         ids=["mixed", "large-classes", "near-far-class"])
 def test_batch_contraction_mode_matches_per_probe(ctx, precision, probes):
     """Contraction mode: every probe equals the per-probe contraction filter
     (its plan over the probe's level 0 and box mips) bit for bit on repeated

@@ -60,12 +60,12 @@ def _run(ctx, ftype, S, So, mips, sub, seed=13, precision=abi.PRECISION_FP16_SCA
     return worst
 
 
-@pytest.mark.parametrize("S,So,mips,sub", [(128, 64, 7, 3000), (64, 32, 6, None)])
+@pytest.mark.parametrize("S,So,mips,sub", [(256, 128, 8, 3000), (128, 64, 7, 3000), (64, 32, 6, None)])
 def test_contract_specular_small(ctx, S, So, mips, sub):
     assert _run(ctx, abi.FILTER_SPECULAR, S, So, mips, sub) <= TOL
 
 
-@pytest.mark.parametrize("S,So,mips,sub", [(128, 64, 7, 3000), (1024, 512, 10, 40)])
+@pytest.mark.parametrize("S,So,mips,sub", [(256, 128, 8, 3000), (1024, 512, 10, 40)])
 def test_contract_split_bf16(ctx, S, So, mips, sub):
     """Split-BF16 near fields: within the bound everywhere, and the
     low-roughness (near/far) mips an order of magnitude closer to FP64 than
@@ -197,9 +197,10 @@ def test_contract_multi_equals_single(ctx, precision, nprobe):
     single-probe calls bit for bit; more probes than reserved is refused.
     The grouped GEMMs take probes in groups of four: n = 3 is one partial
     group, 4 one full group, 9 two full groups and a tail (group indices past
-    the first)."""
+    the first). At 256^2 -> 128^2 x 8, mip 1 is a near/far mip and the
+    others full-sphere sums, so both GEMM kinds take the grouped path."""
     import torch
-    S, So, mips = 128, 64, 7
+    S, So, mips = 256, 128, 8
     levels = abi.mip_count(S)
     rng = np.random.default_rng(41)
     plan = abi.ContractPlan(ctx, abi.FILTER_SPECULAR, S, So, mips, precision)
