@@ -156,10 +156,11 @@ def test_contract_and_plain_against_exact_integral_512_chain(ctx):
     radiance 1e4, 6x1024^2 cube, 512^2 x 10 GGX chain): the FP16 contraction
     filter and the plain 1024-sample filter against the filter integral summed
     over every level-0 texel in FP64, on 24 texels of each rough mip. The
-    contraction's quadrature over box-filtered levels stays within 2e-2 of it
-    (medians within 3e-3; measured max 1.05e-2 at mip 2, where the sun disk's
-    position inside a coarse texel dominates); the plain filter's Monte-Carlo
-    estimate misses it by far more on every mip (measured max 0.24 .. 3.6)."""
+    contraction's quadrature over box-filtered levels stays within 8e-3 of it
+    (medians within 2.5e-3; measured max 4.9e-3 at mip 1, 3.9e-3 at mip 2,
+    the sun disk's blur in the coarse levels among the worst); the plain
+    filter's Monte-Carlo estimate misses it by far more on every mip
+    (measured max 0.24 .. 3.6)."""
     import torch
     S, So, mips = 1024, 512, 10
     eq = abi.scene_equirect(77, 2048, 1024, threads=16)
@@ -185,7 +186,7 @@ def test_contract_and_plain_against_exact_integral_512_chain(ctx):
         ec = np.abs(Cn[off:off + 4 * n].reshape(-1, 4)[ids, :3] - ref) / np.abs(ref)
         print(f"mip {m}: plain vs exact max {ep.max():.3e} median {np.median(ep):.3e}; "
               f"contraction vs exact max {ec.max():.3e} median {np.median(ec):.3e}")
-        assert ec.max() <= 2e-2 and np.median(ec) <= 3e-3
+        assert ec.max() <= 8e-3 and np.median(ec) <= 2.5e-3
         assert ec.max() < ep.max() and np.median(ec) < np.median(ep)
     plan.close()
 
@@ -248,10 +249,11 @@ def test_contract_hot_split_workload(ctx):
     split at 2048 texels: the threshold and hot count equal the oracle's;
     every mip is within 2e-3 of the split FP64 twin; against the level-0
     sum (24 random texels and the 6 nearest the sun per rough mip) the split
-    contraction's worst error is no larger than the unsplit one's, and at
-    the texels nearest the sun it is lower on every rough mip; the delta mip
-    is bit-identical to the unsplit call's; graph replays equal the eager
-    call."""
+    contraction's worst error is no larger than the unsplit one's, at most
+    2e-3 on mips 3-9 and 6e-3 on mips 1-2 (measured 1.5e-3 and 4.9e-3), and
+    at the texels nearest the sun it is lower than the unsplit one's on every
+    rough mip; the delta mip is bit-identical to the unsplit call's; graph
+    replays equal the eager call."""
     import torch
     S, So, mips, K = 1024, 512, 10, 2048
     eq = abi.scene_equirect(77, 2048, 1024, threads=16)
@@ -296,6 +298,7 @@ def test_contract_hot_split_workload(ctx):
               f"(at the sun {eb[:ns].max():.2e})")
         assert e_twin.max() <= TOL
         assert eh.max() <= eb.max() * (1 + 1e-3)
+        assert eh.max() <= (2e-3 if mi >= 3 else 6e-3)
         assert eh[:ns].max() < eb[:ns].max()
     # graph replays of the split call equal the eager result
     s = torch.cuda.Stream()
