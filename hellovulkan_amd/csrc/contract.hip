@@ -849,14 +849,15 @@ constexpr uint32_t kNearParts = 2;  // workgroups per tile (fixed by the 256-lan
 static_assert(kNearWaves % kNearParts == 0, "a tile splits into whole waves");
 static_assert(64 * kNearWaves / kNearParts == 256, "the near GEMM's LDS staging assumes 256-lane workgroups");
 #ifndef HVX_NEAR_ROWS
-#define HVX_NEAR_ROWS 2
+#define HVX_NEAR_ROWS 1
 #endif
 // The single-probe near GEMM: row tiles per wave (a tile's 16 row tiles take
 // 16 / kNearRows waves, in 256-lane workgroups). One row tile per wave
 // doubles the waves that stream the weights but loads each B fragment twice
 // as often (from L2); four row tiles per wave halve the B loads and the
-// waves. Measured on the 512^2 GGX chain (FP16 near GEMM): two 203 us, one
-// 230 us, four 217 us. Each output's accumulation order is the same for all.
+// waves. Measured on the 512^2 GGX chain, whole FP16 call: one 0.595 ms, two
+// 0.625 ms (the smaller work units balance mip 2, whose corner tiles have
+// windows 3.4x the mean). Each output's accumulation order is the same for all.
 constexpr uint32_t kNearRows = HVX_NEAR_ROWS;
 constexpr uint32_t kNearSingleParts = 16 / kNearRows / 4;
 static_assert(kNearRows == 1 || kNearRows == 2 || kNearRows == 4, "1, 2 or 4 row tiles per wave");
@@ -2081,6 +2082,7 @@ int hvx_ibl_contract_plan_create(hvx_ctx* ctx, int type, uint32_t face, uint32_t
     }
     DenseTable& DT = P->dense;
     NearTable& NT = P->near;
+    double near_k[kMaxJobs] = {};  // mean window (K) of each near job
     float* part = P->part;
     for (uint32_t m = 0; m < out_mips; ++m) {
         const MipPlan& M = P->mp[m];
@@ -2116,7 +2118,28 @@ int hvx_ibl_contract_plan_create(hvx_ctx* ctx, int type, uint32_t face, uint32_t
                         split ? (uint64_t)sp.Pw * (sp.Pw / 8) * 32 : 0};
             NT.tiles += (uint32_t)M.tiles.size();
             NT.texels += 6 * M.L.So * M.L.So;
+            double k = 0.0;
+            for (const NearTile& t : M.tiles) k += (double)t.w * t.h;
+            near_k[NT.n - 1] = k / (double)M.tiles.size();
         }
+    }
+    // near jobs in decreasing mean window size: the workgroups with the most
+    // K steps start first and the many shorter ones fill in behind them (the
+    // per-output accumulation does not depend on the order)
+    {
+        std::vector<uint32_t> ord(NT.n);
+        for (uint32_t k = 0; k < NT.n; ++k) ord[k] = k;
+        std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return near_k[a] > near_k[b]; });
+        NearTable sorted = NT;
+        uint32_t tb = 0, xb = 0;
+        for (uint32_t k = 0; k < NT.n; ++k) {
+            sorted.j[k] = NT.j[ord[k]];
+            sorted.j[k].tile_begin = tb;
+            sorted.j[k].texel_begin = xb;
+            tb += sorted.j[k].te * sorted.j[k].te;
+            xb += 6 * sorted.j[k].So * sorted.j[k].So;
+        }
+        NT = sorted;
     }
     if ((e = hipDeviceSynchronize()) != hipSuccess) return fail_destroy(e, "plan set-up");
     *out = P;
