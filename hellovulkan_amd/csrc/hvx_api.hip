@@ -499,6 +499,7 @@ int hvx_cf_shade_learned(hvx_ctx* ctx, const hvx_frag_soa* fr, const hvx_camera_
         return fail(ctx, HVX_ERR_INVALID_ARG, "maxReflectionLod %g exceeds the network's lod range %g",
                     pc->max_reflection_lod, L.mlp.lod_max);
     L.cell_err = d_cell_err;
+    L.cell_lo = hvx::mlp_cell_lower(mlp, cell_face, lod_bins);
     L.mask_face = cell_face;
     L.lod_bins = lod_bins;
     L.tol = tol;

@@ -414,13 +414,16 @@ __global__ __launch_bounds__(256) void mask_points_kernel(uint32_t mf, uint32_t 
 
 // err[cell] = largest relative error (every channel, every point of the
 // cell) of the network against the chain's textureLod value; infinity where
-// the reference is 0 and the prediction is not
+// the reference is 0 and the prediction is not. lo[cell] = half the smallest
+// prediction (every channel, every point): an estimate from below of what
+// the network returns anywhere in the cell.
 __global__ __launch_bounds__(256) void cell_error_kernel(const float4* __restrict__ pts, const float* __restrict__ rgb,
                                                          const float4* __restrict__ chain, uint32_t S, uint32_t mips,
-                                                         uint32_t cells, float* __restrict__ err) {
+                                                         uint32_t cells, float* __restrict__ err,
+                                                         float* __restrict__ lo) {
     const uint32_t cell = blockIdx.x * 256 + threadIdx.x;
     if (cell >= cells) return;
-    float e = 0.0f;
+    float e = 0.0f, mn = INFINITY;
     for (uint32_t p = 0; p < kMaskPoints; ++p) {
         const float4 q = pts[(size_t)cell * kMaskPoints + p];
         const f3 ref = chain_lod(chain, S, mips, f3{q.x, q.y, q.z}, q.w);
@@ -431,9 +434,11 @@ __global__ __launch_bounds__(256) void cell_error_kernel(const float4* __restric
             float rel = d == 0.0f ? 0.0f : (r3[c] != 0.0f ? d / fabsf(r3[c]) : INFINITY);
             if (!(rel == rel)) rel = INFINITY;  // a NaN prediction fails every bound
             e = fmaxf(e, rel);
+            mn = fminf(mn, m[c]);
         }
     }
     err[cell] = e;
+    lo[cell] = mn > 0.0f ? 0.5f * mn : 0.0f;
 }
 
 // The error the fallback decides on: the largest measured error of the cell
@@ -462,12 +467,19 @@ struct hvx_mlp {
     // a training call's inputs encoded once: [n][kIn] and [kIn][enc_cap]
     bf16 *enc = nullptr, *encT = nullptr;
     size_t enc_cap = 0;
+    // per-cell estimate from below of the network's output, from the last
+    // hvx_mlp_cell_errors call (cell grid lo_face x lo_face x lo_bins)
+    float* cell_lo = nullptr;
+    uint32_t lo_face = 0, lo_bins = 0;
     std::vector<void*> batch_allocs;
     std::vector<void*> allocs;
 };
 
 namespace hvx {
 MlpView mlp_view(const hvx_mlp* M) { return MlpView{M->W1, M->W2, M->W3, M->P, M->lod_max}; }
+const float* mlp_cell_lower(const hvx_mlp* M, uint32_t cell_face, uint32_t lod_bins) {
+    return M->cell_lo && M->lo_face == cell_face && M->lo_bins == lod_bins ? M->cell_lo : nullptr;
+}
 }  // namespace hvx
 
 namespace {
@@ -671,6 +683,7 @@ int hvx_mlp_destroy(hvx_mlp* M) {
     for (void* p : M->allocs) (void)hipFree(p);
     if (M->enc) (void)hipFree(M->enc);
     if (M->encT) (void)hipFree(M->encT);
+    if (M->cell_lo) (void)hipFree(M->cell_lo);
     delete M;
     return HVX_OK;
 }
@@ -817,13 +830,29 @@ int hvx_mlp_cell_errors(hvx_mlp* M, const float* d_chain, uint32_t face, uint32_
         if (rgb) (void)hipFree(rgb);
         return mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_cell_errors: %s", hipGetErrorString(e));
     }
+    // the lower estimates travel with the network for hvx_cf_shade_learned
+    if (M->lo_face * M->lo_face * 6 * M->lo_bins != cells || !M->cell_lo) {
+        (void)hipDeviceSynchronize();  // a shading call may still read the old ones
+        if (M->cell_lo) (void)hipFree(M->cell_lo);
+        M->cell_lo = nullptr;
+        M->lo_face = M->lo_bins = 0;
+        if ((e = hipMalloc(&M->cell_lo, (size_t)cells * sizeof(float))) != hipSuccess) {
+            M->cell_lo = nullptr;
+            (void)hipFree(pts);
+            (void)hipFree(rgb);
+            (void)hipFree(raw);
+            return mfail(M->ctx, HVX_ERR_HIP, "hvx_mlp_cell_errors: %s", hipGetErrorString(e));
+        }
+    }
+    M->lo_face = mask_face;
+    M->lo_bins = lod_bins;
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(mask_points_kernel, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, mask_face, lod_bins,
                        M->lod_max, pts);
     hipLaunchKernelGGL(mlp_infer_kernel, dim3(infer_blocks(M, npts)), dim3(256), 0, s,
                        hvx::mlp_view(M), pts, (uint32_t)npts, rgb);
     hipLaunchKernelGGL(cell_error_kernel, dim3((cells + 255) / 256), dim3(256), 0, s, pts, rgb,
-                       reinterpret_cast<const float4*>(d_chain), face, mips, cells, raw);
+                       reinterpret_cast<const float4*>(d_chain), face, mips, cells, raw, M->cell_lo);
     hipLaunchKernelGGL(cell_margin_kernel, dim3((cells + 255) / 256), dim3(256), 0, s, raw, cells, d_err);
     e = hipStreamSynchronize(s);
     (void)hipFree(pts);

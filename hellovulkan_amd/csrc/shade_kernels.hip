@@ -82,6 +82,7 @@ struct ShadeArgs {
     // learned prefiltered radiance (shade_kernel<..., true>)
     MlpView mlp;
     const float* cell_err;  // [lod_bins][6][mask_face][mask_face]: the network's relative error per cell
+    const float* cell_lo;   // nullable, same layout: an estimate from below of the network's output per cell
     uint32_t mask_face, lod_bins;
     float mlp_tol;          // bound on the relative radiance error the network may cause
     uint32_t* served;       // fragments that used the network (nullable)
@@ -172,23 +173,35 @@ __device__ __forceinline__ f3 frag_reflect(const ShadeArgs& a, uint32_t idx, flo
     return sub3(I, mul3(N, 2.0f * dNI));
 }
 
-// The network's measured relative error for (R, lod): the cell of the cube
-// texel of R at the cell-grid resolution (face selection as for sampling)
-// and of lod's bin.
-__device__ __forceinline__ float mlp_cell_error(const ShadeArgs& a, f3 R, float lod) {
+// The cell of (R, lod) in the network's error grid: the cube texel of R at
+// the cell-grid resolution (face selection as for sampling) and lod's bin.
+__device__ __forceinline__ size_t mlp_cell_index(const ShadeArgs& a, f3 R, float lod) {
     const CubeCoord cc = cube_select(R);
     const uint32_t mf = a.mask_face;
     const uint32_t x = min((uint32_t)(cc.s * (float)mf), mf - 1), y = min((uint32_t)(cc.t * (float)mf), mf - 1);
     const uint32_t b = min((uint32_t)(lod / a.mlp.lod_max * (float)a.lod_bins), a.lod_bins - 1);
-    return a.cell_err[((size_t)(b * 6 + (uint32_t)cc.face) * mf + y) * mf + x];
+    return ((size_t)(b * 6 + (uint32_t)cc.face) * mf + y) * mf + x;
+}
+// The network's measured relative error for (R, lod).
+__device__ __forceinline__ float mlp_cell_error(const ShadeArgs& a, f3 R, float lod) {
+    return a.cell_err[mlp_cell_index(a, R, lod)];
 }
 
-// Returns whether the learned radiance was used (cell_err >= 0 offers it:
-// the network's value `learned` and the measured relative error of its cell).
-template <bool EXACT>
-__device__ __forceinline__ bool shade_one(const ShadeArgs& a, uint32_t idx, uint32_t oidx, uint32_t off,
-                                          uint32_t count, float cell_err = -1.0f,
-                                          f3 learned = f3{0.0f, 0.0f, 0.0f}) {
+// What a learned-shading lane keeps from the light loop to the final write:
+// kD * diffuse and the prefiltered radiance's weight per channel, the direct
+// light, the reflection vector and the roughness.
+struct ShadeTail {
+    float kdd[3], sw[3], Lo[3];
+    f3 R;
+    float rough;
+};
+
+// TAIL = false: shades fragment idx and writes its RGB with the map's
+// prefiltered radiance. TAIL = true: stops before the specular ambient term
+// and leaves the rest in *tail (shade_tail_write finishes it).
+template <bool EXACT, bool TAIL = false>
+__device__ __forceinline__ void shade_one(const ShadeArgs& a, uint32_t idx, uint32_t oidx, uint32_t off,
+                                          uint32_t count, ShadeTail* tail = nullptr) {
     const float PI = kPiF;
     const uint32_t nfrag = a.W * a.H;
     // the RGB goes to pixel idx of out, or to position oidx of the binned planes
@@ -415,7 +428,7 @@ __device__ __forceinline__ bool shade_one(const ShadeArgs& a, uint32_t idx, uint
     const f3 I = {-V.x, -V.y, -V.z};
     const float dNI = dot3(N, I);
     const f3 R = sub3(I, mul3(N, 2.0f * dNI));
-    if (!(cell_err >= 0.0f)) {  // plain: the map's prefiltered radiance
+    if constexpr (!TAIL) {  // plain: the map's prefiltered radiance
         const f3 pre = sample_spec_lod(a, R, rough * a.pc.max_reflection_lod);
         const float2 brdf = sample_2d_rg_repeat(a.lut, (int)a.lut_w, (int)a.lut_h, NoV, rough);
         const float irr3[3] = {irr.x, irr.y, irr.z}, pre3[3] = {pre.x, pre.y, pre.z};
@@ -428,40 +441,54 @@ __device__ __forceinline__ bool shade_one(const ShadeArgs& a, uint32_t idx, uint
             const float ambient = (kD * diffuse + specular) * 1.0f;
             dst[(size_t)c * nfrag + di] = (ambient + 0.0f) + Lo[c];
         }
-        return false;
-    }
-    const float2 brdf = sample_2d_rg_repeat(a.lut, (int)a.lut_w, (int)a.lut_h, NoV, rough);
-    const float irr3[3] = {irr.x, irr.y, irr.z};
-    float kD[3], diffuse[3], sw[3];
+    } else {
+        (void)dst;
+        (void)di;
+        const float2 brdf = sample_2d_rg_repeat(a.lut, (int)a.lut_w, (int)a.lut_h, NoV, rough);
+        const float irr3[3] = {irr.x, irr.y, irr.z};
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const float F = F0[c] + (gmax(1.0f - rough, F0[c]) - F0[c]) * p5v;
-        kD[c] = (1.0f - F) * oneMinusMetal;
-        diffuse[c] = irr3[c] * alb[c];
-        sw[c] = F * brdf.x + brdf.y;  // weight of the prefiltered radiance
+        for (int c = 0; c < 3; ++c) {
+            const float F = F0[c] + (gmax(1.0f - rough, F0[c]) - F0[c]) * p5v;
+            const float kD = (1.0f - F) * oneMinusMetal;
+            const float diffuse = irr3[c] * alb[c];
+            tail->kdd[c] = kD * diffuse;
+            tail->sw[c] = F * brdf.x + brdf.y;  // weight of the prefiltered radiance
+            tail->Lo[c] = Lo[c];
+        }
+        tail->R = R;
+        tail->rough = rough;
     }
-    // The learned radiance replaces the map sample where the error it can
-    // cause in the shaded radiance, e/(1-e) * learned * weight per channel
-    // (e = the cell's measured relative error of the network), stays within
-    // tol of the radiance without the specular ambient term (a lower bound
-    // of the shaded value, every term being non-negative).
-    bool use = false;
-    if (cell_err < 1.0f) {
-        const float k = cell_err / (1.0f - cell_err);
-        const float l3[3] = {learned.x, learned.y, learned.z};
-        use = true;
+}
+
+// The learned radiance replaces the map sample where the error it can cause
+// in the shaded radiance, e/(1-e) * learned * weight per channel (e = the
+// cell's measured relative error of the network), stays within tol of the
+// radiance without the specular ambient term (a lower bound of the shaded
+// value, every term being non-negative).
+__device__ __forceinline__ bool learned_serves(const ShadeArgs& a, const ShadeTail& t, float cell_err, f3 learned) {
+    if (!(cell_err >= 0.0f && cell_err < 1.0f)) return false;
+    const float k = cell_err / (1.0f - cell_err);
+    const float l3[3] = {learned.x, learned.y, learned.z};
+    bool use = true;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) use = use && k * l3[c] * sw[c] <= a.mlp_tol * (kD[c] * diffuse[c] + Lo[c]);
-    }
-    const f3 pre = use ? learned : sample_spec_lod(a, R, rough * a.pc.max_reflection_lod);
+    for (int c = 0; c < 3; ++c) use = use && k * l3[c] * t.sw[c] <= a.mlp_tol * (t.kdd[c] + t.Lo[c]);
+    return use;
+}
+
+// The final write of a learned-shading fragment (the plain expression, with
+// `pre` the learned or the map's prefiltered radiance).
+__device__ __forceinline__ void shade_tail_write(const ShadeArgs& a, uint32_t idx, uint32_t oidx, const ShadeTail& t,
+                                                 f3 pre) {
+    const uint32_t nfrag = a.W * a.H;
+    float* const dst = a.out_binned ? a.out_binned : a.out;
+    const uint32_t di = a.out_binned ? oidx : idx;
     const float pre3[3] = {pre.x, pre.y, pre.z};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const float specular = pre3[c] * sw[c];
-        const float ambient = (kD[c] * diffuse[c] + specular) * 1.0f;
-        dst[(size_t)c * nfrag + di] = (ambient + 0.0f) + Lo[c];
+        const float specular = pre3[c] * t.sw[c];
+        const float ambient = (t.kdd[c] + specular) * 1.0f;
+        dst[(size_t)c * nfrag + di] = (ambient + 0.0f) + t.Lo[c];
     }
-    return use;
 }
 
 // Fragments per workgroup (3840x2160, 65,536 lights): plain shading 64 (2.07 ms; 128:
@@ -664,47 +691,67 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_W
         __syncthreads();
     }
     if constexpr (LEARNED) {
-        // Each wave evaluates the network for the 64 fragments it is about to
-        // shade, as four 16-sample tiles; tile T holds the fragments of lanes
-        // 16 T .. 16 T + 15 and its results return to those lanes by shuffles.
+        // Each lane first shades its fragment up to the specular ambient term.
+        // The network is then evaluated only for candidate lanes, those whose
+        // cell could serve even at the cell's lower estimate of the network's
+        // output (a larger output only makes serving harder), packed into
+        // 16-sample tiles; tile T holds candidates 16 T .. 16 T + 15 of the
+        // wave, and the results return to their lanes by shuffles. The final
+        // decision is made on the network's actual output, so the lower
+        // estimate only decides where the network is evaluated.
         __shared__ MlpLds s_mlp;
         __shared__ uint32_t s_served;
+        __shared__ uint8_t s_cand[kThreads / 64][64];
         mlp_stage(s_mlp, a.mlp);
         if (t == 0) s_served = 0;
         __syncthreads();
-        const uint32_t lane = t & 63, i = lane & 15;
+        const uint32_t lane = t & 63, i = lane & 15, wv = t >> 6;
         uint32_t mine = 0;
 #pragma unroll 1
         for (int r = 0; r < kRounds; ++r) {
             const uint32_t local = s_order[r * kThreads + t];
             const uint32_t idx = s_idx[local];
+            const bool live = idx < nfrag;
             float x[4] = {0.0f, 0.0f, 1.0f, 0.0f};
-            float err = -1.0f;
-            if (idx < nfrag) {
+            float err = -1.0f, lo = 0.0f;
+            ShadeTail tl;
+            if (live) {
                 float lod;
                 const f3 R = frag_reflect(a, idx, &lod);
                 x[0] = R.x;
                 x[1] = R.y;
                 x[2] = R.z;
                 x[3] = lod;
-                err = mlp_cell_error(a, R, lod);
+                const size_t cell = mlp_cell_index(a, R, lod);
+                err = a.cell_err[cell];
+                lo = a.cell_lo ? a.cell_lo[cell] : 0.0f;
+                shade_one<EXACT, true>(a, idx, base + local, s_off[local], s_cnt[local], &tl);
             }
+            const bool cand = live && learned_serves(a, tl, err, f3{lo, lo, lo});
+            const unsigned long long cm = __ballot(cand);
+            const uint32_t nc = (uint32_t)__popcll(cm);
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(cm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)cm, 0u));
+            if (cand) s_cand[wv][rank] = (uint8_t)lane;
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             f3 learned = {0.0f, 0.0f, 0.0f};
 #pragma unroll 1
-            for (int T = 0; T < 4; ++T) {
-                const int src = 16 * T + (int)i;
-                const float et = __shfl(err, src, 64);
-                // tiles whose cells all exclude the network (error >= 1) skip it
-                if (__ballot(et >= 0.0f && et < 1.0f) == 0) continue;
+            for (uint32_t T = 0; 16 * T < nc; ++T) {
+                const uint32_t q = 16 * T + i;
+                const int src = (int)s_cand[wv][q < nc ? q : 16 * T];
                 const float xt[4] = {__shfl(x[0], src, 64), __shfl(x[1], src, 64), __shfl(x[2], src, 64),
                                      __shfl(x[3], src, 64)};
-                const float y = mlp_forward_tile(s_mlp, xt, a.mlp.lod_max);  // lane (c, i): channel c of sample i
-                const float c0 = __shfl(y, (int)i, 64), c1 = __shfl(y, 16 + (int)i, 64),
-                            c2 = __shfl(y, 32 + (int)i, 64);
-                if ((lane >> 4) == (uint32_t)T) learned = f3{c0, c1, c2};
+                const float y = mlp_forward_tile(s_mlp, xt, a.mlp.lod_max);  // lane (c, j): channel c of sample j
+                const int j = (int)(rank & 15u);
+                const float c0 = __shfl(y, j, 64), c1 = __shfl(y, 16 + j, 64), c2 = __shfl(y, 32 + j, 64);
+                if (cand && (rank >> 4) == T) learned = f3{c0, c1, c2};
             }
-            if (idx < nfrag)
-                mine += shade_one<EXACT>(a, idx, base + local, s_off[local], s_cnt[local], err, learned) ? 1u : 0u;
+            if (live) {
+                const bool use = cand && learned_serves(a, tl, err, learned);
+                const f3 pre = use ? learned : sample_spec_lod(a, tl.R, tl.rough * a.pc.max_reflection_lod);
+                shade_tail_write(a, idx, base + local, tl, pre);
+                mine += use ? 1u : 0u;
+            }
         }
         if (a.served) {
             if (mine) atomicAdd(&s_served, mine);
@@ -783,6 +830,7 @@ hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const
     if (learned) {
         a.mlp = learned->mlp;
         a.cell_err = learned->cell_err;
+        a.cell_lo = learned->cell_lo;
         a.mask_face = learned->mask_face;
         a.lod_bins = learned->lod_bins;
         a.mlp_tol = learned->tol;

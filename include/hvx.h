@@ -451,7 +451,10 @@ int hvx_mlp_eval(hvx_mlp* mlp, const float* d_inputs, uint32_t n, float* d_rgb, 
  * network against the chain's textureLod value at the cell's 75 test points
  * (a 5 x 5 grid over the texel, corners included, at both bin ends and its
  * middle): a margin over the measured error, since the error between test
- * points is not measured. A set-up call: it allocates and synchronises the
+ * points is not measured. It also keeps, with the network, half the
+ * smallest output the network gave in each cell, which hvx_cf_shade_learned
+ * uses on the same cell grid to skip the network where even that output
+ * could not be used. A set-up call: it allocates and synchronises the
  * stream. */
 int hvx_mlp_cell_errors(hvx_mlp* mlp, const float* d_chain, uint32_t face, uint32_t mips, uint32_t cell_face,
                         uint32_t lod_bins, float* d_err, void* stream);
