@@ -175,8 +175,11 @@ typedef enum hvx_option {
     /* 0 (default): shading keeps the light/half vectors and their cosines
      * exact and evaluates the remaining per-light terms with fused
      * multiply-add and hardware reciprocal/exp2/log2 (within 1e-5 relative of
-     * the exact evaluation); 1: every operation exactly as the FP32
-     * definition of the path (bit-reproducible on any IEEE machine). */
+     * the exact evaluation; waves whose fragments are all rough take hardware
+     * reciprocal square roots, and which fragments share a wave can change
+     * between calls, so results may differ in the last bits from call to
+     * call); 1: every operation exactly as the FP32 definition of the path
+     * (bit-reproducible on any IEEE machine). */
     HVX_OPT_SHADE_EXACT = 1,
     /* 0 (default): off; k in [1, 256]: record a pair of HIP events around
      * each launch of the shading kernel itself (not the binning and light
