@@ -16,7 +16,9 @@
 // loop by the L1 tag rate rather than by arithmetic). Grids too deep to bin
 // are counting-sorted by light count within each workgroup instead. Each
 // fragment is still shaded start to finish by one lane, summing its lights in
-// list order, so the order of the fragments changes no result. The RGB is
+// list order, so the order of the fragments changes no result in the exact
+// variant (in the fast one it decides which waves are all rough enough for
+// the hardware reciprocal square roots: a few ulp). The RGB is
 // written at the fragments' binned positions (coalesced) and unbin_kernel
 // moves it to pixel order through an LDS image of each tile.
 //
@@ -571,7 +573,8 @@ __device__ __forceinline__ uint32_t bin_of(const ShadeArgs& a, uint32_t idx) {
 // One workgroup per screen tile: counting sort of the tile's pixels by depth
 // slice into the tile's segment of the order. Tiles are laid out row band by
 // row band, so tile (cx, cy) starts at y0 * W + (y1 - y0) * x0. Within a
-// slice the order is that of the LDS atomics, which changes no result.
+// slice the order is that of the LDS atomics (see the note on the fast
+// variant at the top of this file).
 __global__ __launch_bounds__(kBinThreads) void bin_kernel(ShadeArgs a) {
     __shared__ uint32_t s_hist[kBinMaxZ];
     __shared__ uint32_t s_wave[kBinThreads / 64];
