@@ -290,8 +290,8 @@ def contract_bench(ctx, stream, d_cube, S, So, mips, Sd, plain_spec_ms, plain_di
                      "plan_build_s": round(build_s, 3),
                      "weight_MB": round(wbytes / 1e6, 1),
                      "weight_stream_TBs": round(wbytes / (ms * 1e-3) / 1e12, 3),
-                     "mips": [{"kind": ["delta", "full", "near+far"][i.kind], "in": i.in_face, "far": i.far_face}
-                              for i in info]}
+                     "mips": [{"kind": ["delta", "full", "near+far"][i.kind], "in": i.in_face, "far": i.far_face,
+                               "weight_MB": round(i.weight_bytes / 1e6, 1)} for i in info]}
         if hot:
             T, n_hot = plan.hot_info()
             out[name].update(bright_texels_max=hot, threshold=T, bright_texels=n_hot)

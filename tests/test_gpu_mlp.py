@@ -226,6 +226,22 @@ def test_trained_network_in_shading(ctx):
     changed = int(np.count_nonzero(rel))
     assert changed <= served
     assert np.mean(rel > 2e-3) <= 1e-4
+    # The per-cell lower estimate only decides where the network is
+    # evaluated: after cell errors on another grid (the estimate of grid
+    # (cf, bins) is then gone and every usable cell is a candidate) the same
+    # frame is shaded identically and serves the same fragments (exact
+    # arithmetic mode, which is reproducible from call to call).
+    ctx.set_option(abi.OPT_SHADE_EXACT, 1)
+    try:
+        learned_x, served_x = _shade(ctx, f, mlp, err, cf, bins, 2e-3)
+        other = torch.empty(bins * 6 * 8 * 8, dtype=torch.float32, device="cuda")
+        mlp.cell_errors(spec, So, mips, 8, bins, other)
+        learned_all, served_all = _shade(ctx, f, mlp, err, cf, bins, 2e-3)
+    finally:
+        ctx.set_option(abi.OPT_SHADE_EXACT, 0)
+    print(f"served with the candidate pre-check {served_x}, without {served_all}")
+    assert served_all == served_x > 0
+    assert np.array_equal(learned_all.view(np.uint32), learned_x.view(np.uint32))
     mlp.close()
 
 
