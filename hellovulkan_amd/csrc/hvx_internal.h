@@ -124,14 +124,14 @@ hipError_t launch_queue_reset(uint32_t* d_queues, uint32_t n, hipStream_t s);
 struct LearnedShade {
     MlpView mlp;
     const float* cell_err;
-    const float* cell_lo;  // nullable: per-cell estimate from below of the network's output
+    const float4* cell_lo;  // nullable: per-cell estimate from below of the network's output (RGB, pad)
     uint32_t mask_face, lod_bins;
     float tol;
     uint32_t* served;
 };
 MlpView mlp_view(const hvx_mlp* mlp);
 // the cell lower estimates of the last hvx_mlp_cell_errors call on that grid, else nullptr
-const float* mlp_cell_lower(const hvx_mlp* mlp, uint32_t cell_face, uint32_t lod_bins);
+const float4* mlp_cell_lower(const hvx_mlp* mlp, uint32_t cell_face, uint32_t lod_bins);
 
 hipError_t launch_tonemap(const float* d_rgb, uint32_t n, int op, int gamma, float* d_out, uint32_t* d_bgra,
                           hipStream_t s);

@@ -68,12 +68,13 @@ HVX_MLP_HD inline void mlp_sincos_direct(float x, int k, float& s, float& c) {
     pc = fmaf(a2, pc, 4.1666666666666667e-2f);                             // 1/4!
     pc = fmaf(a2, pc, -0.5f);
     const float ca = fmaf(a2, pc, 1.0f);
-    switch ((int)q & 3) {
-        case 0: s = sa; c = ca; break;
-        case 1: s = ca; c = -sa; break;
-        case 2: s = -sa; c = -ca; break;
-        default: s = -ca; c = sa; break;
-    }
+    // quadrant qi = q mod 4 (q in [-2, 2]): (s, c) = (sa, ca), (ca, -sa),
+    // (-sa, -ca), (-ca, sa) for qi = 0..3, as selects rather than a branch
+    // per quadrant (lanes of one wave hold different quadrants)
+    const uint32_t qi = (uint32_t)(int)q;
+    const float s0 = (qi & 1u) ? ca : sa, c0 = (qi & 1u) ? sa : ca;
+    s = (qi & 2u) ? -s0 : s0;
+    c = ((qi + 1u) & 2u) ? -c0 : c0;
 }
 
 // Frequency k + 2 from frequency k (f_(k+2) = 2 f_k): sin 2u = 2 sin u cos u,

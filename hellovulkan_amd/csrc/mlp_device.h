@@ -118,6 +118,9 @@ __device__ __forceinline__ float mlp_forward_tile(const MlpLds& L, const float x
             const bf16x8 a = *reinterpret_cast<const bf16x8*>(L.W1 + (16 * nb + i) * kMlpIn + 32 * s + 8 * g);
             acc[nb] = mfma16x16x32(a, b, acc[nb]);
         }
+        // one K block at a time: without the fence the scheduler hoists every
+        // block's weight loads and encoding, past the register budget
+        __builtin_amdgcn_sched_barrier(0);
     }
     __bf16 h[4][4];
     mlp_activate(acc, L.b1, g, h);

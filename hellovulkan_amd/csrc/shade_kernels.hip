@@ -51,6 +51,11 @@
 #ifndef HVX_SHADE_WAVES_PER_EU  // 5: the plain kernel fits 96 VGPRs, 3.5% faster than 4
 #define HVX_SHADE_WAVES_PER_EU 5
 #endif
+// The learned kernel's workgroups hold the network's weights in LDS (37.9 KB
+// of 160 KB per CU), so at most four of its 4-wave workgroups share a CU: the
+// register budget is set for those 4 waves per SIMD (an occupancy the LDS
+// cannot reach leaves the compiler no register limit at all).
+constexpr int kShadeWavesLearned = 4;
 #ifndef HVX_SHADE_SKIP  // waves skip lights out of range of all their lanes
 #define HVX_SHADE_SKIP 1
 #endif
@@ -84,7 +89,7 @@ struct ShadeArgs {
     // learned prefiltered radiance (shade_kernel<..., true>)
     MlpView mlp;
     const float* cell_err;  // [lod_bins][6][mask_face][mask_face]: the network's relative error per cell
-    const float* cell_lo;   // nullable, same layout: an estimate from below of the network's output per cell
+    const float4* cell_lo;  // nullable, same layout: an estimate from below of the network's output per cell (RGB, pad)
     uint32_t mask_face, lod_bins;
     float mlp_tol;          // bound on the relative radiance error the network may cause
     uint32_t* served;       // fragments that used the network (nullable)
@@ -624,7 +629,7 @@ __global__ __launch_bounds__(kBinThreads) void bin_kernel(ShadeArgs a) {
 
 
 template <bool EXACT, bool LEARNED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_WAVES_PER_EU, 8))) void shade_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LEARNED ? kShadeWavesLearned : HVX_SHADE_WAVES_PER_EU, 8))) void shade_kernel(
     ShadeArgs a) {
     constexpr int kChunk = ShadeChunk<LEARNED>::kChunk, kThreads = ShadeChunk<LEARNED>::kThreads,
                   kRounds = ShadeChunk<LEARNED>::kRounds;
@@ -716,7 +721,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_W
             const uint32_t idx = s_idx[local];
             const bool live = idx < nfrag;
             float x[4] = {0.0f, 0.0f, 1.0f, 0.0f};
-            float err = -1.0f, lo = 0.0f;
+            float err = -1.0f;
+            f3 lo = {0.0f, 0.0f, 0.0f};
             ShadeTail tl;
             if (live) {
                 float lod;
@@ -727,10 +733,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HVX_SHADE_W
                 x[3] = lod;
                 const size_t cell = mlp_cell_index(a, R, lod);
                 err = a.cell_err[cell];
-                lo = a.cell_lo ? a.cell_lo[cell] : 0.0f;
+                if (a.cell_lo) {
+                    const float4 l = a.cell_lo[cell];
+                    lo = f3{l.x, l.y, l.z};
+                }
                 shade_one<EXACT, true>(a, idx, base + local, s_off[local], s_cnt[local], &tl);
             }
-            const bool cand = live && learned_serves(a, tl, err, f3{lo, lo, lo});
+            const bool cand = live && learned_serves(a, tl, err, lo);
             const unsigned long long cm = __ballot(cand);
             const uint32_t nc = (uint32_t)__popcll(cm);
             const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(cm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)cm, 0u));
