@@ -861,20 +861,52 @@ static_assert(64 * kNearWaves / kNearParts == 256, "the near GEMM's LDS staging 
 constexpr uint32_t kNearRows = HVX_NEAR_ROWS;
 constexpr uint32_t kNearSingleParts = 16 / kNearRows / 4;
 static_assert(kNearRows == 1 || kNearRows == 2 || kNearRows == 4, "1, 2 or 4 row tiles per wave");
+// XCD-aware order of the single-probe near GEMM: workgroups are dealt to the
+// 8 XCDs round-robin, so consecutive workgroups (the parts of one tile)
+// would each fetch the tile's radiance window into a different L2. Instead
+// hardware workgroup b (XCD b % 8, its k = b / 8-th workgroup there) takes
+// part k % P of tile ((k / (P G)) * 8 + b % 8) * G + (k % (P G)) / P: all
+// parts of a tile, and runs of G neighbouring tiles, share one XCD, and the
+// tiles still start in the order of the table (largest windows first).
+// Measured on the 512^2 GGX chain (whole call, FP16 / split BF16): hardware
+// order 0.597 / 0.998 ms; G = 1: 0.585 / 0.974; 4: 0.581 / 0.932; 8: 0.590 /
+// 0.965; 16: 0.625 / 1.151 (too few XCD turns to balance the mips).
+#ifndef HVX_NEAR_XCD_GROUP
+#define HVX_NEAR_XCD_GROUP 4
+#endif
+constexpr uint32_t kNearXcdGroup = HVX_NEAR_XCD_GROUP;  // 0: hardware order
+constexpr uint32_t kXcdCount = 8;
+// workgroups of the single-probe near GEMM over `tiles` tiles (padding
+// workgroups of the XCD-aware order return at once)
+__host__ __device__ constexpr uint32_t near_single_wgs(uint32_t tiles) {
+    return kNearXcdGroup ? (tiles + kXcdCount * kNearXcdGroup - 1) / (kXcdCount * kNearXcdGroup) * kXcdCount *
+                               kNearXcdGroup * kNearSingleParts
+                         : tiles * kNearSingleParts;
+}
+// logical workgroup (tile * P + part) of hardware workgroup b
+__device__ __forceinline__ uint32_t near_single_block(uint32_t b) {
+    if constexpr (kNearXcdGroup == 0) return b;
+    constexpr uint32_t P = kNearSingleParts, PG = P * kNearXcdGroup;
+    const uint32_t x = b % kXcdCount, k = b / kXcdCount;
+    const uint32_t tile = ((k / PG) * kXcdCount + x) * kNearXcdGroup + (k % PG) / P;
+    return tile * P + k % P;
+}
 
 template <bool SPLIT, int PG>
 __global__ __launch_bounds__(512) void near_gemm_kernel(Probes Q, NearTable T, const float* sc) {
     const uint2 wb = gemm_block(Q);
     const uint32_t p0 = wb.y * PG, np = min((uint32_t)PG, Q.n - p0);
     uint32_t jj = 0;
-    // workgroup b owns part b % kNearSingleParts (four waves' row tiles) of
-    // tile b / kNearSingleParts: finer work units than whole tiles balance
-    // the unequal mips across the CUs
-    const uint32_t gt = wb.x / kNearSingleParts;
+    // logical workgroup lb owns part lb % kNearSingleParts (four waves' row
+    // tiles) of tile lb / kNearSingleParts: finer work units than whole tiles
+    // balance the unequal mips across the CUs
+    const uint32_t lb = near_single_block(wb.x);
+    const uint32_t gt = lb / kNearSingleParts;
+    if (gt >= T.tiles) return;  // padding of the XCD-aware order (uniform; no barriers below)
     while (jj + 1 < T.n && gt >= T.j[jj + 1].tile_begin) ++jj;
     const NearJob& J = T.j[jj];
     const uint32_t tile = gt - J.tile_begin, lane = threadIdx.x & 63;
-    const uint32_t wave = (threadIdx.x >> 6) + (wb.x % kNearSingleParts) * 4;
+    const uint32_t wave = (threadIdx.x >> 6) + (lb % kNearSingleParts) * 4;
     const NearTile Tl = J.tiles[tile];
     // PG probes per wave: each weight fragment is loaded once for the group;
     // each probe's accumulation is the same sequence as with PG = 1
@@ -1844,7 +1876,7 @@ int launch_call(hvx_contract_plan* P, const Probes& Q, uint32_t n, uint32_t chai
     // chain: FP16 near GEMM 203 us unstaged, 246 us staged)
     const bool grouped = n > 1;
     const uint32_t dense_wgs = (P->dense.waves + 3) / 4,
-                   near_wgs = P->near.tiles * (grouped ? kNearParts : kNearSingleParts);
+                   near_wgs = grouped ? P->near.tiles * kNearParts : near_single_wgs(P->near.tiles);
     const uint32_t groups = grouped ? (n + kProbeGroup - 1) / kProbeGroup : 1;
     G.probe_fast = grouped && std::max(dense_wgs, near_wgs) <= 65535u;
     if (P->dense.n) {
