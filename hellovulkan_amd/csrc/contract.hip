@@ -544,6 +544,20 @@ __device__ __forceinline__ void kstep_pipeline(f32x4 (&acc)[PG][2 * RT], uint32_
 constexpr uint32_t kStepUnroll = HVX_KSTEP_UNROLL;
 template <bool SPLIT>
 constexpr uint32_t kstep_unroll() { return SPLIT ? (kStepUnroll + 1) / 2 : kStepUnroll; }
+// The single-probe near GEMM's own depths (FP16, split BF16). Its waves
+// (one row tile each) outnumber what the chip holds, so it gains more from
+// loads in flight per wave than from waves per SIMD. Whole FP16 call on the
+// 512^2 GGX chain: 4 steps (6 waves per SIMD) 0.605 ms, 8 (4 waves) 0.58,
+// 10 0.552, 12 (3 waves) 0.536 - 0.546, 14 0.541, 16 (2 waves) 0.549, 20
+// spills; split BF16: 4 steps 0.943 - 0.999, 5 0.913 - 0.927, 6 0.93, 8 0.908.
+#ifndef HVX_NEAR_UNROLL
+#define HVX_NEAR_UNROLL 12
+#endif
+#ifndef HVX_NEAR_UNROLL_SPLIT
+#define HVX_NEAR_UNROLL_SPLIT 5
+#endif
+template <bool SPLIT>
+constexpr uint32_t near_unroll() { return SPLIT ? HVX_NEAR_UNROLL_SPLIT : HVX_NEAR_UNROLL; }
 
 // Dense: a wave owns two row tiles (32 rows) x 32 columns over one K slice;
 // partial sums go to part[slice][row][32] and are reduced in slice order.
@@ -938,7 +952,7 @@ __global__ __launch_bounds__(512) void near_gemm_kernel(Probes Q, NearTable T, c
     // window spans at least 3 rows and 32 columns (the host pads the
     // footprint by a texel on each side), so KS >= 3.
     uint32_t ls = 0, lr = 0, lcc = 0;
-    kstep_pipeline<kstep_unroll<SPLIT>(), SPLIT, PG, kNearRows>(acc, KS, [&](Op& o) {
+    kstep_pipeline<near_unroll<SPLIT>(), SPLIT, PG, kNearRows>(acc, KS, [&](Op& o) {
         const size_t pidx = (((size_t)(Tl.y0 + (int)J.B + (int)lr) * J.Pw8 + xb + lcc * 4) * 32) + i;
         const size_t o0 = (size_t)ls * kStep;
 #pragma unroll
