@@ -38,6 +38,7 @@ CULL_BATCH_REFERENCE_TAIL = 2
 
 OPT_SHADE_EXACT = 1
 OPT_SHADE_KERNEL_EVENTS = 2
+OPT_MLP_SEPARATE_LAUNCHES = 3
 
 FILTER_DIFFUSE = 0
 FILTER_SPECULAR = 1

@@ -198,6 +198,11 @@ int hvx_ctx_set_option(hvx_ctx* ctx, int option, int64_t value) {
             }
             return HVX_OK;
         }
+        case HVX_OPT_MLP_SEPARATE_LAUNCHES:
+            if (value != 0 && value != 1)
+                return fail(ctx, HVX_ERR_INVALID_ARG, "HVX_OPT_MLP_SEPARATE_LAUNCHES is 0 or 1");
+            ctx->mlp_separate = (int)value;
+            return HVX_OK;
         default: return fail(ctx, HVX_ERR_INVALID_ARG, "unknown option %d", option);
     }
 }
@@ -207,6 +212,7 @@ int hvx_ctx_get_option(const hvx_ctx* ctx, int option, int64_t* value) {
     switch (option) {
         case HVX_OPT_SHADE_EXACT: *value = ctx->shade_exact; return HVX_OK;
         case HVX_OPT_SHADE_KERNEL_EVENTS: *value = (int64_t)(ctx->kev.size() / 2); return HVX_OK;
+        case HVX_OPT_MLP_SEPARATE_LAUNCHES: *value = ctx->mlp_separate; return HVX_OK;
         default: return HVX_ERR_INVALID_ARG;
     }
 }

@@ -38,6 +38,7 @@ struct hvx_ctx {
     std::string fixed_scratch_error;  // why the lookup failed, if it did
     // options (hvx_ctx_set_option)
     int shade_exact = 0;
+    int mlp_separate = 0;  // HVX_OPT_MLP_SEPARATE_LAUNCHES
     // HVX_OPT_SHADE_KERNEL_EVENTS: event pairs around shading-kernel launches
     // (a ring of kev.size() / 2 pairs; kev_count recorded since the last read)
     std::vector<hipEvent_t> kev;

@@ -294,6 +294,175 @@ __global__ __launch_bounds__(256) void reduce_out_kernel(const float* __restrict
     else *loss = acc * loss_scale;
 }
 
+// ------------------------------------------------ fused per-sample step
+// The per-sample part of a training step for a block of 256 samples (eight
+// waves, each two 16-sample row tiles): the three forward layers, the
+// output layer with the loss and the block's output-layer partials, and the
+// two backward products dZ2 = (dZ3 W3) [H2 > 0], dZ1 = (dZ2 W2) [H1 > 0],
+// with the activations kept in LDS between them. Every product is the MFMA
+// sequence of gemm_abt_kernel over the same BF16 operands and the output
+// part is mlp_out_kernel's, so the results equal those separate launches
+// bit for bit; the activations and gradients the weight-gradient GEMMs read
+// (H_l, dZ_l and their transposes) are written as they write them.
+struct StepArgs {
+    const bf16* X;  // [B][kIn] encodings of the batch
+    const float* P;
+    const bf16 *W[3], *W2T, *W3T;
+    const float4* tgt;
+    uint32_t B;
+    float inv_norm;
+    bf16 *H[3], *HT[3];
+    bf16 *dZ[3], *dZT[3];  // dZ1, dZ2, dZ3 and their transposes
+    float* part;            // [B / 256][kOutPart]
+};
+constexpr int kHp = kH + 8;  // LDS row of 64 units (bf16), padded
+
+// C block (16 rows x 64 columns) of A Bt^T: row i of A at arow (this lane's
+// 8 of each 32 K), Bt row i at bt; gemm_abt_kernel's sequence of MFMAs
+template <int KD>
+__device__ __forceinline__ void step_tile(const bf16* arow, const bf16* bt, f32x4 acc[4]) {
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) acc[nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < KD / 32; ++s) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(arow + 32 * s);
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+            const bf16x8 b = *reinterpret_cast<const bf16x8*>(bt + nb * 16 * KD + 32 * s);
+            acc[nb] = hvx::mfma16x16x32(a, b, acc[nb]);
+        }
+    }
+}
+
+// Epilogue of one C block (local rows lr0 .. lr0 + 15 of the block): FWD
+// (bias and ReLU) or BWD (zero where the mask activation is not positive),
+// rounded to BF16 into C, CT (global) and the LDS rows out
+template <bool FWD>
+__device__ __forceinline__ void step_store(const f32x4 acc[4], uint32_t b0, uint32_t lr0, uint32_t B, const float* bias,
+                                           const bf16 (*mask)[kHp], bf16* C, bf16* CT, bf16 (*out)[kHp]) {
+    const uint32_t lane = threadIdx.x & 63, i = lane & 15, gq = lane >> 4;
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) {
+        const uint32_t col = nb * 16 + i;
+        bf16 ht[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const uint32_t lrow = lr0 + 4 * gq + r;
+            float v = acc[nb][r];
+            if (FWD) v = fmaxf(v + bias[col], 0.0f);
+            else if (!((float)mask[lrow][col] > 0.0f)) v = 0.0f;
+            const bf16 h = (bf16)v;
+            C[(size_t)(b0 + lrow) * kH + col] = h;
+            if (out) out[lrow][col] = h;
+            ht[r] = h;
+        }
+        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+        *reinterpret_cast<bf16x4*>(CT + (size_t)col * B + b0 + lr0 + 4 * gq) = bf16x4{ht[0], ht[1], ht[2], ht[3]};
+    }
+}
+
+constexpr uint32_t kStepWaves = 8;                     // waves per 256-sample block
+constexpr uint32_t kStepTiles = 256 / 16 / kStepWaves;  // 16-sample row tiles per wave
+__global__ __launch_bounds__(64 * kStepWaves) void mlp_step_kernel(StepArgs g) {
+    __shared__ __align__(16) bf16 sH[3][256][kHp];  // H1, H2, H3; H3's rows later hold dZ2
+    __shared__ __align__(16) bf16 sZ[256][kHp];     // dZ3
+    __shared__ float sdY[256][3];
+    __shared__ float sl[256];
+    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63, i = lane & 15, gq = lane >> 4;
+    const uint32_t b0 = blockIdx.x * 256, B = g.B;
+    const int bias[3] = {hvx::kMlpB1, hvx::kMlpB2, hvx::kMlpB3};
+    // forward: each wave its own rows, its row tiles interleaved
+#pragma unroll 1
+    for (int l = 0; l < 3; ++l) {
+        f32x4 acc[kStepTiles][4];
+#pragma unroll
+        for (uint32_t rt = 0; rt < kStepTiles; ++rt) {
+            const uint32_t lr0 = (wave * kStepTiles + rt) * 16;
+            if (l == 0)
+                step_tile<kIn>(g.X + (size_t)(b0 + lr0 + i) * kIn + 8 * gq, g.W[0] + (size_t)i * kIn + 8 * gq, acc[rt]);
+            else
+                step_tile<kH>(&sH[l - 1][lr0 + i][8 * gq], g.W[l] + (size_t)i * kH + 8 * gq, acc[rt]);
+        }
+#pragma unroll
+        for (uint32_t rt = 0; rt < kStepTiles; ++rt)
+            step_store<true>(acc[rt], b0, (wave * kStepTiles + rt) * 16, B, g.P + bias[l], nullptr, g.H[l], g.HT[l],
+                             sH[l]);
+        __syncthreads();
+    }
+    // output layer, loss, dZ3 (one thread per sample)
+    const float* W4 = g.P + hvx::kMlpW4;
+    const float* b4 = g.P + hvx::kMlpB4;
+    if (t < 256) {
+        float h[kH];
+#pragma unroll
+        for (int k = 0; k < kH; ++k) h[k] = (float)sH[2][t][k];
+        float y[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) y[c] = hvx::mlp_output(h, W4 + c * kH, b4[c]);
+        const float4 tv = g.tgt[b0 + t];
+        const float e0 = y[0] - tv.x, e1 = y[1] - tv.y, e2 = y[2] - tv.z;
+        sdY[t][0] = e0 * g.inv_norm;
+        sdY[t][1] = e1 * g.inv_norm;
+        sdY[t][2] = e2 * g.inv_norm;
+        sl[t] = (e0 * e0 + e1 * e1) + e2 * e2;
+#pragma unroll
+        for (int k8 = 0; k8 < kH / 8; ++k8) {
+            bf16x8 pk;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int k = 8 * k8 + q;
+                const float dh = (sdY[t][0] * W4[k] + sdY[t][1] * W4[kH + k]) + sdY[t][2] * W4[2 * kH + k];
+                const bf16 v = (bf16)(h[k] > 0.0f ? dh : 0.0f);
+                pk[q] = v;
+                g.dZT[2][(size_t)k * B + b0 + t] = v;
+                sZ[t][k] = v;
+            }
+            reinterpret_cast<bf16x8*>(g.dZ[2] + (size_t)(b0 + t) * kH)[k8] = pk;
+        }
+    }
+    __syncthreads();
+    float* pp = g.part + (size_t)blockIdx.x * kOutPart;
+    if (t < 3 * kH) {
+        const uint32_t c = t / kH, k = t % kH;
+        float acc = 0.0f;
+        for (uint32_t j = 0; j < 256; ++j) acc += sdY[j][c] * (float)sH[2][j][k];
+        pp[t] = acc;
+    } else if (t < 3 * kH + 3) {
+        const uint32_t c = t - 3 * kH;
+        float acc = 0.0f;
+        for (uint32_t j = 0; j < 256; ++j) acc += sdY[j][c];
+        pp[t] = acc;
+    }
+    for (uint32_t hh = 128; hh > 0; hh >>= 1) {
+        __syncthreads();
+        if (t < hh) sl[t] += sl[t + hh];
+    }
+    if (t == 0) pp[kOutPart - 1] = sl[0];
+    __syncthreads();  // H3 is read for the last time above
+    // dZ2 = (dZ3 W3) [H2 > 0] into H3's rows, then dZ1 = (dZ2 W2) [H1 > 0]
+    {
+        f32x4 acc[kStepTiles][4];
+#pragma unroll
+        for (uint32_t rt = 0; rt < kStepTiles; ++rt)
+            step_tile<kH>(&sZ[(wave * kStepTiles + rt) * 16 + i][8 * gq], g.W3T + (size_t)i * kH + 8 * gq, acc[rt]);
+#pragma unroll
+        for (uint32_t rt = 0; rt < kStepTiles; ++rt)
+            step_store<false>(acc[rt], b0, (wave * kStepTiles + rt) * 16, B, nullptr, sH[1], g.dZ[1], g.dZT[1], sH[2]);
+    }
+    __syncthreads();
+    {
+        f32x4 acc[kStepTiles][4];
+#pragma unroll
+        for (uint32_t rt = 0; rt < kStepTiles; ++rt)
+            step_tile<kH>(&sH[2][(wave * kStepTiles + rt) * 16 + i][8 * gq], g.W2T + (size_t)i * kH + 8 * gq,
+                          acc[rt]);
+#pragma unroll
+        for (uint32_t rt = 0; rt < kStepTiles; ++rt)
+            step_store<false>(acc[rt], b0, (wave * kStepTiles + rt) * 16, B, nullptr, sH[0], g.dZ[0], g.dZT[0],
+                              nullptr);
+    }
+}
+
 // out[i] = scale * sum_s part[s * stride + i] for many slices: a workgroup
 // takes 64 outputs and 16 slice ranges (one per wave), each summed in slice
 // order, then adds the 16 range sums in a fixed tree
@@ -471,7 +640,7 @@ struct hvx_mlp {
     bf16 *W1 = nullptr, *W2 = nullptr, *W3 = nullptr, *W2T = nullptr, *W3T = nullptr;
     uint32_t cap = 0;  // batch capacity
     bf16 *X0 = nullptr, *X0T = nullptr, *H[3] = {}, *HT[3] = {}, *dZa = nullptr, *dZaT = nullptr, *dZb = nullptr,
-         *dZbT = nullptr;
+         *dZbT = nullptr, *dZc = nullptr, *dZcT = nullptr;
     float *part = nullptr, *opart = nullptr;
     uint32_t splitk = 1;
     uint64_t steps_done = 0;
@@ -530,6 +699,7 @@ hipError_t ensure_batch(hvx_mlp* M, uint32_t B) {
         if ((e = dalloc(L, &M->H[l], (size_t)B * kH)) || (e = dalloc(L, &M->HT[l], (size_t)B * kH))) return e;
     if ((e = dalloc(L, &M->dZa, (size_t)B * kH)) || (e = dalloc(L, &M->dZaT, (size_t)B * kH)) ||
         (e = dalloc(L, &M->dZb, (size_t)B * kH)) || (e = dalloc(L, &M->dZbT, (size_t)B * kH)) ||
+        (e = dalloc(L, &M->dZc, (size_t)B * kH)) || (e = dalloc(L, &M->dZcT, (size_t)B * kH)) ||
         (e = dalloc(L, &M->part, (size_t)M->splitk * (kH * kIn + kH))) ||
         (e = dalloc(L, &M->opart, (size_t)(B / 256) * kOutPart)))
         return e;
@@ -569,8 +739,15 @@ void forward(hvx_mlp* M, const float4* in, uint32_t n, uint32_t B, hipStream_t s
 // gradient of the mean squared log-radiance error over B samples into G (and
 // the loss into *loss); x0 / x0t (row stride ldx): the samples' encoding, if
 // already made
+void gradient_fused(hvx_mlp* M, const float4* in, const float4* tgt, uint32_t B, float* G, float* loss,
+                    hipStream_t s, const bf16* x0, const bf16* x0t, uint32_t ldx);
+
 void gradient(hvx_mlp* M, const float4* in, const float4* tgt, uint32_t B, float* G, float* loss, hipStream_t s,
               const bf16* x0 = nullptr, const bf16* x0t = nullptr, uint32_t ldx = 0) {
+    if (!M->ctx->mlp_separate) {
+        gradient_fused(M, in, tgt, B, G, loss, s, x0, x0t, ldx);
+        return;
+    }
     forward(M, in, B, B, s, x0);
     const uint32_t nblk = B / 256;
     hipLaunchKernelGGL(mlp_out_kernel<true>, dim3(nblk), dim3(256), 0, s, M->H[2], M->P, tgt, B, B,
@@ -615,6 +792,58 @@ void gradient(hvx_mlp* M, const float4* in, const float4* tgt, uint32_t B, float
             std::swap(dz, nz);
             std::swap(dzT, nzT);
         }
+    }
+}
+
+// The same gradient with the per-sample work in one launch (mlp_step_kernel),
+// then the three weight-gradient GEMMs and their reductions as in gradient().
+void gradient_fused(hvx_mlp* M, const float4* in, const float4* tgt, uint32_t B, float* G, float* loss,
+                    hipStream_t s, const bf16* x0, const bf16* x0t, uint32_t ldx) {
+    if (!x0)
+        hipLaunchKernelGGL(mlp_pe_kernel, dim3((B + 63) / 64), dim3(64), 0, s, in, B, B, M->lod_max, M->X0, M->X0T);
+    const uint32_t nblk = B / 256;
+    StepArgs a{};
+    a.X = x0 ? x0 : M->X0;
+    a.P = M->P;
+    a.W[0] = M->W1;
+    a.W[1] = M->W2;
+    a.W[2] = M->W3;
+    a.W2T = M->W2T;
+    a.W3T = M->W3T;
+    a.tgt = tgt;
+    a.B = B;
+    a.inv_norm = 2.0f / (3.0f * (float)B);
+    for (int l = 0; l < 3; ++l) {
+        a.H[l] = M->H[l];
+        a.HT[l] = M->HT[l];
+    }
+    a.dZ[0] = M->dZc;
+    a.dZT[0] = M->dZcT;
+    a.dZ[1] = M->dZb;
+    a.dZT[1] = M->dZbT;
+    a.dZ[2] = M->dZa;
+    a.dZT[2] = M->dZaT;
+    a.part = M->opart;
+    hipLaunchKernelGGL(mlp_step_kernel, dim3(nblk), dim3(64 * kStepWaves), 0, s, a);
+    hipLaunchKernelGGL(reduce_out_kernel, dim3(1), dim3(256), 0, s, M->opart, nblk, G + hvx::kMlpW4,
+                       1.0f / (3.0f * (float)B), loss);
+    const int wofs[3] = {hvx::kMlpW1, hvx::kMlpW2, hvx::kMlpW3};
+    for (int l = 2; l >= 0; --l) {
+        const uint32_t K = l == 0 ? kIn : kH;
+        GemmArgs g{};
+        g.A = a.dZT[l];
+        g.Bt = l == 0 ? (x0t ? x0t : M->X0T) : M->HT[l - 1];
+        g.ldb = l == 0 && x0t ? ldx : 0;
+        g.M = kH;
+        g.Nc = K;
+        g.Kd = B;
+        g.splitk = M->splitk;
+        g.part = M->part;
+        g.pstride = kH * K + kH;
+        g.colsum = 1;
+        gemm<EPI_PART>(g, s);
+        hipLaunchKernelGGL(reduce_slices_kernel, dim3((kH * K + kH + 63) / 64), dim3(64 * kRedGroups), 0, s,
+                           M->part, M->splitk, (uint32_t)(kH * K + kH), (uint32_t)(kH * K + kH), 1.0f, G + wofs[l]);
     }
 }
 

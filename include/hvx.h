@@ -185,7 +185,13 @@ typedef enum hvx_option {
      * each launch of the shading kernel itself (not the binning and light
      * table launches of the stage), keeping the last k pairs, for
      * hvx_ctx_kernel_times. For measurement. */
-    HVX_OPT_SHADE_KERNEL_EVENTS = 2
+    HVX_OPT_SHADE_KERNEL_EVENTS = 2,
+    /* 0 (default): a network training step (hvx_mlp_gradient, each step of
+     * hvx_mlp_train) runs its per-sample work (three forward layers, output
+     * layer and loss, two backward layer products) in one launch; 1: as
+     * separate layer launches. Both give the same bits; 1 is the reference
+     * the fused step is tested against. */
+    HVX_OPT_MLP_SEPARATE_LAUNCHES = 3
 } hvx_option;
 int hvx_ctx_set_option(hvx_ctx* ctx, int option, int64_t value);
 int hvx_ctx_get_option(const hvx_ctx* ctx, int option, int64_t* value);

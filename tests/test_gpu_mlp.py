@@ -58,6 +58,40 @@ def test_gradient_matches_oracle(ctx, seed, lod_max):
     mlp.close()
 
 
+@pytest.mark.parametrize("batch", [256, 1024])
+def test_fused_step_equals_separate_launches(ctx, batch):
+    """The training step's fused per-sample kernel against the same step run
+    as separate layer launches: the gradient, the loss and the parameters
+    after training steps (pre-encoded samples, revisited, and encoded per
+    step) are bit-identical."""
+    import torch
+    rng = np.random.default_rng(batch)
+    n = 4 * batch
+    x, t = _batch(rng, n, 3.0)
+    dx, dt = to_dev(x), to_dev(t)
+    out = {}
+    for sep in (1, 0):
+        ctx.set_option(abi.OPT_MLP_SEPARATE_LAUNCHES, sep)
+        try:
+            mlp = abi.Mlp(ctx, 3.0, 17)
+            g = torch.zeros(abi.MLP_PARAMS, dtype=torch.float32, device="cuda")
+            loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+            mlp.gradient(dx, dt, batch, g, loss)
+            curve = torch.zeros(12, dtype=torch.float32, device="cuda")
+            mlp.train(dx, dt, n, batch, 12, 3e-3, curve)        # revisits: encoded once
+            curve2 = torch.zeros(2, dtype=torch.float32, device="cuda")
+            mlp.train(dx, dt, n, batch, 2, 1e-3, curve2)        # one pass: encoded per step
+            torch.cuda.synchronize()
+            out[sep] = (g.cpu().numpy(), loss.cpu().numpy(), curve.cpu().numpy(), curve2.cpu().numpy(),
+                        mlp.params())
+            mlp.close()
+        finally:
+            ctx.set_option(abi.OPT_MLP_SEPARATE_LAUNCHES, 0)
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32))
+    assert out[0][2][-1] < out[0][2][0]
+
+
 def test_eval_matches_oracle_forward(ctx):
     import torch
     mlp = abi.Mlp(ctx, 2.0, 11)
