@@ -206,18 +206,21 @@ __global__ __launch_bounds__(256) void gemm_abt_kernel(GemmArgs g) {
 
 // ------------------------------------------------------- output and loss
 constexpr int kOutPart = kH * 3 + 3 + 1;  // dW4, db4, loss
+// samples per output-layer block: the loss and the output layer's gradient
+// are summed per block of this many samples, then over the blocks in order
+constexpr uint32_t kOutBlock = 128;
 
-// Training: one thread per sample (B a multiple of 256). Evaluation (TRAIN
-// false): writes radiance exp(y) - eps for samples < n.
+// Training: one thread per sample (B a multiple of kOutBlock). Evaluation
+// (TRAIN false): writes radiance exp(y) - eps for samples < n.
 template <bool TRAIN>
-__global__ __launch_bounds__(256) void mlp_out_kernel(const bf16* __restrict__ H3, const float* __restrict__ P,
+__global__ __launch_bounds__(kOutBlock) void mlp_out_kernel(const bf16* __restrict__ H3, const float* __restrict__ P,
                                                       const float4* __restrict__ tgt, uint32_t n, uint32_t B,
                                                       float inv_norm, float* __restrict__ Y, bf16* __restrict__ dZ,
                                                       bf16* __restrict__ dZT, float* __restrict__ part) {
-    __shared__ float sdY[256][3];
-    __shared__ float sl[256];
-    __shared__ bf16 sh3[kH][256 + 8];  // the block's H3 transposed (padded rows)
-    const uint32_t t = threadIdx.x, b = blockIdx.x * 256 + t;
+    __shared__ float sdY[kOutBlock][3];
+    __shared__ float sl[kOutBlock];
+    __shared__ bf16 sh3[kH][kOutBlock + 8];  // the block's H3 transposed (padded rows)
+    const uint32_t t = threadIdx.x, b = blockIdx.x * kOutBlock + t;
     const float* W4 = P + hvx::kMlpW4;
     const float* b4 = P + hvx::kMlpB4;
     float h[kH];
@@ -260,18 +263,18 @@ __global__ __launch_bounds__(256) void mlp_out_kernel(const bf16* __restrict__ H
     }
     __syncthreads();
     float* pp = part + (size_t)blockIdx.x * kOutPart;
-    if (t < 3 * kH) {
-        const uint32_t c = t / kH, k = t % kH;
+    for (uint32_t o = t; o < (uint32_t)kOutPart - 1; o += kOutBlock) {
         float acc = 0.0f;
-        for (uint32_t j = 0; j < 256; ++j) acc += sdY[j][c] * (float)sh3[k][j];
-        pp[t] = acc;
-    } else if (t < 3 * kH + 3) {
-        const uint32_t c = t - 3 * kH;
-        float acc = 0.0f;
-        for (uint32_t j = 0; j < 256; ++j) acc += sdY[j][c];
-        pp[t] = acc;
+        if (o < 3 * kH) {
+            const uint32_t c = o / kH, k = o % kH;
+            for (uint32_t j = 0; j < kOutBlock; ++j) acc += sdY[j][c] * (float)sh3[k][j];
+        } else {
+            const uint32_t c = o - 3 * kH;
+            for (uint32_t j = 0; j < kOutBlock; ++j) acc += sdY[j][c];
+        }
+        pp[o] = acc;
     }
-    for (uint32_t hh = 128; hh > 0; hh >>= 1) {
+    for (uint32_t hh = kOutBlock / 2; hh > 0; hh >>= 1) {
         __syncthreads();
         if (t < hh) sl[t] += sl[t + hh];
     }
@@ -295,8 +298,8 @@ __global__ __launch_bounds__(256) void reduce_out_kernel(const float* __restrict
 }
 
 // ------------------------------------------------ fused per-sample step
-// The per-sample part of a training step for a block of 256 samples (eight
-// waves, each two 16-sample row tiles): the three forward layers, the
+// The per-sample part of a training step for a block of kOutBlock samples
+// (eight waves, one 16-sample row tile each): the three forward layers, the
 // output layer with the loss and the block's output-layer partials, and the
 // two backward products dZ2 = (dZ3 W3) [H2 > 0], dZ1 = (dZ2 W2) [H1 > 0],
 // with the activations kept in LDS between them. Every product is the MFMA
@@ -313,7 +316,7 @@ struct StepArgs {
     float inv_norm;
     bf16 *H[3], *HT[3];
     bf16 *dZ[3], *dZT[3];  // dZ1, dZ2, dZ3 and their transposes
-    float* part;            // [B / 256][kOutPart]
+    float* part;            // [B / kOutBlock][kOutPart]
 };
 constexpr int kHp = kH + 8;  // LDS row of 64 units (bf16), padded
 
@@ -361,15 +364,16 @@ __device__ __forceinline__ void step_store(const f32x4 acc[4], uint32_t b0, uint
     }
 }
 
-constexpr uint32_t kStepWaves = 8;                     // waves per 256-sample block
-constexpr uint32_t kStepTiles = 256 / 16 / kStepWaves;  // 16-sample row tiles per wave
+constexpr uint32_t kStepWaves = 8;                           // waves per block
+constexpr uint32_t kStepTiles = kOutBlock / 16 / kStepWaves;  // 16-sample row tiles per wave
+static_assert(kStepTiles >= 1 && kStepTiles * 16 * kStepWaves == kOutBlock, "whole row tiles per wave");
 __global__ __launch_bounds__(64 * kStepWaves) void mlp_step_kernel(StepArgs g) {
-    __shared__ __align__(16) bf16 sH[3][256][kHp];  // H1, H2, H3; H3's rows later hold dZ2
-    __shared__ __align__(16) bf16 sZ[256][kHp];     // dZ3
-    __shared__ float sdY[256][3];
-    __shared__ float sl[256];
+    __shared__ __align__(16) bf16 sH[3][kOutBlock][kHp];  // H1, H2, H3; H3's rows later hold dZ2
+    __shared__ __align__(16) bf16 sZ[kOutBlock][kHp];     // dZ3
+    __shared__ float sdY[kOutBlock][3];
+    __shared__ float sl[kOutBlock];
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63, i = lane & 15, gq = lane >> 4;
-    const uint32_t b0 = blockIdx.x * 256, B = g.B;
+    const uint32_t b0 = blockIdx.x * kOutBlock, B = g.B;
     const int bias[3] = {hvx::kMlpB1, hvx::kMlpB2, hvx::kMlpB3};
     // forward: each wave its own rows, its row tiles interleaved
 #pragma unroll 1
@@ -392,7 +396,7 @@ __global__ __launch_bounds__(64 * kStepWaves) void mlp_step_kernel(StepArgs g) {
     // output layer, loss, dZ3 (one thread per sample)
     const float* W4 = g.P + hvx::kMlpW4;
     const float* b4 = g.P + hvx::kMlpB4;
-    if (t < 256) {
+    if (t < kOutBlock) {
         float h[kH];
 #pragma unroll
         for (int k = 0; k < kH; ++k) h[k] = (float)sH[2][t][k];
@@ -422,18 +426,19 @@ __global__ __launch_bounds__(64 * kStepWaves) void mlp_step_kernel(StepArgs g) {
     }
     __syncthreads();
     float* pp = g.part + (size_t)blockIdx.x * kOutPart;
+    static_assert(64 * kStepWaves >= (uint32_t)kOutPart - 1, "one thread per output-layer partial");
     if (t < 3 * kH) {
         const uint32_t c = t / kH, k = t % kH;
         float acc = 0.0f;
-        for (uint32_t j = 0; j < 256; ++j) acc += sdY[j][c] * (float)sH[2][j][k];
+        for (uint32_t j = 0; j < kOutBlock; ++j) acc += sdY[j][c] * (float)sH[2][j][k];
         pp[t] = acc;
     } else if (t < 3 * kH + 3) {
         const uint32_t c = t - 3 * kH;
         float acc = 0.0f;
-        for (uint32_t j = 0; j < 256; ++j) acc += sdY[j][c];
+        for (uint32_t j = 0; j < kOutBlock; ++j) acc += sdY[j][c];
         pp[t] = acc;
     }
-    for (uint32_t hh = 128; hh > 0; hh >>= 1) {
+    for (uint32_t hh = kOutBlock / 2; hh > 0; hh >>= 1) {
         __syncthreads();
         if (t < hh) sl[t] += sl[t + hh];
     }
@@ -701,7 +706,7 @@ hipError_t ensure_batch(hvx_mlp* M, uint32_t B) {
         (e = dalloc(L, &M->dZb, (size_t)B * kH)) || (e = dalloc(L, &M->dZbT, (size_t)B * kH)) ||
         (e = dalloc(L, &M->dZc, (size_t)B * kH)) || (e = dalloc(L, &M->dZcT, (size_t)B * kH)) ||
         (e = dalloc(L, &M->part, (size_t)M->splitk * (kH * kIn + kH))) ||
-        (e = dalloc(L, &M->opart, (size_t)(B / 256) * kOutPart)))
+        (e = dalloc(L, &M->opart, (size_t)(B / kOutBlock) * kOutPart)))
         return e;
     M->cap = B;
     return hipSuccess;
@@ -749,8 +754,8 @@ void gradient(hvx_mlp* M, const float4* in, const float4* tgt, uint32_t B, float
         return;
     }
     forward(M, in, B, B, s, x0);
-    const uint32_t nblk = B / 256;
-    hipLaunchKernelGGL(mlp_out_kernel<true>, dim3(nblk), dim3(256), 0, s, M->H[2], M->P, tgt, B, B,
+    const uint32_t nblk = B / kOutBlock;
+    hipLaunchKernelGGL(mlp_out_kernel<true>, dim3(nblk), dim3(kOutBlock), 0, s, M->H[2], M->P, tgt, B, B,
                        2.0f / (3.0f * (float)B), nullptr, M->dZa, M->dZaT, M->opart);
     hipLaunchKernelGGL(reduce_out_kernel, dim3(1), dim3(256), 0, s, M->opart, nblk, G + hvx::kMlpW4,
                        1.0f / (3.0f * (float)B), loss);
@@ -801,7 +806,7 @@ void gradient_fused(hvx_mlp* M, const float4* in, const float4* tgt, uint32_t B,
                     hipStream_t s, const bf16* x0, const bf16* x0t, uint32_t ldx) {
     if (!x0)
         hipLaunchKernelGGL(mlp_pe_kernel, dim3((B + 63) / 64), dim3(64), 0, s, in, B, B, M->lod_max, M->X0, M->X0T);
-    const uint32_t nblk = B / 256;
+    const uint32_t nblk = B / kOutBlock;
     StepArgs a{};
     a.X = x0 ? x0 : M->X0;
     a.P = M->P;
