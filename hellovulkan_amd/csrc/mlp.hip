@@ -282,19 +282,30 @@ __global__ __launch_bounds__(kOutBlock) void mlp_out_kernel(const bf16* __restri
 }
 
 
-// The output layer's block partials (kOutPart per block: dW4, db4, loss),
-// summed in block order: dW4 and db4 into out, the loss (scaled) into *loss
-// when asked for
-__global__ __launch_bounds__(256) void reduce_out_kernel(const float* __restrict__ part, uint32_t blocks,
-                                                         float* __restrict__ out, float loss_scale,
-                                                         float* __restrict__ loss) {
-    const uint32_t i = threadIdx.x;
-    if (i >= (uint32_t)kOutPart || (i == (uint32_t)kOutPart - 1 && !loss)) return;
+// The output layer's block partials (kOutPart per block: dW4, db4, loss):
+// a workgroup takes 64 of them and 16 block ranges (one per wave), each
+// summed in block order, then adds the 16 range sums in a fixed tree; dW4
+// and db4 into out, the loss (scaled) into *loss when asked for
+constexpr uint32_t kOutRedGroups = 16;
+__global__ __launch_bounds__(64 * kOutRedGroups) void reduce_out_kernel(const float* __restrict__ part,
+                                                                        uint32_t blocks, float* __restrict__ out,
+                                                                        float loss_scale, float* __restrict__ loss) {
+    __shared__ float sh[kOutRedGroups][64];
+    const uint32_t l = threadIdx.x & 63, gq = threadIdx.x >> 6, i = blockIdx.x * 64 + l;
     float acc = 0.0f;
-#pragma unroll 8
-    for (uint32_t s = 0; s < blocks; ++s) acc += part[(size_t)s * kOutPart + i];
-    if (i < (uint32_t)kOutPart - 1) out[i] = acc;
-    else *loss = acc * loss_scale;
+    if (i < (uint32_t)kOutPart) {
+        const uint32_t s0 = gq * blocks / kOutRedGroups, s1 = (gq + 1) * blocks / kOutRedGroups;
+#pragma unroll 4
+        for (uint32_t s = s0; s < s1; ++s) acc += part[(size_t)s * kOutPart + i];
+    }
+    sh[gq][l] = acc;
+    for (uint32_t h = kOutRedGroups / 2; h > 0; h >>= 1) {
+        __syncthreads();
+        if (gq < h) sh[gq][l] += sh[gq + h][l];
+    }
+    if (gq != 0 || i >= (uint32_t)kOutPart) return;
+    if (i < (uint32_t)kOutPart - 1) out[i] = sh[0][l];
+    else if (loss) *loss = sh[0][l] * loss_scale;
 }
 
 // ------------------------------------------------ fused per-sample step
@@ -757,7 +768,7 @@ void gradient(hvx_mlp* M, const float4* in, const float4* tgt, uint32_t B, float
     const uint32_t nblk = B / kOutBlock;
     hipLaunchKernelGGL(mlp_out_kernel<true>, dim3(nblk), dim3(kOutBlock), 0, s, M->H[2], M->P, tgt, B, B,
                        2.0f / (3.0f * (float)B), nullptr, M->dZa, M->dZaT, M->opart);
-    hipLaunchKernelGGL(reduce_out_kernel, dim3(1), dim3(256), 0, s, M->opart, nblk, G + hvx::kMlpW4,
+    hipLaunchKernelGGL(reduce_out_kernel, dim3((kOutPart + 63) / 64), dim3(64 * kOutRedGroups), 0, s, M->opart, nblk, G + hvx::kMlpW4,
                        1.0f / (3.0f * (float)B), loss);
     const int wofs[3] = {hvx::kMlpW1, hvx::kMlpW2, hvx::kMlpW3};
     const bf16* WT[3] = {nullptr, M->W2T, M->W3T};
@@ -830,7 +841,7 @@ void gradient_fused(hvx_mlp* M, const float4* in, const float4* tgt, uint32_t B,
     a.dZT[2] = M->dZaT;
     a.part = M->opart;
     hipLaunchKernelGGL(mlp_step_kernel, dim3(nblk), dim3(64 * kStepWaves), 0, s, a);
-    hipLaunchKernelGGL(reduce_out_kernel, dim3(1), dim3(256), 0, s, M->opart, nblk, G + hvx::kMlpW4,
+    hipLaunchKernelGGL(reduce_out_kernel, dim3((kOutPart + 63) / 64), dim3(64 * kOutRedGroups), 0, s, M->opart, nblk, G + hvx::kMlpW4,
                        1.0f / (3.0f * (float)B), loss);
     const int wofs[3] = {hvx::kMlpW1, hvx::kMlpW2, hvx::kMlpW3};
     for (int l = 2; l >= 0; --l) {
