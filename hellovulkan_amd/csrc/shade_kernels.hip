@@ -537,34 +537,33 @@ __global__ __launch_bounds__(256) void tile_bounds_kernel(ShadeArgs a) {
         for (uint32_t t = c + 1; t <= T; ++t) b[t] = n;
 }
 
-// One workgroup per screen tile: moves the tile's segment of the binned RGB
-// planes to pixel order through an LDS image of the tile, one plane at a
-// time, so that both the shading kernel's stores (fragment order) and these
-// (tile rows) are coalesced.
+// One workgroup per screen tile and RGB plane: moves the tile's segment of
+// the binned plane to pixel order through an LDS image of the tile, so that
+// both the shading kernel's stores (fragment order) and these (tile rows)
+// are coalesced. A workgroup per plane rather than per tile: with two
+// tile images per CU, 576 tiles (3840x2160) would leave a second round of
+// 64 workgroups; 1,728 shorter ones balance across the CUs.
 __global__ __launch_bounds__(kBinThreads) void unbin_kernel(ShadeArgs a) {
     extern __shared__ float s_img[];
-    const uint32_t cx = blockIdx.x % a.X, cy = blockIdx.x / a.X, t = threadIdx.x;
+    const uint32_t tile = blockIdx.x / 3, c = blockIdx.x % 3;
+    const uint32_t cx = tile % a.X, cy = tile / a.X, t = threadIdx.x;
     const uint32_t x0 = a.tiles[cx], x1 = a.tiles[cx + 1];
     const uint32_t y0 = a.tiles[a.X + 1 + cy], y1 = a.tiles[a.X + 1 + cy + 1];
     const uint32_t w = x1 - x0, area = w * (y1 - y0), nfrag = a.W * a.H;
     if (area == 0) return;  // uniform over the workgroup
     const uint32_t start = y0 * a.W + (y1 - y0) * x0;
     if (area > a.unbin_cap) {  // larger than the bound the image was sized by: scatter directly
-        for (uint32_t c = 0; c < 3; ++c)
-            for (uint32_t i = t; i < area; i += kBinThreads)
-                a.out[(size_t)c * nfrag + a.order[start + i]] = a.out_binned[(size_t)c * nfrag + start + i];
+        for (uint32_t i = t; i < area; i += kBinThreads)
+            a.out[(size_t)c * nfrag + a.order[start + i]] = a.out_binned[(size_t)c * nfrag + start + i];
         return;
     }
-    for (uint32_t c = 0; c < 3; ++c) {
-        for (uint32_t i = t; i < area; i += kBinThreads) {
-            const uint32_t idx = a.order[start + i];
-            s_img[(idx / a.W - y0) * w + (idx % a.W - x0)] = a.out_binned[(size_t)c * nfrag + start + i];
-        }
-        __syncthreads();
-        for (uint32_t i = t; i < area; i += kBinThreads)
-            a.out[(size_t)c * nfrag + (y0 + i / w) * a.W + x0 + i % w] = s_img[i];
-        __syncthreads();
+    for (uint32_t i = t; i < area; i += kBinThreads) {
+        const uint32_t idx = a.order[start + i];
+        s_img[(idx / a.W - y0) * w + (idx % a.W - x0)] = a.out_binned[(size_t)c * nfrag + start + i];
     }
+    __syncthreads();
+    for (uint32_t i = t; i < area; i += kBinThreads)
+        a.out[(size_t)c * nfrag + (y0 + i / w) * a.W + x0 + i % w] = s_img[i];
 }
 
 // Binning key of a fragment: its depth slice, then (rclasses = 2) whether it
@@ -592,7 +591,21 @@ __global__ __launch_bounds__(kBinThreads) void bin_kernel(ShadeArgs a) {
     for (uint32_t z = t; z < a.Z * a.rclasses; z += kBinThreads) s_hist[z] = 0;
     __syncthreads();
     const uint32_t nb = a.Z * a.rclasses;  // bins: (slice, roughness class)
-    for (uint32_t i = t; i < area; i += kBinThreads) {
+    // the bins of a thread's first kBinCache pixels stay in registers for
+    // the second pass (a 120 x 120 tile takes 15 rounds)
+    constexpr uint32_t kBinCache = 16;
+    uint32_t cache[kBinCache];
+#pragma unroll
+    for (uint32_t k = 0; k < kBinCache; ++k) {
+        const uint32_t i = t + k * kBinThreads;
+        cache[k] = 0;
+        if (i < area) {
+            const uint32_t idx = (y0 + i / w) * a.W + x0 + i % w;
+            cache[k] = bin_of(a, idx);
+            atomicAdd(&s_hist[cache[k]], 1u);
+        }
+    }
+    for (uint32_t i = t + kBinCache * kBinThreads; i < area; i += kBinThreads) {
         const uint32_t idx = (y0 + i / w) * a.W + x0 + i % w;
         atomicAdd(&s_hist[bin_of(a, idx)], 1u);
     }
@@ -621,7 +634,12 @@ __global__ __launch_bounds__(kBinThreads) void bin_kernel(ShadeArgs a) {
     __syncthreads();
     if (t < nb) s_hist[t] = start + s_wave[wv] + inc - v;
     __syncthreads();
-    for (uint32_t i = t; i < area; i += kBinThreads) {
+#pragma unroll
+    for (uint32_t k = 0; k < kBinCache; ++k) {
+        const uint32_t i = t + k * kBinThreads;
+        if (i < area) a.order[atomicAdd(&s_hist[cache[k]], 1u)] = (y0 + i / w) * a.W + x0 + i % w;
+    }
+    for (uint32_t i = t + kBinCache * kBinThreads; i < area; i += kBinThreads) {
         const uint32_t idx = (y0 + i / w) * a.W + x0 + i % w;
         a.order[atomicAdd(&s_hist[bin_of(a, idx)], 1u)] = idx;
     }
@@ -876,7 +894,7 @@ hipError_t launch_shade(const hvx_frag_soa& fr, const hvx_camera_ubo& cam, const
         else hipLaunchKernelGGL((shade_kernel<false, false>), grid, block, 0, s, a);
     }
     if (ev_end) (void)hipEventRecord(ev_end, s);
-    if (staged) hipLaunchKernelGGL(unbin_kernel, dim3(a.X * a.Y), dim3(kBinThreads), unbin_lds, s, a);
+    if (staged) hipLaunchKernelGGL(unbin_kernel, dim3(a.X * a.Y * 3), dim3(kBinThreads), unbin_lds, s, a);
     return hipGetLastError();
 }
 
