@@ -1001,6 +1001,145 @@ __global__ __launch_bounds__(512) void near_gemm_kernel(Probes Q, NearTable T, c
     }
 }
 
+// The single-probe near GEMM with the radiance window staged in LDS: a
+// workgroup's four waves (four row tiles of one tile) share each K step's B
+// block (128 half8, twice that split), so the workgroup loads it once into
+// LDS, GS steps at a time, double buffered, with one barrier per GS steps;
+// each wave then issues one global load per step for its weights (two
+// split) instead of also loading the block itself. Weights are loaded a
+// group ahead. Each output's accumulation is the same sequence of MFMAs as
+// in near_gemm_kernel, so the outputs are identical. Measured on the 512^2
+// GGX chain (whole call): FP16 with near_gemm_kernel 0.54 ms; GS = 4: 0.555,
+// 8: 0.496, 12: 0.476, 16: 0.464 (2 waves per SIMD), 20: 0.475, 24 spills;
+// split BF16 with near_gemm_kernel 0.915 ms; GS = 4: 0.837, 6: 0.794, 8:
+// 0.762 (2 waves per SIMD).
+#ifndef HVX_NEAR_LDS_GROUP
+#define HVX_NEAR_LDS_GROUP 16
+#endif
+#ifndef HVX_NEAR_LDS_GROUP_SPLIT
+#define HVX_NEAR_LDS_GROUP_SPLIT 8
+#endif
+constexpr int kNearLdsGroup = HVX_NEAR_LDS_GROUP;             // 0: near_gemm_kernel
+constexpr int kNearLdsGroupSplit = HVX_NEAR_LDS_GROUP_SPLIT;  // 0: near_gemm_kernel
+template <bool SPLIT, int GS>
+__global__ __launch_bounds__(256) void near_gemm_lds_kernel(Probes Q, NearTable T, const float* sc) {
+    static_assert(kNearRows == 1 && GS % 2 == 0, "one row tile per wave, whole B passes per group");
+    const uint32_t lb = near_single_block(blockIdx.x);
+    const uint32_t gt = lb / kNearSingleParts;
+    if (gt >= T.tiles) return;  // padding of the XCD-aware order (uniform, before any barrier)
+    uint32_t jj = 0;
+    while (jj + 1 < T.n && gt >= T.j[jj + 1].tile_begin) ++jj;
+    const NearJob& J = T.j[jj];
+    const uint32_t tile = gt - J.tile_begin, t = threadIdx.x, lane = t & 63;
+    const uint32_t wave = (t >> 6) + (lb % kNearSingleParts) * 4;
+    const NearTile Tl = J.tiles[tile];
+    const uint32_t wc = Tl.w / 32, KS = Tl.h * wc;
+    const uint32_t i = lane & 15, g = lane >> 4, rt0 = wave;
+    constexpr size_t kStep = 16 * 64;
+    constexpr uint32_t kBw = SPLIT ? 256 : 128;  // half8 of a step's B block (hi, then lo)
+    constexpr int kWs = SPLIT ? 2 : 1;            // weight fragments per step (hi, lo)
+    using GH = const __attribute__((address_space(1))) half8*;
+    const GH wv = (GH)(J.W + Tl.woff + (size_t)rt0 * 64 + lane);
+    const GH pv = (GH)scr(Q, J.P, 0);
+    const uint32_t xb0 = (uint32_t)(Tl.x0 + (int)J.B) / 8;
+    const size_t p_lo = J.p_lo, w_lo = J.w_lo;
+    // B block of step s: 128 consecutive half8 at ((row * Pw8 + xb0 + col * 4) * 32)
+    // (its lo part p_lo further), row = s / wc, col = s % wc; steps past the
+    // last repeat it
+    auto b_base = [&](uint32_t s) {
+        s = s < KS ? s : KS - 1;
+        const uint32_t r = s / wc, c = s - r * wc;
+        return ((size_t)(Tl.y0 + (int)J.B + (int)r) * J.Pw8 + xb0 + c * 4) * 32;
+    };
+    constexpr int kPer = GS * kBw / 256;  // half8 of a group's B per thread
+    __shared__ half8 sB[2][GS][kBw];
+    auto b_load = [&](uint32_t s0, half8 (&v)[kPer]) {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const uint32_t e = t + 256u * k, o = e % kBw;
+            v[k] = pv[b_base(s0 + e / kBw) + o % 128 + (o >= 128 ? p_lo : 0)];
+        }
+    };
+    auto b_store = [&](uint32_t buf, const half8 (&v)[kPer]) {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const uint32_t e = t + 256u * k;
+            sB[buf][e / kBw][e % kBw] = v[k];
+        }
+    };
+    auto w_load = [&](uint32_t s0, half8 (&x)[GS][kWs]) {
+#pragma unroll
+        for (int k = 0; k < GS; ++k) {
+            const size_t o = (size_t)(s0 + k < KS ? s0 + k : KS - 1) * kStep;
+            x[k][0] = wv[o];
+            if constexpr (SPLIT) x[k][1] = wv[o + w_lo];
+        }
+    };
+    f32x4 acc[2] = {f32x4{0.0f, 0.0f, 0.0f, 0.0f}, f32x4{0.0f, 0.0f, 0.0f, 0.0f}};
+    half8 nb[kPer], xw[GS][kWs], xn[GS][kWs];
+    b_load(0, nb);
+    w_load(0, xw);
+    b_store(0, nb);
+    __syncthreads();
+    const uint32_t groups = (KS + GS - 1) / GS;
+    for (uint32_t gi = 0; gi < groups; ++gi) {
+        const uint32_t s0 = gi * GS;
+        // the next group's weights, then its B block, are in flight during
+        // this group's MFMAs (the last group reloads itself, unused)
+        const uint32_t sn = gi + 1 < groups ? s0 + GS : s0;
+        w_load(sn, xn);
+        b_load(sn, nb);
+        __asm__ volatile("" ::: "memory");
+        const uint32_t buf = gi & 1;
+#pragma unroll
+        for (int k = 0; k < GS; ++k) {
+            if (s0 + k < KS) {  // uniform over the workgroup
+                const half8* sb = sB[buf][k];
+                if constexpr (SPLIT) {
+                    const bf16x8 x = __builtin_bit_cast(bf16x8, xw[k][0]), y = __builtin_bit_cast(bf16x8, xw[k][1]);
+                    const bf16x8 b0 = __builtin_bit_cast(bf16x8, sb[g * 32 + i]);
+                    const bf16x8 b1 = __builtin_bit_cast(bf16x8, sb[g * 32 + 16 + i]);
+                    const bf16x8 c0 = __builtin_bit_cast(bf16x8, sb[128 + g * 32 + i]);
+                    const bf16x8 c1 = __builtin_bit_cast(bf16x8, sb[128 + g * 32 + 16 + i]);
+                    acc[0] = hvx::mfma16x16x32(x, b0, acc[0]);
+                    acc[0] = hvx::mfma16x16x32(x, c0, acc[0]);
+                    acc[0] = hvx::mfma16x16x32(y, b0, acc[0]);
+                    acc[1] = hvx::mfma16x16x32(x, b1, acc[1]);
+                    acc[1] = hvx::mfma16x16x32(x, c1, acc[1]);
+                    acc[1] = hvx::mfma16x16x32(y, b1, acc[1]);
+                } else {
+                    const half8 b0 = sb[g * 32 + i], b1 = sb[g * 32 + 16 + i];
+                    acc[0] = hvx::mfma16x16x32(xw[k][0], b0, acc[0]);
+                    acc[1] = hvx::mfma16x16x32(xw[k][0], b1, acc[1]);
+                }
+            }
+        }
+        // the other buffer's readers passed the previous barrier
+        b_store(buf ^ 1, nb);
+#pragma unroll
+        for (int k = 0; k < GS; ++k)
+#pragma unroll
+            for (int u = 0; u < kWs; ++u) xw[k][u] = xn[k][u];
+        __syncthreads();
+    }
+    const uint32_t So = J.So, X0 = (tile % J.te) * 16, Y0 = (tile / J.te) * 16;
+    const size_t So2 = (size_t)So * So;
+    float* out = reinterpret_cast<float*>(probe_out(Q, 0) + J.out_off);
+    const float* sq = scr(Q, sc, 0);
+    const uint32_t Y = Y0 + rt0;
+#pragma unroll
+    for (int n = 0; n < 2; ++n) {
+        const f32x4 a = acc[n];
+        const uint32_t col = n * 16 + i, f = col >> 2, c = col & 3;
+        if (col >= 24) continue;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t X = X0 + 4 * g + e, o = Y * So + X;
+            out[((size_t)f * So2 + o) * 4 + c] = c < 3 ? a[e] * J.rowscale[o] * sq[6 + f] : 1.0f;
+        }
+    }
+}
+
 template <bool SPLIT, int PG>
 __global__ __launch_bounds__(256) void near_gemm_staged_kernel(Probes Q, NearTable T, const float* sc) {
     const uint2 wb = gemm_block(Q);
@@ -1910,8 +2049,14 @@ int launch_call(hvx_contract_plan* P, const Probes& Q, uint32_t n, uint32_t chai
             hipLaunchKernelGGL((near_gemm_staged_kernel<true, kProbeGroup>), grid, block, 0, s, G, P->near, P->sc);
         else if (grouped)
             hipLaunchKernelGGL((near_gemm_staged_kernel<false, kProbeGroup>), grid, block, 0, s, G, P->near, P->sc);
+        else if (split && kNearLdsGroupSplit > 0)
+            hipLaunchKernelGGL((near_gemm_lds_kernel<true, (kNearLdsGroupSplit > 0 ? kNearLdsGroupSplit : 2)>), grid,
+                               block, 0, s, G, P->near, P->sc);
         else if (split)
             hipLaunchKernelGGL((near_gemm_kernel<true, 1>), grid, block, 0, s, G, P->near, P->sc);
+        else if (kNearLdsGroup > 0)
+            hipLaunchKernelGGL((near_gemm_lds_kernel<false, (kNearLdsGroup > 0 ? kNearLdsGroup : 2)>), grid, block, 0,
+                               s, G, P->near, P->sc);
         else
             hipLaunchKernelGGL((near_gemm_kernel<false, 1>), grid, block, 0, s, G, P->near, P->sc);
         hipLaunchKernelGGL(far_add_kernel, dim3((P->near.texels + 255) / 256, n), dim3(256), 0, s, Q, P->near);
