@@ -809,6 +809,135 @@ __global__ __launch_bounds__(256) void dense_gemm_staged_kernel(Probes Q, DenseT
     }
 }
 
+// The single-probe dense GEMM with the K slice's B block staged in LDS: the
+// four waves of a workgroup (four row-tile pairs of one K slice) share each
+// step's B block (128 half8, twice that split), loaded once per workgroup,
+// GS steps at a time (GS / 2 split), double buffered, one barrier per group;
+// weights a group ahead. Each output's accumulation is the same sequence of
+// MFMAs as in dense_gemm_kernel, so the partial sums are identical. Its B
+// block is a third of a step's loads, not two thirds as in the near field,
+// so the gain is small: whole FP16 call on the 512^2 GGX chain 0.469 ms with
+// dense_gemm_kernel, GS = 4: 0.466, 8: 0.461 (split BF16 0.767, 0.758, 0.759).
+#ifndef HVX_DENSE_LDS_GROUP
+#define HVX_DENSE_LDS_GROUP 8
+#endif
+constexpr int kDenseLdsGroup = HVX_DENSE_LDS_GROUP;  // 0: dense_gemm_kernel
+template <int GS>
+__global__ __launch_bounds__(256) void dense_gemm_lds_kernel(Probes Q, DenseTable T) {
+    static_assert(GS % 4 == 0, "whole B passes per group, split groups of GS / 2 steps");
+    const uint32_t gw = blockIdx.x * 4 + (threadIdx.x >> 6), t = threadIdx.x, lane = t & 63;
+    if (blockIdx.x * 4 >= T.waves) return;  // whole workgroups (T.waves is a multiple of 4)
+    uint32_t jj = 0;
+    while (jj + 1 < T.n && gw >= T.j[jj + 1].wave_begin) ++jj;  // wave_begin: multiples of 4
+    const DenseJob& J = T.j[jj];
+    const uint32_t rt = J.rt, ks = J.ks, splitk = J.splitk;
+    uint32_t pair, slice;
+    dense_wave(J, gw - J.wave_begin, pair, slice);  // slice: the same for the workgroup's waves
+    const uint32_t rt0 = 2 * pair;
+    const bool live = rt0 < rt;  // padding waves load and wait with the others
+    const bool has1 = rt0 + 1 < rt;
+    const uint32_t kb = (uint32_t)((uint64_t)slice * ks / splitk), ke = (uint32_t)((uint64_t)(slice + 1) * ks / splitk);
+    const uint32_t i = lane & 15, g = lane >> 4;
+    using GH = const __attribute__((address_space(1))) half8*;
+    const GH w0 = (GH)(J.W + (size_t)(live ? rt0 : 0) * ks * 64 + lane);
+    const GH w1 = has1 ? w0 + (size_t)ks * 64 : w0;
+    const GH rv = (GH)scr(Q, J.R, 0);
+    __shared__ half8 sB[2][GS * 128];  // a group's B blocks (GS steps, or GS / 2 split steps of 256)
+    f32x4 acc[4] = {f32x4{0.0f, 0.0f, 0.0f, 0.0f}, f32x4{0.0f, 0.0f, 0.0f, 0.0f}, f32x4{0.0f, 0.0f, 0.0f, 0.0f},
+                    f32x4{0.0f, 0.0f, 0.0f, 0.0f}};
+    auto run = [&](auto split_tag) {
+        constexpr bool SPLIT = decltype(split_tag)::value;
+        constexpr uint32_t G = SPLIT ? GS / 2 : GS;     // steps per group
+        constexpr uint32_t kBw = SPLIT ? 256 : 128;     // half8 per step's B block
+        constexpr int kWs = SPLIT ? 4 : 2;               // weight fragments per step
+        constexpr int kPer = G * kBw / 256;
+        const size_t r_lo = J.r_lo, w_lo = J.w_lo;
+        const uint32_t n = ke - kb, groups = (n + G - 1) / G;
+        if (groups == 0) return;  // uniform over the workgroup
+        auto step_of = [&](uint32_t s) { return kb + (s < n ? s : n - 1); };  // steps past the last repeat it
+        auto b_load = [&](uint32_t s0, half8 (&v)[kPer]) {
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const uint32_t e = t + 256u * k, o = e % kBw;
+                v[k] = rv[(size_t)step_of(s0 + e / kBw) * 128 + o % 128 + (o >= 128 ? r_lo : 0)];
+            }
+        };
+        auto b_store = [&](uint32_t buf, const half8 (&v)[kPer]) {
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) sB[buf][t + 256u * k] = v[k];
+        };
+        auto w_load = [&](uint32_t s0, half8 (&x)[G][kWs]) {
+#pragma unroll
+            for (uint32_t k = 0; k < G; ++k) {
+                const size_t o = (size_t)step_of(s0 + k) * 64;
+                x[k][0] = w0[o];
+                x[k][1] = w1[o];
+                if constexpr (SPLIT) {
+                    x[k][2] = w0[o + w_lo];
+                    x[k][3] = w1[o + w_lo];
+                }
+            }
+        };
+        half8 nb[kPer], xw[G][kWs], xn[G][kWs];
+        b_load(0, nb);
+        w_load(0, xw);
+        b_store(0, nb);
+        __syncthreads();
+        for (uint32_t gi = 0; gi < groups; ++gi) {
+            const uint32_t s0 = gi * G, sn = gi + 1 < groups ? s0 + G : s0;
+            w_load(sn, xn);
+            b_load(sn, nb);
+            __asm__ volatile("" ::: "memory");
+            const half8* sb = sB[gi & 1];
+#pragma unroll
+            for (uint32_t k = 0; k < G; ++k) {
+                if (live && s0 + k < n) {
+                    const half8* bk = sb + k * kBw + g * 32 + i;
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+#pragma unroll
+                        for (int nbk = 0; nbk < 2; ++nbk) {
+                            f32x4& a = acc[2 * h + nbk];
+                            if constexpr (SPLIT) {
+                                const bf16x8 x = __builtin_bit_cast(bf16x8, xw[k][h]);
+                                const bf16x8 y = __builtin_bit_cast(bf16x8, xw[k][2 + h]);
+                                const bf16x8 b = __builtin_bit_cast(bf16x8, bk[16 * nbk]);
+                                const bf16x8 c = __builtin_bit_cast(bf16x8, bk[128 + 16 * nbk]);
+                                a = hvx::mfma16x16x32(x, b, a);
+                                a = hvx::mfma16x16x32(x, c, a);
+                                a = hvx::mfma16x16x32(y, b, a);
+                            } else {
+                                a = hvx::mfma16x16x32(xw[k][h], bk[16 * nbk], a);
+                            }
+                        }
+                }
+            }
+            b_store((gi & 1) ^ 1, nb);
+#pragma unroll
+            for (uint32_t k = 0; k < G; ++k)
+#pragma unroll
+                for (int u = 0; u < kWs; ++u) xw[k][u] = xn[k][u];
+            __syncthreads();
+        }
+    };
+    if (J.w_lo)  // split BF16 (uniform over the workgroup: one job per workgroup)
+        run(std::true_type{});
+    else
+        run(std::false_type{});
+    if (!live) return;
+    float* p = scr(Q, J.part, 0) + (size_t)slice * rt * 16 * 32;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t row0 = rt0 * 16 + 4 * g + r;
+        p[(size_t)row0 * 32 + i] = acc[0][r];
+        p[(size_t)row0 * 32 + 16 + i] = acc[1][r];
+        if (has1) {
+            p[(size_t)(row0 + 16) * 32 + i] = acc[2][r];
+            p[(size_t)(row0 + 16) * 32 + 16 + i] = acc[3][r];
+        }
+    }
+}
+
 // out[f][row] = (sum over slices) * rowscale[row] * 2^e  (alpha 1 in the
 // chain, 0 in a far map)
 __global__ __launch_bounds__(256) void dense_epilogue_kernel(Probes Q, DenseTable T, const float* sc) {
@@ -2036,6 +2165,9 @@ int launch_call(hvx_contract_plan* P, const Probes& Q, uint32_t n, uint32_t chai
         const dim3 grid = G.probe_fast ? dim3(groups, dense_wgs) : dim3(dense_wgs, groups);
         if (grouped)
             hipLaunchKernelGGL(dense_gemm_staged_kernel<kProbeGroup>, grid, dim3(256), 0, s, G, P->dense);
+        else if (kDenseLdsGroup > 0)
+            hipLaunchKernelGGL((dense_gemm_lds_kernel<(kDenseLdsGroup > 0 ? kDenseLdsGroup : 4)>), grid, dim3(256), 0, s,
+                               G, P->dense);
         else
             hipLaunchKernelGGL(dense_gemm_kernel<1>, grid, dim3(256), 0, s, G, P->dense);
         hipLaunchKernelGGL(dense_epilogue_kernel, dim3((P->dense.epi + 255) / 256, n), dim3(256), 0, s, Q, P->dense,
