@@ -9,7 +9,7 @@
 //   T = float   the parity oracle (built -O2 -ffp-contract=off, no fast-math;
 //               the GLSL transcendentals use hvx_math.h so the GPU kernels can
 //               reproduce them to the bit)
-//   T = double  the precision judge (same algorithm, libm transcendentals)
+//   T = double  the precision reference (same algorithm, libm transcendentals)
 // Vector expressions are evaluated in one documented order, left to right as
 // GLSL writes them: dot(a,b) = (ax*bx + ay*by) + az*bz; mat*vec row i =
 // ((m0i*x + m1i*y) + m2i*z) + m3i*w; normalize(v) = v * (1/sqrt(dot(v,v))).
