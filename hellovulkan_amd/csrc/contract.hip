@@ -809,26 +809,38 @@ __global__ __launch_bounds__(256) void dense_gemm_staged_kernel(Probes Q, DenseT
     }
 }
 
-// The single-probe dense GEMM with the K slice's B block staged in LDS: the
-// four waves of a workgroup (four row-tile pairs of one K slice) share each
-// step's B block (128 half8, twice that split), loaded once per workgroup,
+// The dense GEMM with the K slice's B blocks staged in LDS: the four waves
+// of a workgroup (four row-tile pairs of one K slice) share each step's B
+// block (128 half8 per probe, twice that split), loaded once per workgroup,
 // GS steps at a time (GS / 2 split), double buffered, one barrier per group;
-// weights a group ahead. Each output's accumulation is the same sequence of
-// MFMAs as in dense_gemm_kernel, so the partial sums are identical. Its B
-// block is a third of a step's loads, not two thirds as in the near field,
-// so the gain is small: whole FP16 call on the 512^2 GGX chain 0.469 ms with
-// dense_gemm_kernel, GS = 4: 0.466, 8: 0.461 (split BF16 0.767, 0.758, 0.759).
+// weights a group ahead. PG probes per wave (grouped multi-probe calls):
+// each weight fragment is loaded once for the group. Each output's
+// accumulation is the same sequence of MFMAs as in dense_gemm_kernel, so the
+// partial sums are identical. Single probe: its B block is a third of a
+// step's loads, not two thirds as in the near field, so the gain is small:
+// whole FP16 call on the 512^2 GGX chain 0.469 ms with dense_gemm_kernel,
+// GS = 4: 0.466, 8: 0.461 (split BF16 0.767, 0.758, 0.759). Four probes per
+// wave (54 probes 512^2 -> 256^2 x 9, FP16 / split): dense_gemm_staged_kernel
+// (a barrier per step) 4.36 / 6.16 ms; GS = 2: 3.96 / 6.00; 4: 4.56 / 6.50
+// (one wave per SIMD).
 #ifndef HVX_DENSE_LDS_GROUP
 #define HVX_DENSE_LDS_GROUP 8
 #endif
-constexpr int kDenseLdsGroup = HVX_DENSE_LDS_GROUP;  // 0: dense_gemm_kernel
-template <int GS>
+#ifndef HVX_DENSE_LDS_GROUP_MULTI
+#define HVX_DENSE_LDS_GROUP_MULTI 2
+#endif
+constexpr int kDenseLdsGroup = HVX_DENSE_LDS_GROUP;             // 0: dense_gemm_kernel
+constexpr int kDenseLdsGroupMulti = HVX_DENSE_LDS_GROUP_MULTI;  // 0: dense_gemm_staged_kernel
+template <int PG, int GS>
 __global__ __launch_bounds__(256) void dense_gemm_lds_kernel(Probes Q, DenseTable T) {
-    static_assert(GS % 4 == 0, "whole B passes per group, split groups of GS / 2 steps");
-    const uint32_t gw = blockIdx.x * 4 + (threadIdx.x >> 6), t = threadIdx.x, lane = t & 63;
-    if (blockIdx.x * 4 >= T.waves) return;  // whole workgroups (T.waves is a multiple of 4)
+    static_assert(GS % 4 == 0 || (GS % 2 == 0 && PG % 2 == 0), "whole B passes per group, split groups of GS / 2");
+    const uint2 wb = gemm_block(Q);
+    const uint32_t gw0 = wb.x * 4;
+    if (gw0 >= T.waves) return;  // whole workgroups (T.waves is a multiple of 4)
+    const uint32_t t = threadIdx.x, gw = gw0 + (t >> 6), lane = t & 63, p0 = wb.y * PG;
+    const uint32_t np = min((uint32_t)PG, Q.n - p0);
     uint32_t jj = 0;
-    while (jj + 1 < T.n && gw >= T.j[jj + 1].wave_begin) ++jj;  // wave_begin: multiples of 4
+    while (jj + 1 < T.n && gw0 >= T.j[jj + 1].wave_begin) ++jj;  // wave_begin: multiples of 4
     const DenseJob& J = T.j[jj];
     const uint32_t rt = J.rt, ks = J.ks, splitk = J.splitk;
     uint32_t pair, slice;
@@ -841,16 +853,19 @@ __global__ __launch_bounds__(256) void dense_gemm_lds_kernel(Probes Q, DenseTabl
     using GH = const __attribute__((address_space(1))) half8*;
     const GH w0 = (GH)(J.W + (size_t)(live ? rt0 : 0) * ks * 64 + lane);
     const GH w1 = has1 ? w0 + (size_t)ks * 64 : w0;
-    const GH rv = (GH)scr(Q, J.R, 0);
-    __shared__ half8 sB[2][GS * 128];  // a group's B blocks (GS steps, or GS / 2 split steps of 256)
-    f32x4 acc[4] = {f32x4{0.0f, 0.0f, 0.0f, 0.0f}, f32x4{0.0f, 0.0f, 0.0f, 0.0f}, f32x4{0.0f, 0.0f, 0.0f, 0.0f},
-                    f32x4{0.0f, 0.0f, 0.0f, 0.0f}};
+    __shared__ half8 sB[2][GS * PG * 128];  // a group's B blocks: [step][probe][unit (hi, then lo split)]
+    f32x4 acc[PG][4];
+#pragma unroll
+    for (int q = 0; q < PG; ++q)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc[q][k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     auto run = [&](auto split_tag) {
         constexpr bool SPLIT = decltype(split_tag)::value;
-        constexpr uint32_t G = SPLIT ? GS / 2 : GS;     // steps per group
-        constexpr uint32_t kBw = SPLIT ? 256 : 128;     // half8 per step's B block
-        constexpr int kWs = SPLIT ? 4 : 2;               // weight fragments per step
-        constexpr int kPer = G * kBw / 256;
+        constexpr uint32_t G = SPLIT ? GS / 2 : GS;  // steps per group
+        constexpr uint32_t kBw = SPLIT ? 256 : 128;  // half8 per probe and step
+        constexpr int kWs = SPLIT ? 4 : 2;            // weight fragments per step
+        constexpr int kPer = G * PG * kBw / 256;
+        static_assert(G * PG * kBw % 256 == 0, "whole 256-lane passes");
         const size_t r_lo = J.r_lo, w_lo = J.w_lo;
         const uint32_t n = ke - kb, groups = (n + G - 1) / G;
         if (groups == 0) return;  // uniform over the workgroup
@@ -858,8 +873,10 @@ __global__ __launch_bounds__(256) void dense_gemm_lds_kernel(Probes Q, DenseTabl
         auto b_load = [&](uint32_t s0, half8 (&v)[kPer]) {
 #pragma unroll
             for (int k = 0; k < kPer; ++k) {
-                const uint32_t e = t + 256u * k, o = e % kBw;
-                v[k] = rv[(size_t)step_of(s0 + e / kBw) * 128 + o % 128 + (o >= 128 ? r_lo : 0)];
+                const uint32_t e = t + 256u * k, st = e / (PG * kBw), rem = e % (PG * kBw), q = rem / kBw,
+                               o = rem % kBw;
+                const GH rq = (GH)scr(Q, J.R, p0 + (q < np ? q : 0));
+                v[k] = rq[(size_t)step_of(s0 + st) * 128 + o % 128 + (o >= 128 ? r_lo : 0)];
             }
         };
         auto b_store = [&](uint32_t buf, const half8 (&v)[kPer]) {
@@ -892,24 +909,27 @@ __global__ __launch_bounds__(256) void dense_gemm_lds_kernel(Probes Q, DenseTabl
 #pragma unroll
             for (uint32_t k = 0; k < G; ++k) {
                 if (live && s0 + k < n) {
-                    const half8* bk = sb + k * kBw + g * 32 + i;
 #pragma unroll
-                    for (int h = 0; h < 2; ++h)
+                    for (int q = 0; q < PG; ++q) {
+                        const half8* bk = sb + (k * PG + q) * kBw + g * 32 + i;
 #pragma unroll
-                        for (int nbk = 0; nbk < 2; ++nbk) {
-                            f32x4& a = acc[2 * h + nbk];
-                            if constexpr (SPLIT) {
-                                const bf16x8 x = __builtin_bit_cast(bf16x8, xw[k][h]);
-                                const bf16x8 y = __builtin_bit_cast(bf16x8, xw[k][2 + h]);
-                                const bf16x8 b = __builtin_bit_cast(bf16x8, bk[16 * nbk]);
-                                const bf16x8 c = __builtin_bit_cast(bf16x8, bk[128 + 16 * nbk]);
-                                a = hvx::mfma16x16x32(x, b, a);
-                                a = hvx::mfma16x16x32(x, c, a);
-                                a = hvx::mfma16x16x32(y, b, a);
-                            } else {
-                                a = hvx::mfma16x16x32(xw[k][h], bk[16 * nbk], a);
+                        for (int h = 0; h < 2; ++h)
+#pragma unroll
+                            for (int nbk = 0; nbk < 2; ++nbk) {
+                                f32x4& a = acc[q][2 * h + nbk];
+                                if constexpr (SPLIT) {
+                                    const bf16x8 x = __builtin_bit_cast(bf16x8, xw[k][h]);
+                                    const bf16x8 y = __builtin_bit_cast(bf16x8, xw[k][2 + h]);
+                                    const bf16x8 b = __builtin_bit_cast(bf16x8, bk[16 * nbk]);
+                                    const bf16x8 c = __builtin_bit_cast(bf16x8, bk[128 + 16 * nbk]);
+                                    a = hvx::mfma16x16x32(x, b, a);
+                                    a = hvx::mfma16x16x32(x, c, a);
+                                    a = hvx::mfma16x16x32(y, b, a);
+                                } else {
+                                    a = hvx::mfma16x16x32(xw[k][h], bk[16 * nbk], a);
+                                }
                             }
-                        }
+                    }
                 }
             }
             b_store((gi & 1) ^ 1, nb);
@@ -925,15 +945,19 @@ __global__ __launch_bounds__(256) void dense_gemm_lds_kernel(Probes Q, DenseTabl
     else
         run(std::false_type{});
     if (!live) return;
-    float* p = scr(Q, J.part, 0) + (size_t)slice * rt * 16 * 32;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const uint32_t row0 = rt0 * 16 + 4 * g + r;
-        p[(size_t)row0 * 32 + i] = acc[0][r];
-        p[(size_t)row0 * 32 + 16 + i] = acc[1][r];
-        if (has1) {
-            p[(size_t)(row0 + 16) * 32 + i] = acc[2][r];
-            p[(size_t)(row0 + 16) * 32 + 16 + i] = acc[3][r];
+    for (int q = 0; q < PG; ++q) {
+        if ((uint32_t)q >= np) break;
+        float* p = scr(Q, J.part, p0 + q) + (size_t)slice * rt * 16 * 32;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const uint32_t row0 = rt0 * 16 + 4 * g + r;
+            p[(size_t)row0 * 32 + i] = acc[q][0][r];
+            p[(size_t)row0 * 32 + 16 + i] = acc[q][1][r];
+            if (has1) {
+                p[(size_t)(row0 + 16) * 32 + i] = acc[q][2][r];
+                p[(size_t)(row0 + 16) * 32 + 16 + i] = acc[q][3][r];
+            }
         }
     }
 }
@@ -2163,11 +2187,14 @@ int launch_call(hvx_contract_plan* P, const Probes& Q, uint32_t n, uint32_t chai
     G.probe_fast = grouped && std::max(dense_wgs, near_wgs) <= 65535u;
     if (P->dense.n) {
         const dim3 grid = G.probe_fast ? dim3(groups, dense_wgs) : dim3(dense_wgs, groups);
-        if (grouped)
+        if (grouped && kDenseLdsGroupMulti > 0)
+            hipLaunchKernelGGL((dense_gemm_lds_kernel<kProbeGroup, (kDenseLdsGroupMulti > 0 ? kDenseLdsGroupMulti : 4)>),
+                               grid, dim3(256), 0, s, G, P->dense);
+        else if (grouped)
             hipLaunchKernelGGL(dense_gemm_staged_kernel<kProbeGroup>, grid, dim3(256), 0, s, G, P->dense);
         else if (kDenseLdsGroup > 0)
-            hipLaunchKernelGGL((dense_gemm_lds_kernel<(kDenseLdsGroup > 0 ? kDenseLdsGroup : 4)>), grid, dim3(256), 0, s,
-                               G, P->dense);
+            hipLaunchKernelGGL((dense_gemm_lds_kernel<1, (kDenseLdsGroup > 0 ? kDenseLdsGroup : 4)>), grid, dim3(256), 0,
+                               s, G, P->dense);
         else
             hipLaunchKernelGGL(dense_gemm_kernel<1>, grid, dim3(256), 0, s, G, P->dense);
         hipLaunchKernelGGL(dense_epilogue_kernel, dim3((P->dense.epi + 255) / 256, n), dim3(256), 0, s, Q, P->dense,
