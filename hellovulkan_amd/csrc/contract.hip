@@ -1323,9 +1323,10 @@ __global__ __launch_bounds__(256) void near_gemm_staged_kernel(Probes Q, NearTab
     // every wave of the workgroup: it is staged once per step in LDS, double
     // buffered, the next step's copy loaded while this step's MFMAs run. With
     // PG probes per wave this cuts the workgroup's operand reads PG-fold per
-    // step, which pays for the barrier (256-probe batch, contraction mode: 8.8 -> 8.4 ms);
-    // with one probe it does not (512^2 x 10 GGX chain: 0.366 -> 0.392 ms), so
-    // single-probe calls use near_gemm_kernel.
+    // step, which pays for the barrier (256-probe batch, contraction mode: 8.8 -> 8.4 ms).
+    // Single-probe calls stage several steps per barrier instead
+    // (near_gemm_lds_kernel): with a barrier per step one probe lost (512^2 x 10
+    // GGX chain: 0.366 -> 0.392 ms).
     constexpr int kB = PG * 128 * (SPLIT ? 2 : 1);  // half8 per step
     constexpr int kPer = (kB + 255) / 256;           // per thread (256 lanes)
     static_assert(kB % 256 == 0 || kB < 256, "the B block splits evenly over the workgroup");
@@ -2177,9 +2178,10 @@ int launch_call(hvx_contract_plan* P, const Probes& Q, uint32_t n, uint32_t chai
     // each weight fragment, and the group index is the fastest grid dimension,
     // so groups reading one weight block run together and share it in the caches
     Probes G = Q;
-    // single-probe calls use the unstaged GEMMs: staging the B operand in LDS
-    // pays only when several probes share it (measured on the 512^2 x 10 GGX
-    // chain: FP16 near GEMM 203 us unstaged, 246 us staged)
+    // single-probe calls stage the B operand in LDS several K steps per
+    // barrier (near_gemm_lds_kernel, dense_gemm_lds_kernel<1, ...>): with one
+    // barrier per step it paid only when several probes share it (512^2 x 10
+    // GGX chain: FP16 near GEMM 203 us unstaged, 246 us staged per step)
     const bool grouped = n > 1;
     const uint32_t dense_wgs = (P->dense.waves + 3) / 4,
                    near_wgs = grouped ? P->near.tiles * kNearParts : near_single_wgs(P->near.tiles);
