@@ -141,8 +141,7 @@ struct GemmArgs {
 
 // A wave computes a 16 x 64 block of C over one K slice.
 template <int EPI>
-__global__ __launch_bounds__(256) void gemm_abt_kernel(GemmArgs g) {
-    const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+__device__ __forceinline__ void gemm_abt_wave(const GemmArgs& g, uint32_t wave, uint32_t lane) {
     const uint32_t rts = g.M / 16, cgs = g.Nc / 64;
     if (wave >= rts * cgs * g.splitk) return;  // uniform per wave; no barriers below
     const uint32_t slice = wave % g.splitk, t = wave / g.splitk, cg = t % cgs, rt = t / cgs;
@@ -202,6 +201,24 @@ __global__ __launch_bounds__(256) void gemm_abt_kernel(GemmArgs g) {
             *reinterpret_cast<bf16x4*>(g.CT + (size_t)col * g.M + rt * 16 + 4 * gq) = bf16x4{ht[0], ht[1], ht[2], ht[3]};
         }
     }
+}
+
+template <int EPI>
+__global__ __launch_bounds__(256) void gemm_abt_kernel(GemmArgs g) {
+    gemm_abt_wave<EPI>(g, blockIdx.x * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
+}
+
+// The three weight-gradient GEMMs of a step in one launch: waves [0, w1)
+// compute g[0], [w1, w2) g[1], the rest g[2], each as gemm_abt_kernel does.
+struct GemmArgs3 {
+    GemmArgs g[3];
+    uint32_t w1, w2;
+};
+__global__ __launch_bounds__(256) void gemm_part3_kernel(GemmArgs3 a) {
+    const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (wave < a.w1) gemm_abt_wave<EPI_PART>(a.g[0], wave, lane);
+    else if (wave < a.w2) gemm_abt_wave<EPI_PART>(a.g[1], wave - a.w1, lane);
+    else gemm_abt_wave<EPI_PART>(a.g[2], wave - a.w2, lane);
 }
 
 // ------------------------------------------------------- output and loss
@@ -483,12 +500,15 @@ __global__ __launch_bounds__(64 * kStepWaves) void mlp_step_kernel(StepArgs g) {
 // takes 64 outputs and 16 slice ranges (one per wave), each summed in slice
 // order, then adds the 16 range sums in a fixed tree
 constexpr uint32_t kRedGroups = 16;
-__global__ __launch_bounds__(64 * kRedGroups) void reduce_slices_kernel(const float* __restrict__ part,
-                                                                         uint32_t slices, uint32_t stride,
-                                                                         uint32_t len, float scale,
-                                                                         float* __restrict__ out) {
+// out[i] = scale * (sum over slices of part[s * stride + i]) for i < len,
+// except that with `last` set element len - 1 goes to *last (times
+// last_scale) instead
+__device__ __forceinline__ void reduce_slices_block(const float* __restrict__ part, uint32_t slices, uint32_t stride,
+                                                    uint32_t len, float scale, float* __restrict__ out,
+                                                    uint32_t block, float* __restrict__ last = nullptr,
+                                                    float last_scale = 1.0f) {
     __shared__ float sh[kRedGroups][64];
-    const uint32_t l = threadIdx.x & 63, gq = threadIdx.x >> 6, i = blockIdx.x * 64 + l;
+    const uint32_t l = threadIdx.x & 63, gq = threadIdx.x >> 6, i = block * 64 + l;
     float acc = 0.0f;
     if (i < len) {
         const uint32_t s0 = gq * slices / kRedGroups, s1 = (gq + 1) * slices / kRedGroups;
@@ -500,7 +520,37 @@ __global__ __launch_bounds__(64 * kRedGroups) void reduce_slices_kernel(const fl
         __syncthreads();
         if (gq < h) sh[gq][l] += sh[gq + h][l];
     }
-    if (gq == 0 && i < len) out[i] = sh[0][l] * scale;
+    if (gq != 0 || i >= len) return;
+    if (last && i == len - 1) *last = sh[0][l] * last_scale;
+    else out[i] = sh[0][l] * scale;
+}
+__global__ __launch_bounds__(64 * kRedGroups) void reduce_slices_kernel(const float* __restrict__ part,
+                                                                         uint32_t slices, uint32_t stride,
+                                                                         uint32_t len, float scale,
+                                                                         float* __restrict__ out) {
+    reduce_slices_block(part, slices, stride, len, scale, out, blockIdx.x);
+}
+// A step's reductions in one launch: segment k (blocks [b[k], b[k + 1]))
+// reduces len[k] outputs over slices[k] partials of stride[k] into out[k]
+// (scale 1): the three layers' weight gradients and the output layer's block
+// partials, whose last element (the loss, scaled by loss_scale) goes to
+// *loss when it is set (otherwise it is not written).
+struct ReduceStep {
+    const float* part[4];
+    float* out[4];
+    uint32_t len[4], stride[4], slices[4], b[5];
+    float* loss;
+    float loss_scale;
+};
+__global__ __launch_bounds__(64 * kRedGroups) void reduce_step_kernel(ReduceStep r) {
+    const uint32_t blk = blockIdx.x;
+    uint32_t k = 0;
+    while (k < 3 && blk >= r.b[k + 1]) ++k;
+    const bool out_layer = k == 3;
+    // without a loss pointer the output layer's last element is not written
+    const uint32_t len = out_layer && !r.loss ? r.len[k] - 1 : r.len[k];
+    reduce_slices_block(r.part[k], r.slices[k], r.stride[k], len, 1.0f, r.out[k], blk - r.b[k],
+                        out_layer ? r.loss : nullptr, r.loss_scale);
 }
 
 
@@ -716,7 +766,7 @@ hipError_t ensure_batch(hvx_mlp* M, uint32_t B) {
     if ((e = dalloc(L, &M->dZa, (size_t)B * kH)) || (e = dalloc(L, &M->dZaT, (size_t)B * kH)) ||
         (e = dalloc(L, &M->dZb, (size_t)B * kH)) || (e = dalloc(L, &M->dZbT, (size_t)B * kH)) ||
         (e = dalloc(L, &M->dZc, (size_t)B * kH)) || (e = dalloc(L, &M->dZcT, (size_t)B * kH)) ||
-        (e = dalloc(L, &M->part, (size_t)M->splitk * (kH * kIn + kH))) ||
+        (e = dalloc(L, &M->part, (size_t)M->splitk * ((kH * kIn + kH) + 2 * (kH * kH + kH)))) ||
         (e = dalloc(L, &M->opart, (size_t)(B / kOutBlock) * kOutPart)))
         return e;
     M->cap = B;
@@ -841,12 +891,17 @@ void gradient_fused(hvx_mlp* M, const float4* in, const float4* tgt, uint32_t B,
     a.dZT[2] = M->dZaT;
     a.part = M->opart;
     hipLaunchKernelGGL(mlp_step_kernel, dim3(nblk), dim3(64 * kStepWaves), 0, s, a);
-    hipLaunchKernelGGL(reduce_out_kernel, dim3((kOutPart + 63) / 64), dim3(64 * kOutRedGroups), 0, s, M->opart, nblk, G + hvx::kMlpW4,
-                       1.0f / (3.0f * (float)B), loss);
+    // the three weight-gradient GEMMs in one launch, each into its own
+    // partials, then their reductions in one launch (the same per-wave and
+    // per-output work as separate launches)
     const int wofs[3] = {hvx::kMlpW1, hvx::kMlpW2, hvx::kMlpW3};
-    for (int l = 2; l >= 0; --l) {
-        const uint32_t K = l == 0 ? kIn : kH;
-        GemmArgs g{};
+    GemmArgs3 g3{};
+    ReduceStep r3{};
+    float* part = M->part;
+    uint32_t waves = 0;
+    for (int l = 0; l < 3; ++l) {
+        const uint32_t K = l == 0 ? kIn : kH, len = kH * K + kH;
+        GemmArgs& g = g3.g[l];
         g.A = a.dZT[l];
         g.Bt = l == 0 ? (x0t ? x0t : M->X0T) : M->HT[l - 1];
         g.ldb = l == 0 && x0t ? ldx : 0;
@@ -854,13 +909,30 @@ void gradient_fused(hvx_mlp* M, const float4* in, const float4* tgt, uint32_t B,
         g.Nc = K;
         g.Kd = B;
         g.splitk = M->splitk;
-        g.part = M->part;
-        g.pstride = kH * K + kH;
+        g.part = part;
+        g.pstride = len;
         g.colsum = 1;
-        gemm<EPI_PART>(g, s);
-        hipLaunchKernelGGL(reduce_slices_kernel, dim3((kH * K + kH + 63) / 64), dim3(64 * kRedGroups), 0, s,
-                           M->part, M->splitk, (uint32_t)(kH * K + kH), (uint32_t)(kH * K + kH), 1.0f, G + wofs[l]);
+        r3.part[l] = part;
+        r3.out[l] = G + wofs[l];
+        r3.len[l] = r3.stride[l] = len;
+        r3.slices[l] = M->splitk;
+        r3.b[l + 1] = r3.b[l] + (len + 63) / 64;
+        part += (size_t)M->splitk * len;
+        const uint32_t w = (g.M / 16) * (g.Nc / 64) * g.splitk;
+        if (l == 0) g3.w1 = w;
+        else if (l == 1) g3.w2 = g3.w1 + w;
+        waves += w;
     }
+    // the output layer: dW4, db4 and the loss from the step kernel's block partials
+    r3.part[3] = M->opart;
+    r3.out[3] = G + hvx::kMlpW4;
+    r3.len[3] = r3.stride[3] = kOutPart;
+    r3.slices[3] = nblk;
+    r3.b[4] = r3.b[3] + (kOutPart + 63) / 64;
+    r3.loss = loss;
+    r3.loss_scale = 1.0f / (3.0f * (float)B);
+    hipLaunchKernelGGL(gemm_part3_kernel, dim3((waves + 3) / 4), dim3(256), 0, s, g3);
+    hipLaunchKernelGGL(reduce_step_kernel, dim3(r3.b[4]), dim3(64 * kRedGroups), 0, s, r3);
 }
 
 void cast(hvx_mlp* M, hipStream_t s) {
