@@ -829,6 +829,12 @@ __global__ __launch_bounds__(256) void dense_gemm_staged_kernel(Probes Q, DenseT
 #ifndef HVX_DENSE_LDS_GROUP_MULTI
 #define HVX_DENSE_LDS_GROUP_MULTI 2
 #endif
+// non-temporal weight loads in the dense GEMM: no gain for one probe (0.451
+// -> 0.449 ms), and a loss when probe groups share each weight block through
+// the caches (54 probes 512^2: 3.98 -> 4.27 ms)
+#ifndef HVX_DENSE_NT
+#define HVX_DENSE_NT 0
+#endif
 constexpr int kDenseLdsGroup = HVX_DENSE_LDS_GROUP;             // 0: dense_gemm_kernel
 constexpr int kDenseLdsGroupMulti = HVX_DENSE_LDS_GROUP_MULTI;  // 0: dense_gemm_staged_kernel
 template <int PG, int GS>
@@ -887,12 +893,21 @@ __global__ __launch_bounds__(256) void dense_gemm_lds_kernel(Probes Q, DenseTabl
 #pragma unroll
             for (uint32_t k = 0; k < G; ++k) {
                 const size_t o = (size_t)step_of(s0 + k) * 64;
+#if HVX_DENSE_NT
+                x[k][0] = __builtin_nontemporal_load(&w0[o]);
+                x[k][1] = __builtin_nontemporal_load(&w1[o]);
+                if constexpr (SPLIT) {
+                    x[k][2] = __builtin_nontemporal_load(&w0[o + w_lo]);
+                    x[k][3] = __builtin_nontemporal_load(&w1[o + w_lo]);
+                }
+#else
                 x[k][0] = w0[o];
                 x[k][1] = w1[o];
                 if constexpr (SPLIT) {
                     x[k][2] = w0[o + w_lo];
                     x[k][3] = w1[o + w_lo];
                 }
+#endif
             }
         };
         half8 nb[kPer], xw[G][kWs], xn[G][kWs];
@@ -1169,6 +1184,11 @@ __global__ __launch_bounds__(512) void near_gemm_kernel(Probes Q, NearTable T, c
 #ifndef HVX_NEAR_LDS_GROUP
 #define HVX_NEAR_LDS_GROUP 16
 #endif
+// weights are read once per call: non-temporal loads (whole FP16 / split
+// call on the 512^2 GGX chain: 0.460 / 0.754 ms -> 0.449 / 0.723)
+#ifndef HVX_NEAR_NT
+#define HVX_NEAR_NT 1
+#endif
 #ifndef HVX_NEAR_LDS_GROUP_SPLIT
 #define HVX_NEAR_LDS_GROUP_SPLIT 8
 #endif
@@ -1224,8 +1244,13 @@ __global__ __launch_bounds__(256) void near_gemm_lds_kernel(Probes Q, NearTable 
 #pragma unroll
         for (int k = 0; k < GS; ++k) {
             const size_t o = (size_t)(s0 + k < KS ? s0 + k : KS - 1) * kStep;
+#if HVX_NEAR_NT
+            x[k][0] = __builtin_nontemporal_load(&wv[o]);
+            if constexpr (SPLIT) x[k][1] = __builtin_nontemporal_load(&wv[o + w_lo]);
+#else
             x[k][0] = wv[o];
             if constexpr (SPLIT) x[k][1] = wv[o + w_lo];
+#endif
         }
     };
     f32x4 acc[2] = {f32x4{0.0f, 0.0f, 0.0f, 0.0f}, f32x4{0.0f, 0.0f, 0.0f, 0.0f}};
