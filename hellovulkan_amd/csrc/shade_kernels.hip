@@ -499,8 +499,8 @@ __device__ __forceinline__ void shade_tail_write(const ShadeArgs& a, uint32_t id
 }
 
 // Fragments per workgroup (3840x2160, 65,536 lights): plain shading 64 (2.07 ms; 128:
-// 2.09, 256: 2.11, 1024: 2.34); learned shading 256 (3.35 ms; 128: 4.69, 64:
-// 8.12), since each of its workgroups first stages the network's weights.
+// 2.09, 256: 2.11, 1024: 2.34); learned shading 256 (2.30 ms; 128: 3.65, 512:
+// 2.65), since each of its workgroups first stages the network's weights.
 #ifndef HVX_SHADE_CHUNK
 #define HVX_SHADE_CHUNK 64
 #endif
