@@ -262,8 +262,8 @@ int hvx_ctx_destroy(hvx_ctx* ctx) {
     if (!ctx) return HVX_ERR_NULL;
     if (bind_device(ctx) == HVX_OK) {
         (void)hipDeviceSynchronize();
-        void* bufs[] = {ctx->cull_lists,   ctx->cull_meta,    ctx->light_vs,
-                        ctx->ggx_table,    ctx->shade_lights, ctx->shade_bins};
+        void* bufs[] = {ctx->cull_lists,   ctx->cull_meta,    ctx->light_vs,  ctx->ggx_table,
+                        ctx->shade_lights, ctx->shade_bins,   ctx->diffuse_stage};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         for (hipEvent_t e : ctx->kev) (void)hipEventDestroy(e);
@@ -585,7 +585,18 @@ int hvx_ibl_filter(hvx_ctx* ctx, int type, const float* d_cube, uint32_t face_si
     if (type == HVX_FILTER_DIFFUSE) {
         if (out_mips != 1) return fail(ctx, HVX_ERR_INVALID_ARG, "diffuse output has exactly 1 mip");
         if ((st = bind_device(ctx))) return st;
-        hipError_t e = hvx::launch_diffuse(d_cube, face_size, d_out, out_face_size, (hipStream_t)stream);
+        // staged footprints in context scratch; under stream capture with no
+        // buffer large enough yet, the taps are gathered from the cube (the
+        // same results, without the allocation capture forbids)
+        const size_t need = hvx::diffuse_stage_size(face_size);
+        float* stage = nullptr;
+        if (need && (ctx->diffuse_stage_bytes >= need || !hvx::stream_capturing((hipStream_t)stream))) {
+            hipError_t e = hvx::ensure_scratch(ctx, &ctx->diffuse_stage, &ctx->diffuse_stage_bytes, need,
+                                               (hipStream_t)stream);
+            if (e != hipSuccess) return scratch_fail(ctx, e, "diffuse staging allocation");
+            stage = static_cast<float*>(ctx->diffuse_stage);
+        }
+        hipError_t e = hvx::launch_diffuse(d_cube, face_size, d_out, out_face_size, stage, (hipStream_t)stream);
         return e == hipSuccess ? HVX_OK : hip_fail(ctx, e, "diffuse launch");
     }
     if (type != HVX_FILTER_SPECULAR) return fail(ctx, HVX_ERR_INVALID_ARG, "unknown filter type %d", type);

@@ -213,6 +213,8 @@ __device__ __forceinline__ f3 sample_cube(const float4* __restrict__ level, int 
 struct CubeTaps {
     int idx[4];  // taps (i0,j0), (i0+1,j0), (i0,j0+1), (i0+1,j0+1); a corner tap holds 0
     float w00, w10, w01, w11;
+    int face, i0, j0;  // the footprint's face and first texel
+    bool interior;     // all four taps on face `face` (the usual case)
     bool corner;
 };
 
@@ -240,7 +242,11 @@ __device__ __forceinline__ CubeTaps cube_taps(int S, f3 dir) {
     t.w01 = (1.0f - a) * b;
     t.w11 = a * b;
     t.corner = false;
-    if (i0 >= 0 && j0 >= 0 && i0 + 1 < S && j0 + 1 < S) {
+    t.face = cc.face;
+    t.i0 = i0;
+    t.j0 = j0;
+    t.interior = i0 >= 0 && j0 >= 0 && i0 + 1 < S && j0 + 1 < S;
+    if (t.interior) {
         const int base = (cc.face * S + j0) * S + i0;
         t.idx[0] = base;
         t.idx[1] = base + 1;

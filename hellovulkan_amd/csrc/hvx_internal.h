@@ -25,6 +25,8 @@ struct hvx_ctx {
     size_t light_vs_bytes = 0;
     void* ggx_table = nullptr;    // float4 [mips * samples]: tangent-space GGX H
     size_t ggx_table_bytes = 0;
+    void* diffuse_stage = nullptr; // the diffuse filter's staged footprints (64 B each)
+    size_t diffuse_stage_bytes = 0;
     void* shade_lights = nullptr; // float4 [2 * lights]: compact shading light table
     size_t shade_lights_bytes = 0;
     void* shade_bins = nullptr;   // uint32 [fragments + X + Y + 2]: cluster-binned fragment order
@@ -158,7 +160,13 @@ hipError_t launch_cube_mips(float* d_chain, uint32_t S, uint32_t levels, hipStre
 // d_dst + p * dst_stride_floats <- d_src[p] (a device array of n pointers), `floats` floats each
 hipError_t launch_gather_copy(const float* const* d_src, size_t floats, float* d_dst, size_t dst_stride_floats,
                               uint32_t n, hipStream_t s);
-hipError_t launch_diffuse(const float* d_cube, uint32_t S, float* d_out, uint32_t Sd, hipStream_t s);
+// Bytes of the diffuse filter's staged footprints for a level-0 face of S
+// (0: not staged: S < 2, or more than kDiffuseStageMaxBytes).
+constexpr size_t kDiffuseStageMaxBytes = size_t(1) << 30;
+size_t diffuse_stage_size(uint32_t S);
+// d_stage: diffuse_stage_size(S) bytes of scratch, or nullptr (taps
+// gathered from the cube; the same results)
+hipError_t launch_diffuse(const float* d_cube, uint32_t S, float* d_out, uint32_t Sd, float* d_stage, hipStream_t s);
 hipError_t launch_ggx_tables(uint32_t mips, uint32_t n, float4* table, hipStream_t s);
 // d_queues: the eight per-XCD probe queues (zeroed before each run)
 hipError_t launch_probe_batch(const ProbeJob* d_jobs, const ProbeItem* d_items, const uint32_t* d_xcd_begin,

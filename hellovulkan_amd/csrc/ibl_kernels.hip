@@ -5,10 +5,13 @@
 //                                     the row's GGX half-vectors staged in LDS)
 //   Equirect2Cube.frag:87-107      -> equirect_kernel (lane per cube texel)
 //   VulkanImage::GenerateMipmap    -> mip_kernel (2x2 box, lane per texel)
-//   CubeFilterDiffuse.frag:35-136  -> diffuse_kernel (16 lanes per output
-//                                     texel evaluate samples, one lane per
-//                                     texel and channel sums them in order;
-//                                     the phi/theta sin/cos grid in LDS)
+//   CubeFilterDiffuse.frag:35-136  -> diffuse_stage_kernel (each bilinear
+//                                     footprint's texels in one 64-byte
+//                                     entry) and diffuse_kernel (32 lanes per
+//                                     output texel evaluate samples, loads one
+//                                     round ahead; one lane per texel and
+//                                     channel sums them in order; the
+//                                     phi/theta sin/cos grid in LDS)
 //   CubeFilterSpecular.frag:34-146 -> ggx_table_kernel (tangent-space GGX
 //                                     samples per mip, computed once),
 //                                     specular_delta_kernel (mip 0, roughness
@@ -148,21 +151,74 @@ __global__ __launch_bounds__(256) void mip_kernel(const float4* __restrict__ src
 // ---------------------------------------------------------------- diffuse
 // The reference sums 252 x 63 = 15,876 samples per output texel in one
 // sequential loop (phi outer, theta inner). A lane per texel leaves the GPU
-// idle (6,144 texels at 32^2), so here a workgroup owns kDTex texels and its
-// 256 lanes evaluate kDSlot consecutive samples of each texel per round; the
-// weighted values go to LDS and one "summer" lane per texel and channel adds
-// them in the reference's order. Results are bit-identical to the sequential
-// loop. Rounds are double-buffered so sample evaluation of round r+1
-// overlaps the summation of round r.
+// idle (6,144 texels at 32^2), so here a workgroup owns TEX texels and its
+// THREADS lanes evaluate THREADS / TEX consecutive samples of each texel per
+// round; the weighted values go to LDS and one "summer" lane per texel and
+// channel adds them in the reference's order. Results are bit-identical to
+// the sequential loop. The kernel waits on gathers (at 1024^2 every sample
+// reads about two fresh 128-byte lines; a few percent of its time issues
+// arithmetic), so interior samples read their footprint from a staged copy
+// (diffuse_stage_kernel, one line per sample), a lane's loads are issued one
+// round ahead of their use, and the LDS rounds are double-buffered: the
+// gathers of round r+1 are in flight while round r is weighted, stored and
+// summed. 32^2 from 1024^2: 1.97 ms staged with 8 texels per 256-lane
+// workgroup (2.03 / 2.04 ms with 4 / 2); with the taps gathered from the
+// cube 2.66 ms at 4 texels (3.11 / 3.31 / 2.77 ms at 8 / 16 / 2, 3.02 ms at 4
+// texels per 128-lane workgroup); 3.54 ms with 16 texels and no loads ahead.
 constexpr int kMaxPhi = 512, kMaxTheta = 128;
-constexpr int kDTex = 16;               // texels per workgroup
-constexpr int kDSlot = 256 / kDTex;     // samples per texel per round
+#ifndef HVX_DIFFUSE_TEX
+#define HVX_DIFFUSE_TEX 8
+#endif
+constexpr int kDTex = HVX_DIFFUSE_TEX;  // texels per workgroup
+static_assert(kDTex == 2 || kDTex == 4 || kDTex == 8 || kDTex == 16, "texels per workgroup: 2, 4, 8 or 16");
+#ifndef HVX_DIFFUSE_THREADS
+#define HVX_DIFFUSE_THREADS 256
+#endif
+constexpr int kDThreads = HVX_DIFFUSE_THREADS;  // lanes per workgroup
+static_assert(kDThreads == 128 || kDThreads == 256, "lanes per workgroup: 128 or 256");
+#ifndef HVX_DIFFUSE_WAVES_PER_EU
+#define HVX_DIFFUSE_WAVES_PER_EU 4
+#endif
 
-__global__ __launch_bounds__(256) void diffuse_kernel(const float4* __restrict__ cube, uint32_t S,
-                                                     float4* __restrict__ out, uint32_t Sd) {
+// Staged footprints: for every footprint of four taps on one face, first
+// texel (i, j) with i, j < S - 1, one 64-byte entry at ((f (S-1) + j) (S-1)
+// + i) of the RGB of its taps, (t00.rgb, t11.r) (t10.rgb, t11.g)
+// (t01.rgb, t11.b), the last 16 bytes unused. An interior sample then reads
+// one 128-byte line in three loads instead of two rows of texels (2.3 lines
+// on average) in four. The values are the texels' own, so the bilinear sums
+// are unchanged to the bit.
+__global__ __launch_bounds__(256) void diffuse_stage_kernel(const float4* __restrict__ cube, uint32_t S,
+                                                           float4* __restrict__ quads) {
+    const uint32_t Sq = S - 1, per = Sq * Sq;
+    const uint32_t e = blockIdx.x * 256u + threadIdx.x;
+    if (e >= 6u * per) return;
+    const uint32_t f = e / per, r = e % per, j = r / Sq, i = r % Sq;
+    const float4* t = cube + ((size_t)f * S + j) * S + i;
+    const float4 t00 = t[0], t10 = t[1], t01 = t[S], t11 = t[S + 1];
+    float4* q = quads + 4 * (size_t)e;
+    q[0] = make_float4(t00.x, t00.y, t00.z, t11.x);
+    q[1] = make_float4(t10.x, t10.y, t10.z, t11.y);
+    q[2] = make_float4(t01.x, t01.y, t01.z, t11.z);
+}
+
+// One lane's sample of one round: direction, theta terms, taps and loads
+// (q[k] = tap k's texel; or, staged, q[0..2] = the footprint's entry).
+struct DiffSample {
+    f3 L;
+    float ct, st;
+    CubeTaps taps;
+    bool staged;
+    float4 q[4];
+};
+
+template <int TEX, int THREADS>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(HVX_DIFFUSE_WAVES_PER_EU, 8))) void
+diffuse_kernel(const float4* __restrict__ cube, uint32_t S, const float4* __restrict__ quads, float4* __restrict__ out,
+               uint32_t Sd) {
+    constexpr uint32_t SLOT = THREADS / TEX;  // samples per texel per round
     __shared__ float s_phi[kMaxPhi], s_theta[kMaxTheta];
     __shared__ float s_sp[kMaxPhi], s_cp[kMaxPhi], s_st[kMaxTheta], s_ct[kMaxTheta];
-    __shared__ float s_buf[2][kDSlot][kDTex][3];
+    __shared__ float s_buf[2][SLOT][TEX][3];
     __shared__ uint32_t s_np, s_nt;
     const float TWO_PI = kPiF * 2.0f, HALF_PI = kPiF * 0.5f;
     const float delta = 0.025f;
@@ -187,40 +243,75 @@ __global__ __launch_bounds__(256) void diffuse_kernel(const float4* __restrict__
     }
     __syncthreads();
     const uint32_t ntex = 6u * Sd * Sd;
-    // sample lane: texel l % kDTex, slot l / kDTex
-    const uint32_t ti = l % kDTex, slot = l / kDTex;
-    const uint32_t texel = min(blockIdx.x * kDTex + ti, ntex - 1);
+    // sample lane: texel l % TEX, slot l / TEX
+    const uint32_t ti = l % TEX, slot = l / TEX;
+    const uint32_t texel = min(blockIdx.x * TEX + ti, ntex - 1);
     const uint32_t x = texel % Sd, y = (texel / Sd) % Sd, face = texel / (Sd * Sd);
     const f3 N = texel_dir((int)face, (int)x, (int)y, (int)Sd);
     f3 up = make3(0.0f, 1.0f, 0.0f);
     const f3 right = normalize3(cross3(up, N));
     up = normalize3(cross3(N, right));
-    // summer lane: l < 3 * kDTex -> texel l / 3, channel l % 3
-    const bool summer = l < 3u * kDTex;
+    // summer lane: l < 3 * TEX -> texel l / 3, channel l % 3
+    const bool summer = l < 3u * TEX;
     const uint32_t st_i = l / 3, st_c = l % 3;
     float acc = 0.0f;
-    const uint32_t rounds = (total + kDSlot - 1) / kDSlot;
-    for (uint32_t r = 0; r < rounds; ++r) {
-        const uint32_t sidx = r * kDSlot + slot;
-        float (*buf)[kDTex][3] = s_buf[r & 1];
-        if (sidx < total) {
-            const uint32_t ip = sidx / nt, it = sidx % nt;
-            const float sp = s_sp[ip], cp = s_cp[ip], st = s_st[it], ct = s_ct[it];
-            const float tx = st * cp, ty = st * sp, tz = ct;
-            const f3 sv = add3(add3(mul3(right, tx), mul3(up, ty)), mul3(N, tz));
-            const f3 c = sample_cube(cube, (int)S, sv);
-            buf[slot][ti][0] = (c.x * ct) * st;
-            buf[slot][ti][1] = (c.y * ct) * st;
-            buf[slot][ti][2] = (c.z * ct) * st;
+    const uint32_t rounds = (total + SLOT - 1) / SLOT;
+    // sample r * SLOT + slot; slots past the last sample evaluate the last
+    // one again (in range, never summed), so the loads stay unconditional
+    auto begin = [&](DiffSample& p, uint32_t r) {
+        const uint32_t sidx = min(r * SLOT + slot, total - 1);
+        const uint32_t ip = sidx / nt, it = sidx % nt;
+        const float sp = s_sp[ip], cp = s_cp[ip];
+        p.st = s_st[it];
+        p.ct = s_ct[it];
+        const float tx = p.st * cp, ty = p.st * sp, tz = p.ct;
+        p.L = add3(add3(mul3(right, tx), mul3(up, ty)), mul3(N, tz));
+        p.taps = cube_taps((int)S, p.L);
+        // staged entry (interior footprints, when staged) or the four taps
+        p.staged = quads != nullptr && p.taps.interior;
+        const size_t e = p.staged ? 4 * (((size_t)p.taps.face * (S - 1) + (uint32_t)p.taps.j0) * (S - 1) +
+                                         (uint32_t)p.taps.i0)
+                                  : 0;
+        p.q[0] = p.staged ? quads[e] : cube[p.taps.idx[0]];
+        p.q[1] = p.staged ? quads[e + 1] : cube[p.taps.idx[1]];
+        p.q[2] = p.staged ? quads[e + 2] : cube[p.taps.idx[2]];
+        if (!p.staged) p.q[3] = cube[p.taps.idx[3]];
+    };
+    // weight round r's sample, store it, and (summers) add the round's terms
+    // in sample order; every lane of the workgroup calls it for every r
+    auto sum_round = [&](uint32_t r, const DiffSample& p) {
+        float (*buf)[TEX][3] = s_buf[r & 1];
+        // sample_cube's bilinear sum (a corner tap: sample_cube itself)
+        f3 c;
+        if (p.taps.corner) {
+            c = sample_cube(cube, (int)S, p.L);
+        } else {
+            const float4 q0 = p.q[0], q1 = p.q[1], q2 = p.q[2];
+            const f3 t11 = p.staged ? make3(q0.w, q1.w, q2.w) : make3(p.q[3].x, p.q[3].y, p.q[3].z);
+            const CubeTaps& t = p.taps;
+            c = {((t.w00 * q0.x + t.w10 * q1.x) + t.w01 * q2.x) + t.w11 * t11.x,
+                 ((t.w00 * q0.y + t.w10 * q1.y) + t.w01 * q2.y) + t.w11 * t11.y,
+                 ((t.w00 * q0.z + t.w10 * q1.z) + t.w01 * q2.z) + t.w11 * t11.z};
         }
+        buf[slot][ti][0] = (c.x * p.ct) * p.st;
+        buf[slot][ti][1] = (c.y * p.ct) * p.st;
+        buf[slot][ti][2] = (c.z * p.ct) * p.st;
         __syncthreads();
-        if (summer) {
-            const uint32_t kmax = min((uint32_t)kDSlot, total - r * kDSlot);
+        if (summer && r < rounds) {
+            const uint32_t kmax = min(SLOT, total - r * SLOT);
             for (uint32_t k = 0; k < kmax; ++k) acc = acc + buf[k][st_i][st_c];
         }
+    };
+    DiffSample A, B;
+    begin(A, 0);
+    for (uint32_t r = 0; r < rounds; r += 2) {
+        begin(B, r + 1);
+        sum_round(r, A);
+        begin(A, r + 2);
+        sum_round(r + 1, B);
     }
     if (summer) {
-        const uint32_t t = blockIdx.x * kDTex + st_i;
+        const uint32_t t = blockIdx.x * TEX + st_i;
         if (t < ntex) {
             const float inv = 1.0f / (float)total;
             reinterpret_cast<float*>(out + t)[st_c] = (kPiF * acc) * inv;
@@ -557,10 +648,22 @@ hipError_t launch_gather_copy(const float* const* d_src, size_t floats, float* d
     return hipGetLastError();
 }
 
-hipError_t launch_diffuse(const float* d_cube, uint32_t S, float* d_out, uint32_t Sd, hipStream_t s) {
+size_t diffuse_stage_size(uint32_t S) {
+    if (S < 2) return 0;
+    const size_t b = (size_t)6 * (S - 1) * (S - 1) * 64;
+    return b <= kDiffuseStageMaxBytes ? b : 0;
+}
+
+hipError_t launch_diffuse(const float* d_cube, uint32_t S, float* d_out, uint32_t Sd, float* d_stage, hipStream_t s) {
+    const float4* cube = reinterpret_cast<const float4*>(d_cube);
+    float4* quads = diffuse_stage_size(S) ? reinterpret_cast<float4*>(d_stage) : nullptr;
+    if (quads) {
+        const uint32_t E = 6u * (S - 1) * (S - 1);
+        hipLaunchKernelGGL(diffuse_stage_kernel, dim3((E + 255) / 256), dim3(256), 0, s, cube, S, quads);
+    }
     const uint32_t T = 6u * Sd * Sd;
-    hipLaunchKernelGGL(diffuse_kernel, dim3((T + kDTex - 1) / kDTex), dim3(256), 0, s,
-                       reinterpret_cast<const float4*>(d_cube), S, reinterpret_cast<float4*>(d_out), Sd);
+    hipLaunchKernelGGL((diffuse_kernel<kDTex, kDThreads>), dim3((T + kDTex - 1) / kDTex), dim3(kDThreads), 0, s, cube,
+                       S, quads, reinterpret_cast<float4*>(d_out), Sd);
     return hipGetLastError();
 }
 

@@ -95,6 +95,45 @@ def test_diffuse(ctx, S, Sd, sub):
     assert _report(f"diffuse {S}->{Sd}", got[ids], ref) <= TOL
 
 
+def test_diffuse_staged_and_gathered_paths_equal(ctx):
+    """The diffuse filter stages each bilinear footprint inside a face in
+    context scratch (one 64-byte entry per footprint); captured into a graph
+    on a context that holds no staging buffer yet, it gathers the taps from
+    the cube instead, allocating nothing. Both give the same bits: graph
+    replays, with eager work between them, equal the staged eager call; a
+    second eager call (staging buffer reused) equals the first."""
+    import torch
+    S, Sd = 128, 16
+    cube = _cube_from_equirect(ctx, 21, 2 * S, S, S)
+    ref = torch.empty(6 * Sd * Sd * 4, dtype=torch.float32, device="cuda")
+    ctx.filter(abi.FILTER_DIFFUSE, cube, S, ref, Sd, 1, 0)
+    torch.cuda.synchronize()
+    again = torch.full_like(ref, np.nan)
+    ctx.filter(abi.FILTER_DIFFUSE, cube, S, again, Sd, 1, 0)
+    torch.cuda.synchronize()
+    assert torch.equal(again.view(torch.int32), ref.view(torch.int32))
+    fresh = abi.Context(0)
+    try:
+        out = torch.full_like(ref, np.nan)
+        s = torch.cuda.Stream()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            fresh.filter(abi.FILTER_DIFFUSE, cube, S, out, Sd, 1, 0, stream=s)
+        for _ in range(3):
+            out.fill_(np.nan)
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(out.view(torch.int32), ref.view(torch.int32))
+            torch.zeros(1 << 20, device="cuda").add_(1.0)
+        del g
+    finally:
+        fresh.close()
+    ids = np.arange(0, 6 * Sd * Sd, 5, dtype=np.uint32)
+    got = ref.cpu().numpy().reshape(-1, 4)[ids]
+    assert np.array_equal(got.view(np.uint32), O.diffuse(cube.cpu().numpy(), S, Sd, ids).view(np.uint32))
+
+
 @pytest.mark.parametrize("S,So,mips,n,emulate,sub", [
     (64, 16, 5, 256, False, None),
     (64, 16, 5, 64, True, None),
