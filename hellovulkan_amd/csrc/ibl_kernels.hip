@@ -148,6 +148,14 @@ __global__ __launch_bounds__(256) void mip_kernel(const float4* __restrict__ src
                           ((a.z + b.z) + (c.z + d.z)) * 0.25f, ((a.w + b.w) + (c.w + d.w)) * 0.25f);
 }
 
+// Orders a wave's LDS stores before its later LDS loads of what other lanes
+// of the same wave stored (no workgroup barrier needed).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---------------------------------------------------------------- diffuse
 // The reference sums 252 x 63 = 15,876 samples per output texel in one
 // sequential loop (phi outer, theta inner). A lane per texel leaves the GPU
@@ -157,14 +165,17 @@ __global__ __launch_bounds__(256) void mip_kernel(const float4* __restrict__ src
 // channel adds them in the reference's order. Results are bit-identical to
 // the sequential loop. The kernel waits on gathers (at 1024^2 every sample
 // reads about two fresh 128-byte lines; a few percent of its time issues
-// arithmetic), so interior samples read their footprint from a staged copy
-// (diffuse_stage_kernel, one line per sample), a lane's loads are issued one
-// round ahead of their use, and the LDS rounds are double-buffered: the
+// arithmetic), so samples read their footprint from a staged copy
+// (diffuse_stage_kernel, one line per sample), a lane's loads are issued
+// one round ahead of their use, and the LDS rounds are double-buffered: the
 // gathers of round r+1 are in flight while round r is weighted, stored and
-// summed. 32^2 from 1024^2: 1.97 ms staged with 8 texels per 256-lane
-// workgroup (2.03 / 2.04 ms with 4 / 2); with the taps gathered from the
-// cube 2.66 ms at 4 texels (3.11 / 3.31 / 2.77 ms at 8 / 16 / 2, 3.02 ms at 4
-// texels per 128-lane workgroup); 3.54 ms with 16 texels and no loads ahead.
+// summed. 32^2 from 1024^2 (staging included): 1.82 ms with 8 texels per
+// 256-lane workgroup (1.80 with 4); 2.01 ms when only footprints inside a
+// face were staged (the others gathered per lane); gathered from the cube
+// (the path for stream capture before the context holds a staging buffer)
+// 2.66 ms with 4 texels (3.11 / 3.31 / 2.77 ms with 8 / 16 / 2, 3.02 with 4
+// per 128-lane workgroup); 3.54 ms with 16 texels and no loads ahead.
+// Staged loads two rounds ahead (79 VGPRs): no faster (1.81 ms).
 constexpr int kMaxPhi = 512, kMaxTheta = 128;
 #ifndef HVX_DIFFUSE_TEX
 #define HVX_DIFFUSE_TEX 8
@@ -180,38 +191,56 @@ static_assert(kDThreads == 128 || kDThreads == 256, "lanes per workgroup: 128 or
 #define HVX_DIFFUSE_WAVES_PER_EU 4
 #endif
 
-// Staged footprints: for every footprint of four taps on one face, first
-// texel (i, j) with i, j < S - 1, one 64-byte entry at ((f (S-1) + j) (S-1)
-// + i) of the RGB of its taps, (t00.rgb, t11.r) (t10.rgb, t11.g)
-// (t01.rgb, t11.b), the last 16 bytes unused. An interior sample then reads
-// one 128-byte line in three loads instead of two rows of texels (2.3 lines
-// on average) in four. The values are the texels' own, so the bilinear sums
-// are unchanged to the bit.
+// Staged footprints: for every bilinear footprint of level 0, first texel
+// (i0, j0) with -1 <= i0, j0 <= S - 1 on face f, one 64-byte entry at
+// (f (S+1) + j0 + 1) (S+1) + i0 + 1 holding the RGB of its four taps
+// exactly as sample_cube reads them (texels across an edge folded onto the
+// neighbouring face, a tap off both edges the mean of three texels):
+// (t00.rgb, t11.r) (t10.rgb, t11.g) (t01.rgb, t11.b), then 16 bytes of zeros.
+// A sample then reads one 128-byte line instead of two rows of texels (2.3
+// lines on average), and the bilinear sums are unchanged to the bit.
 __global__ __launch_bounds__(256) void diffuse_stage_kernel(const float4* __restrict__ cube, uint32_t S,
                                                            float4* __restrict__ quads) {
-    const uint32_t Sq = S - 1, per = Sq * Sq;
+    const uint32_t Se = S + 1, per = Se * Se;
     const uint32_t e = blockIdx.x * 256u + threadIdx.x;
     if (e >= 6u * per) return;
-    const uint32_t f = e / per, r = e % per, j = r / Sq, i = r % Sq;
-    const float4* t = cube + ((size_t)f * S + j) * S + i;
-    const float4 t00 = t[0], t10 = t[1], t01 = t[S], t11 = t[S + 1];
+    const uint32_t f = e / per, r = e % per;
+    const int j0 = (int)(r / Se) - 1, i0 = (int)(r % Se) - 1, Si = (int)S;
+    f3 t00, t10, t01, t11;
+    if (i0 >= 0 && j0 >= 0 && i0 + 1 < Si && j0 + 1 < Si) {
+        const float4* t = cube + ((size_t)f * S + j0) * S + i0;
+        const float4 a = t[0], b = t[1], c = t[S], d = t[S + 1];
+        t00 = {a.x, a.y, a.z};
+        t10 = {b.x, b.y, b.z};
+        t01 = {c.x, c.y, c.z};
+        t11 = {d.x, d.y, d.z};
+    } else {
+        t00 = cube_texel_seamless(cube, Si, (int)f, i0, j0);
+        t10 = cube_texel_seamless(cube, Si, (int)f, i0 + 1, j0);
+        t01 = cube_texel_seamless(cube, Si, (int)f, i0, j0 + 1);
+        t11 = cube_texel_seamless(cube, Si, (int)f, i0 + 1, j0 + 1);
+    }
     float4* q = quads + 4 * (size_t)e;
     q[0] = make_float4(t00.x, t00.y, t00.z, t11.x);
     q[1] = make_float4(t10.x, t10.y, t10.z, t11.y);
     q[2] = make_float4(t01.x, t01.y, t01.z, t11.z);
+    q[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // whole lines written: no partial-line writes
 }
 
 // One lane's sample of one round: direction, theta terms, taps and loads
-// (q[k] = tap k's texel; or, staged, q[0..2] = the footprint's entry).
+// (gathered: q[k] = tap k's texel; staged: q[0..2] = the footprint's entry).
 struct DiffSample {
     f3 L;
     float ct, st;
     CubeTaps taps;
-    bool staged;
     float4 q[4];
 };
 
-template <int TEX, int THREADS>
+// STAGED: the samples read staged footprints (three 16-byte loads from one
+// line); otherwise their four taps from the cube. Four lanes loading one
+// entry together (16 lines per load instruction instead of 64, the parts
+// passed on through LDS) measured slower: 2.36 ms against 1.82.
+template <int TEX, int THREADS, bool STAGED>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(HVX_DIFFUSE_WAVES_PER_EU, 8))) void
 diffuse_kernel(const float4* __restrict__ cube, uint32_t S, const float4* __restrict__ quads, float4* __restrict__ out,
                uint32_t Sd) {
@@ -267,31 +296,32 @@ diffuse_kernel(const float4* __restrict__ cube, uint32_t S, const float4* __rest
         const float tx = p.st * cp, ty = p.st * sp, tz = p.ct;
         p.L = add3(add3(mul3(right, tx), mul3(up, ty)), mul3(N, tz));
         p.taps = cube_taps((int)S, p.L);
-        // staged entry (interior footprints, when staged) or the four taps
-        p.staged = quads != nullptr && p.taps.interior;
-        const size_t e = p.staged ? 4 * (((size_t)p.taps.face * (S - 1) + (uint32_t)p.taps.j0) * (S - 1) +
-                                         (uint32_t)p.taps.i0)
-                                  : 0;
-        p.q[0] = p.staged ? quads[e] : cube[p.taps.idx[0]];
-        p.q[1] = p.staged ? quads[e + 1] : cube[p.taps.idx[1]];
-        p.q[2] = p.staged ? quads[e + 2] : cube[p.taps.idx[2]];
-        if (!p.staged) p.q[3] = cube[p.taps.idx[3]];
+        if constexpr (STAGED) {
+            // i0, j0 lie in [-1, S - 1] (the face coordinates are in [0, 1]);
+            // the clamp only keeps the index inside the table
+            const int i0 = min(max(p.taps.i0, -1), (int)S - 1), j0 = min(max(p.taps.j0, -1), (int)S - 1);
+            const uint32_t e = ((uint32_t)p.taps.face * (S + 1) + (uint32_t)(j0 + 1)) * (S + 1) + (uint32_t)(i0 + 1);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) p.q[k] = quads[4 * (size_t)e + k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) p.q[k] = cube[p.taps.idx[k]];
+        }
     };
     // weight round r's sample, store it, and (summers) add the round's terms
     // in sample order; every lane of the workgroup calls it for every r
     auto sum_round = [&](uint32_t r, const DiffSample& p) {
         float (*buf)[TEX][3] = s_buf[r & 1];
-        // sample_cube's bilinear sum (a corner tap: sample_cube itself)
         f3 c;
-        if (p.taps.corner) {
-            c = sample_cube(cube, (int)S, p.L);
-        } else {
+        if constexpr (STAGED) {
+            // sample_cube's bilinear sum of the entry's taps
             const float4 q0 = p.q[0], q1 = p.q[1], q2 = p.q[2];
-            const f3 t11 = p.staged ? make3(q0.w, q1.w, q2.w) : make3(p.q[3].x, p.q[3].y, p.q[3].z);
             const CubeTaps& t = p.taps;
-            c = {((t.w00 * q0.x + t.w10 * q1.x) + t.w01 * q2.x) + t.w11 * t11.x,
-                 ((t.w00 * q0.y + t.w10 * q1.y) + t.w01 * q2.y) + t.w11 * t11.y,
-                 ((t.w00 * q0.z + t.w10 * q1.z) + t.w01 * q2.z) + t.w11 * t11.z};
+            c = {((t.w00 * q0.x + t.w10 * q1.x) + t.w01 * q2.x) + t.w11 * q0.w,
+                 ((t.w00 * q0.y + t.w10 * q1.y) + t.w01 * q2.y) + t.w11 * q1.w,
+                 ((t.w00 * q0.z + t.w10 * q1.z) + t.w01 * q2.z) + t.w11 * q2.w};
+        } else {
+            c = cube_combine(cube, (int)S, p.L, p.taps, p.q);
         }
         buf[slot][ti][0] = (c.x * p.ct) * p.st;
         buf[slot][ti][1] = (c.y * p.ct) * p.st;
@@ -418,12 +448,6 @@ struct SpecSplitArgs {
     uint32_t tile_begin[kMaxMips + 1];  // first tile of mip m (mips 1..), tile_begin[1] = 0
     const float4* table;
 };
-
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // One GGX sample of the specular loop, split so its texel loads overlap other
 // work: spec_begin forms L and NoL and issues the four tap loads (always, so
@@ -649,21 +673,24 @@ hipError_t launch_gather_copy(const float* const* d_src, size_t floats, float* d
 }
 
 size_t diffuse_stage_size(uint32_t S) {
-    if (S < 2) return 0;
-    const size_t b = (size_t)6 * (S - 1) * (S - 1) * 64;
+    const size_t b = (size_t)6 * (S + 1) * (S + 1) * 64;
     return b <= kDiffuseStageMaxBytes ? b : 0;
 }
 
 hipError_t launch_diffuse(const float* d_cube, uint32_t S, float* d_out, uint32_t Sd, float* d_stage, hipStream_t s) {
     const float4* cube = reinterpret_cast<const float4*>(d_cube);
     float4* quads = diffuse_stage_size(S) ? reinterpret_cast<float4*>(d_stage) : nullptr;
-    if (quads) {
-        const uint32_t E = 6u * (S - 1) * (S - 1);
-        hipLaunchKernelGGL(diffuse_stage_kernel, dim3((E + 255) / 256), dim3(256), 0, s, cube, S, quads);
-    }
     const uint32_t T = 6u * Sd * Sd;
-    hipLaunchKernelGGL((diffuse_kernel<kDTex, kDThreads>), dim3((T + kDTex - 1) / kDTex), dim3(kDThreads), 0, s, cube,
-                       S, quads, reinterpret_cast<float4*>(d_out), Sd);
+    float4* out = reinterpret_cast<float4*>(d_out);
+    if (quads) {
+        const uint32_t E = 6u * (S + 1) * (S + 1);
+        hipLaunchKernelGGL(diffuse_stage_kernel, dim3((E + 255) / 256), dim3(256), 0, s, cube, S, quads);
+        hipLaunchKernelGGL((diffuse_kernel<kDTex, kDThreads, true>), dim3((T + kDTex - 1) / kDTex), dim3(kDThreads), 0,
+                           s, cube, S, quads, out, Sd);
+    } else {
+        hipLaunchKernelGGL((diffuse_kernel<kDTex, kDThreads, false>), dim3((T + kDTex - 1) / kDTex), dim3(kDThreads),
+                           0, s, cube, S, nullptr, out, Sd);
+    }
     return hipGetLastError();
 }
 

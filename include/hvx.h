@@ -343,11 +343,11 @@ int hvx_ibl_cube_mips(hvx_ctx* ctx, float* d_cube_chain, uint32_t face_size,
  * running Shaders/IBL/CubeFilterDiffuse.frag:35-136 and
  * CubeFilterSpecular.frag:34-146. opts may be NULL.
  * Context scratch: the specular filter's sample table (16 B per mip and
- * sample); the diffuse filter's staged footprints, 64 B for each 2x2
- * footprint inside a face (6 (face_size-1)^2 entries, 402 MB at 1024^2; none
- * when that exceeds 1 GiB). Under stream capture, when the context holds no
- * staging buffer that large yet, the diffuse filter reads the taps from
- * d_cube instead and allocates nothing (the same results, bit for bit). */
+ * sample); the diffuse filter's staged footprints, 64 B for each bilinear
+ * footprint (6 (face_size+1)^2 entries, 403 MB at 1024^2; none when that
+ * exceeds 1 GiB). Under stream capture, when the context holds no staging
+ * buffer that large yet, the diffuse filter reads the taps from d_cube
+ * instead and allocates nothing (the same results, bit for bit). */
 int hvx_ibl_filter(hvx_ctx* ctx, int type, const float* d_cube,
                    uint32_t face_size, float* d_out_chain,
                    uint32_t out_face_size, uint32_t out_mips,

@@ -96,12 +96,13 @@ def test_diffuse(ctx, S, Sd, sub):
 
 
 def test_diffuse_staged_and_gathered_paths_equal(ctx):
-    """The diffuse filter stages each bilinear footprint inside a face in
-    context scratch (one 64-byte entry per footprint); captured into a graph
-    on a context that holds no staging buffer yet, it gathers the taps from
-    the cube instead, allocating nothing. Both give the same bits: graph
-    replays, with eager work between them, equal the staged eager call; a
-    second eager call (staging buffer reused) equals the first."""
+    """The diffuse filter stages every bilinear footprint of level 0 in
+    context scratch (one 64-byte entry each, edge-folded and corner taps
+    included); captured into a graph on a context that holds no staging
+    buffer yet, it gathers the taps from the cube instead, allocating
+    nothing. Both give the same bits: graph replays, with eager work between
+    them, equal the staged eager call; a second eager call (staging buffer
+    reused) equals the first; sampled texels equal the FP32 oracle."""
     import torch
     S, Sd = 128, 16
     cube = _cube_from_equirect(ctx, 21, 2 * S, S, S)
