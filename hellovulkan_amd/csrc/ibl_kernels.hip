@@ -169,8 +169,9 @@ __device__ __forceinline__ void wave_lds_sync() {
 // (diffuse_stage_kernel, one line per sample), a lane's loads are issued
 // one round ahead of their use, and the LDS rounds are double-buffered: the
 // gathers of round r+1 are in flight while round r is weighted, stored and
-// summed. 32^2 from 1024^2 (staging included): 1.82 ms with 8 texels per
-// 256-lane workgroup (1.80 with 4); 2.01 ms when only footprints inside a
+// summed. 32^2 from 1024^2 (staging included): 1.79 ms with 8 texels per
+// 256-lane workgroup (staging 0.12 ms of it; 1.80 with 4 texels and the
+// staging's stores one entry per lane); 2.01 ms when only footprints inside a
 // face were staged (the others gathered per lane); gathered from the cube
 // (the path for stream capture before the context holds a staging buffer)
 // 2.66 ms with 4 texels (3.11 / 3.31 / 2.77 ms with 8 / 16 / 2, 3.02 with 4
@@ -201,30 +202,30 @@ static_assert(kDThreads == 128 || kDThreads == 256, "lanes per workgroup: 128 or
 // lines on average), and the bilinear sums are unchanged to the bit.
 __global__ __launch_bounds__(256) void diffuse_stage_kernel(const float4* __restrict__ cube, uint32_t S,
                                                            float4* __restrict__ quads) {
+    // lane g writes 16-byte part g % 4 of entry g / 4: a wave's stores are
+    // one contiguous kilobyte
     const uint32_t Se = S + 1, per = Se * Se;
-    const uint32_t e = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t g = blockIdx.x * 256u + threadIdx.x, e = g / 4u, k = g % 4u;
     if (e >= 6u * per) return;
-    const uint32_t f = e / per, r = e % per;
-    const int j0 = (int)(r / Se) - 1, i0 = (int)(r % Se) - 1, Si = (int)S;
-    f3 t00, t10, t01, t11;
-    if (i0 >= 0 && j0 >= 0 && i0 + 1 < Si && j0 + 1 < Si) {
-        const float4* t = cube + ((size_t)f * S + j0) * S + i0;
-        const float4 a = t[0], b = t[1], c = t[S], d = t[S + 1];
-        t00 = {a.x, a.y, a.z};
-        t10 = {b.x, b.y, b.z};
-        t01 = {c.x, c.y, c.z};
-        t11 = {d.x, d.y, d.z};
-    } else {
-        t00 = cube_texel_seamless(cube, Si, (int)f, i0, j0);
-        t10 = cube_texel_seamless(cube, Si, (int)f, i0 + 1, j0);
-        t01 = cube_texel_seamless(cube, Si, (int)f, i0, j0 + 1);
-        t11 = cube_texel_seamless(cube, Si, (int)f, i0 + 1, j0 + 1);
+    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // part 3: zeros (whole lines written)
+    if (k < 3u) {
+        // part k: tap k (t00, t10, t01) and channel k of t11
+        const uint32_t f = e / per, r = e % per;
+        const int j0 = (int)(r / Se) - 1, i0 = (int)(r % Se) - 1, Si = (int)S;
+        const int ti = i0 + (k == 1u ? 1 : 0), tj = j0 + (k == 2u ? 1 : 0);
+        f3 t, t11;
+        if (i0 >= 0 && j0 >= 0 && i0 + 1 < Si && j0 + 1 < Si) {
+            const float4 a = cube[((size_t)f * S + tj) * S + ti];
+            const float4 d = cube[((size_t)f * S + j0 + 1) * S + i0 + 1];
+            t = {a.x, a.y, a.z};
+            t11 = {d.x, d.y, d.z};
+        } else {
+            t = cube_texel_seamless(cube, Si, (int)f, ti, tj);
+            t11 = cube_texel_seamless(cube, Si, (int)f, i0 + 1, j0 + 1);
+        }
+        v = make_float4(t.x, t.y, t.z, k == 0u ? t11.x : (k == 1u ? t11.y : t11.z));
     }
-    float4* q = quads + 4 * (size_t)e;
-    q[0] = make_float4(t00.x, t00.y, t00.z, t11.x);
-    q[1] = make_float4(t10.x, t10.y, t10.z, t11.y);
-    q[2] = make_float4(t01.x, t01.y, t01.z, t11.z);
-    q[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // whole lines written: no partial-line writes
+    quads[g] = v;
 }
 
 // One lane's sample of one round: direction, theta terms, taps and loads
@@ -683,8 +684,8 @@ hipError_t launch_diffuse(const float* d_cube, uint32_t S, float* d_out, uint32_
     const uint32_t T = 6u * Sd * Sd;
     float4* out = reinterpret_cast<float4*>(d_out);
     if (quads) {
-        const uint32_t E = 6u * (S + 1) * (S + 1);
-        hipLaunchKernelGGL(diffuse_stage_kernel, dim3((E + 255) / 256), dim3(256), 0, s, cube, S, quads);
+        const uint32_t parts = 4u * 6u * (S + 1) * (S + 1);  // 16-byte parts of the entries
+        hipLaunchKernelGGL(diffuse_stage_kernel, dim3((parts + 255) / 256), dim3(256), 0, s, cube, S, quads);
         hipLaunchKernelGGL((diffuse_kernel<kDTex, kDThreads, true>), dim3((T + kDTex - 1) / kDTex), dim3(kDThreads), 0,
                            s, cube, S, quads, out, Sd);
     } else {
